@@ -1,0 +1,177 @@
+"""Pins the oracle's energy/force routines: closed forms, limits and identities."""
+import numpy as np
+import pytest
+
+from molsim_amd import systems as S
+
+
+def _coulomb_pot(npt, z, rcut, alpha, epsi4pi=1.0, tol=1e-9, r2min=0.1):
+    return dict(npt=npt, form=0, lewald=1, alpha=alpha, rcut=rcut, epsi4pi=epsi4pi, z=z,
+                sig=[0] * npt, eps=[0] * npt, rad=[0] * npt, utoltab=tol, ftoltab=tol,
+                umaxtab=2e4, fmaxtab=2e4, r2uminin=r2min)
+
+
+def _nacl(ncell):
+    pts, q = [], []
+    for i in range(2 * ncell):
+        for j in range(2 * ncell):
+            for k in range(2 * ncell):
+                pts.append((i, j, k))
+                q.append(1.0 if (i + j + k) % 2 == 0 else -1.0)
+    L = 2.0 * ncell
+    ro = np.array(pts, dtype=float) - L / 2 + 0.25
+    return ro, np.array(q), L
+
+
+def test_std_ewald_madelung_nacl(orc):
+    ro, q, L = _nacl(2)                 # 64 ions, nearest-neighbour distance 1
+    ipt = (q < 0).astype(np.int32)
+    alpha, rcut = 2.4, 2.0
+    pot = _coulomb_pot(2, [1.0, -1.0], rcut, alpha)
+    tab = orc.build_table(pot)
+    sysm = orc.System(ro, ipt, q, [L, L, L], rcut, 1.0, tab, npt=2, list_mode=0, ewald=1,
+                      surf=False, alpha=alpha, ncut=14)
+    r = sysm.utotal()
+    madelung = -2.0 * r["u_tot"] / len(q)
+    assert abs(madelung - 1.747564594633) < 2e-6
+    assert np.abs(r["force"]).max() < 1e-5  # every ion sits at a symmetric site
+
+
+def test_spme_converges_to_std_ewald(orc):
+    """The reference's TestEwaldSPM recipe: high-order fine-mesh SPME equals std Ewald."""
+    sp = S.small_spme_system(n=60, L=24.0, alpha=0.35, rcut=10.0, form=0)
+    pot = dict(sp.pot, utoltab=1e-9, ftoltab=1e-9)
+    tab = orc.build_table(pot)
+    common = dict(npt=2, list_mode=0, surf=True, alpha=sp.alpha)
+    std = orc.System(sp.ro, sp.ipt, sp.az, sp.box, sp.rcut, sp.pot["epsi4pi"], tab, ewald=1,
+                     ncut=16, **common).utotal()
+    spm = orc.System(sp.ro, sp.ipt, sp.az, sp.box, sp.rcut, sp.pot["epsi4pi"], tab, ewald=2,
+                     K=64, order=10, **common).utotal()
+    assert abs(spm["u_rec"] - std["u_rec"]) < 1e-7 * abs(std["u_rec"]) + 1e-8
+    assert np.allclose(spm["force"], std["force"], atol=1e-6 * np.abs(std["force"]).max())
+    assert abs(spm["virial"] - std["virial"]) < 1e-6 * abs(std["virial"])
+
+
+@pytest.mark.parametrize("ewald", [1, 2])
+def test_forces_are_minus_energy_gradient(orc, ewald):
+    sp = S.small_spme_system(n=40, L=20.0, alpha=0.4, rcut=8.0, form=1, nmesh=32)
+    tab = orc.build_table(sp.pot)
+    sysm = orc.System(sp.ro, sp.ipt, sp.az, sp.box, sp.rcut, sp.pot["epsi4pi"], tab, npt=2,
+                      list_mode=1, ewald=ewald, surf=True, alpha=sp.alpha, ncut=8, K=32, order=6)
+    base = sysm.utotal()
+    ro0 = sp.ro.copy()
+    h = 1e-5
+    for i in (0, 7, 23):
+        for d in range(3):
+            rp = ro0.copy(); rp[i, d] += h
+            rm = ro0.copy(); rm[i, d] -= h
+            sysm.set_positions(rp); ep = sysm.utotal()["u_tot"]
+            sysm.set_positions(rm); em = sysm.utotal()["u_tot"]
+            fd = -(ep - em) / (2 * h)
+            # the table is accurate to 1e-5 in U and F; the SPME force is the analytic
+            # derivative of the SPME energy
+            assert abs(fd - base["force"][i, d]) < 2e-3 * max(1.0, abs(fd))
+    sysm.set_positions(ro0)
+
+
+def test_translation_invariance_and_newton3(orc):
+    sp = S.small_spme_system(n=80, L=30.0, alpha=0.3, rcut=10.0, form=1, nmesh=32, lsurf=False)
+    tab = orc.build_table(sp.pot)
+    mk = lambda ro: orc.System(ro, sp.ipt, sp.az, sp.box, sp.rcut, sp.pot["epsi4pi"], tab, npt=2,
+                               list_mode=1, ewald=2, surf=False, alpha=sp.alpha, K=32, order=6)
+    a = mk(sp.ro).utotal()
+    # shift by a whole number of mesh spacings: SPME is exactly invariant
+    sh = sp.ro + np.array([3, -5, 2]) * (30.0 / 32)
+    sh = (sh + 15.0) % 30.0 - 15.0
+    b = mk(sh).utotal()
+    assert abs(a["u_tot"] - b["u_tot"]) < 1e-9 * abs(a["u_tot"])
+    assert np.allclose(a["force"], b["force"], atol=1e-8)
+    # real-space pair forces sum to zero; SPME's net force is a small discretisation error
+    assert np.abs(a["force"].sum(0)).max() < 1e-3 * np.abs(a["force"]).max()
+
+
+def test_table_matches_potential(orc):
+    sp = S.polyelectrolyte.__wrapped__ if hasattr(S.polyelectrolyte, "__wrapped__") else None
+    pot = S._pot(2, [-1.0, 1.0], 1, 20.0, S.relpermitt_from_bjerrum(7.1), 0.15, sig=[4.0, 4.0],
+                 eps=[0.0248, 0.0248], rad=[2.0, 2.0])
+    tab = orc.build_table(pot)
+    ub = tab["ubuf"].reshape(-1, 12)
+    for tp, (it, jt) in enumerate([(0, 0), (0, 1), (1, 1)]):
+        lo = tab["iubuflow"][tp] // 12
+        n = tab["nugrid"][tp]
+        seg = ub[lo:lo + n]
+        assert np.all(np.diff(seg[:, 0]) < 0)                     # zlow strictly decreasing
+        assert abs(seg[-1, 0] - tab["r2umin"][tp]) < 1e-12
+        rr = np.sqrt(np.linspace(max(tab["r2umin"][tp], 16.0), 400.0, 3001))
+        for r in rr:
+            r2 = r * r
+            k = np.argmax(seg[:, 0] <= r2)                         # first interval with zlow <= r2
+            c = seg[k]
+            d = r2 - c[0]
+            u = c[1] + d * (c[2] + d * (c[3] + d * (c[4] + d * (c[5] + d * c[6]))))
+            f = c[7] + d * (c[8] + d * (c[9] + d * (c[10] + d * c[11])))
+            u0, u1, _ = orc.pot_eval(pot, it, jt, r)
+            assert abs(u - u0) < 1.5e-5
+            assert abs(f * r + u1) < 1.5e-5 * max(1.0, r)
+        # value continuity at interval joints
+        for a, b in zip(seg[:-1], seg[1:]):
+            d = a[0] - b[0]
+            ub_end = b[1] + d * (b[2] + d * (b[3] + d * (b[4] + d * (b[5] + d * b[6]))))
+            assert abs(ub_end - a[1]) < 1e-9 * max(1.0, abs(a[1]))
+
+
+def test_cell_list_matches_brute_force(orc):
+    rng = np.random.default_rng(7)
+    for L, rc, n in [(50.0, 10.0, 300), (40.0, 13.0, 200), (30.0, 15.0, 100), (57.0, 6.0, 900)]:
+        ro = rng.uniform(-L / 2, L / 2, size=(n, 3))
+        nc, cs, st = orc.cell_init([L, L, L], rc, rc)
+        head, nxt, npart, cellof = orc.cell_build(ro, [L, L, L], nc, cs)
+        members = orc.cell_members(head, nxt)
+        # every particle exactly once, in descending index order (head insertion)
+        assert sorted(sum(members, [])) == list(range(n))
+        for m in members:
+            assert m == sorted(m, reverse=True)
+        assert np.array_equal(npart, [len(m) for m in members])
+        # every pair within rc lies in neighbouring cells (TestCellList semantics)
+        ncell = int(np.prod(nc))
+        cxyz = np.stack([np.arange(ncell) % nc[0], (np.arange(ncell) // nc[0]) % nc[1],
+                         np.arange(ncell) // (nc[0] * nc[1])], 1)
+        neigh = set()
+        for c in range(ncell):
+            for s in st:
+                nb = (cxyz[c] + s) % nc
+                neigh.add((c, int(nb[0] + nc[0] * (nb[1] + nc[1] * nb[2]))))
+        d = ro[:, None, :] - ro[None, :, :]
+        d -= L * np.round(d / L)
+        r2 = (d ** 2).sum(-1)
+        ii, jj = np.nonzero(np.triu(r2 < rc * rc, 1))
+        for i, j in zip(ii, jj):
+            assert (cellof[i], cellof[j]) in neigh
+        # stencil sorted by Manhattan distance
+        man = np.abs(st).sum(1)
+        assert np.all(np.diff(man) >= 0)
+
+
+@pytest.mark.parametrize("ewald", [1, 2])
+def test_dutotal_equals_energy_difference(orc, ewald):
+    sp = S.small_spme_system(n=120, L=36.0, alpha=0.3, rcut=11.0, form=1, nmesh=32)
+    tab = orc.build_table(sp.pot)
+    args = dict(npt=2, list_mode=1, ewald=ewald, surf=True, alpha=sp.alpha, ncut=7, K=32, order=6)
+    sysm = orc.System(sp.ro, sp.ipt, sp.az, sp.box, sp.rcut, sp.pot["epsi4pi"], tab, **args)
+    u0 = sysm.utotal()
+    rng = np.random.default_rng(3)
+    checked = 0
+    for _ in range(12):
+        ip = int(rng.integers(sp.n))
+        rnew = sp.ro[ip] + rng.uniform(-1, 1, 3)
+        rnew = (rnew + 18.0) % 36.0 - 18.0
+        du = sysm.dutotal(ip, rnew)
+        if du["overlap"]:
+            continue
+        ro2 = sp.ro.copy(); ro2[ip] = rnew
+        s2 = orc.System(ro2, sp.ipt, sp.az, sp.box, sp.rcut, sp.pot["epsi4pi"], tab, **args)
+        u1 = s2.utotal()
+        assert abs(du["du_tot"] - (u1["u_tot"] - u0["u_tot"])) < 1e-9 * abs(u0["u_tot"])
+        assert np.allclose(du["du_twob"], u1["u_twob"] - u0["u_twob"], atol=1e-9 * abs(u0["u_tot"]))
+        checked += 1
+    assert checked >= 6
