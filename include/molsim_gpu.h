@@ -1,0 +1,173 @@
+/*
+ * molsim_gpu.h -- C-ABI of the MI355X (gfx950) non-bonded energy/force path for MOLSIM.
+ *
+ * The entry points mirror MOLSIM's own call surface for the charge / monoatomic branch:
+ *
+ *   msg_build_table  <- PotTwoBodyTab1/2/3        Src/potential.F90:868-1131
+ *   msg_set_table    <- the ubuf/iubuflow/r2umin/r2atat state those routines leave behind
+ *                       (Src/mol.F90:1087-1135, Src/particle.F90:2300-2308)
+ *   msg_ewald_setup  <- EwaldSetup                Src/energy.F90:6601-7126
+ *   msg_nlist        <- NList / InitCellList / SetCellList   Src/nlist.F90:553-669,
+ *                       Src/celllist.F90:76-410
+ *   msg_utotal       <- UTotal(iStage)            Src/energy.F90:280-434
+ *   msg_dutotal      <- DUTotal(lhsoverlap, lhepoverlap) for single-particle moves
+ *                                                 Src/denergy.F90:98-279,1240-1543
+ *   msg_commit       <- MCUpdate: EwaldUpdateArray + UpdateCellIp + ro <- rotm
+ *                                                 Src/mc.F90:5203-5235, Src/denergy.F90:1776-1819,
+ *                                                 Src/celllist.F90:366-382
+ *
+ * Conventions (those of the Fortran caller, so an iso_c_binding interface passes its
+ * arrays straight through):
+ *   - positions ro[3*i+d] (Fortran ro(3,np), real(8)), charges az[i], forces force[3*i+d];
+ *   - particle types iptpn[i] and particle indices (ipnptm) are 1-based;
+ *   - iubuflow[tp] is the 1-based start of type pair tp's records in ubuf (12 doubles each);
+ *   - type pairs are numbered as MOLSIM's iptpt: (1,1),(1,2),...,(1,npt),(2,2),...;
+ *   - energies in the caller's energy unit (kJ/mol), lengths in Angstrom; u_twob[0] is the
+ *     sum over type pairs and u_twob[1..nptpt] the per-pair sums, as u%twob(0:nptpt);
+ *     u_rec already contains EpsiFourPi, the self and the surface term, as u%rec.
+ * Every function returns MSG_OK (0) or a negative MSG_ERR_* code; msg_last_error() gives text.
+ * A context is used from one host thread at a time.
+ */
+#ifndef MOLSIM_GPU_H
+#define MOLSIM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSG_MAX_TYPES 4
+
+enum msg_status {
+  MSG_OK = 0,
+  MSG_ERR_ARG = -1,        /* bad argument / state                                     */
+  MSG_ERR_R2UMIN = -2,     /* r2 < r2umin: below the table       (Src/energy.F90:710)  */
+  MSG_ERR_IBUF = -3,       /* table scan past its end             (Src/energy.F90:717)  */
+  MSG_ERR_TABLE = -4,      /* table construction did not converge (Src/potential.F90:1117) */
+  MSG_ERR_TABLE_MEM = -5,  /* table larger than the buffer given                         */
+  MSG_ERR_ODD_MESH = -6,   /* nmesh must be even                  (Src/energy.F90:6891) */
+  MSG_ERR_MESH = -7,       /* nmesh unsupported by the GPU FFT (power of two, 16..256)   */
+  MSG_ERR_HIP = -8,        /* HIP runtime error                                          */
+  MSG_ERR_NOMEM = -9,      /* device allocation failed                                   */
+  MSG_ERR_CELL = -10,      /* cell list: rcell <= 0 or position outside the box          */
+  MSG_ERR_NLIST = -11,     /* Verlet list overflow (nneighpn > maxnneigh, Src/nlist.F90:752) */
+  MSG_ERR_STATE = -12      /* call order: e.g. utotal before set_table                   */
+};
+
+enum msg_list_mode { MSG_LIST_ALL = 0, MSG_LIST_CELL = 1, MSG_LIST_VERLET = 2 };
+enum msg_ewald_mode { MSG_EWALD_NONE = 0, MSG_EWALD_STD = 1, MSG_EWALD_SPM = 2 };
+enum msg_precision {
+  MSG_PREC_MIXED = 0,  /* FP32 pair/spread/FFT arithmetic, FP64 accumulators and Ewald terms */
+  MSG_PREC_FP64 = 1    /* the same kernels in double: the plain FP64 GPU path               */
+};
+
+/* nmlPotential / nmlIntList / nmlSystem parameters the path needs (Src/potential.F90:307-392,
+ * Src/nlist.F90:198-236). */
+typedef struct msg_params {
+  int np;              /* number of particles (one atom each)                  */
+  int npt;             /* number of particle types, <= MSG_MAX_TYPES            */
+  double boxlen[3];    /* box lengths (box-like cell, periodic in x, y, z)      */
+  double rcut;         /* real-space cut-off                                    */
+  double drnlist;      /* list skin; cell size >= rcut + drnlist                */
+  double epsi4pi;      /* EpsiFourPi = Epsi0FourPi / relpermitt                 */
+  int list_mode;       /* msg_list_mode (txintlist: vlist-equivalent / celllist / verlet) */
+  int ewald_mode;      /* msg_ewald_mode (lewald, txewaldrec)                   */
+  int lsurf;           /* surface (dipole) term                                 */
+  double alpha;        /* ualpha                                                */
+  int ncut;            /* std Ewald k-space cut-off                             */
+  int ncutregion;      /* 0 sphere, 1 cube                                      */
+  int nmesh;           /* SPME mesh points per dimension                        */
+  int order;           /* SPME B-spline order                                   */
+  int precision;       /* msg_precision                                         */
+  double force_scale;  /* forces are multiplied by this at the end; UTotal uses
+                          (sclene/scllen)/sclfor (Src/energy.F90:408); 1 keeps energy/length */
+} msg_params;
+
+/* Potential description for msg_build_table (form 0: 'default' with the 1/r term only,
+ * Src/potential.F90:1266-1341; form 1: '(1,6,12)', Src/potential.F90:1349-1425). */
+typedef struct msg_pot {
+  int npt, form, lewald, pad;
+  double alpha, rcut, epsi4pi;
+  double utoltab, ftoltab, umaxtab, fmaxtab, r2uminin;
+  double z[8], sig[8], eps[8], rad[8];
+} msg_pot;
+
+typedef struct msg_ctx msg_ctx;
+
+const char* msg_last_error(void);
+
+/* Host-side table construction (PotTwoBodyTab1/2/3 + SetUBuffer + CheckUBuffer,
+ * Src/potential.F90:868-1258).  Outputs use the C-ABI conventions above (iubuflow 1-based).
+ * Returns the number of doubles written to ubuf (> 0) or an error code. */
+int msg_build_table(const msg_pot* pot, double* ubuf, int maxbuf, int* iubuflow, int* nugrid,
+                    double* r2umin, double* r2atat);
+
+int msg_create(const msg_params* params, msg_ctx** out);
+void msg_destroy(msg_ctx* ctx);
+
+/* Table state as left by PotTwoBodyTab (nptpt = npt*(npt+1)/2 entries per array). */
+int msg_set_table(msg_ctx* ctx, const double* ubuf, int nbuf, const int* iubuflow,
+                  const int* nugrid, const double* r2umin, const double* r2atat);
+
+/* Charges (az) and 1-based particle types (iptpn).  Fixed for the context's life time. */
+int msg_set_particles(msg_ctx* ctx, const double* az, const int* iptpn);
+
+/* Positions ro(3,np) from host memory, or from device memory already on this GPU. */
+int msg_set_positions(msg_ctx* ctx, const double* ro);
+int msg_set_positions_device(msg_ctx* ctx, const double* d_ro);
+
+/* EwaldSetup: std k-factors or SPME influence function for the current box/parameters. */
+int msg_ewald_setup(msg_ctx* ctx);
+
+/* NList(iWriteInput/iSimulationStep): build the cell list (and the Verlet list in
+ * MSG_LIST_VERLET mode) from the current positions. msg_utotal rebuilds the cell list
+ * itself on every call; in Verlet mode it rebuilds the list on skin violation. */
+int msg_nlist(msg_ctx* ctx);
+
+/* Cell list after msg_nlist / msg_utotal: nc[3] cells per dimension, cell_start[ncell+1]
+ * offsets (0-based) into cell_ip[np], the 1-based particle indices of each cell (cell id
+ * ix + nc0*(iy + nc1*iz)) in the order MOLSIM's linked list visits them (iphead, ipnext, ...). */
+int msg_get_cell_list(msg_ctx* ctx, int* nc, int* cell_start, int* cell_ip);
+
+/* UTotal: u_twob[nptpt+1], *u_rec, *u_tot, force[3*np], *virial (any output may be NULL). */
+int msg_utotal(msg_ctx* ctx, double* u_twob, double* u_rec, double* u_tot, double* force,
+               double* virial);
+
+/* Asynchronous UTotal on the context's stream; results stay on the device until
+ * msg_fetch.  d_force (may be NULL) receives the forces in device memory. */
+int msg_utotal_async(msg_ctx* ctx, double* d_force);
+int msg_fetch(msg_ctx* ctx, double* u_twob, double* u_rec, double* u_tot, double* virial);
+
+/* DUTotal for nptm moved particles ipnptm[0..nptm) (1-based) to rotm(3,nptm).
+ * *lhsoverlap = 1 on a hard-core overlap (then only the flag is meaningful, as in the
+ * reference).  du_twob[nptpt+1], *du_rec (x EpsiFourPi, with the surface term), *du_tot. */
+int msg_dutotal(msg_ctx* ctx, int nptm, const int* ipnptm, const double* rotm, int* lhsoverlap,
+                double* du_twob, double* du_rec, double* du_tot);
+
+/* Batched DUTotal: ntrial independent single-particle trials against the SAME current
+ * state (trial t moves ip[t] (1-based) to rotm(3,t)).  Outputs per trial. */
+int msg_dutotal_batch(msg_ctx* ctx, int ntrial, const int* ip, const double* rotm,
+                      int* lhsoverlap, double* du_tot, double* du_rec);
+
+/* MCUpdate after an accepted move: positions, resident structure factors / mesh, cell list. */
+int msg_commit(msg_ctx* ctx, int nptm, const int* ipnptm, const double* rotm);
+
+/* Stream the context launches on (a hipStream_t), for callers that time kernels. */
+void* msg_stream(msg_ctx* ctx);
+
+/* Per-kernel timing with HIP events on the context's stream.  Kernel ids: */
+enum msg_kernel_id {
+  MSG_K_BIN = 0, MSG_K_CELLSORT, MSG_K_PAIR, MSG_K_SPREAD, MSG_K_FFT_X, MSG_K_FFT_Y,
+  MSG_K_FFT_ZCONV, MSG_K_FFT_YI, MSG_K_FFT_XI, MSG_K_GATHER, MSG_K_REDUCE, MSG_K_EWSTD,
+  MSG_K_DE, MSG_K_VLIST, MSG_K_COUNT
+};
+int msg_profile(msg_ctx* ctx, int enable);
+int msg_profile_read(msg_ctx* ctx, int kernel_id, double* total_ms, int* launches);
+const char* msg_kernel_name(int kernel_id);
+
+/* Bytes of device memory the context holds (for admission by HBM footprint). */
+long long msg_device_bytes(msg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

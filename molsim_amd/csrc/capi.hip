@@ -1,0 +1,529 @@
+// C-ABI of the MI355X non-bonded path: context life cycle, state transfer, the UTotal launch
+// sequence (captured once into a hipGraph for the mixed-precision path), the final FP64
+// reduction, error flags and per-kernel event timing.
+#include <cstring>
+#include <mutex>
+
+#include "common.h"
+
+namespace msg {
+
+static thread_local std::string g_err = "";
+
+int set_error(int code, const std::string& text) {
+  g_err = text;
+  return code;
+}
+
+int dalloc(Ctx* c, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return set_error(MSG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  c->bytes += (long long)bytes;
+  return MSG_OK;
+}
+
+static const char* kKernelNames[MSG_K_COUNT] = {
+    "bin", "cellsort", "pair", "spread", "fft_x", "fft_y", "fft_zconv", "fft_yi", "fft_xi",
+    "gather", "reduce", "ewald_std", "dutotal", "vlist"};
+
+void prof_begin(Ctx* c, int kid) {
+  if (!c->prof) return;
+  if (c->ev_used + 2 > (int)(sizeof(c->ev) / sizeof(c->ev[0]))) {
+    // pool exhausted: drain it
+    hipStreamSynchronize(c->stream);
+    for (int k = 0; k + 1 < c->ev_used; k += 2) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]);
+      c->prof_ms[c->ev_kernel[k / 2]] += ms;
+      c->prof_n[c->ev_kernel[k / 2]] += 1;
+    }
+    c->ev_used = 0;
+    c->ev_kernel.clear();
+  }
+  hipEventRecord(c->ev[c->ev_used], c->stream);
+  c->ev_kernel.push_back(kid);
+}
+
+void prof_end(Ctx* c, int kid) {
+  (void)kid;
+  if (!c->prof) return;
+  hipEventRecord(c->ev[c->ev_used + 1], c->stream);
+  c->ev_used += 2;
+}
+
+static void prof_drain(Ctx* c) {
+  if (!c->prof || c->ev_used == 0) return;
+  hipStreamSynchronize(c->stream);
+  for (int k = 0; k + 1 < c->ev_used; k += 2) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]);
+    c->prof_ms[c->ev_kernel[k / 2]] += ms;
+    c->prof_n[c->ev_kernel[k / 2]] += 1;
+  }
+  c->ev_used = 0;
+  c->ev_kernel.clear();
+}
+
+namespace {
+
+__global__ void k_zero(int* a, int na, int* b, int nb, int* flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < na) a[i] = 0;
+  if (i < nb) b[i] = 0;
+  if (i == 0) flags[0] = 0;
+}
+
+struct ReduceArgs {
+  const double* pair;  int npair; int ntp;
+  const double* spme;  int nspme;
+  const double* ewstd; int newstd;
+  const double* dip;   int ndip;
+  int ewald;           // 0 none, 1 std, 2 spm
+  int lsurf;
+  double self_raw;     // -alpha/sqrt(pi) sum q^2
+  double surf_fac;     // 2 pi / (3 V)
+  double epsi4pi;
+  double* out;
+};
+
+// Final FP64 reduction of all partial sums in a fixed order (deterministic).
+__global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
+  __shared__ double sh[8];
+  __shared__ double vals[kMaxTP + 8];
+  for (int tp = 0; tp <= a.ntp; ++tp) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < a.npair; b += 256) s += a.pair[b * (a.ntp + 1) + tp];
+    s = msgd::block_sum(s, sh);
+    if (threadIdx.x == 0) vals[tp] = 0.5 * s;  // every pair was evaluated from both sides
+  }
+  double es = 0.0, vs = 0.0;
+  if (a.ewald == 2) {
+    for (int b = threadIdx.x; b < a.nspme; b += 256) {
+      es += a.spme[2 * b];
+      vs += a.spme[2 * b + 1];
+    }
+  } else if (a.ewald == 1) {
+    for (int b = threadIdx.x; b < a.newstd; b += 256) {
+      es += a.ewstd[2 * b];
+      vs += a.ewstd[2 * b + 1];
+    }
+  }
+  es = msgd::block_sum(es, sh);
+  vs = msgd::block_sum(vs, sh);
+  double m[3];
+  for (int d = 0; d < 3; ++d) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < a.ndip; b += 256) s += a.dip[3 * b + d];
+    m[d] = msgd::block_sum(s, sh);
+  }
+  if (threadIdx.x == 0) {
+    double* o = a.out;
+    double tw = 0.0;
+    for (int tp = 0; tp < a.ntp; ++tp) {
+      o[1 + tp] = vals[tp];
+      tw += vals[tp];
+    }
+    o[0] = tw;
+    double urec = 0.0, virrec = 0.0;
+    if (a.ewald) {
+      // SPME: E = sum w G |F|^2, virrec -= sum w G (1 + 2 m^2 expf) |F|^2
+      // std:  E = sum kfac |S|^2, virrec -= sum (1 - k^2/(2 alpha^2)) term (partials hold +sum)
+      urec = es + a.self_raw;
+      virrec = -vs;
+      if (a.lsurf) {
+        const double term = a.surf_fac * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        urec += term;
+        virrec -= term;
+      }
+    }
+    o[a.ntp + 1] = a.epsi4pi * urec;
+    o[a.ntp + 2] = tw + a.epsi4pi * urec;
+    o[a.ntp + 3] = vals[a.ntp] + a.epsi4pi * virrec;
+    o[a.ntp + 4] = m[0];
+    o[a.ntp + 5] = m[1];
+    o[a.ntp + 6] = m[2];
+  }
+}
+
+}  // namespace
+
+int launch_zero(Ctx* c) {
+  const int na = c->grid.ncell;
+  const int nb = c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.nbrick : 0;
+  const int n = std::max(std::max(na, nb), 1);
+  hipLaunchKernelGGL(k_zero, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_cell_count, na,
+                     c->d_brick_count, nb, c->d_flags);
+  MSG_HIP(hipGetLastError());
+  return MSG_OK;
+}
+
+int launch_reduce(Ctx* c) {
+  ReduceArgs a;
+  a.pair = c->d_part + c->part_off_pair;
+  a.npair = c->npart_pair;
+  a.ntp = c->nptpt;
+  a.spme = c->d_part + c->part_off_spme;
+  a.nspme = c->nzblocks;
+  a.ewstd = c->d_part + c->part_off_ewstd;
+  a.newstd = c->npart_ewstd;
+  a.dip = c->d_part + c->part_off_dip;
+  a.ndip = c->npart_dip;
+  a.ewald = c->P.ewald_mode;
+  a.lsurf = c->P.lsurf;
+  a.self_raw = -c->P.alpha / std::sqrt(M_PI) * c->sumq2;
+  const double vol = c->P.boxlen[0] * c->P.boxlen[1] * c->P.boxlen[2];
+  a.surf_fac = 2.0 * M_PI / (3.0 * vol);
+  a.epsi4pi = c->P.epsi4pi;
+  a.out = c->d_out;
+  prof_begin(c, MSG_K_REDUCE);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->stream, a);
+  prof_end(c, MSG_K_REDUCE);
+  MSG_HIP(hipGetLastError());
+  return MSG_OK;
+}
+
+// The UTotal launch sequence (no host synchronisation inside; capturable).
+static int utotal_sequence(Ctx* c) {
+  int e;
+  if ((e = launch_zero(c))) return e;
+  if ((e = launch_bin(c))) return e;
+  if ((e = launch_sort(c))) return e;
+  if ((e = launch_pair(c, c->P.ewald_mode == MSG_EWALD_NONE))) return e;
+  if (c->P.ewald_mode == MSG_EWALD_SPM) {
+    if ((e = launch_spme(c))) return e;
+  } else if (c->P.ewald_mode == MSG_EWALD_STD) {
+    if ((e = launch_ewstd(c))) return e;
+  }
+  return launch_reduce(c);
+}
+
+static int flags_to_error(int f);
+
+int utotal_refresh(Ctx* c) {
+  if (int e = utotal_sequence(c)) return e;
+  MSG_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  if (int e = flags_to_error(c->h_flags[0])) return e;
+  c->fresh = true;
+  return MSG_OK;
+}
+
+static int check_ready(Ctx* c) {
+  if (!c->have_table) return set_error(MSG_ERR_STATE, "msg_set_table has not been called");
+  if (!c->have_particles) return set_error(MSG_ERR_STATE, "msg_set_particles has not been called");
+  if (!c->have_positions) return set_error(MSG_ERR_STATE, "positions have not been set");
+  if (c->P.ewald_mode != MSG_EWALD_NONE && !c->have_ewald)
+    return set_error(MSG_ERR_STATE, "msg_ewald_setup has not been called");
+  return MSG_OK;
+}
+
+static int flags_to_error(int f) {
+  if (f & FLAG_R2UMIN) return set_error(MSG_ERR_R2UMIN, "pair below the table: r2 < r2umin");
+  if (f & FLAG_CELL) return set_error(MSG_ERR_CELL, "position outside the box");
+  if (f & FLAG_NLIST) return set_error(MSG_ERR_NLIST, "neighbour list overflow");
+  if (f & FLAG_IBUF) return set_error(MSG_ERR_IBUF, "table scan past its end");
+  return MSG_OK;
+}
+
+}  // namespace msg
+
+using namespace msg;
+
+extern "C" {
+
+const char* msg_last_error(void) { return g_err.c_str(); }
+
+const char* msg_kernel_name(int k) {
+  return (k >= 0 && k < MSG_K_COUNT) ? kKernelNames[k] : "?";
+}
+
+int msg_create(const msg_params* p, msg_ctx** out) {
+  if (!p || !out) return set_error(MSG_ERR_ARG, "msg_create: null argument");
+  *out = nullptr;
+  if (p->np < 1 || p->npt < 1 || p->npt > MSG_MAX_TYPES)
+    return set_error(MSG_ERR_ARG, "msg_create: np >= 1 and 1 <= npt <= 4 required");
+  for (int d = 0; d < 3; ++d)
+    if (!(p->boxlen[d] > 0)) return set_error(MSG_ERR_ARG, "msg_create: box lengths must be > 0");
+  if (!(p->rcut > 0)) return set_error(MSG_ERR_CELL, "msg_create: rcut must be > 0");
+  if (p->list_mode < 0 || p->list_mode > 2 || p->ewald_mode < 0 || p->ewald_mode > 2)
+    return set_error(MSG_ERR_ARG, "msg_create: bad list/ewald mode");
+  if (p->list_mode == MSG_LIST_VERLET)
+    return set_error(MSG_ERR_ARG, "msg_create: Verlet lists are not available in this build");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_error(MSG_ERR_HIP, "msg_create: no HIP device");
+  msg_ctx* h = new msg_ctx();
+  Ctx* c = &h->c;
+  c->P = *p;
+  if (c->P.force_scale == 0.0) c->P.force_scale = 1.0;
+  c->np = p->np;
+  c->npt = p->npt;
+  c->nptpt = p->npt * (p->npt + 1) / 2;
+  c->fp64 = p->precision == MSG_PREC_FP64;
+  hipGetDevice(&c->device);
+  int e = MSG_OK;
+  auto fail = [&](int code) {
+    msg_destroy(h);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_error(MSG_ERR_HIP, "hipStreamCreate failed"));
+  const int np = c->np;
+  if ((e = dalloc(c, (void**)&c->d_ro, 3 * np * sizeof(double)))) return fail(e);
+  if ((e = dalloc(c, (void**)&c->d_az, np * sizeof(double)))) return fail(e);
+  if ((e = dalloc(c, (void**)&c->d_ipt, np * sizeof(int)))) return fail(e);
+  if ((e = dalloc(c, (void**)&c->d_force, 3 * np * sizeof(double)))) return fail(e);
+  if ((e = lists_setup(c))) return fail(e);
+  // partial-sum layout
+  c->npart_pair = pair_blocks(c);
+  c->npart_dip = (np + 255) / 256;
+  c->part_off_pair = 0;
+  c->part_off_dip = c->part_off_pair + c->npart_pair * (kMaxTP + 1);
+  c->part_off_spme = c->part_off_dip + 3 * c->npart_dip;
+  const int max_spme_blocks = 2 * 2 * 256 * 130;  // upper bound for nzblocks * 2 (K <= 256)
+  c->part_off_ewstd = c->part_off_spme + max_spme_blocks;
+  const int nparts = c->part_off_ewstd + 2 * 4096;
+  if ((e = dalloc(c, (void**)&c->d_part, nparts * sizeof(double)))) return fail(e);
+  if ((e = dalloc(c, (void**)&c->d_out, 64 * sizeof(double)))) return fail(e);
+  if ((e = dalloc(c, (void**)&c->d_flags, 16 * sizeof(int)))) return fail(e);
+  if (hipHostMalloc((void**)&c->h_out, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    return fail(set_error(MSG_ERR_NOMEM, "hipHostMalloc failed"));
+  if (hipHostMalloc((void**)&c->h_flags, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+    return fail(set_error(MSG_ERR_NOMEM, "hipHostMalloc failed"));
+  for (auto& ev : c->ev) hipEventCreate(&ev);
+  *out = h;
+  return MSG_OK;
+}
+
+void msg_destroy(msg_ctx* h) {
+  if (!h) return;
+  Ctx* c = &h->c;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  void* ptrs[] = {c->d_ro, c->d_az, c->d_ipt, c->d_tcoef, c->d_tbins, c->d_cellid, c->d_cell_chunk,
+                  c->d_cell_count, c->d_cell_start, c->d_cell_fill, c->d_cell_tmp, c->d_cell_ip,
+                  c->d_slot_of, c->d_cpos, c->d_ctype, c->d_slot_cell, c->d_brick_count,
+                  c->d_brick_start, c->d_brick_fill, c->d_brick_tmp, c->d_brick_ip, c->d_mm,
+                  c->d_frac, c->d_Q, c->d_C, c->d_G, c->d_tw, c->d_kfac, c->d_kvec, c->d_sk,
+                  c->d_force, c->d_part, c->d_out, c->d_flags, c->d_de, c->d_de_ip, c->d_de_r,
+                  c->d_tcoef64, c->d_gk};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (c->h_out) hipHostFree(c->h_out);
+  if (c->h_flags) hipHostFree(c->h_flags);
+  for (auto& ev : c->ev)
+    if (ev) hipEventDestroy(ev);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete h;
+}
+
+int msg_set_table(msg_ctx* h, const double* ubuf, int nbuf, const int* iubuflow, const int* nugrid,
+                  const double* r2umin, const double* r2atat) {
+  if (!h || !ubuf || nbuf < 12 || !iubuflow || !nugrid || !r2umin || !r2atat)
+    return set_error(MSG_ERR_ARG, "msg_set_table: bad arguments");
+  Ctx* c = &h->c;
+  const int ntp = c->nptpt;
+  c->h_ubuf.assign(ubuf, ubuf + nbuf);
+  c->h_iubuflow.assign(iubuflow, iubuflow + ntp);
+  c->h_nugrid.assign(nugrid, nugrid + ntp);
+  c->h_r2umin.assign(r2umin, r2umin + ntp);
+  c->h_r2atat.assign(r2atat, r2atat + ntp);
+  if (int e = pack_table(c)) return e;
+  c->have_table = true;
+  c->graph_ok = false;
+  return MSG_OK;
+}
+
+int msg_set_particles(msg_ctx* h, const double* az, const int* iptpn) {
+  if (!h || !az || !iptpn) return set_error(MSG_ERR_ARG, "msg_set_particles: null argument");
+  Ctx* c = &h->c;
+  std::vector<int> t(c->np);
+  double s = 0.0;
+  for (int i = 0; i < c->np; ++i) {
+    t[i] = iptpn[i] - 1;
+    if (t[i] < 0 || t[i] >= c->npt) return set_error(MSG_ERR_ARG, "msg_set_particles: type out of range");
+    s += az[i] * az[i];
+  }
+  c->sumq2 = s;
+  MSG_HIP(hipMemcpy(c->d_az, az, c->np * sizeof(double), hipMemcpyHostToDevice));
+  MSG_HIP(hipMemcpy(c->d_ipt, t.data(), c->np * sizeof(int), hipMemcpyHostToDevice));
+  c->have_particles = true;
+  return MSG_OK;
+}
+
+int msg_set_positions(msg_ctx* h, const double* ro) {
+  if (!h || !ro) return set_error(MSG_ERR_ARG, "msg_set_positions: null argument");
+  Ctx* c = &h->c;
+  MSG_HIP(hipMemcpyAsync(c->d_ro, ro, 3 * c->np * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  c->have_positions = true;
+  c->fresh = false;
+  return MSG_OK;
+}
+
+int msg_set_positions_device(msg_ctx* h, const double* d_ro) {
+  if (!h || !d_ro) return set_error(MSG_ERR_ARG, "msg_set_positions_device: null argument");
+  Ctx* c = &h->c;
+  MSG_HIP(hipMemcpyAsync(c->d_ro, d_ro, 3 * c->np * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  c->have_positions = true;
+  c->fresh = false;
+  return MSG_OK;
+}
+
+int msg_ewald_setup(msg_ctx* h) {
+  if (!h) return set_error(MSG_ERR_ARG, "msg_ewald_setup: null context");
+  Ctx* c = &h->c;
+  int e = MSG_OK;
+  if (c->P.ewald_mode == MSG_EWALD_SPM) e = spme_setup(c);
+  else if (c->P.ewald_mode == MSG_EWALD_STD) e = ewstd_setup(c);
+  if (e) return e;
+  c->have_ewald = true;
+  c->have_gk = false;
+  c->fresh = false;
+  c->graph_ok = false;
+  return MSG_OK;
+}
+
+int msg_nlist(msg_ctx* h) {
+  if (!h) return set_error(MSG_ERR_ARG, "msg_nlist: null context");
+  Ctx* c = &h->c;
+  if (!c->have_positions) return set_error(MSG_ERR_STATE, "positions have not been set");
+  int e;
+  if ((e = launch_zero(c)) || (e = launch_bin(c)) || (e = launch_sort(c))) return e;
+  MSG_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  return flags_to_error(c->h_flags[0]);
+}
+
+int msg_utotal_async(msg_ctx* h, double* d_force) {
+  if (!h) return set_error(MSG_ERR_ARG, "msg_utotal_async: null context");
+  Ctx* c = &h->c;
+  if (int e = check_ready(c)) return e;
+  const bool use_graph = !c->fp64 && !c->prof;
+  if (use_graph) {
+    if (!c->graph_ok) {
+      if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+      if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
+      MSG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int e = utotal_sequence(c);
+      hipGraph_t g = nullptr;
+      hipError_t ce = hipStreamEndCapture(c->stream, &g);
+      if (e) { if (g) hipGraphDestroy(g); return e; }
+      if (ce != hipSuccess) return set_error(MSG_ERR_HIP, std::string("capture: ") + hipGetErrorString(ce));
+      c->graph = g;
+      MSG_HIP(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+      c->graph_ok = true;
+    }
+    MSG_HIP(hipGraphLaunch(c->gexec, c->stream));
+  } else {
+    if (int e = utotal_sequence(c)) return e;
+  }
+  c->fresh = true;
+  if (d_force && d_force != c->d_force)
+    MSG_HIP(hipMemcpyAsync(d_force, c->d_force, 3 * c->np * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  return MSG_OK;
+}
+
+int msg_fetch(msg_ctx* h, double* u_twob, double* u_rec, double* u_tot, double* virial) {
+  if (!h) return set_error(MSG_ERR_ARG, "msg_fetch: null context");
+  Ctx* c = &h->c;
+  MSG_HIP(hipMemcpyAsync(c->h_out, c->d_out, 64 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  MSG_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  prof_drain(c);
+  if (int e = flags_to_error(c->h_flags[0])) return e;
+  const int ntp = c->nptpt;
+  if (u_twob)
+    for (int k = 0; k <= ntp; ++k) u_twob[k] = c->h_out[k];
+  if (u_rec) *u_rec = c->h_out[ntp + 1];
+  if (u_tot) *u_tot = c->h_out[ntp + 2];
+  if (virial) *virial = c->h_out[ntp + 3];
+  return MSG_OK;
+}
+
+int msg_utotal(msg_ctx* h, double* u_twob, double* u_rec, double* u_tot, double* force,
+               double* virial) {
+  if (int e = msg_utotal_async(h, nullptr)) return e;
+  Ctx* c = &h->c;
+  if (force)
+    MSG_HIP(hipMemcpyAsync(force, c->d_force, 3 * c->np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return msg_fetch(h, u_twob, u_rec, u_tot, virial);
+}
+
+int msg_dutotal(msg_ctx* h, int nptm, const int* ipnptm, const double* rotm, int* lhsoverlap,
+                double* du_twob, double* du_rec, double* du_tot) {
+  if (!h || nptm != 1 || !ipnptm || !rotm || !lhsoverlap)
+    return set_error(MSG_ERR_ARG, "msg_dutotal: single-particle moves (nptm = 1) only");
+  Ctx* c = &h->c;
+  if (int e = check_ready(c)) return e;
+  double dt, dr;
+  std::vector<double> twob(c->nptpt + 1);
+  int ov = 0;
+  if (int e = dutotal_batch(c, 1, ipnptm, rotm, &ov, &dt, &dr, twob.data())) return e;
+  *lhsoverlap = ov;
+  if (du_twob)
+    for (int k = 0; k <= c->nptpt; ++k) du_twob[k] = twob[k];
+  if (du_rec) *du_rec = dr;
+  if (du_tot) *du_tot = dt;
+  return MSG_OK;
+}
+
+int msg_dutotal_batch(msg_ctx* h, int ntrial, const int* ip, const double* rotm, int* lhsoverlap,
+                      double* du_tot, double* du_rec) {
+  if (!h || ntrial < 1 || !ip || !rotm || !lhsoverlap || !du_tot)
+    return set_error(MSG_ERR_ARG, "msg_dutotal_batch: bad arguments");
+  Ctx* c = &h->c;
+  if (int e = check_ready(c)) return e;
+  return dutotal_batch(c, ntrial, ip, rotm, lhsoverlap, du_tot, du_rec, nullptr);
+}
+
+int msg_commit(msg_ctx* h, int nptm, const int* ipnptm, const double* rotm) {
+  if (!h || nptm != 1 || !ipnptm || !rotm)
+    return set_error(MSG_ERR_ARG, "msg_commit: single-particle moves (nptm = 1) only");
+  Ctx* c = &h->c;
+  if (ipnptm[0] < 1 || ipnptm[0] > c->np) return set_error(MSG_ERR_ARG, "msg_commit: ip out of range");
+  return commit_move(c, ipnptm[0] - 1, rotm);
+}
+
+int msg_get_cell_list(msg_ctx* h, int* nc, int* cell_start, int* cell_ip) {
+  if (!h || !nc) return set_error(MSG_ERR_ARG, "msg_get_cell_list: null argument");
+  Ctx* c = &h->c;
+  for (int d = 0; d < 3; ++d) nc[d] = c->grid.nc[d];
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  if (cell_start)
+    MSG_HIP(hipMemcpy(cell_start, c->d_cell_start, (c->grid.ncell + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  if (cell_ip) {
+    MSG_HIP(hipMemcpy(cell_ip, c->d_cell_ip, c->np * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < c->np; ++i) cell_ip[i] += 1;
+  }
+  return MSG_OK;
+}
+
+void* msg_stream(msg_ctx* h) { return h ? (void*)h->c.stream : nullptr; }
+
+int msg_profile(msg_ctx* h, int enable) {
+  if (!h) return set_error(MSG_ERR_ARG, "msg_profile: null context");
+  Ctx* c = &h->c;
+  prof_drain(c);
+  c->prof = enable != 0;
+  for (int k = 0; k < MSG_K_COUNT; ++k) {
+    c->prof_ms[k] = 0.0;
+    c->prof_n[k] = 0;
+  }
+  return MSG_OK;
+}
+
+int msg_profile_read(msg_ctx* h, int kid, double* total_ms, int* launches) {
+  if (!h || kid < 0 || kid >= MSG_K_COUNT) return set_error(MSG_ERR_ARG, "msg_profile_read: bad arguments");
+  Ctx* c = &h->c;
+  prof_drain(c);
+  if (total_ms) *total_ms = c->prof_ms[kid];
+  if (launches) *launches = c->prof_n[kid];
+  return MSG_OK;
+}
+
+long long msg_device_bytes(msg_ctx* h) { return h ? h->c.bytes : 0; }
+
+}  // extern "C"

@@ -1,0 +1,266 @@
+// Internal declarations of the MI355X non-bonded path (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/molsim_gpu.h"
+
+namespace msg {
+
+// ------------------------------------------------------------------------------------------
+// Error handling: every failure ends in set_error(code, text) and a negative return value.
+int set_error(int code, const std::string& text);
+
+#define MSG_HIP(call)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return ::msg::set_error(MSG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Device-side error flags (OR-ed into ctx->d_flags[0]).
+enum : int { FLAG_R2UMIN = 1, FLAG_IBUF = 2, FLAG_CELL = 4, FLAG_NLIST = 8 };
+
+constexpr int kMaxT = MSG_MAX_TYPES;
+constexpr int kMaxTP = kMaxT * (kMaxT + 1) / 2;
+constexpr int kTabBins = 256;        // r^2 bins of the interval index per type pair
+constexpr int kMaxTabIntervals = 512;   // records below rcut staged in LDS (<= 48 KB in FP64)
+constexpr int kMaxStencil = 27;
+constexpr int kPairHB = 4;           // home particles per pair-kernel work item
+// SPME bricks: each mesh point belongs to one brick; spread and gather work per brick.
+constexpr int kBX = 16, kBY = 8, kBZ = 8;
+constexpr int kMaxOrder = 9;         // order - 1 <= kBY, kBZ
+
+inline int tp_index(int npt, int i, int j) {
+  if (i > j) { int t = i; i = j; j = t; }
+  return i * npt - i * (i - 1) / 2 + (j - i);
+}
+
+// ------------------------------------------------------------------------------------------
+// Packed tabulated potential for the GPU (built from MOLSIM's ubuf on the host).
+// Intervals of all type pairs are concatenated in the reference's scan order
+// (decreasing zlow); per type pair: [first, last] interval range and a bin table over
+// r^2 in [zlo, rcut^2] that gives the first interval to scan from.
+struct GpuTableMeta {
+  int first[kMaxTP], last[kMaxTP];
+  double zlo[kMaxTP], invw[kMaxTP], r2atat[kMaxTP], r2umin[kMaxTP];
+  int tpidx[kMaxT * kMaxT];
+  int nint, ntp;
+};
+
+// Kernel argument blocks -------------------------------------------------------------------
+struct Box {
+  double L[3], half[3], inv[3];
+};
+
+struct CellGrid {
+  int nc[3];
+  int ncell;
+  double csi[3];  // inverse cell size
+  int nst;
+  int st[3 * kMaxStencil];
+};
+
+struct MeshGrid {
+  int K, H, order;
+  int nb[3];      // bricks per dimension
+  int nbrick;
+  double dkdr[3]; // K / L
+};
+
+enum KernelId : int;
+
+// ------------------------------------------------------------------------------------------
+struct Ctx {
+  msg_params P{};
+  int np = 0, npt = 0, nptpt = 0;
+  bool fp64 = false;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  long long bytes = 0;
+
+  // particles
+  double* d_ro = nullptr;    // 3*np, caller order
+  double* d_az = nullptr;    // np
+  int* d_ipt = nullptr;      // np, 0-based
+  bool have_particles = false, have_positions = false;
+  double sumq2 = 0.0;
+
+  // table
+  bool have_table = false;
+  GpuTableMeta tmeta{};
+  void* d_tcoef = nullptr;   // nint * 12 reals (float or double)
+  int* d_tbins = nullptr;    // ntp * kTabBins
+  std::vector<double> h_ubuf;
+  std::vector<int> h_iubuflow, h_nugrid;
+  std::vector<double> h_r2umin, h_r2atat;
+
+  // cell list
+  CellGrid grid{};
+  int* d_cellid = nullptr;       // np (per particle, caller order)
+  int* d_cell_count = nullptr;   // ncell
+  int* d_cell_start = nullptr;   // ncell+1
+  int* d_cell_fill = nullptr;    // ncell
+  int* d_cell_chunk = nullptr;   // ncell+1: offsets of the pair kernel's home chunks
+  int* d_cell_tmp = nullptr;     // np (unsorted members)
+  int* d_cell_ip = nullptr;      // np (sorted: cell-major, descending ip in a cell)
+  int* d_slot_of = nullptr;      // np: slot of particle ip in the cell order
+  void* d_cpos = nullptr;        // np * 4 reals? (always double4: x, y, z, q)
+  int* d_ctype = nullptr;        // np
+  int* d_slot_cell = nullptr;    // np
+
+  // SPME
+  bool have_ewald = false;
+  MeshGrid mesh{};
+  int* d_brick_count = nullptr, *d_brick_start = nullptr, *d_brick_fill = nullptr;
+  int* d_brick_tmp = nullptr, *d_brick_ip = nullptr;
+  int* d_mm = nullptr;           // np, packed meshmax (caller order)
+  void* d_frac = nullptr;        // np * 3 reals, caller order
+  void* d_Q = nullptr;           // K^3 reals (charge mesh, then phi)
+  void* d_C = nullptr;           // H*K*K complex
+  void* d_G = nullptr;           // H^3 reals (meshfac)
+  void* d_tw = nullptr;          // K complex twiddles exp(-2 pi i m / K)
+  int nzblocks = 0;              // partial sums of the z/conv pass
+
+  // std Ewald
+  int nkvec = 0;
+  double* d_kfac = nullptr;      // nkvec
+  int* d_kvec = nullptr;         // nkvec * 3
+  double* d_sk = nullptr;        // nkvec * 4 * 2 (structure factors, 4 sign octants)
+
+  // outputs and partials
+  double* d_force = nullptr;     // 3*np (caller order)
+  double* d_part = nullptr;      // partial sums (pair, spme, dipole)
+  int npart_pair = 0, npart_dip = 0, part_off_pair = 0, part_off_spme = 0, part_off_dip = 0;
+  int part_off_ewstd = 0, npart_ewstd = 0;
+  double* d_out = nullptr;       // [u_twob(0..nptpt), u_rec, u_tot, virial, dip x,y,z]
+  int* d_flags = nullptr;        // error flags
+  double* h_out = nullptr;       // pinned
+  int* h_flags = nullptr;        // pinned
+
+  // graph of the utotal launch sequence
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_ok = false;
+  double* graph_force_dst = nullptr;
+
+  // profiling
+  bool prof = false;
+  hipEvent_t ev[2 * MSG_K_COUNT * 64];
+  int ev_used = 0;
+  std::vector<int> ev_kernel;
+  double prof_ms[MSG_K_COUNT] = {0};
+  int prof_n[MSG_K_COUNT] = {0};
+
+  // dE: FP64 copy of the packed table, real-space influence kernel g (H^3, folded by
+  // symmetry), per-trial inputs/outputs, and whether phi / S / cells / M match the positions
+  double* d_tcoef64 = nullptr;
+  double* d_gk = nullptr;
+  bool have_gk = false;
+  double* d_de = nullptr;        // per trial: [du_twob(0..ntp), du_rec, du_tot, overlap]
+  int de_cap = 0;
+  int* d_de_ip = nullptr;
+  double* d_de_r = nullptr;
+  bool fresh = false;            // a UTotal has run on the current positions
+};
+
+// allocation helper (tracked in ctx->bytes)
+int dalloc(Ctx* c, void** p, size_t bytes);
+
+// Host-side table packing (table.cpp)
+int pack_table(Ctx* c);
+
+// Launch sequences (each .hip file)
+int lists_setup(Ctx* c);                           // cell grid + buffers
+int launch_bin(Ctx* c);                            // K1 cell id + mesh bin + counts + dipole
+int launch_sort(Ctx* c);                           // scan + scatter + finalize
+int launch_pair(Ctx* c, bool last);                // K4
+int pair_blocks(Ctx* c);                           // workgroups of the pair kernel
+int spme_setup(Ctx* c);                            // K-space arrays
+int launch_spme(Ctx* c);                           // spread, FFTs + conv, gather
+int ewstd_setup(Ctx* c);
+int launch_ewstd(Ctx* c);
+int launch_reduce(Ctx* c);                         // final energies
+int launch_zero(Ctx* c);                           // counters and flags
+int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ovl,
+                  double* du_tot, double* du_rec, double* du_twob);
+int commit_move(Ctx* c, int ip, const double* rnew);
+int utotal_refresh(Ctx* c);                        // UTotal sequence + sync (for dE state)
+
+// profiling wrappers around a launch
+void prof_begin(Ctx* c, int kid);
+void prof_end(Ctx* c, int kid);
+
+}  // namespace msg
+
+struct msg_ctx {
+  msg::Ctx c;
+};
+
+// ------------------------------------------------------------------------------------------
+// Device helpers shared by the kernel files.
+namespace msgd {
+
+// Cardinal B-spline values (and derivatives) of order n at fractional offset x in [0,1):
+// vals[i] = M_n(x+i), ders[i] = M_n'(x+i) (Schoenberg recursion, Src/mollib.F90:1629-1683).
+template <typename real, int N>
+__device__ inline void bspline(real x, real* vals, real* ders) {
+  real last[N];
+  vals[0] = x;
+  vals[1] = real(1) - x;
+#pragma unroll
+  for (int i = 2; i < N; ++i) vals[i] = real(0);
+#pragma unroll
+  for (int i = 0; i < N; ++i) last[i] = real(0);
+#pragma unroll
+  for (int m = 3; m <= N; ++m) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) last[i] = vals[i];
+    const real inv = real(1) / real(m - 1);
+    vals[0] = x * last[0] * inv;
+#pragma unroll
+    for (int i = 1; i <= m - 2; ++i) {
+      const real a = real(i) + x;
+      vals[i] = (a * last[i] + (real(m) - a) * last[i - 1]) * inv;
+    }
+    vals[m - 1] = (real(1) - x) * last[m - 2] * inv;
+  }
+  if (ders) {
+    ders[0] = last[0];
+#pragma unroll
+    for (int i = 1; i <= N - 2; ++i) ders[i] = last[i] - last[i - 1];
+    ders[N - 1] = -last[N - 2];
+  }
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum over a workgroup of blockDim.x (multiple of 64) threads; result valid in thread 0.
+template <typename T>
+__device__ inline T block_sum(T v, T* sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  T s = T(0);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
+  __syncthreads();
+  return s;
+}
+
+}  // namespace msgd

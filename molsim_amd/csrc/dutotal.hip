@@ -1,0 +1,455 @@
+// K11-K14: Monte Carlo energy differences of single-particle trial moves (DUTotal,
+// Src/denergy.F90:98-279) and the commit of an accepted move (MCUpdate,
+// Src/mc.F90:5203-5235), all in FP64.
+//
+// One workgroup per trial; any number of independent trials against the same state run in
+// one launch.  Per trial:
+//   real space   (UTwoBodyANew/OldCellList, Src/denergy.F90:728-928): the stencil cells of the
+//                trial position's cell and of the particle's cell; hard-core or below-table
+//                pairs at the new position flag an overlap; FP64 table records.
+//   SPME         the reference spreads the moved charge into a copy of the mesh and runs a
+//                full r2c per trial (DUTwoBodyEwaldRecSPM, Src/denergy.F90:1413-1495).  Here
+//                the same energy difference is evaluated exactly and locally:
+//                  dE = sum_m dQ(m) phi(m) + 1/2 sum_{m,m'} dQ(m) g(m - m') dQ(m')
+//                where phi = g * Q is the convolved mesh UTotal leaves behind and g the real
+//                -space influence kernel (inverse transform of meshfac).  dQ has 2 p^3 points,
+//                so a trial costs O(p^6) instead of O(K^3 log K).
+//   std Ewald    dS(k) = q (e^{ik.r_new} - e^{ik.r_old}) on the resident structure factors,
+//                dE = sum kfac (|S + dS|^2 - |S|^2) (DUTwoBodyEwaldRecStd, :1275-1335).
+//   surface      (|M + q dr|^2 - |M|^2) 2 pi / (3 V)  (DUTwoBodyEwaldSurf, :1504-1539).
+// The state (cell list, phi or S, dipole M) must match the current positions: after a
+// commit or new positions it is rebuilt by a UTotal pass before the next trials.
+#include "common.h"
+
+namespace msg {
+
+namespace {
+
+struct DeArgs {
+  int np, ntp;
+  const double* ro;
+  const double* az;
+  const int* ipt;
+  // cell list
+  CellGrid g;
+  Box box;
+  const int* cell_start;
+  const int* cell_ip;
+  const double4* cpos;
+  const int* ctype;
+  const int* slot_of;
+  const int* slot_cell;
+  double rcut2;
+  int cut_inclusive;
+  // table (FP64)
+  const double* tcoef;
+  const int* tbins;
+  GpuTableMeta tm;
+  // Ewald
+  int ewald;
+  int lsurf;
+  double surf_fac;
+  const double* Mdip;     // 3 doubles (d_out + ntp + 4)
+  // SPME
+  MeshGrid m;
+  int order;
+  const void* phi;        // K^3 reals (float or double)
+  int phi_fp64;
+  const double* gk;       // H^3 folded real-space kernel
+  // std
+  int nk;
+  const int* kvec;
+  const double* kfac;
+  const double* sk;
+  double tpb[3];
+  double epsi4pi;
+  // trials
+  const int* tip;
+  const double* tr;
+  double* out;            // per trial: ntp + 4 doubles
+};
+
+__device__ inline double table_u(const DeArgs& a, int tp, double r2) {
+  int b = (int)((r2 - a.tm.zlo[tp]) * a.tm.invw[tp]);
+  b = min(max(b, 0), kTabBins - 1);
+  int idx = a.tbins[tp * kTabBins + b];
+  const int last = a.tm.last[tp];
+  while (idx < last && r2 < a.tcoef[12 * idx]) ++idx;
+  const double* t = a.tcoef + 12 * idx;
+  const double d = r2 - t[0];
+  return t[1] + d * (t[2] + d * (t[3] + d * (t[4] + d * (t[5] + d * t[6]))));
+}
+
+template <int P>
+__device__ inline void spline64(double pos, int K, int& mm, double* th) {
+  const double v = floor(pos);
+  msgd::bspline<double, P>(pos - v, th, (double*)nullptr);
+  int k = (int)v % K;
+  mm = k < 0 ? k + K : k;
+}
+
+__device__ inline int fold(int d, int K) {
+  d %= K;
+  if (d < 0) d += K;
+  return d <= K / 2 ? d : K - d;
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_dutotal(DeArgs a) {
+  constexpr int W = 2 * P - 1, W3 = W * W * W;
+  __shared__ double sh[8];
+  __shared__ double thn[3][P], tho[3][P];
+  __shared__ int mmn[3], mmo[3];
+  __shared__ double w0[W3], w1[W3];
+  __shared__ int s_ovl;
+  const int t = blockIdx.x;
+  const int ip = a.tip[t];
+  const double rn[3] = {a.tr[3 * t], a.tr[3 * t + 1], a.tr[3 * t + 2]};
+  const double ro[3] = {a.ro[3 * ip], a.ro[3 * ip + 1], a.ro[3 * ip + 2]};
+  const double q = a.az[ip];
+  const int ti = a.ipt[ip];
+  if (threadIdx.x == 0) s_ovl = 0;
+  __syncthreads();
+
+  // ---------------- real space: new minus old ----------------
+  double e[kMaxT];
+#pragma unroll
+  for (int k = 0; k < kMaxT; ++k) e[k] = 0.0;
+  int ovl = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    const double* r = pass == 0 ? rn : ro;
+    int cc;
+    if (pass == 0) {
+      int ci[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        int k = (int)floor((r[d] + a.box.half[d]) * a.g.csi[d]);
+        ci[d] = min(max(k, 0), a.g.nc[d] - 1);
+      }
+      cc = ci[0] + a.g.nc[0] * (ci[1] + a.g.nc[1] * ci[2]);
+    } else {
+      cc = a.slot_cell[a.slot_of[ip]];
+    }
+    const int cx = cc % a.g.nc[0], cy = (cc / a.g.nc[0]) % a.g.nc[1], cz = cc / (a.g.nc[0] * a.g.nc[1]);
+    const double sgn = pass == 0 ? 1.0 : -1.0;
+    for (int k = 0; k < a.g.nst; ++k) {
+      int x = cx + a.g.st[3 * k], y = cy + a.g.st[3 * k + 1], z = cz + a.g.st[3 * k + 2];
+      x += x < 0 ? a.g.nc[0] : (x >= a.g.nc[0] ? -a.g.nc[0] : 0);
+      y += y < 0 ? a.g.nc[1] : (y >= a.g.nc[1] ? -a.g.nc[1] : 0);
+      z += z < 0 ? a.g.nc[2] : (z >= a.g.nc[2] ? -a.g.nc[2] : 0);
+      const int nb = x + a.g.nc[0] * (y + a.g.nc[1] * z);
+      const int j1 = a.cell_start[nb + 1];
+      for (int j = a.cell_start[nb] + threadIdx.x; j < j1; j += 256) {
+        if (a.cell_ip[j] == ip) continue;
+        const double4 pj = a.cpos[j];
+        double d[3] = {r[0] - pj.x, r[1] - pj.y, r[2] - pj.z};
+#pragma unroll
+        for (int qd = 0; qd < 3; ++qd)
+          if (fabs(d[qd]) > a.box.half[qd]) d[qd] -= copysign(a.box.L[qd], d[qd]);
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (a.cut_inclusive ? (r2 > a.rcut2) : !(r2 < a.rcut2)) continue;
+        const int tj = a.ctype[j];
+        const int tp = a.tm.tpidx[ti * kMaxT + tj];
+        if (pass == 0 && (r2 < a.tm.r2atat[tp] || r2 < a.tm.r2umin[tp])) {
+          ovl = 1;
+          continue;
+        }
+        const double u = table_u(a, tp, r2);
+#pragma unroll
+        for (int k2 = 0; k2 < kMaxT; ++k2) e[k2] += (k2 == tj) ? sgn * u : 0.0;
+      }
+    }
+  }
+  if (ovl) atomicOr(&s_ovl, 1);
+
+  // ---------------- reciprocal space ----------------
+  double drec = 0.0;
+  if (a.ewald == 2) {
+    const int K = a.m.K;
+    if (threadIdx.x < 3) {
+      const int d = threadIdx.x;
+      spline64<P>(a.m.dkdr[d] * rn[d], K, mmn[d], thn[d]);
+      spline64<P>(a.m.dkdr[d] * ro[d], K, mmo[d], tho[d]);
+    }
+    __syncthreads();
+    // windows of g around offset 0 and around mm_new - mm_old
+    const int H = a.m.H;
+    for (int k = threadIdx.x; k < W3; k += 256) {
+      const int ox = k % W - (P - 1), oy = (k / W) % W - (P - 1), oz = k / (W * W) - (P - 1);
+      w0[k] = a.gk[fold(ox, K) + H * (fold(oy, K) + H * fold(oz, K))];
+      w1[k] = a.gk[fold(mmn[0] - mmo[0] + ox, K) +
+                   H * (fold(mmn[1] - mmo[1] + oy, K) + H * fold(mmn[2] - mmo[2] + oz, K))];
+    }
+    __syncthreads();
+    // term 1: sum dQ phi over the two stencils; term 2: pair sums with g
+    double s1 = 0.0, s2 = 0.0;
+    constexpr int P3 = P * P * P;
+    for (int a_ = threadIdx.x; a_ < P3; a_ += 256) {
+      const int ix = a_ % P, iy = (a_ / P) % P, iz = a_ / (P * P);
+      const double tn = thn[0][ix] * thn[1][iy] * thn[2][iz];
+      const double to = tho[0][ix] * tho[1][iy] * tho[2][iz];
+      int xn = mmn[0] - ix, yn = mmn[1] - iy, zn = mmn[2] - iz;
+      int xo = mmo[0] - ix, yo = mmo[1] - iy, zo = mmo[2] - iz;
+      xn += xn < 0 ? K : 0; yn += yn < 0 ? K : 0; zn += zn < 0 ? K : 0;
+      xo += xo < 0 ? K : 0; yo += yo < 0 ? K : 0; zo += zo < 0 ? K : 0;
+      const size_t in = xn + (size_t)K * (yn + (size_t)K * zn), io = xo + (size_t)K * (yo + (size_t)K * zo);
+      const double pn = a.phi_fp64 ? ((const double*)a.phi)[in] : (double)((const float*)a.phi)[in];
+      const double po = a.phi_fp64 ? ((const double*)a.phi)[io] : (double)((const float*)a.phi)[io];
+      s1 += tn * pn - to * po;
+      // point a_ of each stencil against every point b of both stencils:
+      //   offset (new a - new b) = (old a - old b) = (ib - ia) in stencil units (window w0)
+      //   offset (new a - old b) = mmn - mmo + (ib - ia)                       (window w1)
+      double nn = 0.0, oo = 0.0, no = 0.0;
+      for (int b = 0; b < P3; ++b) {
+        const int jx = b % P, jy = (b / P) % P, jz = b / (P * P);
+        const int k = (jx - ix + P - 1) + W * ((jy - iy + P - 1) + W * (jz - iz + P - 1));
+        const double tbn = thn[0][jx] * thn[1][jy] * thn[2][jz];
+        const double tbo = tho[0][jx] * tho[1][jy] * tho[2][jz];
+        nn += tbn * w0[k];
+        oo += tbo * w0[k];
+        no += tbo * w1[k];
+      }
+      s2 += tn * nn + to * oo - 2.0 * tn * no;
+    }
+    drec = q * s1 + 0.5 * q * q * s2;
+  } else if (a.ewald == 1) {
+    for (int kn = threadIdx.x; kn < a.nk; kn += 256) {
+      const double kx = a.kvec[3 * kn] * a.tpb[0], ky = a.kvec[3 * kn + 1] * a.tpb[1],
+                   kz = a.kvec[3 * kn + 2] * a.tpb[2];
+      double sxn, cxn, syn, cyn, szn, czn, sxo, cxo, syo, cyo, szo, czo;
+      sincos(kx * rn[0], &sxn, &cxn);
+      sincos(ky * rn[1], &syn, &cyn);
+      sincos(kz * rn[2], &szn, &czn);
+      sincos(kx * ro[0], &sxo, &cxo);
+      sincos(ky * ro[1], &syo, &cyo);
+      sincos(kz * ro[2], &szo, &czo);
+      double tn = 0.0, to = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double gxn = (c & 2) ? sxn : -sxn, gyn = (c & 1) ? syn : -syn;
+        const double gxo = (c & 2) ? sxo : -sxo, gyo = (c & 1) ? syo : -syo;
+        const double r1n = cxn * cyn - gxn * gyn, i1n = cxn * gyn + gxn * cyn;
+        const double r1o = cxo * cyo - gxo * gyo, i1o = cxo * gyo + gxo * cyo;
+        const double dr = q * ((r1n * czn - i1n * szn) - (r1o * czo - i1o * szo));
+        const double di = q * ((r1n * szn + i1n * czn) - (r1o * szo + i1o * czo));
+        const double Sr = a.sk[8 * kn + 2 * c], Si = a.sk[8 * kn + 2 * c + 1];
+        tn += (Sr + dr) * (Sr + dr) + (Si + di) * (Si + di);
+        to += Sr * Sr + Si * Si;
+      }
+      drec += a.kfac[kn] * (tn - to);
+    }
+  }
+  drec = msgd::block_sum(drec, sh);
+  double out_tp[kMaxTP];
+  for (int tp = 0; tp < a.ntp; ++tp) {
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < kMaxT; ++k) v += (a.tm.tpidx[ti * kMaxT + k] == tp) ? e[k] : 0.0;
+    out_tp[tp] = msgd::block_sum(v, sh);
+  }
+  if (threadIdx.x == 0) {
+    double* o = a.out + (size_t)t * (a.ntp + 4);
+    double tw = 0.0;
+    for (int tp = 0; tp < a.ntp; ++tp) {
+      o[1 + tp] = out_tp[tp];
+      tw += out_tp[tp];
+    }
+    o[0] = tw;
+    if (a.ewald && a.lsurf) {
+      const double M[3] = {a.Mdip[0], a.Mdip[1], a.Mdip[2]};
+      double m2o = 0.0, m2n = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        const double mn = M[d] + q * (rn[d] - ro[d]);
+        m2o += M[d] * M[d];
+        m2n += mn * mn;
+      }
+      drec += a.surf_fac * (m2n - m2o);
+    }
+    const double du_rec = a.ewald ? a.epsi4pi * drec : 0.0;
+    o[a.ntp + 1] = du_rec;
+    o[a.ntp + 2] = tw + du_rec;
+    o[a.ntp + 3] = (double)s_ovl;
+  }
+}
+
+// g(x) = sum_k meshfac(|k|) cos(2 pi k.x / K) over the full spectrum, folded to [0, K/2]^3
+// by symmetry: three cosine stages, each out[.., x, ..] = sum_k cw[x][k] in[.., k, ..].
+__global__ void k_cos_stage(const double* __restrict__ in, double* __restrict__ out,
+                            const double* __restrict__ cw, int H, int axis) {
+  const int n = H * H * H;
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= n) return;
+  int c[3] = {o % H, (o / H) % H, o / (H * H)};
+  const int x = c[axis];
+  double s = 0.0;
+  for (int k = 0; k < H; ++k) {
+    c[axis] = k;
+    s += cw[x * H + k] * in[c[0] + H * (c[1] + H * c[2])];
+  }
+  out[o] = s;
+}
+
+template <int P>
+void launch_de(Ctx* c, const DeArgs& a, int ntrial) {
+  hipLaunchKernelGGL(k_dutotal<P>, dim3(ntrial), dim3(256), 0, c->stream, a);
+}
+
+int make_gk(Ctx* c) {
+  const int K = c->mesh.K, H = c->mesh.H;
+  const size_t n = (size_t)H * H * H;
+  if (!c->d_gk)
+    if (int e = dalloc(c, (void**)&c->d_gk, n * sizeof(double))) return e;
+  // meshfac in FP64 (the device copy may be FP32)
+  const msg_params& P = c->P;
+  std::vector<double> spl(P.order, 0.0), last;
+  spl[1] = 1.0;
+  for (int m = 3; m <= P.order; ++m) {
+    last = spl;
+    spl[0] = 0.0;
+    for (int i = 1; i <= m - 2; ++i) spl[i] = (i * last[i] + (m - i) * last[i - 1]) / (m - 1);
+    spl[m - 1] = last[m - 2] / (m - 1);
+  }
+  std::vector<double> b2(K);
+  for (int mm = 0; mm < K; ++mm) {
+    double re = 0, im = 0;
+    for (int k = 0; k <= P.order - 2; ++k) {
+      re += spl[k + 1] * std::cos(2.0 * M_PI * mm * k / K);
+      im += spl[k + 1] * std::sin(2.0 * M_PI * mm * k / K);
+    }
+    b2[mm] = ((P.order % 2 == 1) && (2 * mm == K)) ? 0.0 : 1.0 / (re * re + im * im);
+  }
+  const double vol = P.boxlen[0] * P.boxlen[1] * P.boxlen[2];
+  std::vector<double> G(n), cw((size_t)H * H);
+  for (int nz = 0; nz < H; ++nz)
+    for (int ny = 0; ny < H; ++ny)
+      for (int nx = 0; nx < H; ++nx) {
+        const size_t o = nx + (size_t)H * (ny + (size_t)H * nz);
+        if (!nx && !ny && !nz) { G[o] = 0.0; continue; }
+        const double m2 = std::pow(nx * 2.0 * M_PI / P.boxlen[0], 2) +
+                          std::pow(ny * 2.0 * M_PI / P.boxlen[1], 2) +
+                          std::pow(nz * 2.0 * M_PI / P.boxlen[2], 2);
+        G[o] = 4.0 * M_PI / vol * std::exp(-m2 / (4.0 * P.alpha * P.alpha)) / m2 * b2[nx] * b2[ny] * b2[nz];
+      }
+  for (int x = 0; x < H; ++x)
+    for (int k = 0; k < H; ++k)
+      cw[(size_t)x * H + k] = ((k == 0 || k == K / 2) ? 1.0 : 2.0) * std::cos(2.0 * M_PI * k * x / K);
+  double *d_a = nullptr, *d_b = nullptr, *d_cw = nullptr;
+  MSG_HIP(hipMalloc(&d_a, n * sizeof(double)));
+  MSG_HIP(hipMalloc(&d_b, n * sizeof(double)));
+  MSG_HIP(hipMalloc(&d_cw, cw.size() * sizeof(double)));
+  MSG_HIP(hipMemcpy(d_a, G.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  MSG_HIP(hipMemcpy(d_cw, cw.data(), cw.size() * sizeof(double), hipMemcpyHostToDevice));
+  const int nb = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(k_cos_stage, dim3(nb), dim3(256), 0, c->stream, d_a, d_b, d_cw, H, 0);
+  hipLaunchKernelGGL(k_cos_stage, dim3(nb), dim3(256), 0, c->stream, d_b, d_a, d_cw, H, 1);
+  hipLaunchKernelGGL(k_cos_stage, dim3(nb), dim3(256), 0, c->stream, d_a, c->d_gk, d_cw, H, 2);
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  hipFree(d_a);
+  hipFree(d_b);
+  hipFree(d_cw);
+  c->have_gk = true;
+  return MSG_OK;
+}
+
+}  // namespace
+
+int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ovl, double* du_tot,
+                  double* du_rec, double* du_twob) {
+  for (int t = 0; t < ntrial; ++t)
+    if (ip[t] < 1 || ip[t] > c->np) return set_error(MSG_ERR_ARG, "DUTotal: particle index out of range");
+  if (c->P.ewald_mode == MSG_EWALD_SPM && !c->have_gk)
+    if (int e = make_gk(c)) return e;
+  if (!c->fresh)
+    if (int e = utotal_refresh(c)) return e;
+  const int ntp = c->nptpt;
+  if (ntrial > c->de_cap) {
+    if (c->d_de) { hipFree(c->d_de); hipFree(c->d_de_ip); hipFree(c->d_de_r); }
+    c->d_de = nullptr;
+    const int cap = std::max(ntrial, 256);
+    if (int e = dalloc(c, (void**)&c->d_de, (size_t)cap * (kMaxTP + 4) * sizeof(double))) return e;
+    if (int e = dalloc(c, (void**)&c->d_de_ip, cap * sizeof(int))) return e;
+    if (int e = dalloc(c, (void**)&c->d_de_r, (size_t)cap * 3 * sizeof(double))) return e;
+    c->de_cap = cap;
+  }
+  std::vector<int> ip0(ntrial);
+  for (int t = 0; t < ntrial; ++t) ip0[t] = ip[t] - 1;
+  MSG_HIP(hipMemcpyAsync(c->d_de_ip, ip0.data(), ntrial * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  MSG_HIP(hipMemcpyAsync(c->d_de_r, rotm, ntrial * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  DeArgs a;
+  a.np = c->np;
+  a.ntp = ntp;
+  a.ro = c->d_ro;
+  a.az = c->d_az;
+  a.ipt = c->d_ipt;
+  a.g = c->grid;
+  for (int d = 0; d < 3; ++d) {
+    a.box.L[d] = c->P.boxlen[d];
+    a.box.half[d] = 0.5 * c->P.boxlen[d];
+    a.box.inv[d] = 1.0 / c->P.boxlen[d];
+  }
+  a.cell_start = c->d_cell_start;
+  a.cell_ip = c->d_cell_ip;
+  a.cpos = (const double4*)c->d_cpos;
+  a.ctype = c->d_ctype;
+  a.slot_of = c->d_slot_of;
+  a.slot_cell = c->d_slot_cell;
+  a.rcut2 = c->P.rcut * c->P.rcut;
+  a.cut_inclusive = c->P.list_mode == MSG_LIST_ALL;
+  a.tcoef = c->d_tcoef64;
+  a.tbins = c->d_tbins;
+  a.tm = c->tmeta;
+  a.ewald = c->P.ewald_mode;
+  a.lsurf = c->P.lsurf;
+  const double vol = c->P.boxlen[0] * c->P.boxlen[1] * c->P.boxlen[2];
+  a.surf_fac = 2.0 * M_PI / (3.0 * vol);
+  a.Mdip = c->d_out + ntp + 4;
+  a.m = c->mesh;
+  a.order = c->mesh.order;
+  a.phi = c->d_Q;
+  a.phi_fp64 = c->fp64;
+  a.gk = c->d_gk;
+  a.nk = c->nkvec;
+  a.kvec = c->d_kvec;
+  a.kfac = c->d_kfac;
+  a.sk = c->d_sk;
+  for (int d = 0; d < 3; ++d) a.tpb[d] = 2.0 * M_PI / c->P.boxlen[d];
+  a.epsi4pi = c->P.epsi4pi;
+  a.tip = c->d_de_ip;
+  a.tr = c->d_de_r;
+  a.out = c->d_de;
+  prof_begin(c, MSG_K_DE);
+  switch (c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.order : 4) {
+    case 3: launch_de<3>(c, a, ntrial); break;
+    case 4: launch_de<4>(c, a, ntrial); break;
+    case 5: launch_de<5>(c, a, ntrial); break;
+    case 6: launch_de<6>(c, a, ntrial); break;
+    case 7: launch_de<7>(c, a, ntrial); break;
+    case 8: launch_de<8>(c, a, ntrial); break;
+    default: return set_error(MSG_ERR_ARG, "DUTotal: SPME order must be 3..8");
+  }
+  prof_end(c, MSG_K_DE);
+  MSG_HIP(hipGetLastError());
+  std::vector<double> res((size_t)ntrial * (ntp + 4));
+  MSG_HIP(hipMemcpyAsync(res.data(), c->d_de, res.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  for (int t = 0; t < ntrial; ++t) {
+    const double* o = res.data() + (size_t)t * (ntp + 4);
+    ovl[t] = o[ntp + 3] != 0.0;
+    du_tot[t] = o[ntp + 2];
+    if (du_rec) du_rec[t] = o[ntp + 1];
+    if (du_twob)
+      for (int k = 0; k <= ntp; ++k) du_twob[(size_t)t * (ntp + 1) + k] = o[k];
+  }
+  return MSG_OK;
+}
+
+// MCUpdate for one accepted single-particle move: the position changes on the device and
+// the derived state (cells, phi or S, dipole) is rebuilt by the next UTotal pass.
+int commit_move(Ctx* c, int ip, const double* rnew) {
+  MSG_HIP(hipMemcpyAsync(c->d_ro + 3 * ip, rnew, 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  c->fresh = false;
+  return MSG_OK;
+}
+
+}  // namespace msg

@@ -1,0 +1,260 @@
+// K4: real-space pair loop over MOLSIM's tabulated potential (UTwoBodyACellList,
+// Src/energy.F90:669-777; UTwoBodyA, :445-544, for the all-pairs mode).
+//
+// Work items are (cell, chunk of 4 home particles); a wavefront takes an item, spreads the
+// candidates of the 27 stencil cells over its lanes and wave-reduces the home forces.  Every
+// pair is evaluated from both of its particles (no atomics, deterministic forces; energies
+// and the virial are halved in the final reduction).  The packed table (records of 12 coefficients in the
+// reference's scan order + an r^2 bin index per type pair) is staged in LDS.
+// Per pair, as in the reference:
+//   PBCr2 single shift (FP64 difference, Src/molaux.F90:149-184) -> r2 in the kernel's real;
+//   r2 >= rcut2 (cell list) / r2 > rcut2 (all pairs): skipped;
+//   r2 < r2atat: usum = fsum = 1e10 (hard-core emulation);
+//   r2 < r2umin: error flag (the reference stops);
+//   else scan records from the bin's start while r2 < zlow, then Horner in d = r2 - zlow.
+#include "common.h"
+
+namespace msg {
+
+namespace {
+
+struct PairArgs {
+  int np;
+  const double4* cpos;
+  const int* ctype;
+  const int* cell_ip;
+  const int* cell_start;
+  const int* cell_chunk;
+  const int* slot_cell;
+  CellGrid g;
+  Box box;
+  double rcut2;
+  int cut_inclusive;
+  const void* tcoef;
+  const int* tbins;
+  GpuTableMeta tm;
+  double* force;      // 3*np caller order
+  double* part;       // per block: ntp energies + virial
+  int* flags;
+  double scale;       // applied to the force when this kernel is the last writer
+};
+
+// Wave-local LDS hand-off (all lanes of one wavefront, no workgroup barrier).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kWaves = 4;  // wavefronts per workgroup; each works through its own items
+
+// Work item = (cell, chunk of up to kPairHB home particles).  The wavefront spreads the
+// candidates of the home cell's stencil over its 64 lanes; every lane evaluates its
+// candidate against all homes of the chunk; home forces are wave-reduced at the end.
+// Energies (per type pair) and the virial accumulate per lane over all items.
+template <typename real, int NTP>
+__global__ __launch_bounds__(256) void k_pair(PairArgs a) {
+  extern __shared__ double s_dyn[];  // table records, then the bin index (sized at launch)
+  __shared__ int s_nbs[kWaves][kMaxStencil + 1], s_pref[kWaves][kMaxStencil + 1];
+  __shared__ real s_zlo[kMaxTP], s_invw[kMaxTP], s_r2at[kMaxTP], s_r2um[kMaxTP];
+  __shared__ int s_last[kMaxTP], s_tp[kMaxT * kMaxT];
+  __shared__ double s_red[kWaves][kMaxTP + 1];
+  const int nint = a.tm.nint;
+  real* s_coef = (real*)s_dyn;
+  int* s_bins = (int*)(s_coef + nint * 12);
+  {
+    const real* src = (const real*)a.tcoef;
+    for (int k = threadIdx.x; k < nint * 12; k += 256) s_coef[k] = src[k];
+    for (int k = threadIdx.x; k < NTP * kTabBins; k += 256) s_bins[k] = a.tbins[k];
+    if (threadIdx.x < NTP) {
+      const int tp = threadIdx.x;
+      s_zlo[tp] = (real)a.tm.zlo[tp];
+      s_invw[tp] = (real)a.tm.invw[tp];
+      s_r2at[tp] = (real)a.tm.r2atat[tp];
+      s_r2um[tp] = (real)a.tm.r2umin[tp];
+      s_last[tp] = a.tm.last[tp];
+    }
+    if (threadIdx.x < kMaxT * kMaxT) s_tp[threadIdx.x] = a.tm.tpidx[threadIdx.x];
+  }
+  __syncthreads();
+
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nitems = a.cell_chunk[a.g.ncell];
+  const real rc2 = (real)a.rcut2;
+  double et[NTP];
+#pragma unroll
+  for (int k = 0; k < NTP; ++k) et[k] = 0.0;
+  double vir = 0.0;
+  int err = 0;
+  for (int item = blockIdx.x * kWaves + wv; item < nitems; item += gridDim.x * kWaves) {
+    // cell of this item: largest c with cell_chunk[c] <= item
+    int lo = 0, hi = a.g.ncell;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (a.cell_chunk[mid] <= item) lo = mid; else hi = mid;
+    }
+    const int c = lo;
+    const int h0 = a.cell_start[c] + (item - a.cell_chunk[c]) * kPairHB;
+    const int nh = min(kPairHB, a.cell_start[c + 1] - h0);
+    // stencil cells: lane k < nst records start and count, then a wave prefix sum
+    const int cx = c % a.g.nc[0], cy = (c / a.g.nc[0]) % a.g.nc[1], cz = c / (a.g.nc[0] * a.g.nc[1]);
+    int cnt = 0, st0 = 0;
+    if (lane < a.g.nst) {
+      int x = cx + a.g.st[3 * lane], y = cy + a.g.st[3 * lane + 1], z = cz + a.g.st[3 * lane + 2];
+      x += x < 0 ? a.g.nc[0] : (x >= a.g.nc[0] ? -a.g.nc[0] : 0);
+      y += y < 0 ? a.g.nc[1] : (y >= a.g.nc[1] ? -a.g.nc[1] : 0);
+      z += z < 0 ? a.g.nc[2] : (z >= a.g.nc[2] ? -a.g.nc[2] : 0);
+      const int nb = x + a.g.nc[0] * (y + a.g.nc[1] * z);
+      st0 = a.cell_start[nb];
+      cnt = a.cell_start[nb + 1] - st0;
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) {
+      const int t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    const int ncand = __shfl(incl, a.g.nst - 1, 64);
+    if (lane < a.g.nst) {
+      s_nbs[wv][lane] = st0;
+      s_pref[wv][lane] = incl - cnt;
+    }
+    if (lane == 0) s_pref[wv][a.g.nst] = ncand;
+    wave_sync();
+    double4 hp[kPairHB];
+    int ht[kPairHB];
+#pragma unroll
+    for (int h = 0; h < kPairHB; ++h) {
+      const int s = h0 + min(h, nh - 1);
+      hp[h] = a.cpos[s];
+      ht[h] = a.ctype[s];
+    }
+    double f[kPairHB][3];
+#pragma unroll
+    for (int h = 0; h < kPairHB; ++h) f[h][0] = f[h][1] = f[h][2] = 0.0;
+    for (int idx = lane; idx < ncand; idx += 64) {
+      int k = 0;
+      while (s_pref[wv][k + 1] <= idx) ++k;
+      const int j = s_nbs[wv][k] + (idx - s_pref[wv][k]);
+      const double4 pj = a.cpos[j];
+      const int tj = a.ctype[j];
+#pragma unroll
+      for (int h = 0; h < kPairHB; ++h) {
+        if (h >= nh || j == h0 + h) continue;
+        double d[3] = {hp[h].x - pj.x, hp[h].y - pj.y, hp[h].z - pj.z};
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          if (fabs(d[q]) > a.box.half[q]) d[q] -= copysign(a.box.L[q], d[q]);
+        const real dx = (real)d[0], dy = (real)d[1], dz = (real)d[2];
+        const real r2 = dx * dx + dy * dy + dz * dz;
+        if (a.cut_inclusive ? (r2 > rc2) : !(r2 < rc2)) continue;
+        const int tp = s_tp[ht[h] * kMaxT + tj];
+        real us, fs;
+        if (r2 < s_r2at[tp]) {
+          us = real(1e10);
+          fs = real(1e10);
+        } else if (r2 < s_r2um[tp]) {
+          err |= FLAG_R2UMIN;
+          continue;
+        } else {
+          int b = (int)((r2 - s_zlo[tp]) * s_invw[tp]);
+          b = min(max(b, 0), kTabBins - 1);
+          int ti = s_bins[tp * kTabBins + b];
+          const int last = s_last[tp];
+          while (ti < last && r2 < s_coef[12 * ti]) ++ti;
+          const real* t = s_coef + 12 * ti;
+          const real dd = r2 - t[0];
+          us = t[1] + dd * (t[2] + dd * (t[3] + dd * (t[4] + dd * (t[5] + dd * t[6]))));
+          fs = t[7] + dd * (t[8] + dd * (t[9] + dd * (t[10] + dd * t[11])));
+        }
+#pragma unroll
+        for (int k2 = 0; k2 < NTP; ++k2) et[k2] += (k2 == tp) ? (double)us : 0.0;
+        f[h][0] += (double)(fs * dx);
+        f[h][1] += (double)(fs * dy);
+        f[h][2] += (double)(fs * dz);
+        vir -= (double)(fs * r2);
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < kPairHB; ++h) {
+      const double fx = msgd::wave_sum(f[h][0]), fy = msgd::wave_sum(f[h][1]), fz = msgd::wave_sum(f[h][2]);
+      if (lane == 0 && h < nh) {
+        const int ip = a.cell_ip[h0 + h];
+        a.force[3 * ip + 0] = fx * a.scale;
+        a.force[3 * ip + 1] = fy * a.scale;
+        a.force[3 * ip + 2] = fz * a.scale;
+      }
+    }
+    wave_sync();  // s_nbs / s_pref are rewritten by the next item
+  }
+  if (err) atomicOr(a.flags, err);
+  // per-workgroup sums per type pair and the virial, fixed order
+#pragma unroll
+  for (int k = 0; k < NTP; ++k) {
+    const double v = msgd::wave_sum(et[k]);
+    if (lane == 0) s_red[wv][k] = v;
+  }
+  {
+    const double v = msgd::wave_sum(vir);
+    if (lane == 0) s_red[wv][NTP] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x <= NTP) {
+    double s = 0.0;
+    for (int w = 0; w < kWaves; ++w) s += s_red[w][threadIdx.x];
+    a.part[blockIdx.x * (a.tm.ntp + 1) + (threadIdx.x == NTP ? a.tm.ntp : threadIdx.x)] = s;
+  }
+}
+
+}  // namespace
+
+// Work items: sum over cells of ceil(count / kPairHB) <= ncell + np / kPairHB; a grid of
+// up to 8 workgroups per CU walks them with a stride.
+int pair_blocks(Ctx* c) {
+  const long items = (long)c->grid.ncell + c->np / kPairHB + 1;
+  return (int)std::max(1L, std::min((items + kWaves - 1) / kWaves, 2048L));
+}
+
+int launch_pair(Ctx* c, bool last) {
+  PairArgs a;
+  a.np = c->np;
+  a.cpos = (const double4*)c->d_cpos;
+  a.ctype = c->d_ctype;
+  a.cell_ip = c->d_cell_ip;
+  a.cell_start = c->d_cell_start;
+  a.slot_cell = c->d_slot_cell;
+  a.cell_chunk = c->d_cell_chunk;
+  a.g = c->grid;
+  for (int d = 0; d < 3; ++d) {
+    a.box.L[d] = c->P.boxlen[d];
+    a.box.half[d] = 0.5 * c->P.boxlen[d];
+    a.box.inv[d] = 1.0 / c->P.boxlen[d];
+  }
+  a.rcut2 = c->P.rcut * c->P.rcut;
+  a.cut_inclusive = c->P.list_mode == MSG_LIST_ALL;
+  a.tcoef = c->d_tcoef;
+  a.tbins = c->d_tbins;
+  a.tm = c->tmeta;
+  a.force = c->d_force;
+  a.part = c->d_part + c->part_off_pair;
+  a.flags = c->d_flags;
+  a.scale = last ? c->P.force_scale : 1.0;
+  const int nb = c->npart_pair;
+  const size_t lds = (size_t)c->tmeta.nint * 12 * (c->fp64 ? 8 : 4) + 8 +
+                     (size_t)c->tmeta.ntp * kTabBins * sizeof(int);
+  prof_begin(c, MSG_K_PAIR);
+#define MSG_PAIR_LAUNCH(R, NTP)                                                              \
+  hipLaunchKernelGGL((k_pair<R, NTP>), dim3(nb), dim3(256), lds, c->stream, a)
+  switch (c->nptpt) {
+    case 1: if (c->fp64) MSG_PAIR_LAUNCH(double, 1); else MSG_PAIR_LAUNCH(float, 1); break;
+    case 3: if (c->fp64) MSG_PAIR_LAUNCH(double, 3); else MSG_PAIR_LAUNCH(float, 3); break;
+    case 6: if (c->fp64) MSG_PAIR_LAUNCH(double, 6); else MSG_PAIR_LAUNCH(float, 6); break;
+    default: if (c->fp64) MSG_PAIR_LAUNCH(double, 10); else MSG_PAIR_LAUNCH(float, 10); break;
+  }
+#undef MSG_PAIR_LAUNCH
+  prof_end(c, MSG_K_PAIR);
+  MSG_HIP(hipGetLastError());
+  return MSG_OK;
+}
+
+}  // namespace msg

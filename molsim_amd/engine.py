@@ -1,0 +1,287 @@
+"""Host-side mirror of MOLSIM's non-bonded call surface over the HIP C-ABI.
+
+``Engine`` wraps one ``msg_ctx`` (``include/molsim_gpu.h``) and exposes the reference's
+operations under the reference's names:
+
+* ``utotal()``        -- UTotal(iStage)               Src/energy.F90:280-434
+* ``dutotal()``       -- DUTotal(lhsoverlap, ...)     Src/denergy.F90:98-279
+* ``mc_update()``     -- MCUpdate                      Src/mc.F90:5203-5235
+* ``ewald_setup()``   -- EwaldSetup                    Src/energy.F90:6601-7126
+* ``nlist()``         -- NList / SetCellList           Src/nlist.F90:553-669
+* ``build_table()``   -- PotTwoBodyTab1/2/3            Src/potential.F90:868-1258
+
+Errors from the library raise ``MolsimGpuError`` carrying the C status code, in place of
+the reference's ``Stop``.  There is no CPU fallback: without the built HIP library every
+call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmolsim_gpu.so")
+_lib = None
+
+LIST_MODES = {"all": 0, "cell": 1, "verlet": 2}
+EWALD_MODES = {"none": 0, "std": 1, "spm": 2}
+PRECISIONS = {"mixed": 0, "fp64": 1}
+KERNELS = ["bin", "cellsort", "pair", "spread", "fft_x", "fft_y", "fft_zconv", "fft_yi",
+           "fft_xi", "gather", "reduce", "ewald_std", "dutotal", "vlist"]
+
+
+class MolsimGpuError(RuntimeError):
+    def __init__(self, code, text):
+        super().__init__(f"molsim_gpu error {code}: {text}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("np", C.c_int), ("npt", C.c_int), ("boxlen", C.c_double * 3),
+        ("rcut", C.c_double), ("drnlist", C.c_double), ("epsi4pi", C.c_double),
+        ("list_mode", C.c_int), ("ewald_mode", C.c_int), ("lsurf", C.c_int),
+        ("alpha", C.c_double), ("ncut", C.c_int), ("ncutregion", C.c_int),
+        ("nmesh", C.c_int), ("order", C.c_int), ("precision", C.c_int),
+        ("force_scale", C.c_double),
+    ]
+
+
+class Pot(C.Structure):
+    _fields_ = [
+        ("npt", C.c_int), ("form", C.c_int), ("lewald", C.c_int), ("pad", C.c_int),
+        ("alpha", C.c_double), ("rcut", C.c_double), ("epsi4pi", C.c_double),
+        ("utoltab", C.c_double), ("ftoltab", C.c_double), ("umaxtab", C.c_double),
+        ("fmaxtab", C.c_double), ("r2uminin", C.c_double),
+        ("z", C.c_double * 8), ("sig", C.c_double * 8), ("eps", C.c_double * 8),
+        ("rad", C.c_double * 8),
+    ]
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+def lib():
+    """Load libmolsim_gpu.so (built by ``__graft_entry__.build()``); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.msg_last_error.restype = C.c_char_p
+        L.msg_kernel_name.restype = C.c_char_p
+        L.msg_build_table.argtypes = [C.POINTER(Pot), _dp, C.c_int, _ip, _ip, _dp, _dp]
+        L.msg_create.argtypes = [C.POINTER(Params), C.POINTER(_vp)]
+        L.msg_destroy.argtypes = [_vp]
+        L.msg_destroy.restype = None
+        L.msg_set_table.argtypes = [_vp, _dp, C.c_int, _ip, _ip, _dp, _dp]
+        L.msg_set_particles.argtypes = [_vp, _dp, _ip]
+        L.msg_set_positions.argtypes = [_vp, _dp]
+        L.msg_set_positions_device.argtypes = [_vp, _vp]
+        L.msg_ewald_setup.argtypes = [_vp]
+        L.msg_nlist.argtypes = [_vp]
+        L.msg_utotal.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+        L.msg_utotal_async.argtypes = [_vp, _vp]
+        L.msg_fetch.argtypes = [_vp, _vp, _vp, _vp, _vp]
+        L.msg_dutotal.argtypes = [_vp, C.c_int, _ip, _dp, C.POINTER(C.c_int), _vp, _vp, _vp]
+        L.msg_dutotal_batch.argtypes = [_vp, C.c_int, _ip, _dp, _ip, _dp, _vp]
+        L.msg_commit.argtypes = [_vp, C.c_int, _ip, _dp]
+        L.msg_get_cell_list.argtypes = [_vp, _ip, _vp, _vp]
+        L.msg_stream.argtypes = [_vp]
+        L.msg_stream.restype = _vp
+        L.msg_profile.argtypes = [_vp, C.c_int]
+        L.msg_profile_read.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.msg_device_bytes.argtypes = [_vp]
+        L.msg_device_bytes.restype = C.c_longlong
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise MolsimGpuError(rc, lib().msg_last_error().decode())
+    return rc
+
+
+def make_pot(pot: dict) -> Pot:
+    p = Pot()
+    p.npt = pot["npt"]
+    p.form = pot.get("form", 0)
+    p.lewald = int(pot.get("lewald", 1))
+    p.alpha = pot.get("alpha", 0.0)
+    p.rcut = pot["rcut"]
+    p.epsi4pi = pot["epsi4pi"]
+    p.utoltab = pot.get("utoltab", 1e-5)
+    p.ftoltab = pot.get("ftoltab", 1e-5)
+    p.umaxtab = pot.get("umaxtab", 2e4)
+    p.fmaxtab = pot.get("fmaxtab", 2e4)
+    p.r2uminin = pot.get("r2uminin", 0.1)
+    for k in ("z", "sig", "eps", "rad"):
+        arr = getattr(p, k)
+        for i, v in enumerate(pot.get(k, [0.0] * p.npt)):
+            arr[i] = v
+    return p
+
+
+def build_table(pot: dict, maxbuf: int = 200000) -> dict:
+    """PotTwoBodyTab: returns MOLSIM's table state (iubuflow 1-based, as MOLSIM holds it)."""
+    p = make_pot(pot)
+    npt = pot["npt"]
+    nptpt = npt * (npt + 1) // 2
+    ubuf = np.zeros(maxbuf)
+    low = np.zeros(nptpt, dtype=np.int32)
+    ng = np.zeros(nptpt, dtype=np.int32)
+    r2umin = np.zeros(nptpt)
+    r2atat = np.zeros(nptpt)
+    n = _check(lib().msg_build_table(C.byref(p), ubuf, maxbuf, low, ng, r2umin, r2atat))
+    return dict(ubuf=ubuf[:n].copy(), iubuflow=low, nugrid=ng, r2umin=r2umin, r2atat=r2atat)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp)
+
+
+class Engine:
+    """One device-resident system (MOLSIM's global state for the non-bonded path)."""
+
+    def __init__(self, spec, precision: str = "mixed", table: dict | None = None,
+                 force_scale: float = 1.0):
+        self.spec = spec
+        P = Params()
+        P.np = spec.n
+        P.npt = spec.npt
+        for d in range(3):
+            P.boxlen[d] = float(spec.box[d])
+        P.rcut = spec.rcut
+        P.drnlist = spec.drnlist
+        P.epsi4pi = spec.pot["epsi4pi"]
+        P.list_mode = LIST_MODES[spec.list_mode]
+        P.ewald_mode = EWALD_MODES[spec.ewald]
+        P.lsurf = int(spec.lsurf)
+        P.alpha = spec.alpha
+        P.ncut = spec.ncut
+        P.ncutregion = 0
+        P.nmesh = spec.nmesh
+        P.order = spec.order
+        P.precision = PRECISIONS[precision]
+        P.force_scale = force_scale
+        self.params = P
+        self.nptpt = spec.npt * (spec.npt + 1) // 2
+        h = _vp()
+        _check(lib().msg_create(C.byref(P), C.byref(h)))
+        self.h = h
+        self.table = table if table is not None else build_table(spec.pot)
+        t = self.table
+        _check(lib().msg_set_table(h, t["ubuf"], t["ubuf"].size, t["iubuflow"], t["nugrid"],
+                                   t["r2umin"], t["r2atat"]))
+        az = np.ascontiguousarray(spec.az, dtype=np.float64)
+        ipt = np.ascontiguousarray(spec.ipt + 1, dtype=np.int32)
+        _check(lib().msg_set_particles(h, az, ipt))
+        self.set_positions(spec.ro)
+        if spec.ewald != "none":
+            self.ewald_setup()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().msg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ---------------------------------------------------------------------------
+    def set_positions(self, ro):
+        ro = np.ascontiguousarray(ro, dtype=np.float64).reshape(-1)
+        _check(lib().msg_set_positions(self.h, ro))
+
+    def set_positions_device(self, ptr: int):
+        _check(lib().msg_set_positions_device(self.h, _vp(ptr)))
+
+    def ewald_setup(self):
+        _check(lib().msg_ewald_setup(self.h))
+
+    def nlist(self):
+        _check(lib().msg_nlist(self.h))
+
+    # -- UTotal ---------------------------------------------------------------------------
+    def utotal(self, want_force: bool = True) -> dict:
+        ut = np.zeros(self.nptpt + 1)
+        urec, utot, vir = C.c_double(), C.c_double(), C.c_double()
+        f = np.zeros(3 * self.spec.n) if want_force else None
+        _check(lib().msg_utotal(self.h, _ptr(ut), C.cast(C.pointer(urec), _vp),
+                                C.cast(C.pointer(utot), _vp), _ptr(f) if f is not None else None,
+                                C.cast(C.pointer(vir), _vp)))
+        return dict(u_twob=ut, u_rec=urec.value, u_tot=utot.value,
+                    force=None if f is None else f.reshape(-1, 3), virial=vir.value)
+
+    def utotal_async(self, d_force: int | None = None):
+        _check(lib().msg_utotal_async(self.h, _vp(d_force) if d_force else None))
+
+    def fetch(self) -> dict:
+        ut = np.zeros(self.nptpt + 1)
+        urec, utot, vir = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().msg_fetch(self.h, _ptr(ut), C.cast(C.pointer(urec), _vp),
+                               C.cast(C.pointer(utot), _vp), C.cast(C.pointer(vir), _vp)))
+        return dict(u_twob=ut, u_rec=urec.value, u_tot=utot.value, virial=vir.value)
+
+    # -- DUTotal / MCUpdate ---------------------------------------------------------------
+    def dutotal(self, ip: int, rnew) -> dict:
+        """Energy change of moving particle ip (0-based here) to rnew."""
+        ipa = np.array([ip + 1], dtype=np.int32)
+        r = np.ascontiguousarray(rnew, dtype=np.float64).reshape(3)
+        ov = C.c_int()
+        dt = np.zeros(self.nptpt + 1)
+        drec, dtot = C.c_double(), C.c_double()
+        _check(lib().msg_dutotal(self.h, 1, ipa, r, C.byref(ov), _ptr(dt),
+                                 C.cast(C.pointer(drec), _vp), C.cast(C.pointer(dtot), _vp)))
+        return dict(overlap=bool(ov.value), du_twob=dt, du_rec=drec.value, du_tot=dtot.value)
+
+    def dutotal_batch(self, ips, rnew) -> dict:
+        """Independent single-particle trials against the same state (0-based ips)."""
+        ips = np.ascontiguousarray(np.asarray(ips) + 1, dtype=np.int32)
+        r = np.ascontiguousarray(rnew, dtype=np.float64).reshape(-1)
+        n = ips.size
+        ov = np.zeros(n, dtype=np.int32)
+        dt = np.zeros(n)
+        dr = np.zeros(n)
+        _check(lib().msg_dutotal_batch(self.h, n, ips, r, ov, dt, _ptr(dr)))
+        return dict(overlap=ov.astype(bool), du_tot=dt, du_rec=dr)
+
+    def mc_update(self, ip: int, rnew):
+        ipa = np.array([ip + 1], dtype=np.int32)
+        _check(lib().msg_commit(self.h, 1, ipa, np.ascontiguousarray(rnew, dtype=np.float64).reshape(3)))
+
+    def cell_list(self):
+        """(nc, cell_start, cell_ip 0-based) of the last list build."""
+        nc = np.zeros(3, dtype=np.int32)
+        _check(lib().msg_get_cell_list(self.h, nc, None, None))
+        start = np.zeros(int(np.prod(nc)) + 1, dtype=np.int32)
+        ips = np.zeros(self.spec.n, dtype=np.int32)
+        _check(lib().msg_get_cell_list(self.h, nc, _ptr(start), _ptr(ips)))
+        return nc, start, ips - 1
+
+    # -- measurement ----------------------------------------------------------------------
+    def stream(self) -> int:
+        return lib().msg_stream(self.h) or 0
+
+    def profile(self, enable: bool):
+        _check(lib().msg_profile(self.h, int(enable)))
+
+    def profile_read(self) -> dict:
+        out = {}
+        for k, name in enumerate(KERNELS):
+            ms, n = C.c_double(), C.c_int()
+            _check(lib().msg_profile_read(self.h, k, C.byref(ms), C.byref(n)))
+            if n.value:
+                out[name] = (ms.value, n.value)
+        return out
+
+    def device_bytes(self) -> int:
+        return int(lib().msg_device_bytes(self.h))

@@ -263,9 +263,9 @@ int orc_build_table(const OrcPot* P, double* ubuf, int maxbuf, int* iubuflow, in
   for (int it = 0; it < P->npt; ++it) {
     for (int jt = it; jt < P->npt; ++jt) {
       int tp = tp_index(P->npt, it, jt);
-      double rumax = std::sqrt(std::pow(1.1 * P->rcut, 2));
+      double rumax = std::sqrt((1.1 * P->rcut) * (1.1 * P->rcut));
       double rumin = std::sqrt(P->r2uminin);
-      r2atat[tp] = std::pow(P->rad[it] + P->rad[jt], 2);
+      r2atat[tp] = (P->rad[it] + P->rad[jt]) * (P->rad[it] + P->rad[jt]);
       iubuflow[tp] = ibuf;
       double rupp = rumax;
       double dr = 0.5 * (rumax - rumin);

@@ -1,0 +1,45 @@
+"""Shared helpers of the GPU parity tests."""
+import numpy as np
+
+import oracle
+from molsim_amd.engine import Engine
+
+LIST = {"all": 0, "cell": 1}
+EWALD = {"none": 0, "std": 1, "spm": 2}
+
+
+def cabi_table(tab):
+    """Oracle table (0-based record offsets) in the C-ABI's MOLSIM convention (1-based)."""
+    t = dict(tab)
+    t["iubuflow"] = (np.asarray(tab["iubuflow"]) + 1).astype(np.int32)
+    return t
+
+
+def engine(sp, tab, precision="mixed"):
+    return Engine(sp, precision=precision, table=cabi_table(tab))
+
+
+def oracle_system(sp, tab):
+    return oracle.System(sp.ro, sp.ipt, sp.az, sp.box, sp.rcut, sp.pot["epsi4pi"], tab, npt=sp.npt,
+                         list_mode=LIST[sp.list_mode], drnlist=sp.drnlist, ewald=EWALD[sp.ewald],
+                         surf=sp.lsurf, alpha=sp.alpha, ncut=sp.ncut, K=sp.nmesh, order=sp.order)
+
+
+def rel_force_err(f, ref):
+    return float(np.linalg.norm(f - ref) / np.linalg.norm(ref))
+
+
+def compare(gpu, ref, ftol, etol):
+    """Relative errors of forces (2-norm over all particles) and energies against the oracle."""
+    ferr = rel_force_err(gpu["force"], ref["force"])
+    scale = abs(ref["u_tot"]) + abs(ref["u_rec"]) + np.abs(ref["u_twob"]).max()
+    errs = dict(force=ferr,
+                u_tot=abs(gpu["u_tot"] - ref["u_tot"]) / scale,
+                u_rec=abs(gpu["u_rec"] - ref["u_rec"]) / scale,
+                u_twob=float(np.abs(gpu["u_twob"] - ref["u_twob"]).max() / scale),
+                virial=abs(gpu["virial"] - ref["virial"]) / (abs(ref["virial"]) + scale))
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    assert ferr <= ftol, errs
+    for k in ("u_tot", "u_rec", "u_twob", "virial"):
+        assert errs[k] <= etol, errs
+    return errs
