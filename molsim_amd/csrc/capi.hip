@@ -85,50 +85,77 @@ struct ReduceArgs {
   double surf_fac;     // 2 pi / (3 V)
   double epsi4pi;
   double* out;
+  int* flags;
+  int* cell_count;
+  int ncell;
+  int* brick_count;
+  int nbrick;
 };
 
-// Final FP64 reduction of all partial sums in a fixed order (deterministic).
-__global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
-  __shared__ double sh[8];
-  __shared__ double vals[kMaxTP + 8];
-  for (int tp = 0; tp <= a.ntp; ++tp) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < a.npair; b += 256) s += a.pair[b * (a.ntp + 1) + tp];
-    s = msgd::block_sum(s, sh);
-    if (threadIdx.x == 0) vals[tp] = 0.5 * s;  // every pair was evaluated from both sides
-  }
-  double es = 0.0, vs = 0.0;
-  if (a.ewald == 2) {
-    for (int b = threadIdx.x; b < a.nspme; b += 256) {
-      es += a.spme[2 * b];
-      vs += a.spme[2 * b + 1];
+// Final FP64 reduction of all partial sums in a fixed order (deterministic).  It is the last
+// kernel of a UTotal pass, so it also hands the error flags to the output block and clears
+// the cell/brick counters and flags for the next pass.
+__global__ __launch_bounds__(1024) void k_reduce(ReduceArgs a) {
+  constexpr int NV = kMaxTP + 1 + 2 + 3;
+  __shared__ double red[16][NV];
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0.0;
+  const int np1 = a.ntp + 1;
+  const int npv = a.npair * np1;  // the per-workgroup rows are contiguous: sum the flat array
+  for (int f0 = threadIdx.x; f0 < npv; f0 += 4 * 1024) {
+    double x[4];
+    int kk[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int f = f0 + u * 1024;
+      x[u] = f < npv ? a.pair[f] : 0.0;
+      kk[u] = f % np1;
     }
-  } else if (a.ewald == 1) {
-    for (int b = threadIdx.x; b < a.newstd; b += 256) {
-      es += a.ewstd[2 * b];
-      vs += a.ewstd[2 * b + 1];
-    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k <= kMaxTP; ++k) v[k] += (k == kk[u]) ? x[u] : 0.0;
   }
-  es = msgd::block_sum(es, sh);
-  vs = msgd::block_sum(vs, sh);
-  double m[3];
-  for (int d = 0; d < 3; ++d) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < a.ndip; b += 256) s += a.dip[3 * b + d];
-    m[d] = msgd::block_sum(s, sh);
+  const double* ew = a.ewald == 2 ? a.spme : a.ewstd;
+  const int nev = a.ewald == 2 ? a.nspme : (a.ewald == 1 ? a.newstd : 0);
+  for (int b = threadIdx.x; b < nev; b += 1024) {
+    v[kMaxTP + 1] += ew[2 * b];
+    v[kMaxTP + 2] += ew[2 * b + 1];
   }
+  for (int b = threadIdx.x; b < a.ndip; b += 1024)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[kMaxTP + 3 + d] += a.dip[3 * b + d];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double t = msgd::wave_sum(v[k]);
+    if (lane == 0) red[w][k] = t;
+  }
+  __syncthreads();
+  __shared__ double tot[NV];
+  if (threadIdx.x < NV) {  // one thread per value, 16 wave partials each
+    double x = 0.0;
+#pragma unroll
+    for (int w2 = 0; w2 < 16; ++w2) x += red[w2][threadIdx.x];
+    tot[threadIdx.x] = x;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
+    const double* t = tot;
     double* o = a.out;
     double tw = 0.0;
     for (int tp = 0; tp < a.ntp; ++tp) {
-      o[1 + tp] = vals[tp];
-      tw += vals[tp];
+      o[1 + tp] = 0.5 * t[tp];  // every pair was evaluated from both of its particles
+      tw += 0.5 * t[tp];
     }
     o[0] = tw;
+    const double es = t[kMaxTP + 1], vs = t[kMaxTP + 2];
+    const double m[3] = {t[kMaxTP + 3], t[kMaxTP + 4], t[kMaxTP + 5]};
     double urec = 0.0, virrec = 0.0;
     if (a.ewald) {
       // SPME: E = sum w G |F|^2, virrec -= sum w G (1 + 2 m^2 expf) |F|^2
-      // std:  E = sum kfac |S|^2, virrec -= sum (1 - k^2/(2 alpha^2)) term (partials hold +sum)
+      // std:  E = sum kfac |S|^2, virrec -= sum (1 - k^2/(2 alpha^2)) kfac |S|^2
       urec = es + a.self_raw;
       virrec = -vs;
       if (a.lsurf) {
@@ -139,11 +166,15 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a) {
     }
     o[a.ntp + 1] = a.epsi4pi * urec;
     o[a.ntp + 2] = tw + a.epsi4pi * urec;
-    o[a.ntp + 3] = vals[a.ntp] + a.epsi4pi * virrec;
+    o[a.ntp + 3] = 0.5 * t[a.ntp] + a.epsi4pi * virrec;
     o[a.ntp + 4] = m[0];
     o[a.ntp + 5] = m[1];
     o[a.ntp + 6] = m[2];
+    o[a.ntp + 7] = (double)a.flags[0];
+    a.flags[0] = 0;
   }
+  for (int i = threadIdx.x; i < a.ncell; i += 1024) a.cell_count[i] = 0;
+  for (int i = threadIdx.x; i < a.nbrick; i += 1024) a.brick_count[i] = 0;
 }
 
 }  // namespace
@@ -176,24 +207,48 @@ int launch_reduce(Ctx* c) {
   a.surf_fac = 2.0 * M_PI / (3.0 * vol);
   a.epsi4pi = c->P.epsi4pi;
   a.out = c->d_out;
+  a.flags = c->d_flags;
+  a.cell_count = c->d_cell_count;
+  a.ncell = c->grid.ncell;
+  a.brick_count = c->d_brick_count;
+  a.nbrick = c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.nbrick : 0;
   prof_begin(c, MSG_K_REDUCE);
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, c->stream, a);
   prof_end(c, MSG_K_REDUCE);
   MSG_HIP(hipGetLastError());
   return MSG_OK;
 }
 
-// The UTotal launch sequence (no host synchronisation inside; capturable).
+// The UTotal launch sequence (no host synchronisation inside; capturable).  Counters and
+// flags are zero on entry (cleared at allocation and by the previous pass's reduction).
+// With SPME the pair kernel runs on a second stream beside the mesh kernels; the gather,
+// which adds to the pair forces, waits for it.  With per-kernel timing on, everything runs
+// on one stream so each kernel is timed alone.
 static int utotal_sequence(Ctx* c) {
   int e;
-  if ((e = launch_zero(c))) return e;
   if ((e = launch_bin(c))) return e;
   if ((e = launch_sort(c))) return e;
-  if ((e = launch_pair(c, c->P.ewald_mode == MSG_EWALD_NONE))) return e;
-  if (c->P.ewald_mode == MSG_EWALD_SPM) {
-    if ((e = launch_spme(c))) return e;
-  } else if (c->P.ewald_mode == MSG_EWALD_STD) {
-    if ((e = launch_ewstd(c))) return e;
+  if (c->P.ewald_mode == MSG_EWALD_SPM && !c->prof) {
+    MSG_HIP(hipEventRecord(c->ev_fork, c->stream));
+    MSG_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    hipStream_t main = c->stream;
+    c->stream = c->side;
+    e = launch_pair(c, false);
+    c->stream = main;
+    if (e) return e;
+    MSG_HIP(hipEventRecord(c->ev_join, c->side));
+    if ((e = launch_spme_mesh(c))) return e;
+    MSG_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    if ((e = launch_spme_gather(c))) return e;
+  } else if (c->P.ewald_mode == MSG_EWALD_SPM) {
+    // per-kernel timing: one stream, so no kernel's time includes another's
+    if ((e = launch_pair(c, false))) return e;
+    if ((e = launch_spme_mesh(c))) return e;
+    if ((e = launch_spme_gather(c))) return e;
+  } else {
+    if ((e = launch_pair(c, c->P.ewald_mode == MSG_EWALD_NONE))) return e;
+    if (c->P.ewald_mode == MSG_EWALD_STD)
+      if ((e = launch_ewstd(c))) return e;
   }
   return launch_reduce(c);
 }
@@ -202,9 +257,9 @@ static int flags_to_error(int f);
 
 int utotal_refresh(Ctx* c) {
   if (int e = utotal_sequence(c)) return e;
-  MSG_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  MSG_HIP(hipMemcpyAsync(c->h_out, c->d_out, 64 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   MSG_HIP(hipStreamSynchronize(c->stream));
-  if (int e = flags_to_error(c->h_flags[0])) return e;
+  if (int e = flags_to_error((int)c->h_out[c->nptpt + 7])) return e;
   c->fresh = true;
   return MSG_OK;
 }
@@ -267,8 +322,11 @@ int msg_create(const msg_params* p, msg_ctx** out) {
     msg_destroy(h);
     return code;
   };
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
-    return fail(set_error(MSG_ERR_HIP, "hipStreamCreate failed"));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+    return fail(set_error(MSG_ERR_HIP, "stream/event creation failed"));
   const int np = c->np;
   if ((e = dalloc(c, (void**)&c->d_ro, 3 * np * sizeof(double)))) return fail(e);
   if ((e = dalloc(c, (void**)&c->d_az, np * sizeof(double)))) return fail(e);
@@ -292,6 +350,8 @@ int msg_create(const msg_params* p, msg_ctx** out) {
   if (hipHostMalloc((void**)&c->h_flags, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess)
     return fail(set_error(MSG_ERR_NOMEM, "hipHostMalloc failed"));
   for (auto& ev : c->ev) hipEventCreate(&ev);
+  if ((e = launch_zero(c))) return fail(e);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_error(MSG_ERR_HIP, "sync failed"));
   *out = h;
   return MSG_OK;
 }
@@ -308,13 +368,16 @@ void msg_destroy(msg_ctx* h) {
                   c->d_brick_start, c->d_brick_fill, c->d_brick_tmp, c->d_brick_ip, c->d_mm,
                   c->d_frac, c->d_Q, c->d_C, c->d_G, c->d_tw, c->d_kfac, c->d_kvec, c->d_sk,
                   c->d_force, c->d_part, c->d_out, c->d_flags, c->d_de, c->d_de_ip, c->d_de_r,
-                  c->d_tcoef64, c->d_gk};
+                  c->d_tcoef64, c->d_gk, c->d_item_cell, c->d_bq, c->d_bmm, c->d_M};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (c->h_out) hipHostFree(c->h_out);
   if (c->h_flags) hipHostFree(c->h_flags);
   for (auto& ev : c->ev)
     if (ev) hipEventDestroy(ev);
+  if (c->side) hipStreamDestroy(c->side);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
   if (c->stream) hipStreamDestroy(c->stream);
   delete h;
 }
@@ -391,9 +454,11 @@ int msg_nlist(msg_ctx* h) {
   Ctx* c = &h->c;
   if (!c->have_positions) return set_error(MSG_ERR_STATE, "positions have not been set");
   int e;
-  if ((e = launch_zero(c)) || (e = launch_bin(c)) || (e = launch_sort(c))) return e;
+  if ((e = launch_bin(c)) || (e = launch_sort(c))) return e;
   MSG_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if ((e = launch_zero(c))) return e;  // counters and flags back to zero for the next pass
   MSG_HIP(hipStreamSynchronize(c->stream));
+  c->fresh = false;
   return flags_to_error(c->h_flags[0]);
 }
 
@@ -430,10 +495,9 @@ int msg_fetch(msg_ctx* h, double* u_twob, double* u_rec, double* u_tot, double* 
   if (!h) return set_error(MSG_ERR_ARG, "msg_fetch: null context");
   Ctx* c = &h->c;
   MSG_HIP(hipMemcpyAsync(c->h_out, c->d_out, 64 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  MSG_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   MSG_HIP(hipStreamSynchronize(c->stream));
   prof_drain(c);
-  if (int e = flags_to_error(c->h_flags[0])) return e;
+  if (int e = flags_to_error((int)c->h_out[c->nptpt + 7])) return e;
   const int ntp = c->nptpt;
   if (u_twob)
     for (int k = 0; k <= ntp; ++k) u_twob[k] = c->h_out[k];

@@ -84,6 +84,8 @@ struct Ctx {
   int np = 0, npt = 0, nptpt = 0;
   bool fp64 = false;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;        // second branch of the UTotal graph (pair kernel)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int device = 0;
   long long bytes = 0;
 
@@ -110,6 +112,8 @@ struct Ctx {
   int* d_cell_start = nullptr;   // ncell+1
   int* d_cell_fill = nullptr;    // ncell
   int* d_cell_chunk = nullptr;   // ncell+1: offsets of the pair kernel's home chunks
+  int* d_item_cell = nullptr;    // pair-kernel work item -> cell
+  double* d_M = nullptr;         // total dipole sum q r of the current pass (3)
   int* d_cell_tmp = nullptr;     // np (unsorted members)
   int* d_cell_ip = nullptr;      // np (sorted: cell-major, descending ip in a cell)
   int* d_slot_of = nullptr;      // np: slot of particle ip in the cell order
@@ -123,6 +127,8 @@ struct Ctx {
   int* d_brick_count = nullptr, *d_brick_start = nullptr, *d_brick_fill = nullptr;
   int* d_brick_tmp = nullptr, *d_brick_ip = nullptr;
   int* d_mm = nullptr;           // np, packed meshmax (caller order)
+  void* d_bq = nullptr;          // np: (frac x, y, z, q) in brick order (real4)
+  int* d_bmm = nullptr;          // np: packed meshmax in brick order
   void* d_frac = nullptr;        // np * 3 reals, caller order
   void* d_Q = nullptr;           // K^3 reals (charge mesh, then phi)
   void* d_C = nullptr;           // H*K*K complex
@@ -185,7 +191,8 @@ int launch_sort(Ctx* c);                           // scan + scatter + finalize
 int launch_pair(Ctx* c, bool last);                // K4
 int pair_blocks(Ctx* c);                           // workgroups of the pair kernel
 int spme_setup(Ctx* c);                            // K-space arrays
-int launch_spme(Ctx* c);                           // spread, FFTs + conv, gather
+int launch_spme_mesh(Ctx* c);                      // spread, FFTs + influence function
+int launch_spme_gather(Ctx* c);                    // force gather (after the pair kernel)
 int ewstd_setup(Ctx* c);
 int launch_ewstd(Ctx* c);
 int launch_reduce(Ctx* c);                         // final energies

@@ -74,21 +74,14 @@ struct EwForceArgs {
   EwArgs e;
   double epsi4pi;
   double surf2fac;
-  const double* dip_part;
-  int ndip;
+  const double* Mtot;
   double* force;
   double scale;
 };
 
 __global__ __launch_bounds__(256) void k_ewstd_force(EwForceArgs f) {
-  __shared__ double M[3];
   const EwArgs& a = f.e;
-  if (threadIdx.x < 3) {
-    double s = 0.0;
-    for (int k = 0; k < f.ndip; ++k) s += f.dip_part[3 * k + threadIdx.x];
-    M[threadIdx.x] = s;
-  }
-  __syncthreads();
+  const double M[3] = {f.Mtot[0], f.Mtot[1], f.Mtot[2]};
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= a.np) return;
   const double q = a.az[i];
@@ -186,8 +179,7 @@ int launch_ewstd(Ctx* c) {
   f.epsi4pi = c->P.epsi4pi;
   const double vol = c->P.boxlen[0] * c->P.boxlen[1] * c->P.boxlen[2];
   f.surf2fac = c->P.lsurf ? 2.0 * (2.0 * M_PI / (3.0 * vol)) : 0.0;
-  f.dip_part = c->d_part + c->part_off_dip;
-  f.ndip = c->npart_dip;
+  f.Mtot = c->d_M;
   f.force = c->d_force;
   f.scale = c->P.force_scale;
   prof_begin(c, MSG_K_EWSTD);

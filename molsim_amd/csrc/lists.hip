@@ -11,6 +11,8 @@
 //   k_finalize  one wavefront per cell ranks its members by index (descending, so the cell
 //               order equals the reference's traversal order bit for bit) and writes the
 //               cell-ordered position/charge/type arrays; bricks are ranked ascending.
+#include <type_traits>
+
 #include "common.h"
 
 namespace msg {
@@ -71,32 +73,45 @@ __global__ __launch_bounds__(256) void k_bin(int np, const double* __restrict__ 
 }
 
 // Exclusive scan over n values of one 1024-thread workgroup; out[n] receives the total.
-// div > 1 scans ceil(cnt / div) instead of cnt.
+// div > 1 scans ceil(cnt / div) instead of cnt.  Each thread owns a contiguous run of
+// values, loaded together, so the scan costs two memory round trips, not one per 1024.
 __device__ void block_scan(int n, const int* __restrict__ cnt, int div, int* out, int* wsum) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int running = 0;
-  for (int base = 0; base < n; base += 1024) {
-    const int idx = base + threadIdx.x;
-    int v = idx < n ? cnt[idx] : 0;
-    if (div > 1) v = (v + div - 1) / div;
-    int incl = v;
+  const int per = (n + 1023) / 1024;
+  const int b0 = threadIdx.x * per, b1 = min(n, b0 + per);
+  int sum = 0;
+  for (int i0 = b0; i0 < b1; i0 += 8) {  // eight independent loads in flight
+    int v[8];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int t = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += t;
-    }
-    if (lane == 63) wsum[w] = incl;
-    __syncthreads();
-    int before = 0, total = 0;
-    for (int k = 0; k < 16; ++k) {
-      if (k < w) before += wsum[k];
-      total += wsum[k];
-    }
-    if (idx < n) out[idx] = running + before + incl - v;
-    running += total;
-    __syncthreads();
+    for (int u = 0; u < 8; ++u) v[u] = i0 + u < b1 ? cnt[i0 + u] : 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) sum += div > 1 ? (v[u] + div - 1) / div : v[u];
   }
-  if (threadIdx.x == 0) out[n] = running;
+  int incl = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  int before = 0, total = 0;
+  for (int k = 0; k < 16; ++k) {
+    if (k < w) before += wsum[k];
+    total += wsum[k];
+  }
+  int run = before + incl - sum;
+  for (int i0 = b0; i0 < b1; i0 += 8) {
+    int v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = i0 + u < b1 ? cnt[i0 + u] : 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (i0 + u < b1) out[i0 + u] = run;
+      run += div > 1 ? (v[u] + div - 1) / div : v[u];
+    }
+  }
+  if (threadIdx.x == 0) out[n] = total;
   __syncthreads();
 }
 
@@ -105,8 +120,29 @@ __device__ void block_scan(int n, const int* __restrict__ cnt, int div, int* out
 __global__ __launch_bounds__(1024) void k_scan(int n0, const int* __restrict__ cnt0, int* start0,
                                               int* fill0, int hb, int* chunk0, int n1,
                                               const int* __restrict__ cnt1, int* start1,
-                                              int* fill1) {
+                                              int* fill1, const double* __restrict__ dip_part,
+                                              int ndip, double* __restrict__ M) {
   __shared__ int wsum[16];
+  __shared__ double dsum[16][3];
+  if (blockIdx.x == 2) {  // total dipole sum q r (surface term), from k_bin's partials
+    double m[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < ndip; b += 1024)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) m[d] += dip_part[3 * b + d];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double t = msgd::wave_sum(m[d]);
+      if ((threadIdx.x & 63) == 0) dsum[threadIdx.x >> 6][d] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < 16; ++w) t += dsum[w][threadIdx.x];
+      M[threadIdx.x] = t;
+    }
+    return;
+  }
   const int n = blockIdx.x == 0 ? n0 : n1;
   const int* cnt = blockIdx.x == 0 ? cnt0 : cnt1;
   int* fill = blockIdx.x == 0 ? fill0 : fill1;
@@ -133,37 +169,79 @@ __global__ __launch_bounds__(256) void k_scatter(int np, const int* __restrict__
   }
 }
 
+struct FinalizeArgs {
+  int ncell, nbrick;
+  const int* cell_start;
+  const int* cell_tmp;
+  const int* cell_chunk;
+  const double* ro;
+  const double* az;
+  const int* ipt;
+  int* cell_ip;
+  int* slot_of;
+  double4* cpos;
+  int* ctype;
+  int* slot_cell;
+  int* item_cell;
+  const int* brick_start;
+  const int* brick_tmp;
+  int* brick_ip;
+  const int* mm;
+  const void* frac;
+  void* bq;       // brick order: (frac x, frac y, frac z, q) reals
+  int* bmm;       // brick order: packed meshmax
+};
+
+// One wavefront per cell (then per brick): rank members by particle index and write the
+// ordered arrays; cells also list their pair-kernel work items.
 template <typename real>
-__global__ __launch_bounds__(256) void k_finalize(
-    int ncell, const int* __restrict__ cell_start, const int* __restrict__ cell_tmp,
-    const double* __restrict__ ro, const double* __restrict__ az, const int* __restrict__ ipt,
-    int* __restrict__ cell_ip, int* __restrict__ slot_of, double4* __restrict__ cpos,
-    int* __restrict__ ctype, int* __restrict__ slot_cell, int nbrick,
-    const int* __restrict__ brick_start, const int* __restrict__ brick_tmp,
-    int* __restrict__ brick_ip) {
+__global__ __launch_bounds__(256) void k_finalize(FinalizeArgs a) {
   const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (wv < ncell) {
-    const int s0 = cell_start[wv], n = cell_start[wv + 1] - s0;
-    for (int k = lane; k < n; k += 64) {
-      const int ip = cell_tmp[s0 + k];
+  if (wv < a.ncell) {
+    const int s0 = a.cell_start[wv], n = a.cell_start[wv + 1] - s0;
+    for (int k = lane; k < ((n + 63) & ~63); k += 64) {
+      const int ip = k < n ? a.cell_tmp[s0 + k] : -1;
       int rank = 0;
-      for (int j = 0; j < n; ++j) rank += cell_tmp[s0 + j] > ip;
+      if (n <= 64) {  // members in registers: rank by broadcasting each one
+        for (int j = 0; j < n; ++j) rank += __shfl(ip, j, 64) > ip;
+      } else {
+        for (int j = 0; j < n; ++j) rank += a.cell_tmp[s0 + j] > ip;
+      }
+      if (k >= n) continue;
       const int dst = s0 + rank;
-      cell_ip[dst] = ip;
-      slot_of[ip] = dst;
-      cpos[dst] = make_double4(ro[3 * ip], ro[3 * ip + 1], ro[3 * ip + 2], az[ip]);
-      ctype[dst] = ipt[ip];
-      slot_cell[dst] = wv;
+      a.cell_ip[dst] = ip;
+      a.slot_of[ip] = dst;
+      a.cpos[dst] = make_double4(a.ro[3 * ip], a.ro[3 * ip + 1], a.ro[3 * ip + 2], a.az[ip]);
+      a.ctype[dst] = a.ipt[ip];
+      a.slot_cell[dst] = wv;
     }
-  } else if (wv - ncell < nbrick) {
-    const int b = wv - ncell;
-    const int s0 = brick_start[b], n = brick_start[b + 1] - s0;
-    for (int k = lane; k < n; k += 64) {
-      const int ip = brick_tmp[s0 + k];
+    const int i0 = a.cell_chunk[wv], ni = a.cell_chunk[wv + 1] - i0;
+    for (int k = lane; k < ni; k += 64) a.item_cell[i0 + k] = wv;
+  } else if (wv - a.ncell < a.nbrick) {
+    const int b = wv - a.ncell;
+    const int s0 = a.brick_start[b], n = a.brick_start[b + 1] - s0;
+    const real* fr = (const real*)a.frac;
+    typedef typename std::conditional<sizeof(real) == 4, float4, double4>::type R4;
+    R4* bq = (R4*)a.bq;
+    for (int k = lane; k < ((n + 63) & ~63); k += 64) {
+      const int ip = k < n ? a.brick_tmp[s0 + k] : 0x7fffffff;
       int rank = 0;
-      for (int j = 0; j < n; ++j) rank += brick_tmp[s0 + j] < ip;
-      brick_ip[s0 + rank] = ip;
+      if (n <= 64) {
+        for (int j = 0; j < n; ++j) rank += __shfl(ip, j, 64) < ip;
+      } else {
+        for (int j = 0; j < n; ++j) rank += a.brick_tmp[s0 + j] < ip;
+      }
+      if (k >= n) continue;
+      const int dst = s0 + rank;
+      a.brick_ip[dst] = ip;
+      R4 v;
+      v.x = fr[3 * ip];
+      v.y = fr[3 * ip + 1];
+      v.z = fr[3 * ip + 2];
+      v.w = (real)a.az[ip];
+      bq[dst] = v;
+      a.bmm[dst] = a.mm[ip];
     }
   }
 }
@@ -218,6 +296,8 @@ int lists_setup(Ctx* c) {
   if (int e = A((void**)&c->d_cell_start, (g.ncell + 1) * sizeof(int))) return e;
   if (int e = A((void**)&c->d_cell_fill, g.ncell * sizeof(int))) return e;
   if (int e = A((void**)&c->d_cell_chunk, (g.ncell + 1) * sizeof(int))) return e;
+  if (int e = A((void**)&c->d_M, 4 * sizeof(double))) return e;
+  if (int e = A((void**)&c->d_item_cell, (g.ncell + np / kPairHB + 2) * sizeof(int))) return e;
   if (int e = A((void**)&c->d_cell_tmp, np * sizeof(int))) return e;
   if (int e = A((void**)&c->d_cell_ip, np * sizeof(int))) return e;
   if (int e = A((void**)&c->d_slot_of, np * sizeof(int))) return e;
@@ -261,17 +341,40 @@ int launch_sort(Ctx* c) {
   const bool mesh = c->P.ewald_mode == MSG_EWALD_SPM;
   const int nbrick = mesh ? c->mesh.nbrick : 0;
   prof_begin(c, MSG_K_CELLSORT);
-  hipLaunchKernelGGL(k_scan, dim3(2), dim3(1024), 0, c->stream, c->grid.ncell, c->d_cell_count,
+  hipLaunchKernelGGL(k_scan, dim3(3), dim3(1024), 0, c->stream, c->grid.ncell, c->d_cell_count,
                      c->d_cell_start, c->d_cell_fill, kPairHB, c->d_cell_chunk, nbrick,
-                     c->d_brick_count, c->d_brick_start, c->d_brick_fill);
+                     c->d_brick_count, c->d_brick_start, c->d_brick_fill,
+                     c->d_part + c->part_off_dip, c->npart_dip, c->d_M);
   hipLaunchKernelGGL(k_scatter, dim3((np + 255) / 256), dim3(256), 0, c->stream, np, c->d_cellid,
                      c->d_cell_start, c->d_cell_fill, c->d_cell_tmp, (int)mesh, c->d_mm, c->mesh,
                      c->d_brick_start, c->d_brick_fill, c->d_brick_tmp);
   const int waves = c->grid.ncell + nbrick;
-  hipLaunchKernelGGL(k_finalize<float>, dim3((waves + 3) / 4), dim3(256), 0, c->stream,
-                     c->grid.ncell, c->d_cell_start, c->d_cell_tmp, c->d_ro, c->d_az, c->d_ipt,
-                     c->d_cell_ip, c->d_slot_of, (double4*)c->d_cpos, c->d_ctype, c->d_slot_cell,
-                     nbrick, c->d_brick_start, c->d_brick_tmp, c->d_brick_ip);
+  FinalizeArgs f;
+  f.ncell = c->grid.ncell;
+  f.nbrick = nbrick;
+  f.cell_start = c->d_cell_start;
+  f.cell_tmp = c->d_cell_tmp;
+  f.cell_chunk = c->d_cell_chunk;
+  f.ro = c->d_ro;
+  f.az = c->d_az;
+  f.ipt = c->d_ipt;
+  f.cell_ip = c->d_cell_ip;
+  f.slot_of = c->d_slot_of;
+  f.cpos = (double4*)c->d_cpos;
+  f.ctype = c->d_ctype;
+  f.slot_cell = c->d_slot_cell;
+  f.item_cell = c->d_item_cell;
+  f.brick_start = c->d_brick_start;
+  f.brick_tmp = c->d_brick_tmp;
+  f.brick_ip = c->d_brick_ip;
+  f.mm = c->d_mm;
+  f.frac = c->d_frac;
+  f.bq = c->d_bq;
+  f.bmm = c->d_bmm;
+  if (c->fp64)
+    hipLaunchKernelGGL(k_finalize<double>, dim3((waves + 3) / 4), dim3(256), 0, c->stream, f);
+  else
+    hipLaunchKernelGGL(k_finalize<float>, dim3((waves + 3) / 4), dim3(256), 0, c->stream, f);
   prof_end(c, MSG_K_CELLSORT);
   MSG_HIP(hipGetLastError());
   return MSG_OK;

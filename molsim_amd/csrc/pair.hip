@@ -25,6 +25,7 @@ struct PairArgs {
   const int* cell_ip;
   const int* cell_start;
   const int* cell_chunk;
+  const int* item_cell;
   const int* slot_cell;
   CellGrid g;
   Box box;
@@ -87,13 +88,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
   double vir = 0.0;
   int err = 0;
   for (int item = blockIdx.x * kWaves + wv; item < nitems; item += gridDim.x * kWaves) {
-    // cell of this item: largest c with cell_chunk[c] <= item
-    int lo = 0, hi = a.g.ncell;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (a.cell_chunk[mid] <= item) lo = mid; else hi = mid;
-    }
-    const int c = lo;
+    const int c = a.item_cell[item];
     const int h0 = a.cell_start[c] + (item - a.cell_chunk[c]) * kPairHB;
     const int nh = min(kPairHB, a.cell_start[c + 1] - h0);
     // stencil cells: lane k < nst records start and count, then a wave prefix sum
@@ -206,11 +201,197 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
   }
 }
 
+// Pair evaluation shared by the kernels: table lookup and accumulation for home h (type ht)
+// and candidate pj (type tj).
+template <typename real, int NTP>
+struct PairAcc {
+  double et[NTP];
+  double vir, fx, fy, fz;
+  int err;
+  __device__ void zero() {
+#pragma unroll
+    for (int q = 0; q < NTP; ++q) et[q] = 0.0;
+    vir = fx = fy = fz = 0.0;
+    err = 0;
+  }
+};
+
+template <typename real, int NTP>
+__device__ inline void pair_eval(const double4& hp, int ht, const double4& pj, int tj, const Box& box,
+                                 real rc2, bool inclusive, const real* s_coef, const int* s_bins,
+                                 const real* s_zlo, const real* s_invw, const real* s_r2at,
+                                 const real* s_r2um, const int* s_last, const int* s_tp,
+                                 PairAcc<real, NTP>& acc) {
+  double d[3] = {hp.x - pj.x, hp.y - pj.y, hp.z - pj.z};
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    if (fabs(d[q]) > box.half[q]) d[q] -= copysign(box.L[q], d[q]);
+  const real dx = (real)d[0], dy = (real)d[1], dz = (real)d[2];
+  const real r2 = dx * dx + dy * dy + dz * dz;
+  if (inclusive ? (r2 > rc2) : !(r2 < rc2)) return;
+  const int tp = s_tp[ht * kMaxT + tj];
+  real us, fs;
+  if (r2 < s_r2at[tp]) {
+    us = real(1e10);
+    fs = real(1e10);
+  } else if (r2 < s_r2um[tp]) {
+    acc.err |= FLAG_R2UMIN;
+    return;
+  } else {
+    int b = (int)((r2 - s_zlo[tp]) * s_invw[tp]);
+    b = min(max(b, 0), kTabBins - 1);
+    int ti = s_bins[tp * kTabBins + b];
+    const int last = s_last[tp];
+    while (ti < last && r2 < s_coef[12 * ti]) ++ti;
+    const real* t = s_coef + 12 * ti;
+    const real dd = r2 - t[0];
+    us = t[1] + dd * (t[2] + dd * (t[3] + dd * (t[4] + dd * (t[5] + dd * t[6]))));
+    fs = t[7] + dd * (t[8] + dd * (t[9] + dd * (t[10] + dd * t[11])));
+  }
+#pragma unroll
+  for (int q = 0; q < NTP; ++q) acc.et[q] += (q == tp) ? (double)us : 0.0;
+  acc.fx += (double)(fs * dx);
+  acc.fy += (double)(fs * dy);
+  acc.fz += (double)(fs * dz);
+  acc.vir -= (double)(fs * r2);
+}
+
+// Neighbour-parallel variant for the full 27-cell stencil: 32 lanes per home particle.  The
+// candidates of the 27 stencil cells are split into 32 contiguous, equal shares (so dense
+// cells do not serialise one lane); each lane walks its share two candidates at a time with
+// both loads issued first.  The group reduces the home force with shuffles.
+template <typename real, int NTP>
+__global__ __launch_bounds__(256) void k_pair_nb(PairArgs a) {
+  extern __shared__ double s_dyn[];
+  __shared__ real s_zlo[kMaxTP], s_invw[kMaxTP], s_r2at[kMaxTP], s_r2um[kMaxTP];
+  __shared__ int s_last[kMaxTP], s_tp[kMaxT * kMaxT];
+  __shared__ double s_red[4][kMaxTP + 1];
+  __shared__ int s_cst[8][kMaxStencil + 1], s_cpf[8][kMaxStencil + 1];
+  const int nint = a.tm.nint;
+  real* s_coef = (real*)s_dyn;
+  int* s_bins = (int*)(s_coef + nint * 12);
+  {
+    const real* src = (const real*)a.tcoef;
+    for (int k = threadIdx.x; k < nint * 12; k += 256) s_coef[k] = src[k];
+    for (int k = threadIdx.x; k < NTP * kTabBins; k += 256) s_bins[k] = a.tbins[k];
+    if (threadIdx.x < NTP) {
+      const int tp = threadIdx.x;
+      s_zlo[tp] = (real)a.tm.zlo[tp];
+      s_invw[tp] = (real)a.tm.invw[tp];
+      s_r2at[tp] = (real)a.tm.r2atat[tp];
+      s_r2um[tp] = (real)a.tm.r2umin[tp];
+      s_last[tp] = a.tm.last[tp];
+    }
+    if (threadIdx.x < kMaxT * kMaxT) s_tp[threadIdx.x] = a.tm.tpidx[threadIdx.x];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, grp = threadIdx.x >> 5;
+  const int s = blockIdx.x * 8 + grp;
+  const int k = threadIdx.x & 31;
+  const bool home = s < a.np;
+  const int hs = home ? s : a.np - 1;
+  const double4 hp = a.cpos[hs];
+  const int ht = a.ctype[hs];
+  const int c = a.slot_cell[hs];
+  int cnt = 0, st0 = 0;
+  if (k < a.g.nst) {
+    int x = c % a.g.nc[0] + a.g.st[3 * k], y = (c / a.g.nc[0]) % a.g.nc[1] + a.g.st[3 * k + 1],
+        z = c / (a.g.nc[0] * a.g.nc[1]) + a.g.st[3 * k + 2];
+    x += x < 0 ? a.g.nc[0] : (x >= a.g.nc[0] ? -a.g.nc[0] : 0);
+    y += y < 0 ? a.g.nc[1] : (y >= a.g.nc[1] ? -a.g.nc[1] : 0);
+    z += z < 0 ? a.g.nc[2] : (z >= a.g.nc[2] ? -a.g.nc[2] : 0);
+    const int nb = x + a.g.nc[0] * (y + a.g.nc[1] * z);
+    st0 = a.cell_start[nb];
+    cnt = a.cell_start[nb + 1] - st0;
+  }
+  int incl = cnt;
+#pragma unroll
+  for (int off = 1; off < 32; off <<= 1) {
+    const int t = __shfl_up(incl, off, 32);
+    if (k >= off) incl += t;
+  }
+  const int ncand = __shfl(incl, 31, 32);
+  if (k < a.g.nst) {
+    s_cst[grp][k] = st0;
+    s_cpf[grp][k] = incl - cnt;
+  }
+  if (k == 0) s_cpf[grp][a.g.nst] = ncand;
+  __syncthreads();  // table in LDS and the per-group cell ranges
+
+  const real rc2 = (real)a.rcut2;
+  PairAcc<real, NTP> acc;
+  acc.zero();
+  const int share = (ncand + 31) >> 5;
+  int i = k * share;
+  const int i1 = home ? min(ncand, i + share) : 0;
+  if (i < i1) {
+    int cc = 0;
+    while (s_cpf[grp][cc + 1] <= i) ++cc;
+    int j = s_cst[grp][cc] + (i - s_cpf[grp][cc]);
+    int cend = s_cst[grp][cc] + (s_cpf[grp][cc + 1] - s_cpf[grp][cc]);
+    while (i < i1) {
+      const int jA = j;
+      ++i; ++j;
+      if (j == cend && i < i1) {
+        do { ++cc; } while (s_cpf[grp][cc + 1] <= i);
+        j = s_cst[grp][cc];
+        cend = j + (s_cpf[grp][cc + 1] - s_cpf[grp][cc]);
+      }
+      const bool haveB = i < i1;
+      const int jB = haveB ? j : jA;
+      if (haveB) {
+        ++i; ++j;
+        if (j == cend && i < i1) {
+          do { ++cc; } while (s_cpf[grp][cc + 1] <= i);
+          j = s_cst[grp][cc];
+          cend = j + (s_cpf[grp][cc + 1] - s_cpf[grp][cc]);
+        }
+      }
+      const double4 pA = a.cpos[jA], pB = a.cpos[jB];
+      const int tA = a.ctype[jA], tB = a.ctype[jB];
+      if (jA != s)
+        pair_eval<real, NTP>(hp, ht, pA, tA, a.box, rc2, false, s_coef, s_bins, s_zlo, s_invw,
+                             s_r2at, s_r2um, s_last, s_tp, acc);
+      if (haveB && jB != s)
+        pair_eval<real, NTP>(hp, ht, pB, tB, a.box, rc2, false, s_coef, s_bins, s_zlo, s_invw,
+                             s_r2at, s_r2um, s_last, s_tp, acc);
+    }
+  }
+#pragma unroll
+  for (int off = 16; off > 0; off >>= 1) {
+    acc.fx += __shfl_xor(acc.fx, off, 64);
+    acc.fy += __shfl_xor(acc.fy, off, 64);
+    acc.fz += __shfl_xor(acc.fz, off, 64);
+  }
+  if (home && k == 0) {
+    const int ip = a.cell_ip[s];
+    a.force[3 * ip + 0] = acc.fx * a.scale;
+    a.force[3 * ip + 1] = acc.fy * a.scale;
+    a.force[3 * ip + 2] = acc.fz * a.scale;
+  }
+  if (acc.err) atomicOr(a.flags, acc.err);
+#pragma unroll
+  for (int q = 0; q < NTP; ++q) {
+    const double v = msgd::wave_sum(acc.et[q]);
+    if (lane == 0) s_red[wv][q] = v;
+  }
+  {
+    const double v = msgd::wave_sum(acc.vir);
+    if (lane == 0) s_red[wv][NTP] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x <= NTP) {
+    double t = 0.0;
+    for (int w = 0; w < 4; ++w) t += s_red[w][threadIdx.x];
+    a.part[blockIdx.x * (a.tm.ntp + 1) + (threadIdx.x == NTP ? a.tm.ntp : threadIdx.x)] = t;
+  }
+}
+
 }  // namespace
 
 // Work items: sum over cells of ceil(count / kPairHB) <= ncell + np / kPairHB; a grid of
 // up to 8 workgroups per CU walks them with a stride.
 int pair_blocks(Ctx* c) {
+  if (c->grid.nst == 27 && c->P.list_mode == MSG_LIST_CELL) return (c->np + 7) / 8;
   const long items = (long)c->grid.ncell + c->np / kPairHB + 1;
   return (int)std::max(1L, std::min((items + kWaves - 1) / kWaves, 2048L));
 }
@@ -224,6 +405,7 @@ int launch_pair(Ctx* c, bool last) {
   a.cell_start = c->d_cell_start;
   a.slot_cell = c->d_slot_cell;
   a.cell_chunk = c->d_cell_chunk;
+  a.item_cell = c->d_item_cell;
   a.g = c->grid;
   for (int d = 0; d < 3; ++d) {
     a.box.L[d] = c->P.boxlen[d];
@@ -243,8 +425,14 @@ int launch_pair(Ctx* c, bool last) {
   const size_t lds = (size_t)c->tmeta.nint * 12 * (c->fp64 ? 8 : 4) + 8 +
                      (size_t)c->tmeta.ntp * kTabBins * sizeof(int);
   prof_begin(c, MSG_K_PAIR);
+  const bool nbpar = c->grid.nst == 27 && c->P.list_mode == MSG_LIST_CELL;
 #define MSG_PAIR_LAUNCH(R, NTP)                                                              \
-  hipLaunchKernelGGL((k_pair<R, NTP>), dim3(nb), dim3(256), lds, c->stream, a)
+  do {                                                                                       \
+    if (nbpar)                                                                               \
+      hipLaunchKernelGGL((k_pair_nb<R, NTP>), dim3(nb), dim3(256), lds, c->stream, a);      \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_pair<R, NTP>), dim3(nb), dim3(256), lds, c->stream, a);         \
+  } while (0)
   switch (c->nptpt) {
     case 1: if (c->fp64) MSG_PAIR_LAUNCH(double, 1); else MSG_PAIR_LAUNCH(float, 1); break;
     case 3: if (c->fp64) MSG_PAIR_LAUNCH(double, 3); else MSG_PAIR_LAUNCH(float, 3); break;

@@ -3,11 +3,9 @@
 //
 //   k_spread    one workgroup per mesh brick (16x8x8 points).  Particles were binned by the
 //               brick of their B-spline base point (meshmax); a brick's contributors are the
-//               particles of its own and the next brick in each dimension.  Each wavefront
-//               accumulates whole particles (one lane per (iz,iy) stencil row) into its own
-//               LDS copy of the brick, sequentially, so there are no atomics at all and the
-//               result is deterministic; the copies are summed in fixed order and every mesh
-//               point is written exactly once.
+//               particles of its own and the next brick in each dimension.  Contributions are
+//               accumulated in LDS as 64-bit fixed-point integers (exact, order-independent,
+//               hence deterministic; no global atomics) and every mesh point is written once.
 //   k_fft_x     r2c along x: two real rows packed as one complex line, Stockham radix-4/2
 //               in LDS, split into the two half spectra.
 //   k_fft_y     complex FFT along y on tiles of T consecutive kx (forward or inverse).
@@ -111,11 +109,13 @@ __global__ __launch_bounds__(256) void k_fft_x(const real* __restrict__ Q,
   typedef typename C2T<real>::t C2;
   constexpr int E = FftCfg<real>::E, T = E / K, TPL = 256 / T, H = K / 2 + 1;
   __shared__ C2 A[E], B[E];
+  __shared__ C2 s_tw[K];  // twiddles staged in LDS
+  for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
   const int line = threadIdx.x / TPL, tl = threadIdx.x % TPL;
   const size_t r0 = 2 * ((size_t)blockIdx.x * T + line), r1 = r0 + 1;
   for (int k = tl; k < K; k += TPL) A[line * K + k] = cmk<C2>(Q[r0 * K + k], Q[r1 * K + k]);
   __syncthreads();
-  const C2* Z = lds_fft<C2, K, T>(A, B, tw, false) + line * K;
+  const C2* Z = lds_fft<C2, K, T>(A, B, s_tw, false) + line * K;
   const real h = real(0.5);
   for (int k = tl; k < H; k += TPL) {
     const C2 z = Z[k], zc = cconj(Z[(K - k) & (K - 1)]);
@@ -133,6 +133,8 @@ __global__ __launch_bounds__(256) void k_fft_xi(const typename C2T<real>::t* __r
   typedef typename C2T<real>::t C2;
   constexpr int E = FftCfg<real>::E, T = E / K, TPL = 256 / T, H = K / 2 + 1;
   __shared__ C2 A[E], B[E];
+  __shared__ C2 s_tw[K];  // twiddles staged in LDS
+  for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
   const int line = threadIdx.x / TPL, tl = threadIdx.x % TPL;
   const size_t r0 = 2 * ((size_t)blockIdx.x * T + line), r1 = r0 + 1;
   for (int k = tl; k < H; k += TPL) {
@@ -142,7 +144,7 @@ __global__ __launch_bounds__(256) void k_fft_xi(const typename C2T<real>::t* __r
     if (k > 0 && k < K / 2) A[line * K + K - k] = cmk<C2>(a.x + b.y, b.x - a.y);  // conj(a) + i conj(b)
   }
   __syncthreads();
-  const C2* Z = lds_fft<C2, K, T>(A, B, tw, true) + line * K;
+  const C2* Z = lds_fft<C2, K, T>(A, B, s_tw, true) + line * K;
   for (int k = tl; k < K; k += TPL) {
     const C2 z = Z[k];
     Q[r0 * K + k] = z.x;
@@ -158,13 +160,15 @@ __global__ __launch_bounds__(256) void k_fft_y(typename C2T<real>::t* __restrict
   typedef typename C2T<real>::t C2;
   constexpr int E = FftCfg<real>::E, T = E / K, H = K / 2 + 1, NTX = (H + T - 1) / T;
   __shared__ C2 A[E], B[E];
+  __shared__ C2 s_tw[K];  // twiddles staged in LDS
+  for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
   const int tx = blockIdx.x % NTX, z = blockIdx.x / NTX, kx0 = tx * T;
   for (int e = threadIdx.x; e < E; e += 256) {
     const int kl = e % T, y = e / T, kx = kx0 + kl;
     A[kl * K + y] = kx < H ? Cm[kx + (size_t)H * (y + (size_t)K * z)] : cmk<C2>(0, 0);
   }
   __syncthreads();
-  const C2* R = lds_fft<C2, K, T>(A, B, tw, inverse != 0);
+  const C2* R = lds_fft<C2, K, T>(A, B, s_tw, inverse != 0);
   for (int e = threadIdx.x; e < E; e += 256) {
     const int kl = e % T, y = e / T, kx = kx0 + kl;
     if (kx < H) Cm[kx + (size_t)H * (y + (size_t)K * z)] = R[kl * K + y];
@@ -185,6 +189,8 @@ __global__ __launch_bounds__(256) void k_fft_zconv(typename C2T<real>::t* __rest
   typedef typename C2T<real>::t C2;
   constexpr int E = FftCfg<real>::E, T = E / K, H = K / 2 + 1, NTX = (H + T - 1) / T;
   __shared__ C2 A[E], B[E];
+  __shared__ C2 s_tw[K];  // twiddles staged in LDS
+  for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
   __shared__ double red[8];
   const int tx = blockIdx.x % NTX, y = blockIdx.x / NTX, kx0 = tx * T;
   const size_t plane = (size_t)H * K;
@@ -193,7 +199,7 @@ __global__ __launch_bounds__(256) void k_fft_zconv(typename C2T<real>::t* __rest
     A[kl * K + z] = kx < H ? Cm[kx + (size_t)H * y + plane * z] : cmk<C2>(0, 0);
   }
   __syncthreads();
-  C2* R = lds_fft<C2, K, T>(A, B, tw, false);
+  C2* R = lds_fft<C2, K, T>(A, B, s_tw, false);
   const int ky = y <= K / 2 ? y : K - y;
   double esum = 0.0, vsum = 0.0;
   for (int e = threadIdx.x; e < E; e += 256) {
@@ -214,7 +220,7 @@ __global__ __launch_bounds__(256) void k_fft_zconv(typename C2T<real>::t* __rest
   }
   __syncthreads();
   C2* other = (R == A) ? B : A;
-  const C2* S = lds_fft<C2, K, T>(R, other, tw, true);
+  const C2* S = lds_fft<C2, K, T>(R, other, s_tw, true);
   for (int e = threadIdx.x; e < E; e += 256) {
     const int kl = e % T, z = e / T, kx = kx0 + kl;
     if (kx < H) Cm[kx + (size_t)H * y + plane * z] = S[kl * K + z];
@@ -230,19 +236,31 @@ __global__ __launch_bounds__(256) void k_fft_zconv(typename C2T<real>::t* __rest
 struct SpreadArgs {
   MeshGrid m;
   const int* brick_start;
-  const int* brick_ip;
-  const int* mm;
-  const void* frac;
-  const double* az;
+  const void* bq;     // brick order (frac x, y, z, q)
+  const int* bmm;     // brick order packed meshmax
   void* Q;
 };
 
-// One workgroup per brick; W wavefronts each own an LDS copy of the brick.
+template <typename real> struct R4T;
+template <> struct R4T<float> { typedef float4 t; };
+template <> struct R4T<double> { typedef double4 t; };
+
+// Fixed-point scale of the brick accumulator: contributions are added as 64-bit integers,
+// so the sum is exact and independent of the order the threads add in (deterministic
+// without ordering the threads); 2^48 leaves |Q| < 32768 per mesh point with a resolution
+// of 3.6e-15, below FP64 rounding of the mesh values.
+constexpr double kQScale = 281474976710656.0;  // 2^48
+
+// One workgroup per brick.  Phase 1: the B-spline weights of up to CAP contributing
+// particles go to LDS.  Phase 2: all threads take (particle, iz, iy) stencil rows and add
+// the clipped x-row into the brick with 64-bit integer LDS atomics.  Phase 3: every mesh
+// point of the brick is written once.
 template <typename real, int P>
 __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
-  constexpr int W = 4, NBP = kBX * kBY * kBZ, CAP = sizeof(real) == 4 ? 256 : 128;
-  constexpr int NACC = sizeof(real) == 4 ? W : 2;  // FP64: two copies (waves pair up, see below)
-  __shared__ real acc[NACC][NBP];
+  typedef typename R4T<real>::t R4;
+  constexpr int CAP = sizeof(real) == 4 ? 256 : 128;
+  constexpr int NBP = kBX * kBY * kBZ;
+  __shared__ unsigned long long acc[NBP];
   __shared__ real th[CAP][3 * P];
   __shared__ int loc[CAP];
   __shared__ int r_beg[8], r_pref[9];
@@ -250,85 +268,82 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   const int b = blockIdx.x;
   const int bx = b % a.m.nb[0], by = (b / a.m.nb[0]) % a.m.nb[1], bz = b / (a.m.nb[0] * a.m.nb[1]);
   const int lo[3] = {bx * kBX, by * kBY, bz * kBZ};
-  if (threadIdx.x == 0) {
+  __shared__ int r_len[8];
+  if (threadIdx.x < 8) {  // the (up to) 8 source bricks: own and next in each dimension
     const int ox = a.m.nb[0] > 1 ? 2 : 1, oy = a.m.nb[1] > 1 ? 2 : 1, oz = a.m.nb[2] > 1 ? 2 : 1;
-    int n = 0, pref = 0;
-    for (int iz = 0; iz < oz; ++iz)
-      for (int iy = 0; iy < oy; ++iy)
-        for (int ix = 0; ix < ox; ++ix) {
-          const int mb = (bx + ix) % a.m.nb[0] +
-                         a.m.nb[0] * ((by + iy) % a.m.nb[1] + a.m.nb[1] * ((bz + iz) % a.m.nb[2]));
-          r_beg[n] = a.brick_start[mb];
-          r_pref[n] = pref;
-          pref += a.brick_start[mb + 1] - a.brick_start[mb];
-          ++n;
-        }
-    for (; n < 9; ++n) r_pref[n] = pref;
+    const int t = threadIdx.x;
+    int beg = 0, len = 0;
+    if (t < ox * oy * oz) {
+      const int ix = t % ox, iy = (t / ox) % oy, iz = t / (ox * oy);
+      const int mb = (bx + ix) % a.m.nb[0] +
+                     a.m.nb[0] * ((by + iy) % a.m.nb[1] + a.m.nb[1] * ((bz + iz) % a.m.nb[2]));
+      beg = a.brick_start[mb];
+      len = a.brick_start[mb + 1] - beg;
+    }
+    r_beg[t] = beg;
+    r_len[t] = len;
   }
-  for (int k = threadIdx.x; k < NACC * NBP; k += 256) (&acc[0][0])[k] = real(0);
+  for (int k = threadIdx.x; k < NBP; k += 256) acc[k] = 0ull;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int pref = 0;
+    for (int t = 0; t < 8; ++t) {
+      r_pref[t] = pref;
+      pref += r_len[t];
+    }
+    r_pref[8] = pref;
+  }
   __syncthreads();
   const int total = r_pref[8];
-  const real* frac = (const real*)a.frac;
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // FP64: waves 0/1 own copies 0/1 and waves 2/3 take turns with them (see phase 2)
+  const R4* bq = (const R4*)a.bq;
   for (int base = 0; base < total; base += CAP) {
     const int cnt = min(CAP, total - base);
     if ((int)threadIdx.x < cnt) {
       const int t = base + threadIdx.x;
       int r = 0;
       while (r < 7 && t >= r_pref[r + 1]) ++r;
-      const int ip = a.brick_ip[r_beg[r] + (t - r_pref[r])];
-      const int packed = a.mm[ip];
-      const int mmv[3] = {packed & 1023, (packed >> 10) & 1023, packed >> 20};
+      const int idx = r_beg[r] + (t - r_pref[r]);
+      const R4 v = bq[idx];
+      const int packed = a.bmm[idx];
       real* T = th[threadIdx.x];
+      msgd::bspline<real, P>(v.x, T, (real*)nullptr);
+      msgd::bspline<real, P>(v.y, T + P, (real*)nullptr);
+      msgd::bspline<real, P>(v.z, T + 2 * P, (real*)nullptr);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) msgd::bspline<real, P>(frac[3 * ip + d], T + d * P, (real*)nullptr);
-      const real q = (real)a.az[ip];
+      for (int i = 0; i < P; ++i) T[2 * P + i] *= v.w;
+      int l[3] = {(packed & 1023) - lo[0], ((packed >> 10) & 1023) - lo[1], (packed >> 20) - lo[2]};
 #pragma unroll
-      for (int i = 0; i < P; ++i) T[2 * P + i] *= q;
-      int l[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        int v = mmv[d] - lo[d];
-        l[d] = v < 0 ? v + K : v;
-      }
+      for (int d = 0; d < 3; ++d) l[d] += l[d] < 0 ? K : 0;
       loc[threadIdx.x] = l[0] | (l[1] << 10) | (l[2] << 20);
     }
     __syncthreads();
-    // phase 2: each wave takes every W-th particle in order; lane = (iz, iy) row
-    const int iz = lane / P, iy = lane % P;
-    for (int pass = 0; pass < (NACC == W ? 1 : 2); ++pass) {
-      const bool mine = NACC == W ? true : ((wv >> 1) == pass);
-      if (mine && lane < P * P) {
-        real* A = acc[NACC == W ? wv : (wv & 1)];
-        for (int p = wv; p < cnt; p += W) {
-          const int lp = loc[p];
-          int ly = ((lp >> 10) & 1023) - iy, lz = (lp >> 20) - iz;
-          ly += ly < 0 ? K : 0;
-          lz += lz < 0 ? K : 0;
-          if (ly >= kBY || lz >= kBZ) continue;
-          const real wyz = th[p][P + iy] * th[p][2 * P + iz];
-          const int lx0 = lp & 1023;
-          real* row = A + kBX * (ly + kBY * lz);
+    for (int task = threadIdx.x; task < cnt * P * P; task += 256) {
+      const int p = task / (P * P), row = task % (P * P), iz = row / P, iy = row % P;
+      const int lp = loc[p];
+      int ly = ((lp >> 10) & 1023) - iy, lz = (lp >> 20) - iz;
+      ly += ly < 0 ? K : 0;
+      lz += lz < 0 ? K : 0;
+      if (ly >= kBY || lz >= kBZ) continue;
+      const real wyz = th[p][P + iy] * th[p][2 * P + iz];
+      const int lx0 = lp & 1023;
+      unsigned long long* rowp = acc + kBX * (ly + kBY * lz);
 #pragma unroll
-          for (int ix = 0; ix < P; ++ix) {
-            int lx = lx0 - ix;
-            lx += lx < 0 ? K : 0;
-            if (lx < kBX) row[lx] += th[p][ix] * wyz;
-          }
+      for (int ix = 0; ix < P; ++ix) {
+        int lx = lx0 - ix;
+        lx += lx < 0 ? K : 0;
+        if (lx < kBX) {
+          const long long v = __double2ll_rn((double)(th[p][ix] * wyz) * kQScale);
+          atomicAdd(rowp + lx, (unsigned long long)v);
         }
       }
-      if (NACC != W) __syncthreads();
     }
     __syncthreads();
   }
   real* Q = (real*)a.Q;
   for (int k = threadIdx.x; k < NBP; k += 256) {
-    real s = acc[0][k];
-#pragma unroll
-    for (int w = 1; w < NACC; ++w) s += acc[w][k];
     const int lx = k % kBX, ly = (k / kBX) % kBY, lz = k / (kBX * kBY);
-    Q[(lo[0] + lx) + (size_t)K * ((lo[1] + ly) + (size_t)K * (lo[2] + lz))] = s;
+    const double v = (double)(long long)acc[k] * (1.0 / kQScale);
+    Q[(lo[0] + lx) + (size_t)K * ((lo[1] + ly) + (size_t)K * (lo[2] + lz))] = (real)v;
   }
 }
 
@@ -336,72 +351,85 @@ struct GatherArgs {
   MeshGrid m;
   int np;
   const int* brick_ip;
-  const int* mm;
-  const void* frac;
-  const double* az;
+  const void* bq;
+  const int* bmm;
   const void* phi;
   double epsi4pi;
   double surf2fac;          // 2 * (2 pi / (3 V)) when lsurf, else 0
-  const double* dip_part;   // per-block partial dipoles (sum q r)
-  int ndip;
+  const double* Mtot;       // total dipole sum q r
   double* force;            // 3*np caller order (pair force already there)
   double scale;
 };
 
+// Eight lanes per particle (brick order); lane iz < P sums one z-plane of the stencil.
 template <typename real, int P>
 __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
-  __shared__ double M[3];
-  if (threadIdx.x < 3) {
-    double s = 0.0;
-    for (int k = 0; k < a.ndip; ++k) s += a.dip_part[3 * k + threadIdx.x];
-    M[threadIdx.x] = s;
-  }
-  __syncthreads();
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= a.np) return;
+  typedef typename R4T<real>::t R4;
+  const double M[3] = {a.Mtot[0], a.Mtot[1], a.Mtot[2]};
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int i = gid >> 3, iz = gid & 7;
+  const bool valid = i < a.np;
   const int K = a.m.K;
-  const int ip = a.brick_ip[i];
-  const int packed = a.mm[ip];
-  const int mmv[3] = {packed & 1023, (packed >> 10) & 1023, packed >> 20};
-  const real* frac = (const real*)a.frac;
-  real th[3][P], dth[3][P];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) msgd::bspline<real, P>(frac[3 * ip + d], th[d], dth[d]);
-  const real* phi = (const real*)a.phi;
-  int xs[P];
-#pragma unroll
-  for (int ix = 0; ix < P; ++ix) {
-    int x = mmv[0] - ix;
-    xs[ix] = x < 0 ? x + K : x;
-  }
   real e0 = 0, e1 = 0, e2 = 0;
+  R4 v;
+  v.x = v.y = v.z = v.w = real(0);
+  if (valid) {
+    v = ((const R4*)a.bq)[i];
+    const int packed = a.bmm[i];
+    if (iz < P) {
+      real tx[P], dtx[P], ty[P], dty[P], tz[P], dtz[P];
+      msgd::bspline<real, P>(v.x, tx, dtx);
+      msgd::bspline<real, P>(v.y, ty, dty);
+      msgd::bspline<real, P>(v.z, tz, dtz);
+      int xs[P];
+      const int mx = packed & 1023, my = (packed >> 10) & 1023, mz = packed >> 20;
 #pragma unroll
-  for (int iz = 0; iz < P; ++iz) {
-    int z = mmv[2] - iz;
-    z += z < 0 ? K : 0;
-#pragma unroll
-    for (int iy = 0; iy < P; ++iy) {
-      int y = mmv[1] - iy;
-      y += y < 0 ? K : 0;
-      const real* row = phi + (size_t)K * (y + (size_t)K * z);
-      real sx = 0, dsx = 0;
-#pragma unroll
-      for (int ix = 0; ix < P; ++ix) {
-        const real v = row[xs[ix]];
-        sx += th[0][ix] * v;
-        dsx += dth[0][ix] * v;
+      for (int k = 0; k < P; ++k) {
+        const int x = mx - k;
+        xs[k] = x < 0 ? x + K : x;
       }
-      e0 += dsx * th[1][iy] * th[2][iz];
-      e1 += sx * dth[1][iy] * th[2][iz];
-      e2 += sx * th[1][iy] * dth[2][iz];
+      int z = mz - iz;
+      z += z < 0 ? K : 0;
+      real thz = real(0), dthz = real(0);
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        thz = (k == iz) ? tz[k] : thz;
+        dthz = (k == iz) ? dtz[k] : dthz;
+      }
+      const real* phi = (const real*)a.phi;
+#pragma unroll
+      for (int jy = 0; jy < P; ++jy) {
+        int y = my - jy;
+        y += y < 0 ? K : 0;
+        const real* row = phi + (size_t)K * (y + (size_t)K * z);
+        real sx = 0, dsx = 0;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+          const real w = row[xs[k]];
+          sx += tx[k] * w;
+          dsx += dtx[k] * w;
+        }
+        e0 += dsx * ty[jy] * thz;
+        e1 += sx * dty[jy] * thz;
+        e2 += sx * ty[jy] * dthz;
+      }
     }
   }
-  const double q = a.az[ip];
-  const double es[3] = {(double)e0, (double)e1, (double)e2};
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double f = -q * es[d] * a.m.dkdr[d] * a.epsi4pi - a.epsi4pi * a.surf2fac * q * M[d];
-    a.force[3 * ip + d] = (a.force[3 * ip + d] + f) * a.scale;
+  for (int off = 4; off > 0; off >>= 1) {
+    e0 += __shfl_xor(e0, off, 64);
+    e1 += __shfl_xor(e1, off, 64);
+    e2 += __shfl_xor(e2, off, 64);
+  }
+  if (valid && iz == 0) {
+    const int ip = a.brick_ip[i];
+    const double q = (double)v.w;
+    const double es[3] = {(double)e0, (double)e1, (double)e2};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double f = -q * es[d] * a.m.dkdr[d] * a.epsi4pi - a.epsi4pi * a.surf2fac * q * M[d];
+      a.force[3 * ip + d] = (a.force[3 * ip + d] + f) * a.scale;
+    }
   }
 }
 
@@ -451,7 +479,7 @@ int launch_fft_dispatch(Ctx* c) {
 template <typename real, int P>
 void launch_spread_gather(Ctx* c, bool spread) {
   if (spread) {
-    SpreadArgs a{c->mesh, c->d_brick_start, c->d_brick_ip, c->d_mm, c->d_frac, c->d_az, c->d_Q};
+    SpreadArgs a{c->mesh, c->d_brick_start, c->d_bq, c->d_bmm, c->d_Q};
     prof_begin(c, MSG_K_SPREAD);
     hipLaunchKernelGGL((k_spread<real, P>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
     prof_end(c, MSG_K_SPREAD);
@@ -460,19 +488,17 @@ void launch_spread_gather(Ctx* c, bool spread) {
     g.m = c->mesh;
     g.np = c->np;
     g.brick_ip = c->d_brick_ip;
-    g.mm = c->d_mm;
-    g.frac = c->d_frac;
-    g.az = c->d_az;
+    g.bq = c->d_bq;
+    g.bmm = c->d_bmm;
     g.phi = c->d_Q;
     g.epsi4pi = c->P.epsi4pi;
     const double vol = c->P.boxlen[0] * c->P.boxlen[1] * c->P.boxlen[2];
     g.surf2fac = c->P.lsurf ? 2.0 * (2.0 * M_PI / (3.0 * vol)) : 0.0;
-    g.dip_part = c->d_part + c->part_off_dip;
-    g.ndip = c->npart_dip;
+    g.Mtot = c->d_M;
     g.force = c->d_force;
     g.scale = c->P.force_scale;
     prof_begin(c, MSG_K_GATHER);
-    hipLaunchKernelGGL((k_gather<real, P>), dim3((c->np + 255) / 256), dim3(256), 0, c->stream, g);
+    hipLaunchKernelGGL((k_gather<real, P>), dim3((8 * c->np + 255) / 256), dim3(256), 0, c->stream, g);
     prof_end(c, MSG_K_GATHER);
   }
 }
@@ -562,8 +588,11 @@ int spme_setup(Ctx* c) {
     if (int e = A((void**)&c->d_brick_tmp, np * sizeof(int))) return e;
     if (int e = A((void**)&c->d_brick_ip, np * sizeof(int))) return e;
     if (int e = A((void**)&c->d_mm, np * sizeof(int))) return e;
+    if (int e = A(&c->d_bq, (size_t)np * 4 * rs)) return e;
+    if (int e = A((void**)&c->d_bmm, np * sizeof(int))) return e;
     if (int e = A(&c->d_frac, (size_t)np * 3 * rs)) return e;
   }
+  MSG_HIP(hipMemset(c->d_brick_count, 0, m.nbrick * sizeof(int)));
   std::vector<double> tw(2 * K);
   for (int i = 0; i < K; ++i) {
     tw[2 * i] = std::cos(2.0 * M_PI * i / K);
@@ -582,17 +611,22 @@ int spme_setup(Ctx* c) {
   return MSG_OK;
 }
 
-int launch_spme(Ctx* c) {
+int launch_spme_mesh(Ctx* c) {
   int e;
   if (c->fp64) {
     if ((e = spread_gather_dispatch<double>(c, true))) return e;
     if ((e = launch_fft_dispatch<double>(c))) return e;
-    if ((e = spread_gather_dispatch<double>(c, false))) return e;
   } else {
     if ((e = spread_gather_dispatch<float>(c, true))) return e;
     if ((e = launch_fft_dispatch<float>(c))) return e;
-    if ((e = spread_gather_dispatch<float>(c, false))) return e;
   }
+  MSG_HIP(hipGetLastError());
+  return MSG_OK;
+}
+
+int launch_spme_gather(Ctx* c) {
+  int e = c->fp64 ? spread_gather_dispatch<double>(c, false) : spread_gather_dispatch<float>(c, false);
+  if (e) return e;
   MSG_HIP(hipGetLastError());
   return MSG_OK;
 }

@@ -1,0 +1,148 @@
+!> iso_c_binding interface of the MI355X non-bonded energy/force path (include/molsim_gpu.h).
+!!
+!! A MOLSIM build uses this module at the three places its non-bonded path is reached
+!! (UTotal, Src/energy.F90:280; DUTotal, Src/denergy.F90:98; MCUpdate, Src/mc.F90:5203) and
+!! passes its own arrays straight through: ro(3,np), az(na), iptpn(np), ubuf(nbuf),
+!! iubuflow(natat) (1-based, as MOLSIM holds them), force(3,na).  Nonzero status codes are
+!! turned into a stop with the library's message by GpuCheck, as MOLSIM's Stop does.
+module MolsimGpu
+
+   use, intrinsic :: iso_c_binding
+   implicit none
+
+   integer(c_int), parameter :: MSG_OK = 0
+   integer(c_int), parameter :: MSG_LIST_ALL = 0, MSG_LIST_CELL = 1, MSG_LIST_VERLET = 2
+   integer(c_int), parameter :: MSG_EWALD_NONE = 0, MSG_EWALD_STD = 1, MSG_EWALD_SPM = 2
+   integer(c_int), parameter :: MSG_PREC_MIXED = 0, MSG_PREC_FP64 = 1
+
+   type, bind(C) :: msg_params
+      integer(c_int) :: np, npt
+      real(c_double) :: boxlen(3)
+      real(c_double) :: rcut, drnlist, epsi4pi
+      integer(c_int) :: list_mode, ewald_mode, lsurf
+      real(c_double) :: alpha
+      integer(c_int) :: ncut, ncutregion, nmesh, order, precision
+      real(c_double) :: force_scale
+   end type msg_params
+
+   type, bind(C) :: msg_pot
+      integer(c_int) :: npt, form, lewald, pad
+      real(c_double) :: alpha, rcut, epsi4pi
+      real(c_double) :: utoltab, ftoltab, umaxtab, fmaxtab, r2uminin
+      real(c_double) :: z(8), sig(8), eps(8), rad(8)
+   end type msg_pot
+
+   interface
+
+      function msg_last_error() bind(C, name='msg_last_error')
+         import :: c_ptr
+         type(c_ptr) :: msg_last_error
+      end function
+
+      integer(c_int) function msg_build_table(pot, ubuf, maxbuf, iubuflow, nugrid, r2umin, r2atat) &
+            bind(C, name='msg_build_table')
+         import :: c_int, c_double, msg_pot
+         type(msg_pot), intent(in) :: pot
+         real(c_double), intent(out) :: ubuf(*)
+         integer(c_int), value :: maxbuf
+         integer(c_int), intent(out) :: iubuflow(*), nugrid(*)
+         real(c_double), intent(out) :: r2umin(*), r2atat(*)
+      end function
+
+      integer(c_int) function msg_create(params, ctx) bind(C, name='msg_create')
+         import :: c_int, c_ptr, msg_params
+         type(msg_params), intent(in) :: params
+         type(c_ptr), intent(out) :: ctx
+      end function
+
+      subroutine msg_destroy(ctx) bind(C, name='msg_destroy')
+         import :: c_ptr
+         type(c_ptr), value :: ctx
+      end subroutine
+
+      integer(c_int) function msg_set_table(ctx, ubuf, nbuf, iubuflow, nugrid, r2umin, r2atat) &
+            bind(C, name='msg_set_table')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: ubuf(*)
+         integer(c_int), value :: nbuf
+         integer(c_int), intent(in) :: iubuflow(*), nugrid(*)
+         real(c_double), intent(in) :: r2umin(*), r2atat(*)
+      end function
+
+      integer(c_int) function msg_set_particles(ctx, az, iptpn) bind(C, name='msg_set_particles')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: az(*)
+         integer(c_int), intent(in) :: iptpn(*)
+      end function
+
+      integer(c_int) function msg_set_positions(ctx, ro) bind(C, name='msg_set_positions')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: ro(3,*)
+      end function
+
+      integer(c_int) function msg_ewald_setup(ctx) bind(C, name='msg_ewald_setup')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
+      integer(c_int) function msg_nlist(ctx) bind(C, name='msg_nlist')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
+      integer(c_int) function msg_utotal(ctx, u_twob, u_rec, u_tot, force, virial) &
+            bind(C, name='msg_utotal')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: u_twob(0:*)
+         real(c_double), intent(out) :: u_rec, u_tot
+         real(c_double), intent(out) :: force(3,*)
+         real(c_double), intent(out) :: virial
+      end function
+
+      integer(c_int) function msg_dutotal(ctx, nptm, ipnptm, rotm, lhsoverlap, du_twob, du_rec, du_tot) &
+            bind(C, name='msg_dutotal')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nptm
+         integer(c_int), intent(in) :: ipnptm(*)
+         real(c_double), intent(in) :: rotm(3,*)
+         integer(c_int), intent(out) :: lhsoverlap
+         real(c_double), intent(out) :: du_twob(0:*)
+         real(c_double), intent(out) :: du_rec, du_tot
+      end function
+
+      integer(c_int) function msg_commit(ctx, nptm, ipnptm, rotm) bind(C, name='msg_commit')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nptm
+         integer(c_int), intent(in) :: ipnptm(*)
+         real(c_double), intent(in) :: rotm(3,*)
+      end function
+
+   end interface
+
+contains
+
+   !> Stop with the library's message on a nonzero status (MOLSIM's Stop semantics).
+   subroutine GpuCheck(rc, where)
+      integer(c_int), intent(in) :: rc
+      character(*), intent(in) :: where
+      character(len=256) :: text
+      character(kind=c_char), pointer :: p(:)
+      integer :: i
+      if (rc == MSG_OK) return
+      call c_f_pointer(msg_last_error(), p, [256])
+      text = ' '
+      do i = 1, 256
+         if (p(i) == c_null_char) exit
+         text(i:i) = p(i)
+      end do
+      write(*,'(a,a,a,i4,a,a)') 'molsim_gpu: ', where, ' failed with status ', rc, ': ', trim(text)
+      stop 1
+   end subroutine GpuCheck
+
+end module MolsimGpu
