@@ -151,6 +151,20 @@ int msg_dutotal_batch(msg_ctx* ctx, int ntrial, const int* ip, const double* rot
 /* MCUpdate after an accepted move: positions, resident structure factors / mesh, cell list. */
 int msg_commit(msg_ctx* ctx, int nptm, const int* ipnptm, const double* rotm);
 
+/* Velocity-Verlet MD on the device (VelVer, Src/md.F90:252-410): msg_md_init takes the
+ * mass of each particle type (in the units that make force * force_scale / mass an
+ * acceleration, as MOLSIM's (sclene/scllen)/sclfor scaling does) and the velocities
+ * vel(3,np) (NULL = zero), and computes the initial accelerations.  msg_md_run advances
+ * nsteps steps of length dt: drift, UTotal (with the Verlet-list skin test and rebuild in
+ * MSG_LIST_VERLET mode, Src/nlist.F90:597-621), kick.  log (may be NULL) receives
+ * [u_tot, kinetic energy] per step. */
+int msg_md_init(msg_ctx* ctx, const double* mass, const double* vel);
+int msg_md_run(msg_ctx* ctx, int nsteps, double dt, double* log);
+int msg_get_positions(msg_ctx* ctx, double* ro);
+int msg_get_velocities(msg_ctx* ctx, double* vel);
+/* Number of Verlet-list builds so far (NList statistics, Src/nlist.F90:945-1074). */
+long long msg_nlist_builds(msg_ctx* ctx);
+
 /* Stream the context launches on (a hipStream_t), for callers that time kernels. */
 void* msg_stream(msg_ctx* ctx);
 

@@ -224,10 +224,17 @@ int launch_reduce(Ctx* c) {
 // With SPME the pair kernel runs on a second stream beside the mesh kernels; the gather,
 // which adds to the pair forces, waits for it.  With per-kernel timing on, everything runs
 // on one stream so each kernel is timed alone.
-static int utotal_sequence(Ctx* c) {
+int utotal_sequence(Ctx* c) {
   int e;
+  bool rebuild = false;
+  if (c->P.list_mode == MSG_LIST_VERLET)
+    if ((e = verlet_check(c, rebuild))) return e;
   if ((e = launch_bin(c))) return e;
   if ((e = launch_sort(c))) return e;
+  if (rebuild) {
+    if ((e = launch_vlist_build(c))) return e;
+    c->vl_valid = true;
+  }
   if (c->P.ewald_mode == MSG_EWALD_SPM && !c->prof) {
     MSG_HIP(hipEventRecord(c->ev_fork, c->stream));
     MSG_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
@@ -303,8 +310,6 @@ int msg_create(const msg_params* p, msg_ctx** out) {
   if (!(p->rcut > 0)) return set_error(MSG_ERR_CELL, "msg_create: rcut must be > 0");
   if (p->list_mode < 0 || p->list_mode > 2 || p->ewald_mode < 0 || p->ewald_mode > 2)
     return set_error(MSG_ERR_ARG, "msg_create: bad list/ewald mode");
-  if (p->list_mode == MSG_LIST_VERLET)
-    return set_error(MSG_ERR_ARG, "msg_create: Verlet lists are not available in this build");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_error(MSG_ERR_HIP, "msg_create: no HIP device");
@@ -333,6 +338,7 @@ int msg_create(const msg_params* p, msg_ctx** out) {
   if ((e = dalloc(c, (void**)&c->d_ipt, np * sizeof(int)))) return fail(e);
   if ((e = dalloc(c, (void**)&c->d_force, 3 * np * sizeof(double)))) return fail(e);
   if ((e = lists_setup(c))) return fail(e);
+  if (p->list_mode == MSG_LIST_VERLET && (e = verlet_setup(c))) return fail(e);
   // partial-sum layout
   c->npart_pair = pair_blocks(c);
   c->npart_dip = (np + 255) / 256;
@@ -368,11 +374,14 @@ void msg_destroy(msg_ctx* h) {
                   c->d_brick_start, c->d_brick_fill, c->d_brick_tmp, c->d_brick_ip, c->d_mm,
                   c->d_frac, c->d_Q, c->d_C, c->d_G, c->d_tw, c->d_kfac, c->d_kvec, c->d_sk,
                   c->d_force, c->d_part, c->d_out, c->d_flags, c->d_de, c->d_de_ip, c->d_de_r,
-                  c->d_tcoef64, c->d_gk, c->d_item_cell, c->d_bq, c->d_bmm, c->d_M};
+                  c->d_tcoef64, c->d_gk, c->d_item_cell, c->d_bq, c->d_bmm, c->d_M,
+                  c->d_vl_home, c->d_vl_n, c->d_vl_nbr, c->d_ro_ref, c->d_dmax, c->d_vel,
+                  c->d_acc, c->d_minv, c->d_ke_part};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (c->h_out) hipHostFree(c->h_out);
   if (c->h_flags) hipHostFree(c->h_flags);
+  if (c->h_dmax) hipHostFree(c->h_dmax);
   for (auto& ev : c->ev)
     if (ev) hipEventDestroy(ev);
   if (c->side) hipStreamDestroy(c->side);
@@ -455,6 +464,10 @@ int msg_nlist(msg_ctx* h) {
   if (!c->have_positions) return set_error(MSG_ERR_STATE, "positions have not been set");
   int e;
   if ((e = launch_bin(c)) || (e = launch_sort(c))) return e;
+  if (c->P.list_mode == MSG_LIST_VERLET) {
+    if ((e = launch_vlist_build(c))) return e;
+    c->vl_valid = true;
+  }
   MSG_HIP(hipMemcpyAsync(c->h_flags, c->d_flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   if ((e = launch_zero(c))) return e;  // counters and flags back to zero for the next pass
   MSG_HIP(hipStreamSynchronize(c->stream));
@@ -466,7 +479,8 @@ int msg_utotal_async(msg_ctx* h, double* d_force) {
   if (!h) return set_error(MSG_ERR_ARG, "msg_utotal_async: null context");
   Ctx* c = &h->c;
   if (int e = check_ready(c)) return e;
-  const bool use_graph = !c->fp64 && !c->prof;
+  // the Verlet-list skin test is a host decision, so that mode launches eagerly
+  const bool use_graph = !c->fp64 && !c->prof && c->P.list_mode != MSG_LIST_VERLET;
   if (use_graph) {
     if (!c->graph_ok) {
       if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }

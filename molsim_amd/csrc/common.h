@@ -166,6 +166,26 @@ struct Ctx {
   double prof_ms[MSG_K_COUNT] = {0};
   int prof_n[MSG_K_COUNT] = {0};
 
+  // Verlet list (MSG_LIST_VERLET): full list at rcut + drnlist built from the cell list;
+  // neighbours are particle indices, column-major [maxn][np]; home order = cell order at build
+  int vl_maxn = 0;
+  int* d_vl_home = nullptr;
+  int* d_vl_n = nullptr;
+  int* d_vl_nbr = nullptr;
+  double* d_ro_ref = nullptr;    // positions at the last build
+  unsigned int* d_dmax = nullptr;  // max displacement^2 since the build (float bits)
+  unsigned int* h_dmax = nullptr;  // pinned
+  bool vl_valid = false;
+  bool disp_done = false;        // the MD drift kernel already computed the displacement
+  long long vl_builds = 0;
+
+  // molecular dynamics (velocity Verlet)
+  double* d_vel = nullptr;       // 3*np
+  double* d_acc = nullptr;       // 3*np
+  double* d_minv = nullptr;      // per type 1/mass
+  double* d_ke_part = nullptr;
+  bool md_ready = false;
+
   // dE: FP64 copy of the packed table, real-space influence kernel g (H^3, folded by
   // symmetry), per-trial inputs/outputs, and whether phi / S / cells / M match the positions
   double* d_tcoef64 = nullptr;
@@ -201,6 +221,11 @@ int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ov
                   double* du_tot, double* du_rec, double* du_twob);
 int commit_move(Ctx* c, int ip, const double* rnew);
 int utotal_refresh(Ctx* c);                        // UTotal sequence + sync (for dE state)
+int utotal_sequence(Ctx* c);
+int verlet_setup(Ctx* c);
+int verlet_check(Ctx* c, bool& rebuild);            // displacement test (host sync)
+int launch_vlist_build(Ctx* c);
+int launch_disp(Ctx* c);
 
 // profiling wrappers around a launch
 void prof_begin(Ctx* c, int kid);

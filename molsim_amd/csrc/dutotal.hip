@@ -394,7 +394,7 @@ int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ov
   a.slot_of = c->d_slot_of;
   a.slot_cell = c->d_slot_cell;
   a.rcut2 = c->P.rcut * c->P.rcut;
-  a.cut_inclusive = c->P.list_mode == MSG_LIST_ALL;
+  a.cut_inclusive = c->P.list_mode != MSG_LIST_CELL;
   a.tcoef = c->d_tcoef64;
   a.tbins = c->d_tbins;
   a.tm = c->tmeta;

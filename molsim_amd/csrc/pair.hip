@@ -386,11 +386,158 @@ __global__ __launch_bounds__(256) void k_pair_nb(PairArgs a) {
   }
 }
 
+// Verlet list build (SetVListMC semantics, Src/nlist.F90:820-884, as a full list that also
+// serves MD): one thread per home particle in cell order scans the 27 stencil cells of the
+// (rcut + skin) grid and keeps every partner within rcut + skin.  Also snapshots the
+// positions the displacement test measures from and clears the displacement maximum.
+struct VlistArgs {
+  int np, maxn;
+  const double4* cpos;
+  const int* cell_ip;
+  const int* cell_start;
+  const int* slot_cell;
+  CellGrid g;
+  Box box;
+  double rl2;          // (rcut + skin)^2
+  const double* ro;
+  double* ro_ref;
+  int* vl_home;
+  int* vl_n;
+  int* vl_nbr;
+  unsigned int* dmax;
+  int* flags;
+};
+
+__global__ __launch_bounds__(256) void k_vlist_build(VlistArgs a) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s == 0) *a.dmax = 0u;
+  if (s >= a.np) return;
+  const double4 hp = a.cpos[s];
+  const int ip = a.cell_ip[s];
+  a.vl_home[s] = ip;
+  a.ro_ref[3 * ip] = a.ro[3 * ip];
+  a.ro_ref[3 * ip + 1] = a.ro[3 * ip + 1];
+  a.ro_ref[3 * ip + 2] = a.ro[3 * ip + 2];
+  const int c = a.slot_cell[s];
+  const int cx = c % a.g.nc[0], cy = (c / a.g.nc[0]) % a.g.nc[1], cz = c / (a.g.nc[0] * a.g.nc[1]);
+  int n = 0;
+  for (int k = 0; k < a.g.nst; ++k) {
+    int x = cx + a.g.st[3 * k], y = cy + a.g.st[3 * k + 1], z = cz + a.g.st[3 * k + 2];
+    x += x < 0 ? a.g.nc[0] : (x >= a.g.nc[0] ? -a.g.nc[0] : 0);
+    y += y < 0 ? a.g.nc[1] : (y >= a.g.nc[1] ? -a.g.nc[1] : 0);
+    z += z < 0 ? a.g.nc[2] : (z >= a.g.nc[2] ? -a.g.nc[2] : 0);
+    const int nb = x + a.g.nc[0] * (y + a.g.nc[1] * z);
+    const int j1 = a.cell_start[nb + 1];
+    for (int j = a.cell_start[nb]; j < j1; ++j) {
+      if (j == s) continue;
+      const double4 pj = a.cpos[j];
+      double d[3] = {hp.x - pj.x, hp.y - pj.y, hp.z - pj.z};
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        if (fabs(d[q]) > a.box.half[q]) d[q] -= copysign(a.box.L[q], d[q]);
+      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (r2 > a.rl2) continue;
+      if (n < a.maxn) a.vl_nbr[(size_t)n * a.np + s] = a.cell_ip[j];
+      ++n;
+    }
+  }
+  if (n > a.maxn) {
+    atomicOr(a.flags, FLAG_NLIST);
+    n = a.maxn;
+  }
+  a.vl_n[s] = n;
+}
+
+// Pair loop over the Verlet list (UTwoBodyA over jpnlist, Src/energy.F90:445-544): one thread
+// per home particle, partner positions read from the caller-order array, two candidates per
+// iteration with both loads issued first.
+struct VlPairArgs {
+  PairArgs p;
+  int maxn;
+  const int* vl_home;
+  const int* vl_n;
+  const int* vl_nbr;
+  const double* ro;
+  const double* az;
+  const int* ipt;
+};
+
+template <typename real, int NTP>
+__global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
+  const PairArgs& a = v.p;
+  extern __shared__ double s_dyn[];
+  __shared__ real s_zlo[kMaxTP], s_invw[kMaxTP], s_r2at[kMaxTP], s_r2um[kMaxTP];
+  __shared__ int s_last[kMaxTP], s_tp[kMaxT * kMaxT];
+  __shared__ double s_red[4][kMaxTP + 1];
+  const int nint = a.tm.nint;
+  real* s_coef = (real*)s_dyn;
+  int* s_bins = (int*)(s_coef + nint * 12);
+  {
+    const real* src = (const real*)a.tcoef;
+    for (int k = threadIdx.x; k < nint * 12; k += 256) s_coef[k] = src[k];
+    for (int k = threadIdx.x; k < NTP * kTabBins; k += 256) s_bins[k] = a.tbins[k];
+    if (threadIdx.x < NTP) {
+      const int tp = threadIdx.x;
+      s_zlo[tp] = (real)a.tm.zlo[tp];
+      s_invw[tp] = (real)a.tm.invw[tp];
+      s_r2at[tp] = (real)a.tm.r2atat[tp];
+      s_r2um[tp] = (real)a.tm.r2umin[tp];
+      s_last[tp] = a.tm.last[tp];
+    }
+    if (threadIdx.x < kMaxT * kMaxT) s_tp[threadIdx.x] = a.tm.tpidx[threadIdx.x];
+  }
+  __syncthreads();
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const real rc2 = (real)a.rcut2;
+  PairAcc<real, NTP> acc;
+  acc.zero();
+  if (s < a.np) {
+    const int ip = v.vl_home[s];
+    const double4 hp = make_double4(v.ro[3 * ip], v.ro[3 * ip + 1], v.ro[3 * ip + 2], 0.0);
+    const int ht = v.ipt[ip];
+    const int n = v.vl_n[s];
+    for (int k = 0; k < n; k += 2) {
+      const int jA = v.vl_nbr[(size_t)k * a.np + s];
+      const bool haveB = k + 1 < n;
+      const int jB = haveB ? v.vl_nbr[(size_t)(k + 1) * a.np + s] : jA;
+      const double4 pA = make_double4(v.ro[3 * jA], v.ro[3 * jA + 1], v.ro[3 * jA + 2], 0.0);
+      const double4 pB = make_double4(v.ro[3 * jB], v.ro[3 * jB + 1], v.ro[3 * jB + 2], 0.0);
+      const int tA = v.ipt[jA], tB = v.ipt[jB];
+      pair_eval<real, NTP>(hp, ht, pA, tA, a.box, rc2, true, s_coef, s_bins, s_zlo, s_invw,
+                           s_r2at, s_r2um, s_last, s_tp, acc);
+      if (haveB)
+        pair_eval<real, NTP>(hp, ht, pB, tB, a.box, rc2, true, s_coef, s_bins, s_zlo, s_invw,
+                             s_r2at, s_r2um, s_last, s_tp, acc);
+    }
+    a.force[3 * ip + 0] = acc.fx * a.scale;
+    a.force[3 * ip + 1] = acc.fy * a.scale;
+    a.force[3 * ip + 2] = acc.fz * a.scale;
+  }
+  if (acc.err) atomicOr(a.flags, acc.err);
+#pragma unroll
+  for (int q = 0; q < NTP; ++q) {
+    const double t = msgd::wave_sum(acc.et[q]);
+    if (lane == 0) s_red[wv][q] = t;
+  }
+  {
+    const double t = msgd::wave_sum(acc.vir);
+    if (lane == 0) s_red[wv][NTP] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x <= NTP) {
+    double t = 0.0;
+    for (int w = 0; w < 4; ++w) t += s_red[w][threadIdx.x];
+    a.part[blockIdx.x * (a.tm.ntp + 1) + (threadIdx.x == NTP ? a.tm.ntp : threadIdx.x)] = t;
+  }
+}
+
 }  // namespace
 
 // Work items: sum over cells of ceil(count / kPairHB) <= ncell + np / kPairHB; a grid of
 // up to 8 workgroups per CU walks them with a stride.
 int pair_blocks(Ctx* c) {
+  if (c->P.list_mode == MSG_LIST_VERLET) return (c->np + 255) / 256;
   if (c->grid.nst == 27 && c->P.list_mode == MSG_LIST_CELL) return (c->np + 7) / 8;
   const long items = (long)c->grid.ncell + c->np / kPairHB + 1;
   return (int)std::max(1L, std::min((items + kWaves - 1) / kWaves, 2048L));
@@ -426,6 +573,32 @@ int launch_pair(Ctx* c, bool last) {
                      (size_t)c->tmeta.ntp * kTabBins * sizeof(int);
   prof_begin(c, MSG_K_PAIR);
   const bool nbpar = c->grid.nst == 27 && c->P.list_mode == MSG_LIST_CELL;
+  if (c->P.list_mode == MSG_LIST_VERLET) {
+    VlPairArgs v;
+    v.p = a;
+    v.p.cut_inclusive = 1;  // UTwoBodyA's r2 > rcut2 test
+    v.maxn = c->vl_maxn;
+    v.vl_home = c->d_vl_home;
+    v.vl_n = c->d_vl_n;
+    v.vl_nbr = c->d_vl_nbr;
+    v.ro = c->d_ro;
+    v.az = c->d_az;
+    v.ipt = c->d_ipt;
+    prof_begin(c, MSG_K_PAIR);
+    switch (c->nptpt) {
+#define MSG_VL(NTP)                                                                            \
+  if (c->fp64) hipLaunchKernelGGL((k_pair_vl<double, NTP>), dim3(nb), dim3(256), lds, c->stream, v); \
+  else hipLaunchKernelGGL((k_pair_vl<float, NTP>), dim3(nb), dim3(256), lds, c->stream, v);
+      case 1: MSG_VL(1) break;
+      case 3: MSG_VL(3) break;
+      case 6: MSG_VL(6) break;
+      default: MSG_VL(10) break;
+#undef MSG_VL
+    }
+    prof_end(c, MSG_K_PAIR);
+    MSG_HIP(hipGetLastError());
+    return MSG_OK;
+  }
 #define MSG_PAIR_LAUNCH(R, NTP)                                                              \
   do {                                                                                       \
     if (nbpar)                                                                               \
@@ -442,6 +615,41 @@ int launch_pair(Ctx* c, bool last) {
 #undef MSG_PAIR_LAUNCH
   prof_end(c, MSG_K_PAIR);
   MSG_HIP(hipGetLastError());
+  return MSG_OK;
+}
+
+}  // namespace msg
+
+namespace msg {
+
+int launch_vlist_build(Ctx* c) {
+  VlistArgs v;
+  v.np = c->np;
+  v.maxn = c->vl_maxn;
+  v.cpos = (const double4*)c->d_cpos;
+  v.cell_ip = c->d_cell_ip;
+  v.cell_start = c->d_cell_start;
+  v.slot_cell = c->d_slot_cell;
+  v.g = c->grid;
+  for (int d = 0; d < 3; ++d) {
+    v.box.L[d] = c->P.boxlen[d];
+    v.box.half[d] = 0.5 * c->P.boxlen[d];
+    v.box.inv[d] = 1.0 / c->P.boxlen[d];
+  }
+  const double rl = c->P.rcut + c->P.drnlist;
+  v.rl2 = rl * rl;
+  v.ro = c->d_ro;
+  v.ro_ref = c->d_ro_ref;
+  v.vl_home = c->d_vl_home;
+  v.vl_n = c->d_vl_n;
+  v.vl_nbr = c->d_vl_nbr;
+  v.dmax = c->d_dmax;
+  v.flags = c->d_flags;
+  prof_begin(c, MSG_K_VLIST);
+  hipLaunchKernelGGL(k_vlist_build, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, v);
+  prof_end(c, MSG_K_VLIST);
+  MSG_HIP(hipGetLastError());
+  c->vl_builds++;
   return MSG_OK;
 }
 

@@ -91,6 +91,12 @@ def lib():
         L.msg_dutotal_batch.argtypes = [_vp, C.c_int, _ip, _dp, _ip, _dp, _vp]
         L.msg_commit.argtypes = [_vp, C.c_int, _ip, _dp]
         L.msg_get_cell_list.argtypes = [_vp, _ip, _vp, _vp]
+        L.msg_md_init.argtypes = [_vp, _dp, _vp]
+        L.msg_md_run.argtypes = [_vp, C.c_int, C.c_double, _vp]
+        L.msg_get_positions.argtypes = [_vp, _dp]
+        L.msg_get_velocities.argtypes = [_vp, _dp]
+        L.msg_nlist_builds.argtypes = [_vp]
+        L.msg_nlist_builds.restype = C.c_longlong
         L.msg_stream.argtypes = [_vp]
         L.msg_stream.restype = _vp
         L.msg_profile.argtypes = [_vp, C.c_int]
@@ -266,6 +272,31 @@ class Engine:
         ips = np.zeros(self.spec.n, dtype=np.int32)
         _check(lib().msg_get_cell_list(self.h, nc, _ptr(start), _ptr(ips)))
         return nc, start, ips - 1
+
+    # -- VelVer MD --------------------------------------------------------------------------
+    def md_init(self, mass, vel=None):
+        """Masses per particle type and initial velocities (N, 3) (None = zero)."""
+        m = np.ascontiguousarray(mass, dtype=np.float64)
+        v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(-1)
+        _check(lib().msg_md_init(self.h, m, _ptr(v) if v is not None else None))
+
+    def md_run(self, nsteps: int, dt: float, log: bool = True):
+        out = np.zeros(2 * nsteps) if log else None
+        _check(lib().msg_md_run(self.h, int(nsteps), float(dt), _ptr(out) if out is not None else None))
+        return None if out is None else out.reshape(nsteps, 2)
+
+    def positions(self):
+        ro = np.zeros(3 * self.spec.n)
+        _check(lib().msg_get_positions(self.h, ro))
+        return ro.reshape(-1, 3)
+
+    def velocities(self):
+        v = np.zeros(3 * self.spec.n)
+        _check(lib().msg_get_velocities(self.h, v))
+        return v.reshape(-1, 3)
+
+    def nlist_builds(self) -> int:
+        return int(lib().msg_nlist_builds(self.h))
 
     # -- measurement ----------------------------------------------------------------------
     def stream(self) -> int:
