@@ -165,6 +165,21 @@ int msg_get_velocities(msg_ctx* ctx, double* vel);
 /* Number of Verlet-list builds so far (NList statistics, Src/nlist.F90:945-1074). */
 long long msg_nlist_builds(msg_ctx* ctx);
 
+/* MOLSIM's random number generator (Random, Src/mollib.F90:2689-2713): *iseed as the
+ * reference's iseed (negative or zero re-seeds), state[2] its two sequences (start with
+ * {-1, -1}), *am its scale. */
+double msg_random(int* iseed, int* state, double* am);
+
+/* ntrial single-particle MC trials (MCPass / SPartMove / Metropolis / MCUpdate,
+ * Src/mc.F90:1470-1750,5135-5235): particle chosen sequentially (isamp 0) or at random
+ * (isamp 1); trial displacement (Random - 1/2) * dtran[type] per axis; Metropolis with
+ * beta * dE (expmax = 87); accepted moves are committed.  The generator state (iseed,
+ * rng_state, am) is read and updated.  Outputs: *du_sum (sum of accepted dE), *naccept,
+ * events[t] (1 accepted, 0 rejected, -1 hard-core overlap; may be NULL). */
+int msg_mc_pass(msg_ctx* ctx, int ntrial, int isamp, const double* dtran, double beta,
+                int* iseed, int* rng_state, double* am, double* du_sum, int* naccept,
+                int* events);
+
 /* Stream the context launches on (a hipStream_t), for callers that time kernels. */
 void* msg_stream(msg_ctx* ctx);
 

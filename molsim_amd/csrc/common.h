@@ -196,6 +196,7 @@ struct Ctx {
   int* d_de_ip = nullptr;
   double* d_de_r = nullptr;
   bool fresh = false;            // a UTotal has run on the current positions
+  int mc_next = 0;               // next particle of a sequential MC pass (isamp = 0)
 };
 
 // allocation helper (tracked in ctx->bytes)

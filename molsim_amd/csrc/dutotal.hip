@@ -445,10 +445,92 @@ int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ov
 
 // MCUpdate for one accepted single-particle move: the position changes on the device and
 // the derived state (cells, phi or S, dipole) is rebuilt by the next UTotal pass.
+int commit_fast(Ctx* c, int ip, const double* rnew, bool& done);
+
 int commit_move(Ctx* c, int ip, const double* rnew) {
+  bool done = false;
+  if (int e = commit_fast(c, ip, rnew, done)) return e;
+  if (done) return MSG_OK;
   MSG_HIP(hipMemcpyAsync(c->d_ro + 3 * ip, rnew, 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
   MSG_HIP(hipStreamSynchronize(c->stream));
   c->fresh = false;
+  return MSG_OK;
+}
+
+}  // namespace msg
+
+namespace msg {
+
+namespace {
+
+// Incremental MCUpdate for the standard-Ewald state (EwaldUpdateArray, Src/denergy.F90:1776-
+// 1819): S(k) += q (e^{ik.r_new} - e^{ik.r_old}) for every vector, the position (caller and
+// cell order) and the dipole sum move with the particle.  One workgroup.
+__global__ __launch_bounds__(256) void k_commit_std(int ip, double rx, double ry, double rz,
+                                                    double* ro, double4* cpos, const int* slot_of,
+                                                    const double* az, int nk, const int* kvec,
+                                                    double tpb0, double tpb1, double tpb2, double* sk,
+                                                    double* Mout, double* Mtot) {
+  const double q = az[ip];
+  const double ro_old[3] = {ro[3 * ip], ro[3 * ip + 1], ro[3 * ip + 2]};
+  const double rn[3] = {rx, ry, rz};
+  for (int kn = threadIdx.x; kn < nk; kn += 256) {
+    const double kx = kvec[3 * kn] * tpb0, ky = kvec[3 * kn + 1] * tpb1, kz = kvec[3 * kn + 2] * tpb2;
+    double sxn, cxn, syn, cyn, szn, czn, sxo, cxo, syo, cyo, szo, czo;
+    sincos(kx * rn[0], &sxn, &cxn);
+    sincos(ky * rn[1], &syn, &cyn);
+    sincos(kz * rn[2], &szn, &czn);
+    sincos(kx * ro_old[0], &sxo, &cxo);
+    sincos(ky * ro_old[1], &syo, &cyo);
+    sincos(kz * ro_old[2], &szo, &czo);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double gxn = (c & 2) ? sxn : -sxn, gyn = (c & 1) ? syn : -syn;
+      const double gxo = (c & 2) ? sxo : -sxo, gyo = (c & 1) ? syo : -syo;
+      const double r1n = cxn * cyn - gxn * gyn, i1n = cxn * gyn + gxn * cyn;
+      const double r1o = cxo * cyo - gxo * gyo, i1o = cxo * gyo + gxo * cyo;
+      sk[8 * kn + 2 * c] += q * ((r1n * czn - i1n * szn) - (r1o * czo - i1o * szo));
+      sk[8 * kn + 2 * c + 1] += q * ((r1n * szn + i1n * czn) - (r1o * szo + i1o * czo));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < 3; ++d) {
+      Mout[d] += q * (rn[d] - ro_old[d]);
+      Mtot[d] = Mout[d];
+      ro[3 * ip + d] = rn[d];
+    }
+    double4 p = cpos[slot_of[ip]];
+    p.x = rn[0];
+    p.y = rn[1];
+    p.z = rn[2];
+    cpos[slot_of[ip]] = p;
+  }
+}
+
+}  // namespace
+
+// Accepted move: incremental for std Ewald when the particle keeps its cell, otherwise the
+// position changes and the derived state is rebuilt by the next UTotal pass.
+int commit_fast(Ctx* c, int ip, const double* rnew, bool& done) {
+  done = false;
+  if (c->P.ewald_mode != MSG_EWALD_STD || !c->fresh) return MSG_OK;
+  std::vector<double> ro(3);
+  MSG_HIP(hipMemcpy(ro.data(), c->d_ro + 3 * ip, 3 * sizeof(double), hipMemcpyDeviceToHost));
+  for (int d = 0; d < 3; ++d) {
+    const double h = 0.5 * c->P.boxlen[d];
+    const int a = std::min(std::max((int)std::floor((ro[d] + h) * c->grid.csi[d]), 0), c->grid.nc[d] - 1);
+    const int b = std::min(std::max((int)std::floor((rnew[d] + h) * c->grid.csi[d]), 0), c->grid.nc[d] - 1);
+    if (a != b) return MSG_OK;
+  }
+  const int ntp = c->nptpt;
+  hipLaunchKernelGGL(k_commit_std, dim3(1), dim3(256), 0, c->stream, ip, rnew[0], rnew[1], rnew[2],
+                     c->d_ro, (double4*)c->d_cpos, c->d_slot_of, c->d_az, c->nkvec, c->d_kvec,
+                     2.0 * M_PI / c->P.boxlen[0], 2.0 * M_PI / c->P.boxlen[1],
+                     2.0 * M_PI / c->P.boxlen[2], c->d_sk, c->d_out + ntp + 4, c->d_M);
+  MSG_HIP(hipGetLastError());
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  done = true;
   return MSG_OK;
 }
 

@@ -95,6 +95,10 @@ def lib():
         L.msg_md_run.argtypes = [_vp, C.c_int, C.c_double, _vp]
         L.msg_get_positions.argtypes = [_vp, _dp]
         L.msg_get_velocities.argtypes = [_vp, _dp]
+        L.msg_random.argtypes = [C.POINTER(C.c_int), _ip, C.POINTER(C.c_double)]
+        L.msg_random.restype = C.c_double
+        L.msg_mc_pass.argtypes = [_vp, C.c_int, C.c_int, _dp, C.c_double, C.POINTER(C.c_int), _ip,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), _vp]
         L.msg_nlist_builds.argtypes = [_vp]
         L.msg_nlist_builds.restype = C.c_longlong
         L.msg_stream.argtypes = [_vp]
@@ -145,6 +149,18 @@ def build_table(pot: dict, maxbuf: int = 200000) -> dict:
     r2atat = np.zeros(nptpt)
     n = _check(lib().msg_build_table(C.byref(p), ubuf, maxbuf, low, ng, r2umin, r2atat))
     return dict(ubuf=ubuf[:n].copy(), iubuflow=low, nugrid=ng, r2umin=r2umin, r2atat=r2atat)
+
+
+class Random:
+    """MOLSIM's Random(iseed) generator (Src/mollib.F90:2689-2713), state carried here."""
+
+    def __init__(self, iseed: int):
+        self.iseed = C.c_int(iseed)
+        self.state = np.array([-1, -1], dtype=np.int32)
+        self.am = C.c_double(0.0)
+
+    def __call__(self) -> float:
+        return lib().msg_random(C.byref(self.iseed), self.state, C.byref(self.am))
 
 
 def _ptr(a):
@@ -272,6 +288,16 @@ class Engine:
         ips = np.zeros(self.spec.n, dtype=np.int32)
         _check(lib().msg_get_cell_list(self.h, nc, _ptr(start), _ptr(ips)))
         return nc, start, ips - 1
+
+    def mc_pass(self, ntrial: int, dtran, beta: float, rng: "Random", isamp: int = 0):
+        """MCPass of single-particle moves; returns (sum of accepted dE, naccept, events)."""
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dtran, dtype=np.float64),
+                                                  (self.spec.npt,)))
+        ev = np.zeros(ntrial, dtype=np.int32)
+        dsum, nacc = C.c_double(), C.c_int()
+        _check(lib().msg_mc_pass(self.h, int(ntrial), int(isamp), dt, float(beta), C.byref(rng.iseed),
+                                 rng.state, C.byref(rng.am), C.byref(dsum), C.byref(nacc), _ptr(ev)))
+        return dsum.value, nacc.value, ev
 
     # -- VelVer MD --------------------------------------------------------------------------
     def md_init(self, mass, vel=None):

@@ -53,3 +53,13 @@ def test_build_table_error_paths():
     with pytest.raises(engine.MolsimGpuError) as e:
         engine.build_table(pot, maxbuf=24)   # two records only
     assert e.value.code == -5
+
+
+@pytest.mark.parametrize("seed", [-1, -12345, 7, -2147483646])
+def test_random_generator_matches_oracle(orc, seed):
+    a = engine.Random(seed)
+    b = orc.Rng(seed)
+    xa = [a() for _ in range(5000)]
+    xb = [b() for _ in range(5000)]
+    assert xa == xb
+    assert a.iseed.value == b.idum.value
