@@ -66,6 +66,126 @@ def timed_steps(eng, steps: int) -> float:
     return time.perf_counter() - t0
 
 
+def run_ensemble(args):
+    """Batched ensemble: 512 independent electrolyte / polyelectrolyte systems (N log-uniform
+    in 200..20000, volume fraction 0.01, SPME 16^3..64^3 by box length, +-1/+-2 ions), all
+    UTotal passes in one captured graph per step, pair tiles from the autotuner; 1000 MC
+    trials per system batched across systems.  Beside it: the plain FP64 GPU path run one
+    system at a time (eager kernels, a host synchronisation per system)."""
+    from molsim_amd import autotune, systems
+    t0 = time.perf_counter()
+    specs = systems.ensemble(args.nsys, workers=8)   # forked workers: before any GPU call
+    gen_s = time.perf_counter() - t0
+    from molsim_amd.engine import Batch, Engine, build_table
+    table = build_table(specs[0].pot)
+    ntot = sum(sp.n for sp in specs)
+    print(f"# ensemble: {len(specs)} systems, {ntot} particles (generated in {gen_s:.1f} s)",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    engs = [Engine(sp, precision="mixed", table=table) for sp in specs]
+    print(f"# {len(engs)} contexts in {time.perf_counter() - t0:.1f} s, "
+          f"{sum(e.device_bytes() for e in engs) / 2**30:.2f} GiB resident",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    tune = autotune.tune(engs, reps=3)
+    autotune.apply(engs, tune)
+    tune_s = time.perf_counter() - t0
+    print(f"# autotune {tune_s:.1f} s: " + ", ".join(
+        f"b{b}(n~{v['n']})->{v['tile']}" for b, v in sorted(tune.items())), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    batch = Batch(engs)
+    cap_s = time.perf_counter() - t0
+    print(f"# batch graph captured in {cap_s:.2f} s", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        batch.utotal_async()
+    batch.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.utotal_async()
+    batch.sync()
+    dt = time.perf_counter() - t0
+    value = ntot * args.steps / dt
+    print(f"# batched: {1e3 * dt / args.steps:.3f} ms per ensemble step, {value:.3e} particle-steps/s",
+          file=sys.stderr, flush=True)
+    if args.profile_only:
+        return
+    res = batch.utotal()
+
+    # 1000 MC trials per system, one batched call
+    rng = np.random.default_rng(7)
+    ips, rns = [], []
+    for sp in specs:
+        ip = rng.integers(0, sp.n, size=args.ntrial)
+        rn = sp.ro[ip] + rng.uniform(-1.0, 1.0, size=(args.ntrial, 3))
+        L = sp.box[0]
+        ips.append(ip)
+        rns.append((rn + L / 2) % L - L / 2)
+    batch.dutotal(ips, rns)
+    t0 = time.perf_counter()
+    de = batch.dutotal(ips, rns)
+    dt_de = time.perf_counter() - t0
+    print(f"# batched dE: {len(specs) * args.ntrial / dt_de:.3e} trials/s", file=sys.stderr, flush=True)
+
+    # plain FP64 GPU path, one system at a time; its results are the reference for the
+    # mixed batch's errors
+    t_sum, t_de, n_de = 0.0, 0.0, 0
+    ferr, eerr, deerr = [], [], []
+    for s, sp in enumerate(specs):
+        e64 = Engine(sp, precision="fp64", table=table)
+        e64.utotal_async()
+        e64.fetch()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            e64.utotal_async()
+            e64.fetch()
+        t_sum += (time.perf_counter() - t0) / 3
+        r64 = e64.utotal()
+        f64 = r64["force"]
+        ferr.append(float(np.linalg.norm(res[s]["force"] - f64) / np.linalg.norm(f64)))
+        eerr.append(abs(res[s]["u_tot"] - r64["u_tot"]) / (abs(r64["u_tot"]) + abs(r64["u_rec"])))
+        k = args.fp64_trials
+        t0 = time.perf_counter()
+        for t in range(k):
+            d = e64.dutotal(int(ips[s][t]), rns[s][t])
+            if not d["overlap"]:
+                deerr.append(abs(d["du_tot"] - de[s]["du_tot"][t]))
+        t_de += time.perf_counter() - t0
+        n_de += k
+        e64.close()
+    value64 = ntot / t_sum
+    print(f"# plain FP64 one at a time: {1e3 * t_sum:.2f} ms per ensemble pass, {value64:.3e} "
+          f"particle-steps/s; speedup {value / value64:.1f}x", file=sys.stderr, flush=True)
+    batch.close()
+    for e in engs:
+        e.close()
+    nb = np.array([sp.n for sp in specs])
+    return {
+        "metric": "ensemble SPME E+F particle-steps/s (512 systems, one graph launch per step)",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64acc",
+        "data": "synthetic (seeded electrolytes and random-walk polyelectrolytes)",
+        "config": {"workload": f"ensemble_{len(specs)}", "systems": len(specs), "particles": int(ntot),
+                   "n_min": int(nb.min()), "n_max": int(nb.max()),
+                   "meshes": {str(k): int((np.array([sp.nmesh for sp in specs]) == k).sum())
+                              for k in (16, 32, 64)},
+                   "rcut": specs[0].rcut, "alpha": specs[0].alpha, "order": specs[0].order,
+                   "parallelism": "single GPU"},
+        "autotune": {str(b): {"n": v["n"], "tile": v["tile"],
+                              "pair_us": {str(t): round(1e3 * ms, 2) for t, ms in v["ms"].items()}}
+                     for b, v in sorted(tune.items())},
+        "fp64_gpu_path": {"value": value64, "ms_per_ensemble_pass": 1e3 * t_sum,
+                          "speedup_of_batched": value / value64,
+                          "what": "plain FP64 path, eager, one system at a time"},
+        "dE": {"trials_per_s": len(specs) * args.ntrial / dt_de,
+               "fp64_one_at_a_time_trials_per_s": n_de / t_de,
+               "speedup": (len(specs) * args.ntrial / dt_de) / (n_de / t_de),
+               "max_abs_err_vs_fp64_kJmol": float(max(deerr)) if deerr else None},
+        "err_vs_fp64_path": {"force_rel_l2_max": float(max(ferr)), "u_tot_rel_max": float(max(eerr))},
+        "setup_s": {"generate": gen_s, "autotune": tune_s, "capture": cap_s},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,10 +196,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps of the timed path only (for rocprofv3)")
+    ap.add_argument("--workload", default="polyelectrolyte",
+                    choices=["polyelectrolyte", "ensemble"])
+    ap.add_argument("--nsys", type=int, default=512, help="ensemble: number of systems")
+    ap.add_argument("--ntrial", type=int, default=1000, help="ensemble: MC trials per system")
+    ap.add_argument("--fp64-trials", type=int, default=20,
+                    help="ensemble: trials per system timed on the plain FP64 path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload == "ensemble":
+        out = run_ensemble(args)
+        if out:
+            print(json.dumps(out), flush=True)
+        return
 
     from molsim_amd import systems
     from molsim_amd.engine import Engine, build_table

@@ -180,6 +180,42 @@ int msg_mc_pass(msg_ctx* ctx, int ntrial, int isamp, const double* dtran, double
                 int* iseed, int* rng_state, double* am, double* du_sum, int* naccept,
                 int* events);
 
+/* Batched ensemble (no counterpart in the reference, which runs one system per process).
+ * n resident cell-list systems (SPME or no Ewald) evaluated together: every stage of the
+ * UTotal pass is one launch over the workgroups of all systems, captured once into a
+ * hipGraph, so msg_batch_utotal is a single graph launch.  After msg_batch_sync,
+ * msg_batch_results gives every system's energies in one download (u_tot, u_rec, virial:
+ * arrays of n, any may be null) and msg_fetch / msg_get_forces a single system's.
+ * msg_batch_dutotal runs ntrial[s] independent trials of every system s (indices 1-based,
+ * indices and positions concatenated over systems in batch order) as one launch with one
+ * synchronisation, after a batched pass if any system moved.
+ * Contract: a member context used on its own must be synchronised (msg_fetch, msg_utotal)
+ * before the next batched call, and the batch synchronised (msg_batch_sync or a call that
+ * returns results) before a member is used on its own.  Tiles (msg_set_tile) and Ewald
+ * set-up are read at msg_batch_create. */
+typedef struct msg_batch msg_batch;
+int msg_batch_create(msg_ctx** ctxs, int n, msg_batch** out);
+void msg_batch_destroy(msg_batch* batch);
+int msg_batch_utotal(msg_batch* batch);
+int msg_batch_sync(msg_batch* batch);
+int msg_batch_query(msg_batch* batch);
+int msg_batch_results(msg_batch* batch, double* u_tot, double* u_rec, double* virial);
+int msg_batch_dutotal(msg_batch* batch, const int* ntrial, const int* ip, const double* rotm,
+                      int* lhsoverlap, double* du_tot);
+
+/* 1 when everything launched on the context's stream has completed, 0 while work is pending. */
+int msg_query(msg_ctx* ctx);
+
+/* Pair-kernel tile of a cell-list system (no counterpart in the reference): pair_lanes lanes
+ * per home particle, 256 / pair_lanes homes per workgroup (8, 16, 32 or 64; 32 by default), or
+ * 0 for the wave-per-cell-chunk kernel.  Results do not depend on the tile beyond FP32
+ * summation order.  Set by the autotuner per size bucket. */
+int msg_set_tile(msg_ctx* ctx, int pair_lanes);
+int msg_get_tile(msg_ctx* ctx);
+
+/* Forces force(3,np) of the last UTotal pass (scaled like msg_utotal's), e.g. after a batch. */
+int msg_get_forces(msg_ctx* ctx, double* force);
+
 /* Stream the context launches on (a hipStream_t), for callers that time kernels. */
 void* msg_stream(msg_ctx* ctx);
 

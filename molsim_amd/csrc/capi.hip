@@ -95,7 +95,7 @@ struct ReduceArgs {
 // Final FP64 reduction of all partial sums in a fixed order (deterministic).  It is the last
 // kernel of a UTotal pass, so it also hands the error flags to the output block and clears
 // the cell/brick counters and flags for the next pass.
-__global__ __launch_bounds__(1024) void k_reduce(ReduceArgs a) {
+__device__ __forceinline__ void reduce_body(const ReduceArgs& a) {
   constexpr int NV = kMaxTP + 1 + 2 + 3;
   __shared__ double red[16][NV];
   double v[NV];
@@ -177,6 +177,17 @@ __global__ __launch_bounds__(1024) void k_reduce(ReduceArgs a) {
   for (int i = threadIdx.x; i < a.nbrick; i += 1024) a.brick_count[i] = 0;
 }
 
+__global__ __launch_bounds__(1024) void k_reduce(ReduceArgs a) { reduce_body(a); }
+
+__global__ __launch_bounds__(1024) void k_reduce_seg(const ReduceArgs* __restrict__ args,
+                                                     const int2* __restrict__ map) {
+  reduce_body(args[map[blockIdx.x].x]);
+}
+
+void seg_reduce(const SegGroup& g, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_seg, dim3(g.nblk), dim3(1024), 0, s, (const ReduceArgs*)g.d_args, g.d_map);
+}
+
 }  // namespace
 
 int launch_zero(Ctx* c) {
@@ -189,7 +200,7 @@ int launch_zero(Ctx* c) {
   return MSG_OK;
 }
 
-int launch_reduce(Ctx* c) {
+static ReduceArgs reduce_args(Ctx* c) {
   ReduceArgs a;
   a.pair = c->d_part + c->part_off_pair;
   a.npair = c->npart_pair;
@@ -212,6 +223,17 @@ int launch_reduce(Ctx* c) {
   a.ncell = c->grid.ncell;
   a.brick_count = c->d_brick_count;
   a.nbrick = c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.nbrick : 0;
+  return a;
+}
+
+int plan_reduce(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out) {
+  SegBuilder<ReduceArgs> b;
+  for (Ctx* c : cs) b.add(0, seg_reduce, reduce_args(c), 1);
+  return b.finish(own, out);
+}
+
+int launch_reduce(Ctx* c) {
+  const ReduceArgs a = reduce_args(c);
   prof_begin(c, MSG_K_REDUCE);
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, c->stream, a);
   prof_end(c, MSG_K_REDUCE);
@@ -341,9 +363,10 @@ int msg_create(const msg_params* p, msg_ctx** out) {
   if (p->list_mode == MSG_LIST_VERLET && (e = verlet_setup(c))) return fail(e);
   // partial-sum layout
   c->npart_pair = pair_blocks(c);
+  c->npart_pair_cap = std::max(c->npart_pair, std::max((np + 3) / 4, 2048 + 1));
   c->npart_dip = (np + 255) / 256;
   c->part_off_pair = 0;
-  c->part_off_dip = c->part_off_pair + c->npart_pair * (kMaxTP + 1);
+  c->part_off_dip = c->part_off_pair + c->npart_pair_cap * (kMaxTP + 1);
   c->part_off_spme = c->part_off_dip + 3 * c->npart_dip;
   const int max_spme_blocks = 2 * 2 * 256 * 130;  // upper bound for nzblocks * 2 (K <= 256)
   c->part_off_ewstd = c->part_off_spme + max_spme_blocks;
@@ -355,7 +378,6 @@ int msg_create(const msg_params* p, msg_ctx** out) {
     return fail(set_error(MSG_ERR_NOMEM, "hipHostMalloc failed"));
   if (hipHostMalloc((void**)&c->h_flags, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess)
     return fail(set_error(MSG_ERR_NOMEM, "hipHostMalloc failed"));
-  for (auto& ev : c->ev) hipEventCreate(&ev);
   if ((e = launch_zero(c))) return fail(e);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_error(MSG_ERR_HIP, "sync failed"));
   *out = h;
@@ -387,6 +409,7 @@ void msg_destroy(msg_ctx* h) {
   if (c->side) hipStreamDestroy(c->side);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->ev_batch) hipEventDestroy(c->ev_batch);
   if (c->stream) hipStreamDestroy(c->stream);
   delete h;
 }
@@ -581,10 +604,36 @@ int msg_get_cell_list(msg_ctx* h, int* nc, int* cell_start, int* cell_ip) {
 
 void* msg_stream(msg_ctx* h) { return h ? (void*)h->c.stream : nullptr; }
 
+int msg_set_tile(msg_ctx* h, int pair_lanes) {
+  if (!h) return set_error(MSG_ERR_ARG, "msg_set_tile: null context");
+  if (pair_lanes != 0 && pair_lanes != 8 && pair_lanes != 16 && pair_lanes != 32 && pair_lanes != 64)
+    return set_error(MSG_ERR_ARG, "msg_set_tile: pair_lanes must be 0, 8, 16, 32 or 64");
+  Ctx* c = &h->c;
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  c->pair_lph = pair_lanes;
+  c->npart_pair = pair_blocks(c);
+  if (c->npart_pair > c->npart_pair_cap) return set_error(MSG_ERR_ARG, "msg_set_tile: tile exceeds reserve");
+  c->graph_ok = false;
+  return MSG_OK;
+}
+
+int msg_get_tile(msg_ctx* h) { return h ? h->c.pair_lph : MSG_ERR_ARG; }
+
+int msg_get_forces(msg_ctx* h, double* force) {
+  if (!h || !force) return set_error(MSG_ERR_ARG, "msg_get_forces: bad arguments");
+  Ctx* c = &h->c;
+  MSG_HIP(hipMemcpyAsync(force, c->d_force, 3 * c->np * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  return MSG_OK;
+}
+
 int msg_profile(msg_ctx* h, int enable) {
   if (!h) return set_error(MSG_ERR_ARG, "msg_profile: null context");
   Ctx* c = &h->c;
   prof_drain(c);
+  if (enable && !c->ev[0])  // timing events are created on first use
+    for (auto& ev : c->ev)
+      if (hipEventCreate(&ev) != hipSuccess) return set_error(MSG_ERR_HIP, "msg_profile: event creation failed");
   c->prof = enable != 0;
   for (int k = 0; k < MSG_K_COUNT; ++k) {
     c->prof_ms[k] = 0.0;

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <array>
+#include <map>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -86,6 +87,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;        // second branch of the UTotal graph (pair kernel)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_batch = nullptr;     // orders the context's own work before a batched pass
   int device = 0;
   long long bytes = 0;
 
@@ -160,7 +162,7 @@ struct Ctx {
 
   // profiling
   bool prof = false;
-  hipEvent_t ev[2 * MSG_K_COUNT * 64];
+  hipEvent_t ev[2 * MSG_K_COUNT * 64] = {};  // per-kernel timing events (created on first use)
   int ev_used = 0;
   std::vector<int> ev_kernel;
   double prof_ms[MSG_K_COUNT] = {0};
@@ -169,6 +171,9 @@ struct Ctx {
   // Verlet list (MSG_LIST_VERLET): full list at rcut + drnlist built from the cell list;
   // neighbours are particle indices, column-major [maxn][np]; home order = cell order at build
   int vl_maxn = 0;
+  std::vector<int> h_de_ip;  // staged trial indices (host)
+  int pair_lph = 32;        // pair tile: lanes per home particle (8..64), 0 = wave-item kernel
+  int npart_pair_cap = 0;   // pair partial slots reserved (largest tile)
   int* d_vl_home = nullptr;
   int* d_vl_n = nullptr;
   int* d_vl_nbr = nullptr;
@@ -210,7 +215,8 @@ int lists_setup(Ctx* c);                           // cell grid + buffers
 int launch_bin(Ctx* c);                            // K1 cell id + mesh bin + counts + dipole
 int launch_sort(Ctx* c);                           // scan + scatter + finalize
 int launch_pair(Ctx* c, bool last);                // K4
-int pair_blocks(Ctx* c);                           // workgroups of the pair kernel
+int pair_blocks(Ctx* c);
+bool pair_nbpar(const Ctx* c);                           // workgroups of the pair kernel
 int spme_setup(Ctx* c);                            // K-space arrays
 int launch_spme_mesh(Ctx* c);                      // spread, FFTs + influence function
 int launch_spme_gather(Ctx* c);                    // force gather (after the pair kernel)
@@ -221,6 +227,8 @@ int launch_zero(Ctx* c);                           // counters and flags
 int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ovl,
                   double* du_tot, double* du_rec, double* du_twob);
 int commit_move(Ctx* c, int ip, const double* rnew);
+int dutotal_launch(Ctx* c, int ntrial, const int* ip, const double* rotm);
+int dutotal_collect(Ctx* c, int ntrial, int* ovl, double* du_tot, double* du_rec, double* du_twob);
 int utotal_refresh(Ctx* c);                        // UTotal sequence + sync (for dE state)
 int utotal_sequence(Ctx* c);
 int verlet_setup(Ctx* c);
@@ -231,6 +239,144 @@ int launch_disp(Ctx* c);
 // profiling wrappers around a launch
 void prof_begin(Ctx* c, int kid);
 void prof_end(Ctx* c, int kid);
+
+// ------------------------------------------------------------------------------------------
+// Segmented launches: one grid holds the workgroups of many systems.  map[b] = (system,
+// workgroup within that system) and args[system] is that system's argument block, so a
+// kernel body written for one system (argument block + workgroup index) serves both the
+// single-system launch and the batched one.
+struct SegGroup;
+typedef void (*SegLaunchFn)(const SegGroup&, hipStream_t);
+struct SegGroup {
+  int nblk = 0;                  // workgroups of all systems
+  size_t lds = 0;                // dynamic LDS bytes (the largest of the systems)
+  const void* d_args = nullptr;  // per-system argument blocks
+  const int2* d_map = nullptr;   // workgroup -> (system, local workgroup)
+  SegLaunchFn launch = nullptr;
+};
+
+// Device memory of launch plans.  Default: each array is its own allocation, copied at
+// once (plans built once).  Arena mode (plans built per launch): arrays are packed into a
+// pinned host staging buffer and land in a device arena by one asynchronous copy (flush);
+// the caller resets the arena only after the launches that read it have completed.
+struct SegOwner {
+  std::vector<void*> ptrs;
+  char* arena = nullptr;   // device
+  char* stage = nullptr;   // pinned host
+  size_t cap = 0, used = 0;
+  ~SegOwner() {
+    for (void* p : ptrs) hipFree(p);
+    if (arena) hipFree(arena);
+    if (stage) hipHostFree(stage);
+  }
+  int init_arena(size_t bytes) {
+    MSG_HIP(hipMalloc((void**)&arena, bytes));
+    MSG_HIP(hipHostMalloc((void**)&stage, bytes, hipHostMallocDefault));
+    cap = bytes;
+    used = 0;
+    return MSG_OK;
+  }
+  void reset() { used = 0; }
+  // Grows the arena to at least `bytes` (only while nothing in flight reads it).
+  int reserve(size_t bytes) {
+    if (arena && bytes <= cap) return MSG_OK;
+    if (arena) hipFree(arena);
+    if (stage) hipHostFree(stage);
+    arena = nullptr;
+    stage = nullptr;
+    return init_arena(bytes);
+  }
+  template <class T>
+  int upload(const std::vector<T>& v, const T** out) {
+    const size_t b = v.size() * sizeof(T);
+    if (arena) {
+      const size_t nb = (b + 255) & ~size_t(255);
+      if (used + nb > cap) return set_error(MSG_ERR_NOMEM, "launch plan exceeds the unit's argument arena");
+      std::copy((const char*)v.data(), (const char*)v.data() + b, stage + used);
+      *out = (const T*)(arena + used);
+      used += nb;
+      return MSG_OK;
+    }
+    void* p = nullptr;
+    MSG_HIP(hipMalloc(&p, std::max<size_t>(1, b)));
+    ptrs.push_back(p);
+    MSG_HIP(hipMemcpy(p, v.data(), b, hipMemcpyHostToDevice));
+    *out = (const T*)p;
+    return MSG_OK;
+  }
+  int flush(hipStream_t s) {
+    if (arena && used) MSG_HIP(hipMemcpyAsync(arena, stage, used, hipMemcpyHostToDevice, s));
+    return MSG_OK;
+  }
+};
+
+// Collects the argument blocks of one stage per template variant, then uploads them.
+template <class A>
+struct SegBuilder {
+  struct G {
+    std::vector<A> args;
+    std::vector<int2> map;
+    size_t lds = 0;
+    SegLaunchFn fn = nullptr;
+  };
+  std::map<int, G> groups;
+  void add(int variant, SegLaunchFn fn, const A& a, int nblk, size_t lds = 0) {
+    if (nblk <= 0) return;
+    G& g = groups[variant];
+    g.fn = fn;
+    g.lds = std::max(g.lds, lds);
+    const int sys = (int)g.args.size();
+    g.args.push_back(a);
+    for (int b = 0; b < nblk; ++b) g.map.push_back(make_int2(sys, b));
+  }
+  int finish(SegOwner& own, std::vector<SegGroup>& out) {
+    for (auto& kv : groups) {
+      SegGroup sg;
+      sg.nblk = (int)kv.second.map.size();
+      sg.lds = kv.second.lds;
+      sg.launch = kv.second.fn;
+      const A* da = nullptr;
+      if (int e = own.upload(kv.second.args, &da)) return e;
+      if (int e = own.upload(kv.second.map, &sg.d_map)) return e;
+      sg.d_args = da;
+      out.push_back(sg);
+    }
+    return MSG_OK;
+  }
+};
+
+// One stage of the batched UTotal sequence over many systems (each .hip file).
+typedef std::vector<SegGroup> SegStage;
+int plan_bin(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out);
+int plan_sort(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& scan, SegStage& scatter,
+              SegStage& finalize);
+int plan_pair(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out);
+int plan_spme(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& spread, SegStage* fft,
+              SegStage& gather);   // fft[5]: x, y, zconv, yi, xi
+int plan_reduce(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out);
+
+// A batched UTotal pass over many systems: its stages and the collection of every
+// system's energies (64 doubles each) into d_outs.
+struct Pass {
+  std::vector<Ctx*> cs;
+  SegStage bin, scan, scatter, finalize, pair, spread, fft[5], gather, reduce;
+  bool any_spme = false;
+  const double* const* d_outp = nullptr;
+  double* d_outs = nullptr;
+};
+int build_pass(const std::vector<Ctx*>& cs, SegOwner& own, Pass& p);
+int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join);
+int check_batchable(Ctx* c);
+
+// Batched MC trials: systems grouped by spline order; d_args holds one block per member.
+struct DeGroup {
+  int order = 0;
+  std::vector<int> sys;          // batch indices of the members
+  const void* d_args = nullptr;
+};
+int plan_de(const std::vector<Ctx*>& cs, SegOwner& own, std::vector<DeGroup>& out);
+int launch_de_seg(const DeGroup& g, const int* d_tstart, const int* d_ip, const double* d_r,
+                  double* d_out, int ntot, hipStream_t s);
 
 }  // namespace msg
 

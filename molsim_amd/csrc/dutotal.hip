@@ -94,17 +94,16 @@ __device__ inline int fold(int d, int K) {
   return d <= K / 2 ? d : K - d;
 }
 
+// One trial: particle ip (0-based) to position trn[0..2]; results to o[0 .. ntp+3].
 template <int P>
-__global__ __launch_bounds__(256) void k_dutotal(DeArgs a) {
+__device__ __forceinline__ void de_body(const DeArgs& a, int ip, const double* trn, double* o) {
   constexpr int W = 2 * P - 1, W3 = W * W * W;
   __shared__ double sh[8];
   __shared__ double thn[3][P], tho[3][P];
   __shared__ int mmn[3], mmo[3];
   __shared__ double w0[W3], w1[W3];
   __shared__ int s_ovl;
-  const int t = blockIdx.x;
-  const int ip = a.tip[t];
-  const double rn[3] = {a.tr[3 * t], a.tr[3 * t + 1], a.tr[3 * t + 2]};
+  const double rn[3] = {trn[0], trn[1], trn[2]};
   const double ro[3] = {a.ro[3 * ip], a.ro[3 * ip + 1], a.ro[3 * ip + 2]};
   const double q = a.az[ip];
   const int ti = a.ipt[ip];
@@ -248,7 +247,6 @@ __global__ __launch_bounds__(256) void k_dutotal(DeArgs a) {
     out_tp[tp] = msgd::block_sum(v, sh);
   }
   if (threadIdx.x == 0) {
-    double* o = a.out + (size_t)t * (a.ntp + 4);
     double tw = 0.0;
     for (int tp = 0; tp < a.ntp; ++tp) {
       o[1 + tp] = out_tp[tp];
@@ -270,6 +268,31 @@ __global__ __launch_bounds__(256) void k_dutotal(DeArgs a) {
     o[a.ntp + 2] = tw + du_rec;
     o[a.ntp + 3] = (double)s_ovl;
   }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_dutotal(DeArgs a) {
+  const int t = blockIdx.x;
+  de_body<P>(a, a.tip[t], a.tr + 3 * t, a.out + (size_t)t * (a.ntp + 4));
+}
+
+// Trials of many systems in one grid: workgroup b is trial b of the concatenation, system
+// s = #{k >= 1 : tstart[k] <= b}; outputs are strided by kMaxTP + 4.
+template <int P>
+__global__ __launch_bounds__(256) void k_dutotal_seg(const DeArgs* __restrict__ args,
+                                                     const int* __restrict__ tstart, int nsys,
+                                                     const int* __restrict__ tip,
+                                                     const double* __restrict__ tr,
+                                                     double* __restrict__ out) {
+  const int b = blockIdx.x;
+  int cnt = 0;
+  for (int k = 1 + threadIdx.x; k < nsys; k += 256) cnt += tstart[k] <= b;
+  __shared__ int s_sys;
+  if (threadIdx.x == 0) s_sys = 0;
+  __syncthreads();
+  if (cnt) atomicAdd(&s_sys, cnt);
+  __syncthreads();
+  de_body<P>(args[s_sys], tip[b], tr + 3 * b, out + (size_t)b * (kMaxTP + 4));
 }
 
 // g(x) = sum_k meshfac(|k|) cos(2 pi k.x / K) over the full spectrum, folded to [0, K/2]^3
@@ -353,8 +376,9 @@ int make_gk(Ctx* c) {
 
 }  // namespace
 
-int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ovl, double* du_tot,
-                  double* du_rec, double* du_twob) {
+static DeArgs de_args(Ctx* c);
+
+int dutotal_launch(Ctx* c, int ntrial, const int* ip, const double* rotm) {
   for (int t = 0; t < ntrial; ++t)
     if (ip[t] < 1 || ip[t] > c->np) return set_error(MSG_ERR_ARG, "DUTotal: particle index out of range");
   if (c->P.ewald_mode == MSG_EWALD_SPM && !c->have_gk)
@@ -363,6 +387,7 @@ int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ov
     if (int e = utotal_refresh(c)) return e;
   const int ntp = c->nptpt;
   if (ntrial > c->de_cap) {
+    MSG_HIP(hipStreamSynchronize(c->stream));
     if (c->d_de) { hipFree(c->d_de); hipFree(c->d_de_ip); hipFree(c->d_de_r); }
     c->d_de = nullptr;
     const int cap = std::max(ntrial, 256);
@@ -371,10 +396,33 @@ int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ov
     if (int e = dalloc(c, (void**)&c->d_de_r, (size_t)cap * 3 * sizeof(double))) return e;
     c->de_cap = cap;
   }
-  std::vector<int> ip0(ntrial);
+  // 0-based indices staged in the context: the copy may still be pending when this returns
+  std::vector<int>& ip0 = c->h_de_ip;
+  ip0.resize(ntrial);
   for (int t = 0; t < ntrial; ++t) ip0[t] = ip[t] - 1;
   MSG_HIP(hipMemcpyAsync(c->d_de_ip, ip0.data(), ntrial * sizeof(int), hipMemcpyHostToDevice, c->stream));
   MSG_HIP(hipMemcpyAsync(c->d_de_r, rotm, ntrial * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  DeArgs a = de_args(c);
+  a.tip = c->d_de_ip;
+  a.tr = c->d_de_r;
+  a.out = c->d_de;
+  prof_begin(c, MSG_K_DE);
+  switch (c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.order : 4) {
+    case 3: launch_de<3>(c, a, ntrial); break;
+    case 4: launch_de<4>(c, a, ntrial); break;
+    case 5: launch_de<5>(c, a, ntrial); break;
+    case 6: launch_de<6>(c, a, ntrial); break;
+    case 7: launch_de<7>(c, a, ntrial); break;
+    case 8: launch_de<8>(c, a, ntrial); break;
+    default: return set_error(MSG_ERR_ARG, "DUTotal: SPME order must be 3..8");
+  }
+  prof_end(c, MSG_K_DE);
+  MSG_HIP(hipGetLastError());
+  return MSG_OK;
+}
+
+static DeArgs de_args(Ctx* c) {
+  const int ntp = c->nptpt;
   DeArgs a;
   a.np = c->np;
   a.ntp = ntp;
@@ -414,21 +462,55 @@ int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ov
   a.sk = c->d_sk;
   for (int d = 0; d < 3; ++d) a.tpb[d] = 2.0 * M_PI / c->P.boxlen[d];
   a.epsi4pi = c->P.epsi4pi;
-  a.tip = c->d_de_ip;
-  a.tr = c->d_de_r;
-  a.out = c->d_de;
-  prof_begin(c, MSG_K_DE);
-  switch (c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.order : 4) {
-    case 3: launch_de<3>(c, a, ntrial); break;
-    case 4: launch_de<4>(c, a, ntrial); break;
-    case 5: launch_de<5>(c, a, ntrial); break;
-    case 6: launch_de<6>(c, a, ntrial); break;
-    case 7: launch_de<7>(c, a, ntrial); break;
-    case 8: launch_de<8>(c, a, ntrial); break;
+  a.tip = nullptr;
+  a.tr = nullptr;
+  a.out = nullptr;
+  return a;
+}
+
+// Batched trials: systems grouped by B-spline order (the kernel's template), one argument
+// block per system uploaded once.
+int plan_de(const std::vector<Ctx*>& cs, SegOwner& own, std::vector<DeGroup>& out) {
+  std::map<int, std::vector<int>> by_order;
+  for (int s = 0; s < (int)cs.size(); ++s) {
+    Ctx* c = cs[s];
+    if (c->P.ewald_mode == MSG_EWALD_SPM && !c->have_gk)
+      if (int e = make_gk(c)) return e;
+    by_order[c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.order : 4].push_back(s);
+  }
+  for (auto& kv : by_order) {
+    if (kv.first < 3 || kv.first > 8) return set_error(MSG_ERR_ARG, "DUTotal: SPME order must be 3..8");
+    std::vector<DeArgs> args;
+    for (int s : kv.second) args.push_back(de_args(cs[s]));
+    DeGroup g;
+    g.order = kv.first;
+    g.sys = kv.second;
+    const DeArgs* d = nullptr;
+    if (int e = own.upload(args, &d)) return e;
+    g.d_args = d;
+    out.push_back(g);
+  }
+  return MSG_OK;
+}
+
+int launch_de_seg(const DeGroup& g, const int* d_tstart, const int* d_ip, const double* d_r,
+                  double* d_out, int ntot, hipStream_t s) {
+  if (ntot <= 0) return MSG_OK;
+  const DeArgs* a = (const DeArgs*)g.d_args;
+  const int n = (int)g.sys.size();
+  switch (g.order) {
+#define MSG_DE_SEG(P) \
+  case P: hipLaunchKernelGGL(k_dutotal_seg<P>, dim3(ntot), dim3(256), 0, s, a, d_tstart, n, d_ip, d_r, d_out); break;
+    MSG_DE_SEG(3) MSG_DE_SEG(4) MSG_DE_SEG(5) MSG_DE_SEG(6) MSG_DE_SEG(7) MSG_DE_SEG(8)
+#undef MSG_DE_SEG
     default: return set_error(MSG_ERR_ARG, "DUTotal: SPME order must be 3..8");
   }
-  prof_end(c, MSG_K_DE);
   MSG_HIP(hipGetLastError());
+  return MSG_OK;
+}
+
+int dutotal_collect(Ctx* c, int ntrial, int* ovl, double* du_tot, double* du_rec, double* du_twob) {
+  const int ntp = c->nptpt;
   std::vector<double> res((size_t)ntrial * (ntp + 4));
   MSG_HIP(hipMemcpyAsync(res.data(), c->d_de, res.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   MSG_HIP(hipStreamSynchronize(c->stream));
@@ -441,6 +523,12 @@ int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ov
       for (int k = 0; k <= ntp; ++k) du_twob[(size_t)t * (ntp + 1) + k] = o[k];
   }
   return MSG_OK;
+}
+
+int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ovl, double* du_tot,
+                  double* du_rec, double* du_twob) {
+  if (int e = dutotal_launch(c, ntrial, ip, rotm)) return e;
+  return dutotal_collect(c, ntrial, ovl, du_tot, du_rec, du_twob);
 }
 
 // MCUpdate for one accepted single-particle move: the position changes on the device and

@@ -19,44 +19,57 @@ namespace msg {
 
 namespace {
 
+struct BinArgs {
+  int np;
+  const double* ro;
+  const double* az;
+  Box box;
+  CellGrid g;
+  int use_mesh;
+  MeshGrid m;
+  int* cellid;
+  int* cell_count;
+  int* mm;
+  void* frac;       // real[3*np]
+  int* brick_count;
+  double* dip_part;
+  int* flags;
+};
+
 template <typename real>
-__global__ __launch_bounds__(256) void k_bin(int np, const double* __restrict__ ro,
-                                             const double* __restrict__ az, Box box, CellGrid g,
-                                             int use_mesh, MeshGrid m, int* __restrict__ cellid,
-                                             int* __restrict__ cell_count, int* __restrict__ mm,
-                                             real* __restrict__ frac, int* __restrict__ brick_count,
-                                             double* __restrict__ dip_part, int* __restrict__ flags) {
+__device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
   __shared__ double sh[8];
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int i = bid * 256 + threadIdx.x;
   double dx = 0.0, dy = 0.0, dz = 0.0;
-  if (i < np) {
-    const double r[3] = {ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]};
-    const double q = az[i];
+  if (i < a.np) {
+    const double r[3] = {a.ro[3 * i], a.ro[3 * i + 1], a.ro[3 * i + 2]};
+    const double q = a.az[i];
     int ci[3];
     bool bad = false;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      int k = (int)floor((r[d] + box.half[d]) * g.csi[d]);
-      if (k < 0 || k > g.nc[d]) bad = true;
-      ci[d] = min(max(k, 0), g.nc[d] - 1);
+      int k = (int)floor((r[d] + a.box.half[d]) * a.g.csi[d]);
+      if (k < 0 || k > a.g.nc[d]) bad = true;
+      ci[d] = min(max(k, 0), a.g.nc[d] - 1);
     }
-    if (bad) atomicOr(flags, FLAG_CELL);
-    const int c = ci[0] + g.nc[0] * (ci[1] + g.nc[1] * ci[2]);
-    cellid[i] = c;
-    atomicAdd(&cell_count[c], 1);
-    if (use_mesh) {
+    if (bad) atomicOr(a.flags, FLAG_CELL);
+    const int c = ci[0] + a.g.nc[0] * (ci[1] + a.g.nc[1] * ci[2]);
+    a.cellid[i] = c;
+    atomicAdd(&a.cell_count[c], 1);
+    if (a.use_mesh) {
+      real* frac = (real*)a.frac;
       int mi[3];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const double p = m.dkdr[d] * r[d];  // UEwaldRecSPM: dkdr * r, Src/energy.F90:1230
+        const double p = a.m.dkdr[d] * r[d];  // UEwaldRecSPM: dkdr * r, Src/energy.F90:1230
         const double v = floor(p);
         frac[3 * i + d] = (real)(p - v);
-        int k = (int)v % m.K;
-        mi[d] = k < 0 ? k + m.K : k;
+        int k = (int)v % a.m.K;
+        mi[d] = k < 0 ? k + a.m.K : k;
       }
-      mm[i] = mi[0] | (mi[1] << 10) | (mi[2] << 20);
-      const int b = mi[0] / kBX + m.nb[0] * (mi[1] / kBY + m.nb[1] * (mi[2] / kBZ));
-      atomicAdd(&brick_count[b], 1);
+      a.mm[i] = mi[0] | (mi[1] << 10) | (mi[2] << 20);
+      const int b = mi[0] / kBX + a.m.nb[0] * (mi[1] / kBY + a.m.nb[1] * (mi[2] / kBZ));
+      atomicAdd(&a.brick_count[b], 1);
     }
     dx = q * r[0];
     dy = q * r[1];
@@ -66,10 +79,20 @@ __global__ __launch_bounds__(256) void k_bin(int np, const double* __restrict__ 
   dy = msgd::block_sum(dy, sh);
   dz = msgd::block_sum(dz, sh);
   if (threadIdx.x == 0) {
-    dip_part[3 * blockIdx.x + 0] = dx;
-    dip_part[3 * blockIdx.x + 1] = dy;
-    dip_part[3 * blockIdx.x + 2] = dz;
+    a.dip_part[3 * bid + 0] = dx;
+    a.dip_part[3 * bid + 1] = dy;
+    a.dip_part[3 * bid + 2] = dz;
   }
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_bin(BinArgs a) { bin_body<real>(a, blockIdx.x); }
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_bin_seg(const BinArgs* __restrict__ args,
+                                                 const int2* __restrict__ map) {
+  const int2 s = map[blockIdx.x];
+  bin_body<real>(args[s.x], s.y);
 }
 
 // Exclusive scan over n values of one 1024-thread workgroup; out[n] receives the total.
@@ -115,20 +138,32 @@ __device__ void block_scan(int n, const int* __restrict__ cnt, int div, int* out
   __syncthreads();
 }
 
+struct ScanArgs {
+  int n0;                 // cells
+  const int* cnt0;
+  int* start0;
+  int* fill0;
+  int hb;                 // homes per pair-kernel chunk
+  int* chunk0;
+  int n1;                 // bricks (0 without SPME)
+  const int* cnt1;
+  int* start1;
+  int* fill1;
+  const double* dip_part;
+  int ndip;
+  double* M;
+};
+
 // Workgroup 0: cell offsets and the per-cell home-chunk offsets of the pair kernel;
-// workgroup 1: brick offsets.  Both clear their fill counters.
-__global__ __launch_bounds__(1024) void k_scan(int n0, const int* __restrict__ cnt0, int* start0,
-                                              int* fill0, int hb, int* chunk0, int n1,
-                                              const int* __restrict__ cnt1, int* start1,
-                                              int* fill1, const double* __restrict__ dip_part,
-                                              int ndip, double* __restrict__ M) {
+// workgroup 1: brick offsets.  Both clear their fill counters.  Workgroup 2: total dipole.
+__device__ __forceinline__ void scan_body(const ScanArgs& a, int bid) {
   __shared__ int wsum[16];
   __shared__ double dsum[16][3];
-  if (blockIdx.x == 2) {  // total dipole sum q r (surface term), from k_bin's partials
+  if (bid == 2) {  // total dipole sum q r (surface term), from k_bin's partials
     double m[3] = {0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < ndip; b += 1024)
+    for (int b = threadIdx.x; b < a.ndip; b += 1024)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) m[d] += dip_part[3 * b + d];
+      for (int d = 0; d < 3; ++d) m[d] += a.dip_part[3 * b + d];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const double t = msgd::wave_sum(m[d]);
@@ -139,34 +174,59 @@ __global__ __launch_bounds__(1024) void k_scan(int n0, const int* __restrict__ c
       double t = 0.0;
 #pragma unroll
       for (int w = 0; w < 16; ++w) t += dsum[w][threadIdx.x];
-      M[threadIdx.x] = t;
+      a.M[threadIdx.x] = t;
     }
     return;
   }
-  const int n = blockIdx.x == 0 ? n0 : n1;
-  const int* cnt = blockIdx.x == 0 ? cnt0 : cnt1;
-  int* fill = blockIdx.x == 0 ? fill0 : fill1;
+  const int n = bid == 0 ? a.n0 : a.n1;
+  const int* cnt = bid == 0 ? a.cnt0 : a.cnt1;
+  int* fill = bid == 0 ? a.fill0 : a.fill1;
   if (n <= 0) return;
-  block_scan(n, cnt, 1, blockIdx.x == 0 ? start0 : start1, wsum);
-  if (blockIdx.x == 0) block_scan(n, cnt, hb, chunk0, wsum);
+  block_scan(n, cnt, 1, bid == 0 ? a.start0 : a.start1, wsum);
+  if (bid == 0) block_scan(n, cnt, a.hb, a.chunk0, wsum);
   for (int i = threadIdx.x; i < n; i += 1024) fill[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_scatter(int np, const int* __restrict__ cellid,
-                                                 const int* __restrict__ cell_start, int* cell_fill,
-                                                 int* __restrict__ cell_tmp, int use_mesh,
-                                                 const int* __restrict__ mm, MeshGrid m,
-                                                 const int* __restrict__ brick_start,
-                                                 int* brick_fill, int* __restrict__ brick_tmp) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= np) return;
-  const int c = cellid[i];
-  cell_tmp[cell_start[c] + atomicAdd(&cell_fill[c], 1)] = i;
-  if (use_mesh) {
-    const int p = mm[i];
-    const int b = (p & 1023) / kBX + m.nb[0] * (((p >> 10) & 1023) / kBY + m.nb[1] * ((p >> 20) / kBZ));
-    brick_tmp[brick_start[b] + atomicAdd(&brick_fill[b], 1)] = i;
+__global__ __launch_bounds__(1024) void k_scan(ScanArgs a) { scan_body(a, blockIdx.x); }
+
+__global__ __launch_bounds__(1024) void k_scan_seg(const ScanArgs* __restrict__ args,
+                                                   const int2* __restrict__ map) {
+  const int2 s = map[blockIdx.x];
+  scan_body(args[s.x], s.y);
+}
+
+struct ScatterArgs {
+  int np;
+  const int* cellid;
+  const int* cell_start;
+  int* cell_fill;
+  int* cell_tmp;
+  int use_mesh;
+  const int* mm;
+  MeshGrid m;
+  const int* brick_start;
+  int* brick_fill;
+  int* brick_tmp;
+};
+
+__device__ __forceinline__ void scatter_body(const ScatterArgs& a, int bid) {
+  const int i = bid * 256 + threadIdx.x;
+  if (i >= a.np) return;
+  const int c = a.cellid[i];
+  a.cell_tmp[a.cell_start[c] + atomicAdd(&a.cell_fill[c], 1)] = i;
+  if (a.use_mesh) {
+    const int p = a.mm[i];
+    const int b = (p & 1023) / kBX + a.m.nb[0] * (((p >> 10) & 1023) / kBY + a.m.nb[1] * ((p >> 20) / kBZ));
+    a.brick_tmp[a.brick_start[b] + atomicAdd(&a.brick_fill[b], 1)] = i;
   }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) { scatter_body(a, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void k_scatter_seg(const ScatterArgs* __restrict__ args,
+                                                     const int2* __restrict__ map) {
+  const int2 s = map[blockIdx.x];
+  scatter_body(args[s.x], s.y);
 }
 
 struct FinalizeArgs {
@@ -195,8 +255,8 @@ struct FinalizeArgs {
 // One wavefront per cell (then per brick): rank members by particle index and write the
 // ordered arrays; cells also list their pair-kernel work items.
 template <typename real>
-__global__ __launch_bounds__(256) void k_finalize(FinalizeArgs a) {
-  const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+__device__ __forceinline__ void finalize_body(const FinalizeArgs& a, int bid) {
+  const int wv = bid * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (wv < a.ncell) {
     const int s0 = a.cell_start[wv], n = a.cell_start[wv + 1] - s0;
@@ -244,6 +304,16 @@ __global__ __launch_bounds__(256) void k_finalize(FinalizeArgs a) {
       a.bmm[dst] = a.mm[ip];
     }
   }
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_finalize(FinalizeArgs a) { finalize_body<real>(a, blockIdx.x); }
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_finalize_seg(const FinalizeArgs* __restrict__ args,
+                                                      const int2* __restrict__ map) {
+  const int2 s = map[blockIdx.x];
+  finalize_body<real>(args[s.x], s.y);
 }
 
 }  // namespace
@@ -317,41 +387,65 @@ static Box make_box(const msg_params& P) {
   return b;
 }
 
-int launch_bin(Ctx* c) {
-  const int np = c->np, nb = (np + 255) / 256;
-  const bool mesh = c->P.ewald_mode == MSG_EWALD_SPM;
-  prof_begin(c, MSG_K_BIN);
-  if (c->fp64)
-    hipLaunchKernelGGL(k_bin<double>, dim3(nb), dim3(256), 0, c->stream, np, c->d_ro, c->d_az,
-                       make_box(c->P), c->grid, (int)mesh, c->mesh, c->d_cellid, c->d_cell_count,
-                       c->d_mm, (double*)c->d_frac, c->d_brick_count,
-                       c->d_part + c->part_off_dip, c->d_flags);
-  else
-    hipLaunchKernelGGL(k_bin<float>, dim3(nb), dim3(256), 0, c->stream, np, c->d_ro, c->d_az,
-                       make_box(c->P), c->grid, (int)mesh, c->mesh, c->d_cellid, c->d_cell_count,
-                       c->d_mm, (float*)c->d_frac, c->d_brick_count,
-                       c->d_part + c->part_off_dip, c->d_flags);
-  prof_end(c, MSG_K_BIN);
-  MSG_HIP(hipGetLastError());
-  return MSG_OK;
+static BinArgs bin_args(Ctx* c) {
+  BinArgs a;
+  a.np = c->np;
+  a.ro = c->d_ro;
+  a.az = c->d_az;
+  a.box = make_box(c->P);
+  a.g = c->grid;
+  a.use_mesh = c->P.ewald_mode == MSG_EWALD_SPM;
+  a.m = c->mesh;
+  a.cellid = c->d_cellid;
+  a.cell_count = c->d_cell_count;
+  a.mm = c->d_mm;
+  a.frac = c->d_frac;
+  a.brick_count = c->d_brick_count;
+  a.dip_part = c->d_part + c->part_off_dip;
+  a.flags = c->d_flags;
+  return a;
 }
 
-int launch_sort(Ctx* c) {
-  const int np = c->np;
+static ScanArgs scan_args(Ctx* c) {
   const bool mesh = c->P.ewald_mode == MSG_EWALD_SPM;
-  const int nbrick = mesh ? c->mesh.nbrick : 0;
-  prof_begin(c, MSG_K_CELLSORT);
-  hipLaunchKernelGGL(k_scan, dim3(3), dim3(1024), 0, c->stream, c->grid.ncell, c->d_cell_count,
-                     c->d_cell_start, c->d_cell_fill, kPairHB, c->d_cell_chunk, nbrick,
-                     c->d_brick_count, c->d_brick_start, c->d_brick_fill,
-                     c->d_part + c->part_off_dip, c->npart_dip, c->d_M);
-  hipLaunchKernelGGL(k_scatter, dim3((np + 255) / 256), dim3(256), 0, c->stream, np, c->d_cellid,
-                     c->d_cell_start, c->d_cell_fill, c->d_cell_tmp, (int)mesh, c->d_mm, c->mesh,
-                     c->d_brick_start, c->d_brick_fill, c->d_brick_tmp);
-  const int waves = c->grid.ncell + nbrick;
+  ScanArgs a;
+  a.n0 = c->grid.ncell;
+  a.cnt0 = c->d_cell_count;
+  a.start0 = c->d_cell_start;
+  a.fill0 = c->d_cell_fill;
+  a.hb = kPairHB;
+  a.chunk0 = c->d_cell_chunk;
+  a.n1 = mesh ? c->mesh.nbrick : 0;
+  a.cnt1 = c->d_brick_count;
+  a.start1 = c->d_brick_start;
+  a.fill1 = c->d_brick_fill;
+  a.dip_part = c->d_part + c->part_off_dip;
+  a.ndip = c->npart_dip;
+  a.M = c->d_M;
+  return a;
+}
+
+static ScatterArgs scatter_args(Ctx* c) {
+  ScatterArgs a;
+  a.np = c->np;
+  a.cellid = c->d_cellid;
+  a.cell_start = c->d_cell_start;
+  a.cell_fill = c->d_cell_fill;
+  a.cell_tmp = c->d_cell_tmp;
+  a.use_mesh = c->P.ewald_mode == MSG_EWALD_SPM;
+  a.mm = c->d_mm;
+  a.m = c->mesh;
+  a.brick_start = c->d_brick_start;
+  a.brick_fill = c->d_brick_fill;
+  a.brick_tmp = c->d_brick_tmp;
+  return a;
+}
+
+static FinalizeArgs finalize_args(Ctx* c) {
+  const bool mesh = c->P.ewald_mode == MSG_EWALD_SPM;
   FinalizeArgs f;
   f.ncell = c->grid.ncell;
-  f.nbrick = nbrick;
+  f.nbrick = mesh ? c->mesh.nbrick : 0;
   f.cell_start = c->d_cell_start;
   f.cell_tmp = c->d_cell_tmp;
   f.cell_chunk = c->d_cell_chunk;
@@ -371,13 +465,76 @@ int launch_sort(Ctx* c) {
   f.frac = c->d_frac;
   f.bq = c->d_bq;
   f.bmm = c->d_bmm;
+  return f;
+}
+
+static int finalize_blocks(const FinalizeArgs& f) { return (f.ncell + f.nbrick + 3) / 4; }
+
+int launch_bin(Ctx* c) {
+  const int nb = (c->np + 255) / 256;
+  const BinArgs a = bin_args(c);
+  prof_begin(c, MSG_K_BIN);
   if (c->fp64)
-    hipLaunchKernelGGL(k_finalize<double>, dim3((waves + 3) / 4), dim3(256), 0, c->stream, f);
+    hipLaunchKernelGGL(k_bin<double>, dim3(nb), dim3(256), 0, c->stream, a);
   else
-    hipLaunchKernelGGL(k_finalize<float>, dim3((waves + 3) / 4), dim3(256), 0, c->stream, f);
+    hipLaunchKernelGGL(k_bin<float>, dim3(nb), dim3(256), 0, c->stream, a);
+  prof_end(c, MSG_K_BIN);
+  MSG_HIP(hipGetLastError());
+  return MSG_OK;
+}
+
+int launch_sort(Ctx* c) {
+  const int np = c->np;
+  prof_begin(c, MSG_K_CELLSORT);
+  hipLaunchKernelGGL(k_scan, dim3(3), dim3(1024), 0, c->stream, scan_args(c));
+  hipLaunchKernelGGL(k_scatter, dim3((np + 255) / 256), dim3(256), 0, c->stream, scatter_args(c));
+  const FinalizeArgs f = finalize_args(c);
+  if (c->fp64)
+    hipLaunchKernelGGL(k_finalize<double>, dim3(finalize_blocks(f)), dim3(256), 0, c->stream, f);
+  else
+    hipLaunchKernelGGL(k_finalize<float>, dim3(finalize_blocks(f)), dim3(256), 0, c->stream, f);
   prof_end(c, MSG_K_CELLSORT);
   MSG_HIP(hipGetLastError());
   return MSG_OK;
+}
+
+// ---- batched stages ------------------------------------------------------------------------
+template <typename real>
+static void seg_bin(const SegGroup& g, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_seg<real>, dim3(g.nblk), dim3(256), 0, s, (const BinArgs*)g.d_args, g.d_map);
+}
+static void seg_scan(const SegGroup& g, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_seg, dim3(g.nblk), dim3(1024), 0, s, (const ScanArgs*)g.d_args, g.d_map);
+}
+static void seg_scatter(const SegGroup& g, hipStream_t s) {
+  hipLaunchKernelGGL(k_scatter_seg, dim3(g.nblk), dim3(256), 0, s, (const ScatterArgs*)g.d_args, g.d_map);
+}
+template <typename real>
+static void seg_finalize(const SegGroup& g, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize_seg<real>, dim3(g.nblk), dim3(256), 0, s, (const FinalizeArgs*)g.d_args, g.d_map);
+}
+
+int plan_bin(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out) {
+  SegBuilder<BinArgs> b;
+  for (Ctx* c : cs)
+    b.add(c->fp64, c->fp64 ? seg_bin<double> : seg_bin<float>, bin_args(c), (c->np + 255) / 256);
+  return b.finish(own, out);
+}
+
+int plan_sort(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& scan, SegStage& scatter,
+              SegStage& finalize) {
+  SegBuilder<ScanArgs> bs;
+  SegBuilder<ScatterArgs> bc;
+  SegBuilder<FinalizeArgs> bf;
+  for (Ctx* c : cs) {
+    bs.add(0, seg_scan, scan_args(c), 3);
+    bc.add(0, seg_scatter, scatter_args(c), (c->np + 255) / 256);
+    const FinalizeArgs f = finalize_args(c);
+    bf.add(c->fp64, c->fp64 ? seg_finalize<double> : seg_finalize<float>, f, finalize_blocks(f));
+  }
+  if (int e = bs.finish(own, scan)) return e;
+  if (int e = bc.finish(own, scatter)) return e;
+  return bf.finish(own, finalize);
 }
 
 }  // namespace msg

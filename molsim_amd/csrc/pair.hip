@@ -38,6 +38,7 @@ struct PairArgs {
   double* part;       // per block: ntp energies + virial
   int* flags;
   double scale;       // applied to the force when this kernel is the last writer
+  int nblk;           // workgroups of this system (stride of the work-item loop)
 };
 
 // Wave-local LDS hand-off (all lanes of one wavefront, no workgroup barrier).
@@ -54,7 +55,7 @@ constexpr int kWaves = 4;  // wavefronts per workgroup; each works through its o
 // candidate against all homes of the chunk; home forces are wave-reduced at the end.
 // Energies (per type pair) and the virial accumulate per lane over all items.
 template <typename real, int NTP>
-__global__ __launch_bounds__(256) void k_pair(PairArgs a) {
+__device__ __forceinline__ void pair_items_body(const PairArgs& a, int bid) {
   extern __shared__ double s_dyn[];  // table records, then the bin index (sized at launch)
   __shared__ int s_nbs[kWaves][kMaxStencil + 1], s_pref[kWaves][kMaxStencil + 1];
   __shared__ real s_zlo[kMaxTP], s_invw[kMaxTP], s_r2at[kMaxTP], s_r2um[kMaxTP];
@@ -87,7 +88,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
   for (int k = 0; k < NTP; ++k) et[k] = 0.0;
   double vir = 0.0;
   int err = 0;
-  for (int item = blockIdx.x * kWaves + wv; item < nitems; item += gridDim.x * kWaves) {
+  for (int item = bid * kWaves + wv; item < nitems; item += a.nblk * kWaves) {
     const int c = a.item_cell[item];
     const int h0 = a.cell_start[c] + (item - a.cell_chunk[c]) * kPairHB;
     const int nh = min(kPairHB, a.cell_start[c + 1] - h0);
@@ -197,8 +198,18 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
   if (threadIdx.x <= NTP) {
     double s = 0.0;
     for (int w = 0; w < kWaves; ++w) s += s_red[w][threadIdx.x];
-    a.part[blockIdx.x * (a.tm.ntp + 1) + (threadIdx.x == NTP ? a.tm.ntp : threadIdx.x)] = s;
+    a.part[bid * (a.tm.ntp + 1) + (threadIdx.x == NTP ? a.tm.ntp : threadIdx.x)] = s;
   }
+}
+
+template <typename real, int NTP>
+__global__ __launch_bounds__(256) void k_pair(PairArgs a) { pair_items_body<real, NTP>(a, blockIdx.x); }
+
+template <typename real, int NTP>
+__global__ __launch_bounds__(256) void k_pair_seg(const PairArgs* __restrict__ args,
+                                                  const int2* __restrict__ map) {
+  const int2 s = map[blockIdx.x];
+  pair_items_body<real, NTP>(args[s.x], s.y);
 }
 
 // Pair evaluation shared by the kernels: table lookup and accumulation for home h (type ht)
@@ -256,17 +267,20 @@ __device__ inline void pair_eval(const double4& hp, int ht, const double4& pj, i
   acc.vir -= (double)(fs * r2);
 }
 
-// Neighbour-parallel variant for the full 27-cell stencil: 32 lanes per home particle.  The
-// candidates of the 27 stencil cells are split into 32 contiguous, equal shares (so dense
+// Neighbour-parallel variant for the full 27-cell stencil: LPH lanes per home particle (the
+// tile shape, 8..64, chosen per system size by the autotuner; 256 / LPH homes per workgroup).
+// The candidates of the 27 stencil cells are split into LPH contiguous, equal shares (so dense
 // cells do not serialise one lane); each lane walks its share two candidates at a time with
 // both loads issued first.  The group reduces the home force with shuffles.
-template <typename real, int NTP>
-__global__ __launch_bounds__(256) void k_pair_nb(PairArgs a) {
+template <typename real, int NTP, int LPH>
+__device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
+  constexpr int HPB = 256 / LPH;                          // homes per workgroup
+  constexpr int MPL = (kMaxStencil + LPH - 1) / LPH;      // stencil cells per lane
   extern __shared__ double s_dyn[];
   __shared__ real s_zlo[kMaxTP], s_invw[kMaxTP], s_r2at[kMaxTP], s_r2um[kMaxTP];
   __shared__ int s_last[kMaxTP], s_tp[kMaxT * kMaxT];
   __shared__ double s_red[4][kMaxTP + 1];
-  __shared__ int s_cst[8][kMaxStencil + 1], s_cpf[8][kMaxStencil + 1];
+  __shared__ int s_cst[HPB][kMaxStencil + 1], s_cpf[HPB][kMaxStencil + 1];
   const int nint = a.tm.nint;
   real* s_coef = (real*)s_dyn;
   int* s_bins = (int*)(s_coef + nint * 12);
@@ -284,35 +298,50 @@ __global__ __launch_bounds__(256) void k_pair_nb(PairArgs a) {
     }
     if (threadIdx.x < kMaxT * kMaxT) s_tp[threadIdx.x] = a.tm.tpidx[threadIdx.x];
   }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, grp = threadIdx.x >> 5;
-  const int s = blockIdx.x * 8 + grp;
-  const int k = threadIdx.x & 31;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, grp = threadIdx.x / LPH;
+  const int s = bid * HPB + grp;
+  const int k = threadIdx.x & (LPH - 1);
   const bool home = s < a.np;
   const int hs = home ? s : a.np - 1;
   const double4 hp = a.cpos[hs];
   const int ht = a.ctype[hs];
   const int c = a.slot_cell[hs];
-  int cnt = 0, st0 = 0;
-  if (k < a.g.nst) {
-    int x = c % a.g.nc[0] + a.g.st[3 * k], y = (c / a.g.nc[0]) % a.g.nc[1] + a.g.st[3 * k + 1],
-        z = c / (a.g.nc[0] * a.g.nc[1]) + a.g.st[3 * k + 2];
-    x += x < 0 ? a.g.nc[0] : (x >= a.g.nc[0] ? -a.g.nc[0] : 0);
-    y += y < 0 ? a.g.nc[1] : (y >= a.g.nc[1] ? -a.g.nc[1] : 0);
-    z += z < 0 ? a.g.nc[2] : (z >= a.g.nc[2] ? -a.g.nc[2] : 0);
-    const int nb = x + a.g.nc[0] * (y + a.g.nc[1] * z);
-    st0 = a.cell_start[nb];
-    cnt = a.cell_start[nb + 1] - st0;
-  }
-  int incl = cnt;
+  const int cx = c % a.g.nc[0], cy = (c / a.g.nc[0]) % a.g.nc[1], cz = c / (a.g.nc[0] * a.g.nc[1]);
+  int st0[MPL], cnt[MPL], tot = 0;
 #pragma unroll
-  for (int off = 1; off < 32; off <<= 1) {
-    const int t = __shfl_up(incl, off, 32);
+  for (int m = 0; m < MPL; ++m) {
+    const int sc = k * MPL + m;
+    cnt[m] = 0;
+    st0[m] = 0;
+    if (sc < a.g.nst) {
+      int x = cx + a.g.st[3 * sc], y = cy + a.g.st[3 * sc + 1], z = cz + a.g.st[3 * sc + 2];
+      x += x < 0 ? a.g.nc[0] : (x >= a.g.nc[0] ? -a.g.nc[0] : 0);
+      y += y < 0 ? a.g.nc[1] : (y >= a.g.nc[1] ? -a.g.nc[1] : 0);
+      z += z < 0 ? a.g.nc[2] : (z >= a.g.nc[2] ? -a.g.nc[2] : 0);
+      const int nb = x + a.g.nc[0] * (y + a.g.nc[1] * z);
+      st0[m] = a.cell_start[nb];
+      cnt[m] = a.cell_start[nb + 1] - st0[m];
+    }
+    tot += cnt[m];
+  }
+  int incl = tot;
+#pragma unroll
+  for (int off = 1; off < LPH; off <<= 1) {
+    const int t = __shfl_up(incl, off, LPH);
     if (k >= off) incl += t;
   }
-  const int ncand = __shfl(incl, 31, 32);
-  if (k < a.g.nst) {
-    s_cst[grp][k] = st0;
-    s_cpf[grp][k] = incl - cnt;
+  const int ncand = __shfl(incl, LPH - 1, LPH);
+  {
+    int pre = incl - tot;
+#pragma unroll
+    for (int m = 0; m < MPL; ++m) {
+      const int sc = k * MPL + m;
+      if (sc < a.g.nst) {
+        s_cst[grp][sc] = st0[m];
+        s_cpf[grp][sc] = pre;
+      }
+      pre += cnt[m];
+    }
   }
   if (k == 0) s_cpf[grp][a.g.nst] = ncand;
   __syncthreads();  // table in LDS and the per-group cell ranges
@@ -320,7 +349,7 @@ __global__ __launch_bounds__(256) void k_pair_nb(PairArgs a) {
   const real rc2 = (real)a.rcut2;
   PairAcc<real, NTP> acc;
   acc.zero();
-  const int share = (ncand + 31) >> 5;
+  const int share = (ncand + LPH - 1) / LPH;
   int i = k * share;
   const int i1 = home ? min(ncand, i + share) : 0;
   if (i < i1) {
@@ -357,7 +386,7 @@ __global__ __launch_bounds__(256) void k_pair_nb(PairArgs a) {
     }
   }
 #pragma unroll
-  for (int off = 16; off > 0; off >>= 1) {
+  for (int off = LPH / 2; off > 0; off >>= 1) {
     acc.fx += __shfl_xor(acc.fx, off, 64);
     acc.fy += __shfl_xor(acc.fy, off, 64);
     acc.fz += __shfl_xor(acc.fz, off, 64);
@@ -382,8 +411,18 @@ __global__ __launch_bounds__(256) void k_pair_nb(PairArgs a) {
   if (threadIdx.x <= NTP) {
     double t = 0.0;
     for (int w = 0; w < 4; ++w) t += s_red[w][threadIdx.x];
-    a.part[blockIdx.x * (a.tm.ntp + 1) + (threadIdx.x == NTP ? a.tm.ntp : threadIdx.x)] = t;
+    a.part[bid * (a.tm.ntp + 1) + (threadIdx.x == NTP ? a.tm.ntp : threadIdx.x)] = t;
   }
+}
+
+template <typename real, int NTP, int LPH>
+__global__ __launch_bounds__(256) void k_pair_nb(PairArgs a) { pair_nb_body<real, NTP, LPH>(a, blockIdx.x); }
+
+template <typename real, int NTP, int LPH>
+__global__ __launch_bounds__(256) void k_pair_nb_seg(const PairArgs* __restrict__ args,
+                                                     const int2* __restrict__ map) {
+  const int2 s = map[blockIdx.x];
+  pair_nb_body<real, NTP, LPH>(args[s.x], s.y);
 }
 
 // Verlet list build (SetVListMC semantics, Src/nlist.F90:820-884, as a full list that also
@@ -536,14 +575,18 @@ __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
 
 // Work items: sum over cells of ceil(count / kPairHB) <= ncell + np / kPairHB; a grid of
 // up to 8 workgroups per CU walks them with a stride.
+bool pair_nbpar(const Ctx* c) {
+  return c->grid.nst == 27 && c->P.list_mode == MSG_LIST_CELL && c->pair_lph > 0;
+}
+
 int pair_blocks(Ctx* c) {
   if (c->P.list_mode == MSG_LIST_VERLET) return (c->np + 255) / 256;
-  if (c->grid.nst == 27 && c->P.list_mode == MSG_LIST_CELL) return (c->np + 7) / 8;
+  if (pair_nbpar(c)) return (c->np + 256 / c->pair_lph - 1) / (256 / c->pair_lph);
   const long items = (long)c->grid.ncell + c->np / kPairHB + 1;
   return (int)std::max(1L, std::min((items + kWaves - 1) / kWaves, 2048L));
 }
 
-int launch_pair(Ctx* c, bool last) {
+static PairArgs pair_args(Ctx* c, bool last) {
   PairArgs a;
   a.np = c->np;
   a.cpos = (const double4*)c->d_cpos;
@@ -568,11 +611,20 @@ int launch_pair(Ctx* c, bool last) {
   a.part = c->d_part + c->part_off_pair;
   a.flags = c->d_flags;
   a.scale = last ? c->P.force_scale : 1.0;
+  a.nblk = c->npart_pair;
+  return a;
+}
+
+static size_t pair_lds(const Ctx* c) {
+  return (size_t)c->tmeta.nint * 12 * (c->fp64 ? 8 : 4) + 8 + (size_t)c->tmeta.ntp * kTabBins * sizeof(int);
+}
+
+int launch_pair(Ctx* c, bool last) {
+  const PairArgs a = pair_args(c, last);
   const int nb = c->npart_pair;
-  const size_t lds = (size_t)c->tmeta.nint * 12 * (c->fp64 ? 8 : 4) + 8 +
-                     (size_t)c->tmeta.ntp * kTabBins * sizeof(int);
+  const size_t lds = pair_lds(c);
   prof_begin(c, MSG_K_PAIR);
-  const bool nbpar = c->grid.nst == 27 && c->P.list_mode == MSG_LIST_CELL;
+  const bool nbpar = pair_nbpar(c);
   if (c->P.list_mode == MSG_LIST_VERLET) {
     VlPairArgs v;
     v.p = a;
@@ -601,8 +653,14 @@ int launch_pair(Ctx* c, bool last) {
   }
 #define MSG_PAIR_LAUNCH(R, NTP)                                                              \
   do {                                                                                       \
-    if (nbpar)                                                                               \
-      hipLaunchKernelGGL((k_pair_nb<R, NTP>), dim3(nb), dim3(256), lds, c->stream, a);      \
+    if (nbpar && c->pair_lph == 8)                                                           \
+      hipLaunchKernelGGL((k_pair_nb<R, NTP, 8>), dim3(nb), dim3(256), lds, c->stream, a);   \
+    else if (nbpar && c->pair_lph == 16)                                                     \
+      hipLaunchKernelGGL((k_pair_nb<R, NTP, 16>), dim3(nb), dim3(256), lds, c->stream, a);  \
+    else if (nbpar && c->pair_lph == 32)                                                     \
+      hipLaunchKernelGGL((k_pair_nb<R, NTP, 32>), dim3(nb), dim3(256), lds, c->stream, a);  \
+    else if (nbpar)                                                                          \
+      hipLaunchKernelGGL((k_pair_nb<R, NTP, 64>), dim3(nb), dim3(256), lds, c->stream, a);  \
     else                                                                                     \
       hipLaunchKernelGGL((k_pair<R, NTP>), dim3(nb), dim3(256), lds, c->stream, a);         \
   } while (0)
@@ -651,6 +709,62 @@ int launch_vlist_build(Ctx* c) {
   MSG_HIP(hipGetLastError());
   c->vl_builds++;
   return MSG_OK;
+}
+
+}  // namespace msg
+
+namespace msg {
+
+namespace {
+
+template <typename real, int NTP, int LPH>
+void seg_pair_nb(const SegGroup& g, hipStream_t s) {
+  hipLaunchKernelGGL((k_pair_nb_seg<real, NTP, LPH>), dim3(g.nblk), dim3(256), g.lds, s,
+                     (const PairArgs*)g.d_args, g.d_map);
+}
+
+template <typename real, int NTP>
+void seg_pair_items(const SegGroup& g, hipStream_t s) {
+  hipLaunchKernelGGL((k_pair_seg<real, NTP>), dim3(g.nblk), dim3(256), g.lds, s,
+                     (const PairArgs*)g.d_args, g.d_map);
+}
+
+template <typename real, int NTP>
+SegLaunchFn pair_fn(int lph) {
+  switch (lph) {
+    case 8: return seg_pair_nb<real, NTP, 8>;
+    case 16: return seg_pair_nb<real, NTP, 16>;
+    case 32: return seg_pair_nb<real, NTP, 32>;
+    case 64: return seg_pair_nb<real, NTP, 64>;
+    default: return seg_pair_items<real, NTP>;
+  }
+}
+
+template <typename real>
+SegLaunchFn pair_fn_ntp(int ntp, int lph) {
+  switch (ntp) {
+    case 1: return pair_fn<real, 1>(lph);
+    case 3: return pair_fn<real, 3>(lph);
+    case 6: return pair_fn<real, 6>(lph);
+    default: return pair_fn<real, 10>(lph);
+  }
+}
+
+}  // namespace
+
+// Pair stage of a batch: one launch per (precision, type-pair count, tile).  The forces are
+// not scaled here (the gather, or the reduction-free path, is the last writer).
+int plan_pair(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out) {
+  SegBuilder<PairArgs> b;
+  for (Ctx* c : cs) {
+    if (c->P.list_mode == MSG_LIST_VERLET) return set_error(MSG_ERR_ARG, "batch: Verlet-list systems run unbatched");
+    const int lph = pair_nbpar(c) ? c->pair_lph : 0;
+    const int ntp = c->nptpt <= 1 ? 1 : c->nptpt <= 3 ? 3 : c->nptpt <= 6 ? 6 : 10;
+    const int key = (c->fp64 ? 100000 : 0) + ntp * 100 + lph;
+    SegLaunchFn fn = c->fp64 ? pair_fn_ntp<double>(ntp, lph) : pair_fn_ntp<float>(ntp, lph);
+    b.add(key, fn, pair_args(c, c->P.ewald_mode == MSG_EWALD_NONE), c->npart_pair, pair_lds(c));
+  }
+  return b.finish(own, out);
 }
 
 }  // namespace msg

@@ -101,18 +101,34 @@ __device__ C2* lds_fft(C2* A, C2* B, const C2* __restrict__ tw, bool inverse) {
   return src;
 }
 
+struct ConvArgs {
+  double tpl2[3];   // (2 pi / L)^2
+  double expf;      // -1 / (4 alpha^2)
+};
+
+// Arguments of the FFT kernels of one system's mesh.
+struct FftArgs {
+  void* Q;            // real mesh K^3
+  void* Cm;           // half-complex mesh H*K*K
+  const void* tw;     // K twiddles
+  const void* G;      // influence function H^3
+  ConvArgs ca;
+  double* part;       // per zconv workgroup: energy, virial
+};
+
 // r2c along x.  Workgroup: T line pairs (2T real rows); row = y + K*z.
 template <typename real, int K>
-__global__ __launch_bounds__(256) void k_fft_x(const real* __restrict__ Q,
-                                               typename C2T<real>::t* __restrict__ Cm,
-                                               const typename C2T<real>::t* __restrict__ tw) {
+__device__ __forceinline__ void fft_x_body(const FftArgs& fa, int bid) {
   typedef typename C2T<real>::t C2;
+  const real* __restrict__ Q = (const real*)fa.Q;
+  C2* __restrict__ Cm = (C2*)fa.Cm;
+  const C2* __restrict__ tw = (const C2*)fa.tw;
   constexpr int E = FftCfg<real>::E, T = E / K, TPL = 256 / T, H = K / 2 + 1;
   __shared__ C2 A[E], B[E];
   __shared__ C2 s_tw[K];  // twiddles staged in LDS
   for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
   const int line = threadIdx.x / TPL, tl = threadIdx.x % TPL;
-  const size_t r0 = 2 * ((size_t)blockIdx.x * T + line), r1 = r0 + 1;
+  const size_t r0 = 2 * ((size_t)bid * T + line), r1 = r0 + 1;
   for (int k = tl; k < K; k += TPL) A[line * K + k] = cmk<C2>(Q[r0 * K + k], Q[r1 * K + k]);
   __syncthreads();
   const C2* Z = lds_fft<C2, K, T>(A, B, s_tw, false) + line * K;
@@ -127,16 +143,17 @@ __global__ __launch_bounds__(256) void k_fft_x(const real* __restrict__ Q,
 
 // c2r along x (the imaginary parts of the kx = 0 and K/2 terms are ignored, as in c2r).
 template <typename real, int K>
-__global__ __launch_bounds__(256) void k_fft_xi(const typename C2T<real>::t* __restrict__ Cm,
-                                                real* __restrict__ Q,
-                                                const typename C2T<real>::t* __restrict__ tw) {
+__device__ __forceinline__ void fft_xi_body(const FftArgs& fa, int bid) {
   typedef typename C2T<real>::t C2;
+  const C2* __restrict__ Cm = (const C2*)fa.Cm;
+  real* __restrict__ Q = (real*)fa.Q;
+  const C2* __restrict__ tw = (const C2*)fa.tw;
   constexpr int E = FftCfg<real>::E, T = E / K, TPL = 256 / T, H = K / 2 + 1;
   __shared__ C2 A[E], B[E];
   __shared__ C2 s_tw[K];  // twiddles staged in LDS
   for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
   const int line = threadIdx.x / TPL, tl = threadIdx.x % TPL;
-  const size_t r0 = 2 * ((size_t)blockIdx.x * T + line), r1 = r0 + 1;
+  const size_t r0 = 2 * ((size_t)bid * T + line), r1 = r0 + 1;
   for (int k = tl; k < H; k += TPL) {
     C2 a = Cm[r0 * H + k], b = Cm[r1 * H + k];
     if (k == 0 || k == K / 2) { a.y = 0; b.y = 0; }
@@ -154,15 +171,15 @@ __global__ __launch_bounds__(256) void k_fft_xi(const typename C2T<real>::t* __r
 
 // Complex FFT along y (stride H) for tiles of T consecutive kx at one z.
 template <typename real, int K>
-__global__ __launch_bounds__(256) void k_fft_y(typename C2T<real>::t* __restrict__ Cm,
-                                               const typename C2T<real>::t* __restrict__ tw,
-                                               int inverse) {
+__device__ __forceinline__ void fft_y_body(const FftArgs& fa, int bid, int inverse) {
   typedef typename C2T<real>::t C2;
+  C2* __restrict__ Cm = (C2*)fa.Cm;
+  const C2* __restrict__ tw = (const C2*)fa.tw;
   constexpr int E = FftCfg<real>::E, T = E / K, H = K / 2 + 1, NTX = (H + T - 1) / T;
   __shared__ C2 A[E], B[E];
   __shared__ C2 s_tw[K];  // twiddles staged in LDS
   for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
-  const int tx = blockIdx.x % NTX, z = blockIdx.x / NTX, kx0 = tx * T;
+  const int tx = bid % NTX, z = bid / NTX, kx0 = tx * T;
   for (int e = threadIdx.x; e < E; e += 256) {
     const int kl = e % T, y = e / T, kx = kx0 + kl;
     A[kl * K + y] = kx < H ? Cm[kx + (size_t)H * (y + (size_t)K * z)] : cmk<C2>(0, 0);
@@ -175,24 +192,21 @@ __global__ __launch_bounds__(256) void k_fft_y(typename C2T<real>::t* __restrict
   }
 }
 
-struct ConvArgs {
-  double tpl2[3];   // (2 pi / L)^2
-  double expf;      // -1 / (4 alpha^2)
-};
-
 // Forward FFT along z, influence-function multiply + FP64 energy/virial, inverse FFT along z.
 template <typename real, int K>
-__global__ __launch_bounds__(256) void k_fft_zconv(typename C2T<real>::t* __restrict__ Cm,
-                                                   const typename C2T<real>::t* __restrict__ tw,
-                                                   const real* __restrict__ G, ConvArgs ca,
-                                                   double* __restrict__ part) {
+__device__ __forceinline__ void fft_zconv_body(const FftArgs& fa, int bid) {
   typedef typename C2T<real>::t C2;
+  C2* __restrict__ Cm = (C2*)fa.Cm;
+  const C2* __restrict__ tw = (const C2*)fa.tw;
+  const real* __restrict__ G = (const real*)fa.G;
+  const ConvArgs& ca = fa.ca;
+  double* __restrict__ part = fa.part;
   constexpr int E = FftCfg<real>::E, T = E / K, H = K / 2 + 1, NTX = (H + T - 1) / T;
   __shared__ C2 A[E], B[E];
   __shared__ C2 s_tw[K];  // twiddles staged in LDS
   for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
   __shared__ double red[8];
-  const int tx = blockIdx.x % NTX, y = blockIdx.x / NTX, kx0 = tx * T;
+  const int tx = bid % NTX, y = bid / NTX, kx0 = tx * T;
   const size_t plane = (size_t)H * K;
   for (int e = threadIdx.x; e < E; e += 256) {
     const int kl = e % T, z = e / T, kx = kx0 + kl;
@@ -203,6 +217,9 @@ __global__ __launch_bounds__(256) void k_fft_zconv(typename C2T<real>::t* __rest
   const int ky = y <= K / 2 ? y : K - y;
   double esum = 0.0, vsum = 0.0;
   for (int e = threadIdx.x; e < E; e += 256) {
+    // no contraction of the FP64 energy/virial terms: the value must not depend on how
+    // the compiler fuses them in the single-system and the batched instantiation
+#pragma clang fp contract(off)
     const int kl = e / K, kz = e % K, kx = kx0 + kl;
     if (kx >= H) continue;
     const int kzz = kz <= K / 2 ? kz : K - kz;
@@ -228,10 +245,27 @@ __global__ __launch_bounds__(256) void k_fft_zconv(typename C2T<real>::t* __rest
   esum = msgd::block_sum(esum, red);
   vsum = msgd::block_sum(vsum, red);
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = esum;
-    part[2 * blockIdx.x + 1] = vsum;
+    part[2 * bid] = esum;
+    part[2 * bid + 1] = vsum;
   }
 }
+
+// Entry points: one system (argument block by value) or a batch (segmented).
+#define MSG_FFT_ENTRY(name, body, ...)                                                      \
+  template <typename real, int K>                                                           \
+  __global__ __launch_bounds__(256) void name(FftArgs a) { body<real, K>(a, blockIdx.x __VA_ARGS__); } \
+  template <typename real, int K>                                                           \
+  __global__ __launch_bounds__(256) void name##_seg(const FftArgs* __restrict__ args,       \
+                                                    const int2* __restrict__ map) {         \
+    const int2 s = map[blockIdx.x];                                                         \
+    body<real, K>(args[s.x], s.y __VA_ARGS__);                                              \
+  }
+MSG_FFT_ENTRY(k_fft_x, fft_x_body)
+MSG_FFT_ENTRY(k_fft_xi, fft_xi_body)
+MSG_FFT_ENTRY(k_fft_y, fft_y_body, , 0)
+MSG_FFT_ENTRY(k_fft_yi, fft_y_body, , 1)
+MSG_FFT_ENTRY(k_fft_zconv, fft_zconv_body)
+#undef MSG_FFT_ENTRY
 
 struct SpreadArgs {
   MeshGrid m;
@@ -256,7 +290,7 @@ constexpr double kQScale = 281474976710656.0;  // 2^48
 // the clipped x-row into the brick with 64-bit integer LDS atomics.  Phase 3: every mesh
 // point of the brick is written once.
 template <typename real, int P>
-__global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
+__device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   typedef typename R4T<real>::t R4;
   constexpr int CAP = sizeof(real) == 4 ? 256 : 128;
   constexpr int NBP = kBX * kBY * kBZ;
@@ -265,7 +299,7 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   __shared__ int loc[CAP];
   __shared__ int r_beg[8], r_pref[9];
   const int K = a.m.K;
-  const int b = blockIdx.x;
+  const int b = bid;
   const int bx = b % a.m.nb[0], by = (b / a.m.nb[0]) % a.m.nb[1], bz = b / (a.m.nb[0] * a.m.nb[1]);
   const int lo[3] = {bx * kBX, by * kBY, bz * kBZ};
   __shared__ int r_len[8];
@@ -347,6 +381,16 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   }
 }
 
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_spread(SpreadArgs a) { spread_body<real, P>(a, blockIdx.x); }
+
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_spread_seg(const SpreadArgs* __restrict__ args,
+                                                    const int2* __restrict__ map) {
+  const int2 s = map[blockIdx.x];
+  spread_body<real, P>(args[s.x], s.y);
+}
+
 struct GatherArgs {
   MeshGrid m;
   int np;
@@ -363,10 +407,10 @@ struct GatherArgs {
 
 // Eight lanes per particle (brick order); lane iz < P sums one z-plane of the stencil.
 template <typename real, int P>
-__global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
+__device__ __forceinline__ void gather_body(const GatherArgs& a, int bid) {
   typedef typename R4T<real>::t R4;
   const double M[3] = {a.Mtot[0], a.Mtot[1], a.Mtot[2]};
-  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int gid = bid * 256 + threadIdx.x;
   const int i = gid >> 3, iz = gid & 7;
   const bool valid = i < a.np;
   const int K = a.m.K;
@@ -433,33 +477,56 @@ __global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
   }
 }
 
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_gather(GatherArgs a) { gather_body<real, P>(a, blockIdx.x); }
+
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_gather_seg(const GatherArgs* __restrict__ args,
+                                                    const int2* __restrict__ map) {
+  const int2 s = map[blockIdx.x];
+  gather_body<real, P>(args[s.x], s.y);
+}
+
 // ------------------------------------------------------------------------------------------
+FftArgs fft_args(Ctx* c) {
+  FftArgs a;
+  a.Q = c->d_Q;
+  a.Cm = c->d_C;
+  a.tw = c->d_tw;
+  a.G = c->d_G;
+  for (int d = 0; d < 3; ++d) a.ca.tpl2[d] = std::pow(2.0 * M_PI / c->P.boxlen[d], 2);
+  a.ca.expf = -1.0 / (4.0 * c->P.alpha * c->P.alpha);
+  a.part = c->d_part + c->part_off_spme;
+  return a;
+}
+
+// Workgroups of the five FFT stages for mesh K: x, y, zconv, yi, xi.
+template <typename real, int K>
+void fft_blocks(int* nb) {
+  constexpr int E = FftCfg<real>::E, T = E / K, H = K / 2 + 1, NTX = (H + T - 1) / T;
+  nb[0] = nb[4] = K * K / 2 / T;
+  nb[1] = nb[2] = nb[3] = NTX * K;
+}
+
 template <typename real, int K>
 void launch_fft_all(Ctx* c) {
-  typedef typename C2T<real>::t C2;
-  constexpr int E = FftCfg<real>::E, T = E / K, H = K / 2 + 1, NTX = (H + T - 1) / T;
-  C2* Cm = (C2*)c->d_C;
-  const C2* tw = (const C2*)c->d_tw;
-  real* Q = (real*)c->d_Q;
-  const int nx = K * K / 2 / T;
+  int nb[5];
+  fft_blocks<real, K>(nb);
+  const FftArgs a = fft_args(c);
   prof_begin(c, MSG_K_FFT_X);
-  hipLaunchKernelGGL((k_fft_x<real, K>), dim3(nx), dim3(256), 0, c->stream, Q, Cm, tw);
+  hipLaunchKernelGGL((k_fft_x<real, K>), dim3(nb[0]), dim3(256), 0, c->stream, a);
   prof_end(c, MSG_K_FFT_X);
   prof_begin(c, MSG_K_FFT_Y);
-  hipLaunchKernelGGL((k_fft_y<real, K>), dim3(NTX * K), dim3(256), 0, c->stream, Cm, tw, 0);
+  hipLaunchKernelGGL((k_fft_y<real, K>), dim3(nb[1]), dim3(256), 0, c->stream, a);
   prof_end(c, MSG_K_FFT_Y);
-  ConvArgs ca;
-  for (int d = 0; d < 3; ++d) ca.tpl2[d] = std::pow(2.0 * M_PI / c->P.boxlen[d], 2);
-  ca.expf = -1.0 / (4.0 * c->P.alpha * c->P.alpha);
   prof_begin(c, MSG_K_FFT_ZCONV);
-  hipLaunchKernelGGL((k_fft_zconv<real, K>), dim3(NTX * K), dim3(256), 0, c->stream, Cm, tw,
-                     (const real*)c->d_G, ca, c->d_part + c->part_off_spme);
+  hipLaunchKernelGGL((k_fft_zconv<real, K>), dim3(nb[2]), dim3(256), 0, c->stream, a);
   prof_end(c, MSG_K_FFT_ZCONV);
   prof_begin(c, MSG_K_FFT_YI);
-  hipLaunchKernelGGL((k_fft_y<real, K>), dim3(NTX * K), dim3(256), 0, c->stream, Cm, tw, 1);
+  hipLaunchKernelGGL((k_fft_yi<real, K>), dim3(nb[3]), dim3(256), 0, c->stream, a);
   prof_end(c, MSG_K_FFT_YI);
   prof_begin(c, MSG_K_FFT_XI);
-  hipLaunchKernelGGL((k_fft_xi<real, K>), dim3(nx), dim3(256), 0, c->stream, (const C2*)Cm, Q, tw);
+  hipLaunchKernelGGL((k_fft_xi<real, K>), dim3(nb[4]), dim3(256), 0, c->stream, a);
   prof_end(c, MSG_K_FFT_XI);
 }
 
@@ -476,27 +543,34 @@ int launch_fft_dispatch(Ctx* c) {
   return MSG_OK;
 }
 
+SpreadArgs spread_args(Ctx* c) { return SpreadArgs{c->mesh, c->d_brick_start, c->d_bq, c->d_bmm, c->d_Q}; }
+
+GatherArgs gather_args(Ctx* c) {
+  GatherArgs g;
+  g.m = c->mesh;
+  g.np = c->np;
+  g.brick_ip = c->d_brick_ip;
+  g.bq = c->d_bq;
+  g.bmm = c->d_bmm;
+  g.phi = c->d_Q;
+  g.epsi4pi = c->P.epsi4pi;
+  const double vol = c->P.boxlen[0] * c->P.boxlen[1] * c->P.boxlen[2];
+  g.surf2fac = c->P.lsurf ? 2.0 * (2.0 * M_PI / (3.0 * vol)) : 0.0;
+  g.Mtot = c->d_M;
+  g.force = c->d_force;
+  g.scale = c->P.force_scale;
+  return g;
+}
+
 template <typename real, int P>
 void launch_spread_gather(Ctx* c, bool spread) {
   if (spread) {
-    SpreadArgs a{c->mesh, c->d_brick_start, c->d_bq, c->d_bmm, c->d_Q};
+    const SpreadArgs a = spread_args(c);
     prof_begin(c, MSG_K_SPREAD);
     hipLaunchKernelGGL((k_spread<real, P>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
     prof_end(c, MSG_K_SPREAD);
   } else {
-    GatherArgs g;
-    g.m = c->mesh;
-    g.np = c->np;
-    g.brick_ip = c->d_brick_ip;
-    g.bq = c->d_bq;
-    g.bmm = c->d_bmm;
-    g.phi = c->d_Q;
-    g.epsi4pi = c->P.epsi4pi;
-    const double vol = c->P.boxlen[0] * c->P.boxlen[1] * c->P.boxlen[2];
-    g.surf2fac = c->P.lsurf ? 2.0 * (2.0 * M_PI / (3.0 * vol)) : 0.0;
-    g.Mtot = c->d_M;
-    g.force = c->d_force;
-    g.scale = c->P.force_scale;
+    const GatherArgs g = gather_args(c);
     prof_begin(c, MSG_K_GATHER);
     hipLaunchKernelGGL((k_gather<real, P>), dim3((8 * c->np + 255) / 256), dim3(256), 0, c->stream, g);
     prof_end(c, MSG_K_GATHER);
@@ -629,6 +703,101 @@ int launch_spme_gather(Ctx* c) {
   if (e) return e;
   MSG_HIP(hipGetLastError());
   return MSG_OK;
+}
+
+}  // namespace msg
+
+namespace msg {
+
+namespace {
+
+template <typename real, int K>
+void seg_fft(const SegGroup& g, hipStream_t s, int stage) {
+  const FftArgs* a = (const FftArgs*)g.d_args;
+  switch (stage) {
+    case 0: hipLaunchKernelGGL((k_fft_x_seg<real, K>), dim3(g.nblk), dim3(256), 0, s, a, g.d_map); break;
+    case 1: hipLaunchKernelGGL((k_fft_y_seg<real, K>), dim3(g.nblk), dim3(256), 0, s, a, g.d_map); break;
+    case 2: hipLaunchKernelGGL((k_fft_zconv_seg<real, K>), dim3(g.nblk), dim3(256), 0, s, a, g.d_map); break;
+    case 3: hipLaunchKernelGGL((k_fft_yi_seg<real, K>), dim3(g.nblk), dim3(256), 0, s, a, g.d_map); break;
+    default: hipLaunchKernelGGL((k_fft_xi_seg<real, K>), dim3(g.nblk), dim3(256), 0, s, a, g.d_map); break;
+  }
+}
+template <typename real, int K, int STAGE>
+void seg_fft_stage(const SegGroup& g, hipStream_t s) { seg_fft<real, K>(g, s, STAGE); }
+
+template <typename real, int K>
+void add_fft(Ctx* c, SegBuilder<FftArgs>* b, int key) {
+  int nb[5];
+  fft_blocks<real, K>(nb);
+  const FftArgs a = fft_args(c);
+  b[0].add(key, seg_fft_stage<real, K, 0>, a, nb[0]);
+  b[1].add(key, seg_fft_stage<real, K, 1>, a, nb[1]);
+  b[2].add(key, seg_fft_stage<real, K, 2>, a, nb[2]);
+  b[3].add(key, seg_fft_stage<real, K, 3>, a, nb[3]);
+  b[4].add(key, seg_fft_stage<real, K, 4>, a, nb[4]);
+}
+
+template <typename real>
+int add_fft_k(Ctx* c, SegBuilder<FftArgs>* b) {
+  const int key = (sizeof(real) == 8 ? 1000 : 0) + c->mesh.K;
+  switch (c->mesh.K) {
+    case 16: add_fft<real, 16>(c, b, key); break;
+    case 32: add_fft<real, 32>(c, b, key); break;
+    case 64: add_fft<real, 64>(c, b, key); break;
+    case 128: add_fft<real, 128>(c, b, key); break;
+    case 256: add_fft<real, 256>(c, b, key); break;
+    default: return set_error(MSG_ERR_MESH, "nmesh must be 16, 32, 64, 128 or 256");
+  }
+  return MSG_OK;
+}
+
+template <typename real, int P>
+void seg_spread(const SegGroup& g, hipStream_t s) {
+  hipLaunchKernelGGL((k_spread_seg<real, P>), dim3(g.nblk), dim3(256), 0, s, (const SpreadArgs*)g.d_args, g.d_map);
+}
+template <typename real, int P>
+void seg_gather(const SegGroup& g, hipStream_t s) {
+  hipLaunchKernelGGL((k_gather_seg<real, P>), dim3(g.nblk), dim3(256), 0, s, (const GatherArgs*)g.d_args, g.d_map);
+}
+
+template <typename real>
+int add_spread_gather(Ctx* c, SegBuilder<SpreadArgs>& bs, SegBuilder<GatherArgs>& bg) {
+  const int key = (sizeof(real) == 8 ? 100 : 0) + c->mesh.order;
+  SegLaunchFn fs, fg;
+  switch (c->mesh.order) {
+    case 3: fs = seg_spread<real, 3>; fg = seg_gather<real, 3>; break;
+    case 4: fs = seg_spread<real, 4>; fg = seg_gather<real, 4>; break;
+    case 5: fs = seg_spread<real, 5>; fg = seg_gather<real, 5>; break;
+    case 6: fs = seg_spread<real, 6>; fg = seg_gather<real, 6>; break;
+    case 7: fs = seg_spread<real, 7>; fg = seg_gather<real, 7>; break;
+    case 8: fs = seg_spread<real, 8>; fg = seg_gather<real, 8>; break;
+    default: return set_error(MSG_ERR_ARG, "SPME order must be 3..8");
+  }
+  bs.add(key, fs, spread_args(c), c->mesh.nbrick);
+  bg.add(key, fg, gather_args(c), (8 * c->np + 255) / 256);
+  return MSG_OK;
+}
+
+}  // namespace
+
+// Mesh stages of a batch: one launch per (precision, order) for spread and gather and per
+// (precision, K) for each FFT stage.  Systems without SPME are skipped.
+int plan_spme(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& spread, SegStage* fft,
+              SegStage& gather) {
+  SegBuilder<SpreadArgs> bs;
+  SegBuilder<GatherArgs> bg;
+  SegBuilder<FftArgs> bf[5];
+  for (Ctx* c : cs) {
+    if (c->P.ewald_mode != MSG_EWALD_SPM) continue;
+    int e = c->fp64 ? add_spread_gather<double>(c, bs, bg) : add_spread_gather<float>(c, bs, bg);
+    if (e) return e;
+    e = c->fp64 ? add_fft_k<double>(c, bf) : add_fft_k<float>(c, bf);
+    if (e) return e;
+  }
+  if (int e = bs.finish(own, spread)) return e;
+  for (int k = 0; k < 5; ++k)
+    if (int e = bf[k].finish(own, fft[k])) return e;
+  return bg.finish(own, gather);
 }
 
 }  // namespace msg

@@ -107,6 +107,18 @@ def lib():
         L.msg_profile_read.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.msg_device_bytes.argtypes = [_vp]
         L.msg_device_bytes.restype = C.c_longlong
+        L.msg_set_tile.argtypes = [_vp, C.c_int]
+        L.msg_get_tile.argtypes = [_vp]
+        L.msg_get_forces.argtypes = [_vp, _dp]
+        L.msg_query.argtypes = [_vp]
+        L.msg_batch_create.argtypes = [C.POINTER(_vp), C.c_int, C.POINTER(_vp)]
+        L.msg_batch_destroy.argtypes = [_vp]
+        L.msg_batch_destroy.restype = None
+        L.msg_batch_utotal.argtypes = [_vp]
+        L.msg_batch_sync.argtypes = [_vp]
+        L.msg_batch_dutotal.argtypes = [_vp, _ip, _ip, _dp, _ip, _dp]
+        L.msg_batch_results.argtypes = [_vp, _vp, _vp, _vp]
+        L.msg_batch_query.argtypes = [_vp]
         _lib = L
     return _lib
 
@@ -342,3 +354,87 @@ class Engine:
 
     def device_bytes(self) -> int:
         return int(lib().msg_device_bytes(self.h))
+
+    # -- tiles, completion, device-resident results --------------------------------------
+    def set_tile(self, pair_lanes: int):
+        _check(lib().msg_set_tile(self.h, int(pair_lanes)))
+
+    def tile(self) -> int:
+        return int(lib().msg_get_tile(self.h))
+
+    def query(self) -> bool:
+        rc = lib().msg_query(self.h)
+        _check(min(rc, 0))
+        return bool(rc)
+
+    def forces(self):
+        f = np.zeros(3 * self.spec.n)
+        _check(lib().msg_get_forces(self.h, f))
+        return f.reshape(-1, 3)
+
+
+class Batch:
+    """Resident systems evaluated together: each stage of the UTotal pass is one launch over
+    all systems (captured once into a graph, so a batch step is one graph launch), and the
+    MC trials of all systems are one launch."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        arr = (_vp * len(self.engines))(*[e.h for e in self.engines])
+        h = _vp()
+        _check(lib().msg_batch_create(arr, len(self.engines), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().msg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def utotal_async(self):
+        _check(lib().msg_batch_utotal(self.h))
+
+    def sync(self):
+        _check(lib().msg_batch_sync(self.h))
+
+    def query(self) -> bool:
+        rc = lib().msg_batch_query(self.h)
+        _check(min(rc, 0))
+        return bool(rc)
+
+    def energies(self) -> dict:
+        """u_tot, u_rec, virial of every system (arrays), one download after the pass."""
+        n = len(self.engines)
+        ut, ur, vi = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(lib().msg_batch_results(self.h, _ptr(ut), _ptr(ur), _ptr(vi)))
+        return dict(u_tot=ut, u_rec=ur, virial=vi)
+
+    def utotal(self, want_force: bool = True) -> list[dict]:
+        self.utotal_async()
+        self.sync()
+        out = []
+        for e in self.engines:
+            r = e.fetch()
+            r["force"] = e.forces() if want_force else None
+            out.append(r)
+        return out
+
+    def dutotal(self, ips, rnews) -> list[dict]:
+        """ips[s] (0-based indices) and rnews[s] (ntrial, 3) per system s."""
+        nt = np.array([len(i) for i in ips], dtype=np.int32)
+        ip = np.ascontiguousarray(np.concatenate([np.asarray(i) for i in ips]) + 1, dtype=np.int32)
+        r = np.ascontiguousarray(np.concatenate([np.asarray(x).reshape(-1, 3) for x in rnews]),
+                                 dtype=np.float64).reshape(-1)
+        ovl = np.zeros(ip.size, dtype=np.int32)
+        du = np.zeros(ip.size)
+        _check(lib().msg_batch_dutotal(self.h, nt, ip, r, ovl, du))
+        out, off = [], 0
+        for n in nt:
+            out.append(dict(overlap=ovl[off:off + n].astype(bool), du_tot=du[off:off + n]))
+            off += n
+        return out

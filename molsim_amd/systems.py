@@ -75,6 +75,7 @@ def _insert_uniform(rng, n, L, dmin, existing=None, max_rounds=200):
 
     pts = rng.uniform(0.0, L, size=(n, 3))
     fixed = None if existing is None or len(existing) == 0 else (existing + 0.5 * L) % L
+    ftree = None if fixed is None or dmin <= 0 else cKDTree(fixed, boxsize=L)
     for _ in range(max_rounds):
         bad = np.zeros(n, dtype=bool)
         if dmin > 0:
@@ -82,8 +83,7 @@ def _insert_uniform(rng, n, L, dmin, existing=None, max_rounds=200):
             pairs = tree.query_pairs(dmin, output_type="ndarray")
             if len(pairs):
                 bad[pairs[:, 1]] = True
-            if fixed is not None:
-                ftree = cKDTree(fixed, boxsize=L)
+            if ftree is not None:
                 hits = ftree.query_ball_point(pts, dmin, return_length=True)
                 bad |= hits > 0
         if not bad.any():
@@ -138,6 +138,42 @@ def _random_walk_chains(rng, nchain, nbead, bond, L, dmin):
             k += 1
         assert k - start == nbead
     return out
+
+
+def _random_walk_chains_batch(rng, nchain, nbead, bond, L, dmin, max_tries=1000):
+    """Self-avoiding random-walk chains grown all at once, one bead per chain per step;
+    a proposal overlapping any placed bead or another proposal is redrawn."""
+    from scipy.spatial import cKDTree
+
+    out = np.zeros((nchain, nbead, 3))
+    placed = np.zeros((0, 3))
+    for b in range(nbead):
+        todo = np.arange(nchain)
+        prop = np.zeros((nchain, 3))
+        ptree = cKDTree(placed + 0.5 * L, boxsize=L) if placed.shape[0] else None
+        for _ in range(max_tries):
+            if b == 0:
+                q = rng.uniform(-0.5 * L, 0.5 * L, size=(todo.size, 3))
+            else:
+                v = rng.normal(size=(todo.size, 3))
+                v *= bond / np.linalg.norm(v, axis=1)[:, None]
+                q = _wrap(out[todo, b - 1] + v, L)
+            prop[todo] = q
+            bad = np.zeros(nchain, dtype=bool)
+            if ptree is not None:
+                hits = ptree.query_ball_point((q + 0.5 * L) % L, dmin, return_length=True)
+                bad[todo[hits > 0]] = True
+            pairs = cKDTree((prop + 0.5 * L) % L, boxsize=L).query_pairs(dmin, output_type="ndarray")
+            if len(pairs):
+                bad[np.maximum(pairs[:, 0], pairs[:, 1])] = True
+            todo = np.nonzero(bad)[0]
+            if todo.size == 0:
+                break
+        else:
+            raise RuntimeError("chain growth failed")
+        out[:, b] = prop
+        placed = np.concatenate([placed, prop])
+    return out.reshape(-1, 3)
 
 
 def _pot(npt, z, form, rcut, relp, alpha, sig=None, eps=None, rad=None, lewald=True):
@@ -223,3 +259,80 @@ def small_spme_system(seed: int = 5, n: int = 400, L: float = 60.0, nmesh: int =
     return SystemSpec(f"small_{n}", ro, ipt, az, np.array([L, L, L]), pot, rcut,
                       list_mode=list_mode, ewald="spm", alpha=alpha, nmesh=nmesh, order=order,
                       lsurf=lsurf)
+
+
+# Four ion types shared by every member of an ensemble, so that one table serves all of them:
+# 0: +1, 1: -1 (also the chain beads), 2: +2, 3: -2.
+ENSEMBLE_CHARGES = (1.0, -1.0, 2.0, -2.0)
+
+
+def ensemble_mesh(L: float, hmax: float = 6.5, kmin: int = 16, kmax: int = 64) -> int:
+    """SPME mesh for box length L: the smallest power of two with spacing L/K <= hmax,
+    clamped to [kmin, kmax] (16^3 at L = 87.5 A up to 64^3 at L = 406 A)."""
+    k = kmin
+    while k < kmax and L / k > hmax:
+        k *= 2
+    return k
+
+
+def ensemble_member(seed: int, n: int, kind: str, f2: float, phi: float = 0.01,
+                    rcut: float = 24.0, alpha: float = 0.125, order: int = 6, lb: float = 7.1,
+                    nbead: int = 32) -> SystemSpec:
+    """One ensemble member: n particles of diameter 4 A at volume fraction phi, SPME order 6,
+    rc 24 A with alpha 0.125/A (alpha rc = 3), LJ (sigma 4 A, eps 0.1 kT) + hard core +
+    Coulomb at Bjerrum length 7.1 A.  ``kind`` "electrolyte": +-1 and +-2 ion pairs, a share
+    f2 of the pairs divalent; "polyelectrolyte": chains of nbead -1 beads (5 A random-walk
+    bonds) neutralised by +1 and +2 counterions, a share f2 of the counter-charge divalent."""
+    rng = np.random.default_rng(seed)
+    L = (n * (math.pi / 6.0) * 4.0 ** 3 / phi) ** (1.0 / 3.0)
+    if kind == "electrolyte":
+        npair = n // 2
+        n2 = int(round(f2 * npair))
+        n1 = npair - n2
+        ipt = np.array([0] * n1 + [1] * n1 + [2] * n2 + [3] * n2, dtype=np.int32)
+        ro = _insert_uniform(rng, ipt.size, L, 4.0)
+    else:
+        nb = max(nbead, int(math.ceil(n / (2.0 - 0.5 * f2) / nbead)) * nbead)
+        nch = nb // nbead
+        n2 = int(round(0.5 * f2 * nb))
+        n1 = nb - 2 * n2
+        beads = _random_walk_chains_batch(rng, nch, nbead, 5.0, L, 4.0)
+        ions = _insert_uniform(rng, n1 + n2, L, 4.0, existing=beads)
+        ro = np.concatenate([beads, ions])
+        ipt = np.array([1] * nb + [0] * n1 + [2] * n2, dtype=np.int32)
+    az = np.asarray(ENSEMBLE_CHARGES)[ipt]
+    assert abs(az.sum()) < 1e-9
+    relp = relpermitt_from_bjerrum(lb)
+    eps = 0.1 / beta_kjmol()
+    pot = _pot(4, ENSEMBLE_CHARGES, 1, rcut, relp, alpha, sig=[4.0] * 4, eps=[eps] * 4,
+               rad=[2.0] * 4)
+    K = ensemble_mesh(L)
+    return SystemSpec(f"{kind}_{ipt.size}_spme{K}", ro, ipt, az, np.array([L, L, L]), pot, rcut,
+                      list_mode="cell", ewald="spm", alpha=alpha, nmesh=K, order=order,
+                      lsurf=True, extra=dict(kind=kind, f2=f2, dtran=2.0, ntrial=1000))
+
+
+def _member_args(a):
+    seed, n, kind, f2, kw = a
+    return ensemble_member(seed, n, kind, f2, **kw)
+
+
+def ensemble(nsys: int = 512, seed: int = 4, nmin: int = 200, nmax: int = 20000,
+             workers: int = 1, **kw) -> list[SystemSpec]:
+    """Independent electrolyte / polyelectrolyte systems with particle counts log-uniform in
+    [nmin, nmax], box at volume fraction 0.01, mesh 16^3..64^3 by box length, seeded
+    shares of divalent ions.  ``workers`` > 1 generates in forked processes (call it before
+    the process touches the GPU)."""
+    rng = np.random.default_rng(seed)
+    logn = rng.uniform(math.log(nmin), math.log(nmax), size=nsys)
+    kinds = rng.choice(["electrolyte", "polyelectrolyte"], size=nsys)
+    f2 = rng.uniform(0.0, 0.5, size=nsys)
+    seeds = rng.integers(1, 2 ** 31 - 1, size=nsys)
+    args = [(int(seeds[i]), max(nmin, int(round(math.exp(logn[i]) / 2)) * 2), str(kinds[i]),
+             float(f2[i]), kw) for i in range(nsys)]
+    if workers > 1:
+        import multiprocessing as mp
+
+        with mp.get_context("fork").Pool(workers) as pool:
+            return pool.map(_member_args, args, chunksize=4)
+    return [_member_args(a) for a in args]
