@@ -203,6 +203,23 @@ int msg_batch_results(msg_batch* batch, double* u_tot, double* u_rec, double* vi
 int msg_batch_dutotal(msg_batch* batch, const int* ntrial, const int* ip, const double* rotm,
                       int* lhsoverlap, double* du_tot);
 
+/* Execution unit for continuous batching (no counterpart in the reference): one batch of
+ * requests at a time on the unit's own streams -- a UTotal pass over ut[0..n_u) (energies;
+ * forces downloaded when want_forces) and ntrial[s] MC trials of each dec[s] (ip 1-based
+ * and rotm concatenated over dec), as segmented launches over all its systems.  Launch
+ * returns at once; several units in flight overlap on the device.  A system must not be in
+ * two units in flight at once.  msg_unit_results waits and copies: u_tot/u_rec/virial
+ * (n_u each), force (concatenated force(3,np) of the ut systems), lhsoverlap/du_tot (all
+ * trials in launch order); any may be null. */
+typedef struct msg_unit msg_unit;
+int msg_unit_create(msg_unit** out);
+void msg_unit_destroy(msg_unit* unit);
+int msg_unit_launch(msg_unit* unit, msg_ctx** ut, int n_u, int want_forces, msg_ctx** dec, int n_d,
+                    const int* ntrial, const int* ip, const double* rotm);
+int msg_unit_query(msg_unit* unit);
+int msg_unit_results(msg_unit* unit, double* u_tot, double* u_rec, double* virial, double* force,
+                     int* lhsoverlap, double* du_tot);
+
 /* 1 when everything launched on the context's stream has completed, 0 while work is pending. */
 int msg_query(msg_ctx* ctx);
 

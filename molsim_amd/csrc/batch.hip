@@ -345,6 +345,204 @@ int msg_batch_dutotal(msg_batch* b, const int* ntrial, const int* ip, const doub
   return MSG_OK;
 }
 
+// ---- execution units (continuous batching) --------------------------------------------------
+// A unit executes one batch of requests at a time on its own streams: a UTotal pass over
+// the systems that asked for energy + forces (and over trial systems whose state is stale),
+// then the MC trials of the trial systems, then one download of energies, forces and trial
+// results.  Several units in flight overlap on the device; a system may be in one in-flight
+// unit at a time (the caller's scheduler guarantees it).
+}  // extern "C"
+
+struct msg_unit {
+  hipStream_t s = nullptr, side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, done = nullptr;
+  SegOwner own;
+  Pass pass;
+  std::vector<Ctx*> ut, dec;
+  std::vector<int> ntrial;
+  std::vector<DeGroup> de;
+  DeLaunch L;
+  bool launched = false, want_forces = false;
+  double* d_de = nullptr;
+  long de_cap = 0;
+  double* h_outs = nullptr;
+  long outs_cap = 0;
+  double* h_de = nullptr;
+  long hde_cap = 0;
+  double* h_f = nullptr;
+  long hf_cap = 0;
+  std::vector<long> foff;
+};
+
+namespace {
+
+int grow_pinned(double** p, long* cap, long need) {
+  if (need <= *cap) return MSG_OK;
+  const long n = std::max(need, 2 * *cap + 1024);
+  if (*p) hipHostFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  MSG_HIP(hipHostMalloc((void**)p, n * sizeof(double), hipHostMallocDefault));
+  *cap = n;
+  return MSG_OK;
+}
+
+// Plans the unit's pass and trials into its arena, doubling the arena until they fit.
+int plan_unit(msg_unit* u, const std::vector<Ctx*>& pcs, const int* ip, const double* rotm) {
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    u->own.reset();
+    u->de.clear();
+    int e = build_pass(pcs, u->own, u->pass);
+    if (!e && !u->dec.empty()) e = plan_de(u->dec, u->own, u->de);
+    if (!e && !u->dec.empty()) e = pack_trials(u->de, u->dec, u->ntrial.data(), ip, rotm, u->own, u->L);
+    if (e != MSG_ERR_NOMEM) return e;
+    if (int e2 = u->own.reserve(2 * u->own.cap)) return e2;
+  }
+  return set_error(MSG_ERR_NOMEM, "unit plan does not fit its arena");
+}
+
+}  // namespace
+
+extern "C" {
+
+void msg_unit_destroy(msg_unit* u) {
+  if (!u) return;
+  if (u->s) hipStreamSynchronize(u->s);
+  if (u->d_de) hipFree(u->d_de);
+  if (u->h_outs) hipHostFree(u->h_outs);
+  if (u->h_de) hipHostFree(u->h_de);
+  if (u->h_f) hipHostFree(u->h_f);
+  if (u->fork) hipEventDestroy(u->fork);
+  if (u->join) hipEventDestroy(u->join);
+  if (u->done) hipEventDestroy(u->done);
+  if (u->side) hipStreamDestroy(u->side);
+  if (u->s) hipStreamDestroy(u->s);
+  delete u;
+}
+
+int msg_unit_create(msg_unit** out) {
+  if (!out) return set_error(MSG_ERR_ARG, "msg_unit_create: null output");
+  msg_unit* u = new msg_unit();
+  *out = nullptr;
+  if (hipStreamCreateWithFlags(&u->s, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&u->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&u->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&u->join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&u->done, hipEventDisableTiming) != hipSuccess) {
+    msg_unit_destroy(u);
+    return set_error(MSG_ERR_HIP, "msg_unit_create: stream/event creation failed");
+  }
+  if (int e = u->own.init_arena(4 << 20)) {
+    msg_unit_destroy(u);
+    return e;
+  }
+  *out = u;
+  return MSG_OK;
+}
+
+int msg_unit_query(msg_unit* u) {
+  if (!u) return set_error(MSG_ERR_ARG, "msg_unit_query: null unit");
+  if (!u->launched) return 1;
+  const hipError_t e = hipEventQuery(u->done);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return set_error(MSG_ERR_HIP, std::string("msg_unit_query: ") + hipGetErrorString(e));
+}
+
+// Launches (asynchronously) a UTotal pass over ut[0..n_u) (forces downloaded when
+// want_forces) and ntrial[s] trials of each dec[s] (ip 1-based and rotm concatenated over
+// dec).  A system may appear in both lists (its pass runs first), but not twice in one.
+int msg_unit_launch(msg_unit* u, msg_ctx** ut, int n_u, int want_forces, msg_ctx** dec, int n_d,
+                    const int* ntrial, const int* ip, const double* rotm) {
+  if (!u || n_u < 0 || n_d < 0 || (n_u && !ut) || (n_d && (!dec || !ntrial || !ip || !rotm)))
+    return set_error(MSG_ERR_ARG, "msg_unit_launch: bad arguments");
+  if (u->launched) {
+    const hipError_t q = hipEventQuery(u->done);
+    if (q == hipErrorNotReady) return set_error(MSG_ERR_STATE, "msg_unit_launch: unit still running");
+    if (q != hipSuccess) return set_error(MSG_ERR_HIP, std::string("msg_unit_launch: ") + hipGetErrorString(q));
+  }
+  u->launched = false;
+  u->ut.clear();
+  u->dec.clear();
+  u->ntrial.assign(ntrial, ntrial + n_d);
+  std::vector<Ctx*> pcs;
+  for (int i = 0; i < n_u; ++i) {
+    if (!ut[i]) return set_error(MSG_ERR_ARG, "msg_unit_launch: null context");
+    u->ut.push_back(&ut[i]->c);
+  }
+  for (int i = 0; i < n_d; ++i) {
+    if (!dec[i]) return set_error(MSG_ERR_ARG, "msg_unit_launch: null context");
+    u->dec.push_back(&dec[i]->c);
+  }
+  for (size_t i = 0; i < u->ut.size(); ++i)
+    for (size_t j = 0; j < i; ++j)
+      if (u->ut[i] == u->ut[j]) return set_error(MSG_ERR_ARG, "msg_unit_launch: system listed twice");
+  for (size_t i = 0; i < u->dec.size(); ++i)
+    for (size_t j = 0; j < i; ++j)
+      if (u->dec[i] == u->dec[j]) return set_error(MSG_ERR_ARG, "msg_unit_launch: system listed twice");
+  pcs = u->ut;
+  for (Ctx* c : u->dec)
+    if (!c->fresh && std::find(pcs.begin(), pcs.end(), c) == pcs.end()) pcs.push_back(c);
+  for (Ctx* c : u->dec)
+    if (int e = check_batchable(c)) return e;
+  if (int e = plan_unit(u, pcs, ip, rotm)) return e;
+  // buffers: trial outputs on the device, pinned host copies of everything downloaded
+  const long ntot = u->dec.empty() ? 0 : u->L.ntot;
+  if (ntot > u->de_cap) {
+    if (u->d_de) hipFree(u->d_de);
+    u->d_de = nullptr;
+    u->de_cap = 0;
+    const long cap = std::max(ntot, 4096L);
+    MSG_HIP(hipMalloc((void**)&u->d_de, cap * kDeStride * sizeof(double)));
+    u->de_cap = cap;
+  }
+  u->want_forces = want_forces != 0;
+  u->foff.assign(u->ut.size() + 1, 0);
+  for (size_t i = 0; i < u->ut.size(); ++i) u->foff[i + 1] = u->foff[i] + 3L * u->ut[i]->np;
+  if (int e = grow_pinned(&u->h_outs, &u->outs_cap, 64L * (long)pcs.size())) return e;
+  if (int e = grow_pinned(&u->h_de, &u->hde_cap, ntot * kDeStride)) return e;
+  if (u->want_forces)
+    if (int e = grow_pinned(&u->h_f, &u->hf_cap, u->foff.back())) return e;
+  // launch
+  std::vector<Ctx*> all = pcs;
+  for (Ctx* c : u->dec) all.push_back(c);
+  if (int e = order_after_contexts(all, u->s)) return e;
+  if (int e = u->own.flush(u->s)) return e;
+  if (int e = launch_pass(u->pass, u->s, u->side, u->fork, u->join)) return e;
+  for (Ctx* c : pcs) c->fresh = true;
+  if (ntot)
+    if (int e = launch_trials(u->de, u->L, u->d_de, u->s)) return e;
+  if (!pcs.empty())
+    MSG_HIP(hipMemcpyAsync(u->h_outs, u->pass.d_outs, 64 * pcs.size() * sizeof(double), hipMemcpyDeviceToHost, u->s));
+  if (ntot)
+    MSG_HIP(hipMemcpyAsync(u->h_de, u->d_de, ntot * kDeStride * sizeof(double), hipMemcpyDeviceToHost, u->s));
+  if (u->want_forces)
+    for (size_t i = 0; i < u->ut.size(); ++i)
+      MSG_HIP(hipMemcpyAsync(u->h_f + u->foff[i], u->ut[i]->d_force, 3 * u->ut[i]->np * sizeof(double),
+                             hipMemcpyDeviceToHost, u->s));
+  MSG_HIP(hipEventRecord(u->done, u->s));
+  u->launched = true;
+  return MSG_OK;
+}
+
+// Waits for the unit and copies its results: u_tot, u_rec, virial of the n_u UTotal
+// systems, their forces (concatenated force(3,np) per system; needs want_forces), and the
+// overlap flags and energy changes of all trials (in launch order).  Any may be null.
+int msg_unit_results(msg_unit* u, double* u_tot, double* u_rec, double* virial, double* force,
+                     int* ovl, double* du_tot) {
+  if (!u || !u->launched) return set_error(MSG_ERR_STATE, "msg_unit_results: nothing launched");
+  MSG_HIP(hipEventSynchronize(u->done));
+  for (size_t s = 0; s < u->pass.cs.size(); ++s) {
+    const bool want = s < u->ut.size();
+    if (int e = read_out(u->h_outs + 64 * s, u->pass.cs[s], (int)s, want ? u_tot : nullptr,
+                         want ? u_rec : nullptr, want ? virial : nullptr))
+      return e;
+  }
+  if (force && u->want_forces) std::copy(u->h_f, u->h_f + u->foff.back(), force);
+  if (!u->dec.empty() && ovl && du_tot) unpack_trials(u->dec, u->ntrial.data(), u->L, u->h_de, ovl, du_tot);
+  return MSG_OK;
+}
+
 // Completion query of a context's stream (1 = idle), for request loops that poll.
 int msg_query(msg_ctx* h) {
   if (!h) return set_error(MSG_ERR_ARG, "msg_query: null context");

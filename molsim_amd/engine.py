@@ -118,6 +118,12 @@ def lib():
         L.msg_batch_sync.argtypes = [_vp]
         L.msg_batch_dutotal.argtypes = [_vp, _ip, _ip, _dp, _ip, _dp]
         L.msg_batch_results.argtypes = [_vp, _vp, _vp, _vp]
+        L.msg_unit_create.argtypes = [C.POINTER(_vp)]
+        L.msg_unit_destroy.argtypes = [_vp]
+        L.msg_unit_destroy.restype = None
+        L.msg_unit_launch.argtypes = [_vp, _vp, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, _vp]
+        L.msg_unit_query.argtypes = [_vp]
+        L.msg_unit_results.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp]
         L.msg_batch_query.argtypes = [_vp]
         _lib = L
     return _lib
@@ -438,3 +444,76 @@ class Batch:
             out.append(dict(overlap=ovl[off:off + n].astype(bool), du_tot=du[off:off + n]))
             off += n
         return out
+
+
+class Unit:
+    """Execution unit of the request loop: one batch of requests at a time (a UTotal pass over
+    some resident systems, MC trials of others) as segmented launches on its own streams."""
+
+    def __init__(self):
+        h = _vp()
+        _check(lib().msg_unit_create(C.byref(h)))
+        self.h = h
+        self._keep = None
+        self._shape = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().msg_unit_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def launch(self, ef_engines, de_engines=(), trials=(), want_forces: bool = True):
+        """ef_engines: systems for energy + forces; de_engines with trials[s] = (ips 0-based,
+        rnew (n, 3)).  Returns at once."""
+        ef = list(ef_engines)
+        de = list(de_engines)
+        ua = (_vp * max(1, len(ef)))(*[e.h for e in ef])
+        da = (_vp * max(1, len(de)))(*[e.h for e in de])
+        nt = np.array([len(t[0]) for t in trials], dtype=np.int32) if de else np.zeros(1, np.int32)
+        if de:
+            ip = np.ascontiguousarray(np.concatenate([np.asarray(t[0]) for t in trials]) + 1, dtype=np.int32)
+            r = np.ascontiguousarray(np.concatenate([np.asarray(t[1]).reshape(-1, 3) for t in trials]),
+                                     dtype=np.float64).reshape(-1)
+        else:
+            ip = np.zeros(1, np.int32)
+            r = np.zeros(3)
+        _check(lib().msg_unit_launch(self.h, ua, len(ef), int(want_forces), da, len(de),
+                                     _ptr(nt), _ptr(ip), _ptr(r)))
+        self._keep = (ua, da, nt, ip, r)
+        self._shape = (ef, de, nt[:len(de)].copy(), want_forces)
+
+    def done(self) -> bool:
+        rc = lib().msg_unit_query(self.h)
+        _check(min(rc, 0))
+        return bool(rc)
+
+    def results(self):
+        """(per ef system: dict(u_tot, u_rec, virial, force)), (per de system: dict(overlap, du_tot))."""
+        ef, de, nt, wf = self._shape
+        n = len(ef)
+        ut, ur, vi = np.zeros(max(n, 1)), np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        nf = sum(3 * e.spec.n for e in ef)
+        f = np.zeros(max(nf, 1)) if wf else None
+        ntot = int(nt.sum())
+        ovl = np.zeros(max(ntot, 1), dtype=np.int32)
+        du = np.zeros(max(ntot, 1))
+        _check(lib().msg_unit_results(self.h, _ptr(ut), _ptr(ur), _ptr(vi),
+                                      _ptr(f) if f is not None else None, _ptr(ovl), _ptr(du)))
+        out_ef, off = [], 0
+        for i, e in enumerate(ef):
+            d = dict(u_tot=ut[i], u_rec=ur[i], virial=vi[i], force=None)
+            if f is not None:
+                d["force"] = f[off:off + 3 * e.spec.n].reshape(-1, 3).copy()
+                off += 3 * e.spec.n
+            out_ef.append(d)
+        out_de, off = [], 0
+        for k in nt:
+            out_de.append(dict(overlap=ovl[off:off + k].astype(bool), du_tot=du[off:off + k].copy()))
+            off += k
+        return out_ef, out_de

@@ -336,3 +336,48 @@ def ensemble(nsys: int = 512, seed: int = 4, nmin: int = 200, nmax: int = 20000,
         with mp.get_context("fork").Pool(workers) as pool:
             return pool.map(_member_args, args, chunksize=4)
     return [_member_args(a) for a in args]
+
+
+def served_system(seed: int, n: int, nbead_max: int = 256, order: int = 6, alpha: float = 0.15,
+                  rcut: float = 20.0, lb: float = 7.1, h: float = 3.125) -> SystemSpec:
+    """The polyelectrolyte composition rescaled to n particles: half of them -1 beads in
+    random-walk chains of up to 256 beads (5 A bonds), half +1 counterions, at the 400 A /
+    32768-particle density (box 400 (n / 32768)^(1/3) A); SPME order 6 with the mesh spacing
+    of 128^3 in 400 A (K the smallest power of two with L/K <= 3.125 A, 16..256), alpha
+    0.15/A, rc 20 A cell list, LJ (sigma 4 A, eps 0.1 kT) + hard core + Coulomb table."""
+    rng = np.random.default_rng(seed)
+    L = 400.0 * (n / 32768.0) ** (1.0 / 3.0)
+    nb_total = n // 2
+    nchain = max(1, int(math.ceil(nb_total / nbead_max)))
+    nbead = max(1, nb_total // nchain)
+    nb = nchain * nbead
+    beads = _random_walk_chains_batch(rng, nchain, nbead, 5.0, L, 4.0)
+    ions = _insert_uniform(rng, nb, L, 4.0, existing=beads)
+    ro = np.concatenate([beads, ions])
+    ipt = np.array([0] * nb + [1] * nb, dtype=np.int32)
+    az = np.where(ipt == 0, -1.0, 1.0)
+    relp = relpermitt_from_bjerrum(lb)
+    eps = 0.1 / beta_kjmol()
+    pot = _pot(2, [-1.0, 1.0], 1, rcut, relp, alpha, sig=[4.0, 4.0], eps=[eps, eps], rad=[2.0, 2.0])
+    K = 16
+    while K < 256 and L / K > h:
+        K *= 2
+    return SystemSpec(f"served_{ro.shape[0]}_spme{K}", ro, ipt, az, np.array([L, L, L]), pot, rcut,
+                      list_mode="cell", ewald="spm", alpha=alpha, nmesh=K, order=order, lsurf=True)
+
+
+def served_pool(nsys: int = 48, seed: int = 5, nmin: int = 100, nmax: int = 100000,
+                workers: int = 1) -> dict:
+    """{sys_id: spec} with particle counts log-uniform in [nmin, nmax]."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(math.log(nmin), math.log(nmax), size=nsys))
+    seeds = rng.integers(1, 2 ** 31 - 1, size=nsys)
+    args = [(int(seeds[i]), max(nmin, int(round(ns[i] / 2)) * 2)) for i in range(nsys)]
+    if workers > 1:
+        import multiprocessing as mp
+
+        with mp.get_context("fork").Pool(workers) as pool:
+            specs = pool.starmap(served_system, args, chunksize=2)
+    else:
+        specs = [served_system(*a) for a in args]
+    return dict(enumerate(specs))
