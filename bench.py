@@ -66,7 +66,7 @@ def timed_steps(eng, steps: int) -> float:
     return time.perf_counter() - t0
 
 
-def run_ensemble(args):
+def run_ensemble(args, specs=None):
     """Batched ensemble: 512 independent electrolyte / polyelectrolyte systems (N log-uniform
     in 200..20000, volume fraction 0.01, SPME 16^3..64^3 by box length, +-1/+-2 ions), all
     UTotal passes in one captured graph per step, pair tiles from the autotuner; 1000 MC
@@ -74,7 +74,8 @@ def run_ensemble(args):
     system at a time (eager kernels, a host synchronisation per system)."""
     from molsim_amd import autotune, systems
     t0 = time.perf_counter()
-    specs = systems.ensemble(args.nsys, workers=8)   # forked workers: before any GPU call
+    if specs is None:
+        specs = systems.ensemble(args.nsys, workers=8)   # forked workers: before any GPU call
     gen_s = time.perf_counter() - t0
     from molsim_amd.engine import Batch, Engine, build_table
     table = build_table(specs[0].pot)
@@ -87,7 +88,7 @@ def run_ensemble(args):
           f"{sum(e.device_bytes() for e in engs) / 2**30:.2f} GiB resident",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    tune = autotune.tune(engs, reps=3)
+    tune = autotune.tune_batched(engs, reps=3)
     autotune.apply(engs, tune)
     tune_s = time.perf_counter() - t0
     print(f"# autotune {tune_s:.1f} s: " + ", ".join(
@@ -171,8 +172,8 @@ def run_ensemble(args):
                               for k in (16, 32, 64)},
                    "rcut": specs[0].rcut, "alpha": specs[0].alpha, "order": specs[0].order,
                    "parallelism": "single GPU"},
-        "autotune": {str(b): {"n": v["n"], "tile": v["tile"],
-                              "pair_us": {str(t): round(1e3 * ms, 2) for t, ms in v["ms"].items()}}
+        "autotune": {str(b): {"n_median": v["n"], "systems": v["members"], "tile": v["tile"],
+                              "bucket_pass_ms": {str(t): round(ms, 3) for t, ms in v["ms"].items()}}
                      for b, v in sorted(tune.items())},
         "fp64_gpu_path": {"value": value64, "ms_per_ensemble_pass": 1e3 * t_sum,
                           "speedup_of_batched": value / value64,
@@ -186,6 +187,252 @@ def run_ensemble(args):
     }
 
 
+def run_serve(args, pool=None):
+    """Served request stream: a pool of resident systems (the polyelectrolyte composition
+    rescaled to N log-uniform in 100..100000), Poisson arrivals of requests for energy +
+    forces or for 256 single-move trial energies, continuous batching over two execution
+    units with HBM-footprint admission.  Beside it: the plain FP64 one-at-a-time path (each
+    request alone, trials one DUTotal call each), whose per-request service times are
+    measured and turned into latencies by the FIFO recursion over the same arrivals."""
+    from molsim_amd import serve, systems
+    t0 = time.perf_counter()
+    if pool is None:
+        pool = systems.served_pool(args.nsys_serve, workers=8)     # forked: before any GPU call
+    gen_s = time.perf_counter() - t0
+    from molsim_amd.engine import Engine, build_table
+    table = build_table(pool[0].pot)
+    ns = np.array([sp.n for sp in pool.values()])
+    print(f"# serve: {len(pool)} systems, N {ns.min()}..{ns.max()} (generated in {gen_s:.1f} s)",
+          file=sys.stderr, flush=True)
+
+    # plain FP64 path: service time of each (system, kind), median of 3
+    rng = np.random.default_rng(11)
+    svc = {}
+    for sid, sp in pool.items():
+        e = Engine(sp, precision="fp64", table=table)
+        e.utotal()
+        ts = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            e.utotal()
+            ts.append(time.perf_counter() - t1)
+        svc[(sid, "ef")] = float(np.median(ts))
+        ips = rng.integers(0, sp.n, size=args.ntrial_serve)
+        rn = sp.ro[ips] + rng.uniform(-1.0, 1.0, size=(args.ntrial_serve, 3))
+        e.dutotal(int(ips[0]), rn[0])          # one-time trial set-up is not request time
+        ts = []
+        for _ in range(2):
+            t1 = time.perf_counter()
+            for k in range(len(ips)):
+                e.dutotal(int(ips[k]), rn[k])
+            ts.append(time.perf_counter() - t1)
+        svc[(sid, "de")] = float(np.median(ts))
+        e.close()
+    mean_svc = 0.5 * np.mean([v for (s_, k), v in svc.items() if k == "ef"]) + \
+        0.5 * np.mean([v for (s_, k), v in svc.items() if k == "de"])
+    cap64 = 1.0 / mean_svc
+    print(f"# plain FP64 path: mean service {1e3 * mean_svc:.2f} ms -> capacity {cap64:.1f} req/s",
+          file=sys.stderr, flush=True)
+
+    def fifo(reqs):
+        end, lat = 0.0, []
+        for r in sorted(reqs, key=lambda q: q.t_arrival):
+            start = max(r.t_arrival, end)
+            end = start + svc[(r.sys_id, r.kind)]
+            lat.append(end - r.t_arrival)
+        lat = np.array(lat) * 1e3
+        return {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                "throughput_rps": len(reqs) / max(end, reqs[-1].t_arrival)}
+
+    srv = serve.Server(pool, table, hbm_budget=int(args.hbm_gb * 2 ** 30), units=args.units,
+                       max_per_unit=64)
+    # residency warm-up (not timed): one energy request and one trial request per system
+    warm = serve.request_stream(pool, qps=1e6, duration=1e-3, seed=1, ntrial=4)[:0]
+    for i, (sid, sp) in enumerate(pool.items()):
+        warm.append(serve.Request(-1 - 2 * i, sid, "ef", 0.0))
+        warm.append(serve.Request(-2 - 2 * i, sid, "de", 0.0, ips=np.arange(4) % sp.n,
+                                  rnew=sp.ro[np.arange(4) % sp.n].copy()))
+    srv.run(warm)
+    srv.done.clear()
+    print(f"# resident: {len(srv.resident)} systems, {srv.bytes / 2 ** 30:.2f} GiB admitted",
+          file=sys.stderr, flush=True)
+    levels = []
+    for mult in args.qps_mult:
+        qps = mult * cap64
+        reqs = serve.request_stream(pool, qps=qps, duration=args.duration, seed=int(100 * mult),
+                                    frac_ef=0.5, ntrial=args.ntrial_serve)
+        base = fifo([serve.Request(r.rid, r.sys_id, r.kind, r.t_arrival) for r in reqs])
+        st = srv.run(reqs)
+        srv.done.clear()
+        lv = {"qps": qps, "qps_over_fp64_capacity": mult, "requests": st["requests"],
+              "served": {"p50_ms": st["latency_ms"]["p50"], "p99_ms": st["latency_ms"]["p99"],
+                         "throughput_rps": st["throughput_rps"],
+                         "p99_ms_ef": st.get("latency_ms_ef", {}).get("p99"),
+                         "p99_ms_de": st.get("latency_ms_de", {}).get("p99")},
+              "fp64_one_at_a_time": {"p50_ms": base["p50"], "p99_ms": base["p99"],
+                                     "throughput_rps": base["throughput_rps"],
+                                     "stable": mult < 1.0}}
+        levels.append(lv)
+        print(f"# qps {qps:9.1f}: served p50 {st['latency_ms']['p50']:.3f} p99 {st['latency_ms']['p99']:.3f} ms "
+              f"({st['throughput_rps']:.0f} rps) | fp64 serial p50 {base['p50']:.2f} p99 {base['p99']:.2f} ms",
+              file=sys.stderr, flush=True)
+    evictions, deferred = srv.evictions, srv.deferred
+    srv.close()
+    main_lv = levels[min(1, len(levels) - 1)]
+    return {
+        "metric": "served SPME E+F / 256-move dE requests: p50/p99 latency and throughput at fixed QPS",
+        "value": main_lv["served"]["throughput_rps"], "unit": "requests/s", "n_gpus": 1,
+        "steps": main_lv["requests"], "warmup": len(pool), "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64acc",
+        "data": "synthetic (polyelectrolyte composition rescaled, seeded Poisson arrivals)",
+        "config": {"workload": f"served_{len(pool)}", "systems": len(pool), "n_min": int(ns.min()),
+                   "n_max": int(ns.max()), "ntrial": args.ntrial_serve, "frac_ef": 0.5,
+                   "units": args.units, "hbm_budget_gib": args.hbm_gb, "duration_s": args.duration,
+                   "parallelism": "single GPU"},
+        "levels": levels,
+        "fp64_capacity_rps": cap64,
+        "evictions": evictions, "deferred": deferred,
+    }
+
+
+def run_electrolyte_mc(args):
+    """The 1000-ion primitive-model electrolyte (std Ewald, rc = L/2, all pairs, FP64):
+    UTotal with forces, then 10^4 single-particle MC trials of at most 1 A per axis, as one
+    MCPass on the device (SPartMove / Metropolis / MCUpdate with MOLSIM's generator), and the
+    same 10^4 trials as independent DUTotal evaluations against the start configuration.
+    CPU leg: the oracle's DUTotal on a bounded sample of the trials."""
+    from molsim_amd import systems
+    from molsim_amd.engine import Engine, Random, build_table
+    import oracle
+    sp = systems.electrolyte_primitive()
+    table = build_table(sp.pot)
+    beta = systems.beta_kjmol()
+    out = {}
+    for prec in ("fp64", "mixed"):
+        eng = Engine(sp, precision=prec, table=table)
+        res = eng.utotal()
+        t0 = time.perf_counter()
+        nrep = 20
+        for _ in range(nrep):
+            eng.utotal()
+        t_ut = (time.perf_counter() - t0) / nrep
+        rng = np.random.default_rng(5)
+        ips = rng.integers(0, sp.n, size=args.mc_trials)
+        rn = sp.ro[ips] + rng.uniform(-1.0, 1.0, size=(args.mc_trials, 3))
+        rn = (rn + sp.box[0] / 2) % sp.box[0] - sp.box[0] / 2
+        eng.dutotal_batch(ips[:16], rn[:16])
+        t0 = time.perf_counter()
+        de = eng.dutotal_batch(ips, rn)
+        t_de = time.perf_counter() - t0
+        gen = Random(-12345)
+        t0 = time.perf_counter()
+        dsum, nacc, ev = eng.mc_pass(args.mc_trials, 2.0, beta, gen, isamp=0)
+        t_mc = time.perf_counter() - t0
+        out[prec] = {"utotal_ms": 1e3 * t_ut, "dE_batch_trials_per_s": args.mc_trials / t_de,
+                     "mc_pass_trials_per_s": args.mc_trials / t_mc, "mc_accepted": int(nacc),
+                     "u_tot": res["u_tot"], "force": res["force"], "de": de}
+        print(f"# {prec}: UTotal {1e3 * t_ut:.3f} ms, batched dE {args.mc_trials / t_de:.3e} trials/s, "
+              f"MCPass {args.mc_trials / t_mc:.3e} trials/s ({nacc} accepted)", file=sys.stderr, flush=True)
+        eng.close()
+    # oracle: forces, a bounded sample of trials (time and error)
+    otab = oracle.build_table(sp.pot)
+    osys = oracle.System(sp.ro, sp.ipt, sp.az, sp.box, sp.rcut, sp.pot["epsi4pi"], otab, npt=sp.npt,
+                         list_mode=0, ewald=1, surf=sp.lsurf, alpha=sp.alpha, ncut=sp.ncut)
+    ref = osys.utotal()
+    rng = np.random.default_rng(5)
+    ips = rng.integers(0, sp.n, size=args.mc_trials)
+    rn = sp.ro[ips] + rng.uniform(-1.0, 1.0, size=(args.mc_trials, 3))
+    rn = (rn + sp.box[0] / 2) % sp.box[0] - sp.box[0] / 2
+    t0 = time.perf_counter()
+    nref, errs = 0, {"fp64": 0.0, "mixed": 0.0}
+    while nref < args.mc_trials and time.perf_counter() - t0 < args.cpu_seconds:
+        r = osys.dutotal(int(ips[nref]), rn[nref])
+        if not r["overlap"]:
+            for prec in errs:
+                errs[prec] = max(errs[prec], abs(out[prec]["de"]["du_tot"][nref] - r["du_tot"]))
+        nref += 1
+    t_cpu = time.perf_counter() - t0
+    fr = ref["force"]
+    return {
+        "metric": "primitive electrolyte: DUTotal trials/s (10^4 single-particle moves), std Ewald FP64",
+        "value": out["fp64"]["dE_batch_trials_per_s"], "unit": "trials/s", "n_gpus": 1,
+        "steps": args.mc_trials, "warmup": 16, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded uniform insertion with overlap rejection)",
+        "config": {"workload": "electrolyte_1000_stdewald", "n": sp.n, "box": float(sp.box[0]),
+                   "alpha": sp.alpha, "ncut": sp.ncut, "rcut": sp.rcut, "list": "all pairs",
+                   "parallelism": "single GPU"},
+        "paths": {p: {k: v for k, v in d.items() if k not in ("force", "de")} for p, d in out.items()},
+        "err_vs_fp64_oracle": {
+            "force_rel_l2_fp64": float(np.linalg.norm(out["fp64"]["force"] - fr) / np.linalg.norm(fr)),
+            "force_rel_l2_mixed": float(np.linalg.norm(out["mixed"]["force"] - fr) / np.linalg.norm(fr)),
+            "dE_max_abs_fp64_kJmol": errs["fp64"], "dE_max_abs_mixed_kJmol": errs["mixed"],
+            "dE_trials_checked": nref},
+        "cpu_baseline": {"value": nref / t_cpu, "unit": "trials/s", "cores": 1, "kind": "port",
+                         "sample": f"{nref} oracle DUTotal trials ({t_cpu:.1f} s)"},
+    }
+
+
+def run_ionic_md(args):
+    """The 1,048,576-particle ionic fluid: SPME 256^3 order 6, rc 18 A Verlet list with a
+    2 A skin, velocity-Verlet MD on the device (dt 0.005, masses 20 g/mol, velocities at
+    298 K), the list rebuilt on skin violation.  value = particle-steps/s over the timed
+    steps.  Beside it: the plain FP64 path over a few steps, and one oracle UTotal of the
+    same system on the host (CPU leg)."""
+    from molsim_amd import systems
+    from molsim_amd.engine import Engine, build_table
+    t0 = time.perf_counter()
+    sp = systems.ionic_fluid()
+    gen_s = time.perf_counter() - t0
+    print(f"# {sp.name}: N={sp.n} L={sp.box[0]:.1f} (generated in {gen_s:.1f} s)", file=sys.stderr, flush=True)
+    table = build_table(sp.pot)
+    mass = [20.0, 20.0]
+    rng = np.random.default_rng(4)
+    kT = 2.479  # kJ/mol at 298 K
+    v0 = rng.normal(size=sp.ro.shape) * np.sqrt(kT * 100.0 / 20.0)
+    res = {}
+    for prec, nsteps in (("mixed", args.md_steps), ("fp64", args.md_fp64_steps)):
+        eng = Engine(sp, precision=prec, table=table, force_scale=100.0)
+        eng.md_init(mass, v0)
+        eng.md_run(args.md_warmup, 0.005, log=False)
+        b0 = eng.nlist_builds()
+        t0 = time.perf_counter()
+        log = eng.md_run(nsteps, 0.005, log=True)
+        dt = time.perf_counter() - t0
+        etot = log[:, 0] * 100.0 + log[:, 1]
+        res[prec] = {"particle_steps_per_s": sp.n * nsteps / dt, "ms_per_step": 1e3 * dt / nsteps,
+                     "steps": nsteps, "list_builds": eng.nlist_builds() - b0,
+                     "energy_drift_rel_K": float(np.abs(etot - etot[0]).max() / abs(log[0, 1]))}
+        if prec == "mixed":
+            res_u = eng.utotal()
+        print(f"# {prec}: {1e3 * dt / nsteps:.3f} ms/step, {res[prec]['list_builds']} rebuilds",
+              file=sys.stderr, flush=True)
+        eng.close()
+    out = {
+        "metric": "ionic fluid MD: SPME E+F particle-steps/s (velocity Verlet, Verlet list)",
+        "value": res["mixed"]["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.md_steps, "warmup": args.md_warmup, "ms_per_step": res["mixed"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64acc",
+        "data": "synthetic (seeded uniform insertion with overlap rejection, Maxwell velocities)",
+        "config": {"workload": sp.name, "n": sp.n, "box": float(sp.box[0]), "nmesh": sp.nmesh,
+                   "order": sp.order, "rcut": sp.rcut, "skin": sp.drnlist, "dt": 0.005,
+                   "parallelism": "single GPU"},
+        "paths": res,
+    }
+    if not args.no_cpu:
+        import oracle
+        otab = oracle.build_table(sp.pot)
+        osys = oracle.System(sp.ro, sp.ipt, sp.az, sp.box, sp.rcut, sp.pot["epsi4pi"], otab,
+                             npt=sp.npt, list_mode=1, ewald=2, surf=sp.lsurf, alpha=sp.alpha,
+                             K=sp.nmesh, order=sp.order)
+        t0 = time.perf_counter()
+        osys.utotal()
+        tcpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": sp.n / tcpu, "unit": "particle-steps/s", "cores": 1,
+                               "kind": "port", "sample": f"one oracle UTotal of the full system ({tcpu:.1f} s)"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,17 +444,34 @@ def main():
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps of the timed path only (for rocprofv3)")
     ap.add_argument("--workload", default="polyelectrolyte",
-                    choices=["polyelectrolyte", "ensemble"])
+                    choices=["polyelectrolyte", "ensemble", "serve", "electrolyte_mc", "ionic_md"])
+    ap.add_argument("--mc-trials", type=int, default=10000, help="electrolyte_mc: MC trials")
+    ap.add_argument("--md-steps", type=int, default=1000, help="ionic_md: timed MD steps")
+    ap.add_argument("--md-warmup", type=int, default=10, help="ionic_md: untimed MD steps")
+    ap.add_argument("--md-fp64-steps", type=int, default=20, help="ionic_md: FP64 path steps")
+    ap.add_argument("--nsys-serve", type=int, default=48, help="serve: resident pool size")
+    ap.add_argument("--ntrial-serve", type=int, default=256, help="serve: moves per dE request")
+    ap.add_argument("--units", type=int, default=2, help="serve: execution units in flight")
+    ap.add_argument("--hbm-gb", type=float, default=8.0, help="serve: HBM budget (GiB)")
+    ap.add_argument("--duration", type=float, default=3.0, help="serve: seconds per QPS level")
+    ap.add_argument("--qps-mult", type=float, nargs="+", default=[0.6, 6.0, 30.0],
+                    help="serve: QPS levels as multiples of the FP64 path's capacity")
     ap.add_argument("--nsys", type=int, default=512, help="ensemble: number of systems")
+    ap.add_argument("--ens-steps", type=int, default=30, help="ensemble steps in the default run")
+    ap.add_argument("--quick", action="store_true",
+                    help="default workload only (no ensemble / served-stream summaries)")
     ap.add_argument("--ntrial", type=int, default=1000, help="ensemble: MC trials per system")
     ap.add_argument("--fp64-trials", type=int, default=20,
                     help="ensemble: trials per system timed on the plain FP64 path")
     args = ap.parse_args()
+    if args.profile_only:
+        args.quick = True
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
-    if args.workload == "ensemble":
-        out = run_ensemble(args)
+    if args.workload != "polyelectrolyte":
+        out = {"ensemble": run_ensemble, "serve": run_serve, "electrolyte_mc": run_electrolyte_mc,
+               "ionic_md": run_ionic_md}[args.workload](args)
         if out:
             print(json.dumps(out), flush=True)
         return
@@ -217,6 +481,10 @@ def main():
 
     t_gen = time.perf_counter()
     sp = systems.polyelectrolyte()
+    # the ensemble and the served pool are generated here, in forked workers, before any
+    # GPU call of this process
+    ens_specs = None if args.quick else systems.ensemble(args.nsys, workers=8)
+    serve_pool = None if args.quick else systems.served_pool(args.nsys_serve, workers=8)
     table = build_table(sp.pot)
     print(f"# workload {sp.name}: N={sp.n} K={sp.nmesh} order={sp.order} "
           f"(generated in {time.perf_counter() - t_gen:.1f} s)", file=sys.stderr, flush=True)
@@ -328,7 +596,21 @@ def main():
             "u_tot_rel": abs(res["u_tot"] - ref["u_tot"]) / abs(ref["u_tot"]),
             "u_rec_rel": abs(res["u_rec"] - ref["u_rec"]) / abs(ref["u_rec"]),
         }
+    if not args.quick:
+        # the batched ensemble and the served request stream, summarised in the same line
+        ens = run_ensemble(args_for(args, steps=args.ens_steps, warmup=3), ens_specs)
+        out["ensemble"] = {k: ens[k] for k in ("value", "unit", "ms_per_step", "fp64_gpu_path", "dE",
+                                              "err_vs_fp64_path", "config")}
+        srv = run_serve(args_for(args, qps_mult=args.qps_mult[:2], duration=2.0), serve_pool)
+        out["serve"] = {k: srv[k] for k in ("levels", "fp64_capacity_rps", "config")}
     print(json.dumps(out), flush=True)
+
+
+def args_for(args, **kw):
+    a = argparse.Namespace(**vars(args))
+    for k, v in kw.items():
+        setattr(a, k, v)
+    return a
 
 
 if __name__ == "__main__":

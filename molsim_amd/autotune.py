@@ -1,5 +1,9 @@
 """Pair-kernel tile autotuner, per system-size bucket.
 
+Two objectives: ``tune`` (latency of one system's pair kernel, for systems served alone)
+and ``tune_batched`` (duration of the batched pass over all of a bucket's systems, for the
+ensemble).
+
 The pair kernel of a cell-list system takes its tile as a run-time choice
 (``msg_set_tile``): ``pair_lanes`` lanes of a wavefront share one home particle's
 27-cell candidate list (256 / pair_lanes homes per workgroup), or 0 for the
@@ -59,6 +63,44 @@ def tune(engines, tiles=TILES, reps: int = 5) -> dict:
         rep.set_tile(old)
         best = min(ms, key=ms.get)
         table[b] = dict(tile=best, n=rep.spec.n, ms=ms)
+    return table
+
+
+def tune_batched(engines, tiles=TILES, reps: int = 5) -> dict:
+    """Like ``tune`` but for throughput inside a batched pass: per size bucket, every tile is
+    set on ALL members of the bucket and the bucket's batched pass (one graph launch) is
+    timed; the tile with the shortest pass wins.  Table entries carry the pass time (ms)
+    per tile."""
+    import time
+
+    from .engine import Batch
+
+    groups: dict[int, list] = {}
+    for e in engines:
+        groups.setdefault(bucket(e.spec.n), []).append(e)
+    table = {}
+    for b, members in sorted(groups.items()):
+        old = [e.tile() for e in members]
+        ms = {}
+        for t in tiles:
+            for e in members:
+                e.set_tile(t)
+            batch = Batch(members)
+            batch.utotal_async()
+            batch.sync()
+            best = float("inf")
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                batch.utotal_async()
+                batch.sync()
+                best = min(best, time.perf_counter() - t0)
+            batch.close()
+            ms[t] = 1e3 * best
+        for e, t in zip(members, old):
+            e.set_tile(t)
+        best_t = min(ms, key=ms.get)
+        table[b] = dict(tile=best_t, n=int(sorted(e.spec.n for e in members)[len(members) // 2]),
+                        ms=ms, members=len(members))
     return table
 
 

@@ -10,6 +10,7 @@
 //
 // DUTotal: the trials of all systems run as one segmented launch per spline order, with one
 // upload of the trial inputs and one download of the results.
+#include <cstdlib>
 #include <vector>
 
 #include "common.h"
@@ -71,13 +72,29 @@ static int run_stage(const SegStage& st, hipStream_t s) {
 
 // The batched UTotal sequence on stream s (capturable): the pair stage runs on `side`
 // beside the mesh stages; the gather waits for it.
+// With MOLSIM_GPU_SERIAL_PASS=1 in the environment every stage runs on one stream, so a
+// kernel profiler sees each stage's own duration (a measurement aid; results are identical).
+static bool serial_pass() {
+  static const bool v = [] {
+    const char* e = std::getenv("MOLSIM_GPU_SERIAL_PASS");
+    return e && e[0] == '1';
+  }();
+  return v;
+}
+
 int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
   if (p.cs.empty()) return MSG_OK;
   if (int e = run_stage(p.bin, s)) return e;
   if (int e = run_stage(p.scan, s)) return e;
   if (int e = run_stage(p.scatter, s)) return e;
   if (int e = run_stage(p.finalize, s)) return e;
-  if (p.any_spme) {
+  if (p.any_spme && serial_pass()) {
+    if (int e = run_stage(p.pair, s)) return e;
+    if (int e = run_stage(p.spread, s)) return e;
+    for (int k = 0; k < 5; ++k)
+      if (int e = run_stage(p.fft[k], s)) return e;
+    if (int e = run_stage(p.gather, s)) return e;
+  } else if (p.any_spme) {
     MSG_HIP(hipEventRecord(fork, s));
     MSG_HIP(hipStreamWaitEvent(side, fork, 0));
     if (int e = run_stage(p.pair, side)) return e;
@@ -432,7 +449,7 @@ int msg_unit_create(msg_unit** out) {
     msg_unit_destroy(u);
     return set_error(MSG_ERR_HIP, "msg_unit_create: stream/event creation failed");
   }
-  if (int e = u->own.init_arena(4 << 20)) {
+  if (int e = u->own.init_arena(16 << 20)) {
     msg_unit_destroy(u);
     return e;
   }

@@ -181,7 +181,9 @@ __global__ __launch_bounds__(1024) void k_reduce(ReduceArgs a) { reduce_body(a);
 
 __global__ __launch_bounds__(1024) void k_reduce_seg(const ReduceArgs* __restrict__ args,
                                                      const int2* __restrict__ map) {
-  reduce_body(args[map[blockIdx.x].x]);
+  const int2 s = map[blockIdx.x];
+  if (s.x < 0) return;  // no work for this XCD slot
+  reduce_body(args[s.x]);
 }
 
 void seg_reduce(const SegGroup& g, hipStream_t s) {

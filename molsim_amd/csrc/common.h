@@ -119,7 +119,7 @@ struct Ctx {
   int* d_cell_tmp = nullptr;     // np (unsorted members)
   int* d_cell_ip = nullptr;      // np (sorted: cell-major, descending ip in a cell)
   int* d_slot_of = nullptr;      // np: slot of particle ip in the cell order
-  void* d_cpos = nullptr;        // np * 4 reals? (always double4: x, y, z, q)
+  void* d_cpos = nullptr;        // np double4 in cell order: x, y, z, particle type
   int* d_ctype = nullptr;        // np
   int* d_slot_cell = nullptr;    // np
 
@@ -310,12 +310,19 @@ struct SegOwner {
   }
 };
 
+// XCD-aware placement: the dispatcher hands workgroup b of a launch to XCD b % kXcd, and
+// each XCD has its own L2.  A segmented launch therefore gives every system one XCD (the
+// least loaded, systems taken largest first) and interleaves the per-XCD workgroup lists,
+// so all of a system's workgroups share one L2 (the pair kernel re-reads each particle
+// ~100 times).  Slots an XCD has no work for are map entries with system -1.
+constexpr int kXcd = 8;
+
 // Collects the argument blocks of one stage per template variant, then uploads them.
 template <class A>
 struct SegBuilder {
   struct G {
     std::vector<A> args;
-    std::vector<int2> map;
+    std::vector<int> nblk;
     size_t lds = 0;
     SegLaunchFn fn = nullptr;
   };
@@ -325,19 +332,38 @@ struct SegBuilder {
     G& g = groups[variant];
     g.fn = fn;
     g.lds = std::max(g.lds, lds);
-    const int sys = (int)g.args.size();
     g.args.push_back(a);
-    for (int b = 0; b < nblk; ++b) g.map.push_back(make_int2(sys, b));
+    g.nblk.push_back(nblk);
+  }
+  static std::vector<int2> place(const std::vector<int>& nblk) {
+    const int n = (int)nblk.size();
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return nblk[x] > nblk[y]; });
+    std::vector<std::vector<int2>> lists(kXcd);
+    std::vector<long> load(kXcd, 0);
+    for (int sys : order) {
+      const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      for (int b = 0; b < nblk[sys]; ++b) lists[x].push_back(make_int2(sys, b));
+      load[x] += nblk[sys];
+    }
+    size_t len = 0;
+    for (auto& l : lists) len = std::max(len, l.size());
+    std::vector<int2> map(len * kXcd, make_int2(-1, 0));
+    for (int x = 0; x < kXcd; ++x)
+      for (size_t k = 0; k < lists[x].size(); ++k) map[k * kXcd + x] = lists[x][k];
+    return map;
   }
   int finish(SegOwner& own, std::vector<SegGroup>& out) {
     for (auto& kv : groups) {
+      const std::vector<int2> map = place(kv.second.nblk);
       SegGroup sg;
-      sg.nblk = (int)kv.second.map.size();
+      sg.nblk = (int)map.size();
       sg.lds = kv.second.lds;
       sg.launch = kv.second.fn;
       const A* da = nullptr;
       if (int e = own.upload(kv.second.args, &da)) return e;
-      if (int e = own.upload(kv.second.map, &sg.d_map)) return e;
+      if (int e = own.upload(map, &sg.d_map)) return e;
       sg.d_args = da;
       out.push_back(sg);
     }

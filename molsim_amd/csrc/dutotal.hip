@@ -385,7 +385,6 @@ int dutotal_launch(Ctx* c, int ntrial, const int* ip, const double* rotm) {
     if (int e = make_gk(c)) return e;
   if (!c->fresh)
     if (int e = utotal_refresh(c)) return e;
-  const int ntp = c->nptpt;
   if (ntrial > c->de_cap) {
     MSG_HIP(hipStreamSynchronize(c->stream));
     if (c->d_de) { hipFree(c->d_de); hipFree(c->d_de_ip); hipFree(c->d_de_r); }

@@ -92,6 +92,7 @@ template <typename real>
 __global__ __launch_bounds__(256) void k_bin_seg(const BinArgs* __restrict__ args,
                                                  const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
+  if (s.x < 0) return;  // no work for this XCD slot
   bin_body<real>(args[s.x], s.y);
 }
 
@@ -192,6 +193,7 @@ __global__ __launch_bounds__(1024) void k_scan(ScanArgs a) { scan_body(a, blockI
 __global__ __launch_bounds__(1024) void k_scan_seg(const ScanArgs* __restrict__ args,
                                                    const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
+  if (s.x < 0) return;  // no work for this XCD slot
   scan_body(args[s.x], s.y);
 }
 
@@ -226,6 +228,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) { scatter_body(a
 __global__ __launch_bounds__(256) void k_scatter_seg(const ScatterArgs* __restrict__ args,
                                                      const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
+  if (s.x < 0) return;  // no work for this XCD slot
   scatter_body(args[s.x], s.y);
 }
 
@@ -272,7 +275,7 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& a, int bid) {
       const int dst = s0 + rank;
       a.cell_ip[dst] = ip;
       a.slot_of[ip] = dst;
-      a.cpos[dst] = make_double4(a.ro[3 * ip], a.ro[3 * ip + 1], a.ro[3 * ip + 2], a.az[ip]);
+      a.cpos[dst] = make_double4(a.ro[3 * ip], a.ro[3 * ip + 1], a.ro[3 * ip + 2], (double)a.ipt[ip]);
       a.ctype[dst] = a.ipt[ip];
       a.slot_cell[dst] = wv;
     }
@@ -313,6 +316,7 @@ template <typename real>
 __global__ __launch_bounds__(256) void k_finalize_seg(const FinalizeArgs* __restrict__ args,
                                                       const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
+  if (s.x < 0) return;  // no work for this XCD slot
   finalize_body<real>(args[s.x], s.y);
 }
 

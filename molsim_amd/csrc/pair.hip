@@ -41,6 +41,25 @@ struct PairArgs {
   int nblk;           // workgroups of this system (stride of the work-item loop)
 };
 
+// Copy of nbytes (a multiple of 16, both ends 16-byte aligned) from global memory to LDS by
+// the 256 threads of a workgroup: 16-byte loads, eight per thread issued before any store,
+// so the copy costs one memory round trip instead of one per element.
+__device__ inline void lds_copy16(void* dst, const void* src, int nbytes) {
+  const int4* s = (const int4*)src;
+  int4* d = (int4*)dst;
+  const int n = nbytes >> 4;
+  constexpr int B = 8;
+  for (int base = threadIdx.x; base < n; base += 256 * B) {
+    int4 v[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (base + u * 256 < n) v[u] = s[base + u * 256];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (base + u * 256 < n) d[base + u * 256] = v[u];
+  }
+}
+
 // Wave-local LDS hand-off (all lanes of one wavefront, no workgroup barrier).
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -56,7 +75,7 @@ constexpr int kWaves = 4;  // wavefronts per workgroup; each works through its o
 // Energies (per type pair) and the virial accumulate per lane over all items.
 template <typename real, int NTP>
 __device__ __forceinline__ void pair_items_body(const PairArgs& a, int bid) {
-  extern __shared__ double s_dyn[];  // table records, then the bin index (sized at launch)
+  extern __shared__ __align__(16) double s_dyn[];  // table records, then the bin index (sized at launch)
   __shared__ int s_nbs[kWaves][kMaxStencil + 1], s_pref[kWaves][kMaxStencil + 1];
   __shared__ real s_zlo[kMaxTP], s_invw[kMaxTP], s_r2at[kMaxTP], s_r2um[kMaxTP];
   __shared__ int s_last[kMaxTP], s_tp[kMaxT * kMaxT];
@@ -66,8 +85,8 @@ __device__ __forceinline__ void pair_items_body(const PairArgs& a, int bid) {
   int* s_bins = (int*)(s_coef + nint * 12);
   {
     const real* src = (const real*)a.tcoef;
-    for (int k = threadIdx.x; k < nint * 12; k += 256) s_coef[k] = src[k];
-    for (int k = threadIdx.x; k < NTP * kTabBins; k += 256) s_bins[k] = a.tbins[k];
+    lds_copy16(s_coef, src, nint * 12 * (int)sizeof(real));
+    lds_copy16(s_bins, a.tbins, NTP * kTabBins * (int)sizeof(int));
     if (threadIdx.x < NTP) {
       const int tp = threadIdx.x;
       s_zlo[tp] = (real)a.tm.zlo[tp];
@@ -123,7 +142,7 @@ __device__ __forceinline__ void pair_items_body(const PairArgs& a, int bid) {
     for (int h = 0; h < kPairHB; ++h) {
       const int s = h0 + min(h, nh - 1);
       hp[h] = a.cpos[s];
-      ht[h] = a.ctype[s];
+      ht[h] = (int)a.cpos[s].w;
     }
     double f[kPairHB][3];
 #pragma unroll
@@ -133,7 +152,7 @@ __device__ __forceinline__ void pair_items_body(const PairArgs& a, int bid) {
       while (s_pref[wv][k + 1] <= idx) ++k;
       const int j = s_nbs[wv][k] + (idx - s_pref[wv][k]);
       const double4 pj = a.cpos[j];
-      const int tj = a.ctype[j];
+      const int tj = (int)pj.w;
 #pragma unroll
       for (int h = 0; h < kPairHB; ++h) {
         if (h >= nh || j == h0 + h) continue;
@@ -209,6 +228,7 @@ template <typename real, int NTP>
 __global__ __launch_bounds__(256) void k_pair_seg(const PairArgs* __restrict__ args,
                                                   const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
+  if (s.x < 0) return;  // no work for this XCD slot
   pair_items_body<real, NTP>(args[s.x], s.y);
 }
 
@@ -216,14 +236,23 @@ __global__ __launch_bounds__(256) void k_pair_seg(const PairArgs* __restrict__ a
 // and candidate pj (type tj).
 template <typename real, int NTP>
 struct PairAcc {
-  double et[NTP];
+  // energies of one home particle's pairs by partner type (the home's type fixes the type
+  // pair); to_tp() folds them into type-pair slots at the end
+  static constexpr int NT = NTP == 1 ? 1 : NTP == 3 ? 2 : NTP == 6 ? 3 : 4;
+  double et[NT];
   double vir, fx, fy, fz;
   int err;
   __device__ void zero() {
 #pragma unroll
-    for (int q = 0; q < NTP; ++q) et[q] = 0.0;
+    for (int q = 0; q < NT; ++q) et[q] = 0.0;
     vir = fx = fy = fz = 0.0;
     err = 0;
+  }
+  __device__ double to_tp(int q, int ht, const int* s_tp) const {
+    double v = 0.0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) v += (s_tp[ht * kMaxT + t] == q) ? et[t] : 0.0;
+    return v;
   }
 };
 
@@ -260,7 +289,7 @@ __device__ inline void pair_eval(const double4& hp, int ht, const double4& pj, i
     fs = t[7] + dd * (t[8] + dd * (t[9] + dd * (t[10] + dd * t[11])));
   }
 #pragma unroll
-  for (int q = 0; q < NTP; ++q) acc.et[q] += (q == tp) ? (double)us : 0.0;
+  for (int t = 0; t < PairAcc<real, NTP>::NT; ++t) acc.et[t] += (t == tj) ? (double)us : 0.0;
   acc.fx += (double)(fs * dx);
   acc.fy += (double)(fs * dy);
   acc.fz += (double)(fs * dz);
@@ -276,7 +305,7 @@ template <typename real, int NTP, int LPH>
 __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
   constexpr int HPB = 256 / LPH;                          // homes per workgroup
   constexpr int MPL = (kMaxStencil + LPH - 1) / LPH;      // stencil cells per lane
-  extern __shared__ double s_dyn[];
+  extern __shared__ __align__(16) double s_dyn[];
   __shared__ real s_zlo[kMaxTP], s_invw[kMaxTP], s_r2at[kMaxTP], s_r2um[kMaxTP];
   __shared__ int s_last[kMaxTP], s_tp[kMaxT * kMaxT];
   __shared__ double s_red[4][kMaxTP + 1];
@@ -286,8 +315,8 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
   int* s_bins = (int*)(s_coef + nint * 12);
   {
     const real* src = (const real*)a.tcoef;
-    for (int k = threadIdx.x; k < nint * 12; k += 256) s_coef[k] = src[k];
-    for (int k = threadIdx.x; k < NTP * kTabBins; k += 256) s_bins[k] = a.tbins[k];
+    lds_copy16(s_coef, src, nint * 12 * (int)sizeof(real));
+    lds_copy16(s_bins, a.tbins, NTP * kTabBins * (int)sizeof(int));
     if (threadIdx.x < NTP) {
       const int tp = threadIdx.x;
       s_zlo[tp] = (real)a.tm.zlo[tp];
@@ -304,16 +333,36 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
   const bool home = s < a.np;
   const int hs = home ? s : a.np - 1;
   const double4 hp = a.cpos[hs];
-  const int ht = a.ctype[hs];
+  const int ht = (int)hp.w;
   const int c = a.slot_cell[hs];
   const int cx = c % a.g.nc[0], cy = (c / a.g.nc[0]) % a.g.nc[1], cz = c / (a.g.nc[0] * a.g.nc[1]);
+  // home position inside its cell, for skipping stencil cells whose nearest point is
+  // beyond the cutoff (every candidate in them would fail the cutoff test anyway)
+  const int cc3[3] = {cx, cy, cz};
+  const double hh[3] = {hp.x, hp.y, hp.z};
+  double hrel[3], csz[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    csz[d] = a.box.L[d] / a.g.nc[d];
+    hrel[d] = (hh[d] + a.box.half[d]) - cc3[d] * csz[d];
+  }
+  const double cull2 = a.rcut2 * (1.0 + 1e-5);
   int st0[MPL], cnt[MPL], tot = 0;
 #pragma unroll
   for (int m = 0; m < MPL; ++m) {
     const int sc = k * MPL + m;
     cnt[m] = 0;
     st0[m] = 0;
+    double bd2 = 0.0;
     if (sc < a.g.nst) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int o = a.g.st[3 * sc + d];
+        const double dist = o < 0 ? hrel[d] : (o > 0 ? csz[d] - hrel[d] : 0.0);
+        bd2 += dist > 0.0 ? dist * dist : 0.0;
+      }
+    }
+    if (sc < a.g.nst && bd2 <= cull2) {
       int x = cx + a.g.st[3 * sc], y = cy + a.g.st[3 * sc + 1], z = cz + a.g.st[3 * sc + 2];
       x += x < 0 ? a.g.nc[0] : (x >= a.g.nc[0] ? -a.g.nc[0] : 0);
       y += y < 0 ? a.g.nc[1] : (y >= a.g.nc[1] ? -a.g.nc[1] : 0);
@@ -357,32 +406,32 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
     while (s_cpf[grp][cc + 1] <= i) ++cc;
     int j = s_cst[grp][cc] + (i - s_cpf[grp][cc]);
     int cend = s_cst[grp][cc] + (s_cpf[grp][cc + 1] - s_cpf[grp][cc]);
+    constexpr int U = 4;  // candidates in flight per lane
     while (i < i1) {
-      const int jA = j;
-      ++i; ++j;
-      if (j == cend && i < i1) {
-        do { ++cc; } while (s_cpf[grp][cc + 1] <= i);
-        j = s_cst[grp][cc];
-        cend = j + (s_cpf[grp][cc + 1] - s_cpf[grp][cc]);
-      }
-      const bool haveB = i < i1;
-      const int jB = haveB ? j : jA;
-      if (haveB) {
-        ++i; ++j;
-        if (j == cend && i < i1) {
-          do { ++cc; } while (s_cpf[grp][cc + 1] <= i);
-          j = s_cst[grp][cc];
-          cend = j + (s_cpf[grp][cc + 1] - s_cpf[grp][cc]);
+      int jv[U];
+      bool hv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        hv[u] = i < i1;
+        jv[u] = hv[u] ? j : s;
+        if (hv[u]) {
+          ++i;
+          ++j;
+          if (j == cend && i < i1) {
+            do { ++cc; } while (s_cpf[grp][cc + 1] <= i);
+            j = s_cst[grp][cc];
+            cend = j + (s_cpf[grp][cc + 1] - s_cpf[grp][cc]);
+          }
         }
       }
-      const double4 pA = a.cpos[jA], pB = a.cpos[jB];
-      const int tA = a.ctype[jA], tB = a.ctype[jB];
-      if (jA != s)
-        pair_eval<real, NTP>(hp, ht, pA, tA, a.box, rc2, false, s_coef, s_bins, s_zlo, s_invw,
-                             s_r2at, s_r2um, s_last, s_tp, acc);
-      if (haveB && jB != s)
-        pair_eval<real, NTP>(hp, ht, pB, tB, a.box, rc2, false, s_coef, s_bins, s_zlo, s_invw,
-                             s_r2at, s_r2um, s_last, s_tp, acc);
+      double4 pv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) pv[u] = a.cpos[jv[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (hv[u] && jv[u] != s)
+          pair_eval<real, NTP>(hp, ht, pv[u], (int)pv[u].w, a.box, rc2, false, s_coef, s_bins, s_zlo,
+                               s_invw, s_r2at, s_r2um, s_last, s_tp, acc);
     }
   }
 #pragma unroll
@@ -400,7 +449,7 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
   if (acc.err) atomicOr(a.flags, acc.err);
 #pragma unroll
   for (int q = 0; q < NTP; ++q) {
-    const double v = msgd::wave_sum(acc.et[q]);
+    const double v = msgd::wave_sum(acc.to_tp(q, ht, s_tp));
     if (lane == 0) s_red[wv][q] = v;
   }
   {
@@ -422,6 +471,7 @@ template <typename real, int NTP, int LPH>
 __global__ __launch_bounds__(256) void k_pair_nb_seg(const PairArgs* __restrict__ args,
                                                      const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
+  if (s.x < 0) return;  // no work for this XCD slot
   pair_nb_body<real, NTP, LPH>(args[s.x], s.y);
 }
 
@@ -504,7 +554,7 @@ struct VlPairArgs {
 template <typename real, int NTP>
 __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
   const PairArgs& a = v.p;
-  extern __shared__ double s_dyn[];
+  extern __shared__ __align__(16) double s_dyn[];
   __shared__ real s_zlo[kMaxTP], s_invw[kMaxTP], s_r2at[kMaxTP], s_r2um[kMaxTP];
   __shared__ int s_last[kMaxTP], s_tp[kMaxT * kMaxT];
   __shared__ double s_red[4][kMaxTP + 1];
@@ -513,8 +563,8 @@ __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
   int* s_bins = (int*)(s_coef + nint * 12);
   {
     const real* src = (const real*)a.tcoef;
-    for (int k = threadIdx.x; k < nint * 12; k += 256) s_coef[k] = src[k];
-    for (int k = threadIdx.x; k < NTP * kTabBins; k += 256) s_bins[k] = a.tbins[k];
+    lds_copy16(s_coef, src, nint * 12 * (int)sizeof(real));
+    lds_copy16(s_bins, a.tbins, NTP * kTabBins * (int)sizeof(int));
     if (threadIdx.x < NTP) {
       const int tp = threadIdx.x;
       s_zlo[tp] = (real)a.tm.zlo[tp];
@@ -554,9 +604,10 @@ __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
     a.force[3 * ip + 2] = acc.fz * a.scale;
   }
   if (acc.err) atomicOr(a.flags, acc.err);
+  const int htv = s < a.np ? v.ipt[v.vl_home[s]] : 0;
 #pragma unroll
   for (int q = 0; q < NTP; ++q) {
-    const double t = msgd::wave_sum(acc.et[q]);
+    const double t = msgd::wave_sum(acc.to_tp(q, htv, s_tp));
     if (lane == 0) s_red[wv][q] = t;
   }
   {

@@ -258,6 +258,7 @@ __device__ __forceinline__ void fft_zconv_body(const FftArgs& fa, int bid) {
   __global__ __launch_bounds__(256) void name##_seg(const FftArgs* __restrict__ args,       \
                                                     const int2* __restrict__ map) {         \
     const int2 s = map[blockIdx.x];                                                         \
+    if (s.x < 0) return;                                                                    \
     body<real, K>(args[s.x], s.y __VA_ARGS__);                                              \
   }
 MSG_FFT_ENTRY(k_fft_x, fft_x_body)
@@ -285,26 +286,45 @@ template <> struct R4T<double> { typedef double4 t; };
 // of 3.6e-15, below FP64 rounding of the mesh values.
 constexpr double kQScale = 281474976710656.0;  // 2^48
 
-// One workgroup per brick.  Phase 1: the B-spline weights of up to CAP contributing
-// particles go to LDS.  Phase 2: all threads take (particle, iz, iy) stencil rows and add
-// the clipped x-row into the brick with 64-bit integer LDS atomics.  Phase 3: every mesh
-// point of the brick is written once.
+// Stencil range of one dimension inside a brick of extent B starting at the particle's
+// local base point l (the stencil covers l, l-1, ..., l-P+1): with one brick across the
+// mesh every point is inside (indices wrap); otherwise only i with 0 <= l - i < B.
+template <int P>
+__device__ inline void brick_range(int l, int B, bool whole, int& i0, int& n) {
+  if (whole) {
+    i0 = 0;
+    n = P;
+  } else {
+    i0 = max(0, l - B + 1);
+    n = max(0, min(P - 1, l) - i0 + 1);
+  }
+}
+
+// One workgroup per brick.  Phase 1: for up to CAP contributing particles (from the brick
+// and the next brick in each dimension) the B-spline weights go to LDS together with the
+// particle's stencil rows that fall inside this brick (clipped y and z ranges).  Phase 2:
+// all threads take (particle, z, y) rows of that clipped set -- found by a binary search over
+// the rows' prefix sums -- and add the clipped x-row into the brick with 64-bit integer LDS
+// atomics.  Phase 3: every mesh point of the brick is written once.
 template <typename real, int P>
 __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   typedef typename R4T<real>::t R4;
-  constexpr int CAP = sizeof(real) == 4 ? 256 : 128;
+  constexpr int CAP = 128;
   constexpr int NBP = kBX * kBY * kBZ;
   __shared__ unsigned long long acc[NBP];
   __shared__ real th[CAP][3 * P];
   __shared__ int loc[CAP];
-  __shared__ int r_beg[8], r_pref[9];
+  __shared__ int rng[CAP];      // packed y0 | ny << 8 | z0 << 16 | nz << 24
+  __shared__ int rpre[CAP + 1]; // prefix sums of rows per particle
+  __shared__ int wsum[4];
+  __shared__ int r_beg[8], r_pref[9], r_len[8];
   const int K = a.m.K;
   const int b = bid;
   const int bx = b % a.m.nb[0], by = (b / a.m.nb[0]) % a.m.nb[1], bz = b / (a.m.nb[0] * a.m.nb[1]);
   const int lo[3] = {bx * kBX, by * kBY, bz * kBZ};
-  __shared__ int r_len[8];
+  const bool whole[3] = {a.m.nb[0] == 1, a.m.nb[1] == 1, a.m.nb[2] == 1};
   if (threadIdx.x < 8) {  // the (up to) 8 source bricks: own and next in each dimension
-    const int ox = a.m.nb[0] > 1 ? 2 : 1, oy = a.m.nb[1] > 1 ? 2 : 1, oz = a.m.nb[2] > 1 ? 2 : 1;
+    const int ox = whole[0] ? 1 : 2, oy = whole[1] ? 1 : 2, oz = whole[2] ? 1 : 2;
     const int t = threadIdx.x;
     int beg = 0, len = 0;
     if (t < ox * oy * oz) {
@@ -330,8 +350,10 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   __syncthreads();
   const int total = r_pref[8];
   const R4* bq = (const R4*)a.bq;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int base = 0; base < total; base += CAP) {
     const int cnt = min(CAP, total - base);
+    int rows = 0;
     if ((int)threadIdx.x < cnt) {
       const int t = base + threadIdx.x;
       int r = 0;
@@ -349,26 +371,52 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) l[d] += l[d] < 0 ? K : 0;
       loc[threadIdx.x] = l[0] | (l[1] << 10) | (l[2] << 20);
+      int y0, ny, z0, nz;
+      brick_range<P>(l[1], kBY, whole[1], y0, ny);
+      brick_range<P>(l[2], kBZ, whole[2], z0, nz);
+      rng[threadIdx.x] = y0 | (ny << 8) | (z0 << 16) | (nz << 24);
+      rows = ny * nz;
     }
+    // exclusive prefix of the rows over the chunk's particles (threads 0..CAP-1)
+    int incl = rows;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
     __syncthreads();
-    for (int task = threadIdx.x; task < cnt * P * P; task += 256) {
-      const int p = task / (P * P), row = task % (P * P), iz = row / P, iy = row % P;
+    int before = 0;
+    for (int k = 0; k < w; ++k) before += wsum[k];
+    if ((int)threadIdx.x < CAP) rpre[threadIdx.x] = before + incl - rows;
+    if (threadIdx.x == 0) rpre[CAP] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    const int ntask = rpre[CAP];
+    for (int task = threadIdx.x; task < ntask; task += 256) {
+      int lo_p = 0, hi_p = cnt - 1;  // last particle with rpre[p] <= task
+      while (lo_p < hi_p) {
+        const int mid = (lo_p + hi_p + 1) >> 1;
+        if (rpre[mid] <= task) lo_p = mid; else hi_p = mid - 1;
+      }
+      const int p = lo_p;
+      const int rr = rng[p];
+      const int ny = (rr >> 8) & 255;
+      const int r = task - rpre[p];
+      const int iy = (rr & 255) + r % ny, iz = ((rr >> 16) & 255) + r / ny;
       const int lp = loc[p];
       int ly = ((lp >> 10) & 1023) - iy, lz = (lp >> 20) - iz;
-      ly += ly < 0 ? K : 0;
+      ly += ly < 0 ? K : 0;  // wraps only where one brick spans the mesh
       lz += lz < 0 ? K : 0;
-      if (ly >= kBY || lz >= kBZ) continue;
       const real wyz = th[p][P + iy] * th[p][2 * P + iz];
       const int lx0 = lp & 1023;
+      int x0, nx;
+      brick_range<P>(lx0, kBX, whole[0], x0, nx);
       unsigned long long* rowp = acc + kBX * (ly + kBY * lz);
-#pragma unroll
-      for (int ix = 0; ix < P; ++ix) {
+      for (int ix = x0; ix < x0 + nx; ++ix) {
         int lx = lx0 - ix;
         lx += lx < 0 ? K : 0;
-        if (lx < kBX) {
-          const long long v = __double2ll_rn((double)(th[p][ix] * wyz) * kQScale);
-          atomicAdd(rowp + lx, (unsigned long long)v);
-        }
+        const long long v = __double2ll_rn((double)(th[p][ix] * wyz) * kQScale);
+        atomicAdd(rowp + lx, (unsigned long long)v);
       }
     }
     __syncthreads();
@@ -388,6 +436,7 @@ template <typename real, int P>
 __global__ __launch_bounds__(256) void k_spread_seg(const SpreadArgs* __restrict__ args,
                                                     const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
+  if (s.x < 0) return;  // no work for this XCD slot
   spread_body<real, P>(args[s.x], s.y);
 }
 
@@ -484,6 +533,7 @@ template <typename real, int P>
 __global__ __launch_bounds__(256) void k_gather_seg(const GatherArgs* __restrict__ args,
                                                     const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
+  if (s.x < 0) return;  // no work for this XCD slot
   gather_body<real, P>(args[s.x], s.y);
 }
 

@@ -98,6 +98,23 @@ def test_autotune_table(members, table):
         e.close()
 
 
+def test_autotune_batched(members, table):
+    engs = [engine(sp, table) for sp in members[:5]]
+    tab = autotune.tune_batched(engs, reps=2)
+    assert set(tab) == {autotune.bucket(sp.n) for sp in members[:5]}
+    for ent in tab.values():
+        assert ent["tile"] in autotune.TILES and min(ent["ms"].values()) == ent["ms"][ent["tile"]]
+    autotune.apply(engs, tab)
+    b = Batch(engs)
+    got = b.utotal()
+    for e, g in zip(engs, got):  # tuned batch still equals the standalone result
+        alone = e.utotal()
+        assert _same(g, alone), e.spec.name
+    b.close()
+    for e in engs:
+        e.close()
+
+
 def _trials(sp, n, seed):
     rng = np.random.default_rng(seed)
     ips = rng.integers(0, sp.n, size=n)
