@@ -40,7 +40,8 @@ def kernel_bytes(name: str, n: int, K: int, fp64: bool) -> float:
     mesh_c = 2 * r * H * K * K
     per_particle = {
         "bin": 24 + 8 + 4 + 4 + 3 * r + 4,
-        "cellsort": 4 * 6 + 32 + 24 + 8 + 4,
+        "scatter": 4 + 4 + 4 + 4,
+        "finalize": 4 + 24 + 4 + 32 + 4 + 4 + 4 + 4 * r + 4 + 4 + 4,
         "pair": 32 + 4 + 4 + 4 + 24,
         "spread": 4 + 4 + 3 * r + 8,
         "gather": 4 + 4 + 3 * r + 8 + 48,
@@ -433,6 +434,31 @@ def run_ionic_md(args):
     return out
 
 
+def run_copy(args):
+    """A plain device-to-device copy of a 1 GiB buffer (torch's copy kernel): the measured HBM
+    bandwidth beside the 8 TB/s peak, and the reference for scaling the profiler's
+    FETCH_SIZE / WRITE_SIZE counters (a copy reads and writes exactly its size)."""
+    import torch
+    n = (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = max(args.steps, 1)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    nbytes = 2 * 4 * n
+    return {"metric": "device copy bandwidth (read + write)", "value": nbytes / dt / 1e9, "unit": "GB/s",
+            "n_gpus": 1, "steps": reps, "warmup": 3, "ms_per_step": 1e3 * dt, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "copy_1GiB", "bytes_read": 4 * n, "bytes_written": 4 * n},
+            "peak_GBps": HBM_PEAK_GBS}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -444,7 +470,7 @@ def main():
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps of the timed path only (for rocprofv3)")
     ap.add_argument("--workload", default="polyelectrolyte",
-                    choices=["polyelectrolyte", "ensemble", "serve", "electrolyte_mc", "ionic_md"])
+                    choices=["polyelectrolyte", "ensemble", "serve", "electrolyte_mc", "ionic_md", "copy"])
     ap.add_argument("--mc-trials", type=int, default=10000, help="electrolyte_mc: MC trials")
     ap.add_argument("--md-steps", type=int, default=1000, help="ionic_md: timed MD steps")
     ap.add_argument("--md-warmup", type=int, default=10, help="ionic_md: untimed MD steps")
@@ -471,7 +497,7 @@ def main():
         sys.exit(2)
     if args.workload != "polyelectrolyte":
         out = {"ensemble": run_ensemble, "serve": run_serve, "electrolyte_mc": run_electrolyte_mc,
-               "ionic_md": run_ionic_md}[args.workload](args)
+               "ionic_md": run_ionic_md, "copy": run_copy}[args.workload](args)
         if out:
             print(json.dumps(out), flush=True)
         return

@@ -175,7 +175,10 @@ double msg_random(int* iseed, int* state, double* am);
  * (isamp 1); trial displacement (Random - 1/2) * dtran[type] per axis; Metropolis with
  * beta * dE (expmax = 87); accepted moves are committed.  The generator state (iseed,
  * rng_state, am) is read and updated.  Outputs: *du_sum (sum of accepted dE), *naccept,
- * events[t] (1 accepted, 0 rejected, -1 hard-core overlap; may be NULL). */
+ * events[t] (1 accepted, 0 rejected, -1 hard-core overlap; may be NULL).
+ * Standard-Ewald and Ewald-free systems run the whole pass on the device (one workgroup,
+ * the generator on the device; a move accepted into another cell ends a launch and the pass
+ * resumes after the cell list is rebuilt); SPME systems run the trials from the host. */
 int msg_mc_pass(msg_ctx* ctx, int ntrial, int isamp, const double* dtran, double beta,
                 int* iseed, int* rng_state, double* am, double* du_sum, int* naccept,
                 int* events);
@@ -238,9 +241,9 @@ void* msg_stream(msg_ctx* ctx);
 
 /* Per-kernel timing with HIP events on the context's stream.  Kernel ids: */
 enum msg_kernel_id {
-  MSG_K_BIN = 0, MSG_K_CELLSORT, MSG_K_PAIR, MSG_K_SPREAD, MSG_K_FFT_X, MSG_K_FFT_Y,
+  MSG_K_BIN = 0, MSG_K_SCAN, MSG_K_PAIR, MSG_K_SPREAD, MSG_K_FFT_X, MSG_K_FFT_Y,
   MSG_K_FFT_ZCONV, MSG_K_FFT_YI, MSG_K_FFT_XI, MSG_K_GATHER, MSG_K_REDUCE, MSG_K_EWSTD,
-  MSG_K_DE, MSG_K_VLIST, MSG_K_COUNT
+  MSG_K_DE, MSG_K_VLIST, MSG_K_SCATTER, MSG_K_FINALIZE, MSG_K_COUNT
 };
 int msg_profile(msg_ctx* ctx, int enable);
 int msg_profile_read(msg_ctx* ctx, int kernel_id, double* total_ms, int* launches);

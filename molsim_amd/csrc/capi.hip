@@ -24,8 +24,8 @@ int dalloc(Ctx* c, void** p, size_t bytes) {
 }
 
 static const char* kKernelNames[MSG_K_COUNT] = {
-    "bin", "cellsort", "pair", "spread", "fft_x", "fft_y", "fft_zconv", "fft_yi", "fft_xi",
-    "gather", "reduce", "ewald_std", "dutotal", "vlist"};
+    "bin", "scan", "pair", "spread", "fft_x", "fft_y", "fft_zconv", "fft_yi", "fft_xi",
+    "gather", "reduce", "ewald_std", "dutotal", "vlist", "scatter", "finalize"};
 
 void prof_begin(Ctx* c, int kid) {
   if (!c->prof) return;

@@ -1,0 +1,427 @@
+// Device-resident Monte Carlo chain: MCPass / SPartMove / GetRandomTrialPos / Metropolis /
+// MCUpdate (Src/mc.F90:1470-1750,4527-4573,5135-5235) for single-particle moves, the
+// trials run one after another inside ONE workgroup of 1024 threads, with MOLSIM's random
+// number generator (Random, Src/mollib.F90:2689-2713) on the device, so a pass of any
+// length costs one launch and no host round trip per trial.
+//
+// Per trial (all FP64):
+//   thread 0   particle (sequential or random), trial position (Random - 1/2) * dtran per
+//              axis, periodic wrap;
+//   all        real-space energy difference over the cell list (DUTwoBody semantics: a
+//              hard-core or below-table pair at the new position is an overlap), standard-
+//              Ewald structure-factor change dS(k) = q (e^{ik.r_new} - e^{ik.r_old}) with
+//              dE = sum kfac (|S + dS|^2 - |S|^2), surface term;
+//   thread 0   Metropolis (expmax = 87 guards, weight 1);
+//   accepted   S += dS (kept in registers from the energy step), position in caller and cell
+//              order, dipole sum.
+// A move accepted into another cell of a multi-cell grid ends the launch: the caller
+// commits it, rebuilds the cell list by a UTotal pass and relaunches for the remaining
+// trials.  SPME systems keep the host-driven chain (an accepted move changes the whole
+// potential mesh).
+#include <cmath>
+
+#include "common.h"
+
+namespace msg {
+
+namespace {
+
+constexpr int kChainThreads = 1024;
+constexpr int kChainKvec = 4;  // k-vectors per thread: nk <= 4096
+
+struct ChainArgs {
+  int np, ntrial, isamp, structural, ewald, lsurf;
+  double* ro;
+  const double* az;
+  const int* ipt;
+  double4* cpos;
+  const int* cell_ip;
+  const int* cell_start;
+  const int* slot_of;
+  const int* slot_cell;
+  CellGrid g;
+  Box box;
+  double rcut2;
+  int cut_inclusive;
+  const double* tcoef;  // FP64 table records
+  const int* tbins;
+  GpuTableMeta tm;
+  int nk;
+  const int* kvec;
+  const double* kfac;
+  double* sk;
+  double tpb[3];
+  double* Mout;         // dipole sum kept with the energies
+  double* Mtot;         // dipole sum read by the force kernels
+  double surf_fac;
+  double epsi4pi;
+  const double* dtran;  // per particle type
+  double beta;
+  int* rng;             // iseed, xorshift word, Park-Miller word
+  double* am;
+  int* next;            // sequential sampling cursor
+  int* events;          // per trial: -1 overlap, 0 rejected, 1 accepted
+  double* stat;         // [0] sum of accepted dE, [1] accepted, [2] trials done, [3] stopped ip, [4..6] its position
+};
+
+// MOLSIM's Random: Marsaglia xorshift combined with Park-Miller by Schrage's method.
+__device__ double chain_random(int* st, double* am) {
+  const int a = 16807, m = 2147483647, q = 127773, r = 2836;
+  if (st[0] <= 0 || st[2] < 0) {
+    const int s = st[0] < 0 ? -st[0] : st[0];
+    *am = (double)__int_as_float(0x3f7fffff) / (double)m;  // the single-precision real below 1
+    st[2] = (888889999 ^ s) | 1;
+    st[1] = 777755555 ^ s;
+    st[0] = s + 1;
+  }
+  unsigned int x = (unsigned int)st[1];
+  x ^= x << 13;
+  x ^= x >> 17;
+  x ^= x << 5;
+  st[1] = (int)x;
+  const int k = st[2] / q;
+  st[2] = a * (st[2] - k * q) - r * k;
+  if (st[2] < 0) st[2] += m;
+  return *am * (double)((m & (st[1] ^ st[2])) | 1);
+}
+
+__device__ inline double chain_table_u(const ChainArgs& a, int tp, double r2) {
+  int b = (int)((r2 - a.tm.zlo[tp]) * a.tm.invw[tp]);
+  b = min(max(b, 0), kTabBins - 1);
+  int idx = a.tbins[tp * kTabBins + b];
+  const int last = a.tm.last[tp];
+  while (idx < last && r2 < a.tcoef[12 * idx]) ++idx;
+  const double* t = a.tcoef + 12 * idx;
+  const double d = r2 - t[0];
+  return t[1] + d * (t[2] + d * (t[3] + d * (t[4] + d * (t[5] + d * t[6]))));
+}
+
+__device__ inline int chain_cell(const ChainArgs& a, const double* r) {
+  int ci[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int k = (int)floor((r[d] + a.box.half[d]) * a.g.csi[d]);
+    ci[d] = min(max(k, 0), a.g.nc[d] - 1);
+  }
+  return ci[0] + a.g.nc[0] * (ci[1] + a.g.nc[1] * ci[2]);
+}
+
+// Sum over the workgroup; the result is valid in every thread.
+__device__ inline double chain_sum(double v, double* sh) {
+  v = msgd::wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < kChainThreads / 64; ++w) s += sh[w];
+  return s;
+}
+
+__global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
+  __shared__ double sh[kChainThreads / 64];
+  __shared__ int s_ip, s_acc, s_stop, s_ovl;
+  __shared__ double s_rn[3];
+  __shared__ int s_rng[3];
+  __shared__ double s_am;
+  __shared__ int s_next;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    for (int k = 0; k < 3; ++k) s_rng[k] = a.rng[k];
+    s_am = *a.am;
+    s_next = *a.next;
+    a.stat[3] = -1.0;  // no move left to commit
+  }
+  double dsum = 0.0;
+  int nacc = 0, done = 0;
+  const double expmax = 87.0;  // Src/mol.F90:62
+  for (int t = 0; t < a.ntrial; ++t) {
+    if (tid == 0) {
+      int ip;
+      if (a.isamp == 0) {
+        ip = s_next;
+        s_next = (s_next + 1) % a.np;
+      } else {
+        ip = (int)(a.np * chain_random(s_rng, &s_am));
+        ip = min(max(ip, 0), a.np - 1);
+      }
+      const double dt = a.dtran[a.ipt[ip]];
+      for (int d = 0; d < 3; ++d) s_rn[d] = a.ro[3 * ip + d] + (chain_random(s_rng, &s_am) - 0.5) * dt;
+      for (int d = 0; d < 3; ++d) {
+        const double L = a.box.L[d];
+        if (fabs(s_rn[d]) > 0.5 * L) s_rn[d] -= L * round(s_rn[d] / L);
+      }
+      s_ip = ip;
+      s_ovl = 0;
+    }
+    __syncthreads();
+    const int ip = s_ip;
+    const double rn[3] = {s_rn[0], s_rn[1], s_rn[2]};
+    const double ro[3] = {a.ro[3 * ip], a.ro[3 * ip + 1], a.ro[3 * ip + 2]};
+    const double q = a.az[ip];
+    const int ti = a.ipt[ip];
+    // real space: new minus old
+    double e = 0.0;
+    int ovl = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      const double* r = pass == 0 ? rn : ro;
+      const int cc = pass == 0 ? chain_cell(a, rn) : a.slot_cell[a.slot_of[ip]];
+      const int cx = cc % a.g.nc[0], cy = (cc / a.g.nc[0]) % a.g.nc[1], cz = cc / (a.g.nc[0] * a.g.nc[1]);
+      const double sgn = pass == 0 ? 1.0 : -1.0;
+      for (int k = 0; k < a.g.nst; ++k) {
+        int x = cx + a.g.st[3 * k], y = cy + a.g.st[3 * k + 1], z = cz + a.g.st[3 * k + 2];
+        x += x < 0 ? a.g.nc[0] : (x >= a.g.nc[0] ? -a.g.nc[0] : 0);
+        y += y < 0 ? a.g.nc[1] : (y >= a.g.nc[1] ? -a.g.nc[1] : 0);
+        z += z < 0 ? a.g.nc[2] : (z >= a.g.nc[2] ? -a.g.nc[2] : 0);
+        const int nb = x + a.g.nc[0] * (y + a.g.nc[1] * z);
+        const int j1 = a.cell_start[nb + 1];
+        for (int j = a.cell_start[nb] + tid; j < j1; j += kChainThreads) {
+          if (a.cell_ip[j] == ip) continue;
+          const double4 pj = a.cpos[j];
+          double d[3] = {r[0] - pj.x, r[1] - pj.y, r[2] - pj.z};
+#pragma unroll
+          for (int qd = 0; qd < 3; ++qd)
+            if (fabs(d[qd]) > a.box.half[qd]) d[qd] -= copysign(a.box.L[qd], d[qd]);
+          const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+          if (a.cut_inclusive ? (r2 > a.rcut2) : !(r2 < a.rcut2)) continue;
+          const int tp = a.tm.tpidx[ti * kMaxT + (int)pj.w];
+          if (pass == 0 && (r2 < a.tm.r2atat[tp] || r2 < a.tm.r2umin[tp])) {
+            ovl = 1;
+            continue;
+          }
+          e += sgn * chain_table_u(a, tp, r2);
+        }
+      }
+    }
+    if (ovl) atomicOr(&s_ovl, 1);
+    // standard Ewald: dS per k-vector kept for the commit
+    double dS[kChainKvec][8];
+    double drec = 0.0;
+    if (a.ewald == 1) {
+#pragma unroll
+      for (int u = 0; u < kChainKvec; ++u) {
+        const int kn = tid + u * kChainThreads;
+        if (kn >= a.nk) continue;
+        const double kx = a.kvec[3 * kn] * a.tpb[0], ky = a.kvec[3 * kn + 1] * a.tpb[1],
+                     kz = a.kvec[3 * kn + 2] * a.tpb[2];
+        double sxn, cxn, syn, cyn, szn, czn, sxo, cxo, syo, cyo, szo, czo;
+        sincos(kx * rn[0], &sxn, &cxn);
+        sincos(ky * rn[1], &syn, &cyn);
+        sincos(kz * rn[2], &szn, &czn);
+        sincos(kx * ro[0], &sxo, &cxo);
+        sincos(ky * ro[1], &syo, &cyo);
+        sincos(kz * ro[2], &szo, &czo);
+        double tn = 0.0, to = 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const double gxn = (c & 2) ? sxn : -sxn, gyn = (c & 1) ? syn : -syn;
+          const double gxo = (c & 2) ? sxo : -sxo, gyo = (c & 1) ? syo : -syo;
+          const double r1n = cxn * cyn - gxn * gyn, i1n = cxn * gyn + gxn * cyn;
+          const double r1o = cxo * cyo - gxo * gyo, i1o = cxo * gyo + gxo * cyo;
+          const double dr = q * ((r1n * czn - i1n * szn) - (r1o * czo - i1o * szo));
+          const double di = q * ((r1n * szn + i1n * czn) - (r1o * szo + i1o * czo));
+          dS[u][2 * c] = dr;
+          dS[u][2 * c + 1] = di;
+          const double Sr = a.sk[8 * kn + 2 * c], Si = a.sk[8 * kn + 2 * c + 1];
+          tn += (Sr + dr) * (Sr + dr) + (Si + di) * (Si + di);
+          to += Sr * Sr + Si * Si;
+        }
+        drec += a.kfac[kn] * (tn - to);
+      }
+    }
+    const double etot = chain_sum(e, sh);
+    const double rtot = chain_sum(drec, sh);
+    __syncthreads();
+    if (tid == 0) {
+      double du_rec = 0.0;
+      if (a.ewald) {
+        double sr = rtot;
+        if (a.lsurf) {
+          double m2o = 0.0, m2n = 0.0;
+          for (int d = 0; d < 3; ++d) {
+            const double mn = a.Mout[d] + q * (rn[d] - ro[d]);
+            m2o += a.Mout[d] * a.Mout[d];
+            m2n += mn * mn;
+          }
+          sr += a.surf_fac * (m2n - m2o);
+        }
+        du_rec = a.epsi4pi * sr;
+      }
+      const double du = etot + du_rec;
+      bool acc;
+      if (s_ovl) {
+        acc = false;
+      } else {
+        const double dured = a.beta * du;
+        if (dured > expmax) acc = false;
+        else if (dured < -expmax) acc = true;
+        else {
+          const double fac = exp(-dured);
+          acc = fac > 1.0 ? true : fac > chain_random(s_rng, &s_am);
+        }
+      }
+      a.events[t] = s_ovl ? -1 : (acc ? 1 : 0);
+      s_acc = acc;
+      s_stop = 0;
+      if (acc) {
+        dsum += du;
+        ++nacc;
+        if (a.structural && chain_cell(a, rn) != a.slot_cell[a.slot_of[ip]]) {
+          s_stop = 1;
+          a.stat[3] = ip;
+          for (int d = 0; d < 3; ++d) a.stat[4 + d] = rn[d];
+        }
+      }
+    }
+    __syncthreads();
+    ++done;
+    if (s_stop) break;
+    if (s_acc) {  // MCUpdate in place
+      if (a.ewald == 1) {
+#pragma unroll
+        for (int u = 0; u < kChainKvec; ++u) {
+          const int kn = tid + u * kChainThreads;
+          if (kn >= a.nk) continue;
+#pragma unroll
+          for (int c = 0; c < 8; ++c) a.sk[8 * kn + c] += dS[u][c];
+        }
+      }
+      if (tid == 0) {
+        for (int d = 0; d < 3; ++d) {
+          a.Mout[d] += q * (rn[d] - ro[d]);
+          a.Mtot[d] = a.Mout[d];
+          a.ro[3 * ip + d] = rn[d];
+        }
+        double4 p = a.cpos[a.slot_of[ip]];
+        p.x = rn[0];
+        p.y = rn[1];
+        p.z = rn[2];
+        a.cpos[a.slot_of[ip]] = p;
+      }
+      __threadfence_block();
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    for (int k = 0; k < 3; ++k) a.rng[k] = s_rng[k];
+    *a.am = s_am;
+    *a.next = s_next;
+    a.stat[0] = dsum;
+    a.stat[1] = nacc;
+    a.stat[2] = done;
+  }
+}
+
+}  // namespace
+
+bool chain_eligible(const Ctx* c) {
+  return c->P.ewald_mode != MSG_EWALD_SPM && c->P.list_mode != MSG_LIST_VERLET && c->nkvec <= kChainThreads * kChainKvec;
+}
+
+// Runs ntrial trials of the chain (several launches when accepted moves change cells).
+int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, int* iseed, int* rng_state,
+             double* am, double* du_sum, int* naccept, int* events) {
+  // device scratch: rng (3 ints) + next (1 int) + events (ntrial ints); am + stat (7) + dtran (npt)
+  int* d_i = nullptr;
+  double* d_d = nullptr;
+  MSG_HIP(hipMalloc((void**)&d_i, (4 + (size_t)std::max(ntrial, 1)) * sizeof(int)));
+  if (hipMalloc((void**)&d_d, (8 + kMaxT) * sizeof(double)) != hipSuccess) {
+    hipFree(d_i);
+    return set_error(MSG_ERR_HIP, "mc_chain: allocation failed");
+  }
+  struct Free {
+    void* a;
+    void* b;
+    ~Free() {
+      hipFree(a);
+      hipFree(b);
+    }
+  } guard{d_i, d_d};
+  int hi[4] = {*iseed, rng_state[0], rng_state[1], c->mc_next};
+  double hd[8 + kMaxT] = {};
+  hd[0] = *am;
+  for (int t = 0; t < c->npt; ++t) hd[8 + t] = dtran[t];
+  MSG_HIP(hipMemcpyAsync(d_i, hi, sizeof(hi), hipMemcpyHostToDevice, c->stream));
+  MSG_HIP(hipMemcpyAsync(d_d, hd, sizeof(hd), hipMemcpyHostToDevice, c->stream));
+  const bool structural = c->grid.ncell > 1;
+  double dsum = 0.0;
+  int nacc = 0, t0 = 0;
+  while (t0 < ntrial) {
+    if (!c->fresh)
+      if (int e = utotal_refresh(c)) return e;
+    ChainArgs a;
+    a.np = c->np;
+    a.ntrial = ntrial - t0;
+    a.isamp = isamp;
+    a.structural = structural;
+    a.ewald = c->P.ewald_mode;
+    a.lsurf = c->P.lsurf;
+    a.ro = c->d_ro;
+    a.az = c->d_az;
+    a.ipt = c->d_ipt;
+    a.cpos = (double4*)c->d_cpos;
+    a.cell_ip = c->d_cell_ip;
+    a.cell_start = c->d_cell_start;
+    a.slot_of = c->d_slot_of;
+    a.slot_cell = c->d_slot_cell;
+    a.g = c->grid;
+    for (int d = 0; d < 3; ++d) {
+      a.box.L[d] = c->P.boxlen[d];
+      a.box.half[d] = 0.5 * c->P.boxlen[d];
+      a.box.inv[d] = 1.0 / c->P.boxlen[d];
+      a.tpb[d] = 2.0 * M_PI / c->P.boxlen[d];
+    }
+    a.rcut2 = c->P.rcut * c->P.rcut;
+    a.cut_inclusive = c->P.list_mode != MSG_LIST_CELL;
+    a.tcoef = c->d_tcoef64;
+    a.tbins = c->d_tbins;
+    a.tm = c->tmeta;
+    a.nk = c->P.ewald_mode == MSG_EWALD_STD ? c->nkvec : 0;
+    a.kvec = c->d_kvec;
+    a.kfac = c->d_kfac;
+    a.sk = c->d_sk;
+    a.Mout = c->d_out + c->nptpt + 4;
+    a.Mtot = c->d_M;
+    const double vol = c->P.boxlen[0] * c->P.boxlen[1] * c->P.boxlen[2];
+    a.surf_fac = 2.0 * M_PI / (3.0 * vol);
+    a.epsi4pi = c->P.epsi4pi;
+    a.dtran = d_d + 8;
+    a.beta = beta;
+    a.rng = d_i;
+    a.am = d_d;
+    a.next = d_i + 3;
+    a.events = d_i + 4 + t0;
+    a.stat = d_d + 1;
+    prof_begin(c, MSG_K_DE);
+    hipLaunchKernelGGL(k_mc_chain, dim3(1), dim3(kChainThreads), 0, c->stream, a);
+    prof_end(c, MSG_K_DE);
+    MSG_HIP(hipGetLastError());
+    double st[7];
+    MSG_HIP(hipMemcpyAsync(st, d_d + 1, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    MSG_HIP(hipStreamSynchronize(c->stream));
+    dsum += st[0];
+    nacc += (int)st[1];
+    const int done = (int)st[2];
+    t0 += done;
+    if (st[3] >= 0.0) {
+      // the last trial was an accepted move into another cell: commit it here (the cell
+      // list is rebuilt by the UTotal pass before the next launch)
+      const int ip = (int)st[3];
+      const double rn[3] = {st[4], st[5], st[6]};
+      if (int e = commit_move(c, ip, rn)) return e;
+    }
+  }
+  MSG_HIP(hipMemcpy(hi, d_i, sizeof(hi), hipMemcpyDeviceToHost));
+  MSG_HIP(hipMemcpy(hd, d_d, sizeof(double), hipMemcpyDeviceToHost));
+  if (events) MSG_HIP(hipMemcpy(events, d_i + 4, ntrial * sizeof(int), hipMemcpyDeviceToHost));
+  *iseed = hi[0];
+  rng_state[0] = hi[1];
+  rng_state[1] = hi[2];
+  *am = hd[0];
+  c->mc_next = hi[3];
+  if (du_sum) *du_sum = dsum;
+  if (naccept) *naccept = nacc;
+  return MSG_OK;
+}
+
+}  // namespace msg

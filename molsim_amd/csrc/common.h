@@ -227,6 +227,9 @@ int launch_zero(Ctx* c);                           // counters and flags
 int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ovl,
                   double* du_tot, double* du_rec, double* du_twob);
 int commit_move(Ctx* c, int ip, const double* rnew);
+bool chain_eligible(const Ctx* c);                 // device-resident MC chain applies
+int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, int* iseed,
+             int* rng_state, double* am, double* du_sum, int* naccept, int* events);
 int dutotal_launch(Ctx* c, int ntrial, const int* ip, const double* rotm);
 int dutotal_collect(Ctx* c, int ntrial, int* ovl, double* du_tot, double* du_rec, double* du_twob);
 int utotal_refresh(Ctx* c);                        // UTotal sequence + sync (for dE state)

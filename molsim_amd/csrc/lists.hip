@@ -489,15 +489,19 @@ int launch_bin(Ctx* c) {
 
 int launch_sort(Ctx* c) {
   const int np = c->np;
-  prof_begin(c, MSG_K_CELLSORT);
+  prof_begin(c, MSG_K_SCAN);
   hipLaunchKernelGGL(k_scan, dim3(3), dim3(1024), 0, c->stream, scan_args(c));
+  prof_end(c, MSG_K_SCAN);
+  prof_begin(c, MSG_K_SCATTER);
   hipLaunchKernelGGL(k_scatter, dim3((np + 255) / 256), dim3(256), 0, c->stream, scatter_args(c));
+  prof_end(c, MSG_K_SCATTER);
   const FinalizeArgs f = finalize_args(c);
+  prof_begin(c, MSG_K_FINALIZE);
   if (c->fp64)
     hipLaunchKernelGGL(k_finalize<double>, dim3(finalize_blocks(f)), dim3(256), 0, c->stream, f);
   else
     hipLaunchKernelGGL(k_finalize<float>, dim3(finalize_blocks(f)), dim3(256), 0, c->stream, f);
-  prof_end(c, MSG_K_CELLSORT);
+  prof_end(c, MSG_K_FINALIZE);
   MSG_HIP(hipGetLastError());
   return MSG_OK;
 }

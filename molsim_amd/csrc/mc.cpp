@@ -55,6 +55,8 @@ int msg_mc_pass(msg_ctx* h, int ntrial, int isamp, const double* dtran, double b
   if (!h || ntrial < 0 || !dtran || !iseed || !rng_state || !am)
     return set_error(MSG_ERR_ARG, "msg_mc_pass: bad arguments");
   Ctx* c = &h->c;
+  if (chain_eligible(c) && !c->prof)  // the whole pass on the device
+    return mc_chain(c, ntrial, isamp, dtran, beta, iseed, rng_state, am, du_sum, naccept, events);
   const double expmax = 87.0;  // Src/mol.F90:62
   std::vector<double> ro(3 * (size_t)c->np);
   MSG_HIP(hipStreamSynchronize(c->stream));

@@ -311,7 +311,10 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   typedef typename R4T<real>::t R4;
   constexpr int CAP = 128;
   constexpr int NBP = kBX * kBY * kBZ;
-  __shared__ unsigned long long acc[NBP];
+  // rows padded to kBX + 1 accumulators: the lanes adding one particle's rows (same x,
+  // consecutive y/z) then fall into different LDS banks instead of one
+  constexpr int ROW = kBX + 1, NACC = ROW * kBY * kBZ;
+  __shared__ unsigned long long acc[NACC];
   __shared__ real th[CAP][3 * P];
   __shared__ int loc[CAP];
   __shared__ int rng[CAP];      // packed y0 | ny << 8 | z0 << 16 | nz << 24
@@ -337,7 +340,7 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
     r_beg[t] = beg;
     r_len[t] = len;
   }
-  for (int k = threadIdx.x; k < NBP; k += 256) acc[k] = 0ull;
+  for (int k = threadIdx.x; k < NACC; k += 256) acc[k] = 0ull;
   __syncthreads();
   if (threadIdx.x == 0) {
     int pref = 0;
@@ -411,7 +414,7 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
       const int lx0 = lp & 1023;
       int x0, nx;
       brick_range<P>(lx0, kBX, whole[0], x0, nx);
-      unsigned long long* rowp = acc + kBX * (ly + kBY * lz);
+      unsigned long long* rowp = acc + ROW * (ly + kBY * lz);
       for (int ix = x0; ix < x0 + nx; ++ix) {
         int lx = lx0 - ix;
         lx += lx < 0 ? K : 0;
@@ -424,7 +427,7 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   real* Q = (real*)a.Q;
   for (int k = threadIdx.x; k < NBP; k += 256) {
     const int lx = k % kBX, ly = (k / kBX) % kBY, lz = k / (kBX * kBY);
-    const double v = (double)(long long)acc[k] * (1.0 / kQScale);
+    const double v = (double)(long long)acc[ROW * (ly + kBY * lz) + lx] * (1.0 / kQScale);
     Q[(lo[0] + lx) + (size_t)K * ((lo[1] + ly) + (size_t)K * (lo[2] + lz))] = (real)v;
   }
 }

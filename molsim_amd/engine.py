@@ -28,8 +28,8 @@ _lib = None
 LIST_MODES = {"all": 0, "cell": 1, "verlet": 2}
 EWALD_MODES = {"none": 0, "std": 1, "spm": 2}
 PRECISIONS = {"mixed": 0, "fp64": 1}
-KERNELS = ["bin", "cellsort", "pair", "spread", "fft_x", "fft_y", "fft_zconv", "fft_yi",
-           "fft_xi", "gather", "reduce", "ewald_std", "dutotal", "vlist"]
+KERNELS = ["bin", "scan", "pair", "spread", "fft_x", "fft_y", "fft_zconv", "fft_yi",
+           "fft_xi", "gather", "reduce", "ewald_std", "dutotal", "vlist", "scatter", "finalize"]
 
 
 class MolsimGpuError(RuntimeError):

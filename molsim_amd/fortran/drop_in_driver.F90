@@ -1,7 +1,7 @@
 !> Fortran caller of the C-ABI through the MolsimGpu module, in the way a MOLSIM build calls
 !! it: reads a system (positions, charges, types, box, Ewald parameters) from a text file,
-!! builds the table (PotTwoBodyTab semantics), runs UTotal, a few DUTotal trials and one
-!! MCUpdate, and prints the results for the caller to compare with the oracle.
+!! builds the table (PotTwoBodyTab semantics), runs UTotal, a few DUTotal trials, one
+!! MCUpdate and a short MCPass, and prints the results for the caller to compare.
 !!
 !! Input file (free format): np npt / boxlen(3) / rcut alpha epsi4pi / nmesh order lsurf /
 !! form lewald / z(npt) / sig(npt) / eps(npt) / rad(npt) / (ro(1:3,ip), az(ip), iptpn(ip), ip=1,np)
@@ -21,6 +21,9 @@ program drop_in_driver
    real(c_double), allocatable :: ro(:,:), az(:), force(:,:), ubuf(:), r2umin(:), r2atat(:), rtrial(:,:)
    real(c_double), allocatable :: u_twob(:), du_twob(:)
    real(c_double) :: boxlen(3), rcut, alpha, epsi4pi, u_rec, u_tot, virial, du_rec, du_tot
+   integer(c_int) :: iseed, rngstate(2), naccept
+   integer(c_int), allocatable :: events(:)
+   real(c_double) :: am, dusum, dtran(8)
 
    call get_command_argument(1, fin)
    open(10, file=trim(fin), status='old')
@@ -105,6 +108,17 @@ program drop_in_driver
    call GpuCheck(msg_commit(ctx, 1_c_int, iptrial(1:1), rtrial(:,1:1)), 'msg_commit')
    call GpuCheck(msg_utotal(ctx, u_twob, u_rec, u_tot, force, virial), 'msg_utotal')
    write(*,'(a,es24.16)') 'u_tot_after ', u_tot
+
+   ! MCPass of 50 sequential single-particle moves with MOLSIM's generator
+   iseed = -4711
+   rngstate = -1
+   am = 0d0
+   dtran = 2d0
+   allocate(events(50))
+   call GpuCheck(msg_mc_pass(ctx, 50_c_int, 0_c_int, dtran, 1d3 / (8.314472d0 * 298d0), iseed, &
+                             rngstate, am, dusum, naccept, events), 'msg_mc_pass')
+   write(*,'(a,i6,es24.16)') 'mc_pass ', naccept, dusum
+   write(*,'(a,i12,2i12)') 'rng_state ', iseed, rngstate
 
    call msg_destroy(ctx)
 

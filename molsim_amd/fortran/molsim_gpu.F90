@@ -123,6 +123,57 @@ module MolsimGpu
          real(c_double), intent(in) :: rotm(3,*)
       end function
 
+      ! VelVer on the device (md.F90:252-410): masses per type, velocities rod(3,np)
+      integer(c_int) function msg_md_init(ctx, mass, vel) bind(C, name='msg_md_init')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: mass(*)
+         real(c_double), intent(in) :: vel(3,*)
+      end function
+
+      integer(c_int) function msg_md_run(ctx, nsteps, dt, log) bind(C, name='msg_md_run')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nsteps
+         real(c_double), value :: dt
+         real(c_double), intent(out) :: log(2,*)
+      end function
+
+      integer(c_int) function msg_get_positions(ctx, ro) bind(C, name='msg_get_positions')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: ro(3,*)
+      end function
+
+      integer(c_int) function msg_get_velocities(ctx, vel) bind(C, name='msg_get_velocities')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: vel(3,*)
+      end function
+
+      ! MOLSIM's Random (mollib.F90:2689-2713) and MCPass of single-particle moves
+      real(c_double) function msg_random(iseed, state, am) bind(C, name='msg_random')
+         import :: c_int, c_double
+         integer(c_int), intent(inout) :: iseed
+         integer(c_int), intent(inout) :: state(2)
+         real(c_double), intent(inout) :: am
+      end function
+
+      integer(c_int) function msg_mc_pass(ctx, ntrial, isamp, dtran, beta, iseed, state, am, &
+                                          du_sum, naccept, events) bind(C, name='msg_mc_pass')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: ntrial, isamp
+         real(c_double), intent(in) :: dtran(*)
+         real(c_double), value :: beta
+         integer(c_int), intent(inout) :: iseed
+         integer(c_int), intent(inout) :: state(2)
+         real(c_double), intent(inout) :: am
+         real(c_double), intent(out) :: du_sum
+         integer(c_int), intent(out) :: naccept
+         integer(c_int), intent(out) :: events(*)
+      end function
+
    end interface
 
 contains

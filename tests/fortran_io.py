@@ -42,6 +42,10 @@ def parse(out):
             res["dutotal"].append((int(t[2]), float(t[3])))
         elif t[0] in ("u_tot", "u_rec", "u_twob0", "virial", "u_tot_after"):
             res[t[0]] = float(t[1])
+        elif t[0] == "mc_pass":
+            res["mc_pass"] = (int(t[1]), float(t[2]))
+        elif t[0] == "rng_state":
+            res["rng_state"] = [int(x) for x in t[1:4]]
     n = len(res["force"])
     res["force"] = np.array([res["force"][i] for i in range(n)])
     return res

@@ -41,3 +41,14 @@ def test_fortran_driver_matches_oracle(tmp_path):
     ro2[trials[0][0]] = trials[0][1]
     ref2 = oracle_system(systems.SystemSpec(**{**sp.__dict__, "ro": ro2}), tab).utotal()
     assert abs(got["u_tot_after"] - ref2["u_tot"]) < 1e-5 * abs(ref2["u_tot"])
+    # the Fortran binding of MCPass against the same pass through the Python mirror
+    from gpu_helpers import cabi_table
+    from molsim_amd.engine import Engine, Random
+    eng = Engine(sp, precision="mixed", table=cabi_table(tab))
+    eng.mc_update(trials[0][0], trials[0][1])
+    rng = Random(-4711)
+    dsum, nacc, _ = eng.mc_pass(50, 2.0, 1e3 / (8.314472 * 298.0), rng)
+    assert got["mc_pass"][0] == nacc
+    assert abs(got["mc_pass"][1] - dsum) < 1e-6 * (1.0 + abs(dsum))
+    assert got["rng_state"] == [rng.iseed.value, int(rng.state[0]), int(rng.state[1])]
+    eng.close()

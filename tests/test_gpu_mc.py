@@ -45,29 +45,41 @@ def _oracle_mc(sp, tab, ntrial, dtran, beta, seed):
         events.append(-1 if du["overlap"] else int(acc))
         if acc:
             ro[ip] = rn
-    return np.array(events), ro
+    return np.array(events), ro, rng
 
 
-@pytest.mark.parametrize("ewald,precision", [("std", "fp64"), ("spm", "fp64"), ("spm", "mixed")])
+@pytest.mark.parametrize("ewald,precision", [("std", "fp64"), ("std_cell", "fp64"), ("spm", "fp64"),
+                                             ("spm", "mixed")])
 def test_mc_pass_matches_oracle_loop(ewald, precision):
+    """std: the device-resident chain (all pairs, never leaves the launch); std_cell: the
+    device chain on a 3x3x3 cell grid (accepted moves into another cell end a launch and the
+    pass resumes after the rebuild); spm: the host-driven chain."""
     sp = systems.small_spme_system(n=200, L=36.0, nmesh=32, alpha=0.3, rcut=11.0)
     if ewald == "std":
         sp = systems.SystemSpec(**{**sp.__dict__, "ewald": "std", "ncut": 6, "list_mode": "all",
                                    "rcut": 17.9})
         sp.pot["rcut"] = 17.9
+    elif ewald == "std_cell":
+        sp = systems.SystemSpec(**{**sp.__dict__, "ewald": "std", "ncut": 6, "list_mode": "cell"})
     tab = oracle.build_table(sp.pot)
     ntrial, dtran, beta = 300, 2.0, systems.beta_kjmol()
     print(f"mc ewald={ewald} precision={precision} n={sp.n}", flush=True)
     eng = Engine(sp, precision=precision, table=cabi_table(tab))
     rng = Random(-4711)
     dsum, nacc, ev = eng.mc_pass(ntrial, dtran, beta, rng)
-    ref_ev, ref_ro = _oracle_mc(sp, tab, ntrial, dtran, beta, -4711)
+    ref_ev, ref_ro, ref_rng = _oracle_mc(sp, tab, ntrial, dtran, beta, -4711)
     agree = float(np.mean(ev == ref_ev))
     print(f"accepted {nacc}/{ntrial}, decisions agreeing with the oracle loop {agree:.4f}", flush=True)
     if precision == "fp64":
         assert np.array_equal(ev, ref_ev)
         ro = eng.positions()
         assert np.abs(ro - ref_ro).max() < 1e-9
+        # the generator state carried through the pass continues the reference's sequence
+        assert rng() == ref_rng()
+        # and the state after the pass equals a fresh evaluation of the final positions
+        full = eng.utotal()
+        ref = oracle_system(systems.SystemSpec(**{**sp.__dict__, "ro": ref_ro}), tab).utotal()
+        assert abs(full["u_tot"] - ref["u_tot"]) <= 1e-9 * abs(ref["u_tot"]) + 1e-9
     else:
         assert agree >= 0.99
     assert 0 < nacc < ntrial
