@@ -87,8 +87,8 @@ int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, hipEvent_t fork,
   if (int e = run_stage(p.bin, s)) return e;
   if (int e = run_stage(p.scan, s)) return e;
   if (int e = run_stage(p.scatter, s)) return e;
-  if (int e = run_stage(p.finalize, s)) return e;
   if (p.any_spme && serial_pass()) {
+    if (int e = run_stage(p.finalize, s)) return e;
     if (int e = run_stage(p.pair, s)) return e;
     if (int e = run_stage(p.spread, s)) return e;
     for (int k = 0; k < 5; ++k)
@@ -97,6 +97,7 @@ int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, hipEvent_t fork,
   } else if (p.any_spme) {
     MSG_HIP(hipEventRecord(fork, s));
     MSG_HIP(hipStreamWaitEvent(side, fork, 0));
+    if (int e = run_stage(p.finalize, side)) return e;  // cell order beside the mesh branch
     if (int e = run_stage(p.pair, side)) return e;
     MSG_HIP(hipEventRecord(join, side));
     if (int e = run_stage(p.spread, s)) return e;
@@ -105,6 +106,7 @@ int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, hipEvent_t fork,
     MSG_HIP(hipStreamWaitEvent(s, join, 0));
     if (int e = run_stage(p.gather, s)) return e;
   } else {
+    if (int e = run_stage(p.finalize, s)) return e;
     if (int e = run_stage(p.pair, s)) return e;
   }
   if (int e = run_stage(p.reduce, s)) return e;

@@ -255,16 +255,22 @@ int utotal_sequence(Ctx* c) {
     if ((e = verlet_check(c, rebuild))) return e;
   if ((e = launch_bin(c))) return e;
   if ((e = launch_sort(c))) return e;
+  // the cell order is only needed by the pair branch: with SPME it completes beside the mesh
+  const bool branch = c->P.ewald_mode == MSG_EWALD_SPM && !c->prof;
+  const bool cells_on_side = branch && c->P.list_mode != MSG_LIST_VERLET;
+  if (!cells_on_side)
+    if ((e = launch_finalize(c))) return e;
   if (rebuild) {
     if ((e = launch_vlist_build(c))) return e;
     c->vl_valid = true;
   }
-  if (c->P.ewald_mode == MSG_EWALD_SPM && !c->prof) {
+  if (branch) {
     MSG_HIP(hipEventRecord(c->ev_fork, c->stream));
     MSG_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     hipStream_t main = c->stream;
     c->stream = c->side;
-    e = launch_pair(c, false);
+    e = cells_on_side ? launch_finalize(c) : MSG_OK;
+    if (!e) e = launch_pair(c, false);
     c->stream = main;
     if (e) return e;
     MSG_HIP(hipEventRecord(c->ev_join, c->side));
@@ -395,7 +401,7 @@ void msg_destroy(msg_ctx* h) {
   void* ptrs[] = {c->d_ro, c->d_az, c->d_ipt, c->d_tcoef, c->d_tbins, c->d_cellid, c->d_cell_chunk,
                   c->d_cell_count, c->d_cell_start, c->d_cell_fill, c->d_cell_tmp, c->d_cell_ip,
                   c->d_slot_of, c->d_cpos, c->d_ctype, c->d_slot_cell, c->d_brick_count,
-                  c->d_brick_start, c->d_brick_fill, c->d_brick_tmp, c->d_brick_ip, c->d_mm,
+                  c->d_brick_start, c->d_brick_fill, c->d_brick_ip, c->d_mm,
                   c->d_frac, c->d_Q, c->d_C, c->d_G, c->d_tw, c->d_kfac, c->d_kvec, c->d_sk,
                   c->d_force, c->d_part, c->d_out, c->d_flags, c->d_de, c->d_de_ip, c->d_de_r,
                   c->d_tcoef64, c->d_gk, c->d_item_cell, c->d_bq, c->d_bmm, c->d_M,
@@ -488,7 +494,7 @@ int msg_nlist(msg_ctx* h) {
   Ctx* c = &h->c;
   if (!c->have_positions) return set_error(MSG_ERR_STATE, "positions have not been set");
   int e;
-  if ((e = launch_bin(c)) || (e = launch_sort(c))) return e;
+  if ((e = launch_bin(c)) || (e = launch_sort(c)) || (e = launch_finalize(c))) return e;
   if (c->P.list_mode == MSG_LIST_VERLET) {
     if ((e = launch_vlist_build(c))) return e;
     c->vl_valid = true;

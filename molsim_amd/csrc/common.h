@@ -127,7 +127,7 @@ struct Ctx {
   bool have_ewald = false;
   MeshGrid mesh{};
   int* d_brick_count = nullptr, *d_brick_start = nullptr, *d_brick_fill = nullptr;
-  int* d_brick_tmp = nullptr, *d_brick_ip = nullptr;
+  int* d_brick_ip = nullptr;
   int* d_mm = nullptr;           // np, packed meshmax (caller order)
   void* d_bq = nullptr;          // np: (frac x, y, z, q) in brick order (real4)
   int* d_bmm = nullptr;          // np: packed meshmax in brick order
@@ -213,7 +213,8 @@ int pack_table(Ctx* c);
 // Launch sequences (each .hip file)
 int lists_setup(Ctx* c);                           // cell grid + buffers
 int launch_bin(Ctx* c);                            // K1 cell id + mesh bin + counts + dipole
-int launch_sort(Ctx* c);                           // scan + scatter + finalize
+int launch_sort(Ctx* c);                           // scan + scatter (bricks complete)
+int launch_finalize(Ctx* c);                       // cell order (after launch_sort)
 int launch_pair(Ctx* c, bool last);                // K4
 int pair_blocks(Ctx* c);
 bool pair_nbpar(const Ctx* c);                           // workgroups of the pair kernel

@@ -7,10 +7,12 @@
 //               and per-cell counts; the same pass bins particles by the SPME mesh brick of
 //               their B-spline base point and forms per-block partial dipoles (UEwaldSurf);
 //   k_scan      exclusive scan of the counts (one workgroup per array);
-//   k_scatter   each particle takes a slot of its cell / brick;
+//   k_scatter   each particle takes a slot of its cell and of its brick, and writes the
+//               brick-ordered spline data directly (no result depends on the order within
+//               a brick);
 //   k_finalize  one wavefront per cell ranks its members by index (descending, so the cell
 //               order equals the reference's traversal order bit for bit) and writes the
-//               cell-ordered position/charge/type arrays; bricks are ranked ascending.
+//               cell-ordered position/type arrays.  It runs beside the mesh branch.
 #include <type_traits>
 
 #include "common.h"
@@ -139,6 +141,73 @@ __device__ void block_scan(int n, const int* __restrict__ cnt, int div, int* out
   __syncthreads();
 }
 
+// Exclusive scans of cnt[0..n) and of ceil(cnt / div) (out2, may be null) by one 1024-thread
+// workgroup from ONE load of the counts: each thread keeps its run of at most VMAX values in
+// registers, so the scan costs one memory round trip.  out1[n] / out2[n] get the totals.
+template <int VMAX>
+__device__ void block_scan_dual(int n, const int* __restrict__ cnt, int div, int* out1, int* out2,
+                                int* wsum) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int per = (n + 1023) / 1024;
+  const int b0 = threadIdx.x * per;
+  int v[VMAX];
+  int s1 = 0, s2 = 0;
+#pragma unroll
+  for (int u = 0; u < VMAX; ++u) {
+    v[u] = (u < per && b0 + u < n) ? cnt[b0 + u] : 0;
+    s1 += v[u];
+    s2 += (v[u] + div - 1) / div;
+  }
+  int i1 = s1, i2 = s2;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t1 = __shfl_up(i1, off, 64), t2 = __shfl_up(i2, off, 64);
+    if (lane >= off) {
+      i1 += t1;
+      i2 += t2;
+    }
+  }
+  if (lane == 63) {
+    wsum[w] = i1;
+    wsum[16 + w] = i2;
+  }
+  __syncthreads();
+  int bef1 = 0, bef2 = 0, tot1 = 0, tot2 = 0;
+  for (int k = 0; k < 16; ++k) {
+    if (k < w) {
+      bef1 += wsum[k];
+      bef2 += wsum[16 + k];
+    }
+    tot1 += wsum[k];
+    tot2 += wsum[16 + k];
+  }
+  int r1 = bef1 + i1 - s1, r2 = bef2 + i2 - s2;
+#pragma unroll
+  for (int u = 0; u < VMAX; ++u) {
+    if (u < per && b0 + u < n) {
+      out1[b0 + u] = r1;
+      if (out2) out2[b0 + u] = r2;
+    }
+    r1 += v[u];
+    r2 += (v[u] + div - 1) / div;
+  }
+  if (threadIdx.x == 0) {
+    out1[n] = tot1;
+    if (out2) out2[n] = tot2;
+  }
+  __syncthreads();
+}
+
+__device__ inline void scan_counts(int n, const int* cnt, int div, int* out1, int* out2, int* wsum) {
+  if (n <= 8 * 1024) block_scan_dual<8>(n, cnt, div, out1, out2, wsum);
+  else if (n <= 32 * 1024) block_scan_dual<32>(n, cnt, div, out1, out2, wsum);
+  else if (n <= 64 * 1024) block_scan_dual<64>(n, cnt, div, out1, out2, wsum);
+  else {
+    block_scan(n, cnt, 1, out1, wsum);
+    if (out2) block_scan(n, cnt, div, out2, wsum);
+  }
+}
+
 struct ScanArgs {
   int n0;                 // cells
   const int* cnt0;
@@ -158,7 +227,7 @@ struct ScanArgs {
 // Workgroup 0: cell offsets and the per-cell home-chunk offsets of the pair kernel;
 // workgroup 1: brick offsets.  Both clear their fill counters.  Workgroup 2: total dipole.
 __device__ __forceinline__ void scan_body(const ScanArgs& a, int bid) {
-  __shared__ int wsum[16];
+  __shared__ int wsum[32];
   __shared__ double dsum[16][3];
   if (bid == 2) {  // total dipole sum q r (surface term), from k_bin's partials
     double m[3] = {0.0, 0.0, 0.0};
@@ -183,8 +252,7 @@ __device__ __forceinline__ void scan_body(const ScanArgs& a, int bid) {
   const int* cnt = bid == 0 ? a.cnt0 : a.cnt1;
   int* fill = bid == 0 ? a.fill0 : a.fill1;
   if (n <= 0) return;
-  block_scan(n, cnt, 1, bid == 0 ? a.start0 : a.start1, wsum);
-  if (bid == 0) block_scan(n, cnt, a.hb, a.chunk0, wsum);
+  scan_counts(n, cnt, a.hb, bid == 0 ? a.start0 : a.start1, bid == 0 ? a.chunk0 : nullptr, wsum);
   for (int i = threadIdx.x; i < n; i += 1024) fill[i] = 0;
 }
 
@@ -208,9 +276,18 @@ struct ScatterArgs {
   MeshGrid m;
   const int* brick_start;
   int* brick_fill;
-  int* brick_tmp;
+  int* brick_ip;
+  const void* frac;
+  const double* az;
+  void* bq;       // brick order: (frac x, frac y, frac z, q) reals
+  int* bmm;       // brick order: packed meshmax
 };
 
+// Each particle takes a slot of its cell (ranked later) and of its mesh brick.  The brick
+// order within a brick is left as the atomics hand it out: spreading sums exact 64-bit
+// fixed-point contributions and the gather works per particle, so no result depends on it,
+// and the mesh branch does not wait for a ranking pass.
+template <typename real>
 __device__ __forceinline__ void scatter_body(const ScatterArgs& a, int bid) {
   const int i = bid * 256 + threadIdx.x;
   if (i >= a.np) return;
@@ -219,26 +296,37 @@ __device__ __forceinline__ void scatter_body(const ScatterArgs& a, int bid) {
   if (a.use_mesh) {
     const int p = a.mm[i];
     const int b = (p & 1023) / kBX + a.m.nb[0] * (((p >> 10) & 1023) / kBY + a.m.nb[1] * ((p >> 20) / kBZ));
-    a.brick_tmp[a.brick_start[b] + atomicAdd(&a.brick_fill[b], 1)] = i;
+    const int dst = a.brick_start[b] + atomicAdd(&a.brick_fill[b], 1);
+    typedef typename std::conditional<sizeof(real) == 4, float4, double4>::type R4;
+    const real* fr = (const real*)a.frac;
+    R4 v;
+    v.x = fr[3 * i];
+    v.y = fr[3 * i + 1];
+    v.z = fr[3 * i + 2];
+    v.w = (real)a.az[i];
+    ((R4*)a.bq)[dst] = v;
+    a.bmm[dst] = p;
+    a.brick_ip[dst] = i;
   }
 }
 
-__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) { scatter_body(a, blockIdx.x); }
+template <typename real>
+__global__ __launch_bounds__(256) void k_scatter(ScatterArgs a) { scatter_body<real>(a, blockIdx.x); }
 
+template <typename real>
 __global__ __launch_bounds__(256) void k_scatter_seg(const ScatterArgs* __restrict__ args,
                                                      const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
   if (s.x < 0) return;  // no work for this XCD slot
-  scatter_body(args[s.x], s.y);
+  scatter_body<real>(args[s.x], s.y);
 }
 
 struct FinalizeArgs {
-  int ncell, nbrick;
+  int ncell;
   const int* cell_start;
   const int* cell_tmp;
   const int* cell_chunk;
   const double* ro;
-  const double* az;
   const int* ipt;
   int* cell_ip;
   int* slot_of;
@@ -246,18 +334,11 @@ struct FinalizeArgs {
   int* ctype;
   int* slot_cell;
   int* item_cell;
-  const int* brick_start;
-  const int* brick_tmp;
-  int* brick_ip;
-  const int* mm;
-  const void* frac;
-  void* bq;       // brick order: (frac x, frac y, frac z, q) reals
-  int* bmm;       // brick order: packed meshmax
 };
 
-// One wavefront per cell (then per brick): rank members by particle index and write the
-// ordered arrays; cells also list their pair-kernel work items.
-template <typename real>
+// One wavefront per cell: rank members by particle index (descending, the reference's
+// linked-list order) and write the cell-ordered arrays; cells also list their pair-kernel
+// work items.
 __device__ __forceinline__ void finalize_body(const FinalizeArgs& a, int bid) {
   const int wv = bid * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -281,43 +362,16 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& a, int bid) {
     }
     const int i0 = a.cell_chunk[wv], ni = a.cell_chunk[wv + 1] - i0;
     for (int k = lane; k < ni; k += 64) a.item_cell[i0 + k] = wv;
-  } else if (wv - a.ncell < a.nbrick) {
-    const int b = wv - a.ncell;
-    const int s0 = a.brick_start[b], n = a.brick_start[b + 1] - s0;
-    const real* fr = (const real*)a.frac;
-    typedef typename std::conditional<sizeof(real) == 4, float4, double4>::type R4;
-    R4* bq = (R4*)a.bq;
-    for (int k = lane; k < ((n + 63) & ~63); k += 64) {
-      const int ip = k < n ? a.brick_tmp[s0 + k] : 0x7fffffff;
-      int rank = 0;
-      if (n <= 64) {
-        for (int j = 0; j < n; ++j) rank += __shfl(ip, j, 64) < ip;
-      } else {
-        for (int j = 0; j < n; ++j) rank += a.brick_tmp[s0 + j] < ip;
-      }
-      if (k >= n) continue;
-      const int dst = s0 + rank;
-      a.brick_ip[dst] = ip;
-      R4 v;
-      v.x = fr[3 * ip];
-      v.y = fr[3 * ip + 1];
-      v.z = fr[3 * ip + 2];
-      v.w = (real)a.az[ip];
-      bq[dst] = v;
-      a.bmm[dst] = a.mm[ip];
-    }
   }
 }
 
-template <typename real>
-__global__ __launch_bounds__(256) void k_finalize(FinalizeArgs a) { finalize_body<real>(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_finalize(FinalizeArgs a) { finalize_body(a, blockIdx.x); }
 
-template <typename real>
 __global__ __launch_bounds__(256) void k_finalize_seg(const FinalizeArgs* __restrict__ args,
                                                       const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
   if (s.x < 0) return;  // no work for this XCD slot
-  finalize_body<real>(args[s.x], s.y);
+  finalize_body(args[s.x], s.y);
 }
 
 }  // namespace
@@ -441,20 +495,21 @@ static ScatterArgs scatter_args(Ctx* c) {
   a.m = c->mesh;
   a.brick_start = c->d_brick_start;
   a.brick_fill = c->d_brick_fill;
-  a.brick_tmp = c->d_brick_tmp;
+  a.brick_ip = c->d_brick_ip;
+  a.frac = c->d_frac;
+  a.az = c->d_az;
+  a.bq = c->d_bq;
+  a.bmm = c->d_bmm;
   return a;
 }
 
 static FinalizeArgs finalize_args(Ctx* c) {
-  const bool mesh = c->P.ewald_mode == MSG_EWALD_SPM;
   FinalizeArgs f;
   f.ncell = c->grid.ncell;
-  f.nbrick = mesh ? c->mesh.nbrick : 0;
   f.cell_start = c->d_cell_start;
   f.cell_tmp = c->d_cell_tmp;
   f.cell_chunk = c->d_cell_chunk;
   f.ro = c->d_ro;
-  f.az = c->d_az;
   f.ipt = c->d_ipt;
   f.cell_ip = c->d_cell_ip;
   f.slot_of = c->d_slot_of;
@@ -462,17 +517,10 @@ static FinalizeArgs finalize_args(Ctx* c) {
   f.ctype = c->d_ctype;
   f.slot_cell = c->d_slot_cell;
   f.item_cell = c->d_item_cell;
-  f.brick_start = c->d_brick_start;
-  f.brick_tmp = c->d_brick_tmp;
-  f.brick_ip = c->d_brick_ip;
-  f.mm = c->d_mm;
-  f.frac = c->d_frac;
-  f.bq = c->d_bq;
-  f.bmm = c->d_bmm;
   return f;
 }
 
-static int finalize_blocks(const FinalizeArgs& f) { return (f.ncell + f.nbrick + 3) / 4; }
+static int finalize_blocks(const FinalizeArgs& f) { return (f.ncell + 3) / 4; }
 
 int launch_bin(Ctx* c) {
   const int nb = (c->np + 255) / 256;
@@ -487,20 +535,27 @@ int launch_bin(Ctx* c) {
   return MSG_OK;
 }
 
+// Scan and scatter: after these the brick arrays are complete (the mesh branch may start);
+// the cell order is completed by launch_finalize.
 int launch_sort(Ctx* c) {
   const int np = c->np;
   prof_begin(c, MSG_K_SCAN);
   hipLaunchKernelGGL(k_scan, dim3(3), dim3(1024), 0, c->stream, scan_args(c));
   prof_end(c, MSG_K_SCAN);
   prof_begin(c, MSG_K_SCATTER);
-  hipLaunchKernelGGL(k_scatter, dim3((np + 255) / 256), dim3(256), 0, c->stream, scatter_args(c));
+  if (c->fp64)
+    hipLaunchKernelGGL(k_scatter<double>, dim3((np + 255) / 256), dim3(256), 0, c->stream, scatter_args(c));
+  else
+    hipLaunchKernelGGL(k_scatter<float>, dim3((np + 255) / 256), dim3(256), 0, c->stream, scatter_args(c));
   prof_end(c, MSG_K_SCATTER);
+  MSG_HIP(hipGetLastError());
+  return MSG_OK;
+}
+
+int launch_finalize(Ctx* c) {
   const FinalizeArgs f = finalize_args(c);
   prof_begin(c, MSG_K_FINALIZE);
-  if (c->fp64)
-    hipLaunchKernelGGL(k_finalize<double>, dim3(finalize_blocks(f)), dim3(256), 0, c->stream, f);
-  else
-    hipLaunchKernelGGL(k_finalize<float>, dim3(finalize_blocks(f)), dim3(256), 0, c->stream, f);
+  hipLaunchKernelGGL(k_finalize, dim3(finalize_blocks(f)), dim3(256), 0, c->stream, f);
   prof_end(c, MSG_K_FINALIZE);
   MSG_HIP(hipGetLastError());
   return MSG_OK;
@@ -514,12 +569,12 @@ static void seg_bin(const SegGroup& g, hipStream_t s) {
 static void seg_scan(const SegGroup& g, hipStream_t s) {
   hipLaunchKernelGGL(k_scan_seg, dim3(g.nblk), dim3(1024), 0, s, (const ScanArgs*)g.d_args, g.d_map);
 }
-static void seg_scatter(const SegGroup& g, hipStream_t s) {
-  hipLaunchKernelGGL(k_scatter_seg, dim3(g.nblk), dim3(256), 0, s, (const ScatterArgs*)g.d_args, g.d_map);
-}
 template <typename real>
+static void seg_scatter(const SegGroup& g, hipStream_t s) {
+  hipLaunchKernelGGL(k_scatter_seg<real>, dim3(g.nblk), dim3(256), 0, s, (const ScatterArgs*)g.d_args, g.d_map);
+}
 static void seg_finalize(const SegGroup& g, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize_seg<real>, dim3(g.nblk), dim3(256), 0, s, (const FinalizeArgs*)g.d_args, g.d_map);
+  hipLaunchKernelGGL(k_finalize_seg, dim3(g.nblk), dim3(256), 0, s, (const FinalizeArgs*)g.d_args, g.d_map);
 }
 
 int plan_bin(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out) {
@@ -536,9 +591,9 @@ int plan_sort(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& scan, SegSta
   SegBuilder<FinalizeArgs> bf;
   for (Ctx* c : cs) {
     bs.add(0, seg_scan, scan_args(c), 3);
-    bc.add(0, seg_scatter, scatter_args(c), (c->np + 255) / 256);
+    bc.add(c->fp64, c->fp64 ? seg_scatter<double> : seg_scatter<float>, scatter_args(c), (c->np + 255) / 256);
     const FinalizeArgs f = finalize_args(c);
-    bf.add(c->fp64, c->fp64 ? seg_finalize<double> : seg_finalize<float>, f, finalize_blocks(f));
+    bf.add(0, seg_finalize, f, finalize_blocks(f));
   }
   if (int e = bs.finish(own, scan)) return e;
   if (int e = bc.finish(own, scatter)) return e;

@@ -712,7 +712,6 @@ int spme_setup(Ctx* c) {
     if (int e = A((void**)&c->d_brick_count, m.nbrick * sizeof(int))) return e;
     if (int e = A((void**)&c->d_brick_start, (m.nbrick + 1) * sizeof(int))) return e;
     if (int e = A((void**)&c->d_brick_fill, m.nbrick * sizeof(int))) return e;
-    if (int e = A((void**)&c->d_brick_tmp, np * sizeof(int))) return e;
     if (int e = A((void**)&c->d_brick_ip, np * sizeof(int))) return e;
     if (int e = A((void**)&c->d_mm, np * sizeof(int))) return e;
     if (int e = A(&c->d_bq, (size_t)np * 4 * rs)) return e;
