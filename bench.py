@@ -58,6 +58,23 @@ def kernel_bytes(name: str, n: int, K: int, fp64: bool) -> float:
     return per_particle.get(name, 0) * n + mesh.get(name, 0)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_polyelectrolyte_32768_spme128.json")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of a kernel of the default workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a copy of known size), or None."""
+    import re
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        data = json.load(f)
+    for name, v in data.get("kernels", {}).items():
+        if re.match(rf"k_{kernel}(_nb)?<", name) or name == f"k_{kernel}":
+            return v["traffic_bytes"]
+    return None
+
+
 def timed_steps(eng, steps: int) -> float:
     eng.fetch()                            # drains the stream: barrier before timing
     t0 = time.perf_counter()
@@ -591,7 +608,8 @@ def main():
         "kernels_us": {k: round(v["avg_us"], 3) for k, v in kernels.items()},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "alg_bytes_per_launch": nbytes},
+                     "traffic": measured_traffic(dominant), "alg_bytes_per_launch": nbytes,
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)},
     }
 
     if not args.no_cpu:

@@ -69,9 +69,10 @@ def lib():
     """Load libmolsim_gpu.so (built by ``__graft_entry__.build()``); raise if absent."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
-        L = C.CDLL(LIB_PATH)
+        path = os.environ.get("MOLSIM_GPU_LIB", LIB_PATH)  # another build of the library, for A/B timing
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is not built; run __graft_entry__.build()")
+        L = C.CDLL(path)
         L.msg_last_error.restype = C.c_char_p
         L.msg_kernel_name.restype = C.c_char_p
         L.msg_build_table.argtypes = [C.POINTER(Pot), _dp, C.c_int, _ip, _ip, _dp, _dp]
