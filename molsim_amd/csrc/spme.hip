@@ -49,9 +49,10 @@ template <> struct FftCfg<float> { static constexpr int E = 2048, NT = 256; };
 template <> struct FftCfg<double> { static constexpr int E = 1024, NT = 256; };
 
 // Stockham autosort FFT of T lines of length K held line-major in LDS buffer A (B is
-// scratch).  Radix-4 stages, then one radix-2 stage when log2 K is odd.  Returns the
-// buffer holding the result.  All NT threads of the workgroup must call it.
-template <typename C2, int K, int T>
+// scratch), line i starting at element i * LS.  Radix-4 stages, then one radix-2 stage when
+// log2 K is odd.  Returns the buffer holding the result.  All NT threads of the workgroup
+// must call it.
+template <typename C2, int K, int T, int LS = K>
 __device__ C2* lds_fft(C2* A, C2* B, const C2* __restrict__ tw, bool inverse) {
   constexpr int NT = 256;
   constexpr int TPL = NT / T;  // threads per line
@@ -63,7 +64,7 @@ __device__ C2* lds_fft(C2* A, C2* B, const C2* __restrict__ tw, bool inverse) {
     const int step = K / (Ns * 4);
     for (int j = tl; j < K / 4; j += TPL) {
       const int jr = j % Ns;
-      const C2* s = src + line * K;
+      const C2* s = src + line * LS;
       C2 v0 = s[j], v1 = s[j + K / 4], v2 = s[j + K / 2], v3 = s[j + 3 * K / 4];
       C2 w1 = tw[jr * step], w2 = tw[2 * jr * step], w3 = tw[3 * jr * step];
       if (inverse) { w1 = cconj(w1); w2 = cconj(w2); w3 = cconj(w3); }
@@ -74,7 +75,7 @@ __device__ C2* lds_fft(C2* A, C2* B, const C2* __restrict__ tw, bool inverse) {
       const C2 d13 = csub(v1, v3);
       // forward: a3 = -i (v1 - v3); inverse: +i (v1 - v3)
       const C2 a3 = inverse ? cmk<C2>(-d13.y, d13.x) : cmk<C2>(d13.y, -d13.x);
-      C2* d = dst + line * K + (j / Ns) * Ns * 4 + jr;
+      C2* d = dst + line * LS + (j / Ns) * Ns * 4 + jr;
       d[0] = cadd(a0, a2);
       d[Ns] = cadd(a1, a3);
       d[2 * Ns] = csub(a0, a2);
@@ -87,11 +88,11 @@ __device__ C2* lds_fft(C2* A, C2* B, const C2* __restrict__ tw, bool inverse) {
     const int step = K / (Ns * 2);
     for (int j = tl; j < K / 2; j += TPL) {
       const int jr = j % Ns;
-      const C2* s = src + line * K;
+      const C2* s = src + line * LS;
       C2 w = tw[jr * step];
       if (inverse) w = cconj(w);
       const C2 v0 = s[j], v1 = cmul(s[j + K / 2], w);
-      C2* d = dst + line * K + (j / Ns) * Ns * 2 + jr;
+      C2* d = dst + line * LS + (j / Ns) * Ns * 2 + jr;
       d[0] = cadd(v0, v1);
       d[Ns] = csub(v0, v1);
     }
@@ -176,19 +177,20 @@ __device__ __forceinline__ void fft_y_body(const FftArgs& fa, int bid, int inver
   C2* __restrict__ Cm = (C2*)fa.Cm;
   const C2* __restrict__ tw = (const C2*)fa.tw;
   constexpr int E = FftCfg<real>::E, T = E / K, H = K / 2 + 1, NTX = (H + T - 1) / T;
-  __shared__ C2 A[E], B[E];
+  constexpr int LS = K + 1;  // padded lines: the transposing loads and stores avoid LDS bank conflicts
+  __shared__ C2 A[T * LS], B[T * LS];
   __shared__ C2 s_tw[K];  // twiddles staged in LDS
   for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
   const int tx = bid % NTX, z = bid / NTX, kx0 = tx * T;
   for (int e = threadIdx.x; e < E; e += 256) {
     const int kl = e % T, y = e / T, kx = kx0 + kl;
-    A[kl * K + y] = kx < H ? Cm[kx + (size_t)H * (y + (size_t)K * z)] : cmk<C2>(0, 0);
+    A[kl * LS + y] = kx < H ? Cm[kx + (size_t)H * (y + (size_t)K * z)] : cmk<C2>(0, 0);
   }
   __syncthreads();
-  const C2* R = lds_fft<C2, K, T>(A, B, s_tw, inverse != 0);
+  const C2* R = lds_fft<C2, K, T, LS>(A, B, s_tw, inverse != 0);
   for (int e = threadIdx.x; e < E; e += 256) {
     const int kl = e % T, y = e / T, kx = kx0 + kl;
-    if (kx < H) Cm[kx + (size_t)H * (y + (size_t)K * z)] = R[kl * K + y];
+    if (kx < H) Cm[kx + (size_t)H * (y + (size_t)K * z)] = R[kl * LS + y];
   }
 }
 
@@ -202,7 +204,8 @@ __device__ __forceinline__ void fft_zconv_body(const FftArgs& fa, int bid) {
   const ConvArgs& ca = fa.ca;
   double* __restrict__ part = fa.part;
   constexpr int E = FftCfg<real>::E, T = E / K, H = K / 2 + 1, NTX = (H + T - 1) / T;
-  __shared__ C2 A[E], B[E];
+  constexpr int LS = K + 1;  // padded lines (see fft_y_body)
+  __shared__ C2 A[T * LS], B[T * LS];
   __shared__ C2 s_tw[K];  // twiddles staged in LDS
   for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
   __shared__ double red[8];
@@ -210,10 +213,10 @@ __device__ __forceinline__ void fft_zconv_body(const FftArgs& fa, int bid) {
   const size_t plane = (size_t)H * K;
   for (int e = threadIdx.x; e < E; e += 256) {
     const int kl = e % T, z = e / T, kx = kx0 + kl;
-    A[kl * K + z] = kx < H ? Cm[kx + (size_t)H * y + plane * z] : cmk<C2>(0, 0);
+    A[kl * LS + z] = kx < H ? Cm[kx + (size_t)H * y + plane * z] : cmk<C2>(0, 0);
   }
   __syncthreads();
-  C2* R = lds_fft<C2, K, T>(A, B, s_tw, false);
+  C2* R = lds_fft<C2, K, T, LS>(A, B, s_tw, false);
   const int ky = y <= K / 2 ? y : K - y;
   double esum = 0.0, vsum = 0.0;
   for (int e = threadIdx.x; e < E; e += 256) {
@@ -224,7 +227,7 @@ __device__ __forceinline__ void fft_zconv_body(const FftArgs& fa, int bid) {
     if (kx >= H) continue;
     const int kzz = kz <= K / 2 ? kz : K - kz;
     const real g = G[kx + H * (ky + H * kzz)];
-    C2 f = R[e];
+    C2 f = R[kl * LS + kz];
     const double f2 = (double)f.x * (double)f.x + (double)f.y * (double)f.y;
     const double w = (kx == 0 || kx == K / 2) ? 0.5 : 1.0;
     const double m2 = ca.tpl2[0] * kx * kx + ca.tpl2[1] * ky * ky + ca.tpl2[2] * kzz * kzz;
@@ -233,14 +236,14 @@ __device__ __forceinline__ void fft_zconv_body(const FftArgs& fa, int bid) {
     vsum += eg * (1.0 + 2.0 * m2 * ca.expf);
     f.x *= g;
     f.y *= g;
-    R[e] = f;
+    R[kl * LS + kz] = f;
   }
   __syncthreads();
   C2* other = (R == A) ? B : A;
-  const C2* S = lds_fft<C2, K, T>(R, other, s_tw, true);
+  const C2* S = lds_fft<C2, K, T, LS>(R, other, s_tw, true);
   for (int e = threadIdx.x; e < E; e += 256) {
     const int kl = e % T, z = e / T, kx = kx0 + kl;
-    if (kx < H) Cm[kx + (size_t)H * y + plane * z] = S[kl * K + z];
+    if (kx < H) Cm[kx + (size_t)H * y + plane * z] = S[kl * LS + z];
   }
   esum = msgd::block_sum(esum, red);
   vsum = msgd::block_sum(vsum, red);
