@@ -408,7 +408,10 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
       const int rr = rng[p];
       const int ny = (rr >> 8) & 255;
       const int r = task - rpre[p];
-      const int iy = (rr & 255) + r % ny, iz = ((rr >> 16) & 255) + r / ny;
+      // r / ny by a reciprocal: r < 64 and ny <= 8 keep the fractional part of r / ny at
+      // least 1/8 away from the next integer, far beyond the reciprocal's error
+      const int q = (int)((float)r * __frcp_rn((float)ny) + 0.01f);
+      const int iy = (rr & 255) + (r - q * ny), iz = ((rr >> 16) & 255) + q;
       const int lp = loc[p];
       int ly = ((lp >> 10) & 1023) - iy, lz = (lp >> 20) - iz;
       ly += ly < 0 ? K : 0;  // wraps only where one brick spans the mesh
@@ -418,7 +421,9 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
       int x0, nx;
       brick_range<P>(lx0, kBX, whole[0], x0, nx);
       unsigned long long* rowp = acc + ROW * (ly + kBY * lz);
-      for (int ix = x0; ix < x0 + nx; ++ix) {
+#pragma unroll
+      for (int ix = 0; ix < P; ++ix) {
+        if (ix < x0 || ix >= x0 + nx) continue;
         int lx = lx0 - ix;
         lx += lx < 0 ? K : 0;
         const long long v = __double2ll_rn((double)(th[p][ix] * wyz) * kQScale);
