@@ -310,7 +310,7 @@ int msg_batch_utotal(msg_batch* b) {
   if (!b) return set_error(MSG_ERR_ARG, "msg_batch_utotal: null batch");
   if (int e = order_after_contexts(b->cs, b->origin)) return e;
   MSG_HIP(hipGraphLaunch(b->exec, b->origin));
-  for (Ctx* c : b->cs) c->fresh = true;
+  for (Ctx* c : b->cs) c->fresh = c->cells_valid = true;
   return MSG_OK;
 }
 
@@ -528,7 +528,7 @@ int msg_unit_launch(msg_unit* u, msg_ctx** ut, int n_u, int want_forces, msg_ctx
   if (int e = order_after_contexts(all, u->s)) return e;
   if (int e = u->own.flush(u->s)) return e;
   if (int e = launch_pass(u->pass, u->s, u->side, u->fork, u->join)) return e;
-  for (Ctx* c : pcs) c->fresh = true;
+  for (Ctx* c : pcs) c->fresh = c->cells_valid = true;
   if (ntot)
     if (int e = launch_trials(u->de, u->L, u->d_de, u->s)) return e;
   if (!pcs.empty())

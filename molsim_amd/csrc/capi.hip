@@ -258,8 +258,12 @@ int utotal_sequence(Ctx* c) {
   // the cell order is only needed by the pair branch: with SPME it completes beside the mesh
   const bool branch = c->P.ewald_mode == MSG_EWALD_SPM && !c->prof;
   const bool cells_on_side = branch && c->P.list_mode != MSG_LIST_VERLET;
-  if (!cells_on_side)
+  // with a Verlet list the cell order is only needed to rebuild the list (later users such
+  // as DUTotal complete it on demand)
+  const bool cells_needed = c->P.list_mode != MSG_LIST_VERLET || rebuild;
+  if (!cells_on_side && cells_needed)
     if ((e = launch_finalize(c))) return e;
+  c->cells_valid = cells_needed;
   if (rebuild) {
     if ((e = launch_vlist_build(c))) return e;
     c->vl_valid = true;
@@ -495,6 +499,7 @@ int msg_nlist(msg_ctx* h) {
   if (!c->have_positions) return set_error(MSG_ERR_STATE, "positions have not been set");
   int e;
   if ((e = launch_bin(c)) || (e = launch_sort(c)) || (e = launch_finalize(c))) return e;
+  c->cells_valid = true;
   if (c->P.list_mode == MSG_LIST_VERLET) {
     if ((e = launch_vlist_build(c))) return e;
     c->vl_valid = true;
@@ -600,6 +605,10 @@ int msg_get_cell_list(msg_ctx* h, int* nc, int* cell_start, int* cell_ip) {
   if (!h || !nc) return set_error(MSG_ERR_ARG, "msg_get_cell_list: null argument");
   Ctx* c = &h->c;
   for (int d = 0; d < 3; ++d) nc[d] = c->grid.nc[d];
+  if (!c->cells_valid) {  // complete the cell order of the last pass
+    if (int e = launch_finalize(c)) return e;
+    c->cells_valid = true;
+  }
   MSG_HIP(hipStreamSynchronize(c->stream));
   if (cell_start)
     MSG_HIP(hipMemcpy(cell_start, c->d_cell_start, (c->grid.ncell + 1) * sizeof(int), hipMemcpyDeviceToHost));

@@ -201,6 +201,7 @@ struct Ctx {
   int* d_de_ip = nullptr;
   double* d_de_r = nullptr;
   bool fresh = false;            // a UTotal has run on the current positions
+  bool cells_valid = false;      // the cell order (cell_ip, cpos, ...) matches the last pass
   int mc_next = 0;               // next particle of a sequential MC pass (isamp = 0)
 };
 

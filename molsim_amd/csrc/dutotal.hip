@@ -385,6 +385,10 @@ int dutotal_launch(Ctx* c, int ntrial, const int* ip, const double* rotm) {
     if (int e = make_gk(c)) return e;
   if (!c->fresh)
     if (int e = utotal_refresh(c)) return e;
+  if (!c->cells_valid) {  // a Verlet-list pass left the cell order of its positions unranked
+    if (int e = launch_finalize(c)) return e;
+    c->cells_valid = true;
+  }
   if (ntrial > c->de_cap) {
     MSG_HIP(hipStreamSynchronize(c->stream));
     if (c->d_de) { hipFree(c->d_de); hipFree(c->d_de_ip); hipFree(c->d_de_r); }
@@ -601,7 +605,7 @@ __global__ __launch_bounds__(256) void k_commit_std(int ip, double rx, double ry
 // position changes and the derived state is rebuilt by the next UTotal pass.
 int commit_fast(Ctx* c, int ip, const double* rnew, bool& done) {
   done = false;
-  if (c->P.ewald_mode != MSG_EWALD_STD || !c->fresh) return MSG_OK;
+  if (c->P.ewald_mode != MSG_EWALD_STD || !c->fresh || !c->cells_valid) return MSG_OK;
   std::vector<double> ro(3);
   MSG_HIP(hipMemcpy(ro.data(), c->d_ro + 3 * ip, 3 * sizeof(double), hipMemcpyDeviceToHost));
   for (int d = 0; d < 3; ++d) {
