@@ -53,6 +53,7 @@ int build_pass(const std::vector<Ctx*>& cs, SegOwner& own, Pass& p) {
   if (int e = plan_bin(cs, own, p.bin)) return e;
   if (int e = plan_sort(cs, own, p.scan, p.scatter, p.finalize)) return e;
   if (int e = plan_pair(cs, own, p.pair)) return e;
+  if (int e = plan_pair_sum(cs, own, p.pair_sum)) return e;
   if (int e = plan_spme(cs, own, p.spread, p.fft, p.gather)) return e;
   if (int e = plan_reduce(cs, own, p.reduce)) return e;
   std::vector<const double*> outp;
@@ -90,6 +91,7 @@ int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, hipEvent_t fork,
   if (p.any_spme && serial_pass()) {
     if (int e = run_stage(p.finalize, s)) return e;
     if (int e = run_stage(p.pair, s)) return e;
+    if (int e = run_stage(p.pair_sum, s)) return e;
     if (int e = run_stage(p.spread, s)) return e;
     for (int k = 0; k < 5; ++k)
       if (int e = run_stage(p.fft[k], s)) return e;
@@ -99,6 +101,7 @@ int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, hipEvent_t fork,
     MSG_HIP(hipStreamWaitEvent(side, fork, 0));
     if (int e = run_stage(p.finalize, side)) return e;  // cell order beside the mesh branch
     if (int e = run_stage(p.pair, side)) return e;
+    if (int e = run_stage(p.pair_sum, side)) return e;
     MSG_HIP(hipEventRecord(join, side));
     if (int e = run_stage(p.spread, s)) return e;
     for (int k = 0; k < 5; ++k)
@@ -108,6 +111,7 @@ int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, hipEvent_t fork,
   } else {
     if (int e = run_stage(p.finalize, s)) return e;
     if (int e = run_stage(p.pair, s)) return e;
+    if (int e = run_stage(p.pair_sum, s)) return e;
   }
   if (int e = run_stage(p.reduce, s)) return e;
   hipLaunchKernelGGL(k_collect, dim3((unsigned)p.cs.size()), dim3(64), 0, s, p.d_outp, p.d_outs);

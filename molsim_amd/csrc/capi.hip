@@ -75,10 +75,10 @@ __global__ void k_zero(int* a, int na, int* b, int nb, int* flags) {
 }
 
 struct ReduceArgs {
-  const double* pair;  int npair; int ntp;
+  const double* pair;  int npair; int ntp;   // npair rows of ntp energies + virial (1: pre-summed)
   const double* spme;  int nspme;
   const double* ewstd; int newstd;
-  const double* dip;   int ndip;
+  const double* M;     // total dipole sum q r, formed by k_scan
   int ewald;           // 0 none, 1 std, 2 spm
   int lsurf;
   double self_raw;     // -alpha/sqrt(pi) sum q^2
@@ -92,40 +92,34 @@ struct ReduceArgs {
   int nbrick;
 };
 
-// Final FP64 reduction of all partial sums in a fixed order (deterministic).  It is the last
-// kernel of a UTotal pass, so it also hands the error flags to the output block and clears
-// the cell/brick counters and flags for the next pass.
-__device__ __forceinline__ void reduce_body(const ReduceArgs& a) {
-  constexpr int NV = kMaxTP + 1 + 2 + 3;
-  __shared__ double red[16][NV];
-  double v[NV];
+// Per-thread part of the fixed-order sum of the pair kernel's rows (np1 = energies by type
+// pair + virial per workgroup): thread t adds the flat elements t, t + 1024, t + 2048, ...
+// in that order into v[column].  Eight loads are issued before any is used; the loads are
+// unconditional (index clamped) so their values stay in registers.
+__device__ __forceinline__ void pair_rows_part(const double* __restrict__ rows, int nrow, int np1,
+                                               double* v) {
+  const int npv = nrow * np1;
+  for (int f0 = threadIdx.x; f0 < npv; f0 += 8 * 1024) {
+    double x[8];
+    int kk[8];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = 0.0;
-  const int np1 = a.ntp + 1;
-  const int npv = a.npair * np1;  // the per-workgroup rows are contiguous: sum the flat array
-  for (int f0 = threadIdx.x; f0 < npv; f0 += 4 * 1024) {
-    double x[4];
-    int kk[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       const int f = f0 + u * 1024;
-      x[u] = f < npv ? a.pair[f] : 0.0;
+      x[u] = rows[min(f, npv - 1)];
+      if (f >= npv) x[u] = 0.0;
       kk[u] = f % np1;
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 8; ++u)
 #pragma unroll
       for (int k = 0; k <= kMaxTP; ++k) v[k] += (k == kk[u]) ? x[u] : 0.0;
   }
-  const double* ew = a.ewald == 2 ? a.spme : a.ewstd;
-  const int nev = a.ewald == 2 ? a.nspme : (a.ewald == 1 ? a.newstd : 0);
-  for (int b = threadIdx.x; b < nev; b += 1024) {
-    v[kMaxTP + 1] += ew[2 * b];
-    v[kMaxTP + 2] += ew[2 * b + 1];
-  }
-  for (int b = threadIdx.x; b < a.ndip; b += 1024)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) v[kMaxTP + 3 + d] += a.dip[3 * b + d];
+}
+
+// Workgroup total of v[0..NV) (1024 threads): wavefront sums, then the 16 wavefront partials
+// in order.  tot[k] is valid after the call in every thread.
+template <int NV>
+__device__ __forceinline__ void block_total(const double* v, double (*red)[NV], double* tot) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
@@ -133,15 +127,86 @@ __device__ __forceinline__ void reduce_body(const ReduceArgs& a) {
     if (lane == 0) red[w][k] = t;
   }
   __syncthreads();
-  __shared__ double tot[NV];
-  if (threadIdx.x < NV) {  // one thread per value, 16 wave partials each
+  if (threadIdx.x < NV) {
     double x = 0.0;
 #pragma unroll
     for (int w2 = 0; w2 < 16; ++w2) x += red[w2][threadIdx.x];
     tot[threadIdx.x] = x;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+}
+
+struct PairSumArgs {
+  const double* rows;
+  int nrow, np1;
+  double* out;
+};
+
+// The pair rows summed into one row.  With SPME this runs on the pair branch, beside the
+// mesh kernels, so the final reduction reads one row.  The result is bitwise the sum the
+// final reduction forms from the rows itself (same order; it only adds zeros to it).
+__device__ __forceinline__ void pair_sum_body(const PairSumArgs& a) {
+  constexpr int NV = kMaxTP + 1;
+  __shared__ double red[16][NV];
+  __shared__ double tot[NV];
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0.0;
+  pair_rows_part(a.rows, a.nrow, a.np1, v);
+  block_total<NV>(v, red, tot);
+  if (threadIdx.x < a.np1) a.out[threadIdx.x] = tot[threadIdx.x];
+}
+
+__global__ __launch_bounds__(1024) void k_pair_sum(PairSumArgs a) { pair_sum_body(a); }
+
+__global__ __launch_bounds__(1024) void k_pair_sum_seg(const PairSumArgs* __restrict__ args,
+                                                       const int2* __restrict__ map) {
+  const int2 s = map[blockIdx.x];
+  if (s.x < 0) return;  // no work for this XCD slot
+  pair_sum_body(args[s.x]);
+}
+
+void seg_pair_sum(const SegGroup& g, hipStream_t s) {
+  hipLaunchKernelGGL(k_pair_sum_seg, dim3(g.nblk), dim3(1024), 0, s, (const PairSumArgs*)g.d_args, g.d_map);
+}
+
+// Final FP64 reduction of all partial sums in a fixed order (deterministic).  It is the last
+// kernel of a UTotal pass, so it also hands the error flags to the output block and clears
+// the cell/brick counters and flags for the next pass.  The k-space partials, dipole and
+// flags are loaded before the pair rows are summed, so their loads overlap.
+__device__ __forceinline__ void reduce_body(const ReduceArgs& a) {
+  constexpr int NV = kMaxTP + 1 + 2;
+  __shared__ double red[16][NV];
+  __shared__ double tot[NV];
+  const int tid = threadIdx.x;
+  const double* ew = a.ewald == 2 ? a.spme : a.ewstd;
+  const int nev = a.ewald == 2 ? a.nspme : (a.ewald == 1 ? a.newstd : 0);
+  double e0 = 0.0, e1 = 0.0;
+  if (tid < nev) {
+    e0 = ew[2 * tid];
+    e1 = ew[2 * tid + 1];
+  }
+  double m[3] = {0.0, 0.0, 0.0};
+  int fl = 0;
+  if (tid == 0) {
+    m[0] = a.M[0];
+    m[1] = a.M[1];
+    m[2] = a.M[2];
+    fl = a.flags[0];
+  }
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0.0;
+  const int np1 = a.ntp + 1;
+  pair_rows_part(a.pair, a.npair, np1, v);
+  v[kMaxTP + 1] = e0;
+  v[kMaxTP + 2] = e1;
+  for (int b = tid + 1024; b < nev; b += 1024) {
+    v[kMaxTP + 1] += ew[2 * b];
+    v[kMaxTP + 2] += ew[2 * b + 1];
+  }
+  block_total<NV>(v, red, tot);
+  if (tid == 0) {
     const double* t = tot;
     double* o = a.out;
     double tw = 0.0;
@@ -151,7 +216,6 @@ __device__ __forceinline__ void reduce_body(const ReduceArgs& a) {
     }
     o[0] = tw;
     const double es = t[kMaxTP + 1], vs = t[kMaxTP + 2];
-    const double m[3] = {t[kMaxTP + 3], t[kMaxTP + 4], t[kMaxTP + 5]};
     double urec = 0.0, virrec = 0.0;
     if (a.ewald) {
       // SPME: E = sum w G |F|^2, virrec -= sum w G (1 + 2 m^2 expf) |F|^2
@@ -170,11 +234,11 @@ __device__ __forceinline__ void reduce_body(const ReduceArgs& a) {
     o[a.ntp + 4] = m[0];
     o[a.ntp + 5] = m[1];
     o[a.ntp + 6] = m[2];
-    o[a.ntp + 7] = (double)a.flags[0];
+    o[a.ntp + 7] = (double)fl;
     a.flags[0] = 0;
   }
-  for (int i = threadIdx.x; i < a.ncell; i += 1024) a.cell_count[i] = 0;
-  for (int i = threadIdx.x; i < a.nbrick; i += 1024) a.brick_count[i] = 0;
+  for (int i = tid; i < a.ncell; i += 1024) a.cell_count[i] = 0;
+  for (int i = tid; i < a.nbrick; i += 1024) a.brick_count[i] = 0;
 }
 
 __global__ __launch_bounds__(1024) void k_reduce(ReduceArgs a) { reduce_body(a); }
@@ -202,17 +266,17 @@ int launch_zero(Ctx* c) {
   return MSG_OK;
 }
 
-static ReduceArgs reduce_args(Ctx* c) {
+// presum: the pair rows were summed into one row by k_pair_sum
+static ReduceArgs reduce_args(Ctx* c, bool presum) {
   ReduceArgs a;
-  a.pair = c->d_part + c->part_off_pair;
-  a.npair = c->npart_pair;
+  a.pair = c->d_part + (presum ? c->part_off_pairsum : c->part_off_pair);
+  a.npair = presum ? 1 : c->npart_pair;
   a.ntp = c->nptpt;
   a.spme = c->d_part + c->part_off_spme;
   a.nspme = c->nzblocks;
   a.ewstd = c->d_part + c->part_off_ewstd;
   a.newstd = c->npart_ewstd;
-  a.dip = c->d_part + c->part_off_dip;
-  a.ndip = c->npart_dip;
+  a.M = c->d_M;
   a.ewald = c->P.ewald_mode;
   a.lsurf = c->P.lsurf;
   a.self_raw = -c->P.alpha / std::sqrt(M_PI) * c->sumq2;
@@ -228,14 +292,35 @@ static ReduceArgs reduce_args(Ctx* c) {
   return a;
 }
 
-int plan_reduce(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out) {
-  SegBuilder<ReduceArgs> b;
-  for (Ctx* c : cs) b.add(0, seg_reduce, reduce_args(c), 1);
+static PairSumArgs pair_sum_args(Ctx* c) {
+  PairSumArgs a;
+  a.rows = c->d_part + c->part_off_pair;
+  a.nrow = c->npart_pair;
+  a.np1 = c->nptpt + 1;
+  a.out = c->d_part + c->part_off_pairsum;
+  return a;
+}
+
+int plan_pair_sum(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out) {
+  SegBuilder<PairSumArgs> b;
+  for (Ctx* c : cs) b.add(0, seg_pair_sum, pair_sum_args(c), 1);
   return b.finish(own, out);
 }
 
-int launch_reduce(Ctx* c) {
-  const ReduceArgs a = reduce_args(c);
+int launch_pair_sum(Ctx* c) {
+  hipLaunchKernelGGL(k_pair_sum, dim3(1), dim3(1024), 0, c->stream, pair_sum_args(c));
+  MSG_HIP(hipGetLastError());
+  return MSG_OK;
+}
+
+int plan_reduce(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out) {
+  SegBuilder<ReduceArgs> b;
+  for (Ctx* c : cs) b.add(0, seg_reduce, reduce_args(c, true), 1);  // batches pre-sum the pair rows
+  return b.finish(own, out);
+}
+
+int launch_reduce(Ctx* c, bool presum) {
+  const ReduceArgs a = reduce_args(c, presum);
   prof_begin(c, MSG_K_REDUCE);
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, c->stream, a);
   prof_end(c, MSG_K_REDUCE);
@@ -275,6 +360,7 @@ int utotal_sequence(Ctx* c) {
     c->stream = c->side;
     e = cells_on_side ? launch_finalize(c) : MSG_OK;
     if (!e) e = launch_pair(c, false);
+    if (!e) e = launch_pair_sum(c);
     c->stream = main;
     if (e) return e;
     MSG_HIP(hipEventRecord(c->ev_join, c->side));
@@ -291,7 +377,7 @@ int utotal_sequence(Ctx* c) {
     if (c->P.ewald_mode == MSG_EWALD_STD)
       if ((e = launch_ewstd(c))) return e;
   }
-  return launch_reduce(c);
+  return launch_reduce(c, branch);
 }
 
 static int flags_to_error(int f);
@@ -382,7 +468,8 @@ int msg_create(const msg_params* p, msg_ctx** out) {
   c->part_off_spme = c->part_off_dip + 3 * c->npart_dip;
   const int max_spme_blocks = 2 * 2 * 256 * 130;  // upper bound for nzblocks * 2 (K <= 256)
   c->part_off_ewstd = c->part_off_spme + max_spme_blocks;
-  const int nparts = c->part_off_ewstd + 2 * 4096;
+  c->part_off_pairsum = c->part_off_ewstd + 2 * 4096;
+  const int nparts = c->part_off_pairsum + kMaxTP + 1;
   if ((e = dalloc(c, (void**)&c->d_part, nparts * sizeof(double)))) return fail(e);
   if ((e = dalloc(c, (void**)&c->d_out, 64 * sizeof(double)))) return fail(e);
   if ((e = dalloc(c, (void**)&c->d_flags, 16 * sizeof(int)))) return fail(e);

@@ -27,7 +27,8 @@ namespace msg {
 namespace {
 
 constexpr int kChainThreads = 1024;
-constexpr int kChainKvec = 4;  // k-vectors per thread: nk <= 4096
+constexpr int kChainKvec = 4;   // k-vectors per thread at most: nk <= 4096
+constexpr int kChainNcut = 31;  // largest k-vector index per axis
 
 struct ChainArgs {
   int np, ntrial, isamp, structural, ewald, lsurf;
@@ -46,7 +47,7 @@ struct ChainArgs {
   const double* tcoef;  // FP64 table records
   const int* tbins;
   GpuTableMeta tm;
-  int nk;
+  int nk, ncut;
   const int* kvec;
   const double* kfac;
   double* sk;
@@ -118,8 +119,13 @@ __device__ inline double chain_sum(double v, double* sh) {
   return s;
 }
 
+// KV: k-vectors per thread (the smallest of 1, 2, 4 that covers nk, so the structure-factor
+// changes kept for the commit fit in registers).
+template <int KV>
 __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
   __shared__ double sh[kChainThreads / 64];
+  // e^{i n (2 pi / L_d) r_d} for n = 0..ncut, per axis d, at the trial (0) and old (1) position
+  __shared__ double2 s_eik[2][3][kChainNcut + 1];
   __shared__ int s_ip, s_acc, s_stop, s_ovl;
   __shared__ double s_rn[3];
   __shared__ int s_rng[3];
@@ -160,11 +166,19 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
     const double ro[3] = {a.ro[3 * ip], a.ro[3 * ip + 1], a.ro[3 * ip + 2]};
     const double q = a.az[ip];
     const int ti = a.ipt[ip];
+    if (a.ewald == 1 && tid < 6 * (a.ncut + 1)) {
+      // one sincos per (position, axis, n); the k-vector loop below reads them
+      const int n = tid % (a.ncut + 1), d = (tid / (a.ncut + 1)) % 3, w = tid / (3 * (a.ncut + 1));
+      const double k = n * a.tpb[d];
+      double sv, cv;
+      sincos(k * (w == 0 ? rn[d] : ro[d]), &sv, &cv);
+      s_eik[w][d][n] = make_double2(cv, sv);
+    }
     // real space: new minus old
     double e = 0.0;
     int ovl = 0;
     for (int pass = 0; pass < 2; ++pass) {
-      const double* r = pass == 0 ? rn : ro;
+      const double r[3] = {pass == 0 ? rn[0] : ro[0], pass == 0 ? rn[1] : ro[1], pass == 0 ? rn[2] : ro[2]};
       const int cc = pass == 0 ? chain_cell(a, rn) : a.slot_cell[a.slot_of[ip]];
       const int cx = cc % a.g.nc[0], cy = (cc / a.g.nc[0]) % a.g.nc[1], cz = cc / (a.g.nc[0] * a.g.nc[1]);
       const double sgn = pass == 0 ? 1.0 : -1.0;
@@ -195,22 +209,19 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
     }
     if (ovl) atomicOr(&s_ovl, 1);
     // standard Ewald: dS per k-vector kept for the commit
-    double dS[kChainKvec][8];
+    __syncthreads();  // s_eik
+    double dS[KV][8];
     double drec = 0.0;
     if (a.ewald == 1) {
 #pragma unroll
-      for (int u = 0; u < kChainKvec; ++u) {
+      for (int u = 0; u < KV; ++u) {
         const int kn = tid + u * kChainThreads;
         if (kn >= a.nk) continue;
-        const double kx = a.kvec[3 * kn] * a.tpb[0], ky = a.kvec[3 * kn + 1] * a.tpb[1],
-                     kz = a.kvec[3 * kn + 2] * a.tpb[2];
-        double sxn, cxn, syn, cyn, szn, czn, sxo, cxo, syo, cyo, szo, czo;
-        sincos(kx * rn[0], &sxn, &cxn);
-        sincos(ky * rn[1], &syn, &cyn);
-        sincos(kz * rn[2], &szn, &czn);
-        sincos(kx * ro[0], &sxo, &cxo);
-        sincos(ky * ro[1], &syo, &cyo);
-        sincos(kz * ro[2], &szo, &czo);
+        const int nx = a.kvec[3 * kn], ny = a.kvec[3 * kn + 1], nz = a.kvec[3 * kn + 2];
+        const double2 ex = s_eik[0][0][nx], ey = s_eik[0][1][ny], ez = s_eik[0][2][nz];
+        const double2 fx = s_eik[1][0][nx], fy = s_eik[1][1][ny], fz = s_eik[1][2][nz];
+        const double cxn = ex.x, sxn = ex.y, cyn = ey.x, syn = ey.y, czn = ez.x, szn = ez.y;
+        const double cxo = fx.x, sxo = fx.y, cyo = fy.x, syo = fy.y, czo = fz.x, szo = fz.y;
         double tn = 0.0, to = 0.0;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -279,7 +290,7 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
     if (s_acc) {  // MCUpdate in place
       if (a.ewald == 1) {
 #pragma unroll
-        for (int u = 0; u < kChainKvec; ++u) {
+        for (int u = 0; u < KV; ++u) {
           const int kn = tid + u * kChainThreads;
           if (kn >= a.nk) continue;
 #pragma unroll
@@ -315,7 +326,8 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
 }  // namespace
 
 bool chain_eligible(const Ctx* c) {
-  return c->P.ewald_mode != MSG_EWALD_SPM && c->P.list_mode != MSG_LIST_VERLET && c->nkvec <= kChainThreads * kChainKvec;
+  return c->P.ewald_mode != MSG_EWALD_SPM && c->P.list_mode != MSG_LIST_VERLET &&
+         (c->P.ewald_mode != MSG_EWALD_STD || (c->nkvec <= kChainThreads * kChainKvec && c->P.ncut <= kChainNcut));
 }
 
 // Runs ntrial trials of the chain (several launches when accepted moves change cells).
@@ -377,6 +389,7 @@ int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, in
     a.tbins = c->d_tbins;
     a.tm = c->tmeta;
     a.nk = c->P.ewald_mode == MSG_EWALD_STD ? c->nkvec : 0;
+    a.ncut = c->P.ewald_mode == MSG_EWALD_STD ? c->P.ncut : 0;
     a.kvec = c->d_kvec;
     a.kfac = c->d_kfac;
     a.sk = c->d_sk;
@@ -393,7 +406,12 @@ int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, in
     a.events = d_i + 4 + t0;
     a.stat = d_d + 1;
     prof_begin(c, MSG_K_DE);
-    hipLaunchKernelGGL(k_mc_chain, dim3(1), dim3(kChainThreads), 0, c->stream, a);
+    if (a.nk <= kChainThreads)
+      hipLaunchKernelGGL(k_mc_chain<1>, dim3(1), dim3(kChainThreads), 0, c->stream, a);
+    else if (a.nk <= 2 * kChainThreads)
+      hipLaunchKernelGGL(k_mc_chain<2>, dim3(1), dim3(kChainThreads), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL(k_mc_chain<4>, dim3(1), dim3(kChainThreads), 0, c->stream, a);
     prof_end(c, MSG_K_DE);
     MSG_HIP(hipGetLastError());
     double st[7];

@@ -149,6 +149,7 @@ struct Ctx {
   double* d_part = nullptr;      // partial sums (pair, spme, dipole)
   int npart_pair = 0, npart_dip = 0, part_off_pair = 0, part_off_spme = 0, part_off_dip = 0;
   int part_off_ewstd = 0, npart_ewstd = 0;
+  int part_off_pairsum = 0;  // the pair rows summed into one row (SPME passes)
   double* d_out = nullptr;       // [u_twob(0..nptpt), u_rec, u_tot, virial, dip x,y,z]
   int* d_flags = nullptr;        // error flags
   double* h_out = nullptr;       // pinned
@@ -224,7 +225,8 @@ int launch_spme_mesh(Ctx* c);                      // spread, FFTs + influence f
 int launch_spme_gather(Ctx* c);                    // force gather (after the pair kernel)
 int ewstd_setup(Ctx* c);
 int launch_ewstd(Ctx* c);
-int launch_reduce(Ctx* c);                         // final energies
+int launch_reduce(Ctx* c, bool presum);           // final energies (presum: pair rows pre-summed)
+int launch_pair_sum(Ctx* c);                       // pair rows summed into one row
 int launch_zero(Ctx* c);                           // counters and flags
 int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ovl,
                   double* du_tot, double* du_rec, double* du_twob);
@@ -385,12 +387,13 @@ int plan_pair(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out);
 int plan_spme(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& spread, SegStage* fft,
               SegStage& gather);   // fft[5]: x, y, zconv, yi, xi
 int plan_reduce(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out);
+int plan_pair_sum(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out);
 
 // A batched UTotal pass over many systems: its stages and the collection of every
 // system's energies (64 doubles each) into d_outs.
 struct Pass {
   std::vector<Ctx*> cs;
-  SegStage bin, scan, scatter, finalize, pair, spread, fft[5], gather, reduce;
+  SegStage bin, scan, scatter, finalize, pair, pair_sum, spread, fft[5], gather, reduce;
   bool any_spme = false;
   const double* const* d_outp = nullptr;
   double* d_outs = nullptr;

@@ -9,7 +9,7 @@
 //   k_ewstd_sk     one workgroup per octant vector: the 4 structure factors
 //                  S = sum_a q_a exp(i k.r_a) (direct sincos in FP64), kfac sum |S|^2 and the
 //                  virial weight; S stays resident for MC energy differences.
-//   k_ewstd_force  one thread per particle: F_a += 2 EpsiFourPi kfac k Im(e_a conj(S)) over
+//   k_ewstd_force  one wavefront per particle: F_a += 2 EpsiFourPi kfac k Im(e_a conj(S)) over
 //                  all vectors (the reference's facmm/facmp/facpm/facpp form), plus the
 //                  surface force; adds to the pair force and applies the force unit scale.
 #include "common.h"
@@ -19,7 +19,7 @@ namespace msg {
 namespace {
 
 struct EwArgs {
-  int np, nk;
+  int np, nk, ncut;
   const double* ro;
   const double* az;
   const int* kvec;     // nk * 3
@@ -79,21 +79,37 @@ struct EwForceArgs {
   double scale;
 };
 
+constexpr int kEwNmax = 32;  // phase-table entries per axis: ncut <= 31
+
+// One wavefront per particle; its lanes take the k-vectors lane, lane + 64, ... and the
+// force components are wave-summed.  The phase factors e^{i n (2 pi / L_d) x_d},
+// n = 0..ncut, are computed once per particle (one sincos per lane) into an LDS table from
+// which every k-vector reads its three factors.
 __global__ __launch_bounds__(256) void k_ewstd_force(EwForceArgs f) {
   const EwArgs& a = f.e;
-  const double M[3] = {f.Mtot[0], f.Mtot[1], f.Mtot[2]};
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= a.np) return;
+  __shared__ double2 s_e[4][3][kEwNmax];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + w;
+  if (i >= a.np) return;  // the whole wavefront leaves; no workgroup barrier follows
   const double q = a.az[i];
   const double x = a.ro[3 * i], y = a.ro[3 * i + 1], z = a.ro[3 * i + 2];
+  const int nc1 = a.ncut + 1;
+  for (int t = lane; t < 3 * nc1; t += 64) {
+    const int d = t / nc1, n = t - d * nc1;
+    const double k = n * a.tpb[d];
+    double sv, cv;
+    sincos(k * (d == 0 ? x : (d == 1 ? y : z)), &sv, &cv);
+    s_e[w][d][n] = make_double2(cv, sv);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   double fx = 0.0, fy = 0.0, fz = 0.0;
-  for (int kn = 0; kn < a.nk; ++kn) {
-    const double kx = a.kvec[3 * kn] * a.tpb[0], ky = a.kvec[3 * kn + 1] * a.tpb[1],
-                 kz = a.kvec[3 * kn + 2] * a.tpb[2];
-    double sx, cx, sy, cy, sz, cz;
-    sincos(kx * x, &sx, &cx);
-    sincos(ky * y, &sy, &cy);
-    sincos(kz * z, &sz, &cz);
+  for (int kn = lane; kn < a.nk; kn += 64) {
+    const int nx = a.kvec[3 * kn], ny = a.kvec[3 * kn + 1], nz = a.kvec[3 * kn + 2];
+    const double kx = nx * a.tpb[0], ky = ny * a.tpb[1], kz = nz * a.tpb[2];
+    const double2 ex = s_e[w][0][nx], ey = s_e[w][1][ny], ez = s_e[w][2][nz];
+    const double cx = ex.x, sx = ex.y, cy = ey.x, sy = ey.y, cz = ez.x, sz = ez.y;
     double fc[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -108,11 +124,14 @@ __global__ __launch_bounds__(256) void k_ewstd_force(EwForceArgs f) {
     fy += kf2 * ky * (-fc[0] + fc[1] - fc[2] + fc[3]);
     fz += kf2 * kz * (fc[0] + fc[1] + fc[2] + fc[3]);
   }
-  const double fr[3] = {fx, fy, fz};
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double fs = -f.epsi4pi * f.surf2fac * q * M[d];
-    f.force[3 * i + d] = (f.force[3 * i + d] + fr[d] + fs) * f.scale;
+  fx = msgd::wave_sum(fx);
+  fy = msgd::wave_sum(fy);
+  fz = msgd::wave_sum(fz);
+  if (lane < 3) {
+    const double fr = lane == 0 ? fx : (lane == 1 ? fy : fz);
+    const double M = f.Mtot[lane];
+    const double fs = -f.epsi4pi * f.surf2fac * q * M;
+    f.force[3 * i + lane] = (f.force[3 * i + lane] + fr + fs) * f.scale;
   }
 }
 
@@ -120,6 +139,7 @@ EwArgs make_args(Ctx* c) {
   EwArgs a;
   a.np = c->np;
   a.nk = c->nkvec;
+  a.ncut = c->P.ncut;
   a.ro = c->d_ro;
   a.az = c->d_az;
   a.kvec = c->d_kvec;
@@ -136,6 +156,7 @@ EwArgs make_args(Ctx* c) {
 int ewstd_setup(Ctx* c) {
   const msg_params& P = c->P;
   if (P.ncut < 1) return set_error(MSG_ERR_ARG, "std Ewald: ncut must be >= 1");
+  if (P.ncut >= kEwNmax) return set_error(MSG_ERR_ARG, "std Ewald: ncut must be < 32");
   const int ncut = P.ncut, ncut2 = P.ncutregion == 1 ? 3 * ncut * ncut : ncut * ncut;
   const double vol = P.boxlen[0] * P.boxlen[1] * P.boxlen[2];
   const double fac0 = 2.0 * M_PI / vol;
@@ -184,7 +205,7 @@ int launch_ewstd(Ctx* c) {
   f.scale = c->P.force_scale;
   prof_begin(c, MSG_K_EWSTD);
   hipLaunchKernelGGL(k_ewstd_sk, dim3(c->nkvec), dim3(256), 0, c->stream, f.e);
-  hipLaunchKernelGGL(k_ewstd_force, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, f);
+  hipLaunchKernelGGL(k_ewstd_force, dim3((c->np + 3) / 4), dim3(256), 0, c->stream, f);
   prof_end(c, MSG_K_EWSTD);
   MSG_HIP(hipGetLastError());
   return MSG_OK;

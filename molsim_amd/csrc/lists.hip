@@ -98,114 +98,75 @@ __global__ __launch_bounds__(256) void k_bin_seg(const BinArgs* __restrict__ arg
   bin_body<real>(args[s.x], s.y);
 }
 
-// Exclusive scan over n values of one 1024-thread workgroup; out[n] receives the total.
-// div > 1 scans ceil(cnt / div) instead of cnt.  Each thread owns a contiguous run of
-// values, loaded together, so the scan costs two memory round trips, not one per 1024.
-__device__ void block_scan(int n, const int* __restrict__ cnt, int div, int* out, int* wsum) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int per = (n + 1023) / 1024;
-  const int b0 = threadIdx.x * per, b1 = min(n, b0 + per);
-  int sum = 0;
-  for (int i0 = b0; i0 < b1; i0 += 8) {  // eight independent loads in flight
-    int v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = i0 + u < b1 ? cnt[i0 + u] : 0;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) sum += div > 1 ? (v[u] + div - 1) / div : v[u];
-  }
-  int incl = sum;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int t = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += t;
-  }
-  if (lane == 63) wsum[w] = incl;
-  __syncthreads();
-  int before = 0, total = 0;
-  for (int k = 0; k < 16; ++k) {
-    if (k < w) before += wsum[k];
-    total += wsum[k];
-  }
-  int run = before + incl - sum;
-  for (int i0 = b0; i0 < b1; i0 += 8) {
-    int v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = i0 + u < b1 ? cnt[i0 + u] : 0;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (i0 + u < b1) out[i0 + u] = run;
-      run += div > 1 ? (v[u] + div - 1) / div : v[u];
-    }
-  }
-  if (threadIdx.x == 0) out[n] = total;
-  __syncthreads();
-}
-
 // Exclusive scans of cnt[0..n) and of ceil(cnt / div) (out2, may be null) by one 1024-thread
-// workgroup from ONE load of the counts: each thread keeps its run of at most VMAX values in
-// registers, so the scan costs one memory round trip.  out1[n] / out2[n] get the totals.
-template <int VMAX>
-__device__ void block_scan_dual(int n, const int* __restrict__ cnt, int div, int* out1, int* out2,
-                                int* wsum) {
+// workgroup, in tiles of 1024 * V values.  In each tile a thread loads its V consecutive
+// values together (one memory round trip per tile; the loads are unconditional, index
+// clamped, so the values stay in registers), the wavefronts scan with shuffles, and the
+// running totals carry into the next tile.  out1[n] / out2[n] get the totals.
+template <int V>
+__device__ void block_scan_tiled(int n, const int* __restrict__ cnt, int div, int* out1, int* out2,
+                                 int* wsum) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int per = (n + 1023) / 1024;
-  const int b0 = threadIdx.x * per;
-  int v[VMAX];
-  int s1 = 0, s2 = 0;
+  int c1 = 0, c2 = 0;
+  for (int base = 0; base < n; base += 1024 * V) {
+    const int b0 = base + threadIdx.x * V;
+    int v[V];
 #pragma unroll
-  for (int u = 0; u < VMAX; ++u) {
-    v[u] = (u < per && b0 + u < n) ? cnt[b0 + u] : 0;
-    s1 += v[u];
-    s2 += (v[u] + div - 1) / div;
-  }
-  int i1 = s1, i2 = s2;
+    for (int u = 0; u < V; ++u) v[u] = cnt[min(b0 + u, n - 1)];
+    int s1 = 0, s2 = 0;
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int t1 = __shfl_up(i1, off, 64), t2 = __shfl_up(i2, off, 64);
-    if (lane >= off) {
-      i1 += t1;
-      i2 += t2;
+    for (int u = 0; u < V; ++u) {
+      if (b0 + u >= n) v[u] = 0;
+      s1 += v[u];
+      s2 += (v[u] + div - 1) / div;
     }
-  }
-  if (lane == 63) {
-    wsum[w] = i1;
-    wsum[16 + w] = i2;
-  }
-  __syncthreads();
-  int bef1 = 0, bef2 = 0, tot1 = 0, tot2 = 0;
-  for (int k = 0; k < 16; ++k) {
-    if (k < w) {
-      bef1 += wsum[k];
-      bef2 += wsum[16 + k];
-    }
-    tot1 += wsum[k];
-    tot2 += wsum[16 + k];
-  }
-  int r1 = bef1 + i1 - s1, r2 = bef2 + i2 - s2;
+    int i1 = s1, i2 = s2;
 #pragma unroll
-  for (int u = 0; u < VMAX; ++u) {
-    if (u < per && b0 + u < n) {
-      out1[b0 + u] = r1;
-      if (out2) out2[b0 + u] = r2;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t1 = __shfl_up(i1, off, 64), t2 = __shfl_up(i2, off, 64);
+      if (lane >= off) {
+        i1 += t1;
+        i2 += t2;
+      }
     }
-    r1 += v[u];
-    r2 += (v[u] + div - 1) / div;
+    if (lane == 63) {
+      wsum[w] = i1;
+      wsum[16 + w] = i2;
+    }
+    __syncthreads();
+    int bef1 = 0, bef2 = 0, tot1 = 0, tot2 = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k < w) {
+        bef1 += wsum[k];
+        bef2 += wsum[16 + k];
+      }
+      tot1 += wsum[k];
+      tot2 += wsum[16 + k];
+    }
+    __syncthreads();  // the next tile rewrites wsum
+    int r1 = c1 + bef1 + i1 - s1, r2 = c2 + bef2 + i2 - s2;
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      if (b0 + u < n) {
+        out1[b0 + u] = r1;
+        if (out2) out2[b0 + u] = r2;
+      }
+      r1 += v[u];
+      r2 += (v[u] + div - 1) / div;
+    }
+    c1 += tot1;
+    c2 += tot2;
   }
   if (threadIdx.x == 0) {
-    out1[n] = tot1;
-    if (out2) out2[n] = tot2;
+    out1[n] = c1;
+    if (out2) out2[n] = c2;
   }
   __syncthreads();
 }
 
 __device__ inline void scan_counts(int n, const int* cnt, int div, int* out1, int* out2, int* wsum) {
-  if (n <= 8 * 1024) block_scan_dual<8>(n, cnt, div, out1, out2, wsum);
-  else if (n <= 32 * 1024) block_scan_dual<32>(n, cnt, div, out1, out2, wsum);
-  else if (n <= 64 * 1024) block_scan_dual<64>(n, cnt, div, out1, out2, wsum);
-  else {
-    block_scan(n, cnt, 1, out1, wsum);
-    if (out2) block_scan(n, cnt, div, out2, wsum);
-  }
+  block_scan_tiled<8>(n, cnt, div, out1, out2, wsum);
 }
 
 struct ScanArgs {
@@ -322,7 +283,7 @@ __global__ __launch_bounds__(256) void k_scatter_seg(const ScatterArgs* __restri
 }
 
 struct FinalizeArgs {
-  int ncell;
+  int ncell, np;
   const int* cell_start;
   const int* cell_tmp;
   const int* cell_chunk;
@@ -338,7 +299,9 @@ struct FinalizeArgs {
 
 // One wavefront per cell: rank members by particle index (descending, the reference's
 // linked-list order) and write the cell-ordered arrays; cells also list their pair-kernel
-// work items.
+// work items.  The members are compared 64 at a time (one coalesced load, then shuffles);
+// a cell that holds every particle (one cell for the whole box) has the closed-form rank
+// np - 1 - ip.
 __device__ __forceinline__ void finalize_body(const FinalizeArgs& a, int bid) {
   const int wv = bid * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -347,10 +310,14 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& a, int bid) {
     for (int k = lane; k < ((n + 63) & ~63); k += 64) {
       const int ip = k < n ? a.cell_tmp[s0 + k] : -1;
       int rank = 0;
-      if (n <= 64) {  // members in registers: rank by broadcasting each one
-        for (int j = 0; j < n; ++j) rank += __shfl(ip, j, 64) > ip;
+      if (n == a.np) {
+        rank = n - 1 - ip;
       } else {
-        for (int j = 0; j < n; ++j) rank += a.cell_tmp[s0 + j] > ip;
+        for (int j0 = 0; j0 < n; j0 += 64) {
+          const int jv = j0 + lane < n ? a.cell_tmp[s0 + j0 + lane] : -1;
+          const int m = min(64, n - j0);
+          for (int j = 0; j < m; ++j) rank += __shfl(jv, j, 64) > ip;
+        }
       }
       if (k >= n) continue;
       const int dst = s0 + rank;
@@ -506,6 +473,7 @@ static ScatterArgs scatter_args(Ctx* c) {
 static FinalizeArgs finalize_args(Ctx* c) {
   FinalizeArgs f;
   f.ncell = c->grid.ncell;
+  f.np = c->np;
   f.cell_start = c->d_cell_start;
   f.cell_tmp = c->d_cell_tmp;
   f.cell_chunk = c->d_cell_chunk;

@@ -43,7 +43,9 @@ struct PairArgs {
 
 // Copy of nbytes (a multiple of 16, both ends 16-byte aligned) from global memory to LDS by
 // the 256 threads of a workgroup: 16-byte loads, eight per thread issued before any store,
-// so the copy costs one memory round trip instead of one per element.
+// so the copy costs one memory round trip instead of one per element.  Every element of
+// the register array is assigned on both sides of the bounds test (a conditionally filled
+// array is demoted to scratch memory, with one wait per load).
 __device__ inline void lds_copy16(void* dst, const void* src, int nbytes) {
   const int4* s = (const int4*)src;
   int4* d = (int4*)dst;
@@ -52,8 +54,7 @@ __device__ inline void lds_copy16(void* dst, const void* src, int nbytes) {
   for (int base = threadIdx.x; base < n; base += 256 * B) {
     int4 v[B];
 #pragma unroll
-    for (int u = 0; u < B; ++u)
-      if (base + u * 256 < n) v[u] = s[base + u * 256];
+    for (int u = 0; u < B; ++u) v[u] = base + u * 256 < n ? s[base + u * 256] : make_int4(0, 0, 0, 0);
 #pragma unroll
     for (int u = 0; u < B; ++u)
       if (base + u * 256 < n) d[base + u * 256] = v[u];
@@ -430,7 +431,7 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (hv[u] && jv[u] != s)
-          pair_eval<real, NTP>(hp, ht, pv[u], (int)pv[u].w, a.box, rc2, false, s_coef, s_bins, s_zlo,
+          pair_eval<real, NTP>(hp, ht, pv[u], (int)pv[u].w, a.box, rc2, a.cut_inclusive, s_coef, s_bins, s_zlo,
                                s_invw, s_r2at, s_r2um, s_last, s_tp, acc);
     }
   }
@@ -626,8 +627,10 @@ __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
 
 // Work items: sum over cells of ceil(count / kPairHB) <= ncell + np / kPairHB; a grid of
 // up to 8 workgroups per CU walks them with a stride.
+// The neighbour-parallel kernel serves every cell-list and all-pairs system (any stencil:
+// 27 cells, or fewer where dimensions collapse to one cell) unless tile 0 is chosen.
 bool pair_nbpar(const Ctx* c) {
-  return c->grid.nst == 27 && c->P.list_mode == MSG_LIST_CELL && c->pair_lph > 0;
+  return c->P.list_mode != MSG_LIST_VERLET && c->pair_lph > 0;
 }
 
 int pair_blocks(Ctx* c) {
@@ -687,7 +690,6 @@ int launch_pair(Ctx* c, bool last) {
     v.ro = c->d_ro;
     v.az = c->d_az;
     v.ipt = c->d_ipt;
-    prof_begin(c, MSG_K_PAIR);
     switch (c->nptpt) {
 #define MSG_VL(NTP)                                                                            \
   if (c->fp64) hipLaunchKernelGGL((k_pair_vl<double, NTP>), dim3(nb), dim3(256), lds, c->stream, v); \
