@@ -330,54 +330,72 @@ int launch_reduce(Ctx* c, bool presum) {
 
 // The UTotal launch sequence (no host synchronisation inside; capturable).  Counters and
 // flags are zero on entry (cleared at allocation and by the previous pass's reduction).
-// With SPME the pair kernel runs on a second stream beside the mesh kernels; the gather,
-// which adds to the pair forces, waits for it.  With per-kernel timing on, everything runs
-// on one stream so each kernel is timed alone.
+// With SPME the work forks after the binning pass: the cell list (scan, scatter, ranking,
+// Verlet rebuild), the pair kernel and the pair-row sum run on a second stream beside the
+// brick scan/scatter and the mesh kernels; the gather, which adds to the pair forces, waits
+// for them.  With per-kernel timing on, everything runs on one stream so each kernel is
+// timed alone.
 int utotal_sequence(Ctx* c) {
   int e;
   bool rebuild = false;
   if (c->P.list_mode == MSG_LIST_VERLET)
     if ((e = verlet_check(c, rebuild))) return e;
   if ((e = launch_bin(c))) return e;
-  if ((e = launch_sort(c))) return e;
-  // the cell order is only needed by the pair branch: with SPME it completes beside the mesh
   const bool branch = c->P.ewald_mode == MSG_EWALD_SPM && !c->prof;
-  const bool cells_on_side = branch && c->P.list_mode != MSG_LIST_VERLET;
   // with a Verlet list the cell order is only needed to rebuild the list (later users such
   // as DUTotal complete it on demand)
   const bool cells_needed = c->P.list_mode != MSG_LIST_VERLET || rebuild;
-  if (!cells_on_side && cells_needed)
-    if ((e = launch_finalize(c))) return e;
+  auto cell_work = [&]() -> int {
+    if (int e2 = launch_sort(c, branch ? 1 : 3)) return e2;
+    if (cells_needed)
+      if (int e2 = launch_finalize(c)) return e2;
+    if (rebuild) {
+      if (int e2 = launch_vlist_build(c)) return e2;
+      c->vl_valid = true;
+    }
+    return MSG_OK;
+  };
   c->cells_valid = cells_needed;
-  if (rebuild) {
-    if ((e = launch_vlist_build(c))) return e;
-    c->vl_valid = true;
-  }
   if (branch) {
     MSG_HIP(hipEventRecord(c->ev_fork, c->stream));
     MSG_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     hipStream_t main = c->stream;
     c->stream = c->side;
-    e = cells_on_side ? launch_finalize(c) : MSG_OK;
+    e = cell_work();
     if (!e) e = launch_pair(c, false);
     if (!e) e = launch_pair_sum(c);
     c->stream = main;
     if (e) return e;
     MSG_HIP(hipEventRecord(c->ev_join, c->side));
+    if ((e = launch_sort(c, 2))) return e;
     if ((e = launch_spme_mesh(c))) return e;
     MSG_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    // the energy reduction needs the pair-row and k-space sums, not the forces: it runs
+    // beside the gather
+    MSG_HIP(hipEventRecord(c->ev_fork2, c->stream));
+    MSG_HIP(hipStreamWaitEvent(c->side, c->ev_fork2, 0));
+    c->stream = c->side;
+    e = launch_reduce(c, true);
+    c->stream = main;
+    if (e) return e;
+    MSG_HIP(hipEventRecord(c->ev_join2, c->side));
     if ((e = launch_spme_gather(c))) return e;
-  } else if (c->P.ewald_mode == MSG_EWALD_SPM) {
-    // per-kernel timing: one stream, so no kernel's time includes another's
-    if ((e = launch_pair(c, false))) return e;
-    if ((e = launch_spme_mesh(c))) return e;
-    if ((e = launch_spme_gather(c))) return e;
+    MSG_HIP(hipStreamWaitEvent(c->stream, c->ev_join2, 0));
+    return MSG_OK;
   } else {
-    if ((e = launch_pair(c, c->P.ewald_mode == MSG_EWALD_NONE))) return e;
-    if (c->P.ewald_mode == MSG_EWALD_STD)
-      if ((e = launch_ewstd(c))) return e;
+    if ((e = cell_work())) return e;
+    if (c->P.ewald_mode == MSG_EWALD_SPM) {
+      // per-kernel timing: one stream, so no kernel's time includes another's
+      if ((e = launch_pair(c, false))) return e;
+      if ((e = launch_spme_mesh(c))) return e;
+      if ((e = launch_spme_gather(c))) return e;
+    } else {
+      if ((e = launch_pair(c, c->P.ewald_mode == MSG_EWALD_NONE))) return e;
+      if (c->P.ewald_mode == MSG_EWALD_STD)
+        if ((e = launch_ewstd(c))) return e;
+    }
   }
-  return launch_reduce(c, branch);
+  return launch_reduce(c, false);
 }
 
 static int flags_to_error(int f);
@@ -450,7 +468,9 @@ int msg_create(const msg_params* p, msg_ctx** out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming) != hipSuccess)
     return fail(set_error(MSG_ERR_HIP, "stream/event creation failed"));
   const int np = c->np;
   if ((e = dalloc(c, (void**)&c->d_ro, 3 * np * sizeof(double)))) return fail(e);
@@ -508,6 +528,8 @@ void msg_destroy(msg_ctx* h) {
   if (c->side) hipStreamDestroy(c->side);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->ev_fork2) hipEventDestroy(c->ev_fork2);
+  if (c->ev_join2) hipEventDestroy(c->ev_join2);
   if (c->ev_batch) hipEventDestroy(c->ev_batch);
   if (c->stream) hipStreamDestroy(c->stream);
   delete h;
@@ -585,7 +607,7 @@ int msg_nlist(msg_ctx* h) {
   Ctx* c = &h->c;
   if (!c->have_positions) return set_error(MSG_ERR_STATE, "positions have not been set");
   int e;
-  if ((e = launch_bin(c)) || (e = launch_sort(c)) || (e = launch_finalize(c))) return e;
+  if ((e = launch_bin(c)) || (e = launch_sort(c, 3)) || (e = launch_finalize(c))) return e;
   c->cells_valid = true;
   if (c->P.list_mode == MSG_LIST_VERLET) {
     if ((e = launch_vlist_build(c))) return e;

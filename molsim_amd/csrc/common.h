@@ -86,7 +86,7 @@ struct Ctx {
   bool fp64 = false;
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;        // second branch of the UTotal graph (pair kernel)
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
   hipEvent_t ev_batch = nullptr;     // orders the context's own work before a batched pass
   int device = 0;
   long long bytes = 0;
@@ -215,7 +215,7 @@ int pack_table(Ctx* c);
 // Launch sequences (each .hip file)
 int lists_setup(Ctx* c);                           // cell grid + buffers
 int launch_bin(Ctx* c);                            // K1 cell id + mesh bin + counts + dipole
-int launch_sort(Ctx* c);                           // scan + scatter (bricks complete)
+int launch_sort(Ctx* c, int which);                // scan + scatter: 1 cells, 2 bricks + dipole, 3 both
 int launch_finalize(Ctx* c);                       // cell order (after launch_sort)
 int launch_pair(Ctx* c, bool last);                // K4
 int pair_blocks(Ctx* c);
@@ -381,8 +381,8 @@ struct SegBuilder {
 // One stage of the batched UTotal sequence over many systems (each .hip file).
 typedef std::vector<SegGroup> SegStage;
 int plan_bin(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out);
-int plan_sort(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& scan, SegStage& scatter,
-              SegStage& finalize);
+int plan_sort(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& scan_c, SegStage& scan_b,
+              SegStage& scatter_c, SegStage& scatter_b, SegStage& finalize);
 int plan_pair(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out);
 int plan_spme(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& spread, SegStage* fft,
               SegStage& gather);   // fft[5]: x, y, zconv, yi, xi
@@ -393,13 +393,14 @@ int plan_pair_sum(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out);
 // system's energies (64 doubles each) into d_outs.
 struct Pass {
   std::vector<Ctx*> cs;
-  SegStage bin, scan, scatter, finalize, pair, pair_sum, spread, fft[5], gather, reduce;
+  SegStage bin, scan_c, scan_b, scatter_c, scatter_b, finalize, pair, pair_sum, spread, fft[5], gather, reduce;
   bool any_spme = false;
   const double* const* d_outp = nullptr;
   double* d_outs = nullptr;
 };
 int build_pass(const std::vector<Ctx*>& cs, SegOwner& own, Pass& p);
-int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join);
+int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, const hipEvent_t* fork,
+                const hipEvent_t* join);  // two events each
 int check_batchable(Ctx* c);
 
 // Batched MC trials: systems grouped by spline order; d_args holds one block per member.

@@ -98,65 +98,67 @@ __global__ __launch_bounds__(256) void k_bin_seg(const BinArgs* __restrict__ arg
   bin_body<real>(args[s.x], s.y);
 }
 
-// Exclusive scans of cnt[0..n) and of ceil(cnt / div) (out2, may be null) by one 1024-thread
-// workgroup, in tiles of 1024 * V values.  In each tile a thread loads its V consecutive
-// values together (one memory round trip per tile; the loads are unconditional, index
-// clamped, so the values stay in registers), the wavefronts scan with shuffles, and the
-// running totals carry into the next tile.  out1[n] / out2[n] get the totals.
-template <int V>
-__device__ void block_scan_tiled(int n, const int* __restrict__ cnt, int div, int* out1, int* out2,
-                                 int* wsum) {
+// Exclusive scans of cnt[0..n) and of ceil(cnt / DIV) (out2, may be null) by one 1024-thread
+// workgroup, in tiles of 1024 * V values.  Thread t takes the values t, t + 1024, ... of a
+// tile (coalesced loads and stores), the wavefronts scan the V rows with shuffles, one
+// barrier exchanges the wavefront totals, and the running totals carry into the next tile.
+// out1[n] / out2[n] get the totals.
+template <int V, int DIV>
+__device__ void block_scan_tiled(int n, const int* __restrict__ cnt, int* out1, int* out2,
+                                 int (*wsum)[16]) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int c1 = 0, c2 = 0;
   for (int base = 0; base < n; base += 1024 * V) {
-    const int b0 = base + threadIdx.x * V;
-    int v[V];
-#pragma unroll
-    for (int u = 0; u < V; ++u) v[u] = cnt[min(b0 + u, n - 1)];
-    int s1 = 0, s2 = 0;
+    int v[V], i1[V], i2[V];
 #pragma unroll
     for (int u = 0; u < V; ++u) {
-      if (b0 + u >= n) v[u] = 0;
-      s1 += v[u];
-      s2 += (v[u] + div - 1) / div;
+      const int idx = base + u * 1024 + threadIdx.x;
+      v[u] = idx < n ? cnt[idx] : 0;
+      i1[u] = v[u];
+      i2[u] = (v[u] + DIV - 1) / DIV;
     }
-    int i1 = s1, i2 = s2;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int t1 = __shfl_up(i1, off, 64), t2 = __shfl_up(i2, off, 64);
-      if (lane >= off) {
-        i1 += t1;
-        i2 += t2;
+    for (int off = 1; off < 64; off <<= 1)
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        const int t1 = __shfl_up(i1[u], off, 64), t2 = __shfl_up(i2[u], off, 64);
+        if (lane >= off) {
+          i1[u] += t1;
+          i2[u] += t2;
+        }
       }
-    }
-    if (lane == 63) {
-      wsum[w] = i1;
-      wsum[16 + w] = i2;
-    }
+    if (lane == 63)
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        wsum[u][w] = i1[u];
+        wsum[V + u][w] = i2[u];
+      }
     __syncthreads();
-    int bef1 = 0, bef2 = 0, tot1 = 0, tot2 = 0;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      if (k < w) {
-        bef1 += wsum[k];
-        bef2 += wsum[16 + k];
-      }
-      tot1 += wsum[k];
-      tot2 += wsum[16 + k];
-    }
-    __syncthreads();  // the next tile rewrites wsum
-    int r1 = c1 + bef1 + i1 - s1, r2 = c2 + bef2 + i2 - s2;
+    int b1 = c1, b2 = c2;
 #pragma unroll
     for (int u = 0; u < V; ++u) {
-      if (b0 + u < n) {
-        out1[b0 + u] = r1;
-        if (out2) out2[b0 + u] = r2;
+      int t1 = 0, t2 = 0, p1 = 0, p2 = 0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int x = wsum[u][k], y = wsum[V + u][k];
+        t1 += x;
+        t2 += y;
+        if (k < w) {
+          p1 += x;
+          p2 += y;
+        }
       }
-      r1 += v[u];
-      r2 += (v[u] + div - 1) / div;
+      const int idx = base + u * 1024 + threadIdx.x;
+      if (idx < n) {
+        out1[idx] = b1 + p1 + i1[u] - v[u];
+        if (out2) out2[idx] = b2 + p2 + i2[u] - (v[u] + DIV - 1) / DIV;
+      }
+      b1 += t1;
+      b2 += t2;
     }
-    c1 += tot1;
-    c2 += tot2;
+    c1 = b1;
+    c2 = b2;
+    __syncthreads();  // the next tile rewrites wsum
   }
   if (threadIdx.x == 0) {
     out1[n] = c1;
@@ -165,16 +167,14 @@ __device__ void block_scan_tiled(int n, const int* __restrict__ cnt, int div, in
   __syncthreads();
 }
 
-__device__ inline void scan_counts(int n, const int* cnt, int div, int* out1, int* out2, int* wsum) {
-  block_scan_tiled<8>(n, cnt, div, out1, out2, wsum);
-}
+constexpr int kScanV = 8;
 
 struct ScanArgs {
+  int bid0;               // role of workgroup 0 of this launch (0 cells, 1 bricks, 2 dipole)
   int n0;                 // cells
   const int* cnt0;
   int* start0;
   int* fill0;
-  int hb;                 // homes per pair-kernel chunk
   int* chunk0;
   int n1;                 // bricks (0 without SPME)
   const int* cnt1;
@@ -185,11 +185,14 @@ struct ScanArgs {
   double* M;
 };
 
-// Workgroup 0: cell offsets and the per-cell home-chunk offsets of the pair kernel;
-// workgroup 1: brick offsets.  Both clear their fill counters.  Workgroup 2: total dipole.
+// Role 0: cell offsets and the per-cell home-chunk offsets of the pair kernel; role 1:
+// brick offsets.  Both clear their fill counters.  Role 2: total dipole.  A launch covers
+// roles bid0, bid0 + 1, ... (with SPME the cell role runs on the pair branch and the brick
+// and dipole roles on the mesh branch).
 __device__ __forceinline__ void scan_body(const ScanArgs& a, int bid) {
-  __shared__ int wsum[32];
+  __shared__ int wsum[2 * kScanV][16];
   __shared__ double dsum[16][3];
+  bid += a.bid0;
   if (bid == 2) {  // total dipole sum q r (surface term), from k_bin's partials
     double m[3] = {0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < a.ndip; b += 1024)
@@ -213,7 +216,8 @@ __device__ __forceinline__ void scan_body(const ScanArgs& a, int bid) {
   const int* cnt = bid == 0 ? a.cnt0 : a.cnt1;
   int* fill = bid == 0 ? a.fill0 : a.fill1;
   if (n <= 0) return;
-  scan_counts(n, cnt, a.hb, bid == 0 ? a.start0 : a.start1, bid == 0 ? a.chunk0 : nullptr, wsum);
+  block_scan_tiled<kScanV, kPairHB>(n, cnt, bid == 0 ? a.start0 : a.start1, bid == 0 ? a.chunk0 : nullptr,
+                                    wsum);
   for (int i = threadIdx.x; i < n; i += 1024) fill[i] = 0;
 }
 
@@ -228,6 +232,7 @@ __global__ __launch_bounds__(1024) void k_scan_seg(const ScanArgs* __restrict__ 
 
 struct ScatterArgs {
   int np;
+  int parts;      // 1: cell slots, 2: brick slots and brick-ordered spline data
   const int* cellid;
   const int* cell_start;
   int* cell_fill;
@@ -252,9 +257,11 @@ template <typename real>
 __device__ __forceinline__ void scatter_body(const ScatterArgs& a, int bid) {
   const int i = bid * 256 + threadIdx.x;
   if (i >= a.np) return;
-  const int c = a.cellid[i];
-  a.cell_tmp[a.cell_start[c] + atomicAdd(&a.cell_fill[c], 1)] = i;
-  if (a.use_mesh) {
+  if (a.parts & 1) {
+    const int c = a.cellid[i];
+    a.cell_tmp[a.cell_start[c] + atomicAdd(&a.cell_fill[c], 1)] = i;
+  }
+  if ((a.parts & 2) && a.use_mesh) {
     const int p = a.mm[i];
     const int b = (p & 1023) / kBX + a.m.nb[0] * (((p >> 10) & 1023) / kBY + a.m.nb[1] * ((p >> 20) / kBZ));
     const int dst = a.brick_start[b] + atomicAdd(&a.brick_fill[b], 1);
@@ -431,14 +438,14 @@ static BinArgs bin_args(Ctx* c) {
   return a;
 }
 
-static ScanArgs scan_args(Ctx* c) {
+static ScanArgs scan_args(Ctx* c, int bid0) {
   const bool mesh = c->P.ewald_mode == MSG_EWALD_SPM;
   ScanArgs a;
+  a.bid0 = bid0;
   a.n0 = c->grid.ncell;
   a.cnt0 = c->d_cell_count;
   a.start0 = c->d_cell_start;
   a.fill0 = c->d_cell_fill;
-  a.hb = kPairHB;
   a.chunk0 = c->d_cell_chunk;
   a.n1 = mesh ? c->mesh.nbrick : 0;
   a.cnt1 = c->d_brick_count;
@@ -450,9 +457,10 @@ static ScanArgs scan_args(Ctx* c) {
   return a;
 }
 
-static ScatterArgs scatter_args(Ctx* c) {
+static ScatterArgs scatter_args(Ctx* c, int parts) {
   ScatterArgs a;
   a.np = c->np;
+  a.parts = parts;
   a.cellid = c->d_cellid;
   a.cell_start = c->d_cell_start;
   a.cell_fill = c->d_cell_fill;
@@ -503,18 +511,21 @@ int launch_bin(Ctx* c) {
   return MSG_OK;
 }
 
-// Scan and scatter: after these the brick arrays are complete (the mesh branch may start);
-// the cell order is completed by launch_finalize.
-int launch_sort(Ctx* c) {
+// Scan and scatter.  which = 1: cell slots (the cell order is completed by launch_finalize);
+// 2: brick slots, brick-ordered spline data and the total dipole (the mesh branch may start
+// after these); 3: both.
+int launch_sort(Ctx* c, int which) {
   const int np = c->np;
+  const int bid0 = which == 2 ? 1 : 0;
+  const int nscan = which == 1 ? 1 : (which == 2 ? 2 : 3);
   prof_begin(c, MSG_K_SCAN);
-  hipLaunchKernelGGL(k_scan, dim3(3), dim3(1024), 0, c->stream, scan_args(c));
+  hipLaunchKernelGGL(k_scan, dim3(nscan), dim3(1024), 0, c->stream, scan_args(c, bid0));
   prof_end(c, MSG_K_SCAN);
   prof_begin(c, MSG_K_SCATTER);
   if (c->fp64)
-    hipLaunchKernelGGL(k_scatter<double>, dim3((np + 255) / 256), dim3(256), 0, c->stream, scatter_args(c));
+    hipLaunchKernelGGL(k_scatter<double>, dim3((np + 255) / 256), dim3(256), 0, c->stream, scatter_args(c, which));
   else
-    hipLaunchKernelGGL(k_scatter<float>, dim3((np + 255) / 256), dim3(256), 0, c->stream, scatter_args(c));
+    hipLaunchKernelGGL(k_scatter<float>, dim3((np + 255) / 256), dim3(256), 0, c->stream, scatter_args(c, which));
   prof_end(c, MSG_K_SCATTER);
   MSG_HIP(hipGetLastError());
   return MSG_OK;
@@ -552,19 +563,27 @@ int plan_bin(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out) {
   return b.finish(own, out);
 }
 
-int plan_sort(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& scan, SegStage& scatter,
-              SegStage& finalize) {
-  SegBuilder<ScanArgs> bs;
-  SegBuilder<ScatterArgs> bc;
+// Batched list stages: cell scan and scatter (the pair branch), brick/dipole scan and brick
+// scatter (the mesh branch), cell ranking.
+int plan_sort(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& scan_c, SegStage& scan_b,
+              SegStage& scatter_c, SegStage& scatter_b, SegStage& finalize) {
+  SegBuilder<ScanArgs> bsc, bsb;
+  SegBuilder<ScatterArgs> bcc, bcb;
   SegBuilder<FinalizeArgs> bf;
   for (Ctx* c : cs) {
-    bs.add(0, seg_scan, scan_args(c), 3);
-    bc.add(c->fp64, c->fp64 ? seg_scatter<double> : seg_scatter<float>, scatter_args(c), (c->np + 255) / 256);
+    const int nb = (c->np + 255) / 256;
+    bsc.add(0, seg_scan, scan_args(c, 0), 1);
+    bsb.add(0, seg_scan, scan_args(c, 1), 2);
+    bcc.add(c->fp64, c->fp64 ? seg_scatter<double> : seg_scatter<float>, scatter_args(c, 1), nb);
+    if (c->P.ewald_mode == MSG_EWALD_SPM)
+      bcb.add(c->fp64, c->fp64 ? seg_scatter<double> : seg_scatter<float>, scatter_args(c, 2), nb);
     const FinalizeArgs f = finalize_args(c);
     bf.add(0, seg_finalize, f, finalize_blocks(f));
   }
-  if (int e = bs.finish(own, scan)) return e;
-  if (int e = bc.finish(own, scatter)) return e;
+  if (int e = bsc.finish(own, scan_c)) return e;
+  if (int e = bsb.finish(own, scan_b)) return e;
+  if (int e = bcc.finish(own, scatter_c)) return e;
+  if (int e = bcb.finish(own, scatter_b)) return e;
   return bf.finish(own, finalize);
 }
 

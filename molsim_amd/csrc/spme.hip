@@ -8,7 +8,12 @@
 //               hence deterministic; no global atomics) and every mesh point is written once.
 //   k_fft_x     r2c along x: two real rows packed as one complex line, Stockham radix-4/2
 //               in LDS, split into the two half spectra.
-//   k_fft_y     complex FFT along y on tiles of T consecutive kx (forward or inverse).
+//   k_fft_y     complex FFT along y on tiles of T lines (forward or inverse).
+// Half-complex layout: kx = 0..K/2-1 as Cm[kx + (K/2)(y + K z)] (rows of K/2 values, so a
+// tile of consecutive kx is an aligned segment), and the Nyquist column kx = K/2 as its own
+// K x K plane Nq[y + K z] behind it.  The y and z passes index lines by L = kx + (K/2) * (z or
+// y) for the K/2 regular columns, followed by the K Nyquist lines, so no workgroup computes
+// padding lines.
 //   k_fft_zconv forward FFT along z, influence-function multiply with the FP64 energy and
 //               virial reduction (the reference's CalcGIF loop), inverse FFT along z.
 //   k_fft_xi    c2r along x (half spectra of two rows -> one complex line -> two real rows).
@@ -134,11 +139,18 @@ __device__ __forceinline__ void fft_x_body(const FftArgs& fa, int bid) {
   __syncthreads();
   const C2* Z = lds_fft<C2, K, T>(A, B, s_tw, false) + line * K;
   const real h = real(0.5);
+  C2* __restrict__ Nq = Cm + (size_t)(K / 2) * K * K;
   for (int k = tl; k < H; k += TPL) {
     const C2 z = Z[k], zc = cconj(Z[(K - k) & (K - 1)]);
     const C2 s = cadd(z, zc), d = csub(z, zc);
-    Cm[r0 * H + k] = cmk<C2>(h * s.x, h * s.y);
-    Cm[r1 * H + k] = cmk<C2>(h * d.y, -h * d.x);  // (z - zc) / (2i)
+    const C2 c0 = cmk<C2>(h * s.x, h * s.y), c1 = cmk<C2>(h * d.y, -h * d.x);  // (z - zc) / (2i)
+    if (k < K / 2) {
+      Cm[r0 * (K / 2) + k] = c0;
+      Cm[r1 * (K / 2) + k] = c1;
+    } else {
+      Nq[r0] = c0;
+      Nq[r1] = c1;
+    }
   }
 }
 
@@ -155,8 +167,9 @@ __device__ __forceinline__ void fft_xi_body(const FftArgs& fa, int bid) {
   for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
   const int line = threadIdx.x / TPL, tl = threadIdx.x % TPL;
   const size_t r0 = 2 * ((size_t)bid * T + line), r1 = r0 + 1;
+  const C2* __restrict__ Nq = Cm + (size_t)(K / 2) * K * K;
   for (int k = tl; k < H; k += TPL) {
-    C2 a = Cm[r0 * H + k], b = Cm[r1 * H + k];
+    C2 a = k < K / 2 ? Cm[r0 * (K / 2) + k] : Nq[r0], b = k < K / 2 ? Cm[r1 * (K / 2) + k] : Nq[r1];
     if (k == 0 || k == K / 2) { a.y = 0; b.y = 0; }
     A[line * K + k] = cmk<C2>(a.x - b.y, a.y + b.x);                 // a + i b
     if (k > 0 && k < K / 2) A[line * K + K - k] = cmk<C2>(a.x + b.y, b.x - a.y);  // conj(a) + i conj(b)
@@ -170,27 +183,48 @@ __device__ __forceinline__ void fft_xi_body(const FftArgs& fa, int bid) {
   }
 }
 
-// Complex FFT along y (stride H) for tiles of T consecutive kx at one z.
+// Address of element w (y for the y pass, z for the z pass) of line L of the half-complex
+// mesh: regular lines L < K^2/2 are (kx, v) = (L mod K/2, L div K/2) with v the other
+// coordinate; lines beyond are the Nyquist column at v = L - K^2/2.
+template <int K, bool YPASS>
+__device__ __forceinline__ size_t hc_index(int L, int w) {
+  constexpr int NH = K / 2, NREG = NH * K;
+  if (L < NREG) {
+    const int kx = L % NH, v = L / NH;
+    return YPASS ? kx + (size_t)NH * (w + (size_t)K * v) : kx + (size_t)NH * (v + (size_t)K * w);
+  }
+  const int v = L - NREG;
+  return (size_t)NH * K * K + (YPASS ? w + (size_t)K * v : v + (size_t)K * w);
+}
+
+// Lines of the y and z passes and the workgroups that cover them (T lines each).
+template <int K, int T>
+struct HcLines {
+  static constexpr int N = (K / 2 + 1) * K;
+  static constexpr int NB = (N + T - 1) / T;
+};
+
+// Complex FFT along y for T lines (consecutive kx at one z, or Nyquist lines).
 template <typename real, int K>
 __device__ __forceinline__ void fft_y_body(const FftArgs& fa, int bid, int inverse) {
   typedef typename C2T<real>::t C2;
   C2* __restrict__ Cm = (C2*)fa.Cm;
   const C2* __restrict__ tw = (const C2*)fa.tw;
-  constexpr int E = FftCfg<real>::E, T = E / K, H = K / 2 + 1, NTX = (H + T - 1) / T;
+  constexpr int E = FftCfg<real>::E, T = E / K, NL = HcLines<K, T>::N;
   constexpr int LS = K + 1;  // padded lines: the transposing loads and stores avoid LDS bank conflicts
   __shared__ C2 A[T * LS], B[T * LS];
   __shared__ C2 s_tw[K];  // twiddles staged in LDS
   for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
-  const int tx = bid % NTX, z = bid / NTX, kx0 = tx * T;
+  const int L0 = bid * T;
   for (int e = threadIdx.x; e < E; e += 256) {
-    const int kl = e % T, y = e / T, kx = kx0 + kl;
-    A[kl * LS + y] = kx < H ? Cm[kx + (size_t)H * (y + (size_t)K * z)] : cmk<C2>(0, 0);
+    const int kl = e % T, y = e / T, L = L0 + kl;
+    A[kl * LS + y] = L < NL ? Cm[hc_index<K, true>(L, y)] : cmk<C2>(0, 0);
   }
   __syncthreads();
   const C2* R = lds_fft<C2, K, T, LS>(A, B, s_tw, inverse != 0);
   for (int e = threadIdx.x; e < E; e += 256) {
-    const int kl = e % T, y = e / T, kx = kx0 + kl;
-    if (kx < H) Cm[kx + (size_t)H * (y + (size_t)K * z)] = R[kl * LS + y];
+    const int kl = e % T, y = e / T, L = L0 + kl;
+    if (L < NL) Cm[hc_index<K, true>(L, y)] = R[kl * LS + y];
   }
 }
 
@@ -203,28 +237,29 @@ __device__ __forceinline__ void fft_zconv_body(const FftArgs& fa, int bid) {
   const real* __restrict__ G = (const real*)fa.G;
   const ConvArgs& ca = fa.ca;
   double* __restrict__ part = fa.part;
-  constexpr int E = FftCfg<real>::E, T = E / K, H = K / 2 + 1, NTX = (H + T - 1) / T;
+  constexpr int E = FftCfg<real>::E, T = E / K, H = K / 2 + 1, NL = HcLines<K, T>::N;
+  constexpr int NH = K / 2, NREG = NH * K;
   constexpr int LS = K + 1;  // padded lines (see fft_y_body)
   __shared__ C2 A[T * LS], B[T * LS];
   __shared__ C2 s_tw[K];  // twiddles staged in LDS
   for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
   __shared__ double red[8];
-  const int tx = bid % NTX, y = bid / NTX, kx0 = tx * T;
-  const size_t plane = (size_t)H * K;
+  const int L0 = bid * T;
   for (int e = threadIdx.x; e < E; e += 256) {
-    const int kl = e % T, z = e / T, kx = kx0 + kl;
-    A[kl * LS + z] = kx < H ? Cm[kx + (size_t)H * y + plane * z] : cmk<C2>(0, 0);
+    const int kl = e % T, z = e / T, L = L0 + kl;
+    A[kl * LS + z] = L < NL ? Cm[hc_index<K, false>(L, z)] : cmk<C2>(0, 0);
   }
   __syncthreads();
   C2* R = lds_fft<C2, K, T, LS>(A, B, s_tw, false);
-  const int ky = y <= K / 2 ? y : K - y;
   double esum = 0.0, vsum = 0.0;
   for (int e = threadIdx.x; e < E; e += 256) {
     // no contraction of the FP64 energy/virial terms: the value must not depend on how
     // the compiler fuses them in the single-system and the batched instantiation
 #pragma clang fp contract(off)
-    const int kl = e / K, kz = e % K, kx = kx0 + kl;
-    if (kx >= H) continue;
+    const int kl = e / K, kz = e % K, L = L0 + kl;
+    if (L >= NL) continue;
+    const int kx = L < NREG ? L % NH : NH, y = L < NREG ? L / NH : L - NREG;
+    const int ky = y <= K / 2 ? y : K - y;
     const int kzz = kz <= K / 2 ? kz : K - kz;
     const real g = G[kx + H * (ky + H * kzz)];
     C2 f = R[kl * LS + kz];
@@ -242,8 +277,8 @@ __device__ __forceinline__ void fft_zconv_body(const FftArgs& fa, int bid) {
   C2* other = (R == A) ? B : A;
   const C2* S = lds_fft<C2, K, T, LS>(R, other, s_tw, true);
   for (int e = threadIdx.x; e < E; e += 256) {
-    const int kl = e % T, z = e / T, kx = kx0 + kl;
-    if (kx < H) Cm[kx + (size_t)H * y + plane * z] = S[kl * LS + z];
+    const int kl = e % T, z = e / T, L = L0 + kl;
+    if (L < NL) Cm[hc_index<K, false>(L, z)] = S[kl * LS + z];
   }
   esum = msgd::block_sum(esum, red);
   vsum = msgd::block_sum(vsum, red);
@@ -564,9 +599,9 @@ FftArgs fft_args(Ctx* c) {
 // Workgroups of the five FFT stages for mesh K: x, y, zconv, yi, xi.
 template <typename real, int K>
 void fft_blocks(int* nb) {
-  constexpr int E = FftCfg<real>::E, T = E / K, H = K / 2 + 1, NTX = (H + T - 1) / T;
+  constexpr int E = FftCfg<real>::E, T = E / K;
   nb[0] = nb[4] = K * K / 2 / T;
-  nb[1] = nb[2] = nb[3] = NTX * K;
+  nb[1] = nb[2] = nb[3] = HcLines<K, T>::NB;
 }
 
 template <typename real, int K>
@@ -741,7 +776,7 @@ int spme_setup(Ctx* c) {
     MSG_HIP(hipMemcpy(c->d_tw, twf.data(), twf.size() * 4, hipMemcpyHostToDevice));
   }
   const int T = (c->fp64 ? FftCfg<double>::E : FftCfg<float>::E) / K;
-  c->nzblocks = ((H + T - 1) / T) * K;
+  c->nzblocks = (H * K + T - 1) / T;  // workgroups of the z pass (HcLines<K, T>::NB)
   return MSG_OK;
 }
 
