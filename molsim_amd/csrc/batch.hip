@@ -84,14 +84,14 @@ static bool serial_pass() {
   return v;
 }
 
-int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, const hipEvent_t* fork, const hipEvent_t* join) {
+int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
   if (p.cs.empty()) return MSG_OK;
   if (int e = run_stage(p.bin, s)) return e;
   const bool branch = p.any_spme && !serial_pass();
   hipStream_t cs = s;  // stream of the cell / pair stages
   if (branch) {
-    MSG_HIP(hipEventRecord(fork[0], s));
-    MSG_HIP(hipStreamWaitEvent(side, fork[0], 0));
+    MSG_HIP(hipEventRecord(fork, s));
+    MSG_HIP(hipStreamWaitEvent(side, fork, 0));
     cs = side;
   }
   if (int e = run_stage(p.scan_c, cs)) return e;
@@ -99,30 +99,17 @@ int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, const hipEvent_t
   if (int e = run_stage(p.finalize, cs)) return e;
   if (int e = run_stage(p.pair, cs)) return e;
   if (int e = run_stage(p.pair_sum, cs)) return e;
-  if (branch) MSG_HIP(hipEventRecord(join[0], side));
+  if (branch) MSG_HIP(hipEventRecord(join, side));
   if (int e = run_stage(p.scan_b, s)) return e;
   if (int e = run_stage(p.scatter_b, s)) return e;
   if (p.any_spme) {
     if (int e = run_stage(p.spread, s)) return e;
     for (int k = 0; k < 5; ++k)
       if (int e = run_stage(p.fft[k], s)) return e;
-    if (branch) {
-      // the energy reduction needs the pair-row and k-space sums, not the forces: it runs
-      // beside the gather
-      MSG_HIP(hipStreamWaitEvent(s, join[0], 0));
-      MSG_HIP(hipEventRecord(fork[1], s));
-      MSG_HIP(hipStreamWaitEvent(side, fork[1], 0));
-      if (int e = run_stage(p.reduce, side)) return e;
-      MSG_HIP(hipEventRecord(join[1], side));
-      if (int e = run_stage(p.gather, s)) return e;
-      MSG_HIP(hipStreamWaitEvent(s, join[1], 0));
-    } else {
-      if (int e = run_stage(p.gather, s)) return e;
-      if (int e = run_stage(p.reduce, s)) return e;
-    }
-  } else {
-    if (int e = run_stage(p.reduce, s)) return e;
+    if (branch) MSG_HIP(hipStreamWaitEvent(s, join, 0));
+    if (int e = run_stage(p.gather, s)) return e;
   }
+  if (int e = run_stage(p.reduce, s)) return e;
   hipLaunchKernelGGL(k_collect, dim3((unsigned)p.cs.size()), dim3(64), 0, s, p.d_outp, p.d_outs);
   MSG_HIP(hipGetLastError());
   return MSG_OK;
@@ -242,7 +229,7 @@ struct msg_batch {
   std::vector<msg_ctx*> ctx;
   std::vector<Ctx*> cs;
   hipStream_t origin = nullptr, side = nullptr;
-  hipEvent_t fork[2] = {}, join[2] = {};
+  hipEvent_t fork = nullptr, join = nullptr;
   SegOwner own;                 // the pass plan and trial groups (built once)
   SegOwner trial_own;           // per-call trial inputs (arena)
   Pass pass;
@@ -280,10 +267,8 @@ int msg_batch_create(msg_ctx** ctxs, int n, msg_batch** out) {
       return fail(set_error(MSG_ERR_HIP, "msg_batch_create: context stream failed"));
   if (hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&b->fork[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->fork[1], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->join[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&b->join[1], hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess)
     return fail(set_error(MSG_ERR_HIP, "msg_batch_create: stream/event creation failed"));
   int e;
   if ((e = build_pass(b->cs, b->own, b->pass))) return fail(e);
@@ -314,10 +299,8 @@ void msg_batch_destroy(msg_batch* b) {
   if (b->graph) hipGraphDestroy(b->graph);
   if (b->d_de) hipFree(b->d_de);
   if (b->h_outs) hipHostFree(b->h_outs);
-  for (int k = 0; k < 2; ++k) {
-    if (b->fork[k]) hipEventDestroy(b->fork[k]);
-    if (b->join[k]) hipEventDestroy(b->join[k]);
-  }
+  if (b->fork) hipEventDestroy(b->fork);
+  if (b->join) hipEventDestroy(b->join);
   if (b->side) hipStreamDestroy(b->side);
   if (b->origin) hipStreamDestroy(b->origin);
   delete b;
@@ -391,7 +374,7 @@ int msg_batch_dutotal(msg_batch* b, const int* ntrial, const int* ip, const doub
 
 struct msg_unit {
   hipStream_t s = nullptr, side = nullptr;
-  hipEvent_t fork[2] = {}, join[2] = {}, done = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, done = nullptr;
   SegOwner own;
   Pass pass;
   std::vector<Ctx*> ut, dec;
@@ -448,10 +431,8 @@ void msg_unit_destroy(msg_unit* u) {
   if (u->h_outs) hipHostFree(u->h_outs);
   if (u->h_de) hipHostFree(u->h_de);
   if (u->h_f) hipHostFree(u->h_f);
-  for (int k = 0; k < 2; ++k) {
-    if (u->fork[k]) hipEventDestroy(u->fork[k]);
-    if (u->join[k]) hipEventDestroy(u->join[k]);
-  }
+  if (u->fork) hipEventDestroy(u->fork);
+  if (u->join) hipEventDestroy(u->join);
   if (u->done) hipEventDestroy(u->done);
   if (u->side) hipStreamDestroy(u->side);
   if (u->s) hipStreamDestroy(u->s);
@@ -464,10 +445,8 @@ int msg_unit_create(msg_unit** out) {
   *out = nullptr;
   if (hipStreamCreateWithFlags(&u->s, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&u->side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&u->fork[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&u->fork[1], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&u->join[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&u->join[1], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&u->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&u->join, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&u->done, hipEventDisableTiming) != hipSuccess) {
     msg_unit_destroy(u);
     return set_error(MSG_ERR_HIP, "msg_unit_create: stream/event creation failed");

@@ -92,48 +92,33 @@ struct ReduceArgs {
   int nbrick;
 };
 
-// Per-thread part of the fixed-order sum of the pair kernel's rows (np1 = energies by type
-// pair + virial per workgroup): thread t adds the flat elements t, t + 1024, t + 2048, ...
-// in that order into v[column].  Eight loads are issued before any is used; the loads are
-// unconditional (index clamped) so their values stay in registers.
-__device__ __forceinline__ void pair_rows_part(const double* __restrict__ rows, int nrow, int np1,
-                                               double* v) {
-  const int npv = nrow * np1;
-  for (int f0 = threadIdx.x; f0 < npv; f0 += 8 * 1024) {
-    double x[8];
-    int kk[8];
+// Fixed-order column totals of the pair kernel's rows (np1 columns: energies by type pair +
+// virial, one row per workgroup) by one 1024-thread workgroup.  Thread t owns column
+// t mod np1 and adds rows t div np1, t div np1 + R, ... (R = 1024 div np1 threads per column,
+// four loads in flight); the R partial sums of a column are then added in a fixed binary tree
+// in LDS.  The totals end in red[0 .. np1).  With one row the totals are the row itself.
+__device__ __forceinline__ void pair_rows_total(const double* __restrict__ rows, int nrow, int np1,
+                                                double* red) {
+  const int R = 1024 / np1;
+  const int c = threadIdx.x % np1, r0 = threadIdx.x / np1;
+  double s = 0.0;
+  if (r0 < R) {
+    for (int r = r0; r < nrow; r += 4 * R) {
+      double x[4];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int f = f0 + u * 1024;
-      x[u] = rows[min(f, npv - 1)];
-      if (f >= npv) x[u] = 0.0;
-      kk[u] = f % np1;
+      for (int u = 0; u < 4; ++u) x[u] = r + u * R < nrow ? rows[(size_t)(r + u * R) * np1 + c] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s += x[u];
     }
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-#pragma unroll
-      for (int k = 0; k <= kMaxTP; ++k) v[k] += (k == kk[u]) ? x[u] : 0.0;
   }
-}
-
-// Workgroup total of v[0..NV) (1024 threads): wavefront sums, then the 16 wavefront partials
-// in order.  tot[k] is valid after the call in every thread.
-template <int NV>
-__device__ __forceinline__ void block_total(const double* v, double (*red)[NV], double* tot) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    const double t = msgd::wave_sum(v[k]);
-    if (lane == 0) red[w][k] = t;
-  }
+  red[threadIdx.x] = s;
   __syncthreads();
-  if (threadIdx.x < NV) {
-    double x = 0.0;
-#pragma unroll
-    for (int w2 = 0; w2 < 16; ++w2) x += red[w2][threadIdx.x];
-    tot[threadIdx.x] = x;
+  int p2 = 1;
+  while (p2 < R) p2 <<= 1;
+  for (int st = p2 >> 1; st > 0; st >>= 1) {
+    if (r0 < st && r0 + st < R) red[threadIdx.x] += red[threadIdx.x + st * np1];
+    __syncthreads();
   }
-  __syncthreads();
 }
 
 struct PairSumArgs {
@@ -143,18 +128,12 @@ struct PairSumArgs {
 };
 
 // The pair rows summed into one row.  With SPME this runs on the pair branch, beside the
-// mesh kernels, so the final reduction reads one row.  The result is bitwise the sum the
-// final reduction forms from the rows itself (same order; it only adds zeros to it).
+// mesh kernels, so the final reduction reads one row (the same totals it would form from
+// the rows itself).
 __device__ __forceinline__ void pair_sum_body(const PairSumArgs& a) {
-  constexpr int NV = kMaxTP + 1;
-  __shared__ double red[16][NV];
-  __shared__ double tot[NV];
-  double v[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = 0.0;
-  pair_rows_part(a.rows, a.nrow, a.np1, v);
-  block_total<NV>(v, red, tot);
-  if (threadIdx.x < a.np1) a.out[threadIdx.x] = tot[threadIdx.x];
+  __shared__ double red[1024];
+  pair_rows_total(a.rows, a.nrow, a.np1, red);
+  if (threadIdx.x < a.np1) a.out[threadIdx.x] = red[threadIdx.x];
 }
 
 __global__ __launch_bounds__(1024) void k_pair_sum(PairSumArgs a) { pair_sum_body(a); }
@@ -172,50 +151,65 @@ void seg_pair_sum(const SegGroup& g, hipStream_t s) {
 
 // Final FP64 reduction of all partial sums in a fixed order (deterministic).  It is the last
 // kernel of a UTotal pass, so it also hands the error flags to the output block and clears
-// the cell/brick counters and flags for the next pass.  The k-space partials, dipole and
-// flags are loaded before the pair rows are summed, so their loads overlap.
+// the cell/brick counters and flags for the next pass.  The k-space partials, dipole, flags
+// and a pre-summed pair row are loaded together at the start.
 __device__ __forceinline__ void reduce_body(const ReduceArgs& a) {
-  constexpr int NV = kMaxTP + 1 + 2;
-  __shared__ double red[16][NV];
-  __shared__ double tot[NV];
+  __shared__ double red[1024];
+  __shared__ double wred[16][2];
   const int tid = threadIdx.x;
   const double* ew = a.ewald == 2 ? a.spme : a.ewstd;
   const int nev = a.ewald == 2 ? a.nspme : (a.ewald == 1 ? a.newstd : 0);
+  const int np1 = a.ntp + 1;
   double e0 = 0.0, e1 = 0.0;
   if (tid < nev) {
     e0 = ew[2 * tid];
     e1 = ew[2 * tid + 1];
   }
   double m[3] = {0.0, 0.0, 0.0};
+  double pt[kMaxTP + 1];
+#pragma unroll
+  for (int k = 0; k <= kMaxTP; ++k) pt[k] = 0.0;
   int fl = 0;
   if (tid == 0) {
     m[0] = a.M[0];
     m[1] = a.M[1];
     m[2] = a.M[2];
     fl = a.flags[0];
-  }
-  double v[NV];
+    if (a.npair == 1)
 #pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = 0.0;
-  const int np1 = a.ntp + 1;
-  pair_rows_part(a.pair, a.npair, np1, v);
-  v[kMaxTP + 1] = e0;
-  v[kMaxTP + 2] = e1;
-  for (int b = tid + 1024; b < nev; b += 1024) {
-    v[kMaxTP + 1] += ew[2 * b];
-    v[kMaxTP + 2] += ew[2 * b + 1];
+      for (int k = 0; k <= kMaxTP; ++k) pt[k] = k < np1 ? a.pair[k] : 0.0;
   }
-  block_total<NV>(v, red, tot);
+  for (int b = tid + 1024; b < nev; b += 1024) {
+    e0 += ew[2 * b];
+    e1 += ew[2 * b + 1];
+  }
+  if (a.npair != 1) {
+    pair_rows_total(a.pair, a.npair, np1, red);
+    if (tid == 0)
+#pragma unroll
+      for (int k = 0; k <= kMaxTP; ++k) pt[k] = k < np1 ? red[k] : 0.0;
+  }
+  e0 = msgd::wave_sum(e0);
+  e1 = msgd::wave_sum(e1);
+  if ((tid & 63) == 0) {
+    wred[tid >> 6][0] = e0;
+    wred[tid >> 6][1] = e1;
+  }
+  __syncthreads();
   if (tid == 0) {
-    const double* t = tot;
+    double es = 0.0, vs = 0.0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+      es += wred[w][0];
+      vs += wred[w][1];
+    }
     double* o = a.out;
     double tw = 0.0;
     for (int tp = 0; tp < a.ntp; ++tp) {
-      o[1 + tp] = 0.5 * t[tp];  // every pair was evaluated from both of its particles
-      tw += 0.5 * t[tp];
+      o[1 + tp] = 0.5 * pt[tp];  // every pair was evaluated from both of its particles
+      tw += 0.5 * pt[tp];
     }
     o[0] = tw;
-    const double es = t[kMaxTP + 1], vs = t[kMaxTP + 2];
     double urec = 0.0, virrec = 0.0;
     if (a.ewald) {
       // SPME: E = sum w G |F|^2, virrec -= sum w G (1 + 2 m^2 expf) |F|^2
@@ -230,7 +224,7 @@ __device__ __forceinline__ void reduce_body(const ReduceArgs& a) {
     }
     o[a.ntp + 1] = a.epsi4pi * urec;
     o[a.ntp + 2] = tw + a.epsi4pi * urec;
-    o[a.ntp + 3] = 0.5 * t[a.ntp] + a.epsi4pi * virrec;
+    o[a.ntp + 3] = 0.5 * pt[a.ntp] + a.epsi4pi * virrec;
     o[a.ntp + 4] = m[0];
     o[a.ntp + 5] = m[1];
     o[a.ntp + 6] = m[2];
@@ -333,7 +327,8 @@ int launch_reduce(Ctx* c, bool presum) {
 // With SPME the work forks after the binning pass: the cell list (scan, scatter, ranking,
 // Verlet rebuild), the pair kernel and the pair-row sum run on a second stream beside the
 // brick scan/scatter and the mesh kernels; the gather, which adds to the pair forces, waits
-// for them.  With per-kernel timing on, everything runs on one stream so each kernel is
+// for them.  One fork only: each cross-stream dependency costs tens of microseconds of
+// graph execution time.  With per-kernel timing on, everything runs on one stream so each kernel is
 // timed alone.
 int utotal_sequence(Ctx* c) {
   int e;
@@ -370,17 +365,9 @@ int utotal_sequence(Ctx* c) {
     if ((e = launch_sort(c, 2))) return e;
     if ((e = launch_spme_mesh(c))) return e;
     MSG_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
-    // the energy reduction needs the pair-row and k-space sums, not the forces: it runs
-    // beside the gather
-    MSG_HIP(hipEventRecord(c->ev_fork2, c->stream));
-    MSG_HIP(hipStreamWaitEvent(c->side, c->ev_fork2, 0));
-    c->stream = c->side;
-    e = launch_reduce(c, true);
-    c->stream = main;
-    if (e) return e;
-    MSG_HIP(hipEventRecord(c->ev_join2, c->side));
     if ((e = launch_spme_gather(c))) return e;
-    MSG_HIP(hipStreamWaitEvent(c->stream, c->ev_join2, 0));
+    // (a second fork for the reduction beside the gather costs more than the reduction)
+    if ((e = launch_reduce(c, true))) return e;
     return MSG_OK;
   } else {
     if ((e = cell_work())) return e;
@@ -468,9 +455,7 @@ int msg_create(const msg_params* p, msg_ctx** out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
     return fail(set_error(MSG_ERR_HIP, "stream/event creation failed"));
   const int np = c->np;
   if ((e = dalloc(c, (void**)&c->d_ro, 3 * np * sizeof(double)))) return fail(e);
@@ -528,8 +513,6 @@ void msg_destroy(msg_ctx* h) {
   if (c->side) hipStreamDestroy(c->side);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
-  if (c->ev_fork2) hipEventDestroy(c->ev_fork2);
-  if (c->ev_join2) hipEventDestroy(c->ev_join2);
   if (c->ev_batch) hipEventDestroy(c->ev_batch);
   if (c->stream) hipStreamDestroy(c->stream);
   delete h;

@@ -86,7 +86,7 @@ struct Ctx {
   bool fp64 = false;
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;        // second branch of the UTotal graph (pair kernel)
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipEvent_t ev_batch = nullptr;     // orders the context's own work before a batched pass
   int device = 0;
   long long bytes = 0;
@@ -399,8 +399,7 @@ struct Pass {
   double* d_outs = nullptr;
 };
 int build_pass(const std::vector<Ctx*>& cs, SegOwner& own, Pass& p);
-int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, const hipEvent_t* fork,
-                const hipEvent_t* join);  // two events each
+int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, hipEvent_t fork, hipEvent_t join);
 int check_batchable(Ctx* c);
 
 // Batched MC trials: systems grouped by spline order; d_args holds one block per member.
