@@ -324,6 +324,25 @@ template <> struct R4T<double> { typedef double4 t; };
 // of 3.6e-15, below FP64 rounding of the mesh values.
 constexpr double kQScale = 281474976710656.0;  // 2^48
 
+// floor(c * 2^48) as a 64-bit two's-complement integer from FP32 arithmetic only: the high
+// word is h = floor(c * 2^16) (|c| < 2^15), the low word the fraction (c * 2^16 - h) * 2^32,
+// both exact in FP32 (the fraction of an FP32 value is representable, and scaling by a power
+// of two is exact); below |c| = 2^-25 the low word is truncated.
+__device__ __forceinline__ unsigned long long fixed48(float c) {
+  const float sc = c * 65536.0f;
+  const float h = floorf(sc);
+  const float f = (sc - h) * 4294967296.0f;
+  const unsigned lo = (unsigned)f;
+  const unsigned hi = (unsigned)(int)h;
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+template <typename real>
+__device__ __forceinline__ unsigned long long to_fixed(real c) {
+  if constexpr (sizeof(real) == 4) return fixed48(c);
+  else return (unsigned long long)__double2ll_rn(c * kQScale);
+}
+
 // Stencil range of one dimension inside a brick of extent B starting at the particle's
 // local base point l (the stencil covers l, l-1, ..., l-P+1): with one brick across the
 // mesh every point is inside (indices wrap); otherwise only i with 0 <= l - i < B.
@@ -341,13 +360,16 @@ __device__ inline void brick_range(int l, int B, bool whole, int& i0, int& n) {
 // One workgroup per brick.  Phase 1: for up to CAP contributing particles (from the brick
 // and the next brick in each dimension) the B-spline weights go to LDS together with the
 // particle's stencil rows that fall inside this brick (clipped y and z ranges).  Phase 2:
-// all threads take (particle, z, y) rows of that clipped set -- found by a binary search over
-// the rows' prefix sums -- and add the clipped x-row into the brick with 64-bit integer LDS
-// atomics.  Phase 3: every mesh point of the brick is written once.
-template <typename real, int P>
+// all threads take (particle, z, y) rows of that clipped set and add the clipped x-row into
+// the brick with 64-bit integer LDS atomics.  A row finds its particle in a row -> particle
+// table that the particles fill (OWN; one LDS read per row, for sparse bricks, where a
+// brick has few chunks) or by a binary search over the rows' prefix sums (dense bricks,
+// where the table's extra pass per chunk costs more than it saves).  Phase 3: every mesh
+// point of the brick is written once.
+template <typename real, int P, bool OWN>
 __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   typedef typename R4T<real>::t R4;
-  constexpr int CAP = 128;
+  constexpr int CAP = 96;
   constexpr int NBP = kBX * kBY * kBZ;
   // rows padded to kBX + 1 accumulators: the lanes adding one particle's rows (same x,
   // consecutive y/z) then fall into different LDS banks instead of one
@@ -357,6 +379,7 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   __shared__ int loc[CAP];
   __shared__ int rng[CAP];      // packed y0 | ny << 8 | z0 << 16 | nz << 24
   __shared__ int rpre[CAP + 1]; // prefix sums of rows per particle
+  __shared__ unsigned char owner[OWN ? CAP * P * P : 1];  // row -> particle of the chunk
   __shared__ int wsum[4];
   __shared__ int r_beg[8], r_pref[9], r_len[8];
   const int K = a.m.K;
@@ -429,23 +452,30 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
     __syncthreads();
     int before = 0;
     for (int k = 0; k < w; ++k) before += wsum[k];
-    if ((int)threadIdx.x < CAP) rpre[threadIdx.x] = before + incl - rows;
-    if (threadIdx.x == 0) rpre[CAP] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    const int first = before + incl - rows;
+    if ((int)threadIdx.x < CAP) rpre[threadIdx.x] = first;
+    if (OWN)
+      for (int r = 0; r < rows; ++r) owner[first + r] = (unsigned char)threadIdx.x;
+    const int ntask = wsum[0] + wsum[1] + wsum[2] + wsum[3];
     __syncthreads();
-    const int ntask = rpre[CAP];
     for (int task = threadIdx.x; task < ntask; task += 256) {
-      int lo_p = 0, hi_p = cnt - 1;  // last particle with rpre[p] <= task
-      while (lo_p < hi_p) {
-        const int mid = (lo_p + hi_p + 1) >> 1;
-        if (rpre[mid] <= task) lo_p = mid; else hi_p = mid - 1;
+      int p;
+      if (OWN) {
+        p = owner[task];
+      } else {
+        int lo_p = 0, hi_p = cnt - 1;  // last particle with rpre[p] <= task
+        while (lo_p < hi_p) {
+          const int mid = (lo_p + hi_p + 1) >> 1;
+          if (rpre[mid] <= task) lo_p = mid; else hi_p = mid - 1;
+        }
+        p = lo_p;
       }
-      const int p = lo_p;
       const int rr = rng[p];
       const int ny = (rr >> 8) & 255;
       const int r = task - rpre[p];
-      // r / ny by a reciprocal: r < 64 and ny <= 8 keep the fractional part of r / ny at
-      // least 1/8 away from the next integer, far beyond the reciprocal's error
-      const int q = (int)((float)r * __frcp_rn((float)ny) + 0.01f);
+      // r / ny by the hardware reciprocal: r < 64 and ny <= 8 keep the fractional part of
+      // r / ny at least 1/8 away from the next integer, far beyond the reciprocal's error
+      const int q = (int)((float)r * __builtin_amdgcn_rcpf((float)ny) + 0.01f);
       const int iy = (rr & 255) + (r - q * ny), iz = ((rr >> 16) & 255) + q;
       const int lp = loc[p];
       int ly = ((lp >> 10) & 1023) - iy, lz = (lp >> 20) - iz;
@@ -461,8 +491,7 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
         if (ix < x0 || ix >= x0 + nx) continue;
         int lx = lx0 - ix;
         lx += lx < 0 ? K : 0;
-        const long long v = __double2ll_rn((double)(th[p][ix] * wyz) * kQScale);
-        atomicAdd(rowp + lx, (unsigned long long)v);
+        atomicAdd(rowp + lx, to_fixed<real>(th[p][ix] * wyz));
       }
     }
     __syncthreads();
@@ -475,16 +504,20 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   }
 }
 
-template <typename real, int P>
-__global__ __launch_bounds__(256) void k_spread(SpreadArgs a) { spread_body<real, P>(a, blockIdx.x); }
+template <typename real, int P, bool OWN>
+__global__ __launch_bounds__(256) void k_spread(SpreadArgs a) { spread_body<real, P, OWN>(a, blockIdx.x); }
 
-template <typename real, int P>
+template <typename real, int P, bool OWN>
 __global__ __launch_bounds__(256) void k_spread_seg(const SpreadArgs* __restrict__ args,
                                                     const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
   if (s.x < 0) return;  // no work for this XCD slot
-  spread_body<real, P>(args[s.x], s.y);
+  spread_body<real, P, OWN>(args[s.x], s.y);
 }
+
+// Spread variant: the row -> particle table for meshes of at most 48 particles per brick on
+// average (few chunks per brick), the binary search for denser ones.
+bool spread_owner_table(const Ctx* c) { return c->np <= 48 * c->mesh.nbrick; }
 
 struct GatherArgs {
   MeshGrid m;
@@ -663,7 +696,10 @@ void launch_spread_gather(Ctx* c, bool spread) {
   if (spread) {
     const SpreadArgs a = spread_args(c);
     prof_begin(c, MSG_K_SPREAD);
-    hipLaunchKernelGGL((k_spread<real, P>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
+    if (spread_owner_table(c))
+      hipLaunchKernelGGL((k_spread<real, P, true>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL((k_spread<real, P, false>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
     prof_end(c, MSG_K_SPREAD);
   } else {
     const GatherArgs g = gather_args(c);
@@ -846,9 +882,10 @@ int add_fft_k(Ctx* c, SegBuilder<FftArgs>* b) {
   return MSG_OK;
 }
 
-template <typename real, int P>
+template <typename real, int P, bool OWN>
 void seg_spread(const SegGroup& g, hipStream_t s) {
-  hipLaunchKernelGGL((k_spread_seg<real, P>), dim3(g.nblk), dim3(256), 0, s, (const SpreadArgs*)g.d_args, g.d_map);
+  hipLaunchKernelGGL((k_spread_seg<real, P, OWN>), dim3(g.nblk), dim3(256), 0, s, (const SpreadArgs*)g.d_args,
+                     g.d_map);
 }
 template <typename real, int P>
 void seg_gather(const SegGroup& g, hipStream_t s) {
@@ -858,17 +895,22 @@ void seg_gather(const SegGroup& g, hipStream_t s) {
 template <typename real>
 int add_spread_gather(Ctx* c, SegBuilder<SpreadArgs>& bs, SegBuilder<GatherArgs>& bg) {
   const int key = (sizeof(real) == 8 ? 100 : 0) + c->mesh.order;
+  const bool own = spread_owner_table(c);
   SegLaunchFn fs, fg;
+#define MSG_SG(P)                                                  \
+  fs = own ? seg_spread<real, P, true> : seg_spread<real, P, false>; \
+  fg = seg_gather<real, P>;
   switch (c->mesh.order) {
-    case 3: fs = seg_spread<real, 3>; fg = seg_gather<real, 3>; break;
-    case 4: fs = seg_spread<real, 4>; fg = seg_gather<real, 4>; break;
-    case 5: fs = seg_spread<real, 5>; fg = seg_gather<real, 5>; break;
-    case 6: fs = seg_spread<real, 6>; fg = seg_gather<real, 6>; break;
-    case 7: fs = seg_spread<real, 7>; fg = seg_gather<real, 7>; break;
-    case 8: fs = seg_spread<real, 8>; fg = seg_gather<real, 8>; break;
+    case 3: MSG_SG(3) break;
+    case 4: MSG_SG(4) break;
+    case 5: MSG_SG(5) break;
+    case 6: MSG_SG(6) break;
+    case 7: MSG_SG(7) break;
+    case 8: MSG_SG(8) break;
     default: return set_error(MSG_ERR_ARG, "SPME order must be 3..8");
   }
-  bs.add(key, fs, spread_args(c), c->mesh.nbrick);
+#undef MSG_SG
+  bs.add(key + (own ? 1000 : 0), fs, spread_args(c), c->mesh.nbrick);
   bg.add(key, fg, gather_args(c), (8 * c->np + 255) / 256);
   return MSG_OK;
 }

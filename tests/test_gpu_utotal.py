@@ -28,8 +28,10 @@ def _small(**kw):
 @pytest.mark.parametrize("precision,ftol,etol", [("mixed", MIXED_FTOL, MIXED_ETOL),
                                                  ("fp64", FP64_TOL, FP64_TOL)])
 @pytest.mark.parametrize("case", ["cell_spme", "cell_spme_nosurf", "all_spme", "cell_k16",
-                                  "cell_order4", "three_types"])
+                                  "cell_order4", "three_types", "dense_bricks"])
 def test_utotal_small(case, precision, ftol, etol):
+    """dense_bricks: 78 particles per 16x8x8 mesh brick, so the spread takes its row search
+    variant (the others have at most 48 per brick and take the row table)."""
     kw = {}
     if case == "cell_spme_nosurf":
         kw = dict(lsurf=False)
@@ -41,7 +43,11 @@ def test_utotal_small(case, precision, ftol, etol):
         kw = dict(order=4, nmesh=64)
     elif case == "three_types":
         kw = dict(charges=(1.0, -2.0, 1.0), n=399)
+    elif case == "dense_bricks":
+        kw = dict(n=2500, L=60.0)
     sp = _small(**kw)
+    if case == "dense_bricks":
+        assert sp.n > 48 * (sp.nmesh // 16) * (sp.nmesh // 8) ** 2
     print(f"case={case} precision={precision} n={sp.n} K={sp.nmesh} order={sp.order}", flush=True)
     tab = oracle.build_table(sp.pot)
     ref = oracle_system(sp, tab).utotal()
