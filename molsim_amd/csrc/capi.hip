@@ -353,6 +353,9 @@ int utotal_sequence(Ctx* c) {
   c->cells_valid = cells_needed;
   if (branch) {
     MSG_HIP(hipEventRecord(c->ev_fork, c->stream));
+    // the mesh branch is the longer chain: recorded first, it is dispatched first
+    if ((e = launch_sort(c, 2))) return e;
+    if ((e = launch_spme_mesh(c))) return e;
     MSG_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     hipStream_t main = c->stream;
     c->stream = c->side;
@@ -362,8 +365,6 @@ int utotal_sequence(Ctx* c) {
     c->stream = main;
     if (e) return e;
     MSG_HIP(hipEventRecord(c->ev_join, c->side));
-    if ((e = launch_sort(c, 2))) return e;
-    if ((e = launch_spme_mesh(c))) return e;
     MSG_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     if ((e = launch_spme_gather(c))) return e;
     // (a second fork for the reduction beside the gather costs more than the reduction)
@@ -465,6 +466,7 @@ int msg_create(const msg_params* p, msg_ctx** out) {
   if ((e = lists_setup(c))) return fail(e);
   if (p->list_mode == MSG_LIST_VERLET && (e = verlet_setup(c))) return fail(e);
   // partial-sum layout
+  c->pair_lph = default_pair_tile(c);
   c->npart_pair = pair_blocks(c);
   c->npart_pair_cap = std::max(c->npart_pair, std::max((np + 3) / 4, 2048 + 1));
   c->npart_dip = (np + 255) / 256;

@@ -218,8 +218,9 @@ int launch_bin(Ctx* c);                            // K1 cell id + mesh bin + co
 int launch_sort(Ctx* c, int which);                // scan + scatter: 1 cells, 2 bricks + dipole, 3 both
 int launch_finalize(Ctx* c);                       // cell order (after launch_sort)
 int launch_pair(Ctx* c, bool last);                // K4
-int pair_blocks(Ctx* c);
-bool pair_nbpar(const Ctx* c);                           // workgroups of the pair kernel
+int pair_blocks(Ctx* c);                           // workgroups of the pair kernel
+bool pair_nbpar(const Ctx* c);                     // neighbour-parallel kernel in use
+int default_pair_tile(const Ctx* c);               // lanes per home for a new context
 int spme_setup(Ctx* c);                            // K-space arrays
 int launch_spme_mesh(Ctx* c);                      // spread, FFTs + influence function
 int launch_spme_gather(Ctx* c);                    // force gather (after the pair kernel)

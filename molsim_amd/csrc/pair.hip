@@ -633,6 +633,18 @@ bool pair_nbpar(const Ctx* c) {
   return c->P.list_mode != MSG_LIST_VERLET && c->pair_lph > 0;
 }
 
+// Default tile: about 16 candidates per lane, i.e. lanes per home = expected candidates
+// (particles in the stencil cells) / 16, rounded up to a power of two in 8..64.  Measured
+// on the 32 768-particle polyelectrolyte (~110 candidates per home), 8 lanes give the
+// shortest UTotal step; all-pairs systems of ~1000 particles take 64.  The autotuner
+// refines the tile per system size.
+int default_pair_tile(const Ctx* c) {
+  const double cand = (double)c->np * c->grid.nst / std::max(1, c->grid.ncell);
+  int lph = 8;
+  while (lph < 64 && lph * 16 < cand) lph *= 2;
+  return lph;
+}
+
 int pair_blocks(Ctx* c) {
   if (c->P.list_mode == MSG_LIST_VERLET) return (c->np + 255) / 256;
   if (pair_nbpar(c)) return (c->np + 256 / c->pair_lph - 1) / (256 / c->pair_lph);
