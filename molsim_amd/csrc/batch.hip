@@ -88,27 +88,36 @@ int launch_pass(const Pass& p, hipStream_t s, hipStream_t side, hipEvent_t fork,
   if (p.cs.empty()) return MSG_OK;
   if (int e = run_stage(p.bin, s)) return e;
   const bool branch = p.any_spme && !serial_pass();
-  hipStream_t cs = s;  // stream of the cell / pair stages
-  if (branch) {
-    MSG_HIP(hipEventRecord(fork, s));
-    MSG_HIP(hipStreamWaitEvent(side, fork, 0));
-    cs = side;
-  }
-  if (int e = run_stage(p.scan_c, cs)) return e;
-  if (int e = run_stage(p.scatter_c, cs)) return e;
-  if (int e = run_stage(p.finalize, cs)) return e;
-  if (int e = run_stage(p.pair, cs)) return e;
-  if (int e = run_stage(p.pair_sum, cs)) return e;
-  if (branch) MSG_HIP(hipEventRecord(join, side));
-  if (int e = run_stage(p.scan_b, s)) return e;
-  if (int e = run_stage(p.scatter_b, s)) return e;
-  if (p.any_spme) {
+  auto mesh = [&]() -> int {
+    if (int e = run_stage(p.scan_b, s)) return e;
+    if (int e = run_stage(p.scatter_b, s)) return e;
+    if (!p.any_spme) return MSG_OK;
     if (int e = run_stage(p.spread, s)) return e;
     for (int k = 0; k < 5; ++k)
       if (int e = run_stage(p.fft[k], s)) return e;
-    if (branch) MSG_HIP(hipStreamWaitEvent(s, join, 0));
-    if (int e = run_stage(p.gather, s)) return e;
+    return MSG_OK;
+  };
+  auto cells_pair = [&](hipStream_t cs) -> int {
+    if (int e = run_stage(p.scan_c, cs)) return e;
+    if (int e = run_stage(p.scatter_c, cs)) return e;
+    if (int e = run_stage(p.finalize, cs)) return e;
+    if (int e = run_stage(p.pair, cs)) return e;
+    return run_stage(p.pair_sum, cs);
+  };
+  if (branch) {
+    // the mesh branch is the longer chain: recorded first, it is dispatched first
+    MSG_HIP(hipEventRecord(fork, s));
+    if (int e = mesh()) return e;
+    MSG_HIP(hipStreamWaitEvent(side, fork, 0));
+    if (int e = cells_pair(side)) return e;
+    MSG_HIP(hipEventRecord(join, side));
+    MSG_HIP(hipStreamWaitEvent(s, join, 0));
+  } else {
+    if (int e = cells_pair(s)) return e;
+    if (int e = mesh()) return e;
   }
+  if (p.any_spme)
+    if (int e = run_stage(p.gather, s)) return e;
   if (int e = run_stage(p.reduce, s)) return e;
   hipLaunchKernelGGL(k_collect, dim3((unsigned)p.cs.size()), dim3(64), 0, s, p.d_outp, p.d_outs);
   MSG_HIP(hipGetLastError());
