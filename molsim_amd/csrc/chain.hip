@@ -70,7 +70,9 @@ __device__ double chain_random(int* st, double* am) {
   const int a = 16807, m = 2147483647, q = 127773, r = 2836;
   if (st[0] <= 0 || st[2] < 0) {
     const int s = st[0] < 0 ? -st[0] : st[0];
-    *am = (double)__int_as_float(0x3f7fffff) / (double)m;  // the single-precision real below 1
+    // the single-precision real below 1 divided by m in single precision, where m rounds
+    // to 2^31: (1 - 2^-24) 2^-31, exact
+    *am = (double)__int_as_float(0x3f7fffff) * 0x1p-31;
     st[2] = (888889999 ^ s) | 1;
     st[1] = 777755555 ^ s;
     st[0] = s + 1;

@@ -21,8 +21,9 @@ struct Generator {
     const int32_t a = 16807, m = 2147483647, q = 127773, r = 2836;
     if (*seed <= 0 || st[1] < 0) {
       const int32_t s = *seed < 0 ? -*seed : *seed;
-      // the default-kind (single precision) real just below 1, divided by m
-      *scale = (double)std::nextafter(1.0f, 0.0f) / (double)m;
+      // the default-kind (single precision) real just below 1 divided by m in single
+      // precision, where m rounds to 2^31: (1 - 2^-24) 2^-31, exact
+      *scale = (double)std::nextafter(1.0f, 0.0f) * 0x1p-31;
       st[1] = (888889999 ^ s) | 1;
       st[0] = 777755555 ^ s;
       *seed = s + 1;

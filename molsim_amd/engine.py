@@ -181,6 +181,19 @@ class Random:
     def __call__(self) -> float:
         return lib().msg_random(C.byref(self.iseed), self.state, C.byref(self.am))
 
+    @classmethod
+    def from_state(cls, iseed: int, am: float, ix: int, iy: int) -> "Random":
+        """Resume from a saved state (iseed, am, ix, iy), as a .cnf file carries it
+        (Src/molsim.F90:876,936)."""
+        g = cls(iseed)
+        g.am.value = am
+        g.state[:] = (ix, iy)
+        return g
+
+    def get_state(self) -> tuple[int, float, int, int]:
+        """(iseed, am, ix, iy), the values IOCnf writes."""
+        return self.iseed.value, self.am.value, int(self.state[0]), int(self.state[1])
+
 
 def _ptr(a):
     return a.ctypes.data_as(_vp)
@@ -205,7 +218,7 @@ class Engine:
         P.lsurf = int(spec.lsurf)
         P.alpha = spec.alpha
         P.ncut = spec.ncut
-        P.ncutregion = 0
+        P.ncutregion = {"sphere": 0, "cube": 1}[getattr(spec, "ncutregion", "sphere")]
         P.nmesh = spec.nmesh
         P.order = spec.order
         P.precision = PRECISIONS[precision]

@@ -126,6 +126,32 @@ class Server:
         self.bytes += held
         return eng
 
+    # -- system state in MOLSIM's configuration files ------------------------------------------
+    def ingest_cnf(self, sys_id: int, path, **read_kw):
+        """Replace a system's positions (and box) by a MOLSIM .cnf file (molsim_amd.cnf).  A
+        resident system with an unchanged box is updated in place; one whose box changed is
+        evicted, so its next request builds it afresh.  A system in flight cannot change."""
+        from . import cnf
+        if sys_id in self.busy:
+            raise RuntimeError(f"system {sys_id} is in flight")
+        spec = cnf.apply_to_spec(self.specs[sys_id], cnf.read_cnf(path, np_expected=self.specs[sys_id].n,
+                                                                    **read_kw))
+        same_box = np.array_equal(spec.box, self.specs[sys_id].box)
+        self.specs[sys_id] = spec
+        eng = self.resident.get(sys_id)
+        if eng is not None:
+            if same_box:
+                eng.spec = spec
+                eng.set_positions(spec.ro)
+            else:
+                self.resident.pop(sys_id).close()
+                self.bytes -= self.reserved.pop(sys_id)
+
+    def emit_cnf(self, sys_id: int, path, istep: int = 0, rng=None):
+        """Write a system's current state as a MOLSIM .cnf file."""
+        from . import cnf
+        cnf.write_cnf(path, cnf.from_spec(self.specs[sys_id], istep, rng))
+
     # -- scheduling ---------------------------------------------------------------------------
     def submit(self, req: Request):
         self.queue.append(req)

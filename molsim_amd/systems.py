@@ -51,6 +51,7 @@ class SystemSpec:
     ewald: str = "spm"          # "none" | "std" | "spm"
     alpha: float = 0.0
     ncut: int = 0
+    ncutregion: str = "sphere"  # std Ewald k-vectors: "sphere" n^2 <= ncut^2, "cube" n^2 <= 3 ncut^2
     nmesh: int = 0
     order: int = 0
     lsurf: bool = True

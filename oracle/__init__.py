@@ -7,8 +7,9 @@ product (``molsim_amd``) never imports it.
 
 Pinning status of each function (how it is checked; see tests/test_oracle_*.py):
 
-* bspline        -- partition of unity, symmetry, closed-form cubic, finite-difference
-                    derivatives.
+* bspline        -- bit for bit against the reference's compiled CardinalBSpline
+                    (oracle/_ref, tests/test_oracle_ref_mollib.py); partition of unity,
+                    symmetry, closed-form cubic, finite-difference derivatives.
 * build_table    -- tabulated U and F against the analytic potential within utoltab/ftoltab
                     (the reference's TestUBuffer semantics) plus C0/C1 continuity at joints.
 * cell lists     -- brute-force O(N^2) enumeration (the reference's TestCellList check).
@@ -19,8 +20,15 @@ Pinning status of each function (how it is checked; see tests/test_oracle_*.py):
 * utotal forces  -- finite differences of the oracle's own energy, translation invariance.
 * dutotal        -- identity dU = U(new) - U(old) from two full evaluations (the reference's
                     drift check).
-* random         -- parity unpinned beyond structural checks (range, determinism, seeding):
-                    the reference ships no value table for its generator.
+* random         -- bit for bit against the reference's compiled Random (oracle/_ref: 10^5
+                    draws and the end state for several seeds; committed vectors in
+                    tests/golden/ref_mollib_vectors.npz).
+* std Ewald cube -- k-vector counts of both regions by enumeration; cube and sphere agree
+                    once both hold the converged sum.
+
+oracle/_ref/libref_mollib.so (``make -C oracle ref``) is the reference's Src/mollib.F90
+compiled where it lies with amdflang, behind the wrapper ref_mollib.F90.  No other file of
+the reference's path builds here (Src/mol.F90 fails to compile with amdflang).
 """
 from __future__ import annotations
 

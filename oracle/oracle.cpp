@@ -88,8 +88,9 @@ double orc_random(int32_t* idum, int32_t* state, double* am) {
   int32_t& ix = state[0];
   int32_t& iy = state[1];
   if (*idum <= 0 || iy < 0) {
-    // nearest(1.0,-1.0) of a default (4-byte) real is 1 - 2^-24
-    *am = (double)(1.0f - 5.9604644775390625e-08f) / (double)im;
+    // am = nearest(1.0,-1.0)/im is evaluated in default (4-byte) real: 1 - 2^-24 divided by
+    // im converted to real(4), which rounds to 2^31; the quotient (1 - 2^-24) 2^-31 is exact
+    *am = (double)(1.0f - 5.9604644775390625e-08f) * 0x1p-31;
     int32_t a = *idum < 0 ? -*idum : *idum;
     iy = ((int32_t)888889999 ^ a) | 1;
     ix = (int32_t)777755555 ^ a;

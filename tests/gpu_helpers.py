@@ -22,7 +22,8 @@ def engine(sp, tab, precision="mixed"):
 def oracle_system(sp, tab):
     return oracle.System(sp.ro, sp.ipt, sp.az, sp.box, sp.rcut, sp.pot["epsi4pi"], tab, npt=sp.npt,
                          list_mode=LIST[sp.list_mode], drnlist=sp.drnlist, ewald=EWALD[sp.ewald],
-                         surf=sp.lsurf, alpha=sp.alpha, ncut=sp.ncut, K=sp.nmesh, order=sp.order)
+                         surf=sp.lsurf, alpha=sp.alpha, ncut=sp.ncut,
+                         region={"sphere": 0, "cube": 1}[sp.ncutregion], K=sp.nmesh, order=sp.order)
 
 
 def rel_force_err(f, ref):

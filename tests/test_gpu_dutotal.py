@@ -27,11 +27,14 @@ def _trials(sp, n, seed, dmax=1.0):
 
 
 @pytest.mark.parametrize("precision,atol,rtol", [("mixed", 2e-5, 1e-6), ("fp64", 1e-8, 0.0)])
-@pytest.mark.parametrize("ewald", ["spm", "std"])
+@pytest.mark.parametrize("ewald", ["spm", "std", "std_cube"])
 def test_dutotal_trials(ewald, precision, atol, rtol):
+    """std_cube: the k-vectors of ncutregion = 'cube' (n^2 <= 3 ncut^2, Src/energy.F90:6754-6760)."""
     sp = systems.small_spme_system(n=300, L=40.0, nmesh=32, alpha=0.3, rcut=12.0)
     if ewald == "std":
         sp = systems.SystemSpec(**{**sp.__dict__, "ewald": "std", "ncut": 6})
+    elif ewald == "std_cube":
+        sp = systems.SystemSpec(**{**sp.__dict__, "ewald": "std", "ncut": 5, "ncutregion": "cube"})
     print(f"dutotal ewald={ewald} precision={precision} n={sp.n}", flush=True)
     tab = oracle.build_table(sp.pot)
     osys = oracle_system(sp, tab)
@@ -87,11 +90,14 @@ def test_mc_commit_sequence():
     eng.close()
 
 
+@pytest.mark.parametrize("region", ["sphere", "cube"])
 @pytest.mark.parametrize("precision,ftol,etol", [("mixed", 1e-5, 1e-5), ("fp64", 1e-9, 1e-9)])
-def test_std_ewald_electrolyte(precision, ftol, etol):
-    """The 1000-ion primitive-model electrolyte: std Ewald (|n| <= 10), rc = L/2, all pairs."""
+def test_std_ewald_electrolyte(precision, ftol, etol, region):
+    """The 1000-ion primitive-model electrolyte: std Ewald (|n| <= 10), rc = L/2, all pairs;
+    k-vectors in the sphere n^2 <= 100 or the cube n^2 <= 300 (1330 octant vectors)."""
     sp = systems.electrolyte_primitive()
-    print(f"electrolyte n={sp.n} precision={precision}", flush=True)
+    sp = systems.SystemSpec(**{**sp.__dict__, "ncutregion": region})
+    print(f"electrolyte n={sp.n} precision={precision} region={region}", flush=True)
     tab = oracle.build_table(sp.pot)
     ref = oracle_system(sp, tab).utotal()
     eng = engine(sp, tab, precision)

@@ -84,3 +84,34 @@ def test_mc_pass_matches_oracle_loop(ewald, precision):
         assert agree >= 0.99
     assert 0 < nacc < ntrial
     eng.close()
+
+
+@pytest.mark.parametrize("ewald", ["std", "spm"])
+def test_mc_chain_resumes_from_cnf(ewald, tmp_path):
+    """A chain interrupted after 150 trials, saved as a MOLSIM .cnf (positions and the
+    generator state iseed/am/ix/iy, Src/molsim.F90:936-938) and resumed in a fresh context,
+    makes the same decisions and ends at the same positions as the uninterrupted chain."""
+    from molsim_amd import cnf
+    sp = systems.small_spme_system(n=200, L=36.0, nmesh=32, alpha=0.3, rcut=11.0)
+    if ewald == "std":
+        sp = systems.SystemSpec(**{**sp.__dict__, "ewald": "std", "ncut": 6, "list_mode": "cell"})
+    tab = cabi_table(oracle.build_table(sp.pot))
+    dtran, beta = 2.0, systems.beta_kjmol()
+    print(f"mc resume ewald={ewald} n={sp.n}", flush=True)
+    a = Engine(sp, precision="fp64", table=tab)
+    ev_all = a.mc_pass(300, dtran, beta, Random(-4711), isamp=1)[2]
+    ro_all = a.positions()
+    a.close()
+    b = Engine(sp, precision="fp64", table=tab)
+    g = Random(-4711)
+    ev1 = b.mc_pass(150, dtran, beta, g, isamp=1)[2]
+    path = tmp_path / "mid.cnf"
+    cnf.write_cnf(path, cnf.from_engine(b, istep=1, rng=g))
+    b.close()
+    c = cnf.read_cnf(path)
+    sp2 = cnf.apply_to_spec(sp, c)
+    e = Engine(sp2, precision="fp64", table=tab)
+    ev2 = e.mc_pass(150, dtran, beta, Random.from_state(*c.rng_state()), isamp=1)[2]
+    assert np.array_equal(np.concatenate([ev1, ev2]), ev_all)
+    assert np.abs(e.positions() - ro_all).max() < 1e-9
+    e.close()

@@ -37,6 +37,29 @@ def test_std_ewald_madelung_nacl(orc):
     assert np.abs(r["force"]).max() < 1e-5  # every ion sits at a symmetric site
 
 
+def test_std_ewald_cube_region(orc):
+    """ncutregion = 'cube' (n^2 <= 3 ncut^2, Src/energy.F90:6754-6760) takes every octant
+    vector of the cube; a cube and a sphere that both hold the converged sum agree, and the
+    cube's sum over the vectors outside the inscribed sphere is what separates the two at a
+    small cut-off."""
+    for ncut in (1, 3, 6):
+        kf = orc.ewald_std_kfac([10.0, 11.0, 12.0], 0.3, ncut, region=1)
+        assert kf.size == (ncut + 1) ** 3 - 1
+        assert orc.ewald_std_kfac([10.0, 11.0, 12.0], 0.3, ncut, region=0).size == sum(
+            1 for x in range(ncut + 1) for y in range(ncut + 1) for z in range(ncut + 1)
+            if 0 < x * x + y * y + z * z <= ncut * ncut)
+    sp = S.small_spme_system(n=60, L=24.0, alpha=0.35, rcut=10.0, form=0)
+    tab = orc.build_table(dict(sp.pot, utoltab=1e-9, ftoltab=1e-9))
+    common = dict(npt=2, list_mode=0, surf=True, alpha=sp.alpha, ewald=1)
+    run = lambda ncut, region: orc.System(sp.ro, sp.ipt, sp.az, sp.box, sp.rcut, sp.pot["epsi4pi"], tab,
+                                          ncut=ncut, region=region, **common).utotal()
+    cube, sphere = run(14, 1), run(25, 0)        # both hold every term above exp(-27)
+    assert abs(cube["u_rec"] - sphere["u_rec"]) < 1e-10 * abs(sphere["u_rec"])
+    assert np.allclose(cube["force"], sphere["force"], atol=1e-9 * np.abs(sphere["force"]).max())
+    small_c, small_s = run(2, 1), run(2, 0)
+    assert abs(small_c["u_rec"] - small_s["u_rec"]) > 1e-6 * abs(small_s["u_rec"])
+
+
 def test_spme_converges_to_std_ewald(orc):
     """The reference's TestEwaldSPM recipe: high-order fine-mesh SPME equals std Ewald."""
     sp = S.small_spme_system(n=60, L=24.0, alpha=0.35, rcut=10.0, form=0)
