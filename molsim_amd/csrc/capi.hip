@@ -5,6 +5,7 @@
 #include <mutex>
 
 #include "common.h"
+#include "reduce.h"
 
 namespace msg {
 
@@ -74,53 +75,6 @@ __global__ void k_zero(int* a, int na, int* b, int nb, int* flags) {
   if (i == 0) flags[0] = 0;
 }
 
-struct ReduceArgs {
-  const double* pair;  int npair; int ntp;   // npair rows of ntp energies + virial (1: pre-summed)
-  const double* spme;  int nspme;
-  const double* ewstd; int newstd;
-  const double* M;     // total dipole sum q r, formed by k_scan
-  int ewald;           // 0 none, 1 std, 2 spm
-  int lsurf;
-  double self_raw;     // -alpha/sqrt(pi) sum q^2
-  double surf_fac;     // 2 pi / (3 V)
-  double epsi4pi;
-  double* out;
-  int* flags;
-  int* cell_count;
-  int ncell;
-  int* brick_count;
-  int nbrick;
-};
-
-// Fixed-order column totals of the pair kernel's rows (np1 columns: energies by type pair +
-// virial, one row per workgroup) by one 1024-thread workgroup.  Thread t owns column
-// t mod np1 and adds rows t div np1, t div np1 + R, ... (R = 1024 div np1 threads per column,
-// four loads in flight); the R partial sums of a column are then added in a fixed binary tree
-// in LDS.  The totals end in red[0 .. np1).  With one row the totals are the row itself.
-__device__ __forceinline__ void pair_rows_total(const double* __restrict__ rows, int nrow, int np1,
-                                                double* red) {
-  const int R = 1024 / np1;
-  const int c = threadIdx.x % np1, r0 = threadIdx.x / np1;
-  double s = 0.0;
-  if (r0 < R) {
-    for (int r = r0; r < nrow; r += 4 * R) {
-      double x[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) x[u] = r + u * R < nrow ? rows[(size_t)(r + u * R) * np1 + c] : 0.0;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) s += x[u];
-    }
-  }
-  red[threadIdx.x] = s;
-  __syncthreads();
-  int p2 = 1;
-  while (p2 < R) p2 <<= 1;
-  for (int st = p2 >> 1; st > 0; st >>= 1) {
-    if (r0 < st && r0 + st < R) red[threadIdx.x] += red[threadIdx.x + st * np1];
-    __syncthreads();
-  }
-}
-
 struct PairSumArgs {
   const double* rows;
   int nrow, np1;
@@ -149,99 +103,13 @@ void seg_pair_sum(const SegGroup& g, hipStream_t s) {
   hipLaunchKernelGGL(k_pair_sum_seg, dim3(g.nblk), dim3(1024), 0, s, (const PairSumArgs*)g.d_args, g.d_map);
 }
 
-// Final FP64 reduction of all partial sums in a fixed order (deterministic).  It is the last
-// kernel of a UTotal pass, so it also hands the error flags to the output block and clears
-// the cell/brick counters and flags for the next pass.  The k-space partials, dipole, flags
-// and a pre-summed pair row are loaded together at the start.
-__device__ __forceinline__ void reduce_body(const ReduceArgs& a) {
-  __shared__ double red[1024];
-  __shared__ double wred[16][2];
-  const int tid = threadIdx.x;
-  const double* ew = a.ewald == 2 ? a.spme : a.ewstd;
-  const int nev = a.ewald == 2 ? a.nspme : (a.ewald == 1 ? a.newstd : 0);
-  const int np1 = a.ntp + 1;
-  double e0 = 0.0, e1 = 0.0;
-  if (tid < nev) {
-    e0 = ew[2 * tid];
-    e1 = ew[2 * tid + 1];
-  }
-  double m[3] = {0.0, 0.0, 0.0};
-  double pt[kMaxTP + 1];
-#pragma unroll
-  for (int k = 0; k <= kMaxTP; ++k) pt[k] = 0.0;
-  int fl = 0;
-  if (tid == 0) {
-    m[0] = a.M[0];
-    m[1] = a.M[1];
-    m[2] = a.M[2];
-    fl = a.flags[0];
-    if (a.npair == 1)
-#pragma unroll
-      for (int k = 0; k <= kMaxTP; ++k) pt[k] = k < np1 ? a.pair[k] : 0.0;
-  }
-  for (int b = tid + 1024; b < nev; b += 1024) {
-    e0 += ew[2 * b];
-    e1 += ew[2 * b + 1];
-  }
-  if (a.npair != 1) {
-    pair_rows_total(a.pair, a.npair, np1, red);
-    if (tid == 0)
-#pragma unroll
-      for (int k = 0; k <= kMaxTP; ++k) pt[k] = k < np1 ? red[k] : 0.0;
-  }
-  e0 = msgd::wave_sum(e0);
-  e1 = msgd::wave_sum(e1);
-  if ((tid & 63) == 0) {
-    wred[tid >> 6][0] = e0;
-    wred[tid >> 6][1] = e1;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    double es = 0.0, vs = 0.0;
-#pragma unroll
-    for (int w = 0; w < 16; ++w) {
-      es += wred[w][0];
-      vs += wred[w][1];
-    }
-    double* o = a.out;
-    double tw = 0.0;
-    for (int tp = 0; tp < a.ntp; ++tp) {
-      o[1 + tp] = 0.5 * pt[tp];  // every pair was evaluated from both of its particles
-      tw += 0.5 * pt[tp];
-    }
-    o[0] = tw;
-    double urec = 0.0, virrec = 0.0;
-    if (a.ewald) {
-      // SPME: E = sum w G |F|^2, virrec -= sum w G (1 + 2 m^2 expf) |F|^2
-      // std:  E = sum kfac |S|^2, virrec -= sum (1 - k^2/(2 alpha^2)) kfac |S|^2
-      urec = es + a.self_raw;
-      virrec = -vs;
-      if (a.lsurf) {
-        const double term = a.surf_fac * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
-        urec += term;
-        virrec -= term;
-      }
-    }
-    o[a.ntp + 1] = a.epsi4pi * urec;
-    o[a.ntp + 2] = tw + a.epsi4pi * urec;
-    o[a.ntp + 3] = 0.5 * pt[a.ntp] + a.epsi4pi * virrec;
-    o[a.ntp + 4] = m[0];
-    o[a.ntp + 5] = m[1];
-    o[a.ntp + 6] = m[2];
-    o[a.ntp + 7] = (double)fl;
-    a.flags[0] = 0;
-  }
-  for (int i = tid; i < a.ncell; i += 1024) a.cell_count[i] = 0;
-  for (int i = tid; i < a.nbrick; i += 1024) a.brick_count[i] = 0;
-}
-
-__global__ __launch_bounds__(1024) void k_reduce(ReduceArgs a) { reduce_body(a); }
+__global__ __launch_bounds__(1024) void k_reduce(ReduceArgs a) { reduce_body<1024>(a); }
 
 __global__ __launch_bounds__(1024) void k_reduce_seg(const ReduceArgs* __restrict__ args,
                                                      const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
   if (s.x < 0) return;  // no work for this XCD slot
-  reduce_body(args[s.x]);
+  reduce_body<1024>(args[s.x]);
 }
 
 void seg_reduce(const SegGroup& g, hipStream_t s) {
@@ -260,8 +128,7 @@ int launch_zero(Ctx* c) {
   return MSG_OK;
 }
 
-// presum: the pair rows were summed into one row by k_pair_sum
-static ReduceArgs reduce_args(Ctx* c, bool presum) {
+ReduceArgs reduce_args(Ctx* c, bool presum) {
   ReduceArgs a;
   a.pair = c->d_part + (presum ? c->part_off_pairsum : c->part_off_pair);
   a.npair = presum ? 1 : c->npart_pair;
@@ -366,10 +233,8 @@ int utotal_sequence(Ctx* c) {
     if (e) return e;
     MSG_HIP(hipEventRecord(c->ev_join, c->side));
     MSG_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
-    if ((e = launch_spme_gather(c))) return e;
-    // (a second fork for the reduction beside the gather costs more than the reduction)
-    if ((e = launch_reduce(c, true))) return e;
-    return MSG_OK;
+    // the final reduction rides as one extra workgroup of the gather launch
+    return launch_spme_gather(c, true);
   } else {
     if ((e = cell_work())) return e;
     if (c->P.ewald_mode == MSG_EWALD_SPM) {
@@ -504,7 +369,7 @@ void msg_destroy(msg_ctx* h) {
                   c->d_force, c->d_part, c->d_out, c->d_flags, c->d_de, c->d_de_ip, c->d_de_r,
                   c->d_tcoef64, c->d_gk, c->d_item_cell, c->d_bq, c->d_bmm, c->d_M,
                   c->d_vl_home, c->d_vl_n, c->d_vl_nbr, c->d_ro_ref, c->d_dmax, c->d_vel,
-                  c->d_acc, c->d_minv, c->d_ke_part};
+                  c->d_acc, c->d_minv, c->d_ke_part, c->d_rank};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (c->h_out) hipHostFree(c->h_out);

@@ -117,6 +117,7 @@ struct Ctx {
   int* d_item_cell = nullptr;    // pair-kernel work item -> cell
   double* d_M = nullptr;         // total dipole sum q r of the current pass (3)
   int* d_cell_tmp = nullptr;     // np (unsorted members)
+  int* d_rank = nullptr;         // 2*np: place of each particle in its cell, then in its brick
   int* d_cell_ip = nullptr;      // np (sorted: cell-major, descending ip in a cell)
   int* d_slot_of = nullptr;      // np: slot of particle ip in the cell order
   void* d_cpos = nullptr;        // np double4 in cell order: x, y, z, particle type
@@ -223,7 +224,7 @@ bool pair_nbpar(const Ctx* c);                     // neighbour-parallel kernel 
 int default_pair_tile(const Ctx* c);               // lanes per home for a new context
 int spme_setup(Ctx* c);                            // K-space arrays
 int launch_spme_mesh(Ctx* c);                      // spread, FFTs + influence function
-int launch_spme_gather(Ctx* c);                    // force gather (after the pair kernel)
+int launch_spme_gather(Ctx* c, bool with_reduce = false);                    // force gather (after the pair kernel)
 int ewstd_setup(Ctx* c);
 int launch_ewstd(Ctx* c);
 int launch_reduce(Ctx* c, bool presum);           // final energies (presum: pair rows pre-summed)

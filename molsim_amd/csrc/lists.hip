@@ -36,6 +36,8 @@ struct BinArgs {
   int* brick_count;
   double* dip_part;
   int* flags;
+  int* rank;        // 2*np: each particle's place in its cell, then in its brick, as the
+                    // counters hand them out (the scatter needs no counters of its own)
 };
 
 template <typename real>
@@ -57,7 +59,7 @@ __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
     if (bad) atomicOr(a.flags, FLAG_CELL);
     const int c = ci[0] + a.g.nc[0] * (ci[1] + a.g.nc[1] * ci[2]);
     a.cellid[i] = c;
-    atomicAdd(&a.cell_count[c], 1);
+    a.rank[i] = atomicAdd(&a.cell_count[c], 1);
     if (a.use_mesh) {
       real* frac = (real*)a.frac;
       int mi[3];
@@ -71,7 +73,7 @@ __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
       }
       a.mm[i] = mi[0] | (mi[1] << 10) | (mi[2] << 20);
       const int b = mi[0] / kBX + a.m.nb[0] * (mi[1] / kBY + a.m.nb[1] * (mi[2] / kBZ));
-      atomicAdd(&a.brick_count[b], 1);
+      a.rank[a.np + i] = atomicAdd(&a.brick_count[b], 1);
     }
     dx = q * r[0];
     dy = q * r[1];
@@ -98,21 +100,22 @@ __global__ __launch_bounds__(256) void k_bin_seg(const BinArgs* __restrict__ arg
   bin_body<real>(args[s.x], s.y);
 }
 
-// Exclusive scans of cnt[0..n) and of ceil(cnt / DIV) (out2, may be null) by one 1024-thread
-// workgroup, in tiles of 1024 * V values.  Thread t takes the values t, t + 1024, ... of a
-// tile (coalesced loads and stores), the wavefronts scan the V rows with shuffles, one
-// barrier exchanges the wavefront totals, and the running totals carry into the next tile.
-// out1[n] / out2[n] get the totals.
-template <int V, int DIV>
+// Exclusive scans of cnt[0..n) and of ceil(cnt / DIV) (out2, may be null) by one workgroup of
+// NT threads, in tiles of NT * V values.  Thread t takes the values t, t + NT, ... of a tile
+// (coalesced loads and stores), the wavefronts scan the V rows with shuffles, one barrier
+// exchanges the wavefront totals, and the running totals carry into the next tile.
+// out1[n] / out2[n] get the totals.  out1 / out2 may point to LDS or global memory.
+template <int NT, int V, int DIV>
 __device__ void block_scan_tiled(int n, const int* __restrict__ cnt, int* out1, int* out2,
-                                 int (*wsum)[16]) {
+                                 int (*wsum)[NT / 64]) {
+  constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int c1 = 0, c2 = 0;
-  for (int base = 0; base < n; base += 1024 * V) {
+  for (int base = 0; base < n; base += NT * V) {
     int v[V], i1[V], i2[V];
 #pragma unroll
     for (int u = 0; u < V; ++u) {
-      const int idx = base + u * 1024 + threadIdx.x;
+      const int idx = base + u * NT + threadIdx.x;
       v[u] = idx < n ? cnt[idx] : 0;
       i1[u] = v[u];
       i2[u] = (v[u] + DIV - 1) / DIV;
@@ -139,7 +142,7 @@ __device__ void block_scan_tiled(int n, const int* __restrict__ cnt, int* out1, 
     for (int u = 0; u < V; ++u) {
       int t1 = 0, t2 = 0, p1 = 0, p2 = 0;
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
+      for (int k = 0; k < NW; ++k) {
         const int x = wsum[u][k], y = wsum[V + u][k];
         t1 += x;
         t2 += y;
@@ -148,7 +151,7 @@ __device__ void block_scan_tiled(int n, const int* __restrict__ cnt, int* out1, 
           p2 += y;
         }
       }
-      const int idx = base + u * 1024 + threadIdx.x;
+      const int idx = base + u * NT + threadIdx.x;
       if (idx < n) {
         out1[idx] = b1 + p1 + i1[u] - v[u];
         if (out2) out2[idx] = b2 + p2 + i2[u] - (v[u] + DIV - 1) / DIV;
@@ -168,6 +171,35 @@ __device__ void block_scan_tiled(int n, const int* __restrict__ cnt, int* out1, 
 }
 
 constexpr int kScanV = 8;
+
+// Total dipole M = sum of k_bin's per-block partials, in the order of a 1024-thread
+// reduction whatever the workgroup size NT (virtual thread v = t + NT*k sums partials v,
+// v + 1024, ...; the 16 virtual wavefront totals are added in order), so the single-system
+// and the batched passes give the same bits.
+template <int NT>
+__device__ void dipole_sum(const double* __restrict__ part, int n, double* M, double (*dsum)[3]) {
+  constexpr int R = 1024 / NT, NW = NT / 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    double m[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x + NT * k; b < n; b += 1024)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) m[d] += part[3 * b + d];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double t = msgd::wave_sum(m[d]);
+      if (lane == 0) dsum[w + NW * k][d] = t;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double t = 0.0;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) t += dsum[v][threadIdx.x];
+    M[threadIdx.x] = t;
+  }
+}
 
 struct ScanArgs {
   int bid0;               // role of workgroup 0 of this launch (0 cells, 1 bricks, 2 dipole)
@@ -194,30 +226,15 @@ __device__ __forceinline__ void scan_body(const ScanArgs& a, int bid) {
   __shared__ double dsum[16][3];
   bid += a.bid0;
   if (bid == 2) {  // total dipole sum q r (surface term), from k_bin's partials
-    double m[3] = {0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < a.ndip; b += 1024)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) m[d] += a.dip_part[3 * b + d];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double t = msgd::wave_sum(m[d]);
-      if ((threadIdx.x & 63) == 0) dsum[threadIdx.x >> 6][d] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-      double t = 0.0;
-#pragma unroll
-      for (int w = 0; w < 16; ++w) t += dsum[w][threadIdx.x];
-      a.M[threadIdx.x] = t;
-    }
+    dipole_sum<1024>(a.dip_part, a.ndip, a.M, dsum);
     return;
   }
   const int n = bid == 0 ? a.n0 : a.n1;
   const int* cnt = bid == 0 ? a.cnt0 : a.cnt1;
   int* fill = bid == 0 ? a.fill0 : a.fill1;
   if (n <= 0) return;
-  block_scan_tiled<kScanV, kPairHB>(n, cnt, bid == 0 ? a.start0 : a.start1, bid == 0 ? a.chunk0 : nullptr,
-                                    wsum);
+  block_scan_tiled<1024, kScanV, kPairHB>(n, cnt, bid == 0 ? a.start0 : a.start1, bid == 0 ? a.chunk0 : nullptr,
+                                          wsum);
   for (int i = threadIdx.x; i < n; i += 1024) fill[i] = 0;
 }
 
@@ -287,6 +304,61 @@ __global__ __launch_bounds__(256) void k_scatter_seg(const ScatterArgs* __restri
   const int2 s = map[blockIdx.x];
   if (s.x < 0) return;  // no work for this XCD slot
   scatter_body<real>(args[s.x], s.y);
+}
+
+// Scatter with the offsets scanned by the workgroup itself, for count arrays that fit in
+// LDS: every workgroup scans the counts k_bin left (cells or bricks) into LDS; workgroup 0
+// also publishes them (with the pair kernel's home-chunk offsets for cells, the total dipole
+// for bricks); a particle's slot is its group's offset plus the rank k_bin's counter gave it.
+// This replaces a separate scan launch and the scatter's own counters.
+constexpr int kLsMax = 8192;  // counts one workgroup scans in LDS
+constexpr int kLsV = 8;
+
+struct ScatterLsArgs {
+  ScatterArgs s;
+  int role;            // 1: cells, 2: bricks
+  int n;               // ncell or nbrick (<= kLsMax)
+  const int* cnt;
+  int* start;          // offsets, n + 1 (written by workgroup 0)
+  int* chunk;          // cells: home-chunk offsets, n + 1 (written by workgroup 0)
+  const int* rank;     // per particle (cell ranks, or brick ranks)
+  const double* dip_part;
+  int ndip;
+  double* M;           // total dipole, formed by workgroup 0 when dipole is set
+  int dipole;
+};
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_scatter_ls(ScatterLsArgs a) {
+  __shared__ int s_start[kLsMax + 1];
+  __shared__ int wsum[2 * kLsV][4];
+  __shared__ double dsum[16][3];
+  const bool first = blockIdx.x == 0;
+  block_scan_tiled<256, kLsV, kPairHB>(a.n, a.cnt, s_start, (first && a.role == 1) ? a.chunk : nullptr, wsum);
+  if (first) {
+    for (int k = threadIdx.x; k <= a.n; k += 256) a.start[k] = s_start[k];
+    if (a.dipole) dipole_sum<256>(a.dip_part, a.ndip, a.M, dsum);
+  }
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.s.np) return;
+  const int rk = a.rank[i];
+  if (a.role == 1) {
+    a.s.cell_tmp[s_start[a.s.cellid[i]] + rk] = i;
+  } else {
+    const int p = a.s.mm[i];
+    const int b = (p & 1023) / kBX + a.s.m.nb[0] * (((p >> 10) & 1023) / kBY + a.s.m.nb[1] * ((p >> 20) / kBZ));
+    const int dst = s_start[b] + rk;
+    typedef typename std::conditional<sizeof(real) == 4, float4, double4>::type R4;
+    const real* fr = (const real*)a.s.frac;
+    R4 v;
+    v.x = fr[3 * i];
+    v.y = fr[3 * i + 1];
+    v.z = fr[3 * i + 2];
+    v.w = (real)a.s.az[i];
+    ((R4*)a.s.bq)[dst] = v;
+    a.s.bmm[dst] = p;
+    a.s.brick_ip[dst] = i;
+  }
 }
 
 struct FinalizeArgs {
@@ -406,6 +478,7 @@ int lists_setup(Ctx* c) {
   if (int e = A(&c->d_cpos, np * sizeof(double4))) return e;
   if (int e = A((void**)&c->d_ctype, np * sizeof(int))) return e;
   if (int e = A((void**)&c->d_slot_cell, np * sizeof(int))) return e;
+  if (int e = A((void**)&c->d_rank, 2 * (size_t)np * sizeof(int))) return e;
   return MSG_OK;
 }
 
@@ -435,6 +508,7 @@ static BinArgs bin_args(Ctx* c) {
   a.brick_count = c->d_brick_count;
   a.dip_part = c->d_part + c->part_off_dip;
   a.flags = c->d_flags;
+  a.rank = c->d_rank;
   return a;
 }
 
@@ -514,7 +588,42 @@ int launch_bin(Ctx* c) {
 // Scan and scatter.  which = 1: cell slots (the cell order is completed by launch_finalize);
 // 2: brick slots, brick-ordered spline data and the total dipole (the mesh branch may start
 // after these); 3: both.
+// Fused scatter (offsets scanned per workgroup) for one role, when its counts fit in LDS.
+static bool sort_fused(const Ctx* c, int role) {
+  return role == 1 ? c->grid.ncell <= kLsMax : c->mesh.nbrick <= kLsMax;
+}
+
+static void launch_scatter_ls(Ctx* c, int role, bool dipole) {
+  ScatterLsArgs a;
+  a.s = scatter_args(c, role);
+  a.role = role;
+  a.n = role == 1 ? c->grid.ncell : c->mesh.nbrick;
+  a.cnt = role == 1 ? c->d_cell_count : c->d_brick_count;
+  a.start = role == 1 ? c->d_cell_start : c->d_brick_start;
+  a.chunk = c->d_cell_chunk;
+  a.rank = c->d_rank + (role == 1 ? 0 : c->np);
+  a.dip_part = c->d_part + c->part_off_dip;
+  a.ndip = c->npart_dip;
+  a.M = c->d_M;
+  a.dipole = dipole;
+  const int nb = (c->np + 255) / 256;
+  if (c->fp64)
+    hipLaunchKernelGGL(k_scatter_ls<double>, dim3(nb), dim3(256), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_scatter_ls<float>, dim3(nb), dim3(256), 0, c->stream, a);
+}
+
 int launch_sort(Ctx* c, int which) {
+  const bool mesh = c->P.ewald_mode == MSG_EWALD_SPM;
+  if ((!(which & 1) || sort_fused(c, 1)) && (!(which & 2) || !mesh || sort_fused(c, 2))) {
+    prof_begin(c, MSG_K_SCATTER);
+    // the total dipole goes with the brick role; without a mesh, with the cell role
+    if (which & 1) launch_scatter_ls(c, 1, (which & 2) && !mesh);
+    if ((which & 2) && mesh) launch_scatter_ls(c, 2, true);
+    prof_end(c, MSG_K_SCATTER);
+    MSG_HIP(hipGetLastError());
+    return MSG_OK;
+  }
   const int np = c->np;
   const int bid0 = which == 2 ? 1 : 0;
   const int nscan = which == 1 ? 1 : (which == 2 ? 2 : 3);

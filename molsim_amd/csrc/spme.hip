@@ -23,6 +23,7 @@
 //               force unit scale.
 // FFT conventions are FFTW's: forward e^{-i}, backward e^{+i}, no normalisation.
 #include "common.h"
+#include "reduce.h"
 
 namespace msg {
 
@@ -608,6 +609,18 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int bid) {
 template <typename real, int P>
 __global__ __launch_bounds__(256) void k_gather(GatherArgs a) { gather_body<real, P>(a, blockIdx.x); }
 
+// The gather with the pass's final reduction as one extra (last) workgroup.  The reduction
+// reads only what earlier kernels of the pass wrote (k-space partials, the pre-summed pair
+// row, the dipole, the flags), so it needs no ordering against the gather's workgroups, and
+// the pass saves one dependent launch.
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_gather_reduce(GatherArgs a, ReduceArgs r) {
+  if (blockIdx.x == gridDim.x - 1)
+    reduce_body<256>(r);
+  else
+    gather_body<real, P>(a, blockIdx.x);
+}
+
 template <typename real, int P>
 __global__ __launch_bounds__(256) void k_gather_seg(const GatherArgs* __restrict__ args,
                                                     const int2* __restrict__ map) {
@@ -692,7 +705,7 @@ GatherArgs gather_args(Ctx* c) {
 }
 
 template <typename real, int P>
-void launch_spread_gather(Ctx* c, bool spread) {
+void launch_spread_gather(Ctx* c, bool spread, bool with_reduce) {
   if (spread) {
     const SpreadArgs a = spread_args(c);
     prof_begin(c, MSG_K_SPREAD);
@@ -703,21 +716,26 @@ void launch_spread_gather(Ctx* c, bool spread) {
     prof_end(c, MSG_K_SPREAD);
   } else {
     const GatherArgs g = gather_args(c);
+    const int nb = (8 * c->np + 255) / 256;
     prof_begin(c, MSG_K_GATHER);
-    hipLaunchKernelGGL((k_gather<real, P>), dim3((8 * c->np + 255) / 256), dim3(256), 0, c->stream, g);
+    if (with_reduce)
+      hipLaunchKernelGGL((k_gather_reduce<real, P>), dim3(nb + 1), dim3(256), 0, c->stream, g,
+                         reduce_args(c, true));
+    else
+      hipLaunchKernelGGL((k_gather<real, P>), dim3(nb), dim3(256), 0, c->stream, g);
     prof_end(c, MSG_K_GATHER);
   }
 }
 
 template <typename real>
-int spread_gather_dispatch(Ctx* c, bool spread) {
+int spread_gather_dispatch(Ctx* c, bool spread, bool with_reduce = false) {
   switch (c->mesh.order) {
-    case 3: launch_spread_gather<real, 3>(c, spread); break;
-    case 4: launch_spread_gather<real, 4>(c, spread); break;
-    case 5: launch_spread_gather<real, 5>(c, spread); break;
-    case 6: launch_spread_gather<real, 6>(c, spread); break;
-    case 7: launch_spread_gather<real, 7>(c, spread); break;
-    case 8: launch_spread_gather<real, 8>(c, spread); break;
+    case 3: launch_spread_gather<real, 3>(c, spread, with_reduce); break;
+    case 4: launch_spread_gather<real, 4>(c, spread, with_reduce); break;
+    case 5: launch_spread_gather<real, 5>(c, spread, with_reduce); break;
+    case 6: launch_spread_gather<real, 6>(c, spread, with_reduce); break;
+    case 7: launch_spread_gather<real, 7>(c, spread, with_reduce); break;
+    case 8: launch_spread_gather<real, 8>(c, spread, with_reduce); break;
     default: return set_error(MSG_ERR_ARG, "SPME order must be 3..8");
   }
   return MSG_OK;
@@ -829,8 +847,9 @@ int launch_spme_mesh(Ctx* c) {
   return MSG_OK;
 }
 
-int launch_spme_gather(Ctx* c) {
-  int e = c->fp64 ? spread_gather_dispatch<double>(c, false) : spread_gather_dispatch<float>(c, false);
+int launch_spme_gather(Ctx* c, bool with_reduce) {
+  int e = c->fp64 ? spread_gather_dispatch<double>(c, false, with_reduce)
+                  : spread_gather_dispatch<float>(c, false, with_reduce);
   if (e) return e;
   MSG_HIP(hipGetLastError());
   return MSG_OK;

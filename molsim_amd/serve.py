@@ -34,8 +34,8 @@ from .engine import Engine, Unit
 
 def footprint(spec) -> int:
     """Upper estimate (bytes) of the device memory a mixed-precision cell-list SPME context
-    holds after UTotal and DUTotal have run: per particle ~ 240 B (positions, charges, cell
-    and brick orderings, forces, B-spline data, pair partials), per mesh point ~ 10 B (FP32
+    holds after UTotal and DUTotal have run: per particle ~ 250 B (positions, charges, cell
+    and brick orderings and ranks, forces, B-spline data, pair partials), per mesh point ~ 10 B (FP32
     mesh, half-complex mesh, influence function, FP64 trial kernel), per cell 40 B, plus
     fixed partial-sum blocks and tables."""
     n = spec.n
@@ -46,7 +46,7 @@ def footprint(spec) -> int:
     for d in range(3):
         ncell *= max(1, int(spec.box[d] // rc))
     mesh = 4 * K ** 3 + 8 * H * K * K + 4 * H ** 3 + 8 * H ** 3 + 64 * K
-    return int(1.15 * (260 * n + mesh + 40 * ncell + (2 << 20)))
+    return int(1.15 * (270 * n + mesh + 40 * ncell + (2 << 20)))
 
 
 @dataclass
