@@ -37,7 +37,8 @@ struct BinArgs {
   double* dip_part;
   int* flags;
   int* rank;        // 2*np: each particle's place in its cell, then in its brick, as the
-                    // counters hand them out (the scatter needs no counters of its own)
+                    // counters hand them out (the scatter needs no counters of its own);
+                    // null where the scatter keeps its own counters (batched passes)
 };
 
 template <typename real>
@@ -59,7 +60,10 @@ __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
     if (bad) atomicOr(a.flags, FLAG_CELL);
     const int c = ci[0] + a.g.nc[0] * (ci[1] + a.g.nc[1] * ci[2]);
     a.cellid[i] = c;
-    a.rank[i] = atomicAdd(&a.cell_count[c], 1);
+    if (a.rank)
+      a.rank[i] = atomicAdd(&a.cell_count[c], 1);
+    else
+      atomicAdd(&a.cell_count[c], 1);
     if (a.use_mesh) {
       real* frac = (real*)a.frac;
       int mi[3];
@@ -73,7 +77,10 @@ __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
       }
       a.mm[i] = mi[0] | (mi[1] << 10) | (mi[2] << 20);
       const int b = mi[0] / kBX + a.m.nb[0] * (mi[1] / kBY + a.m.nb[1] * (mi[2] / kBZ));
-      a.rank[a.np + i] = atomicAdd(&a.brick_count[b], 1);
+      if (a.rank)
+        a.rank[a.np + i] = atomicAdd(&a.brick_count[b], 1);
+      else
+        atomicAdd(&a.brick_count[b], 1);
     }
     dx = q * r[0];
     dy = q * r[1];
@@ -667,8 +674,11 @@ static void seg_finalize(const SegGroup& g, hipStream_t s) {
 
 int plan_bin(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out) {
   SegBuilder<BinArgs> b;
-  for (Ctx* c : cs)
-    b.add(c->fp64, c->fp64 ? seg_bin<double> : seg_bin<float>, bin_args(c), (c->np + 255) / 256);
+  for (Ctx* c : cs) {
+    BinArgs a = bin_args(c);
+    a.rank = nullptr;  // the batched scatter counts slots itself
+    b.add(c->fp64, c->fp64 ? seg_bin<double> : seg_bin<float>, a, (c->np + 255) / 256);
+  }
   return b.finish(own, out);
 }
 

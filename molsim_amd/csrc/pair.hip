@@ -257,97 +257,40 @@ struct PairAcc {
   }
 };
 
-// Table metadata of one home particle's type pairs, indexed by the partner's type and held
-// in registers: a pair selects its interval start, bin scale and hard-core limits from them
-// instead of reading LDS once per candidate.
 template <typename real, int NTP>
-struct HomeTab {
-  static constexpr int NT = PairAcc<real, NTP>::NT;
-  real zlo[NT], invw[NT], r2at[NT], r2um[NT];
-  int bin0[NT], last[NT];
-  __device__ void load(int ht, const real* s_zlo, const real* s_invw, const real* s_r2at,
-                       const real* s_r2um, const int* s_last, const int* s_tp) {
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int tp = s_tp[ht * kMaxT + t];
-      zlo[t] = s_zlo[tp];
-      invw[t] = s_invw[tp];
-      r2at[t] = s_r2at[tp];
-      r2um[t] = s_r2um[tp];
-      bin0[t] = tp * kTabBins;
-      last[t] = s_last[tp];
-    }
-  }
-  template <typename T>
-  __device__ static T pick(const T (&v)[NT], int t) {
-    T r = v[0];
-#pragma unroll
-    for (int q = 1; q < NT; ++q) r = (t == q) ? v[q] : r;
-    return r;
-  }
-};
-
-// One table record (12 coefficients) from LDS in 16-byte pieces.
-__device__ inline void load_rec(const float* s_coef, int ti, float (&t)[12]) {
-  const float4* p = (const float4*)(s_coef + 12 * ti);
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const float4 v = p[k];
-    t[4 * k] = v.x;
-    t[4 * k + 1] = v.y;
-    t[4 * k + 2] = v.z;
-    t[4 * k + 3] = v.w;
-  }
-}
-__device__ inline void load_rec(const double* s_coef, int ti, double (&t)[12]) {
-  const double2* p = (const double2*)(s_coef + 12 * ti);
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const double2 v = p[k];
-    t[2 * k] = v.x;
-    t[2 * k + 1] = v.y;
-  }
-}
-
-// One pair, home against partner j of type tj.  The table path runs without branches for
-// every candidate (the bin is clamped, so even a pair beyond the cut-off reads a valid
-// record) and the result is masked, so the candidates of a lane overlap their LDS reads;
-// only a pair whose bin starts above its interval takes the scan loop, as the reference's
-// linear scan (Src/energy.F90:713-719) would.
-template <typename real, int NTP>
-__device__ inline void pair_eval(const double4& hp, const double4& pj, int tj, const Box& box, real rc2,
-                                 bool inclusive, const real* s_coef, const int* s_bins,
-                                 const HomeTab<real, NTP>& ht, PairAcc<real, NTP>& acc) {
-  typedef HomeTab<real, NTP> HT;
+__device__ inline void pair_eval(const double4& hp, int ht, const double4& pj, int tj, const Box& box,
+                                 real rc2, bool inclusive, const real* s_coef, const int* s_bins,
+                                 const real* s_zlo, const real* s_invw, const real* s_r2at,
+                                 const real* s_r2um, const int* s_last, const int* s_tp,
+                                 PairAcc<real, NTP>& acc) {
   double d[3] = {hp.x - pj.x, hp.y - pj.y, hp.z - pj.z};
 #pragma unroll
   for (int q = 0; q < 3; ++q)
     if (fabs(d[q]) > box.half[q]) d[q] -= copysign(box.L[q], d[q]);
   const real dx = (real)d[0], dy = (real)d[1], dz = (real)d[2];
   const real r2 = dx * dx + dy * dy + dz * dz;
-  const bool in = inclusive ? !(r2 > rc2) : (r2 < rc2);
-  const real r2at = HT::pick(ht.r2at, tj), r2um = HT::pick(ht.r2um, tj);
-  int b = (int)((r2 - HT::pick(ht.zlo, tj)) * HT::pick(ht.invw, tj));
-  b = min(max(b, 0), kTabBins - 1);
-  int ti = s_bins[HT::pick(ht.bin0, tj) + b];
-  real t[12];
-  load_rec(s_coef, ti, t);
-  if (in && r2 < t[0]) {
-    const int last = HT::pick(ht.last, tj);
+  if (inclusive ? (r2 > rc2) : !(r2 < rc2)) return;
+  const int tp = s_tp[ht * kMaxT + tj];
+  real us, fs;
+  if (r2 < s_r2at[tp]) {
+    us = real(1e10);
+    fs = real(1e10);
+  } else if (r2 < s_r2um[tp]) {
+    acc.err |= FLAG_R2UMIN;
+    return;
+  } else {
+    int b = (int)((r2 - s_zlo[tp]) * s_invw[tp]);
+    b = min(max(b, 0), kTabBins - 1);
+    int ti = s_bins[tp * kTabBins + b];
+    const int last = s_last[tp];
     while (ti < last && r2 < s_coef[12 * ti]) ++ti;
-    load_rec(s_coef, ti, t);
+    const real* t = s_coef + 12 * ti;
+    const real dd = r2 - t[0];
+    us = t[1] + dd * (t[2] + dd * (t[3] + dd * (t[4] + dd * (t[5] + dd * t[6]))));
+    fs = t[7] + dd * (t[8] + dd * (t[9] + dd * (t[10] + dd * t[11])));
   }
-  const real dd = r2 - t[0];
-  real us = t[1] + dd * (t[2] + dd * (t[3] + dd * (t[4] + dd * (t[5] + dd * t[6]))));
-  real fs = t[7] + dd * (t[8] + dd * (t[9] + dd * (t[10] + dd * t[11])));
-  const bool hc = r2 < r2at;  // hard core: usum = fsum = 1e10 (Src/energy.F90:707-709)
-  if (in && !hc && r2 < r2um) acc.err |= FLAG_R2UMIN;  // below the table: the reference stops
-  const bool use = in && (hc || !(r2 < r2um));
-  us = use ? (hc ? real(1e10) : us) : real(0);
-  fs = use ? (hc ? real(1e10) : fs) : real(0);
-  const double u64 = (double)us;
 #pragma unroll
-  for (int q = 0; q < PairAcc<real, NTP>::NT; ++q) acc.et[q] += (q == tj) ? u64 : 0.0;
+  for (int t = 0; t < PairAcc<real, NTP>::NT; ++t) acc.et[t] += (t == tj) ? (double)us : 0.0;
   acc.fx += (double)(fs * dx);
   acc.fy += (double)(fs * dy);
   acc.fz += (double)(fs * dz);
@@ -456,8 +399,6 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
   const real rc2 = (real)a.rcut2;
   PairAcc<real, NTP> acc;
   acc.zero();
-  HomeTab<real, NTP> htab;
-  htab.load(ht, s_zlo, s_invw, s_r2at, s_r2um, s_last, s_tp);
   const int share = (ncand + LPH - 1) / LPH;
   int i = k * share;
   const int i1 = home ? min(ncand, i + share) : 0;
@@ -490,7 +431,8 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (hv[u] && jv[u] != s)
-          pair_eval<real, NTP>(hp, pv[u], (int)pv[u].w, a.box, rc2, a.cut_inclusive, s_coef, s_bins, htab, acc);
+          pair_eval<real, NTP>(hp, ht, pv[u], (int)pv[u].w, a.box, rc2, a.cut_inclusive, s_coef, s_bins, s_zlo,
+                               s_invw, s_r2at, s_r2um, s_last, s_tp, acc);
     }
   }
 #pragma unroll
@@ -644,8 +586,6 @@ __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
     const int ip = v.vl_home[s];
     const double4 hp = make_double4(v.ro[3 * ip], v.ro[3 * ip + 1], v.ro[3 * ip + 2], 0.0);
     const int ht = v.ipt[ip];
-    HomeTab<real, NTP> htab;
-    htab.load(ht, s_zlo, s_invw, s_r2at, s_r2um, s_last, s_tp);
     const int n = v.vl_n[s];
     for (int k = 0; k < n; k += 2) {
       const int jA = v.vl_nbr[(size_t)k * a.np + s];
@@ -654,8 +594,11 @@ __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
       const double4 pA = make_double4(v.ro[3 * jA], v.ro[3 * jA + 1], v.ro[3 * jA + 2], 0.0);
       const double4 pB = make_double4(v.ro[3 * jB], v.ro[3 * jB + 1], v.ro[3 * jB + 2], 0.0);
       const int tA = v.ipt[jA], tB = v.ipt[jB];
-      pair_eval<real, NTP>(hp, pA, tA, a.box, rc2, true, s_coef, s_bins, htab, acc);
-      if (haveB) pair_eval<real, NTP>(hp, pB, tB, a.box, rc2, true, s_coef, s_bins, htab, acc);
+      pair_eval<real, NTP>(hp, ht, pA, tA, a.box, rc2, true, s_coef, s_bins, s_zlo, s_invw,
+                           s_r2at, s_r2um, s_last, s_tp, acc);
+      if (haveB)
+        pair_eval<real, NTP>(hp, ht, pB, tB, a.box, rc2, true, s_coef, s_bins, s_zlo, s_invw,
+                             s_r2at, s_r2um, s_last, s_tp, acc);
     }
     a.force[3 * ip + 0] = acc.fx * a.scale;
     a.force[3 * ip + 1] = acc.fy * a.scale;
