@@ -15,6 +15,10 @@
  *   msg_commit       <- MCUpdate: EwaldUpdateArray + UpdateCellIp + ro <- rotm
  *                                                 Src/mc.F90:5203-5235, Src/denergy.F90:1776-1819,
  *                                                 Src/celllist.F90:366-382
+ *   msg_set_weak_charges, msg_dutotal_charge, msg_commit_charge
+ *                    <- lweakcharge: UWeakChargeA, ChargeChange's DUTotal and update
+ *                                                 Src/energy.F90:7180-7276, Src/mc.F90:3683-3800,
+ *                                                 Src/denergy.F90:154-164,1552-1699,3911-4145
  *
  * Conventions (those of the Fortran caller, so an iso_c_binding interface passes its
  * arrays straight through):
@@ -108,7 +112,8 @@ void msg_destroy(msg_ctx* ctx);
 int msg_set_table(msg_ctx* ctx, const double* ubuf, int nbuf, const int* iubuflow,
                   const int* nugrid, const double* r2umin, const double* r2atat);
 
-/* Charges (az) and 1-based particle types (iptpn).  Fixed for the context's life time. */
+/* Charges (az) and 1-based particle types (iptpn).  Types are fixed for the context's life
+ * time; charges change only through the weak-charge entry points below. */
 int msg_set_particles(msg_ctx* ctx, const double* az, const int* iptpn);
 
 /* Positions ro(3,np) from host memory, or from device memory already on this GPU. */
@@ -150,6 +155,23 @@ int msg_dutotal_batch(msg_ctx* ctx, int ntrial, const int* ip, const double* rot
 
 /* MCUpdate after an accepted move: positions, resident structure factors / mesh, cell list. */
 int msg_commit(msg_ctx* ctx, int nptm, const int* ipnptm, const double* rotm);
+
+/* Weak (titrating) charges, lweakcharge (UWeakChargeA Src/energy.F90:7180-7276; DUTotal's
+ * weak-charge branch Src/denergy.F90:154-164,1552-1699,3911-4145): laz[i] (Fortran logical
+ * as int) = 1 when particle i carries the charge zat[type] of its type (zat[npt]); its az
+ * becomes zat[type] or 0, as IOCnf sets it (Src/molsim.F90:895-898).  A pair with an
+ * uncharged particle has no pair energy; hard-core checks of trial moves still apply.
+ * Std Ewald or no Ewald (the reference stops with SPME, Src/denergy.F90:1565-1570). */
+int msg_set_weak_charges(msg_ctx* ctx, const int* laz, const double* zat);
+
+/* ChargeChange trial (Src/mc.F90:3683-3800): the charge states of nptm (1 or 2) particles
+ * ipnptm (1-based) flip, positions stay; outputs as msg_dutotal. */
+int msg_dutotal_charge(msg_ctx* ctx, int nptm, const int* ipnptm, int* lhsoverlap,
+                       double* du_twob, double* du_rec, double* du_tot);
+
+/* MCUpdate of an accepted ChargeChange: laz and az of those particles flip
+ * (Src/mc.F90:3781-3785). */
+int msg_commit_charge(msg_ctx* ctx, int nptm, const int* ipnptm);
 
 /* Velocity-Verlet MD on the device (VelVer, Src/md.F90:252-410): msg_md_init takes the
  * mass of each particle type (in the units that make force * force_scale / mass an

@@ -369,7 +369,7 @@ void msg_destroy(msg_ctx* h) {
                   c->d_force, c->d_part, c->d_out, c->d_flags, c->d_de, c->d_de_ip, c->d_de_r,
                   c->d_tcoef64, c->d_gk, c->d_item_cell, c->d_bq, c->d_bmm, c->d_M,
                   c->d_vl_home, c->d_vl_n, c->d_vl_nbr, c->d_ro_ref, c->d_dmax, c->d_vel,
-                  c->d_acc, c->d_minv, c->d_ke_part, c->d_rank};
+                  c->d_acc, c->d_minv, c->d_ke_part, c->d_rank, c->d_laz};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (c->h_out) hipHostFree(c->h_out);
@@ -415,7 +415,56 @@ int msg_set_particles(msg_ctx* h, const double* az, const int* iptpn) {
   c->sumq2 = s;
   MSG_HIP(hipMemcpy(c->d_az, az, c->np * sizeof(double), hipMemcpyHostToDevice));
   MSG_HIP(hipMemcpy(c->d_ipt, t.data(), c->np * sizeof(int), hipMemcpyHostToDevice));
+  c->h_ipt = t;
+  c->h_az.assign(az, az + c->np);
   c->have_particles = true;
+  return MSG_OK;
+}
+
+int msg_set_weak_charges(msg_ctx* h, const int* laz, const double* zat) {
+  if (!h || !laz || !zat) return set_error(MSG_ERR_ARG, "msg_set_weak_charges: null argument");
+  Ctx* c = &h->c;
+  if (!c->have_particles) return set_error(MSG_ERR_STATE, "msg_set_particles has not been called");
+  if (c->P.ewald_mode == MSG_EWALD_SPM)  // Src/denergy.F90:1565-1570
+    return set_error(MSG_ERR_ARG, "weak charges need std Ewald or no Ewald (SPME is not supported with them)");
+  if (!c->d_laz)
+    if (int e = dalloc(c, (void**)&c->d_laz, c->np * sizeof(int))) return e;
+  c->h_zat.assign(zat, zat + c->npt);
+  c->h_laz.resize(c->np);
+  double s = 0.0;
+  for (int i = 0; i < c->np; ++i) {
+    c->h_laz[i] = laz[i] != 0;
+    c->h_az[i] = c->h_laz[i] ? c->h_zat[c->h_ipt[i]] : 0.0;  // as IOCnf sets az, Src/molsim.F90:895-898
+    s += c->h_az[i] * c->h_az[i];
+  }
+  c->sumq2 = s;
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  MSG_HIP(hipMemcpy(c->d_laz, c->h_laz.data(), c->np * sizeof(int), hipMemcpyHostToDevice));
+  MSG_HIP(hipMemcpy(c->d_az, c->h_az.data(), c->np * sizeof(double), hipMemcpyHostToDevice));
+  c->graph_ok = false;  // the captured pass must see the charge states
+  c->fresh = false;
+  return MSG_OK;
+}
+
+int msg_commit_charge(msg_ctx* h, int nptm, const int* ipnptm) {
+  if (!h || !ipnptm || nptm < 1 || nptm > 2) return set_error(MSG_ERR_ARG, "msg_commit_charge: one or two particles");
+  Ctx* c = &h->c;
+  if (c->h_laz.empty()) return set_error(MSG_ERR_STATE, "msg_commit_charge: weak charges are not set");
+  for (int m = 0; m < nptm; ++m)
+    if (ipnptm[m] < 1 || ipnptm[m] > c->np) return set_error(MSG_ERR_ARG, "msg_commit_charge: ip out of range");
+  if (nptm == 2 && ipnptm[0] == ipnptm[1]) return set_error(MSG_ERR_ARG, "msg_commit_charge: the same particle twice");
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  for (int m = 0; m < nptm; ++m) {  // laz and az of the flipped particles, Src/mc.F90:3781-3785
+    const int i = ipnptm[m] - 1;
+    c->sumq2 -= c->h_az[i] * c->h_az[i];
+    c->h_laz[i] = !c->h_laz[i];
+    c->h_az[i] = c->h_laz[i] ? c->h_zat[c->h_ipt[i]] : 0.0;
+    c->sumq2 += c->h_az[i] * c->h_az[i];
+    MSG_HIP(hipMemcpy(c->d_laz + i, &c->h_laz[i], sizeof(int), hipMemcpyHostToDevice));
+    MSG_HIP(hipMemcpy(c->d_az + i, &c->h_az[i], sizeof(double), hipMemcpyHostToDevice));
+  }
+  c->graph_ok = false;  // self term of the captured reduction
+  c->fresh = false;     // structure factors and the cell records' charge tags are rebuilt
   return MSG_OK;
 }
 
@@ -535,6 +584,23 @@ int msg_dutotal(msg_ctx* h, int nptm, const int* ipnptm, const double* rotm, int
   std::vector<double> twob(c->nptpt + 1);
   int ov = 0;
   if (int e = dutotal_batch(c, 1, ipnptm, rotm, &ov, &dt, &dr, twob.data())) return e;
+  *lhsoverlap = ov;
+  if (du_twob)
+    for (int k = 0; k <= c->nptpt; ++k) du_twob[k] = twob[k];
+  if (du_rec) *du_rec = dr;
+  if (du_tot) *du_tot = dt;
+  return MSG_OK;
+}
+
+int msg_dutotal_charge(msg_ctx* h, int nptm, const int* ipnptm, int* lhsoverlap, double* du_twob,
+                       double* du_rec, double* du_tot) {
+  if (!h || !ipnptm || !lhsoverlap) return set_error(MSG_ERR_ARG, "msg_dutotal_charge: null argument");
+  Ctx* c = &h->c;
+  if (int e = check_ready(c)) return e;
+  std::vector<double> twob(c->nptpt + 1);
+  double dt = 0.0, dr = 0.0;
+  int ov = 0;
+  if (int e = dutotal_charge(c, nptm, ipnptm, &ov, twob.data(), &dr, &dt)) return e;
   *lhsoverlap = ov;
   if (du_twob)
     for (int k = 0; k <= c->nptpt; ++k) du_twob[k] = twob[k];

@@ -328,7 +328,8 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
 }  // namespace
 
 bool chain_eligible(const Ctx* c) {
-  return c->P.ewald_mode != MSG_EWALD_SPM && c->P.list_mode != MSG_LIST_VERLET &&
+  // systems with weak charges run their trials from the host (msg_dutotal honours the states)
+  return c->P.ewald_mode != MSG_EWALD_SPM && c->P.list_mode != MSG_LIST_VERLET && c->h_laz.empty() &&
          (c->P.ewald_mode != MSG_EWALD_STD || (c->nkvec <= kChainThreads * kChainKvec && c->P.ncut <= kChainNcut));
 }
 

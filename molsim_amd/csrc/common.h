@@ -33,6 +33,9 @@ enum : int { FLAG_R2UMIN = 1, FLAG_IBUF = 2, FLAG_CELL = 4, FLAG_NLIST = 8 };
 constexpr int kMaxT = MSG_MAX_TYPES;
 constexpr int kMaxTP = kMaxT * (kMaxT + 1) / 2;
 constexpr int kTabBins = 256;        // r^2 bins of the interval index per type pair
+// Weak charges: the type field of an uncharged particle's cell-ordered record (cpos.w) carries
+// this offset, so the pair kernels see the charge state in the record they load anyway.
+constexpr int kUncharged = 8;
 constexpr int kMaxTabIntervals = 512;   // records below rcut staged in LDS (<= 48 KB in FP64)
 constexpr int kMaxStencil = 27;
 constexpr int kPairHB = 4;           // home particles per pair-kernel work item
@@ -118,6 +121,11 @@ struct Ctx {
   double* d_M = nullptr;         // total dipole sum q r of the current pass (3)
   int* d_cell_tmp = nullptr;     // np (unsorted members)
   int* d_rank = nullptr;         // 2*np: place of each particle in its cell, then in its brick
+  // weak (titrating) charges: charge state per particle (1 charged), null when off; host
+  // copies of the states, charges, full charges per type and types
+  int* d_laz = nullptr;
+  std::vector<int> h_laz, h_ipt;
+  std::vector<double> h_az, h_zat;
   int* d_cell_ip = nullptr;      // np (sorted: cell-major, descending ip in a cell)
   int* d_slot_of = nullptr;      // np: slot of particle ip in the cell order
   void* d_cpos = nullptr;        // np double4 in cell order: x, y, z, particle type
@@ -238,6 +246,8 @@ int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, in
              int* rng_state, double* am, double* du_sum, int* naccept, int* events);
 int dutotal_launch(Ctx* c, int ntrial, const int* ip, const double* rotm);
 int dutotal_collect(Ctx* c, int ntrial, int* ovl, double* du_tot, double* du_rec, double* du_twob);
+int dutotal_charge(Ctx* c, int nptm, const int* ip1, int* ovl, double* du_twob, double* du_rec,
+                   double* du_tot);
 int utotal_refresh(Ctx* c);                        // UTotal sequence + sync (for dE state)
 int utotal_sequence(Ctx* c);
 int verlet_setup(Ctx* c);

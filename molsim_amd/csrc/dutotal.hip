@@ -67,6 +67,7 @@ struct DeArgs {
   const int* tip;
   const double* tr;
   double* out;            // per trial: ntp + 4 doubles
+  const int* laz;         // weak charges (null when off)
 };
 
 __device__ inline double table_u(const DeArgs& a, int tp, double r2) {
@@ -153,6 +154,9 @@ __device__ __forceinline__ void de_body(const DeArgs& a, int ip, const double* t
           ovl = 1;
           continue;
         }
+        // weak charges: UWeakChargeANew/AOld skip an uncharged pair after the overlap checks
+        // (Src/denergy.F90:3955-3959, 4066-4068); the partner's state is in its record
+        if (a.laz && (!a.laz[ip] || (int)pj.w >= kUncharged)) continue;
         const double u = table_u(a, tp, r2);
 #pragma unroll
         for (int k2 = 0; k2 < kMaxT; ++k2) e[k2] += (k2 == tj) ? sgn * u : 0.0;
@@ -293,6 +297,146 @@ __global__ __launch_bounds__(256) void k_dutotal_seg(const DeArgs* __restrict__ 
   if (cnt) atomicAdd(&s_sys, cnt);
   __syncthreads();
   de_body<P>(args[s_sys], tip[b], tr + 3 * b, out + (size_t)b * (kMaxTP + 4));
+}
+
+// ChargeChange trial with weak charges (Src/mc.F90:3683-3800; DUTotal's weak-charge branch,
+// Src/denergy.F90:154-164): the charge states of nptm (1 or 2) particles flip, positions stay.
+// Real space: each flipped particle against its cell-list partners with the new states minus
+// the old (UWeakChargeANew/AOld, :3911-4145; the pair of the two flipped particles once,
+// lptmdutwob), overlap checks before the charge test.  Reciprocal space from the resident
+// structure factors changed by (aztm - az) e^{ik.r} (DUWeakChargeEwaldRecStd, :1586-1650),
+// the self term (:1654-1667) and the surface term (:1671-1695).  One workgroup, all FP64.
+struct DeChargeArgs {
+  DeArgs a;
+  int nptm;
+  int ip[2];
+  int laznew[2];
+  double aznew[2];
+  double alpha_spi;  // alpha / sqrt(pi)
+};
+
+__global__ __launch_bounds__(256) void k_dutotal_charge(DeChargeArgs d) {
+  const DeArgs& a = d.a;
+  __shared__ double sh[8];
+  __shared__ int s_ovl;
+  if (threadIdx.x == 0) s_ovl = 0;
+  __syncthreads();
+  double e[kMaxTP];
+#pragma unroll
+  for (int k = 0; k < kMaxTP; ++k) e[k] = 0.0;
+  int ovl = 0;
+  for (int m = 0; m < d.nptm; ++m) {
+    const int ip = d.ip[m];
+    const int ti = a.ipt[ip];
+    const int li_old = a.laz[ip], li_new = d.laznew[m];
+    const double ri[3] = {a.ro[3 * ip], a.ro[3 * ip + 1], a.ro[3 * ip + 2]};
+    const int cc = a.slot_cell[a.slot_of[ip]];
+    const int cx = cc % a.g.nc[0], cy = (cc / a.g.nc[0]) % a.g.nc[1], cz = cc / (a.g.nc[0] * a.g.nc[1]);
+    for (int k = 0; k < a.g.nst; ++k) {
+      int x = cx + a.g.st[3 * k], y = cy + a.g.st[3 * k + 1], z = cz + a.g.st[3 * k + 2];
+      x += x < 0 ? a.g.nc[0] : (x >= a.g.nc[0] ? -a.g.nc[0] : 0);
+      y += y < 0 ? a.g.nc[1] : (y >= a.g.nc[1] ? -a.g.nc[1] : 0);
+      z += z < 0 ? a.g.nc[2] : (z >= a.g.nc[2] ? -a.g.nc[2] : 0);
+      const int nb = x + a.g.nc[0] * (y + a.g.nc[1] * z);
+      const int j1 = a.cell_start[nb + 1];
+      for (int j = a.cell_start[nb] + threadIdx.x; j < j1; j += 256) {
+        const int jp = a.cell_ip[j];
+        if (jp == ip) continue;
+        int mj = -1;
+        for (int q = 0; q < d.nptm; ++q) mj = d.ip[q] == jp ? q : mj;
+        if (mj >= 0 && jp < ip) continue;  // the flipped pair once
+        const double4 pj = a.cpos[j];
+        double dd[3] = {ri[0] - pj.x, ri[1] - pj.y, ri[2] - pj.z};
+#pragma unroll
+        for (int qd = 0; qd < 3; ++qd)
+          if (fabs(dd[qd]) > a.box.half[qd]) dd[qd] -= copysign(a.box.L[qd], dd[qd]);
+        const double r2 = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+        if (a.cut_inclusive ? (r2 > a.rcut2) : !(r2 < a.rcut2)) continue;
+        const int tj = a.ctype[j];
+        const int tp = a.tm.tpidx[ti * kMaxT + tj];
+        if (r2 < a.tm.r2atat[tp] || r2 < a.tm.r2umin[tp]) {
+          ovl = 1;
+          continue;
+        }
+        const int lj_old = (int)pj.w < kUncharged;
+        const int lj_new = mj >= 0 ? d.laznew[mj] : lj_old;
+        const int on_new = li_new && lj_new, on_old = li_old && lj_old;
+        if (on_new == on_old) continue;
+        const double u = table_u(a, tp, r2);
+        const double du = on_new ? u : -u;
+#pragma unroll
+        for (int k2 = 0; k2 < kMaxTP; ++k2) e[k2] += (k2 == tp) ? du : 0.0;
+      }
+    }
+  }
+  if (ovl) atomicOr(&s_ovl, 1);
+  double dq[2] = {0.0, 0.0}, rm[2][3];
+  for (int m = 0; m < d.nptm; ++m) {
+    dq[m] = d.aznew[m] - a.az[d.ip[m]];
+    for (int k = 0; k < 3; ++k) rm[m][k] = a.ro[3 * d.ip[m] + k];
+  }
+  double drec = 0.0;
+  if (a.ewald == 1) {
+    for (int kn = threadIdx.x; kn < a.nk; kn += 256) {
+      const double kx = a.kvec[3 * kn] * a.tpb[0], ky = a.kvec[3 * kn + 1] * a.tpb[1],
+                   kz = a.kvec[3 * kn + 2] * a.tpb[2];
+      double dr[4] = {0, 0, 0, 0}, di[4] = {0, 0, 0, 0};
+      for (int m = 0; m < d.nptm; ++m) {
+        double sx, cx, sy, cy, sz, cz;
+        sincos(kx * rm[m][0], &sx, &cx);
+        sincos(ky * rm[m][1], &sy, &cy);
+        sincos(kz * rm[m][2], &sz, &cz);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const double gx = (c & 2) ? sx : -sx, gy = (c & 1) ? sy : -sy;
+          const double r1 = cx * cy - gx * gy, i1 = cx * gy + gx * cy;
+          dr[c] += dq[m] * (r1 * cz - i1 * sz);
+          di[c] += dq[m] * (r1 * sz + i1 * cz);
+        }
+      }
+      double tn = 0.0, to = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double Sr = a.sk[8 * kn + 2 * c], Si = a.sk[8 * kn + 2 * c + 1];
+        tn += (Sr + dr[c]) * (Sr + dr[c]) + (Si + di[c]) * (Si + di[c]);
+        to += Sr * Sr + Si * Si;
+      }
+      drec += a.kfac[kn] * (tn - to);
+    }
+  }
+  drec = msgd::block_sum(drec, sh);
+  double out_tp[kMaxTP];
+  for (int tp = 0; tp < a.ntp; ++tp) out_tp[tp] = msgd::block_sum(e[tp], sh);
+  if (threadIdx.x == 0) {
+    double* o = a.out;
+    double tw = 0.0;
+    for (int tp = 0; tp < a.ntp; ++tp) {
+      o[1 + tp] = out_tp[tp];
+      tw += out_tp[tp];
+    }
+    o[0] = tw;
+    if (a.ewald) {
+      for (int m = 0; m < d.nptm; ++m) {
+        const double q0 = a.az[d.ip[m]];
+        drec += d.alpha_spi * (q0 * q0 - d.aznew[m] * d.aznew[m]);
+      }
+      if (a.lsurf) {
+        const double M[3] = {a.Mdip[0], a.Mdip[1], a.Mdip[2]};
+        double m2o = 0.0, m2n = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          double mn = M[k];
+          for (int m = 0; m < d.nptm; ++m) mn += dq[m] * rm[m][k];
+          m2o += M[k] * M[k];
+          m2n += mn * mn;
+        }
+        drec += a.surf_fac * (m2n - m2o);
+      }
+    }
+    const double du_rec = a.ewald ? a.epsi4pi * drec : 0.0;
+    o[a.ntp + 1] = du_rec;
+    o[a.ntp + 2] = tw + du_rec;
+    o[a.ntp + 3] = (double)s_ovl;
+  }
 }
 
 // g(x) = sum_k meshfac(|k|) cos(2 pi k.x / K) over the full spectrum, folded to [0, K/2]^3
@@ -468,6 +612,7 @@ static DeArgs de_args(Ctx* c) {
   a.tip = nullptr;
   a.tr = nullptr;
   a.out = nullptr;
+  a.laz = c->d_laz;
   return a;
 }
 
@@ -532,6 +677,45 @@ int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ov
                   double* du_rec, double* du_twob) {
   if (int e = dutotal_launch(c, ntrial, ip, rotm)) return e;
   return dutotal_collect(c, ntrial, ovl, du_tot, du_rec, du_twob);
+}
+
+// ChargeChange trial: the charge states of nptm particles ip1[] (1-based) flip.
+int dutotal_charge(Ctx* c, int nptm, const int* ip1, int* ovl, double* du_twob, double* du_rec,
+                   double* du_tot) {
+  if (c->h_laz.empty()) return set_error(MSG_ERR_STATE, "ChargeChange: weak charges are not set");
+  if (nptm < 1 || nptm > 2) return set_error(MSG_ERR_ARG, "ChargeChange: one or two particles");
+  for (int m = 0; m < nptm; ++m)
+    if (ip1[m] < 1 || ip1[m] > c->np) return set_error(MSG_ERR_ARG, "ChargeChange: particle index out of range");
+  if (nptm == 2 && ip1[0] == ip1[1]) return set_error(MSG_ERR_ARG, "ChargeChange: the same particle twice");
+  if (!c->fresh)
+    if (int e = utotal_refresh(c)) return e;
+  if (!c->cells_valid) {
+    if (int e = launch_finalize(c)) return e;
+    c->cells_valid = true;
+  }
+  if (c->de_cap < 1) {
+    MSG_HIP(hipStreamSynchronize(c->stream));
+    if (int e = dalloc(c, (void**)&c->d_de, (size_t)256 * (kMaxTP + 4) * sizeof(double))) return e;
+    if (int e = dalloc(c, (void**)&c->d_de_ip, 256 * sizeof(int))) return e;
+    if (int e = dalloc(c, (void**)&c->d_de_r, (size_t)256 * 3 * sizeof(double))) return e;
+    c->de_cap = 256;
+  }
+  DeChargeArgs d;
+  d.a = de_args(c);
+  d.a.out = c->d_de;
+  d.nptm = nptm;
+  for (int m = 0; m < 2; ++m) {
+    const int ip = ip1[m < nptm ? m : 0] - 1;
+    d.ip[m] = ip;
+    d.laznew[m] = !c->h_laz[ip];
+    d.aznew[m] = d.laznew[m] ? c->h_zat[c->h_ipt[ip]] : 0.0;  // aztm, Src/mc.F90:3740-3744
+  }
+  d.alpha_spi = c->P.alpha / std::sqrt(M_PI);
+  prof_begin(c, MSG_K_DE);
+  hipLaunchKernelGGL(k_dutotal_charge, dim3(1), dim3(256), 0, c->stream, d);
+  prof_end(c, MSG_K_DE);
+  MSG_HIP(hipGetLastError());
+  return dutotal_collect(c, 1, ovl, du_tot, du_rec, du_twob);
 }
 
 // MCUpdate for one accepted single-particle move: the position changes on the device and

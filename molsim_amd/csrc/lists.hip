@@ -381,6 +381,7 @@ struct FinalizeArgs {
   int* ctype;
   int* slot_cell;
   int* item_cell;
+  const int* laz;  // weak charges (null when off): an uncharged particle's record type gets + 8
 };
 
 // One wavefront per cell: rank members by particle index (descending, the reference's
@@ -409,7 +410,8 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& a, int bid) {
       const int dst = s0 + rank;
       a.cell_ip[dst] = ip;
       a.slot_of[ip] = dst;
-      a.cpos[dst] = make_double4(a.ro[3 * ip], a.ro[3 * ip + 1], a.ro[3 * ip + 2], (double)a.ipt[ip]);
+      const int tw = a.ipt[ip] + ((a.laz && !a.laz[ip]) ? kUncharged : 0);
+      a.cpos[dst] = make_double4(a.ro[3 * ip], a.ro[3 * ip + 1], a.ro[3 * ip + 2], (double)tw);
       a.ctype[dst] = a.ipt[ip];
       a.slot_cell[dst] = wv;
     }
@@ -574,6 +576,7 @@ static FinalizeArgs finalize_args(Ctx* c) {
   f.ctype = c->d_ctype;
   f.slot_cell = c->d_slot_cell;
   f.item_cell = c->d_item_cell;
+  f.laz = c->d_laz;
   return f;
 }
 

@@ -143,7 +143,7 @@ __device__ __forceinline__ void pair_items_body(const PairArgs& a, int bid) {
     for (int h = 0; h < kPairHB; ++h) {
       const int s = h0 + min(h, nh - 1);
       hp[h] = a.cpos[s];
-      ht[h] = (int)a.cpos[s].w;
+      ht[h] = (int)a.cpos[s].w;  // + kUncharged for an uncharged home (weak charges)
     }
     double f[kPairHB][3];
 #pragma unroll
@@ -154,9 +154,10 @@ __device__ __forceinline__ void pair_items_body(const PairArgs& a, int bid) {
       const int j = s_nbs[wv][k] + (idx - s_pref[wv][k]);
       const double4 pj = a.cpos[j];
       const int tj = (int)pj.w;
+      if (tj >= kUncharged) continue;  // weak charges: uncharged partner
 #pragma unroll
       for (int h = 0; h < kPairHB; ++h) {
-        if (h >= nh || j == h0 + h) continue;
+        if (h >= nh || j == h0 + h || ht[h] >= kUncharged) continue;
         double d[3] = {hp[h].x - pj.x, hp[h].y - pj.y, hp[h].z - pj.z};
 #pragma unroll
         for (int q = 0; q < 3; ++q)
@@ -270,6 +271,7 @@ __device__ inline void pair_eval(const double4& hp, int ht, const double4& pj, i
   const real dx = (real)d[0], dy = (real)d[1], dz = (real)d[2];
   const real r2 = dx * dx + dy * dy + dz * dz;
   if (inclusive ? (r2 > rc2) : !(r2 < rc2)) return;
+  if (tj >= kUncharged) return;  // weak charges: no pair energy with an uncharged partner (UWeakChargeA)
   const int tp = s_tp[ht * kMaxT + tj];
   real us, fs;
   if (r2 < s_r2at[tp]) {
@@ -334,7 +336,9 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
   const bool home = s < a.np;
   const int hs = home ? s : a.np - 1;
   const double4 hp = a.cpos[hs];
-  const int ht = (int)hp.w;
+  const int hw = (int)hp.w;
+  const int ht = hw & (kUncharged - 1);
+  const bool hcharged = hw < kUncharged;  // an uncharged home has no pair energy (weak charges)
   const int c = a.slot_cell[hs];
   const int cx = c % a.g.nc[0], cy = (c / a.g.nc[0]) % a.g.nc[1], cz = c / (a.g.nc[0] * a.g.nc[1]);
   // home position inside its cell, for skipping stencil cells whose nearest point is
@@ -401,7 +405,7 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
   acc.zero();
   const int share = (ncand + LPH - 1) / LPH;
   int i = k * share;
-  const int i1 = home ? min(ncand, i + share) : 0;
+  const int i1 = (home && hcharged) ? min(ncand, i + share) : 0;
   if (i < i1) {
     int cc = 0;
     while (s_cpf[grp][cc + 1] <= i) ++cc;
@@ -550,6 +554,7 @@ struct VlPairArgs {
   const double* ro;
   const double* az;
   const int* ipt;
+  const int* laz;  // weak charges (null when off)
 };
 
 template <typename real, int NTP>
@@ -586,14 +591,15 @@ __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
     const int ip = v.vl_home[s];
     const double4 hp = make_double4(v.ro[3 * ip], v.ro[3 * ip + 1], v.ro[3 * ip + 2], 0.0);
     const int ht = v.ipt[ip];
-    const int n = v.vl_n[s];
+    const int n = (v.laz && !v.laz[ip]) ? 0 : v.vl_n[s];  // an uncharged home has no pair energy
     for (int k = 0; k < n; k += 2) {
       const int jA = v.vl_nbr[(size_t)k * a.np + s];
       const bool haveB = k + 1 < n;
       const int jB = haveB ? v.vl_nbr[(size_t)(k + 1) * a.np + s] : jA;
       const double4 pA = make_double4(v.ro[3 * jA], v.ro[3 * jA + 1], v.ro[3 * jA + 2], 0.0);
       const double4 pB = make_double4(v.ro[3 * jB], v.ro[3 * jB + 1], v.ro[3 * jB + 2], 0.0);
-      const int tA = v.ipt[jA], tB = v.ipt[jB];
+      const int tA = v.ipt[jA] + ((v.laz && !v.laz[jA]) ? kUncharged : 0);
+      const int tB = v.ipt[jB] + ((v.laz && !v.laz[jB]) ? kUncharged : 0);
       pair_eval<real, NTP>(hp, ht, pA, tA, a.box, rc2, true, s_coef, s_bins, s_zlo, s_invw,
                            s_r2at, s_r2um, s_last, s_tp, acc);
       if (haveB)
@@ -702,6 +708,7 @@ int launch_pair(Ctx* c, bool last) {
     v.ro = c->d_ro;
     v.az = c->d_az;
     v.ipt = c->d_ipt;
+    v.laz = c->d_laz;
     switch (c->nptpt) {
 #define MSG_VL(NTP)                                                                            \
   if (c->fp64) hipLaunchKernelGGL((k_pair_vl<double, NTP>), dim3(nb), dim3(256), lds, c->stream, v); \

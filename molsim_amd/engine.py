@@ -91,6 +91,9 @@ def lib():
         L.msg_dutotal.argtypes = [_vp, C.c_int, _ip, _dp, C.POINTER(C.c_int), _vp, _vp, _vp]
         L.msg_dutotal_batch.argtypes = [_vp, C.c_int, _ip, _dp, _ip, _dp, _vp]
         L.msg_commit.argtypes = [_vp, C.c_int, _ip, _dp]
+        L.msg_set_weak_charges.argtypes = [_vp, _ip, _dp]
+        L.msg_dutotal_charge.argtypes = [_vp, C.c_int, _ip, C.POINTER(C.c_int), _vp, _vp, _vp]
+        L.msg_commit_charge.argtypes = [_vp, C.c_int, _ip]
         L.msg_get_cell_list.argtypes = [_vp, _ip, _vp, _vp]
         L.msg_md_init.argtypes = [_vp, _dp, _vp]
         L.msg_md_run.argtypes = [_vp, C.c_int, C.c_double, _vp]
@@ -236,6 +239,8 @@ class Engine:
         ipt = np.ascontiguousarray(spec.ipt + 1, dtype=np.int32)
         _check(lib().msg_set_particles(h, az, ipt))
         self.set_positions(spec.ro)
+        if getattr(spec, "laz", None) is not None:
+            self.set_weak_charges(spec.laz, spec.pot["z"][: spec.npt])
         if spec.ewald != "none":
             self.ewald_setup()
 
@@ -311,6 +316,27 @@ class Engine:
     def mc_update(self, ip: int, rnew):
         ipa = np.array([ip + 1], dtype=np.int32)
         _check(lib().msg_commit(self.h, 1, ipa, np.ascontiguousarray(rnew, dtype=np.float64).reshape(3)))
+
+    # -- weak (titrating) charges --------------------------------------------------------------
+    def set_weak_charges(self, laz, zat):
+        """Charge states laz (N,) and full charges per type zat (npt,) (lweakcharge)."""
+        _check(lib().msg_set_weak_charges(self.h, np.ascontiguousarray(laz, dtype=np.int32),
+                                          np.ascontiguousarray(zat, dtype=np.float64)))
+
+    def dutotal_charge(self, ips) -> dict:
+        """ChargeChange trial: the charge states of particles ips (one or two, 0-based) flip."""
+        ia = np.ascontiguousarray(np.atleast_1d(ips), dtype=np.int32) + 1
+        dt = np.zeros(self.nptpt + 1)
+        ov = C.c_int()
+        dr, du = C.c_double(), C.c_double()
+        _check(lib().msg_dutotal_charge(self.h, int(ia.size), ia, C.byref(ov), _ptr(dt), C.byref(dr),
+                                        C.byref(du)))
+        return dict(overlap=bool(ov.value), du_twob=dt, du_rec=dr.value, du_tot=du.value)
+
+    def commit_charge(self, ips):
+        """Accept a ChargeChange (MCUpdate of the charge states)."""
+        ia = np.ascontiguousarray(np.atleast_1d(ips), dtype=np.int32) + 1
+        _check(lib().msg_commit_charge(self.h, int(ia.size), ia))
 
     def cell_list(self):
         """(nc, cell_start, cell_ip 0-based) of the last list build."""

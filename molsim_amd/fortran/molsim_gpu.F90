@@ -123,6 +123,33 @@ module MolsimGpu
          real(c_double), intent(in) :: rotm(3,*)
       end function
 
+      ! weak (titrating) charges, lweakcharge: laz(np) as 0/1, zat(npt) (energy.F90:7180-7276)
+      integer(c_int) function msg_set_weak_charges(ctx, laz, zat) bind(C, name='msg_set_weak_charges')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), intent(in) :: laz(*)
+         real(c_double), intent(in) :: zat(*)
+      end function
+
+      ! ChargeChange trial (mc.F90:3683-3800): charge states of ipnptm(1:nptm) flip
+      integer(c_int) function msg_dutotal_charge(ctx, nptm, ipnptm, lhsoverlap, du_twob, du_rec, du_tot) &
+            bind(C, name='msg_dutotal_charge')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nptm
+         integer(c_int), intent(in) :: ipnptm(*)
+         integer(c_int), intent(out) :: lhsoverlap
+         real(c_double), intent(out) :: du_twob(*), du_rec, du_tot
+      end function
+
+      ! MCUpdate of an accepted ChargeChange (mc.F90:3781-3785)
+      integer(c_int) function msg_commit_charge(ctx, nptm, ipnptm) bind(C, name='msg_commit_charge')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nptm
+         integer(c_int), intent(in) :: ipnptm(*)
+      end function
+
       ! VelVer on the device (md.F90:252-410): masses per type, velocities rod(3,np)
       integer(c_int) function msg_md_init(ctx, mass, vel) bind(C, name='msg_md_init')
          import :: c_int, c_double, c_ptr

@@ -56,6 +56,7 @@ class SystemSpec:
     order: int = 0
     lsurf: bool = True
     extra: dict = field(default_factory=dict)
+    laz: np.ndarray | None = None  # weak charges: charge state per particle (az = z[type] or 0)
 
     @property
     def n(self) -> int:

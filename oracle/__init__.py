@@ -86,6 +86,7 @@ class OrcSys(C.Structure):
         ("list", C.c_int), ("ewald", C.c_int), ("surf", C.c_int),
         ("alpha", C.c_double), ("ncut", C.c_int), ("region", C.c_int),
         ("K", C.c_int), ("order", C.c_int),
+        ("laz", C.POINTER(C.c_int)), ("zat", C.POINTER(C.c_double)),
     ]
 
 
@@ -128,6 +129,9 @@ def _declare(L):
     L.orc_dutotal.argtypes = [C.POINTER(OrcSys), C.c_int, _dp, C.POINTER(C.c_int), _dp,
                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.orc_dutotal.restype = C.c_int
+    L.orc_dutotal_charge.argtypes = [C.POINTER(OrcSys), C.c_int, _ip, C.POINTER(C.c_int), _dp,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.orc_dutotal_charge.restype = C.c_int
 
 
 def _check(rc, what):
@@ -282,7 +286,9 @@ class System:
     """Owns the arrays an OrcSys points at."""
 
     def __init__(self, ro, ipt, az, box, rcut, epsi4pi, table, *, npt, list_mode=1, drnlist=0.0,
-                 ewald=2, surf=True, alpha=0.0, ncut=0, region=0, K=0, order=0):
+                 ewald=2, surf=True, alpha=0.0, ncut=0, region=0, K=0, order=0, laz=None, zat=None):
+        """laz (N,) charge states and zat (npt,) full charges per type switch on weak charges;
+        az must then hold zat[type] where laz is set and 0 elsewhere, as MOLSIM keeps it."""
         self.ro = np.ascontiguousarray(ro, dtype=np.float64).reshape(-1)
         self.ipt = np.ascontiguousarray(ipt, dtype=np.int32)
         self.az = np.ascontiguousarray(az, dtype=np.float64)
@@ -304,6 +310,11 @@ class System:
         s.r2atat = t["r2atat"].ctypes.data_as(C.POINTER(C.c_double))
         s.list, s.ewald, s.surf = list_mode, ewald, int(surf)
         s.alpha, s.ncut, s.region, s.K, s.order = alpha, ncut, region, K, order
+        self.laz = None if laz is None else np.ascontiguousarray(laz, dtype=np.int32)
+        self.zat = None if zat is None else np.ascontiguousarray(zat, dtype=np.float64)
+        if self.laz is not None:
+            s.laz = self.laz.ctypes.data_as(C.POINTER(C.c_int))
+            s.zat = self.zat.ctypes.data_as(C.POINTER(C.c_double))
         self.s = s
         self.npt = npt
 
@@ -327,4 +338,15 @@ class System:
         drec, dtot = C.c_double(), C.c_double()
         _check(lib().orc_dutotal(C.byref(self.s), int(ip), np.asarray(rnew, dtype=np.float64), C.byref(ov),
                                  dt, C.byref(drec), C.byref(dtot)), "dutotal")
+        return dict(overlap=bool(ov.value), du_twob=dt, du_rec=drec.value, du_tot=dtot.value)
+
+    def dutotal_charge(self, ips):
+        """ChargeChange trial: the charge states of particles ips (one or two, 0-based) flip."""
+        nptpt = self.npt * (self.npt + 1) // 2
+        dt = np.zeros(nptpt + 1)
+        ov = C.c_int()
+        drec, dtot = C.c_double(), C.c_double()
+        ia = np.ascontiguousarray(np.atleast_1d(ips), dtype=np.int32)
+        _check(lib().orc_dutotal_charge(C.byref(self.s), int(ia.size), ia, C.byref(ov), dt,
+                                        C.byref(drec), C.byref(dtot)), "dutotal_charge")
         return dict(overlap=bool(ov.value), du_twob=dt, du_rec=drec.value, du_tot=dtot.value)

@@ -20,6 +20,8 @@
 //   orc_ewald_self/surf Src/energy.F90:1301-1371 UEwaldSelf (atomic part), UEwaldSurf
 //   orc_utotal         Src/energy.F90:280-434     UTotal, charge/monoatomic branch
 //   orc_dutotal        Src/denergy.F90:98-279,728-928,1240-1543  DUTotal single-particle move
+//   orc_dutotal_charge Src/mc.F90:3683-3800, Src/denergy.F90:154-164,1552-1699,3911-4145
+//                      ChargeChange trial with weak charges (DUWeakChargeEwald, UWeakChargeANew/AOld)
 //   orc_fft3_*         FFTW r2c/c2r conventions (unnormalised, forward e^{-i}), own radix-2 code
 //
 // Conventions: positions ro[3*i+d] (Fortran ro(3,np)), particle types 0-based, type pair
@@ -435,10 +437,13 @@ struct OrcTable {
 
 // UTwoBodyACellList, Src/energy.F90:669-777.  u_twob[1+tp] accumulates, u_twob[0] = sum.
 // force[3*np] and *virial are ACCUMULATED into (the caller zeroes them, as UTotal does).
+// With weak charges (laz non-null) a pair with an uncharged particle is skipped before
+// anything else, as UWeakChargeA does (Src/energy.F90:7197-7201).
 int orc_utwob_cell(int np, int npt, const double* ro, const int* ipt, const double* box,
                    double rcut, const double* ubuf, int nbuf, const int* iubuflow,
                    const double* r2umin, const double* r2atat, const int* nc, const double* csize,
-                   const int* stencil, int nst, double* u_twob, double* force, double* virial) {
+                   const int* stencil, int nst, double* u_twob, double* force, double* virial,
+                   const int* laz) {
   int ncell = nc[0] * nc[1] * nc[2];
   std::vector<int> head(ncell), next(np), npart(ncell), cellof(np);
   int rc = orc_cell_build(np, ro, box, nc, csize, head.data(), next.data(), npart.data(),
@@ -453,6 +458,7 @@ int orc_utwob_cell(int np, int npt, const double* ro, const int* ipt, const doub
     for (int s = 0; s < nn; ++s) {
       for (int j = head[nb[s]]; j >= 0; j = next[j]) {
         if (j <= i) continue;
+        if (laz && (!laz[i] || !laz[j])) continue;
         int tp = tp_index(npt, ipt[i], ipt[j]);
         double d[3] = {ro[3 * i] - ro[3 * j], ro[3 * i + 1] - ro[3 * j + 1],
                        ro[3 * i + 2] - ro[3 * j + 2]};
@@ -482,12 +488,14 @@ int orc_utwob_cell(int np, int npt, const double* ro, const int* ipt, const doub
 // hard-core pairs get usum = fsum = 1e10 as in the cell-list routine).
 int orc_utwob_all(int np, int npt, const double* ro, const int* ipt, const double* box, double rcut,
                   const double* ubuf, int nbuf, const int* iubuflow, const double* r2umin,
-                  const double* r2atat, double* u_twob, double* force, double* virial) {
+                  const double* r2atat, double* u_twob, double* force, double* virial,
+                  const int* laz) {
   int nptpt = npt * (npt + 1) / 2;
   for (int k = 0; k <= nptpt; ++k) u_twob[k] = 0;
   double vir = 0, rcut2 = rcut * rcut;
   for (int i = 0; i < np; ++i)
     for (int j = i + 1; j < np; ++j) {
+      if (laz && (!laz[i] || !laz[j])) continue;
       int tp = tp_index(npt, ipt[i], ipt[j]);
       double d[3] = {ro[3 * i] - ro[3 * j], ro[3 * i + 1] - ro[3 * j + 1],
                      ro[3 * i + 2] - ro[3 * j + 2]};
@@ -921,6 +929,10 @@ struct OrcSys {
   double alpha;
   int ncut, region;  // std
   int K, order;      // spm
+  // weak (titrating) charges: laz[i] = 1 if particle i carries its charge zat[type]; null
+  // when every particle is charged (lweakcharge off)
+  const int* laz;
+  const double* zat;
 };
 
 // UTotal charge/monoatomic branch, Src/energy.F90:280-434 (forces left unscaled, i.e. in
@@ -939,10 +951,10 @@ int orc_utotal(const OrcSys* S, double* u_twob, double* u_rec, double* u_tot, do
     int nst = orc_cell_init(S->box, S->rcut + S->drnlist, S->rcut, nc, cs, st.data(), 1000);
     if (nst < 0) return nst;
     rc = orc_utwob_cell(np, S->npt, S->ro, S->ipt, S->box, S->rcut, S->ubuf, S->nbuf, S->iubuflow,
-                        S->r2umin, S->r2atat, nc, cs, st.data(), nst, u_twob, force, virial);
+                        S->r2umin, S->r2atat, nc, cs, st.data(), nst, u_twob, force, virial, S->laz);
   } else {
     rc = orc_utwob_all(np, S->npt, S->ro, S->ipt, S->box, S->rcut, S->ubuf, S->nbuf, S->iubuflow,
-                       S->r2umin, S->r2atat, u_twob, force, virial);
+                       S->r2umin, S->r2atat, u_twob, force, virial, S->laz);
   }
   if (rc) return rc;
   double urec = 0, virrec = 0;
@@ -1021,6 +1033,9 @@ int orc_dutotal(const OrcSys* S, int ip, const double* rnew, int* overlap, doubl
           pbc_r2(S->box, d, r2);
           if (!(r2 < rcut2)) continue;
           if (pass == 0 && (r2 < S->r2atat[tp] || r2 < S->r2umin[tp])) return 0;  // overlap
+          // weak charges: UWeakChargeANew/AOld skip uncharged pairs after the overlap checks
+          // (Src/denergy.F90:3955-3959, 4066-4068)
+          if (S->laz && (!S->laz[ip] || !S->laz[j])) continue;
           double us, fs;
           int e = table_uf(S->ubuf, S->nbuf, S->iubuflow[tp], r2, 0.0, S->r2umin[tp], us, fs);
           if (e) return e;
@@ -1038,6 +1053,7 @@ int orc_dutotal(const OrcSys* S, int ip, const double* rnew, int* overlap, doubl
         pbc_r2(S->box, d, r2);
         if (r2 > rcut2) continue;
         if (pass == 0 && (r2 < S->r2atat[tp] || r2 < S->r2umin[tp])) return 0;
+        if (S->laz && (!S->laz[ip] || !S->laz[j])) continue;
         double us, fs;
         int e = table_uf(S->ubuf, S->nbuf, S->iubuflow[tp], r2, 0.0, S->r2umin[tp], us, fs);
         if (e) return e;
@@ -1107,6 +1123,105 @@ int orc_dutotal(const OrcSys* S, int ip, const double* rnew, int* overlap, doubl
     for (int d = 0; d < 3; ++d) mt[d] = m[d] + S->az[ip] * (rnew[d] - S->ro[3 * ip + d]);
     drec += fac * ((mt[0] * mt[0] + mt[1] * mt[1] + mt[2] * mt[2]) -
                    (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]));
+  }
+  *du_rec = S->epsi4pi * drec;
+  *du_tot = du_twob[0] + *du_rec;
+  return 0;
+}
+
+// ChargeChange trial (Src/mc.F90:3683-3800) of a system with weak charges: the charge states of
+// the nptm (1 or 2) particles ips[] flip (laztm = .not.laz, aztm = zat[type] or 0), positions
+// stay.  DUTotal's weak-charge branch (Src/denergy.F90:154-164): real space over the flipped
+// particles' partners with the new and the old states (UWeakChargeANew/AOld, :3911-4145; the
+// pair of the two flipped particles counted once, lptmdutwob), reciprocal space from the
+// changed structure factors (DUWeakChargeEwaldRecStd, :1586-1650), the self term (:1654-1667)
+// and the surface term (:1671-1695).  Std Ewald or none: the reference stops with SPME.
+int orc_dutotal_charge(const OrcSys* S, int nptm, const int* ips, int* overlap, double* du_twob,
+                       double* du_rec, double* du_tot) {
+  if (!S->laz || !S->zat || nptm < 1 || nptm > 2 || S->ewald == 2) return ORC_ERR_ARG;
+  const int np = S->np, npt = S->npt, nptpt = npt * (npt + 1) / 2;
+  const bool inclusive = S->list != 1;  // all pairs: UTwoBodyA's r2 <= rcut2
+  const double rcut2 = S->rcut * S->rcut;
+  std::vector<int> lazn(S->laz, S->laz + np);
+  std::vector<double> azn(S->az, S->az + np);
+  std::vector<char> moved(np, 0);
+  for (int m = 0; m < nptm; ++m) {
+    const int i = ips[m];
+    if (i < 0 || i >= np) return ORC_ERR_ARG;
+    moved[i] = 1;
+    lazn[i] = !S->laz[i];
+    azn[i] = lazn[i] ? S->zat[S->ipt[i]] : 0.0;
+  }
+  std::vector<double> unew(nptpt + 1, 0.0), uold(nptpt + 1, 0.0);
+  *overlap = 1;
+  *du_rec = 0;
+  *du_tot = 0;
+  for (int k = 0; k <= nptpt; ++k) du_twob[k] = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int* lz = pass == 0 ? lazn.data() : S->laz;
+    for (int m = 0; m < nptm; ++m) {
+      const int i = ips[m];
+      for (int j = 0; j < np; ++j) {
+        if (j == i || (moved[j] && j < i)) continue;
+        const int tp = tp_index(npt, S->ipt[i], S->ipt[j]);
+        double d[3] = {S->ro[3 * i] - S->ro[3 * j], S->ro[3 * i + 1] - S->ro[3 * j + 1],
+                       S->ro[3 * i + 2] - S->ro[3 * j + 2]};
+        double r2;
+        pbc_r2(S->box, d, r2);
+        if (inclusive ? r2 > rcut2 : !(r2 < rcut2)) continue;
+        if (r2 < S->r2atat[tp] || r2 < S->r2umin[tp]) return 0;  // overlap
+        if (!lz[i] || !lz[j]) continue;
+        double us, fs;
+        const int e = table_uf(S->ubuf, S->nbuf, S->iubuflow[tp], r2, 0.0, S->r2umin[tp], us, fs);
+        if (e) return e;
+        (pass == 0 ? unew : uold)[1 + tp] += us;
+      }
+    }
+  }
+  *overlap = 0;
+  double s = 0;
+  for (int k = 1; k <= nptpt; ++k) {
+    du_twob[k] = unew[k] - uold[k];
+    s += du_twob[k];
+  }
+  du_twob[0] = s;
+  double drec = 0;
+  if (S->ewald == 1) {
+    const int H = S->ncut + 1;
+    std::vector<double> kfac(4 * H * H * H);
+    const int nk = orc_ewald_std_kfac(S->box, S->alpha, S->ncut, S->region, kfac.data(), (int)kfac.size());
+    if (nk < 0) return nk;
+    std::vector<double> s0(8 * nk), s1(8 * nk), f(3 * np, 0.0);
+    double u0, u1, v = 0;
+    orc_ewald_std_rec(np, S->ro, S->az, S->box, S->alpha, S->ncut, S->region, S->epsi4pi, kfac.data(),
+                      &u0, f.data(), &v, s0.data());
+    orc_ewald_std_rec(np, S->ro, azn.data(), S->box, S->alpha, S->ncut, S->region, S->epsi4pi,
+                      kfac.data(), &u1, f.data(), &v, s1.data());
+    for (int kn = 0; kn < nk; ++kn) {
+      double tn = 0, to = 0;
+      for (int q = 0; q < 4; ++q) {
+        tn += s1[2 * (4 * kn + q)] * s1[2 * (4 * kn + q)] + s1[2 * (4 * kn + q) + 1] * s1[2 * (4 * kn + q) + 1];
+        to += s0[2 * (4 * kn + q)] * s0[2 * (4 * kn + q)] + s0[2 * (4 * kn + q) + 1] * s0[2 * (4 * kn + q) + 1];
+      }
+      drec += kfac[kn] * (tn - to);
+    }
+  }
+  if (S->ewald) {
+    const double fac = S->alpha / std::sqrt(kPi);
+    for (int m = 0; m < nptm; ++m) {
+      const int i = ips[m];
+      drec += fac * (S->az[i] * S->az[i] - azn[i] * azn[i]);
+    }
+    if (S->surf) {
+      const double vol = S->box[0] * S->box[1] * S->box[2], sf = 2 * kPi / (3 * vol);
+      double m0[3] = {0, 0, 0}, m1[3] = {0, 0, 0};
+      for (int a = 0; a < np; ++a)
+        for (int dd = 0; dd < 3; ++dd) {
+          m0[dd] += S->az[a] * S->ro[3 * a + dd];
+          m1[dd] += azn[a] * S->ro[3 * a + dd];
+        }
+      drec += sf * ((m1[0] * m1[0] + m1[1] * m1[1] + m1[2] * m1[2]) - (m0[0] * m0[0] + m0[1] * m0[1] + m0[2] * m0[2]));
+    }
   }
   *du_rec = S->epsi4pi * drec;
   *du_tot = du_twob[0] + *du_rec;

@@ -20,10 +20,16 @@ def engine(sp, tab, precision="mixed"):
 
 
 def oracle_system(sp, tab):
-    return oracle.System(sp.ro, sp.ipt, sp.az, sp.box, sp.rcut, sp.pot["epsi4pi"], tab, npt=sp.npt,
-                         list_mode=LIST[sp.list_mode], drnlist=sp.drnlist, ewald=EWALD[sp.ewald],
+    """The oracle's system of a spec.  A Verlet-list spec is evaluated over all pairs (the same
+    pairs, r^2 <= rcut^2); weak charges (sp.laz) carry over with az = z[type] or 0."""
+    laz = getattr(sp, "laz", None)
+    zat = None if laz is None else np.asarray(sp.pot["z"][: sp.npt], dtype=np.float64)
+    az = sp.az if laz is None else np.where(np.asarray(laz) != 0, zat[sp.ipt], 0.0)
+    return oracle.System(sp.ro, sp.ipt, az, sp.box, sp.rcut, sp.pot["epsi4pi"], tab, npt=sp.npt,
+                         list_mode=LIST.get(sp.list_mode, 0), drnlist=sp.drnlist, ewald=EWALD[sp.ewald],
                          surf=sp.lsurf, alpha=sp.alpha, ncut=sp.ncut,
-                         region={"sphere": 0, "cube": 1}[sp.ncutregion], K=sp.nmesh, order=sp.order)
+                         region={"sphere": 0, "cube": 1}[sp.ncutregion], K=sp.nmesh, order=sp.order,
+                         laz=laz, zat=zat)
 
 
 def rel_force_err(f, ref):

@@ -1,0 +1,140 @@
+"""GPU parity of the weak-charge (titrating) branch against the FP64 oracle: UTotal with
+uncharged particles (UWeakChargeA, Src/energy.F90:7180-7276) in the all-pairs, cell-list and
+Verlet-list modes, displacement trials, ChargeChange trials of one or two particles
+(Src/mc.F90:3683-3800) with their commit, and a host-driven MC pass.  Tolerances as the other
+parity tests: mixed path 1e-5 (forces, energies against the energy scale), FP64 path 1e-9;
+trial energies 2e-5 kJ/mol + 1e-6 relative (mixed) and 1e-8 kJ/mol (FP64)."""
+import numpy as np
+import pytest
+
+import oracle
+from molsim_amd import systems
+from molsim_amd.engine import Engine, MolsimGpuError, Random
+from gpu_helpers import cabi_table, compare, oracle_system
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"mixed": (1e-5, 1e-5, 2e-5, 1e-6), "fp64": (1e-9, 1e-9, 1e-8, 0.0)}
+
+
+def _weak(list_mode="cell", seed=3, frac=0.6, n=300, L=40.0):
+    rcut = 12.0 if list_mode != "all" else 19.9
+    sp = systems.small_spme_system(seed=seed, n=n, L=L, alpha=0.3, rcut=rcut, list_mode=list_mode)
+    zat = np.asarray(sp.pot["z"][: sp.npt], dtype=np.float64)
+    laz = (np.random.default_rng(seed).random(sp.n) < frac).astype(np.int32)
+    az = np.where(laz == 1, zat[sp.ipt], 0.0)
+    kw = dict(ewald="std", ncut=6, laz=laz, az=az)
+    if list_mode == "verlet":
+        kw["drnlist"] = 1.0
+    return systems.SystemSpec(**{**sp.__dict__, **kw})
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("list_mode", ["all", "cell", "verlet"])
+def test_weak_utotal(list_mode, precision):
+    sp = _weak(list_mode)
+    print(f"weak utotal list={list_mode} precision={precision} n={sp.n} charged={int(sp.laz.sum())}", flush=True)
+    tab = oracle.build_table(sp.pot)
+    ref = oracle_system(sp, tab).utotal()
+    eng = Engine(sp, precision=precision, table=cabi_table(tab))
+    got = eng.utotal()
+    ftol, etol = TOL[precision][:2]
+    compare(got, ref, ftol, etol)
+    # an uncharged particle feels no force at all
+    assert np.all(np.abs(got["force"][sp.laz == 0]) < 1e-12)
+    eng.close()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_weak_displacement_trials(precision):
+    sp = _weak("cell")
+    print(f"weak displacement trials precision={precision}", flush=True)
+    tab = oracle.build_table(sp.pot)
+    osys = oracle_system(sp, tab)
+    eng = Engine(sp, precision=precision, table=cabi_table(tab))
+    rng = np.random.default_rng(11)
+    ips = rng.integers(0, sp.n, size=32)
+    rn = (sp.ro[ips] + rng.uniform(-1, 1, (32, 3)) + 20.0) % 40.0 - 20.0
+    got = eng.dutotal_batch(ips, rn)
+    atol, rtol = TOL[precision][2:]
+    nchk = 0
+    for t in range(len(ips)):
+        ref = osys.dutotal(int(ips[t]), rn[t])
+        assert got["overlap"][t] == ref["overlap"]
+        if ref["overlap"]:
+            continue
+        assert abs(got["du_tot"][t] - ref["du_tot"]) <= atol + rtol * abs(ref["du_tot"]), t
+        nchk += 1
+    assert nchk >= 16
+    eng.close()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("list_mode", ["all", "cell"])
+def test_charge_change_and_commit(list_mode, precision):
+    sp = _weak(list_mode)
+    print(f"charge change list={list_mode} precision={precision}", flush=True)
+    tab = oracle.build_table(sp.pot)
+    eng = Engine(sp, precision=precision, table=cabi_table(tab))
+    atol, rtol = TOL[precision][2:]
+    laz = sp.laz.copy()
+    # one and two flips against the same state, then accept some and check again
+    for step, flip in enumerate([(4,), (7, 19), (2,), (11, 12), (30,), (5, 6)]):
+        cur = systems.SystemSpec(**{**sp.__dict__, "laz": laz.copy()})
+        ref = oracle_system(cur, tab).dutotal_charge(list(flip))
+        got = eng.dutotal_charge(list(flip))
+        assert got["overlap"] == ref["overlap"] is False
+        assert abs(got["du_tot"] - ref["du_tot"]) <= atol + rtol * abs(ref["du_tot"]), (flip, got, ref)
+        assert abs(got["du_rec"] - ref["du_rec"]) <= atol + rtol * abs(ref["du_rec"])
+        assert np.allclose(got["du_twob"], ref["du_twob"], atol=atol * 10)
+        if step % 2 == 0:
+            eng.commit_charge(list(flip))
+            for i in flip:
+                laz[i] = 1 - laz[i]
+    # the state after the accepted flips equals a fresh evaluation
+    cur = systems.SystemSpec(**{**sp.__dict__, "laz": laz.copy()})
+    compare(eng.utotal(), oracle_system(cur, tab).utotal(), *TOL[precision][:2])
+    eng.close()
+
+
+def test_weak_mc_pass_matches_oracle_loop():
+    """MCPass of displacements for a weak-charge system runs from the host (msg_dutotal
+    honours the states); FP64: the same decisions and positions as an oracle-driven loop."""
+    import math
+    sp = _weak("cell", n=200, L=36.0)
+    tab = oracle.build_table(sp.pot)
+    eng = Engine(sp, precision="fp64", table=cabi_table(tab))
+    ntrial, dtran, beta = 150, 2.0, systems.beta_kjmol()
+    rng = Random(-4711)
+    _, nacc, ev = eng.mc_pass(ntrial, dtran, beta, rng)
+    orng = oracle.Rng(-4711)
+    ro = sp.ro.copy()
+    L = sp.box[0]
+    ref_ev = []
+    for t in range(ntrial):
+        ip = t % sp.n
+        rn = ro[ip] + np.array([(orng() - 0.5) * dtran for _ in range(3)])
+        rn = np.where(np.abs(rn) > L / 2, rn - L * np.round(rn / L), rn)
+        osys = oracle_system(systems.SystemSpec(**{**sp.__dict__, "ro": ro}), tab)
+        du = osys.dutotal(ip, rn)
+        if du["overlap"]:
+            acc = False
+        else:
+            dured = beta * du["du_tot"]
+            acc = dured < -87.0 or (dured <= 87.0 and (math.exp(-dured) > 1.0 or math.exp(-dured) > orng()))
+        ref_ev.append(-1 if du["overlap"] else int(acc))
+        if acc:
+            ro[ip] = rn
+    assert np.array_equal(ev, np.array(ref_ev))
+    assert np.abs(eng.positions() - ro).max() < 1e-9
+    assert 0 < nacc < ntrial
+    eng.close()
+
+
+def test_weak_charges_refuse_spme():
+    sp = systems.small_spme_system(n=50, L=30.0, nmesh=16, alpha=0.3, rcut=10.0)
+    eng = Engine(sp)
+    with pytest.raises(MolsimGpuError) as e:
+        eng.set_weak_charges(np.ones(sp.n, dtype=np.int32), sp.pot["z"][: sp.npt])
+    assert e.value.code == -1
+    eng.close()
