@@ -208,7 +208,10 @@ int utotal_sequence(Ctx* c) {
   // as DUTotal complete it on demand)
   const bool cells_needed = c->P.list_mode != MSG_LIST_VERLET || rebuild;
   auto cell_work = [&]() -> int {
-    if (int e2 = launch_sort(c, branch ? 1 : 3)) return e2;
+    // without a rebuild the Verlet pair kernel reads positions in particle order, so the
+    // cell slots are not sorted at all (complete_cells does it for later users)
+    const int which = (cells_needed ? 1 : 0) | (branch ? 0 : 2);
+    if (int e2 = launch_sort(c, which)) return e2;
     if (cells_needed)
       if (int e2 = launch_finalize(c)) return e2;
     if (rebuild) {
@@ -252,6 +255,18 @@ int utotal_sequence(Ctx* c) {
 }
 
 static int flags_to_error(int f);
+
+// The cell order of the current positions for users that need it after a Verlet-list pass
+// that left it out: the binning runs again on the same positions (same cells, same counts),
+// the cells are sorted and ranked, and the counters the binning filled are cleared.
+int complete_cells(Ctx* c) {
+  if (c->cells_valid) return MSG_OK;
+  int e;
+  if ((e = launch_bin(c)) || (e = launch_sort(c, 1)) || (e = launch_finalize(c)) || (e = launch_zero(c)))
+    return e;
+  c->cells_valid = true;
+  return MSG_OK;
+}
 
 int utotal_refresh(Ctx* c) {
   if (int e = utotal_sequence(c)) return e;
@@ -368,7 +383,7 @@ void msg_destroy(msg_ctx* h) {
                   c->d_frac, c->d_Q, c->d_C, c->d_G, c->d_tw, c->d_kfac, c->d_kvec, c->d_sk,
                   c->d_force, c->d_part, c->d_out, c->d_flags, c->d_de, c->d_de_ip, c->d_de_r,
                   c->d_tcoef64, c->d_gk, c->d_item_cell, c->d_bq, c->d_bmm, c->d_M,
-                  c->d_vl_home, c->d_vl_n, c->d_vl_nbr, c->d_ro_ref, c->d_dmax, c->d_vel,
+                  c->d_vl_home, c->d_vl_n, c->d_vl_nbr, c->d_vl_pos, c->d_vl_slot, c->d_ro_ref, c->d_dmax, c->d_vel,
                   c->d_acc, c->d_minv, c->d_ke_part, c->d_rank, c->d_laz};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -418,6 +433,9 @@ int msg_set_particles(msg_ctx* h, const double* az, const int* iptpn) {
   c->h_ipt = t;
   c->h_az.assign(az, az + c->np);
   c->have_particles = true;
+  c->graph_ok = false;   // the captured reduction holds the self term
+  c->fresh = false;
+  c->vl_valid = false;   // the Verlet list's records hold the types
   return MSG_OK;
 }
 
@@ -443,6 +461,7 @@ int msg_set_weak_charges(msg_ctx* h, const int* laz, const double* zat) {
   MSG_HIP(hipMemcpy(c->d_az, c->h_az.data(), c->np * sizeof(double), hipMemcpyHostToDevice));
   c->graph_ok = false;  // the captured pass must see the charge states
   c->fresh = false;
+  c->vl_valid = false;  // the Verlet list's records hold the charge states
   return MSG_OK;
 }
 
@@ -465,6 +484,7 @@ int msg_commit_charge(msg_ctx* h, int nptm, const int* ipnptm) {
   }
   c->graph_ok = false;  // self term of the captured reduction
   c->fresh = false;     // structure factors and the cell records' charge tags are rebuilt
+  c->vl_valid = false;  // so are the Verlet list's (rebuilt at the next pass)
   return MSG_OK;
 }
 
@@ -630,10 +650,7 @@ int msg_get_cell_list(msg_ctx* h, int* nc, int* cell_start, int* cell_ip) {
   if (!h || !nc) return set_error(MSG_ERR_ARG, "msg_get_cell_list: null argument");
   Ctx* c = &h->c;
   for (int d = 0; d < 3; ++d) nc[d] = c->grid.nc[d];
-  if (!c->cells_valid) {  // complete the cell order of the last pass
-    if (int e = launch_finalize(c)) return e;
-    c->cells_valid = true;
-  }
+  if (int e = complete_cells(c)) return e;
   MSG_HIP(hipStreamSynchronize(c->stream));
   if (cell_start)
     MSG_HIP(hipMemcpy(cell_start, c->d_cell_start, (c->grid.ncell + 1) * sizeof(int), hipMemcpyDeviceToHost));

@@ -36,6 +36,10 @@ constexpr int kTabBins = 256;        // r^2 bins of the interval index per type 
 // Weak charges: the type field of an uncharged particle's cell-ordered record (cpos.w) carries
 // this offset, so the pair kernels see the charge state in the record they load anyway.
 constexpr int kUncharged = 8;
+// The Verlet-list displacement maximum is kept in kDmaxSlots words kDmaxStride words apart
+// (separate cache lines): workgroup b adds its maximum to slot b mod kDmaxSlots, so the atomics
+// of a pass over all particles spread over many addresses instead of queueing at one.
+constexpr int kDmaxSlots = 64, kDmaxStride = 64;
 constexpr int kMaxTabIntervals = 512;   // records below rcut staged in LDS (<= 48 KB in FP64)
 constexpr int kMaxStencil = 27;
 constexpr int kPairHB = 4;           // home particles per pair-kernel work item
@@ -179,7 +183,7 @@ struct Ctx {
   int prof_n[MSG_K_COUNT] = {0};
 
   // Verlet list (MSG_LIST_VERLET): full list at rcut + drnlist built from the cell list;
-  // neighbours are particle indices, column-major [maxn][np]; home order = cell order at build
+  // home order = cell order at build ("slots"); neighbours are slots, column-major [maxn][np]
   int vl_maxn = 0;
   std::vector<int> h_de_ip;  // staged trial indices (host)
   int pair_lph = 32;        // pair tile: lanes per home particle (8..64), 0 = wave-item kernel
@@ -187,9 +191,12 @@ struct Ctx {
   int* d_vl_home = nullptr;
   int* d_vl_n = nullptr;
   int* d_vl_nbr = nullptr;
+  double* d_vl_pos = nullptr;    // double4 per slot: position, type (+ kUncharged when uncharged)
+  int* d_vl_slot = nullptr;      // slot of each particle
   double* d_ro_ref = nullptr;    // positions at the last build
-  unsigned int* d_dmax = nullptr;  // max displacement^2 since the build (float bits)
-  unsigned int* h_dmax = nullptr;  // pinned
+  unsigned int* d_dmax = nullptr;  // max displacement^2 since the build (float bits), kDmaxSlots
+                                   // slots kDmaxStride apart; the maximum is that of all slots
+  unsigned int* h_dmax = nullptr;  // pinned copy of the slots
   bool vl_valid = false;
   bool disp_done = false;        // the MD drift kernel already computed the displacement
   long long vl_builds = 0;
@@ -238,6 +245,7 @@ int launch_ewstd(Ctx* c);
 int launch_reduce(Ctx* c, bool presum);           // final energies (presum: pair rows pre-summed)
 int launch_pair_sum(Ctx* c);                       // pair rows summed into one row
 int launch_zero(Ctx* c);                           // counters and flags
+int complete_cells(Ctx* c);                        // cell order of the current positions, if a pass left it out
 int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ovl,
                   double* du_tot, double* du_rec, double* du_twob);
 int commit_move(Ctx* c, int ip, const double* rnew);

@@ -529,10 +529,7 @@ int dutotal_launch(Ctx* c, int ntrial, const int* ip, const double* rotm) {
     if (int e = make_gk(c)) return e;
   if (!c->fresh)
     if (int e = utotal_refresh(c)) return e;
-  if (!c->cells_valid) {  // a Verlet-list pass left the cell order of its positions unranked
-    if (int e = launch_finalize(c)) return e;
-    c->cells_valid = true;
-  }
+  if (int e = complete_cells(c)) return e;  // a Verlet-list pass may have left the cell order out
   if (ntrial > c->de_cap) {
     MSG_HIP(hipStreamSynchronize(c->stream));
     if (c->d_de) { hipFree(c->d_de); hipFree(c->d_de_ip); hipFree(c->d_de_r); }
@@ -689,10 +686,7 @@ int dutotal_charge(Ctx* c, int nptm, const int* ip1, int* ovl, double* du_twob, 
   if (nptm == 2 && ip1[0] == ip1[1]) return set_error(MSG_ERR_ARG, "ChargeChange: the same particle twice");
   if (!c->fresh)
     if (int e = utotal_refresh(c)) return e;
-  if (!c->cells_valid) {
-    if (int e = launch_finalize(c)) return e;
-    c->cells_valid = true;
-  }
+  if (int e = complete_cells(c)) return e;
   if (c->de_cap < 1) {
     MSG_HIP(hipStreamSynchronize(c->stream));
     if (int e = dalloc(c, (void**)&c->d_de, (size_t)256 * (kMaxTP + 4) * sizeof(double))) return e;

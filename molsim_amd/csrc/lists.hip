@@ -36,9 +36,9 @@ struct BinArgs {
   int* brick_count;
   double* dip_part;
   int* flags;
-  int* rank;        // 2*np: each particle's place in its cell, then in its brick, as the
-                    // counters hand them out (the scatter needs no counters of its own);
-                    // null where the scatter keeps its own counters (batched passes)
+  int* rank_cell;   // each particle's place in its cell / brick as the counters hand them
+  int* rank_brick;  // out, for a role whose scatter takes slots from them (null where the
+                    // scatter keeps its own counters: large grids, batched passes)
 };
 
 template <typename real>
@@ -60,8 +60,8 @@ __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
     if (bad) atomicOr(a.flags, FLAG_CELL);
     const int c = ci[0] + a.g.nc[0] * (ci[1] + a.g.nc[1] * ci[2]);
     a.cellid[i] = c;
-    if (a.rank)
-      a.rank[i] = atomicAdd(&a.cell_count[c], 1);
+    if (a.rank_cell)
+      a.rank_cell[i] = atomicAdd(&a.cell_count[c], 1);
     else
       atomicAdd(&a.cell_count[c], 1);
     if (a.use_mesh) {
@@ -77,8 +77,8 @@ __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
       }
       a.mm[i] = mi[0] | (mi[1] << 10) | (mi[2] << 20);
       const int b = mi[0] / kBX + a.m.nb[0] * (mi[1] / kBY + a.m.nb[1] * (mi[2] / kBZ));
-      if (a.rank)
-        a.rank[a.np + i] = atomicAdd(&a.brick_count[b], 1);
+      if (a.rank_brick)
+        a.rank_brick[i] = atomicAdd(&a.brick_count[b], 1);
       else
         atomicAdd(&a.brick_count[b], 1);
     }
@@ -501,6 +501,12 @@ static Box make_box(const msg_params& P) {
   return b;
 }
 
+// Fused scatter (offsets scanned per workgroup, slots from k_bin's ranks) for a role whose
+// counts fit in LDS: 1 cells, 2 bricks.
+static bool sort_fused(const Ctx* c, int role) {
+  return role == 1 ? c->grid.ncell <= kLsMax : c->mesh.nbrick <= kLsMax;
+}
+
 static BinArgs bin_args(Ctx* c) {
   BinArgs a;
   a.np = c->np;
@@ -517,7 +523,8 @@ static BinArgs bin_args(Ctx* c) {
   a.brick_count = c->d_brick_count;
   a.dip_part = c->d_part + c->part_off_dip;
   a.flags = c->d_flags;
-  a.rank = c->d_rank;
+  a.rank_cell = sort_fused(c, 1) ? c->d_rank : nullptr;
+  a.rank_brick = (a.use_mesh && sort_fused(c, 2)) ? c->d_rank + c->np : nullptr;
   return a;
 }
 
@@ -598,11 +605,6 @@ int launch_bin(Ctx* c) {
 // Scan and scatter.  which = 1: cell slots (the cell order is completed by launch_finalize);
 // 2: brick slots, brick-ordered spline data and the total dipole (the mesh branch may start
 // after these); 3: both.
-// Fused scatter (offsets scanned per workgroup) for one role, when its counts fit in LDS.
-static bool sort_fused(const Ctx* c, int role) {
-  return role == 1 ? c->grid.ncell <= kLsMax : c->mesh.nbrick <= kLsMax;
-}
-
 static void launch_scatter_ls(Ctx* c, int role, bool dipole) {
   ScatterLsArgs a;
   a.s = scatter_args(c, role);
@@ -611,7 +613,7 @@ static void launch_scatter_ls(Ctx* c, int role, bool dipole) {
   a.cnt = role == 1 ? c->d_cell_count : c->d_brick_count;
   a.start = role == 1 ? c->d_cell_start : c->d_brick_start;
   a.chunk = c->d_cell_chunk;
-  a.rank = c->d_rank + (role == 1 ? 0 : c->np);
+  a.rank = c->d_rank + (role == 1 ? 0 : c->np);  // as bin_args hands them to k_bin
   a.dip_part = c->d_part + c->part_off_dip;
   a.ndip = c->npart_dip;
   a.M = c->d_M;
@@ -625,27 +627,37 @@ static void launch_scatter_ls(Ctx* c, int role, bool dipole) {
 
 int launch_sort(Ctx* c, int which) {
   const bool mesh = c->P.ewald_mode == MSG_EWALD_SPM;
-  if ((!(which & 1) || sort_fused(c, 1)) && (!(which & 2) || !mesh || sort_fused(c, 2))) {
+  // per role: the fused scatter where the counts fit in LDS, else scan + scatter
+  const bool fc = (which & 1) && sort_fused(c, 1);
+  const bool fb = (which & 2) && mesh && sort_fused(c, 2);
+  const bool oc = (which & 1) && !fc;
+  const bool ob = (which & 2) && mesh && !fb;
+  // the total dipole (which & 2) goes with the brick role; without a mesh, with the cell role
+  const bool dip_fc = (which & 2) && !mesh && fc;
+  const bool dip_old = (which & 2) && !fb && !dip_fc;
+  if (fc || fb) {
     prof_begin(c, MSG_K_SCATTER);
-    // the total dipole goes with the brick role; without a mesh, with the cell role
-    if (which & 1) launch_scatter_ls(c, 1, (which & 2) && !mesh);
-    if ((which & 2) && mesh) launch_scatter_ls(c, 2, true);
+    if (fc) launch_scatter_ls(c, 1, dip_fc);
+    if (fb) launch_scatter_ls(c, 2, true);
     prof_end(c, MSG_K_SCATTER);
-    MSG_HIP(hipGetLastError());
-    return MSG_OK;
   }
-  const int np = c->np;
-  const int bid0 = which == 2 ? 1 : 0;
-  const int nscan = which == 1 ? 1 : (which == 2 ? 2 : 3);
-  prof_begin(c, MSG_K_SCAN);
-  hipLaunchKernelGGL(k_scan, dim3(nscan), dim3(1024), 0, c->stream, scan_args(c, bid0));
-  prof_end(c, MSG_K_SCAN);
-  prof_begin(c, MSG_K_SCATTER);
-  if (c->fp64)
-    hipLaunchKernelGGL(k_scatter<double>, dim3((np + 255) / 256), dim3(256), 0, c->stream, scatter_args(c, which));
-  else
-    hipLaunchKernelGGL(k_scatter<float>, dim3((np + 255) / 256), dim3(256), 0, c->stream, scatter_args(c, which));
-  prof_end(c, MSG_K_SCATTER);
+  if (oc || ob || dip_old) {
+    // scan roles: 0 cells, 1 bricks, 2 dipole; one launch over the contiguous range needed
+    const int r0 = oc ? 0 : (ob ? 1 : 2), r1 = dip_old ? 2 : (ob ? 1 : 0);
+    prof_begin(c, MSG_K_SCAN);
+    hipLaunchKernelGGL(k_scan, dim3(r1 - r0 + 1), dim3(1024), 0, c->stream, scan_args(c, r0));
+    prof_end(c, MSG_K_SCAN);
+    const int parts = (oc ? 1 : 0) | (ob ? 2 : 0);
+    if (parts) {
+      const int np = c->np;
+      prof_begin(c, MSG_K_SCATTER);
+      if (c->fp64)
+        hipLaunchKernelGGL(k_scatter<double>, dim3((np + 255) / 256), dim3(256), 0, c->stream, scatter_args(c, parts));
+      else
+        hipLaunchKernelGGL(k_scatter<float>, dim3((np + 255) / 256), dim3(256), 0, c->stream, scatter_args(c, parts));
+      prof_end(c, MSG_K_SCATTER);
+    }
+  }
   MSG_HIP(hipGetLastError());
   return MSG_OK;
 }
@@ -679,7 +691,7 @@ int plan_bin(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out) {
   SegBuilder<BinArgs> b;
   for (Ctx* c : cs) {
     BinArgs a = bin_args(c);
-    a.rank = nullptr;  // the batched scatter counts slots itself
+    a.rank_cell = a.rank_brick = nullptr;  // the batched scatter counts slots itself
     b.add(c->fp64, c->fp64 ? seg_bin<double> : seg_bin<float>, a, (c->np + 255) / 256);
   }
   return b.finish(own, out);

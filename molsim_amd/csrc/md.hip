@@ -1,12 +1,14 @@
 // Verlet-list displacement test (NList skin rule, Src/nlist.F90:597-621) and the
 // velocity-Verlet caller of UTotal (VelVer, Src/md.F90:252-410), on the device.
 //
-//   k_disp   max |ro - ro_ref|^2 over all particles since the last list build (minimum image).
+//   k_disp   max |ro - ro_ref|^2 over all particles since the last list build (minimum image);
+//            copies the positions into the list's slot-ordered records.
 //            The reference rebuilds when sqrt(r2max1) + sqrt(r2max2) > drnlist over its two
 //            largest accumulated displacements; here the list is rebuilt when
 //            2 * max |dr| > drnlist, which is never later than the reference's rule.
 //   k_drift  ro += dt v + dt^2/2 a, wrapped into the box (PBC, Src/md.F90:289-298), and the
-//            displacement maximum of the moved positions.
+//            displacement maximum of the moved positions (also written into the list's
+//            slot-ordered records).
 //   k_kick   a = F / m, v += dt/2 (a_old + a) (Src/md.F90:348-358), kinetic-energy partials.
 //   k_mdlog  per-step record of U_tot and the kinetic energy.
 #include <cstring>
@@ -27,23 +29,37 @@ __device__ inline float wave_max(float v) {
   return v;
 }
 
+// The workgroup's maximum (256 lanes) into its slot: one atomic per workgroup.
+__device__ inline void dmax_add(unsigned int* dmax, float d2) {
+  __shared__ float sh[4];
+  d2 = wave_max(d2);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = d2;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomic_max_pos(dmax + (blockIdx.x % kDmaxSlots) * kDmaxStride,
+                   fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3])));
+}
+
 __global__ __launch_bounds__(256) void k_disp(int np, const double* __restrict__ ro,
                                               const double* __restrict__ ref, Box box,
-                                              unsigned int* dmax) {
+                                              double* __restrict__ vl_pos,
+                                              const int* __restrict__ vl_slot, unsigned int* dmax) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   float d2 = 0.f;
   if (i < np) {
     double s = 0.0;
+    double* rec = vl_pos + 4 * (size_t)vl_slot[i];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      double x = ro[3 * i + d] - ref[3 * i + d];
+      const double r = ro[3 * i + d];
+      rec[d] = r;
+      double x = r - ref[3 * i + d];
       x -= box.L[d] * rint(x * box.inv[d]);
       s += x * x;
     }
     d2 = (float)s;
   }
-  d2 = wave_max(d2);
-  if ((threadIdx.x & 63) == 0) atomic_max_pos(dmax, d2);
+  dmax_add(dmax, d2);
 }
 
 struct MdArgs {
@@ -57,6 +73,8 @@ struct MdArgs {
   const double* minv;   // per type
   const int* ipt;
   const double* ref;    // Verlet-list reference positions (or null)
+  double* vl_pos;       // the list's slot-ordered records (with ref)
+  const int* vl_slot;
   unsigned int* dmax;
   double* ke_part;
 };
@@ -66,6 +84,7 @@ __global__ __launch_bounds__(256) void k_drift(MdArgs a) {
   float d2 = 0.f;
   if (i < a.np) {
     double s = 0.0;
+    double* rec = a.ref ? a.vl_pos + 4 * (size_t)a.vl_slot[i] : nullptr;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       double x = a.ro[3 * i + d] + a.dt * a.vel[3 * i + d] + 0.5 * a.dt * a.dt * a.acc[3 * i + d];
@@ -73,6 +92,7 @@ __global__ __launch_bounds__(256) void k_drift(MdArgs a) {
       if (x >= a.box.half[d]) x -= a.box.L[d];
       a.ro[3 * i + d] = x;
       if (a.ref) {
+        rec[d] = x;
         double y = x - a.ref[3 * i + d];
         y -= a.box.L[d] * rint(y * a.box.inv[d]);
         s += y * y;
@@ -80,10 +100,7 @@ __global__ __launch_bounds__(256) void k_drift(MdArgs a) {
     }
     d2 = (float)s;
   }
-  if (a.ref) {
-    d2 = wave_max(d2);
-    if ((threadIdx.x & 63) == 0) atomic_max_pos(a.dmax, d2);
-  }
+  if (a.ref) dmax_add(a.dmax, d2);  // a.ref is the same for the whole grid
 }
 
 __global__ __launch_bounds__(256) void k_kick(MdArgs a, int set_only) {
@@ -135,6 +152,8 @@ MdArgs md_args(Ctx* c, double dt) {
   a.minv = c->d_minv;
   a.ipt = c->d_ipt;
   a.ref = c->P.list_mode == MSG_LIST_VERLET && c->vl_valid ? c->d_ro_ref : nullptr;
+  a.vl_pos = c->d_vl_pos;
+  a.vl_slot = c->d_vl_slot;
   a.dmax = c->d_dmax;
   a.ke_part = c->d_ke_part;
   return a;
@@ -155,11 +174,14 @@ int verlet_setup(Ctx* c) {
   if (int e = dalloc(c, (void**)&c->d_vl_home, np * sizeof(int))) return e;
   if (int e = dalloc(c, (void**)&c->d_vl_n, np * sizeof(int))) return e;
   if (int e = dalloc(c, (void**)&c->d_vl_nbr, (size_t)maxn * np * sizeof(int))) return e;
+  if (int e = dalloc(c, (void**)&c->d_vl_pos, 4 * (size_t)np * sizeof(double))) return e;
+  if (int e = dalloc(c, (void**)&c->d_vl_slot, np * sizeof(int))) return e;
   if (int e = dalloc(c, (void**)&c->d_ro_ref, 3 * (size_t)np * sizeof(double))) return e;
-  if (int e = dalloc(c, (void**)&c->d_dmax, 16)) return e;
-  if (hipHostMalloc((void**)&c->h_dmax, 16, hipHostMallocDefault) != hipSuccess)
+  const size_t dbytes = (size_t)kDmaxSlots * kDmaxStride * sizeof(unsigned int);
+  if (int e = dalloc(c, (void**)&c->d_dmax, dbytes)) return e;
+  if (hipHostMalloc((void**)&c->h_dmax, dbytes, hipHostMallocDefault) != hipSuccess)
     return set_error(MSG_ERR_NOMEM, "hipHostMalloc failed");
-  MSG_HIP(hipMemset(c->d_dmax, 0, 16));
+  MSG_HIP(hipMemset(c->d_dmax, 0, dbytes));
   return MSG_OK;
 }
 
@@ -171,7 +193,7 @@ int launch_disp(Ctx* c) {
     b.inv[d] = 1.0 / c->P.boxlen[d];
   }
   hipLaunchKernelGGL(k_disp, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, c->np, c->d_ro,
-                     c->d_ro_ref, b, c->d_dmax);
+                     c->d_ro_ref, b, c->d_vl_pos, c->d_vl_slot, c->d_dmax);
   MSG_HIP(hipGetLastError());
   return MSG_OK;
 }
@@ -184,10 +206,13 @@ int verlet_check(Ctx* c, bool& rebuild) {
   if (!c->disp_done)
     if (int e = launch_disp(c)) return e;
   c->disp_done = false;
-  MSG_HIP(hipMemcpyAsync(c->h_dmax, c->d_dmax, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+  MSG_HIP(hipMemcpyAsync(c->h_dmax, c->d_dmax, (size_t)kDmaxSlots * kDmaxStride * sizeof(unsigned int),
+                         hipMemcpyDeviceToHost, c->stream));
   MSG_HIP(hipStreamSynchronize(c->stream));
+  unsigned int bits = 0;
+  for (int k = 0; k < kDmaxSlots; ++k) bits = std::max(bits, c->h_dmax[k * kDmaxStride]);
   float d2;
-  std::memcpy(&d2, c->h_dmax, sizeof(float));
+  std::memcpy(&d2, &bits, sizeof(float));
   rebuild = 2.0 * std::sqrt((double)d2) > c->P.drnlist;
   return MSG_OK;
 }

@@ -482,8 +482,11 @@ __global__ __launch_bounds__(256) void k_pair_nb_seg(const PairArgs* __restrict_
 
 // Verlet list build (SetVListMC semantics, Src/nlist.F90:820-884, as a full list that also
 // serves MD): one thread per home particle in cell order scans the 27 stencil cells of the
-// (rcut + skin) grid and keeps every partner within rcut + skin.  Also snapshots the
-// positions the displacement test measures from and clears the displacement maximum.
+// (rcut + skin) grid and keeps every partner within rcut + skin, as its slot (the cell order
+// of this build).  The slot-ordered positions and types the pair loop reads are copied from
+// the cell records; the drift and displacement kernels keep their positions current until
+// the next build.  Also snapshots the positions the displacement test measures from and
+// clears the displacement maximum.
 struct VlistArgs {
   int np, maxn;
   const double4* cpos;
@@ -498,17 +501,21 @@ struct VlistArgs {
   int* vl_home;
   int* vl_n;
   int* vl_nbr;
+  double4* vl_pos;
+  int* vl_slot;
   unsigned int* dmax;
   int* flags;
 };
 
 __global__ __launch_bounds__(256) void k_vlist_build(VlistArgs a) {
   const int s = blockIdx.x * 256 + threadIdx.x;
-  if (s == 0) *a.dmax = 0u;
+  if (s < kDmaxSlots) a.dmax[s * kDmaxStride] = 0u;  // workgroup 0 has kDmaxSlots lanes or more
   if (s >= a.np) return;
   const double4 hp = a.cpos[s];
   const int ip = a.cell_ip[s];
   a.vl_home[s] = ip;
+  a.vl_pos[s] = hp;
+  a.vl_slot[ip] = s;
   a.ro_ref[3 * ip] = a.ro[3 * ip];
   a.ro_ref[3 * ip + 1] = a.ro[3 * ip + 1];
   a.ro_ref[3 * ip + 2] = a.ro[3 * ip + 2];
@@ -531,7 +538,7 @@ __global__ __launch_bounds__(256) void k_vlist_build(VlistArgs a) {
         if (fabs(d[q]) > a.box.half[q]) d[q] -= copysign(a.box.L[q], d[q]);
       const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
       if (r2 > a.rl2) continue;
-      if (n < a.maxn) a.vl_nbr[(size_t)n * a.np + s] = a.cell_ip[j];
+      if (n < a.maxn) a.vl_nbr[(size_t)n * a.np + s] = j;
       ++n;
     }
   }
@@ -543,18 +550,16 @@ __global__ __launch_bounds__(256) void k_vlist_build(VlistArgs a) {
 }
 
 // Pair loop over the Verlet list (UTwoBodyA over jpnlist, Src/energy.F90:445-544): one thread
-// per home particle, partner positions read from the caller-order array, two candidates per
-// iteration with both loads issued first.
+// per home slot, partner records (position, tagged type) read from the slot-ordered array,
+// so neighbours of nearby homes share cache lines; two candidates per iteration with both
+// loads issued first.
 struct VlPairArgs {
   PairArgs p;
   int maxn;
   const int* vl_home;
   const int* vl_n;
   const int* vl_nbr;
-  const double* ro;
-  const double* az;
-  const int* ipt;
-  const int* laz;  // weak charges (null when off)
+  const double4* vl_pos;
 };
 
 template <typename real, int NTP>
@@ -587,19 +592,20 @@ __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
   const real rc2 = (real)a.rcut2;
   PairAcc<real, NTP> acc;
   acc.zero();
+  int ht = 0;
   if (s < a.np) {
     const int ip = v.vl_home[s];
-    const double4 hp = make_double4(v.ro[3 * ip], v.ro[3 * ip + 1], v.ro[3 * ip + 2], 0.0);
-    const int ht = v.ipt[ip];
-    const int n = (v.laz && !v.laz[ip]) ? 0 : v.vl_n[s];  // an uncharged home has no pair energy
+    const double4 hp = v.vl_pos[s];
+    const int hw = (int)hp.w;
+    ht = hw & (kUncharged - 1);
+    const int n = hw >= kUncharged ? 0 : v.vl_n[s];  // an uncharged home has no pair energy
     for (int k = 0; k < n; k += 2) {
       const int jA = v.vl_nbr[(size_t)k * a.np + s];
       const bool haveB = k + 1 < n;
       const int jB = haveB ? v.vl_nbr[(size_t)(k + 1) * a.np + s] : jA;
-      const double4 pA = make_double4(v.ro[3 * jA], v.ro[3 * jA + 1], v.ro[3 * jA + 2], 0.0);
-      const double4 pB = make_double4(v.ro[3 * jB], v.ro[3 * jB + 1], v.ro[3 * jB + 2], 0.0);
-      const int tA = v.ipt[jA] + ((v.laz && !v.laz[jA]) ? kUncharged : 0);
-      const int tB = v.ipt[jB] + ((v.laz && !v.laz[jB]) ? kUncharged : 0);
+      const double4 pA = v.vl_pos[jA];
+      const double4 pB = v.vl_pos[jB];
+      const int tA = (int)pA.w, tB = (int)pB.w;
       pair_eval<real, NTP>(hp, ht, pA, tA, a.box, rc2, true, s_coef, s_bins, s_zlo, s_invw,
                            s_r2at, s_r2um, s_last, s_tp, acc);
       if (haveB)
@@ -611,10 +617,9 @@ __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
     a.force[3 * ip + 2] = acc.fz * a.scale;
   }
   if (acc.err) atomicOr(a.flags, acc.err);
-  const int htv = s < a.np ? v.ipt[v.vl_home[s]] : 0;
 #pragma unroll
   for (int q = 0; q < NTP; ++q) {
-    const double t = msgd::wave_sum(acc.to_tp(q, htv, s_tp));
+    const double t = msgd::wave_sum(acc.to_tp(q, ht, s_tp));
     if (lane == 0) s_red[wv][q] = t;
   }
   {
@@ -705,10 +710,7 @@ int launch_pair(Ctx* c, bool last) {
     v.vl_home = c->d_vl_home;
     v.vl_n = c->d_vl_n;
     v.vl_nbr = c->d_vl_nbr;
-    v.ro = c->d_ro;
-    v.az = c->d_az;
-    v.ipt = c->d_ipt;
-    v.laz = c->d_laz;
+    v.vl_pos = (const double4*)c->d_vl_pos;
     switch (c->nptpt) {
 #define MSG_VL(NTP)                                                                            \
   if (c->fp64) hipLaunchKernelGGL((k_pair_vl<double, NTP>), dim3(nb), dim3(256), lds, c->stream, v); \
@@ -773,6 +775,8 @@ int launch_vlist_build(Ctx* c) {
   v.vl_home = c->d_vl_home;
   v.vl_n = c->d_vl_n;
   v.vl_nbr = c->d_vl_nbr;
+  v.vl_pos = (double4*)c->d_vl_pos;
+  v.vl_slot = c->d_vl_slot;
   v.dmax = c->d_dmax;
   v.flags = c->d_flags;
   prof_begin(c, MSG_K_VLIST);

@@ -339,7 +339,7 @@ class Engine:
         _check(lib().msg_commit_charge(self.h, int(ia.size), ia))
 
     def cell_list(self):
-        """(nc, cell_start, cell_ip 0-based) of the last list build."""
+        """(nc, cell_start, cell_ip 0-based) of the current positions (as of the last pass)."""
         nc = np.zeros(3, dtype=np.int32)
         _check(lib().msg_get_cell_list(self.h, nc, None, None))
         start = np.zeros(int(np.prod(nc)) + 1, dtype=np.int32)

@@ -70,7 +70,7 @@ def test_weak_displacement_trials(precision):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-@pytest.mark.parametrize("list_mode", ["all", "cell"])
+@pytest.mark.parametrize("list_mode", ["all", "cell", "verlet"])
 def test_charge_change_and_commit(list_mode, precision):
     sp = _weak(list_mode)
     print(f"charge change list={list_mode} precision={precision}", flush=True)
