@@ -534,13 +534,15 @@ struct GatherArgs {
   double scale;
 };
 
-// Eight lanes per particle (brick order); lane iz < P sums one z-plane of the stencil.
+// Eight lanes per particle (brick order); lane ix < P takes x offset ix of the stencil and
+// walks its P x P (y, z) rows, so the lanes of a particle read neighbouring words of one row
+// (one cache line per particle and load instead of one per z-plane).
 template <typename real, int P>
 __device__ __forceinline__ void gather_body(const GatherArgs& a, int bid) {
   typedef typename R4T<real>::t R4;
   const double M[3] = {a.Mtot[0], a.Mtot[1], a.Mtot[2]};
   const int gid = bid * 256 + threadIdx.x;
-  const int i = gid >> 3, iz = gid & 7;
+  const int i = gid >> 3, ix = gid & 7;
   const bool valid = i < a.np;
   const int K = a.m.K;
   real e0 = 0, e1 = 0, e2 = 0;
@@ -549,42 +551,42 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int bid) {
   if (valid) {
     v = ((const R4*)a.bq)[i];
     const int packed = a.bmm[i];
-    if (iz < P) {
+    if (ix < P) {
       real tx[P], dtx[P], ty[P], dty[P], tz[P], dtz[P];
       msgd::bspline<real, P>(v.x, tx, dtx);
       msgd::bspline<real, P>(v.y, ty, dty);
       msgd::bspline<real, P>(v.z, tz, dtz);
-      int xs[P];
       const int mx = packed & 1023, my = (packed >> 10) & 1023, mz = packed >> 20;
+      int x = mx - ix;
+      x += x < 0 ? K : 0;
+      real thx = real(0), dthx = real(0);
 #pragma unroll
       for (int k = 0; k < P; ++k) {
-        const int x = mx - k;
-        xs[k] = x < 0 ? x + K : x;
+        thx = (k == ix) ? tx[k] : thx;
+        dthx = (k == ix) ? dtx[k] : dthx;
       }
-      int z = mz - iz;
-      z += z < 0 ? K : 0;
-      real thz = real(0), dthz = real(0);
+      int ys[P];
 #pragma unroll
       for (int k = 0; k < P; ++k) {
-        thz = (k == iz) ? tz[k] : thz;
-        dthz = (k == iz) ? dtz[k] : dthz;
+        const int y = my - k;
+        ys[k] = K * (y < 0 ? y + K : y);
       }
-      const real* phi = (const real*)a.phi;
+      const real* phi = (const real*)a.phi + x;
 #pragma unroll
-      for (int jy = 0; jy < P; ++jy) {
-        int y = my - jy;
-        y += y < 0 ? K : 0;
-        const real* row = phi + (size_t)K * (y + (size_t)K * z);
-        real sx = 0, dsx = 0;
+      for (int jz = 0; jz < P; ++jz) {
+        int z = mz - jz;
+        z += z < 0 ? K : 0;
+        const real* plane = phi + (size_t)K * K * z;
+        real sy = 0, dsy = 0;
 #pragma unroll
-        for (int k = 0; k < P; ++k) {
-          const real w = row[xs[k]];
-          sx += tx[k] * w;
-          dsx += dtx[k] * w;
+        for (int jy = 0; jy < P; ++jy) {
+          const real w = plane[ys[jy]];
+          sy += ty[jy] * w;
+          dsy += dty[jy] * w;
         }
-        e0 += dsx * ty[jy] * thz;
-        e1 += sx * dty[jy] * thz;
-        e2 += sx * ty[jy] * dthz;
+        e0 += dthx * tz[jz] * sy;
+        e1 += thx * tz[jz] * dsy;
+        e2 += thx * dtz[jz] * sy;
       }
     }
   }
@@ -594,7 +596,7 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int bid) {
     e1 += __shfl_xor(e1, off, 64);
     e2 += __shfl_xor(e2, off, 64);
   }
-  if (valid && iz == 0) {
+  if (valid && ix == 0) {
     const int ip = a.brick_ip[i];
     const double q = (double)v.w;
     const double es[3] = {(double)e0, (double)e1, (double)e2};
