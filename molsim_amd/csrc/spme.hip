@@ -358,30 +358,27 @@ __device__ inline void brick_range(int l, int B, bool whole, int& i0, int& n) {
   }
 }
 
-// One workgroup per brick.  Phase 1: for up to CAP contributing particles (from the brick
-// and the next brick in each dimension) the B-spline weights go to LDS together with the
-// particle's stencil rows that fall inside this brick (clipped y and z ranges).  Phase 2:
-// all threads take (particle, z, y) rows of that clipped set and add the clipped x-row into
-// the brick with 64-bit integer LDS atomics.  A row finds its particle in a row -> particle
-// table that the particles fill (OWN; one LDS read per row, for sparse bricks, where a
-// brick has few chunks) or by a binary search over the rows' prefix sums (dense bricks,
-// where the table's extra pass per chunk costs more than it saves).  Phase 3: every mesh
-// point of the brick is written once.
-template <typename real, int P, bool OWN>
+// One workgroup per brick.  The candidates are the particles of the brick and of the next
+// brick in each dimension (their stencils reach back into this one), taken 256 at a time:
+// (1) every thread clips its candidate's stencil to the brick from the packed base point
+// alone, and the candidates with points inside are compacted; (2) one thread per contributor
+// puts its B-spline weights (charge folded into z) in LDS; (3) the threads take (contributor,
+// z offset) tasks and add the clipped y rows of clipped x points into the brick with 64-bit
+// integer LDS atomics.  Finally every mesh point of the brick is written once.
+template <typename real, int P>
 __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   typedef typename R4T<real>::t R4;
-  constexpr int CAP = 96;
+  constexpr int CAP = 256;
   constexpr int NBP = kBX * kBY * kBZ;
   // rows padded to kBX + 1 accumulators: the lanes adding one particle's rows (same x,
   // consecutive y/z) then fall into different LDS banks instead of one
   constexpr int ROW = kBX + 1, NACC = ROW * kBY * kBZ;
   __shared__ unsigned long long acc[NACC];
   __shared__ real th[CAP][3 * P];
-  __shared__ int loc[CAP];
-  __shared__ int rng[CAP];      // packed y0 | ny << 8 | z0 << 16 | nz << 24
-  __shared__ int rpre[CAP + 1]; // prefix sums of rows per particle
-  __shared__ unsigned char owner[OWN ? CAP * P * P : 1];  // row -> particle of the chunk
-  __shared__ int wsum[4];
+  __shared__ int cidx[CAP];     // contributor -> its index in brick order
+  __shared__ int loc[CAP];      // local base point lx | ly << 10 | lz << 20
+  __shared__ int rng[CAP];      // clipped ranges x0 | nx << 4 | y0 << 8 | ny << 12 | z0 << 16 | nz << 20
+  __shared__ int wcnt[4];
   __shared__ int r_beg[8], r_pref[9], r_len[8];
   const int K = a.m.K;
   const int b = bid;
@@ -417,82 +414,78 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   const R4* bq = (const R4*)a.bq;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int base = 0; base < total; base += CAP) {
-    const int cnt = min(CAP, total - base);
-    int rows = 0;
-    if ((int)threadIdx.x < cnt) {
-      const int t = base + threadIdx.x;
+    const int t = base + threadIdx.x;
+    bool hit = false;
+    int idx = 0, lp = 0, rr = 0;
+    if (t < total) {
       int r = 0;
       while (r < 7 && t >= r_pref[r + 1]) ++r;
-      const int idx = r_beg[r] + (t - r_pref[r]);
-      const R4 v = bq[idx];
+      idx = r_beg[r] + (t - r_pref[r]);
       const int packed = a.bmm[idx];
+      int l[3] = {(packed & 1023) - lo[0], ((packed >> 10) & 1023) - lo[1], (packed >> 20) - lo[2]};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) l[d] += l[d] < 0 ? K : 0;
+      int x0, nx, y0, ny, z0, nz;
+      brick_range<P>(l[0], kBX, whole[0], x0, nx);
+      brick_range<P>(l[1], kBY, whole[1], y0, ny);
+      brick_range<P>(l[2], kBZ, whole[2], z0, nz);
+      hit = nx > 0 && ny > 0 && nz > 0;
+      lp = l[0] | (l[1] << 10) | (l[2] << 20);
+      rr = x0 | (nx << 4) | (y0 << 8) | (ny << 12) | (z0 << 16) | (nz << 20);
+    }
+    const unsigned long long bal = __ballot(hit);
+    if (lane == 0) wcnt[w] = __popcll(bal);
+    __syncthreads();
+    int off = 0;
+    for (int k = 0; k < w; ++k) off += wcnt[k];
+    const int ncon = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+    if (hit) {
+      const int k = off + __popcll(bal & ((1ull << lane) - 1ull));
+      cidx[k] = idx;
+      loc[k] = lp;
+      rng[k] = rr;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < ncon) {
+      const R4 v = bq[cidx[threadIdx.x]];
       real* T = th[threadIdx.x];
       msgd::bspline<real, P>(v.x, T, (real*)nullptr);
       msgd::bspline<real, P>(v.y, T + P, (real*)nullptr);
       msgd::bspline<real, P>(v.z, T + 2 * P, (real*)nullptr);
 #pragma unroll
       for (int i = 0; i < P; ++i) T[2 * P + i] *= v.w;
-      int l[3] = {(packed & 1023) - lo[0], ((packed >> 10) & 1023) - lo[1], (packed >> 20) - lo[2]};
-#pragma unroll
-      for (int d = 0; d < 3; ++d) l[d] += l[d] < 0 ? K : 0;
-      loc[threadIdx.x] = l[0] | (l[1] << 10) | (l[2] << 20);
-      int y0, ny, z0, nz;
-      brick_range<P>(l[1], kBY, whole[1], y0, ny);
-      brick_range<P>(l[2], kBZ, whole[2], z0, nz);
-      rng[threadIdx.x] = y0 | (ny << 8) | (z0 << 16) | (nz << 24);
-      rows = ny * nz;
     }
-    // exclusive prefix of the rows over the chunk's particles (threads 0..CAP-1)
-    int incl = rows;
+    __syncthreads();
+    for (int task = threadIdx.x; task < ncon * P; task += 256) {
+      const int k = task / P, zs = task - k * P;
+      const int cr = rng[k];
+      if (zs >= ((cr >> 20) & 15)) continue;
+      const int x0 = cr & 15, nx = (cr >> 4) & 15, y0 = (cr >> 8) & 15, ny = (cr >> 12) & 15;
+      const int iz = ((cr >> 16) & 15) + zs;
+      const int cl = loc[k];
+      const int lx0 = cl & 1023, ly0 = (cl >> 10) & 1023;
+      int lz = (cl >> 20) - iz;
+      lz += lz < 0 ? K : 0;  // wraps only where one brick spans the mesh
+      const real* T = th[k];
+      const real wz = T[2 * P + iz];
+      real tx[P];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int t = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += t;
-    }
-    if (lane == 63) wsum[w] = incl;
-    __syncthreads();
-    int before = 0;
-    for (int k = 0; k < w; ++k) before += wsum[k];
-    const int first = before + incl - rows;
-    if ((int)threadIdx.x < CAP) rpre[threadIdx.x] = first;
-    if (OWN)
-      for (int r = 0; r < rows; ++r) owner[first + r] = (unsigned char)threadIdx.x;
-    const int ntask = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    __syncthreads();
-    for (int task = threadIdx.x; task < ntask; task += 256) {
-      int p;
-      if (OWN) {
-        p = owner[task];
-      } else {
-        int lo_p = 0, hi_p = cnt - 1;  // last particle with rpre[p] <= task
-        while (lo_p < hi_p) {
-          const int mid = (lo_p + hi_p + 1) >> 1;
-          if (rpre[mid] <= task) lo_p = mid; else hi_p = mid - 1;
+      for (int ix = 0; ix < P; ++ix) tx[ix] = T[ix];
+#pragma unroll
+      for (int ys = 0; ys < P; ++ys) {
+        if (ys >= ny) break;
+        const int iy = y0 + ys;
+        int ly = ly0 - iy;
+        ly += ly < 0 ? K : 0;
+        const real wyz = T[P + iy] * wz;
+        unsigned long long* rowp = acc + ROW * (ly + kBY * lz);
+#pragma unroll
+        for (int ix = 0; ix < P; ++ix) {
+          if (ix < x0 || ix >= x0 + nx) continue;
+          int lx = lx0 - ix;
+          lx += lx < 0 ? K : 0;
+          atomicAdd(rowp + lx, to_fixed<real>(tx[ix] * wyz));
         }
-        p = lo_p;
-      }
-      const int rr = rng[p];
-      const int ny = (rr >> 8) & 255;
-      const int r = task - rpre[p];
-      // r / ny by the hardware reciprocal: r < 64 and ny <= 8 keep the fractional part of
-      // r / ny at least 1/8 away from the next integer, far beyond the reciprocal's error
-      const int q = (int)((float)r * __builtin_amdgcn_rcpf((float)ny) + 0.01f);
-      const int iy = (rr & 255) + (r - q * ny), iz = ((rr >> 16) & 255) + q;
-      const int lp = loc[p];
-      int ly = ((lp >> 10) & 1023) - iy, lz = (lp >> 20) - iz;
-      ly += ly < 0 ? K : 0;  // wraps only where one brick spans the mesh
-      lz += lz < 0 ? K : 0;
-      const real wyz = th[p][P + iy] * th[p][2 * P + iz];
-      const int lx0 = lp & 1023;
-      int x0, nx;
-      brick_range<P>(lx0, kBX, whole[0], x0, nx);
-      unsigned long long* rowp = acc + ROW * (ly + kBY * lz);
-#pragma unroll
-      for (int ix = 0; ix < P; ++ix) {
-        if (ix < x0 || ix >= x0 + nx) continue;
-        int lx = lx0 - ix;
-        lx += lx < 0 ? K : 0;
-        atomicAdd(rowp + lx, to_fixed<real>(th[p][ix] * wyz));
       }
     }
     __syncthreads();
@@ -505,20 +498,16 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   }
 }
 
-template <typename real, int P, bool OWN>
-__global__ __launch_bounds__(256) void k_spread(SpreadArgs a) { spread_body<real, P, OWN>(a, blockIdx.x); }
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_spread(SpreadArgs a) { spread_body<real, P>(a, blockIdx.x); }
 
-template <typename real, int P, bool OWN>
+template <typename real, int P>
 __global__ __launch_bounds__(256) void k_spread_seg(const SpreadArgs* __restrict__ args,
                                                     const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
   if (s.x < 0) return;  // no work for this XCD slot
-  spread_body<real, P, OWN>(args[s.x], s.y);
+  spread_body<real, P>(args[s.x], s.y);
 }
-
-// Spread variant: the row -> particle table for meshes of at most 48 particles per brick on
-// average (few chunks per brick), the binary search for denser ones.
-bool spread_owner_table(const Ctx* c) { return c->np <= 48 * c->mesh.nbrick; }
 
 struct GatherArgs {
   MeshGrid m;
@@ -711,10 +700,7 @@ void launch_spread_gather(Ctx* c, bool spread, bool with_reduce) {
   if (spread) {
     const SpreadArgs a = spread_args(c);
     prof_begin(c, MSG_K_SPREAD);
-    if (spread_owner_table(c))
-      hipLaunchKernelGGL((k_spread<real, P, true>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
-    else
-      hipLaunchKernelGGL((k_spread<real, P, false>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL((k_spread<real, P>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
     prof_end(c, MSG_K_SPREAD);
   } else {
     const GatherArgs g = gather_args(c);
@@ -903,9 +889,9 @@ int add_fft_k(Ctx* c, SegBuilder<FftArgs>* b) {
   return MSG_OK;
 }
 
-template <typename real, int P, bool OWN>
+template <typename real, int P>
 void seg_spread(const SegGroup& g, hipStream_t s) {
-  hipLaunchKernelGGL((k_spread_seg<real, P, OWN>), dim3(g.nblk), dim3(256), 0, s, (const SpreadArgs*)g.d_args,
+  hipLaunchKernelGGL((k_spread_seg<real, P>), dim3(g.nblk), dim3(256), 0, s, (const SpreadArgs*)g.d_args,
                      g.d_map);
 }
 template <typename real, int P>
@@ -916,10 +902,9 @@ void seg_gather(const SegGroup& g, hipStream_t s) {
 template <typename real>
 int add_spread_gather(Ctx* c, SegBuilder<SpreadArgs>& bs, SegBuilder<GatherArgs>& bg) {
   const int key = (sizeof(real) == 8 ? 100 : 0) + c->mesh.order;
-  const bool own = spread_owner_table(c);
   SegLaunchFn fs, fg;
-#define MSG_SG(P)                                                  \
-  fs = own ? seg_spread<real, P, true> : seg_spread<real, P, false>; \
+#define MSG_SG(P)              \
+  fs = seg_spread<real, P>; \
   fg = seg_gather<real, P>;
   switch (c->mesh.order) {
     case 3: MSG_SG(3) break;
@@ -931,7 +916,7 @@ int add_spread_gather(Ctx* c, SegBuilder<SpreadArgs>& bs, SegBuilder<GatherArgs>
     default: return set_error(MSG_ERR_ARG, "SPME order must be 3..8");
   }
 #undef MSG_SG
-  bs.add(key + (own ? 1000 : 0), fs, spread_args(c), c->mesh.nbrick);
+  bs.add(key, fs, spread_args(c), c->mesh.nbrick);
   bg.add(key, fg, gather_args(c), (8 * c->np + 255) / 256);
   return MSG_OK;
 }
