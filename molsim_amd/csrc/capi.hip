@@ -202,11 +202,11 @@ int utotal_sequence(Ctx* c) {
   bool rebuild = false;
   if (c->P.list_mode == MSG_LIST_VERLET)
     if ((e = verlet_check(c, rebuild))) return e;
-  if ((e = launch_bin(c))) return e;
   const bool branch = c->P.ewald_mode == MSG_EWALD_SPM && !c->prof;
   // with a Verlet list the cell order is only needed to rebuild the list (later users such
   // as DUTotal complete it on demand)
   const bool cells_needed = c->P.list_mode != MSG_LIST_VERLET || rebuild;
+  if ((e = launch_bin(c, cells_needed))) return e;
   auto cell_work = [&]() -> int {
     // without a rebuild the Verlet pair kernel reads positions in particle order, so the
     // cell slots are not sorted at all (complete_cells does it for later users)
@@ -222,9 +222,14 @@ int utotal_sequence(Ctx* c) {
   };
   c->cells_valid = cells_needed;
   if (branch) {
+    // Without cell work (a Verlet-list pass without a rebuild) the pair kernel would fill the
+    // device before the brick scatter, a kernel of scattered atomics, got workgroups: the
+    // brick sort then runs before the fork.
+    const bool sort_first = !cells_needed;
+    if (sort_first && (e = launch_sort(c, 2))) return e;
     MSG_HIP(hipEventRecord(c->ev_fork, c->stream));
     // the mesh branch is the longer chain: recorded first, it is dispatched first
-    if ((e = launch_sort(c, 2))) return e;
+    if (!sort_first && (e = launch_sort(c, 2))) return e;
     if ((e = launch_spme_mesh(c))) return e;
     MSG_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     hipStream_t main = c->stream;

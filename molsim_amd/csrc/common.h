@@ -230,7 +230,7 @@ int pack_table(Ctx* c);
 
 // Launch sequences (each .hip file)
 int lists_setup(Ctx* c);                           // cell grid + buffers
-int launch_bin(Ctx* c);                            // K1 cell id + mesh bin + counts + dipole
+int launch_bin(Ctx* c, bool cells = true);         // K1 cell id + mesh bin + counts + dipole
 int launch_sort(Ctx* c, int which);                // scan + scatter: 1 cells, 2 bricks + dipole, 3 both
 int launch_finalize(Ctx* c);                       // cell order (after launch_sort)
 int launch_pair(Ctx* c, bool last);                // K4

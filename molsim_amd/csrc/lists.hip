@@ -27,6 +27,7 @@ struct BinArgs {
   const double* az;
   Box box;
   CellGrid g;
+  int use_cells;    // cell ids and counts (0: only the box check, for passes with no cell sort)
   int use_mesh;
   MeshGrid m;
   int* cellid;
@@ -58,12 +59,14 @@ __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
       ci[d] = min(max(k, 0), a.g.nc[d] - 1);
     }
     if (bad) atomicOr(a.flags, FLAG_CELL);
-    const int c = ci[0] + a.g.nc[0] * (ci[1] + a.g.nc[1] * ci[2]);
-    a.cellid[i] = c;
-    if (a.rank_cell)
-      a.rank_cell[i] = atomicAdd(&a.cell_count[c], 1);
-    else
-      atomicAdd(&a.cell_count[c], 1);
+    if (a.use_cells) {
+      const int c = ci[0] + a.g.nc[0] * (ci[1] + a.g.nc[1] * ci[2]);
+      a.cellid[i] = c;
+      if (a.rank_cell)
+        a.rank_cell[i] = atomicAdd(&a.cell_count[c], 1);
+      else
+        atomicAdd(&a.cell_count[c], 1);
+    }
     if (a.use_mesh) {
       real* frac = (real*)a.frac;
       int mi[3];
@@ -514,6 +517,7 @@ static BinArgs bin_args(Ctx* c) {
   a.az = c->d_az;
   a.box = make_box(c->P);
   a.g = c->grid;
+  a.use_cells = 1;
   a.use_mesh = c->P.ewald_mode == MSG_EWALD_SPM;
   a.m = c->mesh;
   a.cellid = c->d_cellid;
@@ -589,9 +593,13 @@ static FinalizeArgs finalize_args(Ctx* c) {
 
 static int finalize_blocks(const FinalizeArgs& f) { return (f.ncell + 3) / 4; }
 
-int launch_bin(Ctx* c) {
+int launch_bin(Ctx* c, bool cells) {
   const int nb = (c->np + 255) / 256;
-  const BinArgs a = bin_args(c);
+  BinArgs a = bin_args(c);
+  if (!cells) {
+    a.use_cells = 0;
+    a.rank_cell = nullptr;
+  }
   prof_begin(c, MSG_K_BIN);
   if (c->fp64)
     hipLaunchKernelGGL(k_bin<double>, dim3(nb), dim3(256), 0, c->stream, a);

@@ -40,20 +40,28 @@ __device__ inline void dmax_add(unsigned int* dmax, float d2) {
                    fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3])));
 }
 
+// The list's record of a particle: position and type, + kUncharged when uncharged (written
+// whole, so the scattered stores fill their 32-byte sectors).
+__device__ inline void put_record(double4* vl_pos, int slot, const double* r, int type,
+                                  const int* laz, int i) {
+  vl_pos[slot] = make_double4(r[0], r[1], r[2], (double)(type + ((laz && !laz[i]) ? kUncharged : 0)));
+}
+
 __global__ __launch_bounds__(256) void k_disp(int np, const double* __restrict__ ro,
                                               const double* __restrict__ ref, Box box,
-                                              double* __restrict__ vl_pos,
-                                              const int* __restrict__ vl_slot, unsigned int* dmax) {
+                                              double4* __restrict__ vl_pos,
+                                              const int* __restrict__ vl_slot,
+                                              const int* __restrict__ ipt,
+                                              const int* __restrict__ laz, unsigned int* dmax) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   float d2 = 0.f;
   if (i < np) {
     double s = 0.0;
-    double* rec = vl_pos + 4 * (size_t)vl_slot[i];
+    const double r[3] = {ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]};
+    put_record(vl_pos, vl_slot[i], r, ipt[i], laz, i);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const double r = ro[3 * i + d];
-      rec[d] = r;
-      double x = r - ref[3 * i + d];
+      double x = r[d] - ref[3 * i + d];
       x -= box.L[d] * rint(x * box.inv[d]);
       s += x * x;
     }
@@ -72,8 +80,9 @@ struct MdArgs {
   const double* force;
   const double* minv;   // per type
   const int* ipt;
+  const int* laz;       // weak charges (or null)
   const double* ref;    // Verlet-list reference positions (or null)
-  double* vl_pos;       // the list's slot-ordered records (with ref)
+  double4* vl_pos;      // the list's slot-ordered records (with ref)
   const int* vl_slot;
   unsigned int* dmax;
   double* ke_part;
@@ -84,20 +93,21 @@ __global__ __launch_bounds__(256) void k_drift(MdArgs a) {
   float d2 = 0.f;
   if (i < a.np) {
     double s = 0.0;
-    double* rec = a.ref ? a.vl_pos + 4 * (size_t)a.vl_slot[i] : nullptr;
+    double xs[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       double x = a.ro[3 * i + d] + a.dt * a.vel[3 * i + d] + 0.5 * a.dt * a.dt * a.acc[3 * i + d];
       x -= a.box.L[d] * rint(x * a.box.inv[d]);  // PBC: back into [-L/2, L/2]
       if (x >= a.box.half[d]) x -= a.box.L[d];
       a.ro[3 * i + d] = x;
+      xs[d] = x;
       if (a.ref) {
-        rec[d] = x;
         double y = x - a.ref[3 * i + d];
         y -= a.box.L[d] * rint(y * a.box.inv[d]);
         s += y * y;
       }
     }
+    if (a.ref) put_record(a.vl_pos, a.vl_slot[i], xs, a.ipt[i], a.laz, i);
     d2 = (float)s;
   }
   if (a.ref) dmax_add(a.dmax, d2);  // a.ref is the same for the whole grid
@@ -152,7 +162,8 @@ MdArgs md_args(Ctx* c, double dt) {
   a.minv = c->d_minv;
   a.ipt = c->d_ipt;
   a.ref = c->P.list_mode == MSG_LIST_VERLET && c->vl_valid ? c->d_ro_ref : nullptr;
-  a.vl_pos = c->d_vl_pos;
+  a.laz = c->d_laz;
+  a.vl_pos = (double4*)c->d_vl_pos;
   a.vl_slot = c->d_vl_slot;
   a.dmax = c->d_dmax;
   a.ke_part = c->d_ke_part;
@@ -193,7 +204,7 @@ int launch_disp(Ctx* c) {
     b.inv[d] = 1.0 / c->P.boxlen[d];
   }
   hipLaunchKernelGGL(k_disp, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, c->np, c->d_ro,
-                     c->d_ro_ref, b, c->d_vl_pos, c->d_vl_slot, c->d_dmax);
+                     c->d_ro_ref, b, (double4*)c->d_vl_pos, c->d_vl_slot, c->d_ipt, c->d_laz, c->d_dmax);
   MSG_HIP(hipGetLastError());
   return MSG_OK;
 }
