@@ -299,6 +299,86 @@ __device__ inline void pair_eval(const double4& hp, int ht, const double4& pj, i
   acc.vir -= (double)(fs * r2);
 }
 
+// Table metadata of one (home type, partner type) combination, read with one LDS access:
+// the bin range and overlap/table limits of its type pair, the type pair and its last
+// interval.
+template <typename real>
+struct alignas(4 * sizeof(real)) PairMeta {
+  real zlo, invw, r2at, r2um;
+};
+
+// pair_eval with the metadata of the combination in one record (s_pm / s_pi indexed by
+// ht * kMaxT + tj) and the interval's 12 coefficients read whole before the interval
+// search checks them (the search advances for a few per cent of pairs): three dependent LDS
+// accesses per pair instead of six.  Same arithmetic as pair_eval.
+template <typename real, int NTP>
+__device__ inline void pair_eval_m(const double4& hp, int ht, const double4& pj, int tj, const Box& box,
+                                   real rc2, bool inclusive, const real* s_coef, const int* s_bins,
+                                   const PairMeta<real>* s_pm, const int2* s_pi, PairAcc<real, NTP>& acc) {
+  typedef typename std::conditional<sizeof(real) == 4, float4, double2>::type V16;
+  constexpr int NV = 12 * (int)sizeof(real) / 16;
+  double d[3] = {hp.x - pj.x, hp.y - pj.y, hp.z - pj.z};
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    if (fabs(d[q]) > box.half[q]) d[q] -= copysign(box.L[q], d[q]);
+  const real dx = (real)d[0], dy = (real)d[1], dz = (real)d[2];
+  const real r2 = dx * dx + dy * dy + dz * dz;
+  if (inclusive ? (r2 > rc2) : !(r2 < rc2)) return;
+  if (tj >= kUncharged) return;  // weak charges: no pair energy with an uncharged partner (UWeakChargeA)
+  const int mi = ht * kMaxT + tj;
+  const PairMeta<real> pm = s_pm[mi];
+  const int2 pi = s_pi[mi];
+  real us, fs;
+  if (r2 < pm.r2at) {
+    us = real(1e10);
+    fs = real(1e10);
+  } else if (r2 < pm.r2um) {
+    acc.err |= FLAG_R2UMIN;
+    return;
+  } else {
+    int b = (int)((r2 - pm.zlo) * pm.invw);
+    b = min(max(b, 0), kTabBins - 1);
+    int ti = s_bins[pi.x * kTabBins + b];
+    alignas(16) real t[12];
+    {
+      const V16* rp = (const V16*)(s_coef + 12 * ti);
+#pragma unroll
+      for (int k = 0; k < NV; ++k) *(V16*)(t + k * 16 / (int)sizeof(real)) = rp[k];
+    }
+    while (ti < pi.y && r2 < t[0]) {
+      ++ti;
+      const V16* rp = (const V16*)(s_coef + 12 * ti);
+#pragma unroll
+      for (int k = 0; k < NV; ++k) *(V16*)(t + k * 16 / (int)sizeof(real)) = rp[k];
+    }
+    const real dd = r2 - t[0];
+    us = t[1] + dd * (t[2] + dd * (t[3] + dd * (t[4] + dd * (t[5] + dd * t[6]))));
+    fs = t[7] + dd * (t[8] + dd * (t[9] + dd * (t[10] + dd * t[11])));
+  }
+#pragma unroll
+  for (int t = 0; t < PairAcc<real, NTP>::NT; ++t) acc.et[t] += (t == tj) ? (double)us : 0.0;
+  acc.fx += (double)(fs * dx);
+  acc.fy += (double)(fs * dy);
+  acc.fz += (double)(fs * dz);
+  acc.vir -= (double)(fs * r2);
+}
+
+// Fills the combination records (threads 0 .. kMaxT^2 - 1) and the type-pair index.
+template <typename real>
+__device__ inline void fill_pair_meta(const GpuTableMeta& tm, PairMeta<real>* s_pm, int2* s_pi, int* s_tp) {
+  if (threadIdx.x < kMaxT * kMaxT) {
+    const int tp = tm.tpidx[threadIdx.x];
+    s_tp[threadIdx.x] = tp;
+    PairMeta<real> m;
+    m.zlo = (real)tm.zlo[tp];
+    m.invw = (real)tm.invw[tp];
+    m.r2at = (real)tm.r2atat[tp];
+    m.r2um = (real)tm.r2umin[tp];
+    s_pm[threadIdx.x] = m;
+    s_pi[threadIdx.x] = make_int2(tp, tm.last[tp]);
+  }
+}
+
 // Neighbour-parallel variant for the full 27-cell stencil: LPH lanes per home particle (the
 // tile shape, 8..64, chosen per system size by the autotuner; 256 / LPH homes per workgroup).
 // The candidates of the 27 stencil cells are split into LPH contiguous, equal shares (so dense
@@ -309,8 +389,9 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
   constexpr int HPB = 256 / LPH;                          // homes per workgroup
   constexpr int MPL = (kMaxStencil + LPH - 1) / LPH;      // stencil cells per lane
   extern __shared__ __align__(16) double s_dyn[];
-  __shared__ real s_zlo[kMaxTP], s_invw[kMaxTP], s_r2at[kMaxTP], s_r2um[kMaxTP];
-  __shared__ int s_last[kMaxTP], s_tp[kMaxT * kMaxT];
+  __shared__ PairMeta<real> s_pm[kMaxT * kMaxT];
+  __shared__ int2 s_pi[kMaxT * kMaxT];
+  __shared__ int s_tp[kMaxT * kMaxT];
   __shared__ double s_red[4][kMaxTP + 1];
   __shared__ int s_cst[HPB][kMaxStencil + 1], s_cpf[HPB][kMaxStencil + 1];
   const int nint = a.tm.nint;
@@ -320,15 +401,7 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
     const real* src = (const real*)a.tcoef;
     lds_copy16(s_coef, src, nint * 12 * (int)sizeof(real));
     lds_copy16(s_bins, a.tbins, NTP * kTabBins * (int)sizeof(int));
-    if (threadIdx.x < NTP) {
-      const int tp = threadIdx.x;
-      s_zlo[tp] = (real)a.tm.zlo[tp];
-      s_invw[tp] = (real)a.tm.invw[tp];
-      s_r2at[tp] = (real)a.tm.r2atat[tp];
-      s_r2um[tp] = (real)a.tm.r2umin[tp];
-      s_last[tp] = a.tm.last[tp];
-    }
-    if (threadIdx.x < kMaxT * kMaxT) s_tp[threadIdx.x] = a.tm.tpidx[threadIdx.x];
+    fill_pair_meta<real>(a.tm, s_pm, s_pi, s_tp);
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, grp = threadIdx.x / LPH;
   const int s = bid * HPB + grp;
@@ -435,8 +508,8 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (hv[u] && jv[u] != s)
-          pair_eval<real, NTP>(hp, ht, pv[u], (int)pv[u].w, a.box, rc2, a.cut_inclusive, s_coef, s_bins, s_zlo,
-                               s_invw, s_r2at, s_r2um, s_last, s_tp, acc);
+          pair_eval_m<real, NTP>(hp, ht, pv[u], (int)pv[u].w, a.box, rc2, a.cut_inclusive, s_coef, s_bins,
+                                 s_pm, s_pi, acc);
     }
   }
 #pragma unroll
@@ -566,8 +639,9 @@ template <typename real, int NTP>
 __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
   const PairArgs& a = v.p;
   extern __shared__ __align__(16) double s_dyn[];
-  __shared__ real s_zlo[kMaxTP], s_invw[kMaxTP], s_r2at[kMaxTP], s_r2um[kMaxTP];
-  __shared__ int s_last[kMaxTP], s_tp[kMaxT * kMaxT];
+  __shared__ PairMeta<real> s_pm[kMaxT * kMaxT];
+  __shared__ int2 s_pi[kMaxT * kMaxT];
+  __shared__ int s_tp[kMaxT * kMaxT];
   __shared__ double s_red[4][kMaxTP + 1];
   const int nint = a.tm.nint;
   real* s_coef = (real*)s_dyn;
@@ -576,15 +650,7 @@ __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
     const real* src = (const real*)a.tcoef;
     lds_copy16(s_coef, src, nint * 12 * (int)sizeof(real));
     lds_copy16(s_bins, a.tbins, NTP * kTabBins * (int)sizeof(int));
-    if (threadIdx.x < NTP) {
-      const int tp = threadIdx.x;
-      s_zlo[tp] = (real)a.tm.zlo[tp];
-      s_invw[tp] = (real)a.tm.invw[tp];
-      s_r2at[tp] = (real)a.tm.r2atat[tp];
-      s_r2um[tp] = (real)a.tm.r2umin[tp];
-      s_last[tp] = a.tm.last[tp];
-    }
-    if (threadIdx.x < kMaxT * kMaxT) s_tp[threadIdx.x] = a.tm.tpidx[threadIdx.x];
+    fill_pair_meta<real>(a.tm, s_pm, s_pi, s_tp);
   }
   __syncthreads();
   const int s = blockIdx.x * 256 + threadIdx.x;
@@ -599,18 +665,20 @@ __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
     const int hw = (int)hp.w;
     ht = hw & (kUncharged - 1);
     const int n = hw >= kUncharged ? 0 : v.vl_n[s];  // an uncharged home has no pair energy
+    // the next pair of neighbour slots is loaded one iteration ahead
+    int jA = n > 0 ? v.vl_nbr[s] : 0;
+    int jB = n > 1 ? v.vl_nbr[(size_t)a.np + s] : jA;
     for (int k = 0; k < n; k += 2) {
-      const int jA = v.vl_nbr[(size_t)k * a.np + s];
       const bool haveB = k + 1 < n;
-      const int jB = haveB ? v.vl_nbr[(size_t)(k + 1) * a.np + s] : jA;
       const double4 pA = v.vl_pos[jA];
       const double4 pB = v.vl_pos[jB];
-      const int tA = (int)pA.w, tB = (int)pB.w;
-      pair_eval<real, NTP>(hp, ht, pA, tA, a.box, rc2, true, s_coef, s_bins, s_zlo, s_invw,
-                           s_r2at, s_r2um, s_last, s_tp, acc);
+      const int nA = k + 2 < n ? v.vl_nbr[(size_t)(k + 2) * a.np + s] : jA;
+      const int nB = k + 3 < n ? v.vl_nbr[(size_t)(k + 3) * a.np + s] : nA;
+      pair_eval_m<real, NTP>(hp, ht, pA, (int)pA.w, a.box, rc2, true, s_coef, s_bins, s_pm, s_pi, acc);
       if (haveB)
-        pair_eval<real, NTP>(hp, ht, pB, tB, a.box, rc2, true, s_coef, s_bins, s_zlo, s_invw,
-                             s_r2at, s_r2um, s_last, s_tp, acc);
+        pair_eval_m<real, NTP>(hp, ht, pB, (int)pB.w, a.box, rc2, true, s_coef, s_bins, s_pm, s_pi, acc);
+      jA = nA;
+      jB = nB;
     }
     a.force[3 * ip + 0] = acc.fx * a.scale;
     a.force[3 * ip + 1] = acc.fy * a.scale;
