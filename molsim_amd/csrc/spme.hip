@@ -360,11 +360,12 @@ __device__ inline void brick_range(int l, int B, bool whole, int& i0, int& n) {
 
 // One workgroup per brick.  The candidates are the particles of the brick and of the next
 // brick in each dimension (their stencils reach back into this one), taken 256 at a time:
-// (1) every thread clips its candidate's stencil to the brick from the packed base point
-// alone, and the candidates with points inside are compacted; (2) one thread per contributor
-// puts its B-spline weights (charge folded into z) in LDS; (3) the threads take (contributor,
-// z offset) tasks and add the clipped y rows of clipped x points into the brick with 64-bit
-// integer LDS atomics.  Finally every mesh point of the brick is written once.
+// (1) every thread clips its candidate's stencil to the brick from the packed base point,
+// and the candidates with points inside are compacted; (2) the thread that found a
+// contributor puts its B-spline weights (charge folded into z) in LDS at the contributor's
+// place; (3) the threads take (contributor, z offset) tasks and add the clipped y rows of
+// clipped x points into the brick with 64-bit integer LDS atomics.  Finally every mesh point
+// of the brick is written once.
 template <typename real, int P>
 __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   typedef typename R4T<real>::t R4;
@@ -375,7 +376,6 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   constexpr int ROW = kBX + 1, NACC = ROW * kBY * kBZ;
   __shared__ unsigned long long acc[NACC];
   __shared__ real th[CAP][3 * P];
-  __shared__ int cidx[CAP];     // contributor -> its index in brick order
   __shared__ int loc[CAP];      // local base point lx | ly << 10 | lz << 20
   __shared__ int rng[CAP];      // clipped ranges x0 | nx << 4 | y0 << 8 | ny << 12 | z0 << 16 | nz << 20
   __shared__ int wcnt[4];
@@ -416,12 +416,14 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   for (int base = 0; base < total; base += CAP) {
     const int t = base + threadIdx.x;
     bool hit = false;
-    int idx = 0, lp = 0, rr = 0;
+    int lp = 0, rr = 0;
+    R4 v;
     if (t < total) {
       int r = 0;
       while (r < 7 && t >= r_pref[r + 1]) ++r;
-      idx = r_beg[r] + (t - r_pref[r]);
+      const int idx = r_beg[r] + (t - r_pref[r]);
       const int packed = a.bmm[idx];
+      v = bq[idx];
       int l[3] = {(packed & 1023) - lo[0], ((packed >> 10) & 1023) - lo[1], (packed >> 20) - lo[2]};
 #pragma unroll
       for (int d = 0; d < 3; ++d) l[d] += l[d] < 0 ? K : 0;
@@ -441,14 +443,9 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
     const int ncon = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
     if (hit) {
       const int k = off + __popcll(bal & ((1ull << lane) - 1ull));
-      cidx[k] = idx;
       loc[k] = lp;
       rng[k] = rr;
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < ncon) {
-      const R4 v = bq[cidx[threadIdx.x]];
-      real* T = th[threadIdx.x];
+      real* T = th[k];
       msgd::bspline<real, P>(v.x, T, (real*)nullptr);
       msgd::bspline<real, P>(v.y, T + P, (real*)nullptr);
       msgd::bspline<real, P>(v.z, T + 2 * P, (real*)nullptr);
