@@ -554,12 +554,14 @@ __global__ __launch_bounds__(256) void k_pair_nb_seg(const PairArgs* __restrict_
 }
 
 // Verlet list build (SetVListMC semantics, Src/nlist.F90:820-884, as a full list that also
-// serves MD): one thread per home particle in cell order scans the 27 stencil cells of the
-// (rcut + skin) grid and keeps every partner within rcut + skin, as its slot (the cell order
-// of this build).  The slot-ordered positions and types the pair loop reads are copied from
-// the cell records; the drift and displacement kernels keep their positions current until
-// the next build.  Also snapshots the positions the displacement test measures from and
-// clears the displacement maximum.
+// serves MD).  Homes are taken in cell order (their "slots"); kVlLph lanes per home walk the
+// candidates of its stencil cells in cell order (cells whose nearest point lies beyond rcut +
+// skin are skipped), kVlLph candidates per round; a ballot over the group ranks the partners
+// within rcut + skin, which are appended as slots in that order, so the list is the one a
+// single thread per home walking the cells would write.  The slot-ordered positions and types
+// the pair loop reads are copied from the cell records; the drift and displacement kernels
+// keep their positions current until the next build.  Also snapshots the positions the
+// displacement test measures from and clears the displacement maximum.
 struct VlistArgs {
   int np, maxn;
   const double4* cpos;
@@ -578,48 +580,159 @@ struct VlistArgs {
   int* vl_slot;
   unsigned int* dmax;
   int* flags;
+  int stage;           // lists staged in LDS (rows of maxn + 1) and written out row by row
 };
 
+constexpr int kVlLph = 16;
+constexpr int kVlStageBytes = 64 * 1024;  // LDS for staged lists: maxn up to 1023
+
 __global__ __launch_bounds__(256) void k_vlist_build(VlistArgs a) {
-  const int s = blockIdx.x * 256 + threadIdx.x;
-  if (s < kDmaxSlots) a.dmax[s * kDmaxStride] = 0u;  // workgroup 0 has kDmaxSlots lanes or more
-  if (s >= a.np) return;
-  const double4 hp = a.cpos[s];
-  const int ip = a.cell_ip[s];
-  a.vl_home[s] = ip;
-  a.vl_pos[s] = hp;
-  a.vl_slot[ip] = s;
-  a.ro_ref[3 * ip] = a.ro[3 * ip];
-  a.ro_ref[3 * ip + 1] = a.ro[3 * ip + 1];
-  a.ro_ref[3 * ip + 2] = a.ro[3 * ip + 2];
-  const int c = a.slot_cell[s];
+  constexpr int LPH = kVlLph, HPB = 256 / LPH, MPL = (kMaxStencil + LPH - 1) / LPH;
+  __shared__ int s_cst[HPB][kMaxStencil + 1], s_cpf[HPB][kMaxStencil + 1];
+  __shared__ int s_n[HPB];
+  extern __shared__ int s_list[];  // [HPB][maxn + 1] with a.stage
+  const int ls = a.maxn + 1;
+  const int tid = threadIdx.x, lane = tid & 63, grp = tid / LPH, k = tid & (LPH - 1);
+  const int s = blockIdx.x * HPB + grp;
+  if (blockIdx.x == 0 && tid < kDmaxSlots) a.dmax[tid * kDmaxStride] = 0u;
+  const bool home = s < a.np;
+  const int hs = home ? s : a.np - 1;
+  const double4 hp = a.cpos[hs];
+  if (home && k == 0) {
+    const int ip = a.cell_ip[s];
+    a.vl_home[s] = ip;
+    a.vl_pos[s] = hp;
+    a.vl_slot[ip] = s;
+    a.ro_ref[3 * ip] = a.ro[3 * ip];
+    a.ro_ref[3 * ip + 1] = a.ro[3 * ip + 1];
+    a.ro_ref[3 * ip + 2] = a.ro[3 * ip + 2];
+  }
+  const int c = a.slot_cell[hs];
   const int cx = c % a.g.nc[0], cy = (c / a.g.nc[0]) % a.g.nc[1], cz = c / (a.g.nc[0] * a.g.nc[1]);
-  int n = 0;
-  for (int k = 0; k < a.g.nst; ++k) {
-    int x = cx + a.g.st[3 * k], y = cy + a.g.st[3 * k + 1], z = cz + a.g.st[3 * k + 2];
-    x += x < 0 ? a.g.nc[0] : (x >= a.g.nc[0] ? -a.g.nc[0] : 0);
-    y += y < 0 ? a.g.nc[1] : (y >= a.g.nc[1] ? -a.g.nc[1] : 0);
-    z += z < 0 ? a.g.nc[2] : (z >= a.g.nc[2] ? -a.g.nc[2] : 0);
-    const int nb = x + a.g.nc[0] * (y + a.g.nc[1] * z);
-    const int j1 = a.cell_start[nb + 1];
-    for (int j = a.cell_start[nb]; j < j1; ++j) {
-      if (j == s) continue;
-      const double4 pj = a.cpos[j];
-      double d[3] = {hp.x - pj.x, hp.y - pj.y, hp.z - pj.z};
+  // home position inside its cell, for the distance bound of each stencil cell (the bound
+  // carries a margin, so no partner is lost)
+  const int cc3[3] = {cx, cy, cz};
+  const double hh[3] = {hp.x, hp.y, hp.z};
+  double hrel[3], csz[3];
 #pragma unroll
-      for (int q = 0; q < 3; ++q)
-        if (fabs(d[q]) > a.box.half[q]) d[q] -= copysign(a.box.L[q], d[q]);
-      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-      if (r2 > a.rl2) continue;
-      if (n < a.maxn) a.vl_nbr[(size_t)n * a.np + s] = j;
-      ++n;
+  for (int d = 0; d < 3; ++d) {
+    csz[d] = a.box.L[d] / a.g.nc[d];
+    hrel[d] = (hh[d] + a.box.half[d]) - cc3[d] * csz[d];
+  }
+  const double cull2 = a.rl2 * (1.0 + 1e-5);
+  int st0[MPL], cnt[MPL], tot = 0;
+#pragma unroll
+  for (int m = 0; m < MPL; ++m) {
+    const int sc = k * MPL + m;
+    st0[m] = 0;
+    cnt[m] = 0;
+    if (sc < a.g.nst) {
+      double bd2 = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int o = a.g.st[3 * sc + d];
+        const double dist = o < 0 ? hrel[d] : (o > 0 ? csz[d] - hrel[d] : 0.0);
+        bd2 += dist > 0.0 ? dist * dist : 0.0;
+      }
+      if (bd2 <= cull2) {
+        int x = cx + a.g.st[3 * sc], y = cy + a.g.st[3 * sc + 1], z = cz + a.g.st[3 * sc + 2];
+        x += x < 0 ? a.g.nc[0] : (x >= a.g.nc[0] ? -a.g.nc[0] : 0);
+        y += y < 0 ? a.g.nc[1] : (y >= a.g.nc[1] ? -a.g.nc[1] : 0);
+        z += z < 0 ? a.g.nc[2] : (z >= a.g.nc[2] ? -a.g.nc[2] : 0);
+        const int nb = x + a.g.nc[0] * (y + a.g.nc[1] * z);
+        st0[m] = a.cell_start[nb];
+        cnt[m] = a.cell_start[nb + 1] - st0[m];
+      }
+    }
+    tot += cnt[m];
+  }
+  int incl = tot;
+#pragma unroll
+  for (int off = 1; off < LPH; off <<= 1) {
+    const int t = __shfl_up(incl, off, LPH);
+    if (k >= off) incl += t;
+  }
+  const int ncand = __shfl(incl, LPH - 1, LPH);
+  {
+    int pre = incl - tot;
+#pragma unroll
+    for (int m = 0; m < MPL; ++m) {
+      const int sc = k * MPL + m;
+      if (sc < a.g.nst) {
+        s_cst[grp][sc] = st0[m];
+        s_cpf[grp][sc] = pre;
+      }
+      pre += cnt[m];
     }
   }
-  if (n > a.maxn) {
-    atomicOr(a.flags, FLAG_NLIST);
-    n = a.maxn;
+  if (k == 0) s_cpf[grp][a.g.nst] = ncand;
+  __syncthreads();
+  // rounds: the most any group of the wavefront needs, so every lane reaches each ballot
+  int rounds = (ncand + LPH - 1) / LPH;
+#pragma unroll
+  for (int off = LPH; off < 64; off <<= 1) rounds = max(rounds, __shfl_xor(rounds, off, 64));
+  const int gsh = lane & ~(LPH - 1);  // the group's first lane in the wavefront
+  // four rounds per iteration: their candidate loads are issued before the first test
+  constexpr int U = 4;
+  int n = 0, cc = 0;
+  for (int r = 0; r < rounds; r += U) {
+    int jv[U];
+    bool val[U];
+    double4 pv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = (r + u) * LPH + k;
+      val[u] = home && i < ncand;
+      jv[u] = s;
+      if (val[u]) {
+        while (s_cpf[grp][cc + 1] <= i) ++cc;
+        jv[u] = s_cst[grp][cc] + (i - s_cpf[grp][cc]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) pv[u] = a.cpos[val[u] ? jv[u] : hs];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      bool hit = false;
+      if (val[u] && jv[u] != s) {
+        double d[3] = {hp.x - pv[u].x, hp.y - pv[u].y, hp.z - pv[u].z};
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          if (fabs(d[q]) > a.box.half[q]) d[q] -= copysign(a.box.L[q], d[q]);
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        hit = !(r2 > a.rl2);
+      }
+      const unsigned gm = (unsigned)(__ballot(hit) >> gsh) & ((1u << LPH) - 1u);
+      if (hit) {
+        const int m = n + __popc(gm & ((1u << k) - 1u));
+        if (m < a.maxn) {
+          if (a.stage)
+            s_list[grp * ls + m] = jv[u];
+          else
+            a.vl_nbr[(size_t)m * a.np + s] = jv[u];
+        }
+      }
+      n += __popc(gm);
+    }
   }
-  a.vl_n[s] = n;
+  if (k == 0) {
+    if (home && n > a.maxn) atomicOr(a.flags, FLAG_NLIST);
+    n = home ? min(n, a.maxn) : 0;
+    s_n[grp] = n;
+    if (home) a.vl_n[s] = n;
+  }
+  if (a.stage) {
+    // the workgroup's lists, row by row: the HPB homes of a row are contiguous in vl_nbr
+    __syncthreads();
+    int nmax = 0;
+#pragma unroll
+    for (int h = 0; h < HPB; ++h) nmax = max(nmax, s_n[h]);
+    const int s0 = blockIdx.x * HPB;
+    for (int idx = tid; idx < nmax * HPB; idx += 256) {
+      const int m = idx / HPB, h = idx - m * HPB;
+      if (m < s_n[h]) a.vl_nbr[(size_t)m * a.np + s0 + h] = s_list[h * ls + m];
+    }
+  }
 }
 
 // Pair loop over the Verlet list (UTwoBodyA over jpnlist, Src/energy.F90:445-544): one thread
@@ -848,7 +961,10 @@ int launch_vlist_build(Ctx* c) {
   v.dmax = c->d_dmax;
   v.flags = c->d_flags;
   prof_begin(c, MSG_K_VLIST);
-  hipLaunchKernelGGL(k_vlist_build, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, v);
+  const size_t lds = (size_t)(256 / kVlLph) * (c->vl_maxn + 1) * sizeof(int);
+  v.stage = lds <= (size_t)kVlStageBytes;
+  hipLaunchKernelGGL(k_vlist_build, dim3((c->np + 256 / kVlLph - 1) / (256 / kVlLph)), dim3(256),
+                     v.stage ? lds : 0, c->stream, v);
   prof_end(c, MSG_K_VLIST);
   MSG_HIP(hipGetLastError());
   c->vl_builds++;
