@@ -189,7 +189,8 @@ int launch_reduce(Ctx* c, bool presum) {
   return MSG_OK;
 }
 
-// The UTotal launch sequence (no host synchronisation inside; capturable).  Counters and
+// The UTotal launch sequence (capturable; with a Verlet list the host waits once, for the
+// skin test, which is why that mode launches eagerly).  Counters and
 // flags are zero on entry (cleared at allocation and by the previous pass's reduction).
 // With SPME the work forks after the binning pass: the cell list (scan, scatter, ranking,
 // Verlet rebuild), the pair kernel and the pair-row sum run on a second stream beside the
@@ -200,14 +201,32 @@ int launch_reduce(Ctx* c, bool presum) {
 int utotal_sequence(Ctx* c) {
   int e;
   bool rebuild = false;
-  if (c->P.list_mode == MSG_LIST_VERLET)
-    if ((e = verlet_check(c, rebuild))) return e;
+  const bool verlet = c->P.list_mode == MSG_LIST_VERLET;
   const bool branch = c->P.ewald_mode == MSG_EWALD_SPM && !c->prof;
+  // Verlet list with SPME: the binning without cells and the brick sort do not depend on the
+  // skin test, so they are queued before the host waits for its result (and run before the
+  // fork, where the pair kernel cannot crowd out the brick scatter's atomics); a rebuild
+  // bins the cells on the pair branch.
+  const bool early = verlet && branch;
+  if (early) {
+    bool pending;
+    if ((e = verlet_check_begin(c, pending))) return e;
+    if ((e = launch_bin(c, false)) || (e = launch_sort(c, 2))) return e;
+    if (pending) {
+      if ((e = verlet_check_end(c, rebuild))) return e;
+    } else {
+      rebuild = true;
+    }
+  } else if (verlet) {
+    if ((e = verlet_check(c, rebuild))) return e;
+  }
   // with a Verlet list the cell order is only needed to rebuild the list (later users such
   // as DUTotal complete it on demand)
-  const bool cells_needed = c->P.list_mode != MSG_LIST_VERLET || rebuild;
-  if ((e = launch_bin(c, cells_needed))) return e;
+  const bool cells_needed = !verlet || rebuild;
+  if (!early && (e = launch_bin(c, cells_needed))) return e;
   auto cell_work = [&]() -> int {
+    if (early && rebuild)
+      if (int e2 = launch_bin(c, true, false)) return e2;
     // without a rebuild the Verlet pair kernel reads positions in particle order, so the
     // cell slots are not sorted at all (complete_cells does it for later users)
     const int which = (cells_needed ? 1 : 0) | (branch ? 0 : 2);
@@ -222,14 +241,9 @@ int utotal_sequence(Ctx* c) {
   };
   c->cells_valid = cells_needed;
   if (branch) {
-    // Without cell work (a Verlet-list pass without a rebuild) the pair kernel would fill the
-    // device before the brick scatter, a kernel of scattered atomics, got workgroups: the
-    // brick sort then runs before the fork.
-    const bool sort_first = !cells_needed;
-    if (sort_first && (e = launch_sort(c, 2))) return e;
     MSG_HIP(hipEventRecord(c->ev_fork, c->stream));
     // the mesh branch is the longer chain: recorded first, it is dispatched first
-    if (!sort_first && (e = launch_sort(c, 2))) return e;
+    if (!early && (e = launch_sort(c, 2))) return e;
     if ((e = launch_spme_mesh(c))) return e;
     MSG_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     hipStream_t main = c->stream;
@@ -341,7 +355,8 @@ int msg_create(const msg_params* p, msg_ctx** out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_dmax, hipEventDisableTiming) != hipSuccess)
     return fail(set_error(MSG_ERR_HIP, "stream/event creation failed"));
   const int np = c->np;
   if ((e = dalloc(c, (void**)&c->d_ro, 3 * np * sizeof(double)))) return fail(e);
@@ -400,6 +415,7 @@ void msg_destroy(msg_ctx* h) {
   if (c->side) hipStreamDestroy(c->side);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->ev_dmax) hipEventDestroy(c->ev_dmax);
   if (c->ev_batch) hipEventDestroy(c->ev_batch);
   if (c->stream) hipStreamDestroy(c->stream);
   delete h;

@@ -94,6 +94,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;        // second branch of the UTotal graph (pair kernel)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_dmax = nullptr;  // the displacement maximum's copy to the host has landed
   hipEvent_t ev_batch = nullptr;     // orders the context's own work before a batched pass
   int device = 0;
   long long bytes = 0;
@@ -230,7 +231,7 @@ int pack_table(Ctx* c);
 
 // Launch sequences (each .hip file)
 int lists_setup(Ctx* c);                           // cell grid + buffers
-int launch_bin(Ctx* c, bool cells = true);         // K1 cell id + mesh bin + counts + dipole
+int launch_bin(Ctx* c, bool cells = true, bool mesh = true);  // K1 cell id + mesh bin + counts + dipole
 int launch_sort(Ctx* c, int which);                // scan + scatter: 1 cells, 2 bricks + dipole, 3 both
 int launch_finalize(Ctx* c);                       // cell order (after launch_sort)
 int launch_pair(Ctx* c, bool last);                // K4
@@ -260,6 +261,8 @@ int utotal_refresh(Ctx* c);                        // UTotal sequence + sync (fo
 int utotal_sequence(Ctx* c);
 int verlet_setup(Ctx* c);
 int verlet_check(Ctx* c, bool& rebuild);            // displacement test (host sync)
+int verlet_check_begin(Ctx* c, bool& pending);       // its first half: queue the maximum's copy
+int verlet_check_end(Ctx* c, bool& rebuild);         // its second half: wait and decide
 int launch_vlist_build(Ctx* c);
 int launch_disp(Ctx* c);
 

@@ -89,6 +89,7 @@ __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
     dy = q * r[1];
     dz = q * r[2];
   }
+  if (!a.dip_part) return;  // the same for the whole grid
   dx = msgd::block_sum(dx, sh);
   dy = msgd::block_sum(dy, sh);
   dz = msgd::block_sum(dz, sh);
@@ -593,12 +594,17 @@ static FinalizeArgs finalize_args(Ctx* c) {
 
 static int finalize_blocks(const FinalizeArgs& f) { return (f.ncell + 3) / 4; }
 
-int launch_bin(Ctx* c, bool cells) {
+int launch_bin(Ctx* c, bool cells, bool mesh) {
   const int nb = (c->np + 255) / 256;
   BinArgs a = bin_args(c);
   if (!cells) {
     a.use_cells = 0;
     a.rank_cell = nullptr;
+  }
+  if (!mesh) {  // cell ids and counts alone, for a pass whose bricks were binned before
+    a.use_mesh = 0;
+    a.rank_brick = nullptr;
+    a.dip_part = nullptr;
   }
   prof_begin(c, MSG_K_BIN);
   if (c->fp64)

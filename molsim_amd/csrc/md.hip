@@ -209,17 +209,35 @@ int launch_disp(Ctx* c) {
   return MSG_OK;
 }
 
-int verlet_check(Ctx* c, bool& rebuild) {
-  if (!c->vl_valid) {
-    rebuild = true;
-    return MSG_OK;
-  }
+// The skin test in two halves, so that work that does not depend on its outcome can be queued
+// in between: the first queues the copy of the displacement maximum to the host (pending =
+// false when no list exists yet: a build is due anyway); the second waits for the copy alone
+// and decides.
+int verlet_check_begin(Ctx* c, bool& pending) {
+  pending = false;
+  if (!c->vl_valid) return MSG_OK;
   if (!c->disp_done)
     if (int e = launch_disp(c)) return e;
   c->disp_done = false;
   MSG_HIP(hipMemcpyAsync(c->h_dmax, c->d_dmax, (size_t)kDmaxSlots * kDmaxStride * sizeof(unsigned int),
                          hipMemcpyDeviceToHost, c->stream));
-  MSG_HIP(hipStreamSynchronize(c->stream));
+  MSG_HIP(hipEventRecord(c->ev_dmax, c->stream));
+  pending = true;
+  return MSG_OK;
+}
+
+int verlet_check(Ctx* c, bool& rebuild) {
+  bool pending;
+  if (int e = verlet_check_begin(c, pending)) return e;
+  if (!pending) {
+    rebuild = true;
+    return MSG_OK;
+  }
+  return verlet_check_end(c, rebuild);
+}
+
+int verlet_check_end(Ctx* c, bool& rebuild) {
+  MSG_HIP(hipEventSynchronize(c->ev_dmax));
   unsigned int bits = 0;
   for (int k = 0; k < kDmaxSlots; ++k) bits = std::max(bits, c->h_dmax[k * kDmaxStride]);
   float d2;
