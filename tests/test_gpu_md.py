@@ -61,6 +61,27 @@ def test_verlet_utotal_matches_oracle(precision, tol):
     eng.close()
 
 
+@pytest.mark.parametrize("precision,tol", [("mixed", 1e-5), ("fp64", 1e-9)])
+def test_verlet_dense_list_matches_oracle(precision, tol):
+    """A dense list (about 550 partners per particle, list width 1152): too wide for the build
+    to stage in LDS, so its partners go straight to the list; one collapsed cell."""
+    sp = systems.small_spme_system(seed=8, n=1200, L=44.0, nmesh=32, alpha=0.22, rcut=18.0)
+    sp = systems.SystemSpec(**{**sp.__dict__, "list_mode": "verlet", "drnlist": 3.0})
+    print(f"dense verlet n={sp.n} precision={precision}", flush=True)
+    tab = oracle.build_table(sp.pot)
+    osp = systems.SystemSpec(**{**sp.__dict__, "list_mode": "cell"})
+    eng = engine(sp, tab, precision)
+    compare(eng.utotal(), oracle_system(osp, tab).utotal(), tol, tol)
+    rng = np.random.default_rng(2)
+    ro2 = sp.ro + rng.uniform(-1.0, 1.0, sp.ro.shape) / np.sqrt(3)
+    ro2 = (ro2 + 22.0) % 44.0 - 22.0
+    eng.set_positions(ro2)
+    got = eng.utotal()
+    assert eng.nlist_builds() == 1
+    compare(got, oracle_system(systems.SystemSpec(**{**osp.__dict__, "ro": ro2}), tab).utotal(), tol, tol)
+    eng.close()
+
+
 def _vv_reference(sp, tab, mass, v0, dt, nsteps, scale):
     """Velocity Verlet with oracle forces (the VelVer update order and PBC wrap)."""
     L = sp.box[0]
