@@ -123,6 +123,13 @@ int msg_set_positions_device(msg_ctx* ctx, const double* d_ro);
 /* EwaldSetup: std k-factors or SPME influence function for the current box/parameters. */
 int msg_ewald_setup(msg_ctx* ctx);
 
+/* Box change, as SetBoxParam + EwaldSetup + InitCellList do in VolChange (Src/mc.F90:3423-3436)
+ * and after a box rescaling in MD (Src/md.F90:699-700): boxlen[3] replaces the box, and the
+ * cell grid, the Verlet list width and, after msg_ewald_setup, the Ewald set-up follow it.
+ * The caller then sets the (scaled) positions.  Batches created before the change refuse to
+ * run (MSG_ERR_STATE); execution units pick the new box up at their next launch. */
+int msg_set_box(msg_ctx* ctx, const double* boxlen);
+
 /* NList(iWriteInput/iSimulationStep): build the cell list (and the Verlet list in
  * MSG_LIST_VERLET mode) from the current positions. msg_utotal rebuilds the cell list
  * itself on every call; in Verlet mode it rebuilds the list on skin violation. */

@@ -237,6 +237,7 @@ static void unpack_trials(const std::vector<Ctx*>& cs, const int* ntrial, const 
 struct msg_batch {
   std::vector<msg_ctx*> ctx;
   std::vector<Ctx*> cs;
+  std::vector<long long> box_gen;  // each context's box generation when the batch was planned
   hipStream_t origin = nullptr, side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   SegOwner own;                 // the pass plan and trial groups (built once)
@@ -269,6 +270,7 @@ int msg_batch_create(msg_ctx** ctxs, int n, msg_batch** out) {
     if (int e = check_batchable(c)) return fail(e);
     b->ctx.push_back(ctxs[i]);
     b->cs.push_back(c);
+    b->box_gen.push_back(c->box_gen);
   }
   // the contexts' own pending work must be done before the batch touches their buffers
   for (Ctx* c : b->cs)
@@ -315,8 +317,17 @@ void msg_batch_destroy(msg_batch* b) {
   delete b;
 }
 
+// The plan holds every system's box: a box changed since (msg_set_box) makes it stale.
+static int batch_current(const msg_batch* b, const char* who) {
+  for (size_t s = 0; s < b->cs.size(); ++s)
+    if (b->cs[s]->box_gen != b->box_gen[s])
+      return set_error(MSG_ERR_STATE, std::string(who) + ": a system's box changed after msg_batch_create");
+  return MSG_OK;
+}
+
 int msg_batch_utotal(msg_batch* b) {
   if (!b) return set_error(MSG_ERR_ARG, "msg_batch_utotal: null batch");
+  if (int e = batch_current(b, "msg_batch_utotal")) return e;
   if (int e = order_after_contexts(b->cs, b->origin)) return e;
   MSG_HIP(hipGraphLaunch(b->exec, b->origin));
   for (Ctx* c : b->cs) c->fresh = c->cells_valid = true;
@@ -343,6 +354,7 @@ int msg_batch_dutotal(msg_batch* b, const int* ntrial, const int* ip, const doub
                       int* ovl, double* du_tot) {
   if (!b || !ntrial || !ip || !rotm || !ovl || !du_tot)
     return set_error(MSG_ERR_ARG, "msg_batch_dutotal: bad arguments");
+  if (int e = batch_current(b, "msg_batch_dutotal")) return e;
   // the trials read phi / S / cells of the current positions
   bool stale = false;
   for (Ctx* c : b->cs) stale |= !c->fresh;

@@ -542,6 +542,34 @@ int msg_ewald_setup(msg_ctx* h) {
   return MSG_OK;
 }
 
+// Box change (VolChange and the other callers of SetBoxParam + EwaldSetup + InitCellList,
+// Src/mc.F90:3423-3436, md.F90:699-700): the cell grid and its buffers, the Verlet list
+// width, the SPME influence function or the std-Ewald k-vectors follow the new box; the
+// caller then sets the (scaled) positions.  Everything derived from the old box is dropped.
+int msg_set_box(msg_ctx* h, const double* boxlen) {
+  if (!h || !boxlen) return set_error(MSG_ERR_ARG, "msg_set_box: null argument");
+  Ctx* c = &h->c;
+  for (int d = 0; d < 3; ++d)
+    if (!(boxlen[d] > 0.0)) return set_error(MSG_ERR_ARG, "msg_set_box: box lengths must be > 0");
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  for (int d = 0; d < 3; ++d) c->P.boxlen[d] = boxlen[d];
+  int e = lists_setup(c);
+  if (!e && c->P.list_mode == MSG_LIST_VERLET) e = verlet_setup(c);
+  if (!e && c->have_ewald) e = msg_ewald_setup(h);
+  if (e) return e;
+  c->npart_pair = pair_blocks(c);
+  if (c->npart_pair > c->npart_pair_cap) return set_error(MSG_ERR_ARG, "msg_set_box: pair partials exceed reserve");
+  if ((e = launch_zero(c))) return e;  // the counters of the new grid
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  c->fresh = false;
+  c->cells_valid = false;
+  c->vl_valid = false;
+  c->have_gk = false;
+  c->graph_ok = false;
+  c->box_gen++;
+  return MSG_OK;
+}
+
 int msg_nlist(msg_ctx* h) {
   if (!h) return set_error(MSG_ERR_ARG, "msg_nlist: null context");
   Ctx* c = &h->c;

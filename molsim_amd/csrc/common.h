@@ -199,6 +199,9 @@ struct Ctx {
                                    // slots kDmaxStride apart; the maximum is that of all slots
   unsigned int* h_dmax = nullptr;  // pinned copy of the slots
   bool vl_valid = false;
+  int ncell_cap = 0;             // cells the cell buffers hold
+  int vl_cap = 0;                // list width vl_nbr holds
+  long long box_gen = 0;         // bumped by msg_set_box (a batch planned before is stale)
   bool disp_done = false;        // the MD drift kernel already computed the displacement
   long long vl_builds = 0;
 

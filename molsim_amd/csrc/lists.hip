@@ -478,13 +478,29 @@ int lists_setup(Ctx* c) {
 
   const int np = c->np;
   auto A = [&](void** p, size_t b) { return dalloc(c, p, b); };
+  // per-cell buffers: (re)allocated when the grid outgrows them (a new box); the counters
+  // of a new grid are cleared by the caller
+  if (g.ncell > c->ncell_cap) {
+    void** cellbuf[5] = {(void**)&c->d_cell_count, (void**)&c->d_cell_start, (void**)&c->d_cell_fill,
+                         (void**)&c->d_cell_chunk, (void**)&c->d_item_cell};
+    for (void** p : cellbuf)
+      if (*p) {
+        hipFree(*p);
+        *p = nullptr;
+      }
+    if (c->ncell_cap > 0)
+      c->bytes -= (long long)(5 * c->ncell_cap + np / kPairHB + 4) * (long long)sizeof(int);
+    c->ncell_cap = 0;
+    if (int e = A((void**)&c->d_cell_count, g.ncell * sizeof(int))) return e;
+    if (int e = A((void**)&c->d_cell_start, (g.ncell + 1) * sizeof(int))) return e;
+    if (int e = A((void**)&c->d_cell_fill, g.ncell * sizeof(int))) return e;
+    if (int e = A((void**)&c->d_cell_chunk, (g.ncell + 1) * sizeof(int))) return e;
+    if (int e = A((void**)&c->d_item_cell, (g.ncell + np / kPairHB + 2) * sizeof(int))) return e;
+    c->ncell_cap = g.ncell;
+  }
+  if (c->d_cellid) return MSG_OK;  // the per-particle buffers exist already
   if (int e = A((void**)&c->d_cellid, np * sizeof(int))) return e;
-  if (int e = A((void**)&c->d_cell_count, g.ncell * sizeof(int))) return e;
-  if (int e = A((void**)&c->d_cell_start, (g.ncell + 1) * sizeof(int))) return e;
-  if (int e = A((void**)&c->d_cell_fill, g.ncell * sizeof(int))) return e;
-  if (int e = A((void**)&c->d_cell_chunk, (g.ncell + 1) * sizeof(int))) return e;
   if (int e = A((void**)&c->d_M, 4 * sizeof(double))) return e;
-  if (int e = A((void**)&c->d_item_cell, (g.ncell + np / kPairHB + 2) * sizeof(int))) return e;
   if (int e = A((void**)&c->d_cell_tmp, np * sizeof(int))) return e;
   if (int e = A((void**)&c->d_cell_ip, np * sizeof(int))) return e;
   if (int e = A((void**)&c->d_slot_of, np * sizeof(int))) return e;

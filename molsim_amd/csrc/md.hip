@@ -182,9 +182,18 @@ int verlet_setup(Ctx* c) {
   maxn = std::min(((maxn + 31) / 32) * 32, std::max(c->np - 1, 32));
   c->vl_maxn = maxn;
   const int np = c->np;
+  if (maxn > c->vl_cap) {  // the list width grows with the density (a smaller box)
+    if (c->d_vl_nbr) {
+      hipFree(c->d_vl_nbr);
+      c->d_vl_nbr = nullptr;
+      c->bytes -= (long long)c->vl_cap * np * (long long)sizeof(int);
+    }
+    if (int e = dalloc(c, (void**)&c->d_vl_nbr, (size_t)maxn * np * sizeof(int))) return e;
+    c->vl_cap = maxn;
+  }
+  if (c->d_vl_home) return MSG_OK;  // the per-particle buffers exist already
   if (int e = dalloc(c, (void**)&c->d_vl_home, np * sizeof(int))) return e;
   if (int e = dalloc(c, (void**)&c->d_vl_n, np * sizeof(int))) return e;
-  if (int e = dalloc(c, (void**)&c->d_vl_nbr, (size_t)maxn * np * sizeof(int))) return e;
   if (int e = dalloc(c, (void**)&c->d_vl_pos, 4 * (size_t)np * sizeof(double))) return e;
   if (int e = dalloc(c, (void**)&c->d_vl_slot, np * sizeof(int))) return e;
   if (int e = dalloc(c, (void**)&c->d_ro_ref, 3 * (size_t)np * sizeof(double))) return e;

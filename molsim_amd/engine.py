@@ -7,6 +7,7 @@ operations under the reference's names:
 * ``dutotal()``       -- DUTotal(lhsoverlap, ...)     Src/denergy.F90:98-279
 * ``mc_update()``     -- MCUpdate                      Src/mc.F90:5203-5235
 * ``ewald_setup()``   -- EwaldSetup                    Src/energy.F90:6601-7126
+* ``set_box()``       -- SetBoxParam + EwaldSetup + InitCellList (VolChange) Src/mc.F90:3423-3436
 * ``nlist()``         -- NList / SetCellList           Src/nlist.F90:553-669
 * ``build_table()``   -- PotTwoBodyTab1/2/3            Src/potential.F90:868-1258
 
@@ -84,6 +85,7 @@ def lib():
         L.msg_set_positions.argtypes = [_vp, _dp]
         L.msg_set_positions_device.argtypes = [_vp, _vp]
         L.msg_ewald_setup.argtypes = [_vp]
+        L.msg_set_box.argtypes = [_vp, _dp]
         L.msg_nlist.argtypes = [_vp]
         L.msg_utotal.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
         L.msg_utotal_async.argtypes = [_vp, _vp]
@@ -265,6 +267,13 @@ class Engine:
 
     def ewald_setup(self):
         _check(lib().msg_ewald_setup(self.h))
+
+    def set_box(self, boxlen):
+        """Box change (SetBoxParam + EwaldSetup + InitCellList, Src/mc.F90:3423-3436); set the
+        scaled positions afterwards."""
+        box = np.ascontiguousarray(boxlen, dtype=np.float64).reshape(3)
+        _check(lib().msg_set_box(self.h, box))
+        self.spec = self.spec.__class__(**{**self.spec.__dict__, "box": box.copy()})
 
     def nlist(self):
         _check(lib().msg_nlist(self.h))

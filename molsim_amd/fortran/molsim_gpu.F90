@@ -88,6 +88,12 @@ module MolsimGpu
          type(c_ptr), value :: ctx
       end function
 
+      integer(c_int) function msg_set_box(ctx, boxlen) bind(C, name='msg_set_box')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: boxlen(3)
+      end function
+
       integer(c_int) function msg_nlist(ctx) bind(C, name='msg_nlist')
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx

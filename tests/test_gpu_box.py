@@ -1,0 +1,85 @@
+"""GPU parity of a box change (VolChange, Src/mc.F90:3395-3440: SetBoxParam + EwaldSetup +
+InitCellList, then UTotal of the scaled configuration) against the FP64 oracle built for the
+new box.  The box is expanded by 1.25 and then compressed to 0.8 of the start, so the cell grid
+changes both ways (5 -> 6 -> 4 cells per side at rc 12 A); trial energies after the change read
+the new cell order and Ewald set-up.  A batch planned before the change refuses to run.
+Tolerances as the other parity tests: mixed 1e-5, FP64 1e-9; trial energies 2e-5 kJ/mol +
+1e-6 relative (mixed), 1e-8 kJ/mol (FP64)."""
+import numpy as np
+import pytest
+
+import oracle
+from molsim_amd import systems
+from molsim_amd.engine import Batch, MolsimGpuError
+from gpu_helpers import compare, engine, oracle_system
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"mixed": (1e-5, 2e-5, 1e-6), "fp64": (1e-9, 1e-8, 0.0)}
+
+
+def _system(list_mode, ewald):
+    sp = systems.small_spme_system(seed=4, n=400, L=60.0, nmesh=32, alpha=0.3, rcut=12.0,
+                                   list_mode=list_mode)
+    pot = dict(sp.pot, rad=[0.0] * sp.npt)  # no hard core: the compressed box brings ions closer
+    kw = dict(pot=pot, ewald=ewald)
+    if ewald == "std":
+        kw["ncut"] = 6
+    if list_mode == "verlet":
+        kw["drnlist"] = 1.0
+    return systems.SystemSpec(**{**sp.__dict__, **kw})
+
+
+def _oracle_spec(sp, ratio):
+    """The oracle's system for the scaled box (cell list where the GPU uses a Verlet list)."""
+    lm = "cell" if sp.list_mode == "verlet" else sp.list_mode
+    return systems.SystemSpec(**{**sp.__dict__, "ro": sp.ro * ratio, "box": sp.box * ratio, "list_mode": lm})
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("list_mode,ewald", [("cell", "spm"), ("cell", "std"), ("verlet", "spm"), ("all", "std")])
+def test_box_change_matches_oracle(list_mode, ewald, precision):
+    sp = _system(list_mode, ewald)
+    print(f"box change list={list_mode} ewald={ewald} precision={precision}", flush=True)
+    tab = oracle.build_table(sp.pot)
+    ftol, atol, rtol = TOL[precision]
+    eng = engine(sp, tab, precision)
+    compare(eng.utotal(), oracle_system(_oracle_spec(sp, 1.0), tab).utotal(), ftol, ftol)
+    rng = np.random.default_rng(5)
+    for ratio in (1.25, 0.8):
+        eng.set_box(sp.box * ratio)
+        eng.set_positions(sp.ro * ratio)
+        osys = oracle_system(_oracle_spec(sp, ratio), tab)
+        compare(eng.utotal(), osys.utotal(), ftol, ftol)
+        if list_mode == "all":
+            continue
+        L = sp.box[0] * ratio
+        ips = rng.integers(0, sp.n, size=8)
+        rn = (sp.ro[ips] * ratio + rng.uniform(-1.0, 1.0, (8, 3)) + L / 2) % L - L / 2
+        de = eng.dutotal_batch(ips, rn)
+        for t in range(len(ips)):
+            ref = osys.dutotal(int(ips[t]), rn[t])
+            assert de["overlap"][t] == ref["overlap"]
+            if not ref["overlap"]:
+                assert abs(de["du_tot"][t] - ref["du_tot"]) <= atol + rtol * abs(ref["du_tot"]), (ratio, t)
+    eng.close()
+
+
+def test_batch_refuses_a_changed_box():
+    sp = _system("cell", "spm")
+    tab = oracle.build_table(sp.pot)
+    engs = [engine(sp, tab), engine(sp, tab)]
+    batch = Batch(engs)
+    batch.utotal()
+    engs[1].set_box(sp.box * 1.1)
+    engs[1].set_positions(sp.ro * 1.1)
+    with pytest.raises(MolsimGpuError):
+        batch.utotal()
+    batch.close()
+    # a batch planned after the change runs, and matches the oracle for the new box
+    batch = Batch(engs)
+    got = batch.utotal()
+    compare(got[1], oracle_system(_oracle_spec(sp, 1.1), tab).utotal(), 1e-5, 1e-5)
+    batch.close()
+    for e in engs:
+        e.close()
