@@ -393,6 +393,18 @@ def run_electrolyte_mc(args):
     }
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (from /proc/cpuinfo), for the CPU baseline's record."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def run_ionic_md(args):
     """The 1,048,576-particle ionic fluid: SPME 256^3 order 6, rc 18 A Verlet list with a
     2 A skin, velocity-Verlet MD on the device (dt 0.005, masses 20 g/mol, velocities at
@@ -449,7 +461,16 @@ def run_ionic_md(args):
         osys.utotal()
         tcpu = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": sp.n / tcpu, "unit": "particle-steps/s", "cores": 1,
-                               "kind": "port", "sample": f"one oracle UTotal of the full system ({tcpu:.1f} s)"}
+                               "kind": "port", "sample": f"one oracle UTotal of the full system ({tcpu:.1f} s)",
+                               "cpu": cpu_model()}
+        # the same on all host cores (the oracle's OpenMP build)
+        t0 = time.perf_counter()
+        osys.utotal(omp=True)
+        tomp = time.perf_counter() - t0
+        out["cpu_baseline_all_cores"] = {"value": sp.n / tomp, "unit": "particle-steps/s",
+                                         "cores": oracle.threads(omp=True), "kind": "port",
+                                         "sample": f"one UTotal of the full system by the oracle's OpenMP "
+                                                   f"build ({tomp:.1f} s)", "cpu": cpu_model()}
     return out
 
 
@@ -634,7 +655,21 @@ def main():
         out["cpu_baseline"] = {"value": sp.n * nrun / tcpu, "unit": "particle-steps/s", "cores": 1,
                                "kind": "port",
                                "sample": f"{nrun} full UTotal evaluations of the same workload "
-                                         f"({tcpu:.1f} s, host has {os.cpu_count()} cpus)"}
+                                         f"({tcpu:.1f} s, host has {os.cpu_count()} cpus)",
+                               "cpu": cpu_model()}
+        # the same on all host cores (the oracle's OpenMP build), a sample of half the time
+        t0 = time.perf_counter()
+        nomp = 0
+        while True:
+            osys.utotal(omp=True)
+            nomp += 1
+            if time.perf_counter() - t0 > args.cpu_seconds / 2:
+                break
+        tomp = time.perf_counter() - t0
+        out["cpu_baseline_all_cores"] = {"value": sp.n * nomp / tomp, "unit": "particle-steps/s",
+                                         "cores": oracle.threads(omp=True), "kind": "port",
+                                         "sample": f"{nomp} full UTotal evaluations by the oracle's OpenMP "
+                                                   f"build ({tomp:.1f} s)", "cpu": cpu_model()}
         fr = ref["force"]
         out["err_vs_fp64_oracle"] = {
             "force_rel_l2": float(np.linalg.norm(res["force"] - fr) / np.linalg.norm(fr)),

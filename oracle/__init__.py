@@ -59,6 +59,27 @@ def lib():
     return _lib
 
 
+_LIB_OMP_PATH = os.path.join(_HERE, "liboracle_omp.so")
+_lib_omp = None
+
+
+def lib_omp():
+    """The same oracle built with OpenMP (pair loop, FFT lines and gather shared out over the
+    host's threads): only for timing the CPU baseline on all cores, never for checking."""
+    global _lib_omp
+    if _lib_omp is None:
+        if not os.path.exists(_LIB_OMP_PATH):
+            subprocess.run(["make", "-s", "-C", _HERE, "-B", "liboracle_omp.so"], check=True)
+        _lib_omp = C.CDLL(_LIB_OMP_PATH)
+        _declare(_lib_omp)
+    return _lib_omp
+
+
+def threads(omp: bool = False) -> int:
+    """Threads the oracle's loops use (1 for the checking oracle)."""
+    return int((lib_omp() if omp else lib()).orc_threads())
+
+
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
@@ -321,12 +342,14 @@ class System:
     def set_positions(self, ro):
         self.ro[:] = np.asarray(ro, dtype=np.float64).reshape(-1)
 
-    def utotal(self):
+    def utotal(self, omp: bool = False):
+        """UTotal; omp=True runs the OpenMP build (multi-core timing only)."""
         nptpt = self.npt * (self.npt + 1) // 2
         ut = np.zeros(nptpt + 1)
         f = np.zeros(self.ro.size)
         urec, utot, vir = C.c_double(), C.c_double(), C.c_double()
-        _check(lib().orc_utotal(C.byref(self.s), ut, C.byref(urec), C.byref(utot), f, C.byref(vir)),
+        L = lib_omp() if omp else lib()
+        _check(L.orc_utotal(C.byref(self.s), ut, C.byref(urec), C.byref(utot), f, C.byref(vir)),
                "utotal")
         return dict(u_twob=ut, u_rec=urec.value, u_tot=utot.value, force=f.reshape(-1, 3),
                     virial=vir.value)

@@ -28,6 +28,9 @@
 // index tp(i,j) = MOLSIM iptpt(i+1,j+1)-1, u_twob[0] = sum, u_twob[1+tp] per pair.
 // Meshes are x-fastest: Q[x + K*(y + K*z)], FQ[kx + (K/2+1)*(ky + K*kz)].
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -85,6 +88,15 @@ void orc_bspline(int n, double pos, int* kmax, double* vals, double* ders) {
 
 // ----------------------------------------------------------------------------------
 // Random, Src/mollib.F90:2689-2713.  state = {ix, iy}; *am is the scale; *idum the seed.
+// Threads the oracle's loops use: 1, or OpenMP's count in liboracle_omp.so.
+int orc_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 double orc_random(int32_t* idum, int32_t* state, double* am) {
   const int32_t ia = 16807, im = 2147483647, iq = 127773, ir = 2836;
   int32_t& ix = state[0];
@@ -452,9 +464,9 @@ int orc_utwob_cell(int np, int npt, const double* ro, const int* ipt, const doub
   int nptpt = npt * (npt + 1) / 2;
   for (int k = 0; k <= nptpt; ++k) u_twob[k] = 0;
   double vir = 0, rcut2 = rcut * rcut;
-  std::vector<int> nb(nst);
-  for (int i = 0; i < np; ++i) {
-    int nn = cell_neigh(cellof[i], nc, stencil, nst, nb.data());
+  // the pairs (i, j > i) of home i, added into f / ut / v
+  auto home = [&](int i, int* nb, double* f, double* ut, double& v) -> int {
+    int nn = cell_neigh(cellof[i], nc, stencil, nst, nb);
     for (int s = 0; s < nn; ++s) {
       for (int j = head[nb[s]]; j >= 0; j = next[j]) {
         if (j <= i) continue;
@@ -468,15 +480,46 @@ int orc_utwob_cell(int np, int npt, const double* ro, const int* ipt, const doub
         double us, fs;
         int e = table_uf(ubuf, nbuf, iubuflow[tp], r2, r2atat[tp], r2umin[tp], us, fs);
         if (e) return e;
-        u_twob[1 + tp] += us;
+        ut[1 + tp] += us;
         for (int k = 0; k < 3; ++k) {
-          force[3 * i + k] += fs * d[k];
-          force[3 * j + k] -= fs * d[k];
+          f[3 * i + k] += fs * d[k];
+          f[3 * j + k] -= fs * d[k];
         }
-        vir -= fs * r2;
+        v -= fs * r2;
       }
     }
+    return 0;
+  };
+#ifdef _OPENMP
+  // liboracle_omp.so (CPU timing on all host cores only): homes shared out over the threads,
+  // each adding into arrays of its own, which are summed at the end
+  int err = 0;
+#pragma omp parallel
+  {
+    std::vector<int> nbt(nst);
+    std::vector<double> ft(3 * (size_t)np, 0.0), utt(nptpt + 1, 0.0);
+    double vt = 0;
+#pragma omp for schedule(dynamic, 64)
+    for (int i = 0; i < np; ++i) {
+      int e = home(i, nbt.data(), ft.data(), utt.data(), vt);
+      if (e) {
+#pragma omp atomic write
+        err = e;
+      }
+    }
+#pragma omp critical
+    {
+      for (size_t k = 0; k < ft.size(); ++k) force[k] += ft[k];
+      for (int k = 1; k <= nptpt; ++k) u_twob[k] += utt[k];
+      vir += vt;
+    }
   }
+  if (err) return err;
+#else
+  std::vector<int> nb(nst);
+  for (int i = 0; i < np; ++i)
+    if (int e = home(i, nb.data(), force, u_twob, vir)) return e;
+#endif
   double s = 0;
   for (int k = 1; k <= nptpt; ++k) s += u_twob[k];
   u_twob[0] = s;
@@ -659,15 +702,23 @@ static void fft1(cplx* a, int n, int stride, int sign, std::vector<cplx>& buf) {
   }
 }
 
+// (the line loops are shared out over the threads in liboracle_omp.so; each line is
+// transformed the same way either way)
 static void fft3_full(cplx* a, int K, int sign) {
-  std::vector<cplx> buf;
   size_t K2 = (size_t)K * K;
-  for (int z = 0; z < K; ++z)
-    for (int y = 0; y < K; ++y) fft1(a + (size_t)K * (y + (size_t)K * z), K, 1, sign, buf);
-  for (int z = 0; z < K; ++z)
-    for (int x = 0; x < K; ++x) fft1(a + x + K2 * z, K, K, sign, buf);
-  for (int y = 0; y < K; ++y)
-    for (int x = 0; x < K; ++x) fft1(a + x + (size_t)K * y, K, (int)K2, sign, buf);
+#pragma omp parallel
+  {
+    std::vector<cplx> buf;
+#pragma omp for collapse(2)
+    for (int z = 0; z < K; ++z)
+      for (int y = 0; y < K; ++y) fft1(a + (size_t)K * (y + (size_t)K * z), K, 1, sign, buf);
+#pragma omp for collapse(2)
+    for (int z = 0; z < K; ++z)
+      for (int x = 0; x < K; ++x) fft1(a + x + K2 * z, K, K, sign, buf);
+#pragma omp for collapse(2)
+    for (int y = 0; y < K; ++y)
+      for (int x = 0; x < K; ++x) fft1(a + x + (size_t)K * y, K, (int)K2, sign, buf);
+  }
 }
 
 // fftw_plan_dft_r2c_3d(K,K,K) semantics on an x-fastest array: out[(K/2+1)*K*K] complex.
@@ -856,8 +907,11 @@ double orc_spme_energy(int K, const double* energyfac, const double* Q) {
 // Force gather of UEwaldRecSPM, Src/energy.F90:1258-1286 (phi = convolved mesh).
 void orc_spme_gather(int na, const double* r, const double* az, const double* box, int K,
                      int order, double epsi4pi, const double* phi, double* force) {
+#pragma omp parallel
+  {
   std::vector<double> th(3 * order), dth(3 * order);
   int mm[3];
+#pragma omp for
   for (int a = 0; a < na; ++a) {
     for (int d = 0; d < 3; ++d) {
       int km;
@@ -879,6 +933,7 @@ void orc_spme_gather(int na, const double* r, const double* az, const double* bo
       }
     }
     for (int d = 0; d < 3; ++d) force[3 * a + d] -= az[a] * es[d] * (K / box[d]) * epsi4pi;
+  }
   }
 }
 

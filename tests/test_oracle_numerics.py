@@ -89,3 +89,19 @@ def test_random_structure(orc):
     assert abs(xs.mean() - 0.5) < 0.01
     r2 = orc.Rng(-12345)
     assert np.array_equal(xs[:50], [r2() for _ in range(50)])
+
+
+def test_openmp_build_agrees_with_the_oracle():
+    """liboracle_omp.so (the multi-core CPU baseline's timing build) computes what the oracle
+    does; only the summation order of the shared-out loops differs."""
+    import oracle
+    from molsim_amd import systems
+    sp = systems.small_spme_system(seed=2, n=300, L=40.0, nmesh=32, alpha=0.3, rcut=12.0)
+    tab = oracle.build_table(sp.pot)
+    osys = oracle.System(sp.ro, sp.ipt, sp.az, sp.box, sp.rcut, sp.pot["epsi4pi"], tab, npt=sp.npt,
+                         list_mode=1, ewald=2, surf=sp.lsurf, alpha=sp.alpha, K=sp.nmesh, order=sp.order)
+    a, b = osys.utotal(), osys.utotal(omp=True)
+    assert oracle.threads() == 1
+    scale = abs(a["u_tot"]) + abs(a["u_rec"]) + np.abs(a["u_twob"]).max()
+    assert abs(a["u_tot"] - b["u_tot"]) < 1e-12 * scale
+    assert np.abs(a["force"] - b["force"]).max() < 1e-12 * np.abs(a["force"]).max()
