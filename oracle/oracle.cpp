@@ -814,6 +814,43 @@ void orc_spme_spread(int na, const double* r, const double* az, const double* bo
                      int order, double* Q) {
   size_t n = (size_t)K * K * K;
   for (size_t i = 0; i < n; ++i) Q[i] = 0;
+#ifdef _OPENMP
+  // liboracle_omp.so (CPU timing on all host cores only): particles shared out over the
+  // threads, each spreading into a mesh of its own; the meshes are summed at the end
+  int nth = omp_get_max_threads();
+  std::vector<std::vector<double>> Qt(nth);
+#pragma omp parallel
+  {
+    const int t = omp_get_thread_num();
+    Qt[t].assign(n, 0.0);
+    double* Qp = Qt[t].data();
+    std::vector<double> th(3 * order);
+    int mm[3];
+#pragma omp for
+    for (int a = 0; a < na; ++a) {
+      for (int d = 0; d < 3; ++d) {
+        int km;
+        orc_bspline(order, K / box[d] * r[3 * a + d], &km, &th[d * order], nullptr);
+        mm[d] = ((km + K) % K + K) % K;
+      }
+      for (int iz = 0; iz < order; ++iz) {
+        int z = (K + mm[2] - iz) % K;
+        double qz = az[a] * th[2 * order + iz];
+        for (int iy = 0; iy < order; ++iy) {
+          int y = (K + mm[1] - iy) % K;
+          double qyz = qz * th[order + iy];
+          for (int ix = 0; ix < order; ++ix) {
+            int x = (K + mm[0] - ix) % K;
+            Qp[x + (size_t)K * (y + (size_t)K * z)] += qyz * th[ix];
+          }
+        }
+      }
+    }
+#pragma omp for
+    for (long long i = 0; i < (long long)n; ++i)
+      for (int u = 0; u < nth; ++u) Q[i] += Qt[u][i];
+  }
+#else
   std::vector<double> th(3 * order);
   int mm[3];
   for (int a = 0; a < na; ++a) {
@@ -835,6 +872,7 @@ void orc_spme_spread(int na, const double* r, const double* az, const double* bo
       }
     }
   }
+#endif
 }
 
 // SPMFFTRec, Src/energy.F90:6525-6587: r2c(Q); E = sum energyfac |F|^2; W -= sum virfac
