@@ -83,3 +83,15 @@ def test_batch_refuses_a_changed_box():
     batch.close()
     for e in engs:
         e.close()
+
+
+def test_set_box_rejects_bad_lengths():
+    sp = _system("cell", "spm")
+    tab = oracle.build_table(sp.pot)
+    eng = engine(sp, tab)
+    for bad in ([0.0, 60.0, 60.0], [60.0, -1.0, 60.0], [60.0, 60.0, np.nan]):
+        with pytest.raises(MolsimGpuError):
+            eng.set_box(np.array(bad))
+    # the context is unchanged and still evaluates its system
+    compare(eng.utotal(), oracle_system(_oracle_spec(sp, 1.0), tab).utotal(), 1e-5, 1e-5)
+    eng.close()
