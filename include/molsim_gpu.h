@@ -7,6 +7,8 @@
  *   msg_set_table    <- the ubuf/iubuflow/r2umin/r2atat state those routines leave behind
  *                       (Src/mol.F90:1087-1135, Src/particle.F90:2300-2308)
  *   msg_ewald_setup  <- EwaldSetup                Src/energy.F90:6601-7126
+ *   msg_set_box      <- SetBoxParam + EwaldSetup + InitCellList after a box change (VolChange)
+ *                                                 Src/mc.F90:3423-3436, Src/md.F90:699-700
  *   msg_nlist        <- NList / InitCellList / SetCellList   Src/nlist.F90:553-669,
  *                       Src/celllist.F90:76-410
  *   msg_utotal       <- UTotal(iStage)            Src/energy.F90:280-434
