@@ -227,8 +227,11 @@ int utotal_sequence(Ctx* c) {
   auto cell_work = [&]() -> int {
     if (early && rebuild)
       if (int e2 = launch_bin(c, true, false)) return e2;
-    // without a rebuild the Verlet pair kernel reads positions in particle order, so the
-    // cell slots are not sorted at all (complete_cells does it for later users)
+    // without a rebuild the Verlet pair kernel reads its slot records, refreshed here from
+    // the current positions, and the cell slots are not sorted at all (complete_cells does
+    // it for later users)
+    if (verlet && !rebuild)
+      if (int e2 = launch_vl_records(c)) return e2;
     const int which = (cells_needed ? 1 : 0) | (branch ? 0 : 2);
     if (int e2 = launch_sort(c, which)) return e2;
     if (cells_needed)

@@ -264,6 +264,7 @@ int utotal_refresh(Ctx* c);                        // UTotal sequence + sync (fo
 int utotal_sequence(Ctx* c);
 int verlet_setup(Ctx* c);
 int verlet_check(Ctx* c, bool& rebuild);            // displacement test (host sync)
+int launch_vl_records(Ctx* c);                     // Verlet slot records of the current positions
 int verlet_check_begin(Ctx* c, bool& pending);       // its first half: queue the maximum's copy
 int verlet_check_end(Ctx* c, bool& rebuild);         // its second half: wait and decide
 int launch_vlist_build(Ctx* c);

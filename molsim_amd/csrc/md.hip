@@ -1,14 +1,14 @@
 // Verlet-list displacement test (NList skin rule, Src/nlist.F90:597-621) and the
 // velocity-Verlet caller of UTotal (VelVer, Src/md.F90:252-410), on the device.
 //
-//   k_disp   max |ro - ro_ref|^2 over all particles since the last list build (minimum image);
-//            copies the positions into the list's slot-ordered records.
+//   k_disp   max |ro - ro_ref|^2 over all particles since the last list build (minimum image).
 //            The reference rebuilds when sqrt(r2max1) + sqrt(r2max2) > drnlist over its two
 //            largest accumulated displacements; here the list is rebuilt when
 //            2 * max |dr| > drnlist, which is never later than the reference's rule.
 //   k_drift  ro += dt v + dt^2/2 a, wrapped into the box (PBC, Src/md.F90:289-298), and the
-//            displacement maximum of the moved positions (also written into the list's
-//            slot-ordered records).
+//            displacement maximum of the moved positions.
+//   k_vl_records  the list's slot-ordered records of the current positions (each pass
+//            without a rebuild; the build writes them itself).
 //   k_kick   a = F / m, v += dt/2 (a_old + a) (Src/md.F90:348-358), kinetic-energy partials.
 //   k_mdlog  per-step record of U_tot and the kinetic energy.
 #include <cstring>
@@ -40,25 +40,28 @@ __device__ inline void dmax_add(unsigned int* dmax, float d2) {
                    fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3])));
 }
 
-// The list's record of a particle: position and type, + kUncharged when uncharged (written
-// whole, so the scattered stores fill their 32-byte sectors).
-__device__ inline void put_record(double4* vl_pos, int slot, const double* r, int type,
-                                  const int* laz, int i) {
-  vl_pos[slot] = make_double4(r[0], r[1], r[2], (double)(type + ((laz && !laz[i]) ? kUncharged : 0)));
+// The Verlet list's slot-ordered records (position, type + kUncharged when uncharged) of the
+// current positions: one thread per slot gathers its particle's (a gather with coalesced
+// stores, where a scatter from the drift would read each record's line before writing it).
+__global__ __launch_bounds__(256) void k_vl_records(int np, const int* __restrict__ vl_home,
+                                                    const double* __restrict__ ro,
+                                                    const int* __restrict__ ipt,
+                                                    const int* __restrict__ laz, double4* vl_pos) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= np) return;
+  const int i = vl_home[s];
+  vl_pos[s] = make_double4(ro[3 * i], ro[3 * i + 1], ro[3 * i + 2],
+                           (double)(ipt[i] + ((laz && !laz[i]) ? kUncharged : 0)));
 }
 
 __global__ __launch_bounds__(256) void k_disp(int np, const double* __restrict__ ro,
                                               const double* __restrict__ ref, Box box,
-                                              double4* __restrict__ vl_pos,
-                                              const int* __restrict__ vl_slot,
-                                              const int* __restrict__ ipt,
-                                              const int* __restrict__ laz, unsigned int* dmax) {
+                                              unsigned int* dmax) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   float d2 = 0.f;
   if (i < np) {
     double s = 0.0;
     const double r[3] = {ro[3 * i], ro[3 * i + 1], ro[3 * i + 2]};
-    put_record(vl_pos, vl_slot[i], r, ipt[i], laz, i);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       double x = r[d] - ref[3 * i + d];
@@ -80,10 +83,7 @@ struct MdArgs {
   const double* force;
   const double* minv;   // per type
   const int* ipt;
-  const int* laz;       // weak charges (or null)
   const double* ref;    // Verlet-list reference positions (or null)
-  double4* vl_pos;      // the list's slot-ordered records (with ref)
-  const int* vl_slot;
   unsigned int* dmax;
   double* ke_part;
 };
@@ -107,7 +107,6 @@ __global__ __launch_bounds__(256) void k_drift(MdArgs a) {
         s += y * y;
       }
     }
-    if (a.ref) put_record(a.vl_pos, a.vl_slot[i], xs, a.ipt[i], a.laz, i);
     d2 = (float)s;
   }
   if (a.ref) dmax_add(a.dmax, d2);  // a.ref is the same for the whole grid
@@ -162,9 +161,6 @@ MdArgs md_args(Ctx* c, double dt) {
   a.minv = c->d_minv;
   a.ipt = c->d_ipt;
   a.ref = c->P.list_mode == MSG_LIST_VERLET && c->vl_valid ? c->d_ro_ref : nullptr;
-  a.laz = c->d_laz;
-  a.vl_pos = (double4*)c->d_vl_pos;
-  a.vl_slot = c->d_vl_slot;
   a.dmax = c->d_dmax;
   a.ke_part = c->d_ke_part;
   return a;
@@ -205,6 +201,13 @@ int verlet_setup(Ctx* c) {
   return MSG_OK;
 }
 
+int launch_vl_records(Ctx* c) {
+  hipLaunchKernelGGL(k_vl_records, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, c->np, c->d_vl_home,
+                     c->d_ro, c->d_ipt, c->d_laz, (double4*)c->d_vl_pos);
+  MSG_HIP(hipGetLastError());
+  return MSG_OK;
+}
+
 int launch_disp(Ctx* c) {
   Box b;
   for (int d = 0; d < 3; ++d) {
@@ -213,7 +216,7 @@ int launch_disp(Ctx* c) {
     b.inv[d] = 1.0 / c->P.boxlen[d];
   }
   hipLaunchKernelGGL(k_disp, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, c->np, c->d_ro,
-                     c->d_ro_ref, b, (double4*)c->d_vl_pos, c->d_vl_slot, c->d_ipt, c->d_laz, c->d_dmax);
+                     c->d_ro_ref, b, c->d_dmax);
   MSG_HIP(hipGetLastError());
   return MSG_OK;
 }
