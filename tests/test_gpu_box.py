@@ -95,3 +95,31 @@ def test_set_box_rejects_bad_lengths():
     # the context is unchanged and still evaluates its system
     compare(eng.utotal(), oracle_system(_oracle_spec(sp, 1.0), tab).utotal(), 1e-5, 1e-5)
     eng.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("mixed", 1e-5), ("fp64", 1e-9)])
+def test_box_change_during_md(precision, tol):
+    """Box rescaling between MD runs (Src/md.F90:699-700): the Verlet list is rebuilt for the
+    new box, and the forces of the rescaled configuration equal the oracle's."""
+    from molsim_amd.engine import Engine
+    from gpu_helpers import cabi_table
+    sp = _system("verlet", "spm")
+    tab = oracle.build_table(sp.pot)
+    eng = Engine(sp, precision=precision, table=cabi_table(tab), force_scale=100.0)
+    rng = np.random.default_rng(9)
+    eng.md_init([20.0, 20.0], rng.normal(size=sp.ro.shape))
+    eng.md_run(5, 0.002)
+    builds = eng.nlist_builds()
+    ro = eng.positions()
+    ratio = 1.05
+    eng.set_box(sp.box * ratio)
+    eng.set_positions(ro * ratio)
+    got = eng.utotal()
+    assert eng.nlist_builds() == builds + 1
+    osp = systems.SystemSpec(**{**sp.__dict__, "ro": ro * ratio, "box": sp.box * ratio, "list_mode": "cell"})
+    ref = oracle_system(osp, tab).utotal()
+    ref["force"] = ref["force"] * 100.0  # the engine's forces carry the force scale
+    compare(got, ref, tol, tol)
+    log = eng.md_run(5, 0.002)
+    assert np.all(np.isfinite(log))
+    eng.close()
