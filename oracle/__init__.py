@@ -25,6 +25,14 @@ Pinning status of each function (how it is checked; see tests/test_oracle_*.py):
                     tests/golden/ref_mollib_vectors.npz).
 * std Ewald cube -- k-vector counts of both regions by enumeration; cube and sphere agree
                     once both hold the converged sum.
+* vlist          -- the set of pairs within rcut + drnlist by numpy brute force (a different
+                    minimum-image formula); each pair exactly once in the MD half lists, in the
+                    row the parity rule names; MC rows = symmetric closure; a pair exactly at
+                    rcut + drnlist is excluded (r2 < rs2cut); cell-list form = O(N^2) form.
+* nlist_skin     -- hand-computed r2max1/r2max2 for displacement sequences where the
+                    reference's update order differs from the true two largest values.
+* celllist update-- a Python list model of head insertion / unlinking after random move
+                    sequences; prev/next consistency; membership equals a fresh SetCellList.
 
 oracle/_ref/libref_mollib.so (``make -C oracle ref``) is the reference's Src/mollib.F90
 compiled where it lies with amdflang, behind the wrapper ref_mollib.F90.  No other file of
@@ -153,6 +161,15 @@ def _declare(L):
     L.orc_dutotal_charge.argtypes = [C.POINTER(OrcSys), C.c_int, _ip, C.POINTER(C.c_int), _dp,
                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.orc_dutotal_charge.restype = C.c_int
+    for f in (L.orc_vlist, L.orc_vlist_cells):
+        f.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int, _ip, _ip]
+        f.restype = C.c_int
+    L.orc_nlist_skin.argtypes = [C.c_int, _dp, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.orc_nlist_skin.restype = C.c_int
+    L.orc_celllist_set.argtypes = [C.c_int, _dp, _dp, _ip, _dp, _ip, _ip, _ip, _ip, _ip]
+    L.orc_celllist_set.restype = C.c_int
+    L.orc_celllist_update.argtypes = [C.c_int, _dp, _dp, _ip, _dp, _ip, _ip, _ip, _ip, _ip]
+    L.orc_celllist_update.restype = C.c_int
 
 
 def _check(rc, what):
@@ -263,6 +280,66 @@ def cell_members(head, nxt):
             j = int(nxt[j])
         out.append(lst)
     return out
+
+
+VLIST_KIND = {"md": 0, "mc": 1}
+
+
+def vlist(kind, ro, box, rcut, drnlist, maxn=None, cells=False, omp=False):
+    """Verlet lists as SetVListMD (kind "md": parity half list) or SetVListMC ("mc": full list)
+    build them: (nneighpn (N,), rows) with rows[i] the 0-based partners of particle i in
+    ascending order.  cells=True takes the cell-list form (same rows, O(N)); omp=True runs it on
+    all host cores (a large-system check, the OpenMP build of the same code)."""
+    ro = np.ascontiguousarray(ro, dtype=np.float64).reshape(-1)
+    n = ro.size // 3
+    if maxn is None:
+        maxn = max(n - 1, 1) if not cells else 2048
+    nn = np.zeros(n, dtype=np.int32)
+    jl = np.zeros(n * maxn, dtype=np.int32)
+    L = lib_omp() if omp else lib()
+    f = L.orc_vlist_cells if cells else L.orc_vlist
+    _check(f(VLIST_KIND[kind], n, ro, np.asarray(box, dtype=np.float64), float(rcut), float(drnlist), maxn, nn, jl),
+           "vlist")
+    jl = jl.reshape(n, maxn)
+    return nn, [jl[i, :nn[i]].copy() for i in range(n)]
+
+
+def nlist_skin(drosum, drnlist):
+    """NList skin test (inlist = 0): (rebuild, r2max1, r2max2) from the displacements summed
+    since the last build, drosum (N, 3)."""
+    d = np.ascontiguousarray(drosum, dtype=np.float64).reshape(-1)
+    m1, m2 = C.c_double(), C.c_double()
+    rb = lib().orc_nlist_skin(d.size // 3, d, float(drnlist), C.byref(m1), C.byref(m2))
+    return bool(rb), m1.value, m2.value
+
+
+class LinkedCellList:
+    """MOLSIM's cell list with its doubly linked per-cell lists (SetCellList, UpdateCellIp)."""
+
+    def __init__(self, ro, box, rcell, rcut):
+        self.box = np.asarray(box, dtype=np.float64)
+        self.nc, self.cs, _ = cell_init(self.box, rcell, rcut)
+        self.ro = np.ascontiguousarray(ro, dtype=np.float64).reshape(-1).copy()
+        n = self.ro.size // 3
+        ncell = int(np.prod(self.nc))
+        self.head = np.zeros(ncell, dtype=np.int32)
+        self.npart = np.zeros(ncell, dtype=np.int32)
+        self.next = np.zeros(n, dtype=np.int32)
+        self.prev = np.zeros(n, dtype=np.int32)
+        self.cellof = np.zeros(n, dtype=np.int32)
+        _check(lib().orc_celllist_set(n, self.ro, self.box, self.nc, self.cs, self.head, self.npart,
+                                      self.next, self.prev, self.cellof), "celllist_set")
+
+    def update(self, ip, rnew):
+        """MCUpdate's ro(ip) <- rnew followed by UpdateCellIp(ip); True when the cell changed."""
+        self.ro[3 * ip:3 * ip + 3] = np.asarray(rnew, dtype=np.float64)
+        return bool(_check(lib().orc_celllist_update(int(ip), self.ro, self.box, self.nc, self.cs, self.head,
+                                                     self.npart, self.next, self.prev, self.cellof),
+                           "celllist_update"))
+
+    def members(self):
+        """Per cell, the order a traversal from iphead along ipnext visits."""
+        return cell_members(self.head, self.next)
 
 
 def fft3_r2c(q):

@@ -23,6 +23,11 @@
 //   orc_dutotal_charge Src/mc.F90:3683-3800, Src/denergy.F90:154-164,1552-1699,3911-4145
 //                      ChargeChange trial with weak charges (DUWeakChargeEwald, UWeakChargeANew/AOld)
 //   orc_fft3_*         FFTW r2c/c2r conventions (unnormalised, forward e^{-i}), own radix-2 code
+//   orc_vlist          Src/nlist.F90:724-759,820-884  SetVListMD (parity half list), SetVListMC
+//   orc_vlist_cells    the same rows from a cell list (large systems)
+//   orc_nlist_skin     Src/nlist.F90:597-621      NList skin test (inlist = 0), r2max1/r2max2 order
+//   orc_celllist_set   Src/celllist.F90:307-327,384-410  SetCellList with ipnext/ipprev
+//   orc_celllist_update Src/celllist.F90:329-382  UpdateCellIp = RmIpFromCell + AddIpToCell
 //
 // Conventions: positions ro[3*i+d] (Fortran ro(3,np)), particle types 0-based, type pair
 // index tp(i,j) = MOLSIM iptpt(i+1,j+1)-1, u_twob[0] = sum, u_twob[1+tp] per pair.
@@ -1319,6 +1324,180 @@ int orc_dutotal_charge(const OrcSys* S, int nptm, const int* ips, int* overlap, 
   *du_rec = S->epsi4pi * drec;
   *du_tot = du_twob[0] + *du_rec;
   return 0;
+}
+
+// ----------------------------------------------------------------------------------
+// Verlet lists at rcut + drnlist, serial branch of SetVList (Src/nlist.F90:724-759 MD,
+// :820-884 MC; rs2cut = (rcut + drnlist)^2 and r2 < rs2cut with PBCr2).
+//   kind 0, SetVListMD: row ip holds partner jp only by the parity rule of :742-745 (1-based
+//          ip + jp odd: the lower index keeps the pair; even: the higher one), so every pair
+//          is listed once; partners in ascending order (the jp loop runs 1..np).
+//   kind 1, SetVListMC: the full symmetric list; the ip < jp loop appends jp to row ip and
+//          ip to row jp, which also leaves every row in ascending order.
+// jpnlist[ip * maxn + k] holds 0-based indices, nneighpn[ip] the row lengths.  Returns 0, or
+// ORC_ERR_NLIST when a row would exceed maxn (the reference stops, :752, :875-877).  O(np^2)
+// pair tests, as the reference.
+enum { ORC_ERR_NLIST = -11 };
+
+static inline bool vlist_md_keeps(int ip, int jp) {
+  // (ip+1) + (jp+1) has the parity of ip + jp
+  const int itemp = (ip + jp) & 1;
+  if (ip < jp && itemp == 0) return false;
+  if (ip > jp && itemp != 0) return false;
+  return ip != jp;
+}
+
+int orc_vlist(int kind, int np, const double* ro, const double* box, double rcut, double drnlist,
+              int maxn, int* nneighpn, int* jpnlist) {
+  const double rs2cut = (rcut + drnlist) * (rcut + drnlist);
+  for (int i = 0; i < np; ++i) nneighpn[i] = 0;
+  if (kind == 0) {
+    for (int ip = 0; ip < np; ++ip)
+      for (int jp = 0; jp < np; ++jp) {
+        if (!vlist_md_keeps(ip, jp)) continue;
+        double d[3] = {ro[3 * ip] - ro[3 * jp], ro[3 * ip + 1] - ro[3 * jp + 1], ro[3 * ip + 2] - ro[3 * jp + 2]};
+        double r2;
+        pbc_r2(box, d, r2);
+        if (r2 < rs2cut) {
+          if (nneighpn[ip] >= maxn) return ORC_ERR_NLIST;
+          jpnlist[(size_t)ip * maxn + nneighpn[ip]++] = jp;
+        }
+      }
+    return 0;
+  }
+  for (int ip = 0; ip < np; ++ip)
+    for (int jp = ip + 1; jp < np; ++jp) {
+      double d[3] = {ro[3 * ip] - ro[3 * jp], ro[3 * ip + 1] - ro[3 * jp + 1], ro[3 * ip + 2] - ro[3 * jp + 2]};
+      double r2;
+      pbc_r2(box, d, r2);
+      if (r2 < rs2cut) {
+        if (nneighpn[ip] >= maxn || nneighpn[jp] >= maxn) return ORC_ERR_NLIST;
+        jpnlist[(size_t)ip * maxn + nneighpn[ip]++] = jp;
+        jpnlist[(size_t)jp * maxn + nneighpn[jp]++] = ip;
+      }
+    }
+  return 0;
+}
+
+// The same lists from a cell list (InitCellList at rcell = rcut + drnlist, Src/celllist.F90:
+// 76-265): the candidates of each particle are the particles of its stencil cells, the same
+// pair test and parity rule are applied, and each row is sorted ascending -- the rows
+// orc_vlist writes, in O(np) instead of O(np^2) (this is the large-system form, for lists of
+// a million particles; its rows equal orc_vlist's bit for bit, which the tests check).
+int orc_vlist_cells(int kind, int np, const double* ro, const double* box, double rcut, double drnlist,
+                    int maxn, int* nneighpn, int* jpnlist) {
+  const double rl = rcut + drnlist, rs2cut = rl * rl;
+  int nc[3];
+  double cs[3];
+  std::vector<int> st(3 * 4096);
+  const int nst = orc_cell_init(box, rl, rl, nc, cs, st.data(), 4096);
+  if (nst < 0) return nst;
+  const int ncell = nc[0] * nc[1] * nc[2];
+  std::vector<int> head(ncell), nxt(np), npart(ncell), cellof(np);
+  if (int e = orc_cell_build(np, ro, box, nc, cs, head.data(), nxt.data(), npart.data(), cellof.data())) return e;
+  int err = 0;
+#pragma omp parallel
+  {
+    std::vector<int> nb(nst), row;
+#pragma omp for schedule(dynamic, 256)
+    for (int ip = 0; ip < np; ++ip) {
+      row.clear();
+      const int nn = cell_neigh(cellof[ip], nc, st.data(), nst, nb.data());
+      for (int s = 0; s < nn; ++s)
+        for (int jp = head[nb[s]]; jp >= 0; jp = nxt[jp]) {
+          if (kind == 0 ? !vlist_md_keeps(ip, jp) : jp == ip) continue;
+          double d[3] = {ro[3 * ip] - ro[3 * jp], ro[3 * ip + 1] - ro[3 * jp + 1], ro[3 * ip + 2] - ro[3 * jp + 2]};
+          double r2;
+          pbc_r2(box, d, r2);
+          if (r2 < rs2cut) row.push_back(jp);
+        }
+      std::sort(row.begin(), row.end());
+      if ((int)row.size() > maxn) {
+#pragma omp atomic write
+        err = ORC_ERR_NLIST;
+        nneighpn[ip] = maxn;
+      } else {
+        nneighpn[ip] = (int)row.size();
+      }
+      for (int k = 0; k < nneighpn[ip]; ++k) jpnlist[(size_t)ip * maxn + k] = row[k];
+    }
+  }
+  return err;
+}
+
+// NList skin test with an adjusted update interval (inlist = 0), Src/nlist.F90:600-612:
+// over the particles in index order, r2 = sum(drosum(1:3,ip)**2); a value above r2max1
+// replaces r2max1 WITHOUT passing the old maximum on to r2max2, otherwise a value above
+// r2max2 replaces r2max2 (so r2max2 is not always the second largest value).  drosum holds
+// each particle's displacement summed since the last build.  Returns 1 when
+// sqrt(r2max1) + sqrt(r2max2) > drnlist (the list is to be rebuilt), else 0.
+int orc_nlist_skin(int np, const double* drosum, double drnlist, double* r2max1, double* r2max2) {
+  double m1 = 0.0, m2 = 0.0;
+  for (int ip = 0; ip < np; ++ip) {
+    const double* d = drosum + 3 * ip;
+    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (r2 > m1)
+      m1 = r2;
+    else if (r2 > m2)
+      m2 = r2;
+  }
+  *r2max1 = m1;
+  *r2max2 = m2;
+  return std::sqrt(m1) + std::sqrt(m2) > drnlist ? 1 : 0;
+}
+
+// SetCellList with MOLSIM's doubly linked lists (AddIpToCell by head insertion in particle
+// order, Src/celllist.F90:307-327,384-410).  head[c] (-1: empty, the reference's 0), npart[c],
+// next[i] / prev[i] (-1: none), cellof[i].  Cell of a particle as pcellro (:295-305).
+int orc_celllist_set(int np, const double* ro, const double* box, const int* nc, const double* csize,
+                     int* head, int* npart, int* next, int* prev, int* cellof) {
+  const int ncell = nc[0] * nc[1] * nc[2];
+  if (int e = orc_cell_build(np, ro, box, nc, csize, head, next, npart, cellof)) return e;
+  for (int i = 0; i < np; ++i) prev[i] = -1;
+  for (int c = 0; c < ncell; ++c)
+    for (int i = head[c]; i >= 0; i = next[i])
+      if (next[i] >= 0) prev[next[i]] = i;
+  return 0;
+}
+
+// UpdateCellIp(ip), Src/celllist.F90:329-382: when the cell of ro[ip] differs from the cell ip
+// is linked in, RmIpFromCell unlinks it (its four cases: alone, head, tail, inside) and
+// AddIpToCell makes it the head of its new cell.  Returns 1 when the particle moved to another
+// cell, 0 when it stayed, or an error code.
+int orc_celllist_update(int ip, const double* ro, const double* box, const int* nc, const double* csize,
+                        int* head, int* npart, int* next, int* prev, int* cellof) {
+  int ci[3];
+  for (int k = 0; k < 3; ++k) {
+    ci[k] = cell_coord(ro[3 * ip + k], box[k], 1.0 / csize[k]);
+    if (ci[k] < 0 || ci[k] > nc[k]) return ORC_ERR_ARG;
+    if (ci[k] == nc[k]) ci[k] = nc[k] - 1;
+  }
+  const int cnew = ci[0] + nc[0] * (ci[1] + nc[1] * ci[2]), cold = cellof[ip];
+  if (cnew == cold) return 0;
+  // RmIpFromCell
+  npart[cold]--;
+  const int nx = next[ip], pv = prev[ip];
+  if (nx < 0) {
+    if (head[cold] == ip) head[cold] = -1;
+    else next[pv] = -1;
+  } else {
+    if (head[cold] == ip) {
+      head[cold] = nx;
+      prev[nx] = -1;
+    } else {
+      prev[nx] = pv;
+      next[pv] = nx;
+    }
+  }
+  // AddIpToCell
+  npart[cnew]++;
+  const int jp = head[cnew];
+  head[cnew] = ip;
+  next[ip] = jp;
+  if (jp >= 0) prev[jp] = ip;
+  prev[ip] = -1;
+  cellof[ip] = cnew;
+  return 1;
 }
 
 }  // extern "C"
