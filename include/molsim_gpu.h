@@ -185,16 +185,39 @@ int msg_commit_charge(msg_ctx* ctx, int nptm, const int* ipnptm);
 /* Velocity-Verlet MD on the device (VelVer, Src/md.F90:252-410): msg_md_init takes the
  * mass of each particle type (in the units that make force * force_scale / mass an
  * acceleration, as MOLSIM's (sclene/scllen)/sclfor scaling does) and the velocities
- * vel(3,np) (NULL = zero), and computes the initial accelerations.  msg_md_run advances
- * nsteps steps of length dt: drift, UTotal (with the Verlet-list skin test and rebuild in
- * MSG_LIST_VERLET mode, Src/nlist.F90:597-621), kick.  log (may be NULL) receives
- * [u_tot, kinetic energy] per step. */
+ * vel(3,np) (NULL = zero), and computes the initial accelerations (building the Verlet list
+ * for the current positions, NList(iWriteInput), if none is valid).  msg_md_run advances
+ * nsteps steps of length dt, each VelVer (drift, UTotal, kick) followed in MSG_LIST_VERLET
+ * mode by NList(iSimulationStep) (Src/nlist.F90:597-621): with inlist = 0 the displacements
+ * summed since the last build (drosum) decide, rebuilding when sqrt(r2max1) + sqrt(r2max2)
+ * > drnlist with r2max1/r2max2 formed in the reference's particle order; with inlist > 0 the
+ * list is rebuilt every inlist-th step of the macrostep.  A rebuild uses the positions of the
+ * step just done.  log (may be NULL) receives [u_tot, kinetic energy] per step.  An error flag
+ * raised by any step (list overflow, r2 < r2umin, position outside the box, table scan past
+ * its end) is returned at the end of the call. */
 int msg_md_init(msg_ctx* ctx, const double* mass, const double* vel);
 int msg_md_run(msg_ctx* ctx, int nsteps, double dt, double* log);
 int msg_get_positions(msg_ctx* ctx, double* ro);
 int msg_get_velocities(msg_ctx* ctx, double* vel);
 /* Number of Verlet-list builds so far (NList statistics, Src/nlist.F90:945-1074). */
 long long msg_nlist_builds(msg_ctx* ctx);
+/* inlist of nmlIntList (Src/nlist.F90:88-95,615-618): 0 (default) rebuilds by the skin test,
+ * > 0 every inlist-th MD step of the macrostep. */
+int msg_nlist_interval(msg_ctx* ctx, int inlist);
+/* maxnneigh (Src/nlist.F90:240-246), the longest list a particle may have; a build that needs
+ * more returns MSG_ERR_NLIST, as the reference stops (:752, :875-877).  0 (default) sizes it
+ * from the density with a margin; the reference's max(np-1, ...) would take O(np^2) memory. */
+int msg_set_nlist_width(msg_ctx* ctx, int maxnneigh);
+/* NList(iBeforeMacrostep) (Src/nlist.F90:593-595): the displacement sums start again (without
+ * a rebuild) and the macrostep's step count restarts. */
+int msg_nlist_macrostep(msg_ctx* ctx);
+/* The current Verlet list in MOLSIM's layout (built first for the current positions if none is
+ * valid): kind 0 as SetVListMD (Src/nlist.F90:724-759, each pair once, kept by the particle the
+ * (ip + jp) parity rule names), kind 1 as SetVListMC (:820-884, the full list).
+ * jpnlist[(ip-1)*maxnneigh + k] (Fortran jpnlist(maxnneigh, np)) receives the 1-based
+ * partners of particle ip in ascending order, nneighpn[ip-1] their number.  A list longer
+ * than maxnneigh returns MSG_ERR_NLIST (the reference stops, :752). */
+int msg_get_verlet_list(msg_ctx* ctx, int kind, int maxnneigh, int* nneighpn, int* jpnlist);
 
 /* MOLSIM's random number generator (Random, Src/mollib.F90:2689-2713): *iseed as the
  * reference's iseed (negative or zero re-seeds), state[2] its two sequences (start with

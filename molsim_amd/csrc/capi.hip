@@ -72,7 +72,7 @@ __global__ void k_zero(int* a, int na, int* b, int nb, int* flags) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < na) a[i] = 0;
   if (i < nb) b[i] = 0;
-  if (i == 0) flags[0] = 0;
+  if (i == 0 && flags) flags[0] = 0;
 }
 
 struct PairSumArgs {
@@ -124,6 +124,16 @@ int launch_zero(Ctx* c) {
   const int n = std::max(std::max(na, nb), 1);
   hipLaunchKernelGGL(k_zero, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_cell_count, na,
                      c->d_brick_count, nb, c->d_flags);
+  MSG_HIP(hipGetLastError());
+  return MSG_OK;
+}
+
+// The cell counters a list rebuild's binning filled (the brick counters, which that binning
+// leaves alone, and the flags, which a following list build may raise, are kept).
+int launch_zero_counts(Ctx* c) {
+  const int na = c->grid.ncell;
+  hipLaunchKernelGGL(k_zero, dim3((na + 255) / 256), dim3(256), 0, c->stream, c->d_cell_count, na,
+                     c->d_brick_count, 0, (int*)nullptr);
   MSG_HIP(hipGetLastError());
   return MSG_OK;
 }
@@ -207,18 +217,24 @@ int utotal_sequence(Ctx* c) {
   // skin test, so they are queued before the host waits for its result (and run before the
   // fork, where the pair kernel cannot crowd out the brick scatter's atomics); a rebuild
   // bins the cells on the pair branch.
+  // Within msg_md_run the rebuilds are decided at the end of each step (NList after VelVer);
+  // a pass then builds a list only when none is valid.
   const bool early = verlet && branch;
   if (early) {
-    bool pending;
-    if ((e = verlet_check_begin(c, pending))) return e;
+    bool pending = false;
+    if (c->md_managed) rebuild = !c->vl_valid;
+    else if ((e = verlet_check_begin(c, pending))) return e;
     if ((e = launch_bin(c, false)) || (e = launch_sort(c, 2))) return e;
-    if (pending) {
-      if ((e = verlet_check_end(c, rebuild))) return e;
-    } else {
-      rebuild = true;
+    if (!c->md_managed) {
+      if (pending) {
+        if ((e = verlet_check_end(c, rebuild))) return e;
+      } else {
+        rebuild = true;
+      }
     }
   } else if (verlet) {
-    if ((e = verlet_check(c, rebuild))) return e;
+    if (c->md_managed) rebuild = !c->vl_valid;
+    else if ((e = verlet_check(c, rebuild))) return e;
   }
   // with a Verlet list the cell order is only needed to rebuild the list (later users such
   // as DUTotal complete it on demand)
@@ -359,7 +375,8 @@ int msg_create(const msg_params* p, msg_ctx** out) {
       hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_dmax, hipEventDisableTiming) != hipSuccess)
+      hipEventCreateWithFlags(&c->ev_skin, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_cells, hipEventDisableTiming) != hipSuccess)
     return fail(set_error(MSG_ERR_HIP, "stream/event creation failed"));
   const int np = c->np;
   if ((e = dalloc(c, (void**)&c->d_ro, 3 * np * sizeof(double)))) return fail(e);
@@ -383,6 +400,8 @@ int msg_create(const msg_params* p, msg_ctx** out) {
   if ((e = dalloc(c, (void**)&c->d_part, nparts * sizeof(double)))) return fail(e);
   if ((e = dalloc(c, (void**)&c->d_out, 64 * sizeof(double)))) return fail(e);
   if ((e = dalloc(c, (void**)&c->d_flags, 16 * sizeof(int)))) return fail(e);
+  // word 0: the pass's flags, 1: sticky over a multi-step call, 2: the list export's own
+  if (hipMemset(c->d_flags, 0, 16 * sizeof(int)) != hipSuccess) return fail(set_error(MSG_ERR_HIP, "hipMemset failed"));
   if (hipHostMalloc((void**)&c->h_out, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
     return fail(set_error(MSG_ERR_NOMEM, "hipHostMalloc failed"));
   if (hipHostMalloc((void**)&c->h_flags, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess)
@@ -406,19 +425,21 @@ void msg_destroy(msg_ctx* h) {
                   c->d_frac, c->d_Q, c->d_C, c->d_G, c->d_tw, c->d_kfac, c->d_kvec, c->d_sk,
                   c->d_force, c->d_part, c->d_out, c->d_flags, c->d_de, c->d_de_ip, c->d_de_r,
                   c->d_tcoef64, c->d_gk, c->d_item_cell, c->d_bq, c->d_bmm, c->d_M,
-                  c->d_vl_home, c->d_vl_n, c->d_vl_nbr, c->d_vl_pos, c->d_vl_slot, c->d_ro_ref, c->d_dmax, c->d_vel,
-                  c->d_acc, c->d_minv, c->d_ke_part, c->d_rank, c->d_laz};
+                  c->d_vl_home, c->d_vl_n, c->d_vl_nbr, c->d_vl_pos, c->d_vl_slot, c->d_ro_ref, c->d_vel,
+                  c->d_acc, c->d_minv, c->d_ke_part, c->d_rank, c->d_laz, c->d_drosum, c->d_r2, c->d_bm,
+                  c->d_pm, c->d_skin};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (c->h_out) hipHostFree(c->h_out);
   if (c->h_flags) hipHostFree(c->h_flags);
-  if (c->h_dmax) hipHostFree(c->h_dmax);
+  if (c->h_skin) hipHostFree(c->h_skin);
   for (auto& ev : c->ev)
     if (ev) hipEventDestroy(ev);
   if (c->side) hipStreamDestroy(c->side);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
-  if (c->ev_dmax) hipEventDestroy(c->ev_dmax);
+  if (c->ev_skin) hipEventDestroy(c->ev_skin);
+  if (c->ev_cells) hipEventDestroy(c->ev_cells);
   if (c->ev_batch) hipEventDestroy(c->ev_batch);
   if (c->stream) hipStreamDestroy(c->stream);
   delete h;
@@ -555,21 +576,43 @@ int msg_set_box(msg_ctx* h, const double* boxlen) {
   for (int d = 0; d < 3; ++d)
     if (!(boxlen[d] > 0.0)) return set_error(MSG_ERR_ARG, "msg_set_box: box lengths must be > 0");
   MSG_HIP(hipStreamSynchronize(c->stream));
-  for (int d = 0; d < 3; ++d) c->P.boxlen[d] = boxlen[d];
-  int e = lists_setup(c);
-  if (!e && c->P.list_mode == MSG_LIST_VERLET) e = verlet_setup(c);
-  if (!e && c->have_ewald) e = msg_ewald_setup(h);
-  if (e) return e;
-  c->npart_pair = pair_blocks(c);
-  if (c->npart_pair > c->npart_pair_cap) return set_error(MSG_ERR_ARG, "msg_set_box: pair partials exceed reserve");
-  if ((e = launch_zero(c))) return e;  // the counters of the new grid
-  MSG_HIP(hipStreamSynchronize(c->stream));
+  // Everything derived from the old box is dropped before any buffer is replaced, so a failed
+  // call leaves no graph or batch that still points at freed memory; on failure the context
+  // keeps the old box lengths and is usable again after a successful msg_set_box.
   c->fresh = false;
   c->cells_valid = false;
   c->vl_valid = false;
   c->have_gk = false;
   c->graph_ok = false;
+  if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+  if (c->graph) { hipGraphDestroy(c->graph); c->graph = nullptr; }
   c->box_gen++;
+  double old[3];
+  for (int d = 0; d < 3; ++d) {
+    old[d] = c->P.boxlen[d];
+    c->P.boxlen[d] = boxlen[d];
+  }
+  int e = lists_setup(c);
+  if (!e && c->P.list_mode == MSG_LIST_VERLET) e = verlet_setup(c);
+  if (!e && c->have_ewald) e = msg_ewald_setup(h);
+  if (!e) {
+    c->npart_pair = pair_blocks(c);
+    if (c->npart_pair > c->npart_pair_cap) e = set_error(MSG_ERR_ARG, "msg_set_box: pair partials exceed reserve");
+  }
+  if (e) {
+    // back to the old box: a failed allocation left the old buffers in place and they hold
+    // the old grid, so this set-up allocates nothing; the Ewald arrays may be half rewritten,
+    // so the Ewald set-up must be done again (msg_ewald_setup or another msg_set_box)
+    for (int d = 0; d < 3; ++d) c->P.boxlen[d] = old[d];
+    lists_setup(c);
+    if (c->P.list_mode == MSG_LIST_VERLET) verlet_setup(c);
+    c->npart_pair = pair_blocks(c);
+    c->have_ewald = false;
+    return e;
+  }
+  if ((e = launch_zero(c))) return e;  // the counters of the new grid
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  c->have_gk = false;
   return MSG_OK;
 }
 

@@ -27,7 +27,9 @@ int set_error(int code, const std::string& text);
       return ::msg::set_error(MSG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// Device-side error flags (OR-ed into ctx->d_flags[0]).
+// Device-side error flags (OR-ed into ctx->d_flags[0]; each pass's reduction hands them to the
+// output block, clears word 0 and ORs them into the sticky word d_flags[1], which only the
+// multi-step callers clear).
 enum : int { FLAG_R2UMIN = 1, FLAG_IBUF = 2, FLAG_CELL = 4, FLAG_NLIST = 8 };
 
 constexpr int kMaxT = MSG_MAX_TYPES;
@@ -36,10 +38,6 @@ constexpr int kTabBins = 256;        // r^2 bins of the interval index per type 
 // Weak charges: the type field of an uncharged particle's cell-ordered record (cpos.w) carries
 // this offset, so the pair kernels see the charge state in the record they load anyway.
 constexpr int kUncharged = 8;
-// The Verlet-list displacement maximum is kept in kDmaxSlots words kDmaxStride words apart
-// (separate cache lines): workgroup b adds its maximum to slot b mod kDmaxSlots, so the atomics
-// of a pass over all particles spread over many addresses instead of queueing at one.
-constexpr int kDmaxSlots = 64, kDmaxStride = 64;
 constexpr int kMaxTabIntervals = 512;   // records below rcut staged in LDS (<= 48 KB in FP64)
 constexpr int kMaxStencil = 27;
 constexpr int kPairHB = 4;           // home particles per pair-kernel work item
@@ -94,7 +92,8 @@ struct Ctx {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;        // second branch of the UTotal graph (pair kernel)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  hipEvent_t ev_dmax = nullptr;  // the displacement maximum's copy to the host has landed
+  hipEvent_t ev_skin = nullptr;  // the skin test's (r2max1, r2max2) copy to the host has landed
+  hipEvent_t ev_cells = nullptr; // a list rebuild's cell order is complete (its list build may follow on the side stream)
   hipEvent_t ev_batch = nullptr;     // orders the context's own work before a batched pass
   int device = 0;
   long long bytes = 0;
@@ -195,14 +194,24 @@ struct Ctx {
   double* d_vl_pos = nullptr;    // double4 per slot: position, type (+ kUncharged when uncharged)
   int* d_vl_slot = nullptr;      // slot of each particle
   double* d_ro_ref = nullptr;    // positions at the last build
-  unsigned int* d_dmax = nullptr;  // max displacement^2 since the build (float bits), kDmaxSlots
-                                   // slots kDmaxStride apart; the maximum is that of all slots
-  unsigned int* h_dmax = nullptr;  // pinned copy of the slots
+  // NList skin test (Src/nlist.F90:597-621): per-particle squared displacement since the build
+  // (d_r2, index order), per-256 block maxima and their exclusive prefix maxima, and the
+  // result (r2max1, r2max2) as the bits of two non-negative doubles (pinned copy h_skin)
+  double* d_r2 = nullptr;
+  double* d_bm = nullptr;
+  double* d_pm = nullptr;
+  unsigned long long* d_skin = nullptr;
+  unsigned long long* h_skin = nullptr;
+  double* d_drosum = nullptr;    // 3*np: MD displacements summed since the build (drosum)
+  bool drosum_reset = false;     // a list was built: the next drift starts the sums again
+  int inlist = 0;                // 0: rebuild by the skin test; > 0: every inlist-th MD step
+  int vl_width_user = 0;         // list width the caller set (MOLSIM's maxnneigh); 0: from the density
+  long long istep2 = 0;          // MD steps of the current macrostep (NList's istep2)
+  bool md_managed = false;       // msg_md_run decides rebuilds itself (at the end of a step)
   bool vl_valid = false;
   int ncell_cap = 0;             // cells the cell buffers hold
   int vl_cap = 0;                // list width vl_nbr holds
   long long box_gen = 0;         // bumped by msg_set_box (a batch planned before is stale)
-  bool disp_done = false;        // the MD drift kernel already computed the displacement
   long long vl_builds = 0;
 
   // molecular dynamics (velocity Verlet)
@@ -263,12 +272,14 @@ int dutotal_charge(Ctx* c, int nptm, const int* ip1, int* ovl, double* du_twob, 
 int utotal_refresh(Ctx* c);                        // UTotal sequence + sync (for dE state)
 int utotal_sequence(Ctx* c);
 int verlet_setup(Ctx* c);
-int verlet_check(Ctx* c, bool& rebuild);            // displacement test (host sync)
+int verlet_check(Ctx* c, bool& rebuild);            // skin test of the current positions (host sync)
 int launch_vl_records(Ctx* c);                     // Verlet slot records of the current positions
-int verlet_check_begin(Ctx* c, bool& pending);       // its first half: queue the maximum's copy
+int verlet_check_begin(Ctx* c, bool& pending);       // its first half: queue the test and its copy
 int verlet_check_end(Ctx* c, bool& rebuild);         // its second half: wait and decide
 int launch_vlist_build(Ctx* c);
-int launch_disp(Ctx* c);
+int launch_skin(Ctx* c);                           // r2max1 / r2max2 of d_r2 and d_bm, copied to h_skin
+bool skin_decide(const Ctx* c);                    // sqrt(r2max1) + sqrt(r2max2) > drnlist (after ev_skin)
+int launch_zero_counts(Ctx* c);                    // cell and brick counters (the flags kept)
 
 // profiling wrappers around a launch
 void prof_begin(Ctx* c, int kid);

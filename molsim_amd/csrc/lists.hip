@@ -481,21 +481,26 @@ int lists_setup(Ctx* c) {
   // per-cell buffers: (re)allocated when the grid outgrows them (a new box); the counters
   // of a new grid are cleared by the caller
   if (g.ncell > c->ncell_cap) {
+    // the new buffers first: if one allocation fails, the old ones are still in place
     void** cellbuf[5] = {(void**)&c->d_cell_count, (void**)&c->d_cell_start, (void**)&c->d_cell_fill,
                          (void**)&c->d_cell_chunk, (void**)&c->d_item_cell};
-    for (void** p : cellbuf)
-      if (*p) {
-        hipFree(*p);
-        *p = nullptr;
+    const size_t sz[5] = {g.ncell * sizeof(int), (g.ncell + 1) * sizeof(int), g.ncell * sizeof(int),
+                          (g.ncell + 1) * sizeof(int), (g.ncell + np / kPairHB + 2) * sizeof(int)};
+    void* fresh[5] = {};
+    for (int k = 0; k < 5; ++k)
+      if (int e = A(&fresh[k], sz[k])) {
+        for (int q = 0; q < k; ++q) {
+          hipFree(fresh[q]);
+          c->bytes -= (long long)sz[q];
+        }
+        return e;
       }
+    for (int k = 0; k < 5; ++k) {
+      if (*cellbuf[k]) hipFree(*cellbuf[k]);
+      *cellbuf[k] = fresh[k];
+    }
     if (c->ncell_cap > 0)
       c->bytes -= (long long)(5 * c->ncell_cap + np / kPairHB + 4) * (long long)sizeof(int);
-    c->ncell_cap = 0;
-    if (int e = A((void**)&c->d_cell_count, g.ncell * sizeof(int))) return e;
-    if (int e = A((void**)&c->d_cell_start, (g.ncell + 1) * sizeof(int))) return e;
-    if (int e = A((void**)&c->d_cell_fill, g.ncell * sizeof(int))) return e;
-    if (int e = A((void**)&c->d_cell_chunk, (g.ncell + 1) * sizeof(int))) return e;
-    if (int e = A((void**)&c->d_item_cell, (g.ncell + np / kPairHB + 2) * sizeof(int))) return e;
     c->ncell_cap = g.ncell;
   }
   if (c->d_cellid) return MSG_OK;  // the per-particle buffers exist already

@@ -557,11 +557,12 @@ __global__ __launch_bounds__(256) void k_pair_nb_seg(const PairArgs* __restrict_
 // serves MD).  Homes are taken in cell order (their "slots"); kVlLph lanes per home walk the
 // candidates of its stencil cells in cell order (cells whose nearest point lies beyond rcut +
 // skin are skipped), kVlLph candidates per round; a ballot over the group ranks the partners
-// within rcut + skin, which are appended as slots in that order, so the list is the one a
-// single thread per home walking the cells would write.  The slot-ordered positions and types
-// the pair loop reads are copied from the cell records; the drift and displacement kernels
-// keep their positions current until the next build.  Also snapshots the positions the
-// displacement test measures from and clears the displacement maximum.
+// with r2 < (rcut + skin)^2 (PBCr2's single shift and sum of squares in FP64, not contracted,
+// so the pair test is the reference's bit for bit), which are appended as slots in that order,
+// so the list is the one a single thread per home walking the cells would write.  The
+// slot-ordered positions and types the pair loop reads are copied from the cell records; the
+// slot-record kernel keeps their positions current until the next build.  Also snapshots the
+// positions (from the cell records) the displacement test of a UTotal caller measures from.
 struct VlistArgs {
   int np, maxn;
   const double4* cpos;
@@ -571,14 +572,12 @@ struct VlistArgs {
   CellGrid g;
   Box box;
   double rl2;          // (rcut + skin)^2
-  const double* ro;
   double* ro_ref;
   int* vl_home;
   int* vl_n;
   int* vl_nbr;
   double4* vl_pos;
   int* vl_slot;
-  unsigned int* dmax;
   int* flags;
   int stage;           // lists staged in LDS (rows of maxn + 1) and written out row by row
 };
@@ -594,7 +593,6 @@ __global__ __launch_bounds__(256) void k_vlist_build(VlistArgs a) {
   const int ls = a.maxn + 1;
   const int tid = threadIdx.x, lane = tid & 63, grp = tid / LPH, k = tid & (LPH - 1);
   const int s = blockIdx.x * HPB + grp;
-  if (blockIdx.x == 0 && tid < kDmaxSlots) a.dmax[tid * kDmaxStride] = 0u;
   const bool home = s < a.np;
   const int hs = home ? s : a.np - 1;
   const double4 hp = a.cpos[hs];
@@ -603,9 +601,9 @@ __global__ __launch_bounds__(256) void k_vlist_build(VlistArgs a) {
     a.vl_home[s] = ip;
     a.vl_pos[s] = hp;
     a.vl_slot[ip] = s;
-    a.ro_ref[3 * ip] = a.ro[3 * ip];
-    a.ro_ref[3 * ip + 1] = a.ro[3 * ip + 1];
-    a.ro_ref[3 * ip + 2] = a.ro[3 * ip + 2];
+    a.ro_ref[3 * ip] = hp.x;
+    a.ro_ref[3 * ip + 1] = hp.y;
+    a.ro_ref[3 * ip + 2] = hp.z;
   }
   const int c = a.slot_cell[hs];
   const int cx = c % a.g.nc[0], cy = (c / a.g.nc[0]) % a.g.nc[1], cz = c / (a.g.nc[0] * a.g.nc[1]);
@@ -695,12 +693,13 @@ __global__ __launch_bounds__(256) void k_vlist_build(VlistArgs a) {
     for (int u = 0; u < U; ++u) {
       bool hit = false;
       if (val[u] && jv[u] != s) {
+#pragma clang fp contract(off)
         double d[3] = {hp.x - pv[u].x, hp.y - pv[u].y, hp.z - pv[u].z};
 #pragma unroll
         for (int q = 0; q < 3; ++q)
           if (fabs(d[q]) > a.box.half[q]) d[q] -= copysign(a.box.L[q], d[q]);
         const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        hit = !(r2 > a.rl2);
+        hit = r2 < a.rl2;
       }
       const unsigned gm = (unsigned)(__ballot(hit) >> gsh) & ((1u << LPH) - 1u);
       if (hit) {
@@ -812,6 +811,58 @@ __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
     double t = 0.0;
     for (int w = 0; w < 4; ++w) t += s_red[w][threadIdx.x];
     a.part[blockIdx.x * (a.tm.ntp + 1) + (threadIdx.x == NTP ? a.tm.ntp : threadIdx.x)] = t;
+  }
+}
+
+// The list in MOLSIM's layout (Src/nlist.F90:724-884): jpnlist(maxnneigh, np), column ip
+// holding the 1-based partners of particle ip in ascending order, nneighpn(ip) of them, as
+// SetVListMD (kind 0: a pair (ip, jp) is kept by ip when ip < jp and ip + jp is odd, or ip > jp
+// and ip + jp is even, so every pair appears once) or SetVListMC (kind 1: the full list) writes
+// them.  One wavefront per home: its partners (slots) become particle indices, the kind's rule
+// filters them, a ballot compacts them into LDS, and each is placed by its rank (the number of
+// smaller partners; indices are distinct).
+struct VlExportArgs {
+  int np, kind, ldo;     // ldo: the caller's maxnneigh
+  int width, wpb;        // LDS row width (the list width) and wavefronts per workgroup
+  const int* vl_home;
+  const int* vl_n;
+  const int* vl_nbr;
+  int* nneigh;
+  int* jpn;
+  int* flags;
+};
+
+__global__ __launch_bounds__(256) void k_vlist_export(VlExportArgs a) {
+  extern __shared__ int s_rows[];  // [wavefronts][row width]
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int s = blockIdx.x * a.wpb + wv;
+  if (s >= a.np) return;  // whole wavefronts leave; no workgroup barrier follows
+  int* row = s_rows + wv * a.width;
+  const int ip = a.vl_home[s], n = a.vl_n[s];
+  int cnt = 0;
+  for (int k0 = 0; k0 < n; k0 += 64) {
+    const int k = k0 + lane;
+    int j = -1;
+    bool keep = false;
+    if (k < n) {
+      j = a.vl_home[a.vl_nbr[(size_t)k * a.np + s]];
+      const int par = (ip + j) & 1;
+      keep = a.kind == 1 || (ip < j ? par != 0 : par == 0);
+    }
+    const unsigned long long bal = __ballot(keep);
+    if (keep) row[cnt + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+    cnt += __popcll(bal);
+  }
+  wave_sync();
+  for (int k = lane; k < cnt; k += 64) {
+    const int x = row[k];
+    int r = 0;
+    for (int m = 0; m < cnt; ++m) r += row[m] < x;
+    if (r < a.ldo) a.jpn[(size_t)ip * a.ldo + r] = x + 1;
+  }
+  if (lane == 0) {
+    a.nneigh[ip] = min(cnt, a.ldo);
+    if (cnt > a.ldo) atomicOr(a.flags, FLAG_NLIST);
   }
 }
 
@@ -951,14 +1002,12 @@ int launch_vlist_build(Ctx* c) {
   }
   const double rl = c->P.rcut + c->P.drnlist;
   v.rl2 = rl * rl;
-  v.ro = c->d_ro;
   v.ro_ref = c->d_ro_ref;
   v.vl_home = c->d_vl_home;
   v.vl_n = c->d_vl_n;
   v.vl_nbr = c->d_vl_nbr;
   v.vl_pos = (double4*)c->d_vl_pos;
   v.vl_slot = c->d_vl_slot;
-  v.dmax = c->d_dmax;
   v.flags = c->d_flags;
   prof_begin(c, MSG_K_VLIST);
   const size_t lds = (size_t)(256 / kVlLph) * (c->vl_maxn + 1) * sizeof(int);
@@ -968,6 +1017,7 @@ int launch_vlist_build(Ctx* c) {
   prof_end(c, MSG_K_VLIST);
   MSG_HIP(hipGetLastError());
   c->vl_builds++;
+  c->drosum_reset = true;  // the displacement sums start again with the new list
   return MSG_OK;
 }
 
@@ -1028,3 +1078,59 @@ int plan_pair(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out) {
 }
 
 }  // namespace msg
+
+using namespace msg;
+
+extern "C" {
+
+// The current Verlet list in MOLSIM's layout (kind 0: SetVListMD, 1: SetVListMC); built for
+// the current positions first if none is valid (as msg_nlist).
+int msg_get_verlet_list(msg_ctx* h, int kind, int maxnneigh, int* nneighpn, int* jpnlist) {
+  if (!h || (kind != 0 && kind != 1) || maxnneigh < 1 || !nneighpn || !jpnlist)
+    return set_error(MSG_ERR_ARG, "msg_get_verlet_list: bad arguments");
+  Ctx* c = &h->c;
+  if (c->P.list_mode != MSG_LIST_VERLET) return set_error(MSG_ERR_STATE, "msg_get_verlet_list: not in Verlet-list mode");
+  if (!c->vl_valid)
+    if (int e = msg_nlist(h)) return e;
+  const int width = c->vl_maxn;
+  const int wpb = width <= 4096 ? 4 : 1;
+  if ((size_t)wpb * width * sizeof(int) > 65536) return set_error(MSG_ERR_ARG, "msg_get_verlet_list: list width above 16384");
+  const int np = c->np;
+  int *d_nn = nullptr, *d_jl = nullptr;
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  MSG_HIP(hipMalloc(&d_nn, np * sizeof(int)));
+  if (hipMalloc(&d_jl, (size_t)np * maxnneigh * sizeof(int)) != hipSuccess) {
+    hipFree(d_nn);
+    return set_error(MSG_ERR_NOMEM, "msg_get_verlet_list: no device memory for the copy");
+  }
+  VlExportArgs a;
+  a.np = np;
+  a.kind = kind;
+  a.ldo = maxnneigh;
+  a.width = width;
+  a.wpb = wpb;
+  a.vl_home = c->d_vl_home;
+  a.vl_n = c->d_vl_n;
+  a.vl_nbr = c->d_vl_nbr;
+  a.nneigh = d_nn;
+  a.jpn = d_jl;
+  a.flags = c->d_flags + 2;  // a word of its own: an overflow of the caller's width is this call's error
+  int e = MSG_OK;
+  hipLaunchKernelGGL(k_vlist_export, dim3((np + wpb - 1) / wpb), dim3(64 * wpb), (size_t)wpb * width * sizeof(int),
+                     c->stream, a);
+  hipError_t he = hipGetLastError();
+  int ovf = 0;
+  if (he == hipSuccess) he = hipMemcpyAsync(nneighpn, d_nn, np * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(jpnlist, d_jl, (size_t)np * maxnneigh * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (he == hipSuccess) he = hipMemcpyAsync(&ovf, c->d_flags + 2, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (he == hipSuccess) he = hipMemsetAsync(c->d_flags + 2, 0, sizeof(int), c->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+  hipFree(d_nn);
+  hipFree(d_jl);
+  if (he != hipSuccess) return set_error(MSG_ERR_HIP, std::string("msg_get_verlet_list: ") + hipGetErrorString(he));
+  if (ovf) e = set_error(MSG_ERR_NLIST, "msg_get_verlet_list: a list is longer than maxnneigh");
+  return e;
+}
+
+}  // extern "C"

@@ -147,6 +147,7 @@ __device__ inline void reduce_body(const ReduceArgs& a) {
     o[a.ntp + 6] = m[2];
     o[a.ntp + 7] = (double)fl;
     a.flags[0] = 0;
+    a.flags[1] |= fl;  // sticky over the passes of a multi-step call
   }
   for (int i = tid; i < a.ncell; i += NT) a.cell_count[i] = 0;
   for (int i = tid; i < a.nbrick; i += NT) a.brick_count[i] = 0;

@@ -107,6 +107,10 @@ def lib():
                                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int), _vp]
         L.msg_nlist_builds.argtypes = [_vp]
         L.msg_nlist_builds.restype = C.c_longlong
+        L.msg_nlist_interval.argtypes = [_vp, C.c_int]
+        L.msg_nlist_macrostep.argtypes = [_vp]
+        L.msg_set_nlist_width.argtypes = [_vp, C.c_int]
+        L.msg_get_verlet_list.argtypes = [_vp, C.c_int, C.c_int, _ip, _ip]
         L.msg_stream.argtypes = [_vp]
         L.msg_stream.restype = _vp
         L.msg_profile.argtypes = [_vp, C.c_int]
@@ -390,6 +394,30 @@ class Engine:
 
     def nlist_builds(self) -> int:
         return int(lib().msg_nlist_builds(self.h))
+
+    def nlist_interval(self, inlist: int):
+        """inlist: 0 rebuilds by the skin test, > 0 every inlist-th MD step of a macrostep."""
+        _check(lib().msg_nlist_interval(self.h, int(inlist)))
+
+    def set_nlist_width(self, maxnneigh: int):
+        """The list width (MOLSIM's maxnneigh); 0 sizes it from the density."""
+        _check(lib().msg_set_nlist_width(self.h, int(maxnneigh)))
+
+    def nlist_macrostep(self):
+        """NList(iBeforeMacrostep): displacement sums and the step count start again."""
+        _check(lib().msg_nlist_macrostep(self.h))
+
+    def verlet_list(self, kind: str = "md", maxnneigh: int = 256):
+        """The Verlet list in MOLSIM's layout ("md": SetVListMD's parity half list, "mc":
+        SetVListMC's full list): (nneighpn (N,), jpnlist (N, maxnneigh) of 0-based partners in
+        ascending order, -1 past each row's end)."""
+        n = self.spec.n
+        nn = np.zeros(n, dtype=np.int32)
+        jl = np.zeros(n * maxnneigh, dtype=np.int32)
+        _check(lib().msg_get_verlet_list(self.h, {"md": 0, "mc": 1}[kind], int(maxnneigh), nn, jl))
+        jl = jl.reshape(n, maxnneigh) - 1
+        jl[np.arange(maxnneigh)[None, :] >= nn[:, None]] = -1
+        return nn, jl
 
     # -- measurement ----------------------------------------------------------------------
     def stream(self) -> int:

@@ -72,8 +72,10 @@ _lib_omp = None
 
 
 def lib_omp():
-    """The same oracle built with OpenMP (pair loop, FFT lines and gather shared out over the
-    host's threads): only for timing the CPU baseline on all cores, never for checking."""
+    """The same oracle built with OpenMP (pair loop, FFT lines, gather and list rows shared out
+    over the host's threads): the CPU baseline on all cores, and the checker only where the
+    single-threaded build would take minutes (the million-particle system); a CPU test checks
+    that both builds agree."""
     global _lib_omp
     if _lib_omp is None:
         if not os.path.exists(_LIB_OMP_PATH):
@@ -285,15 +287,16 @@ def cell_members(head, nxt):
 VLIST_KIND = {"md": 0, "mc": 1}
 
 
-def vlist(kind, ro, box, rcut, drnlist, maxn=None, cells=False, omp=False):
+def vlist(kind, ro, box, rcut, drnlist, maxn=None, cells=False, table=False, omp=False):
     """Verlet lists as SetVListMD (kind "md": parity half list) or SetVListMC ("mc": full list)
     build them: (nneighpn (N,), rows) with rows[i] the 0-based partners of particle i in
-    ascending order.  cells=True takes the cell-list form (same rows, O(N)); omp=True runs it on
-    all host cores (a large-system check, the OpenMP build of the same code)."""
+    ascending order.  cells=True takes the cell-list form (same rows, O(N)); with omp=True its
+    rows are shared out over the host's cores (each row is formed the same way).  table=True
+    returns rows as one (N, maxn) array with -1 past each row's end (large systems)."""
     ro = np.ascontiguousarray(ro, dtype=np.float64).reshape(-1)
     n = ro.size // 3
     if maxn is None:
-        maxn = max(n - 1, 1) if not cells else 2048
+        maxn = max(n - 1, 1) if not cells else 512
     nn = np.zeros(n, dtype=np.int32)
     jl = np.zeros(n * maxn, dtype=np.int32)
     L = lib_omp() if omp else lib()
@@ -301,6 +304,9 @@ def vlist(kind, ro, box, rcut, drnlist, maxn=None, cells=False, omp=False):
     _check(f(VLIST_KIND[kind], n, ro, np.asarray(box, dtype=np.float64), float(rcut), float(drnlist), maxn, nn, jl),
            "vlist")
     jl = jl.reshape(n, maxn)
+    if table:
+        jl[np.arange(maxn)[None, :] >= nn[:, None]] = -1
+        return nn, jl
     return nn, [jl[i, :nn[i]].copy() for i in range(n)]
 
 

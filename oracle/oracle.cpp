@@ -1395,6 +1395,18 @@ int orc_vlist_cells(int kind, int np, const double* ro, const double* box, doubl
   const int ncell = nc[0] * nc[1] * nc[2];
   std::vector<int> head(ncell), nxt(np), npart(ncell), cellof(np);
   if (int e = orc_cell_build(np, ro, box, nc, cs, head.data(), nxt.data(), npart.data(), cellof.data())) return e;
+  // the cells' members copied out contiguously (positions with them), so a candidate loop
+  // reads consecutive memory instead of following the linked lists through the whole array
+  std::vector<int> cstart(ncell + 1, 0), cidx(np);
+  std::vector<double> cpos(3 * (size_t)np);
+  for (int c = 0; c < ncell; ++c) cstart[c + 1] = cstart[c] + npart[c];
+  for (int c = 0; c < ncell; ++c) {
+    int k = cstart[c];
+    for (int j = head[c]; j >= 0; j = nxt[j], ++k) {
+      cidx[k] = j;
+      for (int d = 0; d < 3; ++d) cpos[3 * (size_t)k + d] = ro[3 * j + d];
+    }
+  }
   int err = 0;
 #pragma omp parallel
   {
@@ -1404,9 +1416,11 @@ int orc_vlist_cells(int kind, int np, const double* ro, const double* box, doubl
       row.clear();
       const int nn = cell_neigh(cellof[ip], nc, st.data(), nst, nb.data());
       for (int s = 0; s < nn; ++s)
-        for (int jp = head[nb[s]]; jp >= 0; jp = nxt[jp]) {
+        for (int k = cstart[nb[s]]; k < cstart[nb[s] + 1]; ++k) {
+          const int jp = cidx[k];
           if (kind == 0 ? !vlist_md_keeps(ip, jp) : jp == ip) continue;
-          double d[3] = {ro[3 * ip] - ro[3 * jp], ro[3 * ip + 1] - ro[3 * jp + 1], ro[3 * ip + 2] - ro[3 * jp + 2]};
+          const double* pj = &cpos[3 * (size_t)k];
+          double d[3] = {ro[3 * ip] - pj[0], ro[3 * ip + 1] - pj[1], ro[3 * ip + 2] - pj[2]};
           double r2;
           pbc_r2(box, d, r2);
           if (r2 < rs2cut) row.push_back(jp);

@@ -2,6 +2,7 @@
 import numpy as np
 
 import oracle
+from molsim_amd import systems
 from molsim_amd.engine import Engine
 
 LIST = {"all": 0, "cell": 1}
@@ -50,3 +51,40 @@ def compare(gpu, ref, ftol, etol):
     for k in ("u_tot", "u_rec", "u_twob", "virial"):
         assert errs[k] <= etol, errs
     return errs
+
+
+def anint(x):
+    """Fortran ANINT: nearest integer, halves away from zero."""
+    return np.sign(x) * np.floor(np.abs(x) + 0.5)
+
+
+def vv_reference(sp, tab, mass, v0, dt, nsteps, scale, inlist=0):
+    """VelVer (Src/md.F90:252-410) + NList(iSimulationStep) (Src/nlist.F90:597-621) driven by
+    oracle forces: drostep = dt v + (dt^2/2) a, ro += drostep, PBC with ANINT, then
+    a = F / m and v += (dt/2)(a + a_old); after every step the displacement sums decide a
+    rebuild by the oracle's skin test (inlist = 0) or every inlist-th step.  Returns positions,
+    velocities, the rebuild steps (1-based) and the positions of the last rebuild."""
+    L = np.asarray(sp.box, dtype=np.float64)
+    osp = systems.SystemSpec(**{**sp.__dict__, "list_mode": "cell"})
+    x = sp.ro.copy()
+    v = v0.copy()
+    minv = (1.0 / np.asarray(mass))[sp.ipt][:, None]
+    a = minv * (oracle_system(osp, tab).utotal()["force"] * scale)
+    drosum = np.zeros_like(x)
+    fac = 0.5 * (dt * dt)
+    rebuilds, x_build = [], x.copy()
+    for step in range(1, nsteps + 1):
+        ds = dt * v + fac * a
+        x = x + ds
+        x = np.where(np.abs(x) > L / 2, x - L * anint(x * (1.0 / L)), x)
+        drosum += ds
+        f = oracle_system(systems.SystemSpec(**{**osp.__dict__, "ro": x}), tab).utotal()["force"]
+        an = minv * (f * scale)
+        v = v + (0.5 * dt) * (an + a)
+        a = an
+        rb = (step % inlist == 0) if inlist > 0 else oracle.nlist_skin(drosum, sp.drnlist)[0]
+        if rb:
+            drosum[:] = 0.0
+            rebuilds.append(step)
+            x_build = x.copy()
+    return x, v, rebuilds, x_build

@@ -97,29 +97,39 @@ def test_set_box_rejects_bad_lengths():
     eng.close()
 
 
+@pytest.mark.parametrize("ratio", [1.05, 0.95])
 @pytest.mark.parametrize("precision,tol", [("mixed", 1e-5), ("fp64", 1e-9)])
-def test_box_change_during_md(precision, tol):
-    """Box rescaling between MD runs (Src/md.F90:699-700): the Verlet list is rebuilt for the
-    new box, and the forces of the rescaled configuration equal the oracle's."""
+def test_box_change_during_md(precision, tol, ratio):
+    """Box rescaling between MD runs (Src/md.F90:699-700), expansion and compression: the
+    Verlet list is rebuilt for the new box (at the width the new density needs: no overflow),
+    the forces of the rescaled configuration equal the oracle's, and the next steps follow the
+    oracle-driven velocity-Verlet loop."""
     from molsim_amd.engine import Engine
-    from gpu_helpers import cabi_table
+    from gpu_helpers import cabi_table, vv_reference
     sp = _system("verlet", "spm")
     tab = oracle.build_table(sp.pot)
     eng = Engine(sp, precision=precision, table=cabi_table(tab), force_scale=100.0)
     rng = np.random.default_rng(9)
-    eng.md_init([20.0, 20.0], rng.normal(size=sp.ro.shape))
+    mass = [20.0, 20.0]
+    eng.md_init(mass, rng.normal(size=sp.ro.shape))
     eng.md_run(5, 0.002)
     builds = eng.nlist_builds()
     ro = eng.positions()
-    ratio = 1.05
+    v = eng.velocities()
     eng.set_box(sp.box * ratio)
     eng.set_positions(ro * ratio)
     got = eng.utotal()
     assert eng.nlist_builds() == builds + 1
-    osp = systems.SystemSpec(**{**sp.__dict__, "ro": ro * ratio, "box": sp.box * ratio, "list_mode": "cell"})
-    ref = oracle_system(osp, tab).utotal()
+    osp = systems.SystemSpec(**{**sp.__dict__, "ro": ro * ratio, "box": sp.box * ratio})
+    ref = oracle_system(systems.SystemSpec(**{**osp.__dict__, "list_mode": "cell"}), tab).utotal()
     ref["force"] = ref["force"] * 100.0  # the engine's forces carry the force scale
     compare(got, ref, tol, tol)
-    log = eng.md_run(5, 0.002)
-    assert np.all(np.isfinite(log))
+    # the accelerations of the rescaled configuration (md_init), then steps against the oracle
+    eng.md_init(mass, v)
+    eng.md_run(4, 0.002)
+    xr, vr, _, _ = vv_reference(osp, tab, mass, v, 0.002, 4, 100.0)
+    dx = eng.positions() - xr
+    dx -= osp.box * np.rint(dx / osp.box)
+    print(f"ratio {ratio}: max |dx| {np.abs(dx).max():.2e} after 4 steps", flush=True)
+    assert np.abs(dx).max() < tol
     eng.close()

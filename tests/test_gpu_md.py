@@ -3,8 +3,8 @@ against the FP64 oracle.
 
 * UTotal over the Verlet list (rc + skin) equals the oracle's cell-list UTotal: forces within
   1e-5 relative (mixed) / 1e-9 (FP64).
-* MD: a Python velocity-Verlet loop driven by the oracle's forces, with the same wrap into the
-  box, is the trajectory reference; after 10 steps the FP64 path's positions agree to 1e-9 A
+* MD: a Python velocity-Verlet loop driven by the oracle's forces (VelVer's update order and
+  PBC), is the trajectory reference; after 10 steps the FP64 path's positions agree to 1e-9 A
   and the mixed path's to 1e-5 A.
 * Energy conservation over a few hundred steps, and the skin rule triggers rebuilds.
 """
@@ -13,7 +13,7 @@ import pytest
 
 import oracle
 from molsim_amd import systems
-from gpu_helpers import cabi_table, compare, engine, oracle_system
+from gpu_helpers import cabi_table, compare, engine, oracle_system, vv_reference
 
 pytestmark = pytest.mark.gpu
 
@@ -82,25 +82,6 @@ def test_verlet_dense_list_matches_oracle(precision, tol):
     eng.close()
 
 
-def _vv_reference(sp, tab, mass, v0, dt, nsteps, scale):
-    """Velocity Verlet with oracle forces (the VelVer update order and PBC wrap)."""
-    L = sp.box[0]
-    osp = systems.SystemSpec(**{**sp.__dict__, "list_mode": "cell"})
-    x = sp.ro.copy()
-    v = v0.copy()
-    minv = (1.0 / np.asarray(mass))[sp.ipt][:, None]
-    a = oracle_system(osp, tab).utotal()["force"] * scale * minv
-    for _ in range(nsteps):
-        x = x + dt * v + 0.5 * dt * dt * a
-        x -= L * np.rint(x / L)
-        x[x >= L / 2] -= L
-        f = oracle_system(systems.SystemSpec(**{**osp.__dict__, "ro": x}), tab).utotal()["force"]
-        an = f * scale * minv
-        v = v + 0.5 * dt * (a + an)
-        a = an
-    return x, v
-
-
 @pytest.mark.parametrize("precision,xtol", [("fp64", 1e-9), ("mixed", 1e-5)])
 def test_md_trajectory_matches_oracle(precision, xtol):
     sp = _fluid(n=1500, L=80.0)
@@ -116,7 +97,7 @@ def test_md_trajectory_matches_oracle(precision, xtol):
     log = eng.md_run(nsteps, dt)
     x = eng.positions()
     v = eng.velocities()
-    xr, vr = _vv_reference(sp, tab, mass, v0, dt, nsteps, scale)
+    xr, vr, _, _ = vv_reference(sp, tab, mass, v0, dt, nsteps, scale)
     dx = x - xr
     dx -= sp.box[0] * np.rint(dx / sp.box[0])
     print(f"max |dx| {np.abs(dx).max():.2e}, max |dv| {np.abs(v - vr).max():.2e}", flush=True)
