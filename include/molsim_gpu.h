@@ -162,6 +162,17 @@ int msg_dutotal(msg_ctx* ctx, int nptm, const int* ipnptm, const double* rotm, i
 int msg_dutotal_batch(msg_ctx* ctx, int ntrial, const int* ip, const double* rotm,
                       int* lhsoverlap, double* du_tot, double* du_rec);
 
+/* msg_dutotal_batch without allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops).  The trial
+ * inputs and per-trial records go through a fixed device scratch array of the library (2.6 MB
+ * per GPU, held until the process ends) instead of context buffers, so ntrial is at most
+ * 16384 (MSG_ERR_ARG beyond).  Same results as msg_dutotal_batch bit for bit.  It needs the
+ * state the allocating entry points set up: with SPME, one msg_dutotal / msg_dutotal_batch
+ * call must have run on the context (else MSG_ERR_STATE).  All calls on one GPU share the
+ * array: calls that overlap on one GPU (several host threads) race. */
+int msg_dutotal_batch_noalloc(msg_ctx* ctx, int ntrial, const int* ip, const double* rotm,
+                              int* lhsoverlap, double* du_tot, double* du_rec);
+
 /* MCUpdate after an accepted move: positions, resident structure factors / mesh, cell list. */
 int msg_commit(msg_ctx* ctx, int nptm, const int* ipnptm, const double* rotm);
 

@@ -315,7 +315,7 @@ int utotal_refresh(Ctx* c) {
   return MSG_OK;
 }
 
-static int check_ready(Ctx* c) {
+int check_ready_state(Ctx* c) {
   if (!c->have_table) return set_error(MSG_ERR_STATE, "msg_set_table has not been called");
   if (!c->have_particles) return set_error(MSG_ERR_STATE, "msg_set_particles has not been called");
   if (!c->have_positions) return set_error(MSG_ERR_STATE, "positions have not been set");
@@ -407,6 +407,7 @@ int msg_create(const msg_params* p, msg_ctx** out) {
   if (hipHostMalloc((void**)&c->h_flags, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess)
     return fail(set_error(MSG_ERR_NOMEM, "hipHostMalloc failed"));
   if ((e = launch_zero(c))) return fail(e);
+  if ((e = noalloc_setup(c->device))) return fail(e);  // the scratch of msg_dutotal_batch_noalloc
   if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_error(MSG_ERR_HIP, "sync failed"));
   *out = h;
   return MSG_OK;
@@ -637,7 +638,7 @@ int msg_nlist(msg_ctx* h) {
 int msg_utotal_async(msg_ctx* h, double* d_force) {
   if (!h) return set_error(MSG_ERR_ARG, "msg_utotal_async: null context");
   Ctx* c = &h->c;
-  if (int e = check_ready(c)) return e;
+  if (int e = check_ready_state(c)) return e;
   // the Verlet-list skin test is a host decision, so that mode launches eagerly
   const bool use_graph = !c->fp64 && !c->prof && c->P.list_mode != MSG_LIST_VERLET;
   if (use_graph) {
@@ -694,7 +695,7 @@ int msg_dutotal(msg_ctx* h, int nptm, const int* ipnptm, const double* rotm, int
   if (!h || nptm != 1 || !ipnptm || !rotm || !lhsoverlap)
     return set_error(MSG_ERR_ARG, "msg_dutotal: single-particle moves (nptm = 1) only");
   Ctx* c = &h->c;
-  if (int e = check_ready(c)) return e;
+  if (int e = check_ready_state(c)) return e;
   double dt, dr;
   std::vector<double> twob(c->nptpt + 1);
   int ov = 0;
@@ -711,7 +712,7 @@ int msg_dutotal_charge(msg_ctx* h, int nptm, const int* ipnptm, int* lhsoverlap,
                        double* du_rec, double* du_tot) {
   if (!h || !ipnptm || !lhsoverlap) return set_error(MSG_ERR_ARG, "msg_dutotal_charge: null argument");
   Ctx* c = &h->c;
-  if (int e = check_ready(c)) return e;
+  if (int e = check_ready_state(c)) return e;
   std::vector<double> twob(c->nptpt + 1);
   double dt = 0.0, dr = 0.0;
   int ov = 0;
@@ -729,7 +730,7 @@ int msg_dutotal_batch(msg_ctx* h, int ntrial, const int* ip, const double* rotm,
   if (!h || ntrial < 1 || !ip || !rotm || !lhsoverlap || !du_tot)
     return set_error(MSG_ERR_ARG, "msg_dutotal_batch: bad arguments");
   Ctx* c = &h->c;
-  if (int e = check_ready(c)) return e;
+  if (int e = check_ready_state(c)) return e;
   return dutotal_batch(c, ntrial, ip, rotm, lhsoverlap, du_tot, du_rec, nullptr);
 }
 

@@ -270,6 +270,8 @@ int dutotal_collect(Ctx* c, int ntrial, int* ovl, double* du_tot, double* du_rec
 int dutotal_charge(Ctx* c, int nptm, const int* ip1, int* ovl, double* du_twob, double* du_rec,
                    double* du_tot);
 int utotal_refresh(Ctx* c);                        // UTotal sequence + sync (for dE state)
+int check_ready_state(Ctx* c);                     // table, particles, positions, Ewald set up
+int noalloc_setup(int device);                     // scratch address of msg_dutotal_batch_noalloc
 int utotal_sequence(Ctx* c);
 int verlet_setup(Ctx* c);
 int verlet_check(Ctx* c, bool& rebuild);            // skin test of the current positions (host sync)

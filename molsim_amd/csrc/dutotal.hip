@@ -19,6 +19,8 @@
 //   surface      (|M + q dr|^2 - |M|^2) 2 pi / (3 V)  (DUTwoBodyEwaldSurf, :1504-1539).
 // The state (cell list, phi or S, dipole M) must match the current positions: after a
 // commit or new positions it is rebuilt by a UTotal pass before the next trials.
+#include <mutex>
+
 #include "common.h"
 
 namespace msg {
@@ -65,6 +67,7 @@ struct DeArgs {
   double epsi4pi;
   // trials
   const int* tip;
+  int tip_base;           // 0, or 1 when tip holds the caller's 1-based indices
   const double* tr;
   double* out;            // per trial: ntp + 4 doubles
   const int* laz;         // weak charges (null when off)
@@ -277,7 +280,7 @@ __device__ __forceinline__ void de_body(const DeArgs& a, int ip, const double* t
 template <int P>
 __global__ __launch_bounds__(256) void k_dutotal(DeArgs a) {
   const int t = blockIdx.x;
-  de_body<P>(a, a.tip[t], a.tr + 3 * t, a.out + (size_t)t * (a.ntp + 4));
+  de_body<P>(a, a.tip[t] - a.tip_base, a.tr + 3 * t, a.out + (size_t)t * (a.ntp + 4));
 }
 
 // Trials of many systems in one grid: workgroup b is trial b of the concatenation, system
@@ -607,6 +610,7 @@ static DeArgs de_args(Ctx* c) {
   for (int d = 0; d < 3; ++d) a.tpb[d] = 2.0 * M_PI / c->P.boxlen[d];
   a.epsi4pi = c->P.epsi4pi;
   a.tip = nullptr;
+  a.tip_base = 0;
   a.tr = nullptr;
   a.out = nullptr;
   a.laz = c->d_laz;
@@ -804,3 +808,124 @@ int commit_fast(Ctx* c, int ip, const double* rnew, bool& done) {
 }
 
 }  // namespace msg
+
+// ------------------------------------------------------------------------------------------
+// DUTotal for callers that cannot allocate (graph capture, tight serving loops): the trial
+// inputs and the per-trial records live in `scratch`, a file-scope device array of a fixed
+// size that each GPU the library runs on holds until the process ends.  Its device address is
+// looked up once per device at context creation (hipGetSymbolAddress, with that device
+// current); host code uses only that address, never the array's name.  Every call on a GPU
+// shares that GPU's array: calls that overlap on one GPU (from several threads or contexts)
+// race.  A call writes every slot it reads back, so nothing needs clearing between calls.
+namespace msg {
+
+namespace {
+
+constexpr int kNoAllocTrials = 16384;             // trials per call at most
+constexpr size_t kRecD = kMaxTP + 4;              // doubles per trial record
+constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t kOffRec = 0;
+constexpr size_t kOffIp = kOffRec + align256(kNoAllocTrials * kRecD * sizeof(double));
+constexpr size_t kOffR = kOffIp + align256(kNoAllocTrials * sizeof(int));
+constexpr size_t kOffOvl = kOffR + align256(3 * (size_t)kNoAllocTrials * sizeof(double));
+constexpr size_t kOffTot = kOffOvl + align256(kNoAllocTrials * sizeof(int));
+constexpr size_t kOffDrec = kOffTot + align256(kNoAllocTrials * sizeof(double));
+constexpr size_t kScratchBytes = kOffDrec + align256(kNoAllocTrials * sizeof(double));  // 2.6 MB
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+// The records of the trials unpacked into the caller's output layout (overlap flags,
+// du_tot, du_rec), so each comes back to the host with one copy.
+__global__ __launch_bounds__(256) void k_de_unpack(const double* __restrict__ rec, int stride, int ntp,
+                                                   int ntrial, int* ovl, double* tot, double* drec) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= ntrial) return;
+  const double* o = rec + (size_t)t * stride;
+  ovl[t] = o[ntp + 3] != 0.0;
+  tot[t] = o[ntp + 2];
+  drec[t] = o[ntp + 1];
+}
+
+}  // namespace
+
+// Once per device, with that device current (context creation): the address of this
+// device's copy of the scratch array.
+int noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return set_error(MSG_ERR_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return MSG_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return set_error(MSG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch[device] = (unsigned char*)ptr;
+  return MSG_OK;
+}
+
+}  // namespace msg
+
+using namespace msg;
+
+extern "C" {
+
+int msg_dutotal_batch_noalloc(msg_ctx* h, int ntrial, const int* ip, const double* rotm, int* lhsoverlap,
+                              double* du_tot, double* du_rec) {
+  if (!h || !ip || !rotm || !lhsoverlap || !du_tot)
+    return set_error(MSG_ERR_ARG, "msg_dutotal_batch_noalloc: null argument");
+  if (ntrial < 1 || ntrial > kNoAllocTrials)
+    return set_error(MSG_ERR_ARG, "msg_dutotal_batch_noalloc: ntrial must be 1..16384 (the scratch holds 16384 trials)");
+  Ctx* c = &h->c;
+  if (int e = check_ready_state(c)) return e;
+  for (int t = 0; t < ntrial; ++t)
+    if (ip[t] < 1 || ip[t] > c->np) return set_error(MSG_ERR_ARG, "msg_dutotal_batch_noalloc: particle index out of range");
+  unsigned char* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (c->device >= 0 && c->device < kMaxDevices) base = g_scratch[c->device];
+  }
+  if (!base) return set_error(MSG_ERR_STATE, "msg_dutotal_batch_noalloc: no scratch address for this device");
+  if (c->P.ewald_mode == MSG_EWALD_SPM && !c->have_gk)
+    return set_error(MSG_ERR_STATE,
+                     "msg_dutotal_batch_noalloc: the SPME trial tables are built by the first msg_dutotal / "
+                     "msg_dutotal_batch call (it allocates); call one of them once first");
+  if (!c->fresh)
+    if (int e = utotal_refresh(c)) return e;
+  if (int e = complete_cells(c)) return e;
+  double* rec = (double*)(base + kOffRec);
+  int* d_ip = (int*)(base + kOffIp);
+  double* d_r = (double*)(base + kOffR);
+  int* d_ovl = (int*)(base + kOffOvl);
+  double* d_tot = (double*)(base + kOffTot);
+  double* d_drec = (double*)(base + kOffDrec);
+  MSG_HIP(hipMemcpyAsync(d_ip, ip, ntrial * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  MSG_HIP(hipMemcpyAsync(d_r, rotm, 3 * (size_t)ntrial * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  DeArgs a = de_args(c);
+  a.tip = d_ip;
+  a.tip_base = 1;
+  a.tr = d_r;
+  a.out = rec;
+  // (no per-kernel timing here: its event bookkeeping grows a host vector)
+  switch (c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.order : 4) {
+    case 3: launch_de<3>(c, a, ntrial); break;
+    case 4: launch_de<4>(c, a, ntrial); break;
+    case 5: launch_de<5>(c, a, ntrial); break;
+    case 6: launch_de<6>(c, a, ntrial); break;
+    case 7: launch_de<7>(c, a, ntrial); break;
+    case 8: launch_de<8>(c, a, ntrial); break;
+    default: return set_error(MSG_ERR_ARG, "DUTotal: SPME order must be 3..8");
+  }
+  hipLaunchKernelGGL(k_de_unpack, dim3((ntrial + 255) / 256), dim3(256), 0, c->stream, rec, c->nptpt + 4,
+                     c->nptpt, ntrial, d_ovl, d_tot, d_drec);
+  MSG_HIP(hipGetLastError());
+  MSG_HIP(hipMemcpyAsync(lhsoverlap, d_ovl, ntrial * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  MSG_HIP(hipMemcpyAsync(du_tot, d_tot, ntrial * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (du_rec) MSG_HIP(hipMemcpyAsync(du_rec, d_drec, ntrial * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  MSG_HIP(hipStreamSynchronize(c->stream));
+  return MSG_OK;
+}
+
+}  // extern "C"

@@ -92,6 +92,7 @@ def lib():
         L.msg_fetch.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.msg_dutotal.argtypes = [_vp, C.c_int, _ip, _dp, C.POINTER(C.c_int), _vp, _vp, _vp]
         L.msg_dutotal_batch.argtypes = [_vp, C.c_int, _ip, _dp, _ip, _dp, _vp]
+        L.msg_dutotal_batch_noalloc.argtypes = [_vp, C.c_int, _ip, _dp, _ip, _dp, _vp]
         L.msg_commit.argtypes = [_vp, C.c_int, _ip, _dp]
         L.msg_set_weak_charges.argtypes = [_vp, _ip, _dp]
         L.msg_dutotal_charge.argtypes = [_vp, C.c_int, _ip, C.POINTER(C.c_int), _vp, _vp, _vp]
@@ -324,6 +325,18 @@ class Engine:
         dt = np.zeros(n)
         dr = np.zeros(n)
         _check(lib().msg_dutotal_batch(self.h, n, ips, r, ov, dt, _ptr(dr)))
+        return dict(overlap=ov.astype(bool), du_tot=dt, du_rec=dr)
+
+    def dutotal_batch_noalloc(self, ips, rnew) -> dict:
+        """dutotal_batch through the library's fixed scratch (no device allocation; at most
+        16384 trials)."""
+        ips = np.ascontiguousarray(np.asarray(ips) + 1, dtype=np.int32)
+        r = np.ascontiguousarray(rnew, dtype=np.float64).reshape(-1)
+        n = ips.size
+        ov = np.zeros(n, dtype=np.int32)
+        dt = np.zeros(n)
+        dr = np.zeros(n)
+        _check(lib().msg_dutotal_batch_noalloc(self.h, n, ips, r, ov, dt, _ptr(dr)))
         return dict(overlap=ov.astype(bool), du_tot=dt, du_rec=dr)
 
     def mc_update(self, ip: int, rnew):
