@@ -51,7 +51,8 @@ enum msg_status {
   MSG_ERR_TABLE = -4,      /* table construction did not converge (Src/potential.F90:1117) */
   MSG_ERR_TABLE_MEM = -5,  /* table larger than the buffer given                         */
   MSG_ERR_ODD_MESH = -6,   /* nmesh must be even                  (Src/energy.F90:6891) */
-  MSG_ERR_MESH = -7,       /* nmesh unsupported by the GPU FFT (power of two, 16..256)   */
+  MSG_ERR_MESH = -7,       /* nmesh below the order (or 4), above 1024, or too small a mesh
+                              for a high order's spread workspace                          */
   MSG_ERR_HIP = -8,        /* HIP runtime error                                          */
   MSG_ERR_NOMEM = -9,      /* device allocation failed                                   */
   MSG_ERR_CELL = -10,      /* cell list: rcell <= 0 or position outside the box          */
@@ -81,8 +82,9 @@ typedef struct msg_params {
   double alpha;        /* ualpha                                                */
   int ncut;            /* std Ewald k-space cut-off                             */
   int ncutregion;      /* 0 sphere, 1 cube                                      */
-  int nmesh;           /* SPME mesh points per dimension                        */
-  int order;           /* SPME B-spline order                                   */
+  int nmesh;           /* SPME mesh points per dimension: any even number (the reference's
+                          EwaldSetup, Src/energy.F90:6891-6892), order..1024        */
+  int order;           /* SPME B-spline order, 2..11                            */
   int precision;       /* msg_precision                                         */
   double force_scale;  /* forces are multiplied by this at the end; UTotal uses
                           (sclene/scllen)/sclfor (Src/energy.F90:408); 1 keeps energy/length */

@@ -41,9 +41,14 @@ constexpr int kUncharged = 8;
 constexpr int kMaxTabIntervals = 512;   // records below rcut staged in LDS (<= 48 KB in FP64)
 constexpr int kMaxStencil = 27;
 constexpr int kPairHB = 4;           // home particles per pair-kernel work item
-// SPME bricks: each mesh point belongs to one brick; spread and gather work per brick.
+// SPME bricks: each mesh point belongs to one brick; spread and gather work per brick.  Meshes
+// of a power-of-two K in 16..256 with orders 3..8 take bricks of exactly 16 x 8 x 8 points;
+// other meshes and orders (the general path) split each dimension into nb bricks of
+// floor(K / nb) or ceil(K / nb) points, nb = max(1, floor(K / kBG)), kBG >= order - 1, so a
+// particle's stencil reaches only its own brick and the next one in every dimension.
 constexpr int kBX = 16, kBY = 8, kBZ = 8;
-constexpr int kMaxOrder = 9;         // order - 1 <= kBY, kBZ
+constexpr int kMinOrder = 2, kMaxOrder = 11;  // SPME orders (Src/potential.F90:359-360,3268)
+constexpr int kMaxRadices = 16;               // FFT stages of the general path
 
 inline int tp_index(int npt, int i, int j) {
   if (i > j) { int t = i; i = j; j = t; }
@@ -80,7 +85,16 @@ struct MeshGrid {
   int nb[3];      // bricks per dimension
   int nbrick;
   double dkdr[3]; // K / L
+  int gen;        // general path: any even K, orders 2..11 (runtime-K FFT, bricks of any extent)
 };
+
+// Brick of mesh point m (0 <= m < K) along a dimension split into nb bricks whose first points
+// are floor(b K / nb): the largest b with floor(b K / nb) <= m.  With K a multiple of nb this
+// is m / (K / nb).
+__host__ __device__ inline int brick_of(int m, int nb, int K) { return ((m + 1) * nb - 1) / K; }
+__host__ __device__ inline int brick_index(const MeshGrid& g, int mx, int my, int mz) {
+  return brick_of(mx, g.nb[0], g.K) + g.nb[0] * (brick_of(my, g.nb[1], g.K) + g.nb[1] * brick_of(mz, g.nb[2], g.K));
+}
 
 enum KernelId : int;
 
@@ -150,6 +164,8 @@ struct Ctx {
   void* d_G = nullptr;           // H^3 reals (meshfac)
   void* d_tw = nullptr;          // K complex twiddles exp(-2 pi i m / K)
   int nzblocks = 0;              // partial sums of the z/conv pass
+  int nrad = 0;                  // general path: FFT radices, product K
+  int rad[kMaxRadices] = {};
 
   // std Ewald
   int nkvec = 0;

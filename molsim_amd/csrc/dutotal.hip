@@ -101,7 +101,10 @@ __device__ inline int fold(int d, int K) {
 // One trial: particle ip (0-based) to position trn[0..2]; results to o[0 .. ntp+3].
 template <int P>
 __device__ __forceinline__ void de_body(const DeArgs& a, int ip, const double* trn, double* o) {
-  constexpr int W = 2 * P - 1, W3 = W * W * W;
+  constexpr int W = 2 * P - 1;
+  // windows of g staged in LDS up to order 8; above, they are read from g itself
+  constexpr bool kStage = P <= 8;
+  constexpr int W3 = kStage ? W * W * W : 1;
   __shared__ double sh[8];
   __shared__ double thn[3][P], tho[3][P];
   __shared__ int mmn[3], mmo[3];
@@ -180,11 +183,13 @@ __device__ __forceinline__ void de_body(const DeArgs& a, int ip, const double* t
     __syncthreads();
     // windows of g around offset 0 and around mm_new - mm_old
     const int H = a.m.H;
-    for (int k = threadIdx.x; k < W3; k += 256) {
-      const int ox = k % W - (P - 1), oy = (k / W) % W - (P - 1), oz = k / (W * W) - (P - 1);
-      w0[k] = a.gk[fold(ox, K) + H * (fold(oy, K) + H * fold(oz, K))];
-      w1[k] = a.gk[fold(mmn[0] - mmo[0] + ox, K) +
-                   H * (fold(mmn[1] - mmo[1] + oy, K) + H * fold(mmn[2] - mmo[2] + oz, K))];
+    if constexpr (kStage) {
+      for (int k = threadIdx.x; k < W3; k += 256) {
+        const int ox = k % W - (P - 1), oy = (k / W) % W - (P - 1), oz = k / (W * W) - (P - 1);
+        w0[k] = a.gk[fold(ox, K) + H * (fold(oy, K) + H * fold(oz, K))];
+        w1[k] = a.gk[fold(mmn[0] - mmo[0] + ox, K) +
+                     H * (fold(mmn[1] - mmo[1] + oy, K) + H * fold(mmn[2] - mmo[2] + oz, K))];
+      }
     }
     __syncthreads();
     // term 1: sum dQ phi over the two stencils; term 2: pair sums with g
@@ -208,12 +213,21 @@ __device__ __forceinline__ void de_body(const DeArgs& a, int ip, const double* t
       double nn = 0.0, oo = 0.0, no = 0.0;
       for (int b = 0; b < P3; ++b) {
         const int jx = b % P, jy = (b / P) % P, jz = b / (P * P);
-        const int k = (jx - ix + P - 1) + W * ((jy - iy + P - 1) + W * (jz - iz + P - 1));
         const double tbn = thn[0][jx] * thn[1][jy] * thn[2][jz];
         const double tbo = tho[0][jx] * tho[1][jy] * tho[2][jz];
-        nn += tbn * w0[k];
-        oo += tbo * w0[k];
-        no += tbo * w1[k];
+        double g0, g1;
+        if constexpr (kStage) {
+          const int k = (jx - ix + P - 1) + W * ((jy - iy + P - 1) + W * (jz - iz + P - 1));
+          g0 = w0[k];
+          g1 = w1[k];
+        } else {
+          g0 = a.gk[fold(jx - ix, K) + H * (fold(jy - iy, K) + H * fold(jz - iz, K))];
+          g1 = a.gk[fold(mmn[0] - mmo[0] + jx - ix, K) +
+                    H * (fold(mmn[1] - mmo[1] + jy - iy, K) + H * fold(mmn[2] - mmo[2] + jz - iz, K))];
+        }
+        nn += tbn * g0;
+        oo += tbo * g0;
+        no += tbo * g1;
       }
       s2 += tn * nn + to * oo - 2.0 * tn * no;
     }
@@ -555,13 +569,17 @@ int dutotal_launch(Ctx* c, int ntrial, const int* ip, const double* rotm) {
   a.out = c->d_de;
   prof_begin(c, MSG_K_DE);
   switch (c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.order : 4) {
+    case 2: launch_de<2>(c, a, ntrial); break;
     case 3: launch_de<3>(c, a, ntrial); break;
     case 4: launch_de<4>(c, a, ntrial); break;
     case 5: launch_de<5>(c, a, ntrial); break;
     case 6: launch_de<6>(c, a, ntrial); break;
     case 7: launch_de<7>(c, a, ntrial); break;
     case 8: launch_de<8>(c, a, ntrial); break;
-    default: return set_error(MSG_ERR_ARG, "DUTotal: SPME order must be 3..8");
+    case 9: launch_de<9>(c, a, ntrial); break;
+    case 10: launch_de<10>(c, a, ntrial); break;
+    case 11: launch_de<11>(c, a, ntrial); break;
+    default: return set_error(MSG_ERR_ARG, "DUTotal: SPME order must be 2..11");
   }
   prof_end(c, MSG_K_DE);
   MSG_HIP(hipGetLastError());
@@ -628,7 +646,7 @@ int plan_de(const std::vector<Ctx*>& cs, SegOwner& own, std::vector<DeGroup>& ou
     by_order[c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.order : 4].push_back(s);
   }
   for (auto& kv : by_order) {
-    if (kv.first < 3 || kv.first > 8) return set_error(MSG_ERR_ARG, "DUTotal: SPME order must be 3..8");
+    if (kv.first < kMinOrder || kv.first > kMaxOrder) return set_error(MSG_ERR_ARG, "DUTotal: SPME order must be 2..11");
     std::vector<DeArgs> args;
     for (int s : kv.second) args.push_back(de_args(cs[s]));
     DeGroup g;
@@ -650,9 +668,10 @@ int launch_de_seg(const DeGroup& g, const int* d_tstart, const int* d_ip, const 
   switch (g.order) {
 #define MSG_DE_SEG(P) \
   case P: hipLaunchKernelGGL(k_dutotal_seg<P>, dim3(ntot), dim3(256), 0, s, a, d_tstart, n, d_ip, d_r, d_out); break;
-    MSG_DE_SEG(3) MSG_DE_SEG(4) MSG_DE_SEG(5) MSG_DE_SEG(6) MSG_DE_SEG(7) MSG_DE_SEG(8)
+    MSG_DE_SEG(2) MSG_DE_SEG(3) MSG_DE_SEG(4) MSG_DE_SEG(5) MSG_DE_SEG(6) MSG_DE_SEG(7) MSG_DE_SEG(8)
+    MSG_DE_SEG(9) MSG_DE_SEG(10) MSG_DE_SEG(11)
 #undef MSG_DE_SEG
-    default: return set_error(MSG_ERR_ARG, "DUTotal: SPME order must be 3..8");
+    default: return set_error(MSG_ERR_ARG, "DUTotal: SPME order must be 2..11");
   }
   MSG_HIP(hipGetLastError());
   return MSG_OK;
@@ -910,13 +929,17 @@ int msg_dutotal_batch_noalloc(msg_ctx* h, int ntrial, const int* ip, const doubl
   a.out = rec;
   // (no per-kernel timing here: its event bookkeeping grows a host vector)
   switch (c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.order : 4) {
+    case 2: launch_de<2>(c, a, ntrial); break;
     case 3: launch_de<3>(c, a, ntrial); break;
     case 4: launch_de<4>(c, a, ntrial); break;
     case 5: launch_de<5>(c, a, ntrial); break;
     case 6: launch_de<6>(c, a, ntrial); break;
     case 7: launch_de<7>(c, a, ntrial); break;
     case 8: launch_de<8>(c, a, ntrial); break;
-    default: return set_error(MSG_ERR_ARG, "DUTotal: SPME order must be 3..8");
+    case 9: launch_de<9>(c, a, ntrial); break;
+    case 10: launch_de<10>(c, a, ntrial); break;
+    case 11: launch_de<11>(c, a, ntrial); break;
+    default: return set_error(MSG_ERR_ARG, "DUTotal: SPME order must be 2..11");
   }
   hipLaunchKernelGGL(k_de_unpack, dim3((ntrial + 255) / 256), dim3(256), 0, c->stream, rec, c->nptpt + 4,
                      c->nptpt, ntrial, d_ovl, d_tot, d_drec);

@@ -79,7 +79,7 @@ __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
         mi[d] = k < 0 ? k + a.m.K : k;
       }
       a.mm[i] = mi[0] | (mi[1] << 10) | (mi[2] << 20);
-      const int b = mi[0] / kBX + a.m.nb[0] * (mi[1] / kBY + a.m.nb[1] * (mi[2] / kBZ));
+      const int b = brick_index(a.m, mi[0], mi[1], mi[2]);
       if (a.rank_brick)
         a.rank_brick[i] = atomicAdd(&a.brick_count[b], 1);
       else
@@ -291,7 +291,7 @@ __device__ __forceinline__ void scatter_body(const ScatterArgs& a, int bid) {
   }
   if ((a.parts & 2) && a.use_mesh) {
     const int p = a.mm[i];
-    const int b = (p & 1023) / kBX + a.m.nb[0] * (((p >> 10) & 1023) / kBY + a.m.nb[1] * ((p >> 20) / kBZ));
+    const int b = brick_index(a.m, p & 1023, (p >> 10) & 1023, p >> 20);
     const int dst = a.brick_start[b] + atomicAdd(&a.brick_fill[b], 1);
     typedef typename std::conditional<sizeof(real) == 4, float4, double4>::type R4;
     const real* fr = (const real*)a.frac;
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(256) void k_scatter_ls(ScatterLsArgs a) {
     a.s.cell_tmp[s_start[a.s.cellid[i]] + rk] = i;
   } else {
     const int p = a.s.mm[i];
-    const int b = (p & 1023) / kBX + a.s.m.nb[0] * (((p >> 10) & 1023) / kBY + a.s.m.nb[1] * ((p >> 20) / kBZ));
+    const int b = brick_index(a.s.m, p & 1023, (p >> 10) & 1023, p >> 20);
     const int dst = s_start[b] + rk;
     typedef typename std::conditional<sizeof(real) == 4, float4, double4>::type R4;
     const real* fr = (const real*)a.s.frac;

@@ -121,6 +121,9 @@ struct FftArgs {
   const void* G;      // influence function H^3
   ConvArgs ca;
   double* part;       // per zconv workgroup: energy, virial
+  // general path (runtime K): mesh size, the Stockham radices (product K)
+  int K, nrad;
+  int rad[kMaxRadices];
 };
 
 // r2c along x.  Workgroup: T line pairs (2T real rows); row = y + K*z.
@@ -306,6 +309,223 @@ MSG_FFT_ENTRY(k_fft_y, fft_y_body, , 0)
 MSG_FFT_ENTRY(k_fft_yi, fft_y_body, , 1)
 MSG_FFT_ENTRY(k_fft_zconv, fft_zconv_body)
 #undef MSG_FFT_ENTRY
+
+// ------------------------------------------------------------------------------------------
+// General path: any even K (the reference's EwaldSetup takes any even nmesh, e.g. its default
+// 48 and the 40 of its test decks).  The same five passes with K known only at run time: a
+// Stockham autosort FFT over K's factors (4, then 2, 3, 5, 7, ... in any order) where each
+// output of a stage is formed by one thread as
+//   y[jh Ns r + q Ns + jr] = sum_{m<r} x[jh Ns + jr + m K/r] w^{m (jr + q Ns) K / (Ns r)}
+// (w = e^{-2 pi i / K}, conjugated for the inverse; Ns = product of the earlier radices), the
+// radix-r generalisation of the radix-4 butterfly above.  Lines are held in LDS, T per
+// workgroup (as many as fit).
+template <typename C2>
+__device__ C2* lds_fft_g(C2* A, C2* B, const C2* __restrict__ tw, int K, int T, int LS, int nrad,
+                         const int* rad, bool inverse) {
+  C2* src = A;
+  C2* dst = B;
+  int Ns = 1;
+  for (int st = 0; st < nrad; ++st) {
+    const int r = rad[st], Kr = K / r, step = K / (Ns * r);
+    for (int idx = threadIdx.x; idx < T * K; idx += 256) {
+      const int line = idx / K, o = idx - line * K;
+      const int jr = o % Ns, q = (o / Ns) % r, jh = o / (Ns * r);
+      const C2* x = src + line * LS + jh * Ns + jr;
+      const int e = (jr + q * Ns) * step;  // < K
+      C2 acc = x[0];
+      int ex = 0;
+      for (int m = 1; m < r; ++m) {
+        ex += e;
+        if (ex >= K) ex -= K;
+        C2 w = tw[ex];
+        if (inverse) w = cconj(w);
+        acc = cadd(acc, cmul(x[m * Kr], w));
+      }
+      dst[line * LS + o] = acc;
+    }
+    __syncthreads();
+    C2* t = src; src = dst; dst = t;
+    Ns *= r;
+  }
+  return src;
+}
+
+template <typename real>
+__device__ __forceinline__ size_t hc_index_g(int K, int L, int w, bool ypass) {
+  const int NH = K / 2, NREG = NH * K;
+  if (L < NREG) {
+    const int kx = L % NH, v = L / NH;
+    return ypass ? kx + (size_t)NH * (w + (size_t)K * v) : kx + (size_t)NH * (v + (size_t)K * w);
+  }
+  const int v = L - NREG;
+  return (size_t)NH * K * K + (ypass ? w + (size_t)K * v : v + (size_t)K * w);
+}
+
+// Lines per workgroup of the general passes: x passes hold T line pairs of K, the y/z passes
+// T lines of K + 1 (padded) elements.
+template <typename real>
+__host__ __device__ inline int gen_lines_x(int K) { return FftCfg<real>::E / K; }
+template <typename real>
+__host__ __device__ inline int gen_lines_yz(int K) { return FftCfg<real>::E / (K + 1); }
+
+template <typename real>
+__device__ __forceinline__ void gfft_x_body(const FftArgs& fa, int bid) {
+  typedef typename C2T<real>::t C2;
+  constexpr int E = FftCfg<real>::E;
+  __shared__ C2 A[E], B[E], s_tw[E];
+  const int K = fa.K, H = K / 2 + 1, T = gen_lines_x<real>(K), NP = K * K / 2;
+  const real* __restrict__ Q = (const real*)fa.Q;
+  C2* __restrict__ Cm = (C2*)fa.Cm;
+  for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = ((const C2*)fa.tw)[k];
+  for (int idx = threadIdx.x; idx < T * K; idx += 256) {
+    const int line = idx / K, k = idx - line * K, p = bid * T + line;
+    A[idx] = p < NP ? cmk<C2>(Q[(size_t)(2 * p) * K + k], Q[(size_t)(2 * p + 1) * K + k]) : cmk<C2>(0, 0);
+  }
+  __syncthreads();
+  const C2* Z = lds_fft_g<C2>(A, B, s_tw, K, T, K, fa.nrad, fa.rad, false);
+  const real h = real(0.5);
+  C2* __restrict__ Nq = Cm + (size_t)(K / 2) * K * K;
+  for (int idx = threadIdx.x; idx < T * H; idx += 256) {
+    const int line = idx / H, k = idx - line * H, p = bid * T + line;
+    if (p >= NP) continue;
+    const size_t r0 = 2 * (size_t)p, r1 = r0 + 1;
+    const C2 z = Z[line * K + k], zc = cconj(Z[line * K + (k == 0 ? 0 : K - k)]);
+    const C2 sm = cadd(z, zc), df = csub(z, zc);
+    const C2 c0 = cmk<C2>(h * sm.x, h * sm.y), c1 = cmk<C2>(h * df.y, -h * df.x);
+    if (k < K / 2) {
+      Cm[r0 * (K / 2) + k] = c0;
+      Cm[r1 * (K / 2) + k] = c1;
+    } else {
+      Nq[r0] = c0;
+      Nq[r1] = c1;
+    }
+  }
+}
+
+template <typename real>
+__device__ __forceinline__ void gfft_xi_body(const FftArgs& fa, int bid) {
+  typedef typename C2T<real>::t C2;
+  constexpr int E = FftCfg<real>::E;
+  __shared__ C2 A[E], B[E], s_tw[E];
+  const int K = fa.K, H = K / 2 + 1, T = gen_lines_x<real>(K), NP = K * K / 2;
+  const C2* __restrict__ Cm = (const C2*)fa.Cm;
+  real* __restrict__ Q = (real*)fa.Q;
+  for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = ((const C2*)fa.tw)[k];
+  const C2* __restrict__ Nq = Cm + (size_t)(K / 2) * K * K;
+  for (int idx = threadIdx.x; idx < T * H; idx += 256) {
+    const int line = idx / H, k = idx - line * H, p = bid * T + line;
+    C2 a = cmk<C2>(0, 0), b = cmk<C2>(0, 0);
+    if (p < NP) {
+      const size_t r0 = 2 * (size_t)p, r1 = r0 + 1;
+      a = k < K / 2 ? Cm[r0 * (K / 2) + k] : Nq[r0];
+      b = k < K / 2 ? Cm[r1 * (K / 2) + k] : Nq[r1];
+    }
+    if (k == 0 || k == K / 2) { a.y = 0; b.y = 0; }
+    A[line * K + k] = cmk<C2>(a.x - b.y, a.y + b.x);                                   // a + i b
+    if (k > 0 && k < K / 2) A[line * K + K - k] = cmk<C2>(a.x + b.y, b.x - a.y);     // conj(a) + i conj(b)
+  }
+  __syncthreads();
+  const C2* Z = lds_fft_g<C2>(A, B, s_tw, K, T, K, fa.nrad, fa.rad, true);
+  for (int idx = threadIdx.x; idx < T * K; idx += 256) {
+    const int line = idx / K, k = idx - line * K, p = bid * T + line;
+    if (p >= NP) continue;
+    const C2 z = Z[idx];
+    Q[(size_t)(2 * p) * K + k] = z.x;
+    Q[(size_t)(2 * p + 1) * K + k] = z.y;
+  }
+}
+
+template <typename real>
+__device__ __forceinline__ void gfft_y_body(const FftArgs& fa, int bid, int inverse) {
+  typedef typename C2T<real>::t C2;
+  constexpr int E = FftCfg<real>::E;
+  __shared__ C2 A[E], B[E], s_tw[E];
+  const int K = fa.K, T = gen_lines_yz<real>(K), LS = K + 1, NL = (K / 2 + 1) * K;
+  C2* __restrict__ Cm = (C2*)fa.Cm;
+  for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = ((const C2*)fa.tw)[k];
+  const int L0 = bid * T;
+  for (int e = threadIdx.x; e < T * K; e += 256) {
+    const int kl = e % T, y = e / T, L = L0 + kl;
+    A[kl * LS + y] = L < NL ? Cm[hc_index_g<real>(K, L, y, true)] : cmk<C2>(0, 0);
+  }
+  __syncthreads();
+  const C2* R = lds_fft_g<C2>(A, B, s_tw, K, T, LS, fa.nrad, fa.rad, inverse != 0);
+  for (int e = threadIdx.x; e < T * K; e += 256) {
+    const int kl = e % T, y = e / T, L = L0 + kl;
+    if (L < NL) Cm[hc_index_g<real>(K, L, y, true)] = R[kl * LS + y];
+  }
+}
+
+template <typename real>
+__device__ __forceinline__ void gfft_zconv_body(const FftArgs& fa, int bid) {
+  typedef typename C2T<real>::t C2;
+  constexpr int E = FftCfg<real>::E;
+  __shared__ C2 A[E], B[E], s_tw[E];
+  __shared__ double red[8];
+  const int K = fa.K, H = K / 2 + 1, T = gen_lines_yz<real>(K), LS = K + 1, NL = H * K;
+  const int NH = K / 2, NREG = NH * K;
+  C2* __restrict__ Cm = (C2*)fa.Cm;
+  const real* __restrict__ G = (const real*)fa.G;
+  const ConvArgs& ca = fa.ca;
+  for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = ((const C2*)fa.tw)[k];
+  const int L0 = bid * T;
+  for (int e = threadIdx.x; e < T * K; e += 256) {
+    const int kl = e % T, z = e / T, L = L0 + kl;
+    A[kl * LS + z] = L < NL ? Cm[hc_index_g<real>(K, L, z, false)] : cmk<C2>(0, 0);
+  }
+  __syncthreads();
+  C2* R = lds_fft_g<C2>(A, B, s_tw, K, T, LS, fa.nrad, fa.rad, false);
+  double esum = 0.0, vsum = 0.0;
+  for (int e = threadIdx.x; e < T * K; e += 256) {
+#pragma clang fp contract(off)
+    const int kl = e / K, kz = e % K, L = L0 + kl;
+    if (L >= NL) continue;
+    const int kx = L < NREG ? L % NH : NH, y = L < NREG ? L / NH : L - NREG;
+    const int ky = y <= K / 2 ? y : K - y;
+    const int kzz = kz <= K / 2 ? kz : K - kz;
+    const real g = G[kx + H * (ky + H * kzz)];
+    C2 f = R[kl * LS + kz];
+    const double f2 = (double)f.x * (double)f.x + (double)f.y * (double)f.y;
+    const double w = (kx == 0 || kx == K / 2) ? 0.5 : 1.0;
+    const double m2 = ca.tpl2[0] * kx * kx + ca.tpl2[1] * ky * ky + ca.tpl2[2] * kzz * kzz;
+    const double eg = w * (double)g * f2;
+    esum += eg;
+    vsum += eg * (1.0 + 2.0 * m2 * ca.expf);
+    f.x *= g;
+    f.y *= g;
+    R[kl * LS + kz] = f;
+  }
+  __syncthreads();
+  C2* other = (R == A) ? B : A;
+  const C2* S = lds_fft_g<C2>(R, other, s_tw, K, T, LS, fa.nrad, fa.rad, true);
+  for (int e = threadIdx.x; e < T * K; e += 256) {
+    const int kl = e % T, z = e / T, L = L0 + kl;
+    if (L < NL) Cm[hc_index_g<real>(K, L, z, false)] = S[kl * LS + z];
+  }
+  esum = msgd::block_sum(esum, red);
+  vsum = msgd::block_sum(vsum, red);
+  if (threadIdx.x == 0) {
+    fa.part[2 * bid] = esum;
+    fa.part[2 * bid + 1] = vsum;
+  }
+}
+
+#define MSG_GFFT_ENTRY(name, body, ...)                                                     \
+  template <typename real>                                                                  \
+  __global__ __launch_bounds__(256) void name(FftArgs a) { body<real>(a, blockIdx.x __VA_ARGS__); } \
+  template <typename real>                                                                  \
+  __global__ __launch_bounds__(256) void name##_seg(const FftArgs* __restrict__ args,       \
+                                                    const int2* __restrict__ map) {         \
+    const int2 s = map[blockIdx.x];                                                         \
+    if (s.x < 0) return;                                                                    \
+    body<real>(args[s.x], s.y __VA_ARGS__);                                                 \
+  }
+MSG_GFFT_ENTRY(k_gfft_x, gfft_x_body)
+MSG_GFFT_ENTRY(k_gfft_xi, gfft_xi_body)
+MSG_GFFT_ENTRY(k_gfft_y, gfft_y_body, , 0)
+MSG_GFFT_ENTRY(k_gfft_yi, gfft_y_body, , 1)
+MSG_GFFT_ENTRY(k_gfft_zconv, gfft_zconv_body)
+#undef MSG_GFFT_ENTRY
 
 struct SpreadArgs {
   MeshGrid m;
@@ -506,6 +726,159 @@ __global__ __launch_bounds__(256) void k_spread_seg(const SpreadArgs* __restrict
   spread_body<real, P>(args[s.x], s.y);
 }
 
+// General-path spread: the same algorithm over bricks of any extent.  Brick b of a dimension
+// split into nb bricks spans [floor(b K / nb), floor((b + 1) K / nb)), at least order - 1
+// points, so again only a brick's own and next bricks hold its contributors.  The fixed-point
+// accumulator is sized at launch for the largest brick (dynamic LDS); fewer candidates are
+// staged per round for high orders.
+template <typename real, int P>
+__device__ __forceinline__ void spread_body_g(const SpreadArgs& a, int bid) {
+  typedef typename R4T<real>::t R4;
+  constexpr int CAP = P <= 8 ? 128 : 64;
+  extern __shared__ unsigned long long acc_g[];
+  __shared__ real th[CAP][3 * P];
+  __shared__ int loc[CAP];
+  __shared__ int rng[CAP];
+  __shared__ int wcnt[4];
+  __shared__ int r_beg[8], r_pref[9], r_len[8];
+  const int K = a.m.K;
+  const int nbx = a.m.nb[0], nby = a.m.nb[1], nbz = a.m.nb[2];
+  const int bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / (nbx * nby);
+  const int bb[3] = {bx, by, bz}, nbd[3] = {nbx, nby, nbz};
+  int lo[3], ext[3];
+  bool whole[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = bb[d] * K / nbd[d];
+    ext[d] = (bb[d] + 1) * K / nbd[d] - lo[d];
+    whole[d] = nbd[d] == 1;
+  }
+  const int ROW = ext[0] + 1, NACC = ROW * ext[1] * ext[2];
+  if (threadIdx.x < 8) {
+    const int ox = whole[0] ? 1 : 2, oy = whole[1] ? 1 : 2, oz = whole[2] ? 1 : 2;
+    const int t = threadIdx.x;
+    int beg = 0, len = 0;
+    if (t < ox * oy * oz) {
+      const int ix = t % ox, iy = (t / ox) % oy, iz = t / (ox * oy);
+      const int mb = (bx + ix) % nbx + nbx * ((by + iy) % nby + nby * ((bz + iz) % nbz));
+      beg = a.brick_start[mb];
+      len = a.brick_start[mb + 1] - beg;
+    }
+    r_beg[t] = beg;
+    r_len[t] = len;
+  }
+  for (int k = threadIdx.x; k < NACC; k += 256) acc_g[k] = 0ull;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int pref = 0;
+    for (int t = 0; t < 8; ++t) {
+      r_pref[t] = pref;
+      pref += r_len[t];
+    }
+    r_pref[8] = pref;
+  }
+  __syncthreads();
+  const int total = r_pref[8];
+  const R4* bq = (const R4*)a.bq;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int base = 0; base < total; base += CAP) {
+    const int t = base + threadIdx.x;
+    bool hit = false;
+    int lp = 0, rr = 0;
+    R4 v;
+    if (threadIdx.x < CAP && t < total) {
+      int r = 0;
+      while (r < 7 && t >= r_pref[r + 1]) ++r;
+      const int idx = r_beg[r] + (t - r_pref[r]);
+      const int packed = a.bmm[idx];
+      v = bq[idx];
+      int l[3] = {(packed & 1023) - lo[0], ((packed >> 10) & 1023) - lo[1], (packed >> 20) - lo[2]};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) l[d] += l[d] < 0 ? K : 0;
+      int x0, nx, y0, ny, z0, nz;
+      brick_range<P>(l[0], ext[0], whole[0], x0, nx);
+      brick_range<P>(l[1], ext[1], whole[1], y0, ny);
+      brick_range<P>(l[2], ext[2], whole[2], z0, nz);
+      hit = nx > 0 && ny > 0 && nz > 0;
+      lp = l[0] | (l[1] << 10) | (l[2] << 20);
+      rr = x0 | (nx << 4) | (y0 << 8) | (ny << 12) | (z0 << 16) | (nz << 20);
+    }
+    const unsigned long long bal = __ballot(hit);
+    if (lane == 0) wcnt[w] = __popcll(bal);
+    __syncthreads();
+    int off = 0;
+    for (int k = 0; k < w; ++k) off += wcnt[k];
+    const int ncon = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+    if (hit) {
+      const int k = off + __popcll(bal & ((1ull << lane) - 1ull));
+      loc[k] = lp;
+      rng[k] = rr;
+      real* T = th[k];
+      msgd::bspline<real, P>(v.x, T, (real*)nullptr);
+      msgd::bspline<real, P>(v.y, T + P, (real*)nullptr);
+      msgd::bspline<real, P>(v.z, T + 2 * P, (real*)nullptr);
+#pragma unroll
+      for (int i = 0; i < P; ++i) T[2 * P + i] *= v.w;
+    }
+    __syncthreads();
+    for (int task = threadIdx.x; task < ncon * P; task += 256) {
+      const int k = task / P, zs = task - k * P;
+      const int cr = rng[k];
+      if (zs >= ((cr >> 20) & 15)) continue;
+      const int x0 = cr & 15, nx = (cr >> 4) & 15, y0 = (cr >> 8) & 15, ny = (cr >> 12) & 15;
+      const int iz = ((cr >> 16) & 15) + zs;
+      const int cl = loc[k];
+      const int lx0 = cl & 1023, ly0 = (cl >> 10) & 1023;
+      int lz = (cl >> 20) - iz;
+      lz += lz < 0 ? K : 0;
+      const real* T = th[k];
+      const real wz = T[2 * P + iz];
+      for (int ys = 0; ys < ny; ++ys) {
+        const int iy = y0 + ys;
+        int ly = ly0 - iy;
+        ly += ly < 0 ? K : 0;
+        const real wyz = T[P + iy] * wz;
+        unsigned long long* rowp = acc_g + ROW * (ly + ext[1] * lz);
+        for (int ix = x0; ix < x0 + nx; ++ix) {
+          int lx = lx0 - ix;
+          lx += lx < 0 ? K : 0;
+          atomicAdd(rowp + lx, to_fixed<real>(T[ix] * wyz));
+        }
+      }
+    }
+    __syncthreads();
+  }
+  real* Q = (real*)a.Q;
+  const int npts = ext[0] * ext[1] * ext[2];
+  for (int k = threadIdx.x; k < npts; k += 256) {
+    const int lx = k % ext[0], ly = (k / ext[0]) % ext[1], lz = k / (ext[0] * ext[1]);
+    const double v = (double)(long long)acc_g[ROW * (ly + ext[1] * lz) + lx] * (1.0 / kQScale);
+    Q[(lo[0] + lx) + (size_t)K * ((lo[1] + ly) + (size_t)K * (lo[2] + lz))] = (real)v;
+  }
+}
+
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_spread_g(SpreadArgs a) { spread_body_g<real, P>(a, blockIdx.x); }
+
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_spread_g_seg(const SpreadArgs* __restrict__ args,
+                                                      const int2* __restrict__ map) {
+  const int2 s = map[blockIdx.x];
+  if (s.x < 0) return;  // no work for this XCD slot
+  spread_body_g<real, P>(args[s.x], s.y);
+}
+
+// Dynamic LDS of the general spread: the accumulator of the largest brick.
+inline size_t spread_g_lds(const MeshGrid& m) {
+  int e[3];
+  for (int d = 0; d < 3; ++d) e[d] = (m.K + m.nb[d] - 1) / m.nb[d];
+  return (size_t)(e[0] + 1) * e[1] * e[2] * sizeof(unsigned long long);
+}
+template <typename real, int P>
+constexpr size_t spread_g_static_lds() {
+  return (size_t)(P <= 8 ? 128 : 64) * (3 * P * sizeof(real) + 2 * sizeof(int)) + 128;
+}
+
 struct GatherArgs {
   MeshGrid m;
   int np;
@@ -520,15 +893,16 @@ struct GatherArgs {
   double scale;
 };
 
-// Eight lanes per particle (brick order); lane ix < P takes x offset ix of the stencil and
-// walks its P x P (y, z) rows, so the lanes of a particle read neighbouring words of one row
-// (one cache line per particle and load instead of one per z-plane).
-template <typename real, int P>
+// LPP lanes per particle (brick order; 8, or 16 for orders above 8); lane ix < P takes x
+// offset ix of the stencil and walks its P x P (y, z) rows, so the lanes of a particle read
+// neighbouring words of one row (one cache line per particle and load instead of one per
+// z-plane).
+template <typename real, int P, int LPP = 8>
 __device__ __forceinline__ void gather_body(const GatherArgs& a, int bid) {
   typedef typename R4T<real>::t R4;
   const double M[3] = {a.Mtot[0], a.Mtot[1], a.Mtot[2]};
   const int gid = bid * 256 + threadIdx.x;
-  const int i = gid >> 3, ix = gid & 7;
+  const int i = gid / LPP, ix = gid % LPP;
   const bool valid = i < a.np;
   const int K = a.m.K;
   real e0 = 0, e1 = 0, e2 = 0;
@@ -577,7 +951,7 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int bid) {
     }
   }
 #pragma unroll
-  for (int off = 4; off > 0; off >>= 1) {
+  for (int off = LPP / 2; off > 0; off >>= 1) {
     e0 += __shfl_xor(e0, off, 64);
     e1 += __shfl_xor(e1, off, 64);
     e2 += __shfl_xor(e2, off, 64);
@@ -596,6 +970,30 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int bid) {
 
 template <typename real, int P>
 __global__ __launch_bounds__(256) void k_gather(GatherArgs a) { gather_body<real, P>(a, blockIdx.x); }
+
+template <typename real, int P>
+constexpr int gather_lanes() { return P <= 8 ? 8 : 16; }
+
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_gather_g(GatherArgs a) {
+  gather_body<real, P, gather_lanes<real, P>()>(a, blockIdx.x);
+}
+
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_gather_g_seg(const GatherArgs* __restrict__ args,
+                                                      const int2* __restrict__ map) {
+  const int2 s = map[blockIdx.x];
+  if (s.x < 0) return;  // no work for this XCD slot
+  gather_body<real, P, gather_lanes<real, P>()>(args[s.x], s.y);
+}
+
+template <typename real, int P>
+__global__ __launch_bounds__(256) void k_gather_reduce_g(GatherArgs a, ReduceArgs r) {
+  if (blockIdx.x == gridDim.x - 1)
+    reduce_body<256>(r);
+  else
+    gather_body<real, P, gather_lanes<real, P>()>(a, blockIdx.x);
+}
 
 // The gather with the pass's final reduction as one extra (last) workgroup.  The reduction
 // reads only what earlier kernels of the pass wrote (k-space partials, the pre-summed pair
@@ -627,7 +1025,40 @@ FftArgs fft_args(Ctx* c) {
   for (int d = 0; d < 3; ++d) a.ca.tpl2[d] = std::pow(2.0 * M_PI / c->P.boxlen[d], 2);
   a.ca.expf = -1.0 / (4.0 * c->P.alpha * c->P.alpha);
   a.part = c->d_part + c->part_off_spme;
+  a.K = c->mesh.K;
+  a.nrad = c->nrad;
+  for (int k = 0; k < kMaxRadices; ++k) a.rad[k] = c->rad[k];
   return a;
+}
+
+// The five general passes (runtime K): x, y, zconv, yi, xi.
+template <typename real>
+void gfft_blocks(int K, int* nb) {
+  const int tx = gen_lines_x<real>(K), tyz = gen_lines_yz<real>(K);
+  nb[0] = nb[4] = (K * K / 2 + tx - 1) / tx;
+  nb[1] = nb[2] = nb[3] = ((K / 2 + 1) * K + tyz - 1) / tyz;
+}
+
+template <typename real>
+void launch_gfft_all(Ctx* c) {
+  int nb[5];
+  gfft_blocks<real>(c->mesh.K, nb);
+  const FftArgs a = fft_args(c);
+  prof_begin(c, MSG_K_FFT_X);
+  hipLaunchKernelGGL((k_gfft_x<real>), dim3(nb[0]), dim3(256), 0, c->stream, a);
+  prof_end(c, MSG_K_FFT_X);
+  prof_begin(c, MSG_K_FFT_Y);
+  hipLaunchKernelGGL((k_gfft_y<real>), dim3(nb[1]), dim3(256), 0, c->stream, a);
+  prof_end(c, MSG_K_FFT_Y);
+  prof_begin(c, MSG_K_FFT_ZCONV);
+  hipLaunchKernelGGL((k_gfft_zconv<real>), dim3(nb[2]), dim3(256), 0, c->stream, a);
+  prof_end(c, MSG_K_FFT_ZCONV);
+  prof_begin(c, MSG_K_FFT_YI);
+  hipLaunchKernelGGL((k_gfft_yi<real>), dim3(nb[3]), dim3(256), 0, c->stream, a);
+  prof_end(c, MSG_K_FFT_YI);
+  prof_begin(c, MSG_K_FFT_XI);
+  hipLaunchKernelGGL((k_gfft_xi<real>), dim3(nb[4]), dim3(256), 0, c->stream, a);
+  prof_end(c, MSG_K_FFT_XI);
 }
 
 // Workgroups of the five FFT stages for mesh K: x, y, zconv, yi, xi.
@@ -662,6 +1093,10 @@ void launch_fft_all(Ctx* c) {
 
 template <typename real>
 int launch_fft_dispatch(Ctx* c) {
+  if (c->mesh.gen) {
+    launch_gfft_all<real>(c);
+    return MSG_OK;
+  }
   switch (c->mesh.K) {
     case 16: launch_fft_all<real, 16>(c); break;
     case 32: launch_fft_all<real, 32>(c); break;
@@ -712,8 +1147,38 @@ void launch_spread_gather(Ctx* c, bool spread, bool with_reduce) {
   }
 }
 
+template <typename real, int P>
+void launch_spread_gather_g(Ctx* c, bool spread, bool with_reduce) {
+  if (spread) {
+    const SpreadArgs a = spread_args(c);
+    prof_begin(c, MSG_K_SPREAD);
+    hipLaunchKernelGGL((k_spread_g<real, P>), dim3(c->mesh.nbrick), dim3(256), spread_g_lds(c->mesh), c->stream, a);
+    prof_end(c, MSG_K_SPREAD);
+  } else {
+    const GatherArgs g = gather_args(c);
+    const int nb = (gather_lanes<real, P>() * c->np + 255) / 256;
+    prof_begin(c, MSG_K_GATHER);
+    if (with_reduce)
+      hipLaunchKernelGGL((k_gather_reduce_g<real, P>), dim3(nb + 1), dim3(256), 0, c->stream, g,
+                         reduce_args(c, true));
+    else
+      hipLaunchKernelGGL((k_gather_g<real, P>), dim3(nb), dim3(256), 0, c->stream, g);
+    prof_end(c, MSG_K_GATHER);
+  }
+}
+
 template <typename real>
 int spread_gather_dispatch(Ctx* c, bool spread, bool with_reduce = false) {
+  if (c->mesh.gen) {
+    switch (c->mesh.order) {
+#define MSG_SG_G(P) case P: launch_spread_gather_g<real, P>(c, spread, with_reduce); break;
+      MSG_SG_G(2) MSG_SG_G(3) MSG_SG_G(4) MSG_SG_G(5) MSG_SG_G(6) MSG_SG_G(7) MSG_SG_G(8) MSG_SG_G(9)
+      MSG_SG_G(10) MSG_SG_G(11)
+#undef MSG_SG_G
+      default: return set_error(MSG_ERR_ARG, "SPME order must be 2..11");
+    }
+    return MSG_OK;
+  }
   switch (c->mesh.order) {
     case 3: launch_spread_gather<real, 3>(c, spread, with_reduce); break;
     case 4: launch_spread_gather<real, 4>(c, spread, with_reduce); break;
@@ -734,15 +1199,36 @@ int spme_setup(Ctx* c) {
   const msg_params& P = c->P;
   const int K = P.nmesh, p = P.order;
   if (K % 2) return set_error(MSG_ERR_ODD_MESH, "nmesh must be even");
-  if (K < 16 || K > 256 || (K & (K - 1))) return set_error(MSG_ERR_MESH, "nmesh must be 16, 32, 64, 128 or 256");
-  if (p < 3 || p > 8) return set_error(MSG_ERR_ARG, "SPME order must be 3..8");
+  if (p < kMinOrder || p > kMaxOrder) return set_error(MSG_ERR_ARG, "SPME order must be 2..11");
+  if (K < p || K < 4 || K > 1024)
+    return set_error(MSG_ERR_MESH, "nmesh must be at least the order (and 4) and at most 1024");
   MeshGrid& m = c->mesh;
   m.K = K;
   m.H = K / 2 + 1;
   m.order = p;
-  m.nb[0] = K / kBX;
-  m.nb[1] = K / kBY;
-  m.nb[2] = K / kBZ;
+  const bool pow2 = K >= 16 && K <= 256 && (K & (K - 1)) == 0;
+  m.gen = !(pow2 && p >= 3 && p <= 8);
+  if (!m.gen) {
+    m.nb[0] = K / kBX;
+    m.nb[1] = K / kBY;
+    m.nb[2] = K / kBZ;
+  } else {
+    // bricks of at least max(8, p - 1) points, so a stencil reaches only the next brick
+    const int bmin = std::max(8, p - 1);
+    for (int d = 0; d < 3; ++d) m.nb[d] = std::max(1, K / bmin);
+    const size_t lds = spread_g_lds(m) + (c->fp64 ? (p <= 8 ? 128 : 64) * (3 * p * 8 + 8) : (p <= 8 ? 128 : 64) * (3 * p * 4 + 8)) + 256;
+    if (lds > 65536) return set_error(MSG_ERR_MESH, "nmesh too small for this SPME order (one brick spans the mesh)");
+    if (K > (c->fp64 ? FftCfg<double>::E : FftCfg<float>::E) - 1)
+      return set_error(MSG_ERR_MESH, "nmesh too large for the general FFT");
+    // Stockham radices: fours, a two, then the odd prime factors
+    int rest = K, n = 0;
+    while (rest % 4 == 0 && n < kMaxRadices) { c->rad[n++] = 4; rest /= 4; }
+    while (rest % 2 == 0 && n < kMaxRadices) { c->rad[n++] = 2; rest /= 2; }
+    for (int f = 3; rest > 1 && n < kMaxRadices; f += 2)
+      while (rest % f == 0 && n < kMaxRadices) { c->rad[n++] = f; rest /= f; }
+    if (rest != 1) return set_error(MSG_ERR_MESH, "nmesh has too many factors for the general FFT");
+    c->nrad = n;
+  }
   m.nbrick = m.nb[0] * m.nb[1] * m.nb[2];
   for (int d = 0; d < 3; ++d) m.dkdr[d] = K / P.boxlen[d];
   // spline at pos = 0: M_p(1..p-1) are the only non-zero integer-point values
@@ -814,8 +1300,13 @@ int spme_setup(Ctx* c) {
     MSG_HIP(hipMemcpy(c->d_G, Gf.data(), Gf.size() * 4, hipMemcpyHostToDevice));
     MSG_HIP(hipMemcpy(c->d_tw, twf.data(), twf.size() * 4, hipMemcpyHostToDevice));
   }
-  const int T = (c->fp64 ? FftCfg<double>::E : FftCfg<float>::E) / K;
-  c->nzblocks = (H * K + T - 1) / T;  // workgroups of the z pass (HcLines<K, T>::NB)
+  if (m.gen) {
+    const int T = c->fp64 ? gen_lines_yz<double>(K) : gen_lines_yz<float>(K);
+    c->nzblocks = (H * K + T - 1) / T;  // workgroups of the general z pass
+  } else {
+    const int T = (c->fp64 ? FftCfg<double>::E : FftCfg<float>::E) / K;
+    c->nzblocks = (H * K + T - 1) / T;  // workgroups of the z pass (HcLines<K, T>::NB)
+  }
   return MSG_OK;
 }
 
@@ -872,8 +1363,32 @@ void add_fft(Ctx* c, SegBuilder<FftArgs>* b, int key) {
   b[4].add(key, seg_fft_stage<real, K, 4>, a, nb[4]);
 }
 
+template <typename real, int STAGE>
+void seg_gfft_stage(const SegGroup& g, hipStream_t s) {
+  const FftArgs* a = (const FftArgs*)g.d_args;
+  switch (STAGE) {
+    case 0: hipLaunchKernelGGL((k_gfft_x_seg<real>), dim3(g.nblk), dim3(256), 0, s, a, g.d_map); break;
+    case 1: hipLaunchKernelGGL((k_gfft_y_seg<real>), dim3(g.nblk), dim3(256), 0, s, a, g.d_map); break;
+    case 2: hipLaunchKernelGGL((k_gfft_zconv_seg<real>), dim3(g.nblk), dim3(256), 0, s, a, g.d_map); break;
+    case 3: hipLaunchKernelGGL((k_gfft_yi_seg<real>), dim3(g.nblk), dim3(256), 0, s, a, g.d_map); break;
+    default: hipLaunchKernelGGL((k_gfft_xi_seg<real>), dim3(g.nblk), dim3(256), 0, s, a, g.d_map); break;
+  }
+}
+
 template <typename real>
 int add_fft_k(Ctx* c, SegBuilder<FftArgs>* b) {
+  if (c->mesh.gen) {  // one launch per stage covers every general-path mesh of this precision
+    int nb[5];
+    gfft_blocks<real>(c->mesh.K, nb);
+    const FftArgs a = fft_args(c);
+    const int key = sizeof(real) == 8 ? 2001 : 2000;
+    b[0].add(key, seg_gfft_stage<real, 0>, a, nb[0]);
+    b[1].add(key, seg_gfft_stage<real, 1>, a, nb[1]);
+    b[2].add(key, seg_gfft_stage<real, 2>, a, nb[2]);
+    b[3].add(key, seg_gfft_stage<real, 3>, a, nb[3]);
+    b[4].add(key, seg_gfft_stage<real, 4>, a, nb[4]);
+    return MSG_OK;
+  }
   const int key = (sizeof(real) == 8 ? 1000 : 0) + c->mesh.K;
   switch (c->mesh.K) {
     case 16: add_fft<real, 16>(c, b, key); break;
@@ -896,8 +1411,40 @@ void seg_gather(const SegGroup& g, hipStream_t s) {
   hipLaunchKernelGGL((k_gather_seg<real, P>), dim3(g.nblk), dim3(256), 0, s, (const GatherArgs*)g.d_args, g.d_map);
 }
 
+template <typename real, int P>
+void seg_spread_g(const SegGroup& g, hipStream_t s) {
+  hipLaunchKernelGGL((k_spread_g_seg<real, P>), dim3(g.nblk), dim3(256), g.lds, s, (const SpreadArgs*)g.d_args,
+                     g.d_map);
+}
+template <typename real, int P>
+void seg_gather_g(const SegGroup& g, hipStream_t s) {
+  hipLaunchKernelGGL((k_gather_g_seg<real, P>), dim3(g.nblk), dim3(256), 0, s, (const GatherArgs*)g.d_args, g.d_map);
+}
+
+template <typename real>
+int add_spread_gather_g(Ctx* c, SegBuilder<SpreadArgs>& bs, SegBuilder<GatherArgs>& bg) {
+  const int key = (sizeof(real) == 8 ? 300 : 200) + c->mesh.order;
+  SegLaunchFn fs, fg;
+  int lanes = 8;
+  switch (c->mesh.order) {
+#define MSG_SGG(P)               \
+  case P:                        \
+    fs = seg_spread_g<real, P>;  \
+    fg = seg_gather_g<real, P>;  \
+    lanes = gather_lanes<real, P>(); \
+    break;
+    MSG_SGG(2) MSG_SGG(3) MSG_SGG(4) MSG_SGG(5) MSG_SGG(6) MSG_SGG(7) MSG_SGG(8) MSG_SGG(9) MSG_SGG(10) MSG_SGG(11)
+#undef MSG_SGG
+    default: return set_error(MSG_ERR_ARG, "SPME order must be 2..11");
+  }
+  bs.add(key, fs, spread_args(c), c->mesh.nbrick, spread_g_lds(c->mesh));
+  bg.add(key, fg, gather_args(c), (lanes * c->np + 255) / 256);
+  return MSG_OK;
+}
+
 template <typename real>
 int add_spread_gather(Ctx* c, SegBuilder<SpreadArgs>& bs, SegBuilder<GatherArgs>& bg) {
+  if (c->mesh.gen) return add_spread_gather_g<real>(c, bs, bg);
   const int key = (sizeof(real) == 8 ? 100 : 0) + c->mesh.order;
   SegLaunchFn fs, fg;
 #define MSG_SG(P)              \

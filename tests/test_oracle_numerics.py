@@ -39,7 +39,7 @@ def test_bspline_cubic_closed_form(orc):
         assert np.allclose(v, [m4(x + i) for i in range(4)], atol=1e-15)
 
 
-@pytest.mark.parametrize("K", [8, 16, 32])
+@pytest.mark.parametrize("K", [8, 16, 32, 10, 40, 48])
 def test_fft3_matches_numpy(orc, K):
     rng = np.random.default_rng(K)
     q = rng.normal(size=(K, K, K))          # q[z, y, x], x fastest
@@ -50,16 +50,20 @@ def test_fft3_matches_numpy(orc, K):
     assert np.allclose(back, np.fft.irfftn(ref, s=(K, K, K), axes=(0, 1, 2)) * K ** 3, rtol=1e-12, atol=1e-9)
 
 
-def test_spme_setup_closed_form(orc):
+@pytest.mark.parametrize("K,order", [(16, 6), (48, 5), (40, 6), (40, 11)])
+def test_spme_setup_closed_form(orc, K, order):
     box = np.array([30.0, 30.0, 30.0])
-    K, order, alpha = 16, 6, 0.35
+    alpha = 0.35
     mf, ef, vf = orc.spme_setup(box, alpha, K, order)
     H = K // 2 + 1
     # b^2(m) = 1/|sum_k M_p(k+1) exp(2 pi i m k / K)|^2 from numpy's FFT of the spline
     _, spl, _ = orc.bspline(order, 0.0)
     arr = np.zeros(K)
     arr[: order - 1] = spl[1:order]
-    b2 = 1.0 / np.abs(np.fft.ifft(arr) * K) ** 2
+    with np.errstate(divide="ignore"):
+        b2 = 1.0 / np.abs(np.fft.ifft(arr) * K) ** 2
+    if order % 2:
+        b2[K // 2] = 0.0  # the spline's transform vanishes there for odd orders (set to 0)
     n = np.arange(H)
     kx = 2 * np.pi * n / box[0]
     M2 = kx[None, None, :] ** 2 + kx[None, :, None] ** 2 + kx[:, None, None] ** 2
