@@ -330,7 +330,10 @@ int msg_batch_utotal(msg_batch* b) {
   if (int e = batch_current(b, "msg_batch_utotal")) return e;
   if (int e = order_after_contexts(b->cs, b->origin)) return e;
   MSG_HIP(hipGraphLaunch(b->exec, b->origin));
-  for (Ctx* c : b->cs) c->fresh = c->cells_valid = true;
+  for (Ctx* c : b->cs) {
+    c->fresh = c->cells_valid = true;
+    c->cells_maintained = false;  // the batched pass sorted the cells afresh
+  }
   return MSG_OK;
 }
 
@@ -549,7 +552,10 @@ int msg_unit_launch(msg_unit* u, msg_ctx** ut, int n_u, int want_forces, msg_ctx
   if (int e = order_after_contexts(all, u->s)) return e;
   if (int e = u->own.flush(u->s)) return e;
   if (int e = launch_pass(u->pass, u->s, u->side, u->fork, u->join)) return e;
-  for (Ctx* c : pcs) c->fresh = c->cells_valid = true;
+  for (Ctx* c : pcs) {
+    c->fresh = c->cells_valid = true;
+    c->cells_maintained = false;
+  }
   if (ntot)
     if (int e = launch_trials(u->de, u->L, u->d_de, u->s)) return e;
   if (!pcs.empty())

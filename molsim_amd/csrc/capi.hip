@@ -237,8 +237,25 @@ int utotal_sequence(Ctx* c) {
     else if ((e = verlet_check(c, rebuild))) return e;
   }
   // with a Verlet list the cell order is only needed to rebuild the list (later users such
-  // as DUTotal complete it on demand)
-  const bool cells_needed = !verlet || rebuild;
+  // as DUTotal complete it on demand); a cell order that accepted moves have patched
+  // (UpdateCellIp) is kept, as the reference keeps its linked lists between moves
+  const bool maintained = c->cells_maintained && c->cells_valid && !verlet;
+  const bool cells_needed = (!verlet || rebuild) && !maintained;
+  // the wave-per-cell-chunk pair kernel needs the chunk table of a fresh sort: a patched
+  // order takes the neighbour-parallel kernel for this pass
+  const int lph_saved = c->pair_lph, npair_saved = c->npart_pair;
+  if (maintained && c->pair_lph == 0) {
+    c->pair_lph = 8;
+    c->npart_pair = pair_blocks(c);
+  }
+  struct Restore {
+    Ctx* c;
+    int lph, np;
+    ~Restore() {
+      c->pair_lph = lph;
+      c->npart_pair = np;
+    }
+  } restore{c, lph_saved, npair_saved};
   if (!early && (e = launch_bin(c, cells_needed))) return e;
   auto cell_work = [&]() -> int {
     if (early && rebuild)
@@ -258,7 +275,8 @@ int utotal_sequence(Ctx* c) {
     }
     return MSG_OK;
   };
-  c->cells_valid = cells_needed;
+  c->cells_valid = cells_needed || maintained;
+  if (cells_needed) c->cells_maintained = false;
   if (branch) {
     MSG_HIP(hipEventRecord(c->ev_fork, c->stream));
     // the mesh branch is the longer chain: recorded first, it is dispatched first
@@ -508,6 +526,7 @@ int msg_set_weak_charges(msg_ctx* h, const int* laz, const double* zat) {
   c->graph_ok = false;  // the captured pass must see the charge states
   c->fresh = false;
   c->vl_valid = false;  // the Verlet list's records hold the charge states
+  c->cells_maintained = false;  // the cell records' tags are set by a fresh sort
   return MSG_OK;
 }
 
@@ -527,6 +546,7 @@ int msg_commit_charge(msg_ctx* h, int nptm, const int* ipnptm) {
     c->sumq2 += c->h_az[i] * c->h_az[i];
     MSG_HIP(hipMemcpy(c->d_laz + i, &c->h_laz[i], sizeof(int), hipMemcpyHostToDevice));
     MSG_HIP(hipMemcpy(c->d_az + i, &c->h_az[i], sizeof(double), hipMemcpyHostToDevice));
+    if (int e = set_charge_tag(c, i)) return e;  // a patched cell order keeps its records
   }
   c->graph_ok = false;  // self term of the captured reduction
   c->fresh = false;     // structure factors and the cell records' charge tags are rebuilt
@@ -541,6 +561,7 @@ int msg_set_positions(msg_ctx* h, const double* ro) {
   MSG_HIP(hipStreamSynchronize(c->stream));
   c->have_positions = true;
   c->fresh = false;
+  c->cells_maintained = false;  // new positions: the next pass sorts the cells (SetCellList)
   return MSG_OK;
 }
 
@@ -550,6 +571,7 @@ int msg_set_positions_device(msg_ctx* h, const double* d_ro) {
   MSG_HIP(hipMemcpyAsync(c->d_ro, d_ro, 3 * c->np * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   c->have_positions = true;
   c->fresh = false;
+  c->cells_maintained = false;
   return MSG_OK;
 }
 
@@ -582,6 +604,7 @@ int msg_set_box(msg_ctx* h, const double* boxlen) {
   // keeps the old box lengths and is usable again after a successful msg_set_box.
   c->fresh = false;
   c->cells_valid = false;
+  c->cells_maintained = false;
   c->vl_valid = false;
   c->have_gk = false;
   c->graph_ok = false;
@@ -624,6 +647,7 @@ int msg_nlist(msg_ctx* h) {
   int e;
   if ((e = launch_bin(c)) || (e = launch_sort(c, 3)) || (e = launch_finalize(c))) return e;
   c->cells_valid = true;
+  c->cells_maintained = false;  // SetCellList: the fresh order
   if (c->P.list_mode == MSG_LIST_VERLET) {
     if ((e = launch_vlist_build(c))) return e;
     c->vl_valid = true;
@@ -640,7 +664,7 @@ int msg_utotal_async(msg_ctx* h, double* d_force) {
   Ctx* c = &h->c;
   if (int e = check_ready_state(c)) return e;
   // the Verlet-list skin test is a host decision, so that mode launches eagerly
-  const bool use_graph = !c->fp64 && !c->prof && c->P.list_mode != MSG_LIST_VERLET;
+  const bool use_graph = !c->fp64 && !c->prof && c->P.list_mode != MSG_LIST_VERLET && !c->cells_maintained;
   if (use_graph) {
     if (!c->graph_ok) {
       if (c->gexec) { hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }

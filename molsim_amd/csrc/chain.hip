@@ -12,14 +12,14 @@
 //              Ewald structure-factor change dS(k) = q (e^{ik.r_new} - e^{ik.r_old}) with
 //              dE = sum kfac (|S + dS|^2 - |S|^2), surface term;
 //   thread 0   Metropolis (expmax = 87 guards, weight 1);
-//   accepted   S += dS (kept in registers from the energy step), position in caller and cell
-//              order, dipole sum.
-// A move accepted into another cell of a multi-cell grid ends the launch: the caller
-// commits it, rebuilds the cell list by a UTotal pass and relaunches for the remaining
-// trials.  SPME systems keep the host-driven chain (an accepted move changes the whole
-// potential mesh).
+//   accepted   S += dS (kept in registers from the energy step), the dipole sum, the position,
+//              and UpdateCellIp (Src/celllist.F90:366-382) on the cell list in place: a move
+//              into another cell makes the particle its new cell's head and shifts the slots in
+//              between, so the chain runs on without leaving the launch.
+// SPME systems keep the host-driven chain (an accepted move changes the whole potential mesh).
 #include <cmath>
 
+#include "cellmove.h"
 #include "common.h"
 
 namespace msg {
@@ -31,15 +31,16 @@ constexpr int kChainKvec = 4;   // k-vectors per thread at most: nk <= 4096
 constexpr int kChainNcut = 31;  // largest k-vector index per axis
 
 struct ChainArgs {
-  int np, ntrial, isamp, structural, ewald, lsurf;
+  int np, ntrial, isamp, ewald, lsurf;
   double* ro;
   const double* az;
   const int* ipt;
   double4* cpos;
-  const int* cell_ip;
-  const int* cell_start;
-  const int* slot_of;
-  const int* slot_cell;
+  int* cell_ip;
+  int* cell_start;
+  int* slot_of;
+  int* slot_cell;
+  int* ctype;
   CellGrid g;
   Box box;
   double rcut2;
@@ -62,7 +63,7 @@ struct ChainArgs {
   double* am;
   int* next;            // sequential sampling cursor
   int* events;          // per trial: -1 overlap, 0 rejected, 1 accepted
-  double* stat;         // [0] sum of accepted dE, [1] accepted, [2] trials done, [3] stopped ip, [4..6] its position
+  double* stat;         // [0] sum of accepted dE, [1] accepted, [2] trials done, [3] moves into another cell
 };
 
 // MOLSIM's Random: Marsaglia xorshift combined with Park-Miller by Schrage's method.
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
   __shared__ double sh[kChainThreads / 64];
   // e^{i n (2 pi / L_d) r_d} for n = 0..ncut, per axis d, at the trial (0) and old (1) position
   __shared__ double2 s_eik[2][3][kChainNcut + 1];
-  __shared__ int s_ip, s_acc, s_stop, s_ovl;
+  __shared__ int s_ip, s_acc, s_ovl;
   __shared__ double s_rn[3];
   __shared__ int s_rng[3];
   __shared__ double s_am;
@@ -138,10 +139,18 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
     for (int k = 0; k < 3; ++k) s_rng[k] = a.rng[k];
     s_am = *a.am;
     s_next = *a.next;
-    a.stat[3] = -1.0;  // no move left to commit
   }
+  CellMoveArgs cm;
+  cm.g = a.g;
+  cm.box = a.box;
+  cm.cell_start = a.cell_start;
+  cm.cell_ip = a.cell_ip;
+  cm.cpos = a.cpos;
+  cm.ctype = a.ctype;
+  cm.slot_of = a.slot_of;
+  cm.slot_cell = a.slot_cell;
   double dsum = 0.0;
-  int nacc = 0, done = 0;
+  int nacc = 0, done = 0, ncross = 0;
   const double expmax = 87.0;  // Src/mol.F90:62
   for (int t = 0; t < a.ntrial; ++t) {
     if (tid == 0) {
@@ -275,20 +284,13 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
       }
       a.events[t] = s_ovl ? -1 : (acc ? 1 : 0);
       s_acc = acc;
-      s_stop = 0;
       if (acc) {
         dsum += du;
         ++nacc;
-        if (a.structural && chain_cell(a, rn) != a.slot_cell[a.slot_of[ip]]) {
-          s_stop = 1;
-          a.stat[3] = ip;
-          for (int d = 0; d < 3; ++d) a.stat[4 + d] = rn[d];
-        }
       }
     }
     __syncthreads();
     ++done;
-    if (s_stop) break;
     if (s_acc) {  // MCUpdate in place
       if (a.ewald == 1) {
 #pragma unroll
@@ -299,17 +301,13 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
           for (int c = 0; c < 8; ++c) a.sk[8 * kn + c] += dS[u][c];
         }
       }
+      ncross += cell_move<kChainThreads>(cm, ip, rn);
       if (tid == 0) {
         for (int d = 0; d < 3; ++d) {
           a.Mout[d] += q * (rn[d] - ro[d]);
           a.Mtot[d] = a.Mout[d];
           a.ro[3 * ip + d] = rn[d];
         }
-        double4 p = a.cpos[a.slot_of[ip]];
-        p.x = rn[0];
-        p.y = rn[1];
-        p.z = rn[2];
-        a.cpos[a.slot_of[ip]] = p;
       }
       __threadfence_block();
     }
@@ -322,6 +320,7 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
     a.stat[0] = dsum;
     a.stat[1] = nacc;
     a.stat[2] = done;
+    a.stat[3] = ncross;
   }
 }
 
@@ -333,10 +332,10 @@ bool chain_eligible(const Ctx* c) {
          (c->P.ewald_mode != MSG_EWALD_STD || (c->nkvec <= kChainThreads * kChainKvec && c->P.ncut <= kChainNcut));
 }
 
-// Runs ntrial trials of the chain (several launches when accepted moves change cells).
+// Runs ntrial trials of the chain in one launch.
 int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, int* iseed, int* rng_state,
              double* am, double* du_sum, int* naccept, int* events) {
-  // device scratch: rng (3 ints) + next (1 int) + events (ntrial ints); am + stat (7) + dtran (npt)
+  // device scratch: rng (3 ints) + next (1 int) + events (ntrial ints); am + stat (4) + dtran (npt)
   int* d_i = nullptr;
   double* d_d = nullptr;
   MSG_HIP(hipMalloc((void**)&d_i, (4 + (size_t)std::max(ntrial, 1)) * sizeof(int)));
@@ -358,90 +357,75 @@ int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, in
   for (int t = 0; t < c->npt; ++t) hd[8 + t] = dtran[t];
   MSG_HIP(hipMemcpyAsync(d_i, hi, sizeof(hi), hipMemcpyHostToDevice, c->stream));
   MSG_HIP(hipMemcpyAsync(d_d, hd, sizeof(hd), hipMemcpyHostToDevice, c->stream));
-  const bool structural = c->grid.ncell > 1;
-  double dsum = 0.0;
-  int nacc = 0, t0 = 0;
-  while (t0 < ntrial) {
-    if (!c->fresh)
-      if (int e = utotal_refresh(c)) return e;
-    ChainArgs a;
-    a.np = c->np;
-    a.ntrial = ntrial - t0;
-    a.isamp = isamp;
-    a.structural = structural;
-    a.ewald = c->P.ewald_mode;
-    a.lsurf = c->P.lsurf;
-    a.ro = c->d_ro;
-    a.az = c->d_az;
-    a.ipt = c->d_ipt;
-    a.cpos = (double4*)c->d_cpos;
-    a.cell_ip = c->d_cell_ip;
-    a.cell_start = c->d_cell_start;
-    a.slot_of = c->d_slot_of;
-    a.slot_cell = c->d_slot_cell;
-    a.g = c->grid;
-    for (int d = 0; d < 3; ++d) {
-      a.box.L[d] = c->P.boxlen[d];
-      a.box.half[d] = 0.5 * c->P.boxlen[d];
-      a.box.inv[d] = 1.0 / c->P.boxlen[d];
-      a.tpb[d] = 2.0 * M_PI / c->P.boxlen[d];
-    }
-    a.rcut2 = c->P.rcut * c->P.rcut;
-    a.cut_inclusive = c->P.list_mode != MSG_LIST_CELL;
-    a.tcoef = c->d_tcoef64;
-    a.tbins = c->d_tbins;
-    a.tm = c->tmeta;
-    a.nk = c->P.ewald_mode == MSG_EWALD_STD ? c->nkvec : 0;
-    a.ncut = c->P.ewald_mode == MSG_EWALD_STD ? c->P.ncut : 0;
-    a.kvec = c->d_kvec;
-    a.kfac = c->d_kfac;
-    a.sk = c->d_sk;
-    a.Mout = c->d_out + c->nptpt + 4;
-    a.Mtot = c->d_M;
-    const double vol = c->P.boxlen[0] * c->P.boxlen[1] * c->P.boxlen[2];
-    a.surf_fac = 2.0 * M_PI / (3.0 * vol);
-    a.epsi4pi = c->P.epsi4pi;
-    a.dtran = d_d + 8;
-    a.beta = beta;
-    a.rng = d_i;
-    a.am = d_d;
-    a.next = d_i + 3;
-    a.events = d_i + 4 + t0;
-    a.stat = d_d + 1;
-    prof_begin(c, MSG_K_DE);
-    if (a.nk <= kChainThreads)
-      hipLaunchKernelGGL(k_mc_chain<1>, dim3(1), dim3(kChainThreads), 0, c->stream, a);
-    else if (a.nk <= 2 * kChainThreads)
-      hipLaunchKernelGGL(k_mc_chain<2>, dim3(1), dim3(kChainThreads), 0, c->stream, a);
-    else
-      hipLaunchKernelGGL(k_mc_chain<4>, dim3(1), dim3(kChainThreads), 0, c->stream, a);
-    prof_end(c, MSG_K_DE);
-    MSG_HIP(hipGetLastError());
-    double st[7];
-    MSG_HIP(hipMemcpyAsync(st, d_d + 1, sizeof(st), hipMemcpyDeviceToHost, c->stream));
-    MSG_HIP(hipStreamSynchronize(c->stream));
-    dsum += st[0];
-    nacc += (int)st[1];
-    const int done = (int)st[2];
-    t0 += done;
-    if (st[3] >= 0.0) {
-      // the last trial was an accepted move into another cell: commit it here (the cell
-      // list is rebuilt by the UTotal pass before the next launch)
-      const int ip = (int)st[3];
-      const double rn[3] = {st[4], st[5], st[6]};
-      if (int e = commit_move(c, ip, rn)) return e;
-    }
+  if (!c->fresh)
+    if (int e = utotal_refresh(c)) return e;
+  ChainArgs a;
+  a.np = c->np;
+  a.ntrial = ntrial;
+  a.isamp = isamp;
+  a.ewald = c->P.ewald_mode;
+  a.lsurf = c->P.lsurf;
+  a.ro = c->d_ro;
+  a.az = c->d_az;
+  a.ipt = c->d_ipt;
+  a.cpos = (double4*)c->d_cpos;
+  a.cell_ip = c->d_cell_ip;
+  a.cell_start = c->d_cell_start;
+  a.slot_of = c->d_slot_of;
+  a.slot_cell = c->d_slot_cell;
+  a.ctype = c->d_ctype;
+  a.g = c->grid;
+  for (int d = 0; d < 3; ++d) {
+    a.box.L[d] = c->P.boxlen[d];
+    a.box.half[d] = 0.5 * c->P.boxlen[d];
+    a.box.inv[d] = 1.0 / c->P.boxlen[d];
+    a.tpb[d] = 2.0 * M_PI / c->P.boxlen[d];
   }
-  MSG_HIP(hipMemcpy(hi, d_i, sizeof(hi), hipMemcpyDeviceToHost));
-  MSG_HIP(hipMemcpy(hd, d_d, sizeof(double), hipMemcpyDeviceToHost));
-  if (events) MSG_HIP(hipMemcpy(events, d_i + 4, ntrial * sizeof(int), hipMemcpyDeviceToHost));
+  a.rcut2 = c->P.rcut * c->P.rcut;
+  a.cut_inclusive = c->P.list_mode != MSG_LIST_CELL;
+  a.tcoef = c->d_tcoef64;
+  a.tbins = c->d_tbins;
+  a.tm = c->tmeta;
+  a.nk = c->P.ewald_mode == MSG_EWALD_STD ? c->nkvec : 0;
+  a.ncut = c->P.ewald_mode == MSG_EWALD_STD ? c->P.ncut : 0;
+  a.kvec = c->d_kvec;
+  a.kfac = c->d_kfac;
+  a.sk = c->d_sk;
+  a.Mout = c->d_out + c->nptpt + 4;
+  a.Mtot = c->d_M;
+  const double vol = c->P.boxlen[0] * c->P.boxlen[1] * c->P.boxlen[2];
+  a.surf_fac = 2.0 * M_PI / (3.0 * vol);
+  a.epsi4pi = c->P.epsi4pi;
+  a.dtran = d_d + 8;
+  a.beta = beta;
+  a.rng = d_i;
+  a.am = d_d;
+  a.next = d_i + 3;
+  a.events = d_i + 4;
+  a.stat = d_d + 1;
+  prof_begin(c, MSG_K_DE);
+  if (a.nk <= kChainThreads)
+    hipLaunchKernelGGL(k_mc_chain<1>, dim3(1), dim3(kChainThreads), 0, c->stream, a);
+  else if (a.nk <= 2 * kChainThreads)
+    hipLaunchKernelGGL(k_mc_chain<2>, dim3(1), dim3(kChainThreads), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_mc_chain<4>, dim3(1), dim3(kChainThreads), 0, c->stream, a);
+  prof_end(c, MSG_K_DE);
+  MSG_HIP(hipGetLastError());
+  double st[4];
+  MSG_HIP(hipMemcpyAsync(st, d_d + 1, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  MSG_HIP(hipMemcpyAsync(hi, d_i, sizeof(hi), hipMemcpyDeviceToHost, c->stream));
+  MSG_HIP(hipMemcpyAsync(hd, d_d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (events) MSG_HIP(hipMemcpyAsync(events, d_i + 4, ntrial * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  MSG_HIP(hipStreamSynchronize(c->stream));
   *iseed = hi[0];
   rng_state[0] = hi[1];
   rng_state[1] = hi[2];
   *am = hd[0];
   c->mc_next = hi[3];
-  if (du_sum) *du_sum = dsum;
-  if (naccept) *naccept = nacc;
+  if (st[3] > 0.0) c->cells_maintained = true;  // the cell order carries the chain's moves
+  if (du_sum) *du_sum = st[0];
+  if (naccept) *naccept = (int)st[1];
   return MSG_OK;
 }
 

@@ -248,6 +248,8 @@ struct Ctx {
   double* d_de_r = nullptr;
   bool fresh = false;            // a UTotal has run on the current positions
   bool cells_valid = false;      // the cell order (cell_ip, cpos, ...) matches the last pass
+  bool cells_maintained = false; // accepted moves have patched the cell order (UpdateCellIp):
+                                 // passes keep it instead of sorting the cells afresh
   int mc_next = 0;               // next particle of a sequential MC pass (isamp = 0)
 };
 
@@ -277,7 +279,8 @@ int launch_zero(Ctx* c);                           // counters and flags
 int complete_cells(Ctx* c);                        // cell order of the current positions, if a pass left it out
 int dutotal_batch(Ctx* c, int ntrial, const int* ip, const double* rotm, int* ovl,
                   double* du_tot, double* du_rec, double* du_twob);
-int commit_move(Ctx* c, int ip, const double* rnew);
+int commit_move(Ctx* c, int ip, const double* rnew);   // MCUpdate: S(k), dipole, position, UpdateCellIp
+int set_charge_tag(Ctx* c, int ip);                  // cell record tag after a charge-state flip
 bool chain_eligible(const Ctx* c);                 // device-resident MC chain applies
 int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, int* iseed,
              int* rng_state, double* am, double* du_sum, int* naccept, int* events);

@@ -22,6 +22,7 @@
 #include <mutex>
 
 #include "common.h"
+#include "cellmove.h"
 
 namespace msg {
 
@@ -735,39 +736,40 @@ int dutotal_charge(Ctx* c, int nptm, const int* ip1, int* ovl, double* du_twob, 
   return dutotal_collect(c, 1, ovl, du_tot, du_rec, du_twob);
 }
 
-// MCUpdate for one accepted single-particle move: the position changes on the device and
-// the derived state (cells, phi or S, dipole) is rebuilt by the next UTotal pass.
-int commit_fast(Ctx* c, int ip, const double* rnew, bool& done);
-
-int commit_move(Ctx* c, int ip, const double* rnew) {
-  bool done = false;
-  if (int e = commit_fast(c, ip, rnew, done)) return e;
-  if (done) return MSG_OK;
-  MSG_HIP(hipMemcpyAsync(c->d_ro + 3 * ip, rnew, 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  MSG_HIP(hipStreamSynchronize(c->stream));
-  c->fresh = false;
-  return MSG_OK;
-}
-
 }  // namespace msg
 
 namespace msg {
 
 namespace {
 
-// Incremental MCUpdate for the standard-Ewald state (EwaldUpdateArray, Src/denergy.F90:1776-
-// 1819): S(k) += q (e^{ik.r_new} - e^{ik.r_old}) for every vector, the position (caller and
-// cell order) and the dipole sum move with the particle.  One workgroup.
-__global__ __launch_bounds__(256) void k_commit_std(int ip, double rx, double ry, double rz,
-                                                    double* ro, double4* cpos, const int* slot_of,
-                                                    const double* az, int nk, const int* kvec,
-                                                    double tpb0, double tpb1, double tpb2, double* sk,
-                                                    double* Mout, double* Mtot) {
-  const double q = az[ip];
-  const double ro_old[3] = {ro[3 * ip], ro[3 * ip + 1], ro[3 * ip + 2]};
-  const double rn[3] = {rx, ry, rz};
-  for (int kn = threadIdx.x; kn < nk; kn += 256) {
-    const double kx = kvec[3 * kn] * tpb0, ky = kvec[3 * kn + 1] * tpb1, kz = kvec[3 * kn + 2] * tpb2;
+// MCUpdate for one accepted single-particle move (Src/mc.F90:5203-5235): EwaldUpdateArray for
+// standard Ewald (S(k) += q (e^{ik.r_new} - e^{ik.r_old}), Src/denergy.F90:1776-1819), the dipole
+// sum, ro <- rotm and UpdateCellIp (the cell list patched in place, head insertion), all in one
+// workgroup; no UTotal pass follows for standard Ewald or no Ewald.  With SPME the convolved
+// mesh is refreshed by the next pass (which keeps the patched cell order).
+
+struct CommitArgs {
+  int ip;
+  double rn[3];
+  double* ro;
+  const double* az;
+  int nk;                 // standard Ewald: k-vectors to update (0: none)
+  const int* kvec;
+  double tpb[3];
+  double* sk;
+  double* Mout;
+  double* Mtot;
+  int cells;              // the cell list is valid: patch it
+  CellMoveArgs cm;
+};
+
+__global__ __launch_bounds__(1024) void k_commit(CommitArgs a) {
+  const int ip = a.ip;
+  const double q = a.az[ip];
+  const double ro_old[3] = {a.ro[3 * ip], a.ro[3 * ip + 1], a.ro[3 * ip + 2]};
+  const double* rn = a.rn;
+  for (int kn = threadIdx.x; kn < a.nk; kn += 1024) {
+    const double kx = a.kvec[3 * kn] * a.tpb[0], ky = a.kvec[3 * kn + 1] * a.tpb[1], kz = a.kvec[3 * kn + 2] * a.tpb[2];
     double sxn, cxn, syn, cyn, szn, czn, sxo, cxo, syo, cyo, szo, czo;
     sincos(kx * rn[0], &sxn, &cxn);
     sincos(ky * rn[1], &syn, &cyn);
@@ -781,48 +783,74 @@ __global__ __launch_bounds__(256) void k_commit_std(int ip, double rx, double ry
       const double gxo = (c & 2) ? sxo : -sxo, gyo = (c & 1) ? syo : -syo;
       const double r1n = cxn * cyn - gxn * gyn, i1n = cxn * gyn + gxn * cyn;
       const double r1o = cxo * cyo - gxo * gyo, i1o = cxo * gyo + gxo * cyo;
-      sk[8 * kn + 2 * c] += q * ((r1n * czn - i1n * szn) - (r1o * czo - i1o * szo));
-      sk[8 * kn + 2 * c + 1] += q * ((r1n * szn + i1n * czn) - (r1o * szo + i1o * czo));
+      a.sk[8 * kn + 2 * c] += q * ((r1n * czn - i1n * szn) - (r1o * czo - i1o * szo));
+      a.sk[8 * kn + 2 * c + 1] += q * ((r1n * szn + i1n * czn) - (r1o * szo + i1o * czo));
     }
   }
-  __syncthreads();
+  if (a.cells) cell_move<1024>(a.cm, ip, rn);
+  __syncthreads();  // every thread has read the old position
   if (threadIdx.x == 0) {
     for (int d = 0; d < 3; ++d) {
-      Mout[d] += q * (rn[d] - ro_old[d]);
-      Mtot[d] = Mout[d];
-      ro[3 * ip + d] = rn[d];
+      a.Mout[d] += q * (rn[d] - ro_old[d]);
+      a.Mtot[d] = a.Mout[d];
+      a.ro[3 * ip + d] = rn[d];
     }
-    double4 p = cpos[slot_of[ip]];
-    p.x = rn[0];
-    p.y = rn[1];
-    p.z = rn[2];
-    cpos[slot_of[ip]] = p;
   }
+}
+
+// The cell-ordered record of a particle whose charge state flipped: its type tag.
+__global__ void k_set_tag(int ip, const int* slot_of, double4* cpos, double tag) {
+  if (threadIdx.x == 0) cpos[slot_of[ip]].w = tag;
 }
 
 }  // namespace
 
-// Accepted move: incremental for std Ewald when the particle keeps its cell, otherwise the
-// position changes and the derived state is rebuilt by the next UTotal pass.
-int commit_fast(Ctx* c, int ip, const double* rnew, bool& done) {
-  done = false;
-  if (c->P.ewald_mode != MSG_EWALD_STD || !c->fresh || !c->cells_valid) return MSG_OK;
-  std::vector<double> ro(3);
-  MSG_HIP(hipMemcpy(ro.data(), c->d_ro + 3 * ip, 3 * sizeof(double), hipMemcpyDeviceToHost));
+CellMoveArgs cell_move_args(Ctx* c) {
+  CellMoveArgs m;
+  m.g = c->grid;
   for (int d = 0; d < 3; ++d) {
-    const double h = 0.5 * c->P.boxlen[d];
-    const int a = std::min(std::max((int)std::floor((ro[d] + h) * c->grid.csi[d]), 0), c->grid.nc[d] - 1);
-    const int b = std::min(std::max((int)std::floor((rnew[d] + h) * c->grid.csi[d]), 0), c->grid.nc[d] - 1);
-    if (a != b) return MSG_OK;
+    m.box.L[d] = c->P.boxlen[d];
+    m.box.half[d] = 0.5 * c->P.boxlen[d];
+    m.box.inv[d] = 1.0 / c->P.boxlen[d];
   }
-  const int ntp = c->nptpt;
-  hipLaunchKernelGGL(k_commit_std, dim3(1), dim3(256), 0, c->stream, ip, rnew[0], rnew[1], rnew[2],
-                     c->d_ro, (double4*)c->d_cpos, c->d_slot_of, c->d_az, c->nkvec, c->d_kvec,
-                     2.0 * M_PI / c->P.boxlen[0], 2.0 * M_PI / c->P.boxlen[1],
-                     2.0 * M_PI / c->P.boxlen[2], c->d_sk, c->d_out + ntp + 4, c->d_M);
+  m.cell_start = c->d_cell_start;
+  m.cell_ip = c->d_cell_ip;
+  m.cpos = (double4*)c->d_cpos;
+  m.ctype = c->d_ctype;
+  m.slot_of = c->d_slot_of;
+  m.slot_cell = c->d_slot_cell;
+  return m;
+}
+
+int commit_move(Ctx* c, int ip, const double* rnew) {
+  CommitArgs a;
+  a.ip = ip;
+  for (int d = 0; d < 3; ++d) {
+    a.rn[d] = rnew[d];
+    a.tpb[d] = 2.0 * M_PI / c->P.boxlen[d];
+  }
+  a.ro = c->d_ro;
+  a.az = c->d_az;
+  // the structure factors are updated when they hold the current state (else a pass rebuilds them)
+  a.nk = (c->P.ewald_mode == MSG_EWALD_STD && c->fresh) ? c->nkvec : 0;
+  a.kvec = c->d_kvec;
+  a.sk = c->d_sk;
+  a.Mout = c->d_out + c->nptpt + 4;
+  a.Mtot = c->d_M;
+  a.cells = c->cells_valid ? 1 : 0;
+  a.cm = cell_move_args(c);
+  hipLaunchKernelGGL(k_commit, dim3(1), dim3(1024), 0, c->stream, a);
   MSG_HIP(hipGetLastError());
-  MSG_HIP(hipStreamSynchronize(c->stream));
-  done = true;
+  if (c->P.ewald_mode == MSG_EWALD_SPM) c->fresh = false;  // the convolved mesh is the old one
+  if (c->cells_valid) c->cells_maintained = true;
+  return MSG_OK;
+}
+
+int set_charge_tag(Ctx* c, int ip) {
+  if (!c->cells_valid) return MSG_OK;  // the next pass ranks the cells afresh
+  const double tag = (double)(c->h_ipt[ip] + (c->h_laz[ip] ? 0 : kUncharged));
+  hipLaunchKernelGGL(k_set_tag, dim3(1), dim3(64), 0, c->stream, ip, c->d_slot_of, (double4*)c->d_cpos, tag);
+  MSG_HIP(hipGetLastError());
   return MSG_OK;
 }
 

@@ -440,6 +440,7 @@ int msg_md_run(msg_ctx* h, int nsteps, double dt, double* log) {
     hipLaunchKernelGGL(k_drift, dim3(nb), dim3(256), 0, c->stream, a);
     if (skin) c->drosum_reset = false;
     c->fresh = false;
+    c->cells_maintained = false;  // every particle moved: the cells are sorted afresh
     if (skin && (e = launch_skin(c))) break;
     if ((e = utotal_sequence(c))) break;
     a = md_args(c, dt, false);

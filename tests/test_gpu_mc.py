@@ -20,6 +20,8 @@ def _oracle_mc(sp, tab, ntrial, dtran, beta, seed):
     ro = sp.ro.copy()
     L = sp.box[0]
     events = []
+    # MOLSIM's linked cell lists, updated by UpdateCellIp on every accepted move
+    cells = oracle.LinkedCellList(sp.ro, sp.box, sp.rcut + sp.drnlist, sp.rcut) if sp.list_mode == "cell" else None
     osys = oracle_system(systems.SystemSpec(**{**sp.__dict__, "ro": ro}), tab)
     nxt = 0
     for _ in range(ntrial):
@@ -45,15 +47,18 @@ def _oracle_mc(sp, tab, ntrial, dtran, beta, seed):
         events.append(-1 if du["overlap"] else int(acc))
         if acc:
             ro[ip] = rn
-    return np.array(events), ro, rng
+            if cells is not None:
+                cells.update(ip, rn)
+    return np.array(events), ro, rng, cells
 
 
 @pytest.mark.parametrize("ewald,precision", [("std", "fp64"), ("std_cell", "fp64"), ("spm", "fp64"),
                                              ("spm", "mixed")])
 def test_mc_pass_matches_oracle_loop(ewald, precision):
-    """std: the device-resident chain (all pairs, never leaves the launch); std_cell: the
-    device chain on a 3x3x3 cell grid (accepted moves into another cell end a launch and the
-    pass resumes after the rebuild); spm: the host-driven chain."""
+    """std: the device-resident chain (all pairs); std_cell: the device chain on a 3x3x3 cell
+    grid (accepted moves into another cell patch the cell list inside the launch, as
+    UpdateCellIp does); spm: the host-driven chain.  With a cell list, the cell order after the
+    pass equals MOLSIM's linked lists after the same UpdateCellIp calls."""
     sp = systems.small_spme_system(n=200, L=36.0, nmesh=32, alpha=0.3, rcut=11.0)
     if ewald == "std":
         sp = systems.SystemSpec(**{**sp.__dict__, "ewald": "std", "ncut": 6, "list_mode": "all",
@@ -67,7 +72,7 @@ def test_mc_pass_matches_oracle_loop(ewald, precision):
     eng = Engine(sp, precision=precision, table=cabi_table(tab))
     rng = Random(-4711)
     dsum, nacc, ev = eng.mc_pass(ntrial, dtran, beta, rng)
-    ref_ev, ref_ro, ref_rng = _oracle_mc(sp, tab, ntrial, dtran, beta, -4711)
+    ref_ev, ref_ro, ref_rng, ref_cells = _oracle_mc(sp, tab, ntrial, dtran, beta, -4711)
     agree = float(np.mean(ev == ref_ev))
     print(f"accepted {nacc}/{ntrial}, decisions agreeing with the oracle loop {agree:.4f}", flush=True)
     if precision == "fp64":
@@ -76,6 +81,10 @@ def test_mc_pass_matches_oracle_loop(ewald, precision):
         assert np.abs(ro - ref_ro).max() < 1e-9
         # the generator state carried through the pass continues the reference's sequence
         assert rng() == ref_rng()
+        if ref_cells is not None:
+            nc, start, cip = eng.cell_list()
+            got = [list(cip[start[c]:start[c + 1]]) for c in range(len(start) - 1)]
+            assert got == ref_cells.members()
         # and the state after the pass equals a fresh evaluation of the final positions
         full = eng.utotal()
         ref = oracle_system(systems.SystemSpec(**{**sp.__dict__, "ro": ref_ro}), tab).utotal()
