@@ -439,7 +439,7 @@ void msg_destroy(msg_ctx* h) {
   if (c->graph) hipGraphDestroy(c->graph);
   void* ptrs[] = {c->d_ro, c->d_az, c->d_ipt, c->d_tcoef, c->d_tbins, c->d_cellid, c->d_cell_chunk,
                   c->d_cell_count, c->d_cell_start, c->d_cell_fill, c->d_cell_tmp, c->d_cell_ip,
-                  c->d_slot_of, c->d_cpos, c->d_ctype, c->d_slot_cell, c->d_brick_count,
+                  c->d_slot_of, c->d_cpos, c->d_crel, c->d_ctype, c->d_slot_cell, c->d_brick_count,
                   c->d_brick_start, c->d_brick_fill, c->d_brick_ip, c->d_mm,
                   c->d_frac, c->d_Q, c->d_C, c->d_G, c->d_tw, c->d_kfac, c->d_kvec, c->d_sk,
                   c->d_force, c->d_part, c->d_out, c->d_flags, c->d_de, c->d_de_ip, c->d_de_r,
@@ -785,8 +785,9 @@ void* msg_stream(msg_ctx* h) { return h ? (void*)h->c.stream : nullptr; }
 
 int msg_set_tile(msg_ctx* h, int pair_lanes) {
   if (!h) return set_error(MSG_ERR_ARG, "msg_set_tile: null context");
-  if (pair_lanes != 0 && pair_lanes != 8 && pair_lanes != 16 && pair_lanes != 32 && pair_lanes != 64)
-    return set_error(MSG_ERR_ARG, "msg_set_tile: pair_lanes must be 0, 8, 16, 32 or 64");
+  if (pair_lanes != 0 && pair_lanes != 8 && pair_lanes != 16 && pair_lanes != 32 && pair_lanes != 64 &&
+      pair_lanes != 108 && pair_lanes != 116)
+    return set_error(MSG_ERR_ARG, "msg_set_tile: pair_lanes must be 0, 8, 16, 32, 64, 108 or 116");
   Ctx* c = &h->c;
   MSG_HIP(hipStreamSynchronize(c->stream));
   c->pair_lph = pair_lanes;

@@ -20,7 +20,17 @@ struct CellMoveArgs {
   int* ctype;
   int* slot_of;
   int* slot_cell;
+  float4* crel;   // mixed path: cell-relative FP32 records (null on the FP64 path)
 };
+
+// A position's FP32 record relative to the lower corner of cell c (as the cell sort writes it).
+__device__ inline float4 cell_rel(const CellGrid& g, const Box& box, int c, const double* r, double w) {
+  const int cc[3] = {c % g.nc[0], (c / g.nc[0]) % g.nc[1], c / (g.nc[0] * g.nc[1])};
+  float v[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) v[d] = (float)(r[d] + box.half[d] - cc[d] / g.csi[d]);
+  return make_float4(v[0], v[1], v[2], (float)w);
+}
 
 // Cell of a position as pcellro (Src/celllist.F90:295-305), clamped as the binning clamps.
 __device__ inline int cell_of_pos(const CellGrid& g, const Box& box, const double* r) {
@@ -51,7 +61,10 @@ __device__ int cell_move(const CellMoveArgs& a, int ip, const double* rn) {
   const double4 moved = make_double4(rn[0], rn[1], rn[2], rec.w);
   if (A == B) {
     __syncthreads();
-    if (threadIdx.x == 0) a.cpos[so] = moved;
+    if (threadIdx.x == 0) {
+      a.cpos[so] = moved;
+      if (a.crel) a.crel[so] = cell_rel(a.g, a.box, B, rn, rec.w);
+    }
     __syncthreads();
     return 0;
   }
@@ -64,11 +77,13 @@ __device__ int cell_move(const CellMoveArgs& a, int ip, const double* rn) {
       const bool v = s <= dst;
       int ci = 0, ct = 0, sc = 0;
       double4 p = make_double4(0, 0, 0, 0);
+      float4 pr = make_float4(0, 0, 0, 0);
       if (v) {
         ci = a.cell_ip[s];
         ct = a.ctype[s];
         sc = a.slot_cell[s];
         p = a.cpos[s];
+        if (a.crel) pr = a.crel[s];
       }
       __syncthreads();
       if (v) {
@@ -76,6 +91,7 @@ __device__ int cell_move(const CellMoveArgs& a, int ip, const double* rn) {
         a.ctype[s - 1] = ct;
         a.slot_cell[s - 1] = sc;
         a.cpos[s - 1] = p;
+        if (a.crel) a.crel[s - 1] = pr;
         a.slot_of[ci] = s - 1;
       }
       __syncthreads();
@@ -89,11 +105,13 @@ __device__ int cell_move(const CellMoveArgs& a, int ip, const double* rn) {
       const bool v = s >= dst;
       int ci = 0, ct = 0, sc = 0;
       double4 p = make_double4(0, 0, 0, 0);
+      float4 pr = make_float4(0, 0, 0, 0);
       if (v) {
         ci = a.cell_ip[s];
         ct = a.ctype[s];
         sc = a.slot_cell[s];
         p = a.cpos[s];
+        if (a.crel) pr = a.crel[s];
       }
       __syncthreads();
       if (v) {
@@ -101,6 +119,7 @@ __device__ int cell_move(const CellMoveArgs& a, int ip, const double* rn) {
         a.ctype[s + 1] = ct;
         a.slot_cell[s + 1] = sc;
         a.cpos[s + 1] = p;
+        if (a.crel) a.crel[s + 1] = pr;
         a.slot_of[ci] = s + 1;
       }
       __syncthreads();
@@ -112,6 +131,7 @@ __device__ int cell_move(const CellMoveArgs& a, int ip, const double* rn) {
     a.ctype[dst] = tp;
     a.slot_cell[dst] = B;
     a.cpos[dst] = moved;
+    if (a.crel) a.crel[dst] = cell_rel(a.g, a.box, B, rn, rec.w);
     a.slot_of[ip] = dst;
   }
   __syncthreads();

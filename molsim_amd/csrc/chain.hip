@@ -41,6 +41,7 @@ struct ChainArgs {
   int* slot_of;
   int* slot_cell;
   int* ctype;
+  float4* crel;
   CellGrid g;
   Box box;
   double rcut2;
@@ -149,6 +150,7 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
   cm.ctype = a.ctype;
   cm.slot_of = a.slot_of;
   cm.slot_cell = a.slot_cell;
+  cm.crel = a.crel;
   double dsum = 0.0;
   int nacc = 0, done = 0, ncross = 0;
   const double expmax = 87.0;  // Src/mol.F90:62
@@ -374,6 +376,7 @@ int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, in
   a.slot_of = c->d_slot_of;
   a.slot_cell = c->d_slot_cell;
   a.ctype = c->d_ctype;
+  a.crel = (float4*)c->d_crel;
   a.g = c->grid;
   for (int d = 0; d < 3; ++d) {
     a.box.L[d] = c->P.boxlen[d];

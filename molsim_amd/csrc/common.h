@@ -147,6 +147,8 @@ struct Ctx {
   int* d_cell_ip = nullptr;      // np (sorted: cell-major, descending ip in a cell)
   int* d_slot_of = nullptr;      // np: slot of particle ip in the cell order
   void* d_cpos = nullptr;        // np double4 in cell order: x, y, z, particle type
+  void* d_crel = nullptr;        // mixed path: np float4 in cell order, the position relative to
+                                 // its cell's lower corner and the type (tag) as cpos
   int* d_ctype = nullptr;        // np
   int* d_slot_cell = nullptr;    // np
 
@@ -202,7 +204,8 @@ struct Ctx {
   // home order = cell order at build ("slots"); neighbours are slots, column-major [maxn][np]
   int vl_maxn = 0;
   std::vector<int> h_de_ip;  // staged trial indices (host)
-  int pair_lph = 32;        // pair tile: lanes per home particle (8..64), 0 = wave-item kernel
+  int pair_lph = 32;        // pair tile: lanes per home particle (8..64), 0 = wave-item kernel,
+                            // 100 + lanes (108, 116) = cell-tile kernel (mixed path, cell lists)
   int npart_pair_cap = 0;   // pair partial slots reserved (largest tile)
   int* d_vl_home = nullptr;
   int* d_vl_n = nullptr;
@@ -267,6 +270,8 @@ int launch_finalize(Ctx* c);                       // cell order (after launch_s
 int launch_pair(Ctx* c, bool last);                // K4
 int pair_blocks(Ctx* c);                           // workgroups of the pair kernel
 bool pair_nbpar(const Ctx* c);                     // neighbour-parallel kernel in use
+bool pair_tiled(const Ctx* c);                     // cell-tile kernel in use
+int pair_tile_tx(const Ctx* c);                    // its cells per tile along x
 int default_pair_tile(const Ctx* c);               // lanes per home for a new context
 int spme_setup(Ctx* c);                            // K-space arrays
 int launch_spme_mesh(Ctx* c);                      // spread, FFTs + influence function

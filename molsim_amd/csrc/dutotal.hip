@@ -799,8 +799,11 @@ __global__ __launch_bounds__(1024) void k_commit(CommitArgs a) {
 }
 
 // The cell-ordered record of a particle whose charge state flipped: its type tag.
-__global__ void k_set_tag(int ip, const int* slot_of, double4* cpos, double tag) {
-  if (threadIdx.x == 0) cpos[slot_of[ip]].w = tag;
+__global__ void k_set_tag(int ip, const int* slot_of, double4* cpos, float4* crel, double tag) {
+  if (threadIdx.x == 0) {
+    cpos[slot_of[ip]].w = tag;
+    if (crel) crel[slot_of[ip]].w = (float)tag;
+  }
 }
 
 }  // namespace
@@ -819,6 +822,7 @@ CellMoveArgs cell_move_args(Ctx* c) {
   m.ctype = c->d_ctype;
   m.slot_of = c->d_slot_of;
   m.slot_cell = c->d_slot_cell;
+  m.crel = (float4*)c->d_crel;
   return m;
 }
 
@@ -849,7 +853,8 @@ int commit_move(Ctx* c, int ip, const double* rnew) {
 int set_charge_tag(Ctx* c, int ip) {
   if (!c->cells_valid) return MSG_OK;  // the next pass ranks the cells afresh
   const double tag = (double)(c->h_ipt[ip] + (c->h_laz[ip] ? 0 : kUncharged));
-  hipLaunchKernelGGL(k_set_tag, dim3(1), dim3(64), 0, c->stream, ip, c->d_slot_of, (double4*)c->d_cpos, tag);
+  hipLaunchKernelGGL(k_set_tag, dim3(1), dim3(64), 0, c->stream, ip, c->d_slot_of, (double4*)c->d_cpos,
+                     (float4*)c->d_crel, tag);
   MSG_HIP(hipGetLastError());
   return MSG_OK;
 }

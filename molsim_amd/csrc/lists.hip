@@ -386,6 +386,9 @@ struct FinalizeArgs {
   int* slot_cell;
   int* item_cell;
   const int* laz;  // weak charges (null when off): an uncharged particle's record type gets + 8
+  float4* crel;    // mixed path: cell-relative FP32 records (null on the FP64 path)
+  CellGrid g;
+  double half[3];  // L / 2
 };
 
 // One wavefront per cell: rank members by particle index (descending, the reference's
@@ -416,6 +419,13 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& a, int bid) {
       a.slot_of[ip] = dst;
       const int tw = a.ipt[ip] + ((a.laz && !a.laz[ip]) ? kUncharged : 0);
       a.cpos[dst] = make_double4(a.ro[3 * ip], a.ro[3 * ip + 1], a.ro[3 * ip + 2], (double)tw);
+      if (a.crel) {
+        const int cc[3] = {wv % a.g.nc[0], (wv / a.g.nc[0]) % a.g.nc[1], wv / (a.g.nc[0] * a.g.nc[1])};
+        float r[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) r[d] = (float)(a.ro[3 * ip + d] + a.half[d] - cc[d] / a.g.csi[d]);
+        a.crel[dst] = make_float4(r[0], r[1], r[2], (float)tw);
+      }
       a.ctype[dst] = a.ipt[ip];
       a.slot_cell[dst] = wv;
     }
@@ -510,6 +520,8 @@ int lists_setup(Ctx* c) {
   if (int e = A((void**)&c->d_cell_ip, np * sizeof(int))) return e;
   if (int e = A((void**)&c->d_slot_of, np * sizeof(int))) return e;
   if (int e = A(&c->d_cpos, np * sizeof(double4))) return e;
+  if (!c->fp64)
+    if (int e = A(&c->d_crel, np * sizeof(float4))) return e;
   if (int e = A((void**)&c->d_ctype, np * sizeof(int))) return e;
   if (int e = A((void**)&c->d_slot_cell, np * sizeof(int))) return e;
   if (int e = A((void**)&c->d_rank, 2 * (size_t)np * sizeof(int))) return e;
@@ -606,6 +618,9 @@ static FinalizeArgs finalize_args(Ctx* c) {
   f.cell_ip = c->d_cell_ip;
   f.slot_of = c->d_slot_of;
   f.cpos = (double4*)c->d_cpos;
+  f.crel = (float4*)c->d_crel;
+  f.g = c->grid;
+  for (int d = 0; d < 3; ++d) f.half[d] = 0.5 * c->P.boxlen[d];
   f.ctype = c->d_ctype;
   f.slot_cell = c->d_slot_cell;
   f.item_cell = c->d_item_cell;

@@ -21,6 +21,9 @@ namespace {
 struct PairArgs {
   int np;
   const double4* cpos;
+  const float4* crel;   // mixed path: FP32 cell-relative records (the neighbour-parallel kernel reads these)
+  int xcd;              // single-system launch: workgroups renumbered so each XCD takes a contiguous run
+  int tile_tx;          // cell-tile kernel: cells per tile along x
   const int* ctype;
   const int* cell_ip;
   const int* cell_start;
@@ -542,15 +545,573 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
   }
 }
 
+// Workgroup b of a single-system launch of n workgroups runs on XCD b mod 8 (round-robin
+// dispatch); renumbered so that XCD x takes the x-th contiguous run of the logical workgroups,
+// consecutive homes (spatially close in cell order) share an L2 and their neighbour cells'
+// records are fetched into one L2 instead of eight.
+__device__ inline int xcd_logical(int b, int n) {
+  const int x = b % kXcd, k = b / kXcd, q = n / kXcd, r = n % kXcd;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
+// Mixed-path neighbour-parallel pair kernel over FP32 cell-relative records: a home's
+// displacement to a candidate of stencil cell s is (rel_home - s * cellsize) - rel_cand, which
+// is the minimum image for every pair within the cut-off when a dimension has at least three
+// cells (cells >= rcut); a dimension collapsed to one cell takes PBCr2's single shift in FP32.
+// The candidate records are 16 bytes (position and type), and no FP64 arithmetic is spent per
+// candidate.  Otherwise as pair_nb_body.
+template <int NTP, int LPH>
+__device__ __forceinline__ void pair_nb_rel_body(const PairArgs& a, int bid) {
+  constexpr int HPB = 256 / LPH;
+  constexpr int MPL = (kMaxStencil + LPH - 1) / LPH;
+  extern __shared__ __align__(16) double s_dyn[];
+  __shared__ PairMeta<float> s_pm[kMaxT * kMaxT];
+  __shared__ int2 s_pi[kMaxT * kMaxT];
+  __shared__ int s_tp[kMaxT * kMaxT];
+  __shared__ double s_red[4][kMaxTP + 1];
+  __shared__ int s_cst[HPB][kMaxStencil + 1], s_cpf[HPB][kMaxStencil + 1];
+  const int nint = a.tm.nint;
+  float* s_coef = (float*)s_dyn;
+  int* s_bins = (int*)(s_coef + nint * 12);
+  {
+    lds_copy16(s_coef, a.tcoef, nint * 12 * (int)sizeof(float));
+    lds_copy16(s_bins, a.tbins, NTP * kTabBins * (int)sizeof(int));
+    fill_pair_meta<float>(a.tm, s_pm, s_pi, s_tp);
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, grp = threadIdx.x / LPH;
+  const int s = bid * HPB + grp;
+  const int k = threadIdx.x & (LPH - 1);
+  const bool home = s < a.np;
+  const int hs = home ? s : a.np - 1;
+  const float4 hp = a.crel[hs];
+  const int hw = (int)hp.w;
+  const int ht = hw & (kUncharged - 1);
+  const bool hcharged = hw < kUncharged;
+  const int c = a.slot_cell[hs];
+  const int cx = c % a.g.nc[0], cy = (c / a.g.nc[0]) % a.g.nc[1], cz = c / (a.g.nc[0] * a.g.nc[1]);
+  float csz[3], hr[3] = {hp.x, hp.y, hp.z};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) csz[d] = (float)(1.0 / a.g.csi[d]);
+  // stencil cells whose nearest point is beyond the cut-off (with a margin) are skipped
+  const float cull2 = (float)(a.rcut2 * (1.0 + 1e-4)) + 1e-3f;
+  int st0[MPL], cnt[MPL], tot = 0;
+#pragma unroll
+  for (int m = 0; m < MPL; ++m) {
+    const int sc = k * MPL + m;
+    cnt[m] = 0;
+    st0[m] = 0;
+    float bd2 = 0.f;
+    if (sc < a.g.nst) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int o = a.g.st[3 * sc + d];
+        const float dist = o < 0 ? hr[d] : (o > 0 ? csz[d] - hr[d] : 0.f);
+        bd2 += dist > 0.f ? dist * dist : 0.f;
+      }
+    }
+    if (sc < a.g.nst && bd2 <= cull2) {
+      int x = cx + a.g.st[3 * sc], y = cy + a.g.st[3 * sc + 1], z = cz + a.g.st[3 * sc + 2];
+      x += x < 0 ? a.g.nc[0] : (x >= a.g.nc[0] ? -a.g.nc[0] : 0);
+      y += y < 0 ? a.g.nc[1] : (y >= a.g.nc[1] ? -a.g.nc[1] : 0);
+      z += z < 0 ? a.g.nc[2] : (z >= a.g.nc[2] ? -a.g.nc[2] : 0);
+      const int nb = x + a.g.nc[0] * (y + a.g.nc[1] * z);
+      st0[m] = a.cell_start[nb];
+      cnt[m] = a.cell_start[nb + 1] - st0[m];
+    }
+    tot += cnt[m];
+  }
+  int incl = tot;
+#pragma unroll
+  for (int off = 1; off < LPH; off <<= 1) {
+    const int t = __shfl_up(incl, off, LPH);
+    if (k >= off) incl += t;
+  }
+  const int ncand = __shfl(incl, LPH - 1, LPH);
+  {
+    int pre = incl - tot;
+#pragma unroll
+    for (int m = 0; m < MPL; ++m) {
+      const int sc = k * MPL + m;
+      if (sc < a.g.nst) {
+        s_cst[grp][sc] = st0[m];
+        s_cpf[grp][sc] = pre;
+      }
+      pre += cnt[m];
+    }
+  }
+  if (k == 0) s_cpf[grp][a.g.nst] = ncand;
+  __syncthreads();
+
+  // collapsed dimensions (one cell): PBCr2's single shift in FP32
+  const bool col = a.g.nc[0] == 1 || a.g.nc[1] == 1 || a.g.nc[2] == 1;
+  const float Lf[3] = {(float)a.box.L[0], (float)a.box.L[1], (float)a.box.L[2]};
+  const float Hf[3] = {(float)a.box.half[0], (float)a.box.half[1], (float)a.box.half[2]};
+  const float rc2 = (float)a.rcut2;
+  PairAcc<float, NTP> acc;
+  acc.zero();
+  const int share = (ncand + LPH - 1) / LPH;
+  int i = k * share;
+  const int i1 = (home && hcharged) ? min(ncand, i + share) : 0;
+  if (i < i1) {
+    int cc = 0;
+    while (s_cpf[grp][cc + 1] <= i) ++cc;
+    int j = s_cst[grp][cc] + (i - s_cpf[grp][cc]);
+    int cend = s_cst[grp][cc] + (s_cpf[grp][cc + 1] - s_cpf[grp][cc]);
+    // the home's position shifted by the current stencil cell's offset
+    float sx = hr[0] - a.g.st[3 * cc] * csz[0], sy = hr[1] - a.g.st[3 * cc + 1] * csz[1],
+          sz = hr[2] - a.g.st[3 * cc + 2] * csz[2];
+    constexpr int U = 4;  // candidates in flight per lane
+    while (i < i1) {
+      int jv[U];
+      bool hv[U];
+      float3 hsv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        hv[u] = i < i1;
+        jv[u] = hv[u] ? j : s;
+        hsv[u] = make_float3(sx, sy, sz);
+        if (hv[u]) {
+          ++i;
+          ++j;
+          if (j == cend && i < i1) {
+            do { ++cc; } while (s_cpf[grp][cc + 1] <= i);
+            j = s_cst[grp][cc];
+            cend = j + (s_cpf[grp][cc + 1] - s_cpf[grp][cc]);
+            sx = hr[0] - a.g.st[3 * cc] * csz[0];
+            sy = hr[1] - a.g.st[3 * cc + 1] * csz[1];
+            sz = hr[2] - a.g.st[3 * cc + 2] * csz[2];
+          }
+        }
+      }
+      float4 pv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) pv[u] = a.crel[jv[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (!hv[u] || jv[u] == s) continue;
+        float dx = hsv[u].x - pv[u].x, dy = hsv[u].y - pv[u].y, dz = hsv[u].z - pv[u].z;
+        if (col) {
+          if (fabsf(dx) > Hf[0]) dx -= copysignf(Lf[0], dx);
+          if (fabsf(dy) > Hf[1]) dy -= copysignf(Lf[1], dy);
+          if (fabsf(dz) > Hf[2]) dz -= copysignf(Lf[2], dz);
+        }
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (!(r2 < rc2)) continue;
+        const int tj = (int)pv[u].w;
+        if (tj >= kUncharged) continue;  // weak charges: uncharged partner
+        const int mi = ht * kMaxT + tj;
+        const PairMeta<float> pm = s_pm[mi];
+        const int2 pi = s_pi[mi];
+        float us, fs;
+        if (r2 < pm.r2at) {
+          us = 1e10f;
+          fs = 1e10f;
+        } else if (r2 < pm.r2um) {
+          acc.err |= FLAG_R2UMIN;
+          continue;
+        } else {
+          int b = (int)((r2 - pm.zlo) * pm.invw);
+          b = min(max(b, 0), kTabBins - 1);
+          int ti = s_bins[pi.x * kTabBins + b];
+          alignas(16) float t[12];
+          {
+            const float4* rp = (const float4*)(s_coef + 12 * ti);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) *(float4*)(t + 4 * q) = rp[q];
+          }
+          while (ti < pi.y && r2 < t[0]) {
+            ++ti;
+            const float4* rp = (const float4*)(s_coef + 12 * ti);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) *(float4*)(t + 4 * q) = rp[q];
+          }
+          const float dd = r2 - t[0];
+          us = t[1] + dd * (t[2] + dd * (t[3] + dd * (t[4] + dd * (t[5] + dd * t[6]))));
+          fs = t[7] + dd * (t[8] + dd * (t[9] + dd * (t[10] + dd * t[11])));
+        }
+#pragma unroll
+        for (int q = 0; q < PairAcc<float, NTP>::NT; ++q) acc.et[q] += (q == tj) ? (double)us : 0.0;
+        acc.fx += (double)(fs * dx);
+        acc.fy += (double)(fs * dy);
+        acc.fz += (double)(fs * dz);
+        acc.vir -= (double)(fs * r2);
+      }
+    }
+  }
+#pragma unroll
+  for (int off = LPH / 2; off > 0; off >>= 1) {
+    acc.fx += __shfl_xor(acc.fx, off, 64);
+    acc.fy += __shfl_xor(acc.fy, off, 64);
+    acc.fz += __shfl_xor(acc.fz, off, 64);
+  }
+  if (home && k == 0) {
+    const int ip = a.cell_ip[s];
+    a.force[3 * ip + 0] = acc.fx * a.scale;
+    a.force[3 * ip + 1] = acc.fy * a.scale;
+    a.force[3 * ip + 2] = acc.fz * a.scale;
+  }
+  if (acc.err) atomicOr(a.flags, acc.err);
+#pragma unroll
+  for (int q = 0; q < NTP; ++q) {
+    const double v = msgd::wave_sum(acc.to_tp(q, ht, s_tp));
+    if (lane == 0) s_red[wv][q] = v;
+  }
+  {
+    const double v = msgd::wave_sum(acc.vir);
+    if (lane == 0) s_red[wv][NTP] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x <= NTP) {
+    double t = 0.0;
+    for (int w = 0; w < 4; ++w) t += s_red[w][threadIdx.x];
+    a.part[bid * (a.tm.ntp + 1) + (threadIdx.x == NTP ? a.tm.ntp : threadIdx.x)] = t;
+  }
+}
+
+// Cell-tile pair kernel (mixed path): a workgroup takes a tile of up to tile_tx consecutive
+// cells along x of one (y, z) cell row.  The stencil cells of all its homes are the cells of
+// the 3 x 3 neighbouring rows over the tile's x range widened by one cell on each side; in
+// cell order each such row segment is a run of consecutive cells, so the workgroup stages
+// their records into LDS with coalesced loads, converted to coordinates relative to the tile's
+// first cell (the cell-relative FP32 position plus the row's cell offset), and every candidate
+// is then read from LDS.  The displacement of a home and a candidate is the difference of
+// their tile coordinates: the minimum image for every pair within the cut-off when each
+// dimension has at least three cells; a dimension collapsed to one cell takes PBCr2's single
+// shift in FP32.  LPH lanes per home walk its 27 stencil cells (cells beyond the cut-off from
+// the home skipped) as contiguous LDS ranges.  A tile whose neighbourhood holds more records
+// than the LDS area (very dense cells) reads its candidates from global memory instead.
+constexpr int kTileRec = 2048;   // staged records (32 KB)
+constexpr int kTileCells = 128;  // cells of a tile's neighbourhood (9 rows of tile_tx + 2 cells)
+
+template <int NTP, int LPH>
+__device__ __forceinline__ void pair_tile_body(const PairArgs& a, int bid) {
+  constexpr int HPB = 256 / LPH;
+  constexpr int MPL = (kMaxStencil + LPH - 1) / LPH;
+  extern __shared__ __align__(16) double s_dyn[];
+  __shared__ PairMeta<float> s_pm[kMaxT * kMaxT];
+  __shared__ int2 s_pi[kMaxT * kMaxT];
+  __shared__ int s_tp[kMaxT * kMaxT];
+  __shared__ double s_red[4][kMaxTP + 1];
+  __shared__ int s_cst[HPB][kMaxStencil + 1], s_cpf[HPB][kMaxStencil + 1];
+  __shared__ int s_cell[kTileCells], s_coff[kTileCells + 1], s_cbeg[kTileCells];
+  __shared__ float4 s_rec[kTileRec];
+  static_assert(HPB * kMaxStencil <= kTileRec, "stencil offsets share the staging area");
+  float4 (*s_csh)[kMaxStencil] = (float4 (*)[kMaxStencil])s_rec;  // global mode: the stencil cells' offsets
+  __shared__ int s_wsum[4];
+  const int nint = a.tm.nint;
+  float* s_coef = (float*)s_dyn;
+  int* s_bins = (int*)(s_coef + nint * 12);
+  {
+    lds_copy16(s_coef, a.tcoef, nint * 12 * (int)sizeof(float));
+    lds_copy16(s_bins, a.tbins, NTP * kTabBins * (int)sizeof(int));
+    fill_pair_meta<float>(a.tm, s_pm, s_pi, s_tp);
+  }
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nx = a.g.nc[0], ny = a.g.nc[1], nz = a.g.nc[2];
+  const int ntx = (nx + a.tile_tx - 1) / a.tile_tx;
+  const int tx_i = bid % ntx, cy = (bid / ntx) % ny, cz = bid / (ntx * ny);
+  const int x0 = tx_i * a.tile_tx, tx = min(a.tile_tx, nx - x0);
+  const int rx = nx > 1 ? 1 : 0, ry = ny > 1 ? 1 : 0, rz = nz > 1 ? 1 : 0;
+  const int wseg = tx + 2 * rx;                 // cells per row segment
+  const int nrow = (2 * ry + 1) * (2 * rz + 1);
+  const int ncells = nrow * wseg;               // <= kTileCells (tile_tx <= 12)
+  float csz[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) csz[d] = (float)(1.0 / a.g.csi[d]);
+  // the neighbourhood's cells: row r = (dy, dz), segment position q (unwrapped x = x0 - rx + q)
+  int cnt = 0;
+  if (tid < ncells) {
+    const int r = tid / wseg, q = tid - r * wseg;
+    const int dy = r % (2 * ry + 1) - ry, dz = r / (2 * ry + 1) - rz;
+    int x = x0 - rx + q, y = cy + dy, z = cz + dz;
+    x += x < 0 ? nx : (x >= nx ? -nx : 0);
+    y += y < 0 ? ny : (y >= ny ? -ny : 0);
+    z += z < 0 ? nz : (z >= nz ? -nz : 0);
+    const int c = x + nx * (y + ny * z);
+    s_cell[tid] = c;
+    s_cbeg[tid] = a.cell_start[c];
+    cnt = a.cell_start[c + 1] - s_cbeg[tid];
+  }
+  // offsets of the cells' records in the staged area (ncells <= 128: two wavefronts)
+  {
+    int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    if (lane == 63) s_wsum[wv] = incl;
+    __syncthreads();
+    const int base = wv == 1 ? s_wsum[0] : 0;
+    if (tid < ncells) s_coff[tid + 1] = base + incl;
+    if (tid == 0) s_coff[0] = 0;
+  }
+  __syncthreads();
+  const int total = s_coff[ncells];
+  const bool staged = total <= kTileRec;
+  if (staged) {
+    // cooperative copy: record k belongs to the cell whose offsets bracket it
+    for (int k = tid; k < total; k += 256) {
+      int lo = 0, hi = ncells - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_coff[mid] <= k) lo = mid;
+        else hi = mid - 1;
+      }
+      const int r = lo / wseg, q = lo - r * wseg;
+      const int dy = r % (2 * ry + 1) - ry, dz = r / (2 * ry + 1) - rz;
+      const float4 v = a.crel[s_cbeg[lo] + (k - s_coff[lo])];
+      s_rec[k] = make_float4(v.x + (q - rx) * csz[0], v.y + dy * csz[1], v.z + dz * csz[2], v.w);
+    }
+  }
+  __syncthreads();
+  // homes: the tile's cells, row (0, 0) at segment positions rx .. rx + tx - 1
+  const int r0 = ry + (2 * ry + 1) * rz;
+  const int hc0 = r0 * wseg + rx;
+  const int hslot0 = s_cbeg[hc0];
+  const int nh = s_coff[hc0 + tx] - s_coff[hc0];
+  const bool col = nx == 1 || ny == 1 || nz == 1;
+  const float Lf[3] = {(float)a.box.L[0], (float)a.box.L[1], (float)a.box.L[2]};
+  const float Hf[3] = {(float)a.box.half[0], (float)a.box.half[1], (float)a.box.half[2]};
+  const float rc2 = (float)a.rcut2;
+  const float cull2 = (float)(a.rcut2 * (1.0 + 1e-4)) + 1e-3f;
+  const int grp = tid / LPH, k = tid & (LPH - 1);
+  constexpr int NT = PairAcc<float, NTP>::NT;  // partner types
+  double et[NT], etp[NTP];                      // a home's energies by partner type; this lane's by type pair
+#pragma unroll
+  for (int q = 0; q < NT; ++q) et[q] = 0.0;
+#pragma unroll
+  for (int q = 0; q < NTP; ++q) etp[q] = 0.0;
+  double vacc = 0.0;
+  int errs = 0;
+  for (int h0 = 0; h0 < nh; h0 += HPB) {
+    const int h = h0 + grp;
+    const bool home = h < nh;
+    const int hh = home ? h : nh - 1;
+    // the home's cell: segment position hq (row 0), its record and tile coordinates
+    int hq = 0;
+    while (hq + 1 < tx && s_coff[hc0 + hq + 1] - s_coff[hc0] <= hh) ++hq;
+    const int hidx = s_coff[hc0] + hh;            // index in the staged area
+    float4 hp;
+    if (staged) {
+      hp = s_rec[hidx];
+    } else {
+      hp = a.crel[hslot0 + hh];                   // relative to its own cell
+    }
+    const int hw = (int)hp.w;
+    const int ht = hw & (kUncharged - 1);
+    const bool hcharged = hw < kUncharged;
+    // the home's cell box in the same coordinates (for the stencil-cell distance bound)
+    const float bx0 = staged ? hq * csz[0] : 0.f;
+    const float hr[3] = {hp.x - bx0, hp.y, hp.z};  // position within the home cell
+    int st0[MPL], cn[MPL], tot = 0;
+#pragma unroll
+    for (int m = 0; m < MPL; ++m) {
+      const int sc = k * MPL + m;
+      st0[m] = 0;
+      cn[m] = 0;
+      if (sc < a.g.nst) {
+        const int ox = a.g.st[3 * sc], oy = a.g.st[3 * sc + 1], oz = a.g.st[3 * sc + 2];
+        const int o3[3] = {ox, oy, oz};
+        float bd2 = 0.f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const float dist = o3[d] < 0 ? hr[d] : (o3[d] > 0 ? csz[d] - hr[d] : 0.f);
+          bd2 += dist > 0.f ? dist * dist : 0.f;
+        }
+        if (bd2 <= cull2) {
+          const int cidx = ((oy + ry) + (2 * ry + 1) * (oz + rz)) * wseg + (hq + rx + ox);
+          cn[m] = s_coff[cidx + 1] - s_coff[cidx];
+          if (staged) {
+            st0[m] = s_coff[cidx];
+          } else {
+            st0[m] = s_cbeg[cidx];
+            s_csh[grp][sc] = make_float4(ox * csz[0], oy * csz[1], oz * csz[2], 0.f);
+          }
+        }
+      }
+      tot += cn[m];
+    }
+    int incl = tot;
+#pragma unroll
+    for (int off = 1; off < LPH; off <<= 1) {
+      const int t = __shfl_up(incl, off, LPH);
+      if (k >= off) incl += t;
+    }
+    const int ncand = __shfl(incl, LPH - 1, LPH);
+    {
+      int pre = incl - tot;
+#pragma unroll
+      for (int m = 0; m < MPL; ++m) {
+        const int sc = k * MPL + m;
+        if (sc < a.g.nst) {
+          s_cst[grp][sc] = st0[m];
+          s_cpf[grp][sc] = pre;
+        }
+        pre += cn[m];
+      }
+    }
+    if (k == 0) s_cpf[grp][a.g.nst] = ncand;
+    __syncthreads();
+    const int self = staged ? hidx : hslot0 + hh;
+    const int share = (ncand + LPH - 1) / LPH;
+    int i = k * share;
+    const int i1 = (home && hcharged) ? min(ncand, i + share) : 0;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (i < i1) {
+      int cc = 0;
+      while (s_cpf[grp][cc + 1] <= i) ++cc;
+      int j = s_cst[grp][cc] + (i - s_cpf[grp][cc]);
+      int cend = s_cst[grp][cc] + (s_cpf[grp][cc + 1] - s_cpf[grp][cc]);
+      float4 sh = staged ? make_float4(0.f, 0.f, 0.f, 0.f) : s_csh[grp][cc];
+      constexpr int U = 4;
+      while (i < i1) {
+        int jv[U];
+        bool hv[U];
+        float4 shv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          hv[u] = i < i1;
+          jv[u] = hv[u] ? j : self;
+          shv[u] = sh;
+          if (hv[u]) {
+            ++i;
+            ++j;
+            if (j == cend && i < i1) {
+              do { ++cc; } while (s_cpf[grp][cc + 1] <= i);
+              j = s_cst[grp][cc];
+              cend = j + (s_cpf[grp][cc + 1] - s_cpf[grp][cc]);
+              if (!staged) sh = s_csh[grp][cc];
+            }
+          }
+        }
+        float4 pv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) pv[u] = staged ? s_rec[jv[u]] : a.crel[jv[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (!hv[u] || jv[u] == self) continue;
+          float dx = hp.x - (pv[u].x + shv[u].x), dy = hp.y - (pv[u].y + shv[u].y),
+                dz = hp.z - (pv[u].z + shv[u].z);
+          if (col) {
+            if (fabsf(dx) > Hf[0]) dx -= copysignf(Lf[0], dx);
+            if (fabsf(dy) > Hf[1]) dy -= copysignf(Lf[1], dy);
+            if (fabsf(dz) > Hf[2]) dz -= copysignf(Lf[2], dz);
+          }
+          const float r2 = dx * dx + dy * dy + dz * dz;
+          if (!(r2 < rc2)) continue;
+          const int tj = (int)pv[u].w;
+          if (tj >= kUncharged) continue;  // weak charges: uncharged partner
+          const int mi = ht * kMaxT + tj;
+          const PairMeta<float> pm = s_pm[mi];
+          const int2 pi = s_pi[mi];
+          float us, fs;
+          if (r2 < pm.r2at) {
+            us = 1e10f;
+            fs = 1e10f;
+          } else if (r2 < pm.r2um) {
+            errs |= FLAG_R2UMIN;
+            continue;
+          } else {
+            int b = (int)((r2 - pm.zlo) * pm.invw);
+            b = min(max(b, 0), kTabBins - 1);
+            int ti = s_bins[pi.x * kTabBins + b];
+            alignas(16) float t[12];
+            {
+              const float4* rp = (const float4*)(s_coef + 12 * ti);
+#pragma unroll
+              for (int q = 0; q < 3; ++q) *(float4*)(t + 4 * q) = rp[q];
+            }
+            while (ti < pi.y && r2 < t[0]) {
+              ++ti;
+              const float4* rp = (const float4*)(s_coef + 12 * ti);
+#pragma unroll
+              for (int q = 0; q < 3; ++q) *(float4*)(t + 4 * q) = rp[q];
+            }
+            const float dd = r2 - t[0];
+            us = t[1] + dd * (t[2] + dd * (t[3] + dd * (t[4] + dd * (t[5] + dd * t[6]))));
+            fs = t[7] + dd * (t[8] + dd * (t[9] + dd * (t[10] + dd * t[11])));
+          }
+#pragma unroll
+          for (int q = 0; q < NT; ++q) et[q] += (q == tj) ? (double)us : 0.0;
+          fx += (double)(fs * dx);
+          fy += (double)(fs * dy);
+          fz += (double)(fs * dz);
+          vacc -= (double)(fs * r2);
+        }
+      }
+    }
+#pragma unroll
+    for (int off = LPH / 2; off > 0; off >>= 1) {
+      fx += __shfl_xor(fx, off, 64);
+      fy += __shfl_xor(fy, off, 64);
+      fz += __shfl_xor(fz, off, 64);
+    }
+    if (home && k == 0) {
+      const int ip = a.cell_ip[hslot0 + h];
+      a.force[3 * ip + 0] = fx * a.scale;
+      a.force[3 * ip + 1] = fy * a.scale;
+      a.force[3 * ip + 2] = fz * a.scale;
+    }
+    // this home's energies by partner type into type-pair slots (the home type fixes the pair)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const double v = et[t];
+      et[t] = 0.0;
+#pragma unroll
+      for (int q = 0; q < NTP; ++q) etp[q] += (s_tp[ht * kMaxT + t] == q) ? v : 0.0;
+    }
+    __syncthreads();  // s_cst / s_cpf / s_csh are rewritten for the next homes
+  }
+  if (errs) atomicOr(a.flags, errs);
+#pragma unroll
+  for (int q = 0; q < NTP; ++q) {
+    const double v = msgd::wave_sum(etp[q]);
+    if (lane == 0) s_red[wv][q] = v;
+  }
+  {
+    const double v = msgd::wave_sum(vacc);
+    if (lane == 0) s_red[wv][NTP] = v;
+  }
+  __syncthreads();
+  if (tid <= NTP) {
+    double t = 0.0;
+    for (int w = 0; w < 4; ++w) t += s_red[w][tid];
+    a.part[bid * (a.tm.ntp + 1) + (tid == NTP ? a.tm.ntp : tid)] = t;
+  }
+}
+
+template <int NTP, int LPH>
+__global__ __launch_bounds__(256) void k_pair_tile(PairArgs a) {
+  pair_tile_body<NTP, LPH>(a, a.xcd ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x);
+}
+
+template <int NTP, int LPH>
+__global__ __launch_bounds__(256) void k_pair_tile_seg(const PairArgs* __restrict__ args,
+                                                       const int2* __restrict__ map) {
+  const int2 s = map[blockIdx.x];
+  if (s.x < 0) return;  // no work for this XCD slot
+  pair_tile_body<NTP, LPH>(args[s.x], s.y);
+}
+
 template <typename real, int NTP, int LPH>
-__global__ __launch_bounds__(256) void k_pair_nb(PairArgs a) { pair_nb_body<real, NTP, LPH>(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_pair_nb(PairArgs a) {
+  const int bid = a.xcd ? xcd_logical(blockIdx.x, gridDim.x) : blockIdx.x;
+  if constexpr (sizeof(real) == 4)
+    pair_nb_rel_body<NTP, LPH>(a, bid);
+  else
+    pair_nb_body<real, NTP, LPH>(a, bid);
+}
 
 template <typename real, int NTP, int LPH>
 __global__ __launch_bounds__(256) void k_pair_nb_seg(const PairArgs* __restrict__ args,
                                                      const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
   if (s.x < 0) return;  // no work for this XCD slot
-  pair_nb_body<real, NTP, LPH>(args[s.x], s.y);
+  if constexpr (sizeof(real) == 4)
+    pair_nb_rel_body<NTP, LPH>(args[s.x], s.y);
+  else
+    pair_nb_body<real, NTP, LPH>(args[s.x], s.y);
 }
 
 // Verlet list build (SetVListMC semantics, Src/nlist.F90:820-884, as a full list that also
@@ -873,7 +1434,29 @@ __global__ __launch_bounds__(256) void k_vlist_export(VlExportArgs a) {
 // The neighbour-parallel kernel serves every cell-list and all-pairs system (any stencil:
 // 27 cells, or fewer where dimensions collapse to one cell) unless tile 0 is chosen.
 bool pair_nbpar(const Ctx* c) {
-  return c->P.list_mode != MSG_LIST_VERLET && c->pair_lph > 0;
+  return c->P.list_mode != MSG_LIST_VERLET && c->pair_lph > 0 && !pair_tiled(c);
+}
+
+// Cell-tile kernel (tile codes 100 + lanes per home): mixed path, cell lists.
+bool pair_tiled(const Ctx* c) {
+  return c->P.list_mode == MSG_LIST_CELL && !c->fp64 && c->pair_lph >= 100;
+}
+
+// Lanes per home of the neighbour-parallel kernel (a cell-tile code on the FP64 path runs it
+// with the same lanes).
+static int nbpar_lanes(const Ctx* c) { return c->pair_lph >= 100 ? c->pair_lph - 100 : c->pair_lph; }
+
+// Cells per tile along x: about 16 homes per tile at the mean density, at most 12 cells (the
+// neighbourhood's 9 rows of tile + 2 cells fit the kernel's cell table), and few enough tiles
+// for the pair partial rows reserved.
+int pair_tile_tx(const Ctx* c) {
+  const int nx = c->grid.nc[0];
+  const double ppc = (double)c->np / std::max(1, c->grid.ncell);
+  int tx = (int)std::lround(16.0 / std::max(ppc, 1e-9));
+  tx = std::max(1, std::min(std::min(tx, 12), nx));
+  const long rows = (long)c->grid.nc[1] * c->grid.nc[2];
+  while (tx < std::min(12, nx) && rows * ((nx + tx - 1) / tx) > c->npart_pair_cap) ++tx;
+  return tx;
 }
 
 // Default tile: about 16 candidates per lane, i.e. lanes per home = expected candidates
@@ -890,7 +1473,14 @@ int default_pair_tile(const Ctx* c) {
 
 int pair_blocks(Ctx* c) {
   if (c->P.list_mode == MSG_LIST_VERLET) return (c->np + 255) / 256;
-  if (pair_nbpar(c)) return (c->np + 256 / c->pair_lph - 1) / (256 / c->pair_lph);
+  if (pair_tiled(c)) {
+    const int tx = pair_tile_tx(c);
+    return c->grid.nc[1] * c->grid.nc[2] * ((c->grid.nc[0] + tx - 1) / tx);
+  }
+  if (pair_nbpar(c) || c->pair_lph >= 100) {
+    const int l = nbpar_lanes(c);
+    return (c->np + 256 / l - 1) / (256 / l);
+  }
   const long items = (long)c->grid.ncell + c->np / kPairHB + 1;
   return (int)std::max(1L, std::min((items + kWaves - 1) / kWaves, 2048L));
 }
@@ -899,6 +1489,8 @@ static PairArgs pair_args(Ctx* c, bool last) {
   PairArgs a;
   a.np = c->np;
   a.cpos = (const double4*)c->d_cpos;
+  a.crel = (const float4*)c->d_crel;
+  a.xcd = 1;
   a.ctype = c->d_ctype;
   a.cell_ip = c->d_cell_ip;
   a.cell_start = c->d_cell_start;
@@ -921,6 +1513,7 @@ static PairArgs pair_args(Ctx* c, bool last) {
   a.flags = c->d_flags;
   a.scale = last ? c->P.force_scale : 1.0;
   a.nblk = c->npart_pair;
+  a.tile_tx = pair_tiled(c) ? pair_tile_tx(c) : 0;
   return a;
 }
 
@@ -957,15 +1550,32 @@ int launch_pair(Ctx* c, bool last) {
     MSG_HIP(hipGetLastError());
     return MSG_OK;
   }
+  const int lph = nbpar_lanes(c);
+  if (pair_tiled(c)) {
+#define MSG_TILE(NTP)                                                                             \
+  if (lph == 8) hipLaunchKernelGGL((k_pair_tile<NTP, 8>), dim3(nb), dim3(256), lds, c->stream, a); \
+  else hipLaunchKernelGGL((k_pair_tile<NTP, 16>), dim3(nb), dim3(256), lds, c->stream, a);
+    switch (c->nptpt) {
+      case 1: MSG_TILE(1) break;
+      case 3: MSG_TILE(3) break;
+      case 6: MSG_TILE(6) break;
+      default: MSG_TILE(10) break;
+    }
+#undef MSG_TILE
+    prof_end(c, MSG_K_PAIR);
+    MSG_HIP(hipGetLastError());
+    return MSG_OK;
+  }
+  const bool nbp = nbpar || c->pair_lph >= 100;
 #define MSG_PAIR_LAUNCH(R, NTP)                                                              \
   do {                                                                                       \
-    if (nbpar && c->pair_lph == 8)                                                           \
+    if (nbp && lph == 8)                                                                     \
       hipLaunchKernelGGL((k_pair_nb<R, NTP, 8>), dim3(nb), dim3(256), lds, c->stream, a);   \
-    else if (nbpar && c->pair_lph == 16)                                                     \
+    else if (nbp && lph == 16)                                                               \
       hipLaunchKernelGGL((k_pair_nb<R, NTP, 16>), dim3(nb), dim3(256), lds, c->stream, a);  \
-    else if (nbpar && c->pair_lph == 32)                                                     \
+    else if (nbp && lph == 32)                                                               \
       hipLaunchKernelGGL((k_pair_nb<R, NTP, 32>), dim3(nb), dim3(256), lds, c->stream, a);  \
-    else if (nbpar)                                                                          \
+    else if (nbp)                                                                            \
       hipLaunchKernelGGL((k_pair_nb<R, NTP, 64>), dim3(nb), dim3(256), lds, c->stream, a);  \
     else                                                                                     \
       hipLaunchKernelGGL((k_pair<R, NTP>), dim3(nb), dim3(256), lds, c->stream, a);         \
@@ -1050,6 +1660,21 @@ SegLaunchFn pair_fn(int lph) {
   }
 }
 
+template <int NTP, int LPH>
+void seg_pair_tile(const SegGroup& g, hipStream_t s) {
+  hipLaunchKernelGGL((k_pair_tile_seg<NTP, LPH>), dim3(g.nblk), dim3(256), g.lds, s,
+                     (const PairArgs*)g.d_args, g.d_map);
+}
+
+SegLaunchFn pair_tile_fn(int ntp, int lph) {
+  switch (ntp) {
+    case 1: return lph == 8 ? seg_pair_tile<1, 8> : seg_pair_tile<1, 16>;
+    case 3: return lph == 8 ? seg_pair_tile<3, 8> : seg_pair_tile<3, 16>;
+    case 6: return lph == 8 ? seg_pair_tile<6, 8> : seg_pair_tile<6, 16>;
+    default: return lph == 8 ? seg_pair_tile<10, 8> : seg_pair_tile<10, 16>;
+  }
+}
+
 template <typename real>
 SegLaunchFn pair_fn_ntp(int ntp, int lph) {
   switch (ntp) {
@@ -1068,8 +1693,14 @@ int plan_pair(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out) {
   SegBuilder<PairArgs> b;
   for (Ctx* c : cs) {
     if (c->P.list_mode == MSG_LIST_VERLET) return set_error(MSG_ERR_ARG, "batch: Verlet-list systems run unbatched");
-    const int lph = pair_nbpar(c) ? c->pair_lph : 0;
     const int ntp = c->nptpt <= 1 ? 1 : c->nptpt <= 3 ? 3 : c->nptpt <= 6 ? 6 : 10;
+    if (pair_tiled(c)) {
+      const int key = 200000 + ntp * 1000 + c->pair_lph;
+      b.add(key, pair_tile_fn(ntp, nbpar_lanes(c)), pair_args(c, c->P.ewald_mode == MSG_EWALD_NONE), c->npart_pair,
+            pair_lds(c));
+      continue;
+    }
+    const int lph = (pair_nbpar(c) || c->pair_lph >= 100) ? nbpar_lanes(c) : 0;
     const int key = (c->fp64 ? 100000 : 0) + ntp * 100 + lph;
     SegLaunchFn fn = c->fp64 ? pair_fn_ntp<double>(ntp, lph) : pair_fn_ntp<float>(ntp, lph);
     b.add(key, fn, pair_args(c, c->P.ewald_mode == MSG_EWALD_NONE), c->npart_pair, pair_lds(c));
