@@ -24,6 +24,8 @@ struct PairArgs {
   const float4* crel;   // mixed path: FP32 cell-relative records (the neighbour-parallel kernel reads these)
   int xcd;              // single-system launch: workgroups renumbered so each XCD takes a contiguous run
   int tile_tx;          // cell-tile kernel: cells per tile along x
+  float csz[3];         // cell edge lengths (FP32 of 1 / csi)
+  float ncinv[2];       // 1 / nc[0], 1 / (nc[0] * nc[1]) in FP32 (cell coordinates of a cell index)
   const int* ctype;
   const int* cell_ip;
   const int* cell_start;
@@ -554,29 +556,104 @@ __device__ inline int xcd_logical(int b, int n) {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
+constexpr int kNbRound = 6;    // candidates per lane and test round (loads in flight)
+constexpr int kNbEval = 4;     // in-cut pairs per lane evaluated together (loads in flight)
+constexpr int kNbSlotBits = 27;  // a code (unsigned) holds a slot below 2^27 and a stencil index above
+
+// Quotient of 0 <= x < 2^24 by d >= 1 through the FP32 reciprocal dinv ~ 1 / d: the product
+// is off by at most one, and one correction step makes it exact.
+__device__ __forceinline__ int div_small(int x, int d, float dinv) {
+  int q = (int)((float)x * dinv);
+  const int r = x - q * d;
+  q += r < 0 ? -1 : (r >= d ? 1 : 0);
+  return q;
+}
+
+// Inclusive prefix sum and total of x over aligned segments of L = 8 or 16 lanes (inside a row
+// of 16 lanes), with DPP row shifts: no LDS round trip.  k = lane index within the segment.
+template <int L, int OFF>
+__device__ __forceinline__ void seg_scan_step(int& inc, int& suf, int k) {
+  // row_shr:OFF / row_shl:OFF; lanes shifted in from outside the row read 0, lanes from the
+  // other segment of the row are masked
+  const int lo = __builtin_amdgcn_update_dpp(0, inc, 0x110 + OFF, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, suf, 0x100 + OFF, 0xF, 0xF, true);
+  inc += k >= OFF ? lo : 0;
+  suf += k + OFF < L ? hi : 0;
+}
+
+template <int L>
+__device__ __forceinline__ int seg_scan(int x, int k, int& total) {
+  static_assert(L == 8 || L == 16, "segments inside a DPP row");
+  int inc = x, suf = x;
+  seg_scan_step<L, 1>(inc, suf, k);
+  seg_scan_step<L, 2>(inc, suf, k);
+  seg_scan_step<L, 4>(inc, suf, k);
+  if constexpr (L == 16) seg_scan_step<L, 8>(inc, suf, k);
+  total = inc + suf - x;
+  return inc;
+}
+
+// Displacement of a home (position hr relative to its cell) and a candidate (record p relative
+// to its own cell) whose cell is the home's cell + stencil offset o: (hr - o * cellsize) - p,
+// with PBCr2's single shift in FP32 in a dimension collapsed to one cell.  Returns r^2.
+__device__ __forceinline__ float rel_disp(const float* hr, float4 sh, float4 p, bool col, const float* Lf,
+                                          const float* Hf, float& dx, float& dy, float& dz) {
+  dx = (hr[0] - sh.x) - p.x;
+  dy = (hr[1] - sh.y) - p.y;
+  dz = (hr[2] - sh.z) - p.z;
+  if (col) {
+    if (fabsf(dx) > Hf[0]) dx -= copysignf(Lf[0], dx);
+    if (fabsf(dy) > Hf[1]) dy -= copysignf(Lf[1], dy);
+    if (fabsf(dz) > Hf[2]) dz -= copysignf(Lf[2], dz);
+  }
+  return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
 // Mixed-path neighbour-parallel pair kernel over FP32 cell-relative records: a home's
 // displacement to a candidate of stencil cell s is (rel_home - s * cellsize) - rel_cand, which
 // is the minimum image for every pair within the cut-off when a dimension has at least three
 // cells (cells >= rcut); a dimension collapsed to one cell takes PBCr2's single shift in FP32.
-// The candidate records are 16 bytes (position and type), and no FP64 arithmetic is spent per
-// candidate.  Otherwise as pair_nb_body.
+// The candidate records are 16 bytes (position and type); no FP64 arithmetic is spent per
+// candidate.
+//
+// LPH lanes take one home.  Its candidates (the particles of the stencil cells not beyond the
+// cut-off from the home) are written as codes (slot | stencil index << 27) into an LDS list
+// of CAP entries per home, a chunk at a time.  The lanes then test the list interleaved
+// (entry e by lane e mod LPH: balanced, no search for cell boundaries) and compact the codes of
+// the pairs inside the cut-off to the front of the list, in list order; the table lookup and
+// accumulation then run over the compacted codes, again interleaved, so they cost once per
+// in-cut pair instead of once per candidate slot of the wavefront.  Every step is a fixed
+// function of the cell list, so the sums are reproducible.
 template <int NTP, int LPH>
 __device__ __forceinline__ void pair_nb_rel_body(const PairArgs& a, int bid) {
   constexpr int HPB = 256 / LPH;
   constexpr int MPL = (kMaxStencil + LPH - 1) / LPH;
+  constexpr int KU = kNbRound;
+  constexpr int CAP = LPH * KU > 128 ? LPH * KU : 128;  // list entries per home
+  constexpr int PB = kNbEval;
+  constexpr unsigned kMask = (1u << kNbSlotBits) - 1;
+  constexpr int NT = PairAcc<float, NTP>::NT;
   extern __shared__ __align__(16) double s_dyn[];
   __shared__ PairMeta<float> s_pm[kMaxT * kMaxT];
   __shared__ int2 s_pi[kMaxT * kMaxT];
   __shared__ int s_tp[kMaxT * kMaxT];
   __shared__ double s_red[4][kMaxTP + 1];
-  __shared__ int s_cst[HPB][kMaxStencil + 1], s_cpf[HPB][kMaxStencil + 1];
+  __shared__ float4 s_sh[kMaxStencil];
+  __shared__ int s_so[kMaxStencil];  // stencil offsets + 1 packed as ox | oy << 2 | oz << 4
+  __shared__ unsigned s_list[HPB][CAP];
   const int nint = a.tm.nint;
   float* s_coef = (float*)s_dyn;
   int* s_bins = (int*)(s_coef + nint * 12);
+  const float csz[3] = {a.csz[0], a.csz[1], a.csz[2]};
   {
     lds_copy16(s_coef, a.tcoef, nint * 12 * (int)sizeof(float));
     lds_copy16(s_bins, a.tbins, NTP * kTabBins * (int)sizeof(int));
     fill_pair_meta<float>(a.tm, s_pm, s_pi, s_tp);
+    if (threadIdx.x < a.g.nst) {
+      const int* o = a.g.st + 3 * threadIdx.x;
+      s_sh[threadIdx.x] = make_float4(o[0] * csz[0], o[1] * csz[1], o[2] * csz[2], 0.f);
+      s_so[threadIdx.x] = (o[0] + 1) | (o[1] + 1) << 2 | (o[2] + 1) << 4;
+    }
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, grp = threadIdx.x / LPH;
   const int s = bid * HPB + grp;
@@ -584,14 +661,29 @@ __device__ __forceinline__ void pair_nb_rel_body(const PairArgs& a, int bid) {
   const bool home = s < a.np;
   const int hs = home ? s : a.np - 1;
   const float4 hp = a.crel[hs];
+  const int c = a.slot_cell[hs];
+  __syncthreads();  // table, metadata and stencil in LDS
   const int hw = (int)hp.w;
   const int ht = hw & (kUncharged - 1);
   const bool hcharged = hw < kUncharged;
-  const int c = a.slot_cell[hs];
-  const int cx = c % a.g.nc[0], cy = (c / a.g.nc[0]) % a.g.nc[1], cz = c / (a.g.nc[0] * a.g.nc[1]);
-  float csz[3], hr[3] = {hp.x, hp.y, hp.z};
+  const float hr[3] = {hp.x, hp.y, hp.z};
+  // the home's cell coordinates, and per dimension and offset -1 / +1 the wrapped neighbour
+  // coordinate and the squared distance from the home to that side of its cell
+  int cc3[3];
+  cc3[2] = div_small(c, a.g.nc[0] * a.g.nc[1], a.ncinv[1]);
+  const int cxy = c - cc3[2] * a.g.nc[0] * a.g.nc[1];
+  cc3[1] = div_small(cxy, a.g.nc[0], a.ncinv[0]);
+  cc3[0] = cxy - cc3[1] * a.g.nc[0];
+  int cm[3], cp[3];
+  float dm[3], dp[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) csz[d] = (float)(1.0 / a.g.csi[d]);
+  for (int d = 0; d < 3; ++d) {
+    cm[d] = cc3[d] == 0 ? a.g.nc[d] - 1 : cc3[d] - 1;
+    cp[d] = cc3[d] == a.g.nc[d] - 1 ? 0 : cc3[d] + 1;
+    dm[d] = hr[d] * hr[d];
+    const float u = csz[d] - hr[d];
+    dp[d] = u > 0.f ? u * u : 0.f;
+  }
   // stencil cells whose nearest point is beyond the cut-off (with a margin) are skipped
   const float cull2 = (float)(a.rcut2 * (1.0 + 1e-4)) + 1e-3f;
   int st0[MPL], cnt[MPL], tot = 0;
@@ -600,116 +692,133 @@ __device__ __forceinline__ void pair_nb_rel_body(const PairArgs& a, int bid) {
     const int sc = k * MPL + m;
     cnt[m] = 0;
     st0[m] = 0;
-    float bd2 = 0.f;
-    if (sc < a.g.nst) {
+    if (home && hcharged && sc < a.g.nst) {
+      const int so = s_so[sc];
+      int xyz[3];
+      float bd2 = 0.f;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const int o = a.g.st[3 * sc + d];
-        const float dist = o < 0 ? hr[d] : (o > 0 ? csz[d] - hr[d] : 0.f);
-        bd2 += dist > 0.f ? dist * dist : 0.f;
+        const int o = (so >> (2 * d)) & 3;
+        xyz[d] = o == 0 ? cm[d] : (o == 2 ? cp[d] : cc3[d]);
+        bd2 += o == 0 ? dm[d] : (o == 2 ? dp[d] : 0.f);
       }
-    }
-    if (sc < a.g.nst && bd2 <= cull2) {
-      int x = cx + a.g.st[3 * sc], y = cy + a.g.st[3 * sc + 1], z = cz + a.g.st[3 * sc + 2];
-      x += x < 0 ? a.g.nc[0] : (x >= a.g.nc[0] ? -a.g.nc[0] : 0);
-      y += y < 0 ? a.g.nc[1] : (y >= a.g.nc[1] ? -a.g.nc[1] : 0);
-      z += z < 0 ? a.g.nc[2] : (z >= a.g.nc[2] ? -a.g.nc[2] : 0);
-      const int nb = x + a.g.nc[0] * (y + a.g.nc[1] * z);
-      st0[m] = a.cell_start[nb];
-      cnt[m] = a.cell_start[nb + 1] - st0[m];
+      if (bd2 <= cull2) {
+        const int nb = xyz[0] + a.g.nc[0] * (xyz[1] + a.g.nc[1] * xyz[2]);
+        st0[m] = a.cell_start[nb];
+        cnt[m] = a.cell_start[nb + 1] - st0[m];
+      }
     }
     tot += cnt[m];
   }
-  int incl = tot;
+  int ncand, incl;
+  if constexpr (LPH <= 16) {
+    incl = seg_scan<LPH>(tot, k, ncand);
+  } else {
+    incl = tot;
 #pragma unroll
-  for (int off = 1; off < LPH; off <<= 1) {
-    const int t = __shfl_up(incl, off, LPH);
-    if (k >= off) incl += t;
-  }
-  const int ncand = __shfl(incl, LPH - 1, LPH);
-  {
-    int pre = incl - tot;
-#pragma unroll
-    for (int m = 0; m < MPL; ++m) {
-      const int sc = k * MPL + m;
-      if (sc < a.g.nst) {
-        s_cst[grp][sc] = st0[m];
-        s_cpf[grp][sc] = pre;
-      }
-      pre += cnt[m];
+    for (int off = 1; off < LPH; off <<= 1) {
+      const int t = __shfl_up(incl, off, LPH);
+      if (k >= off) incl += t;
     }
+    ncand = __shfl(incl, LPH - 1, LPH);
   }
-  if (k == 0) s_cpf[grp][a.g.nst] = ncand;
-  __syncthreads();
+  const int pre = incl - tot;  // this lane's cells start at this candidate index
 
-  // collapsed dimensions (one cell): PBCr2's single shift in FP32
   const bool col = a.g.nc[0] == 1 || a.g.nc[1] == 1 || a.g.nc[2] == 1;
   const float Lf[3] = {(float)a.box.L[0], (float)a.box.L[1], (float)a.box.L[2]};
   const float Hf[3] = {(float)a.box.half[0], (float)a.box.half[1], (float)a.box.half[2]};
   const float rc2 = (float)a.rcut2;
-  PairAcc<float, NTP> acc;
-  acc.zero();
-  const int share = (ncand + LPH - 1) / LPH;
-  int i = k * share;
-  const int i1 = (home && hcharged) ? min(ncand, i + share) : 0;
-  if (i < i1) {
-    int cc = 0;
-    while (s_cpf[grp][cc + 1] <= i) ++cc;
-    int j = s_cst[grp][cc] + (i - s_cpf[grp][cc]);
-    int cend = s_cst[grp][cc] + (s_cpf[grp][cc + 1] - s_cpf[grp][cc]);
-    // the home's position shifted by the current stencil cell's offset
-    float sx = hr[0] - a.g.st[3 * cc] * csz[0], sy = hr[1] - a.g.st[3 * cc + 1] * csz[1],
-          sz = hr[2] - a.g.st[3 * cc + 2] * csz[2];
-    constexpr int U = 4;  // candidates in flight per lane
-    while (i < i1) {
-      int jv[U];
-      bool hv[U];
-      float3 hsv[U];
+  unsigned* list = s_list[grp];
+  double et[NT], vir = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        hv[u] = i < i1;
-        jv[u] = hv[u] ? j : s;
-        hsv[u] = make_float3(sx, sy, sz);
-        if (hv[u]) {
-          ++i;
-          ++j;
-          if (j == cend && i < i1) {
-            do { ++cc; } while (s_cpf[grp][cc + 1] <= i);
-            j = s_cst[grp][cc];
-            cend = j + (s_cpf[grp][cc + 1] - s_cpf[grp][cc]);
-            sx = hr[0] - a.g.st[3 * cc] * csz[0];
-            sy = hr[1] - a.g.st[3 * cc + 1] * csz[1];
-            sz = hr[2] - a.g.st[3 * cc + 2] * csz[2];
-          }
-        }
+  for (int q = 0; q < NT; ++q) et[q] = 0.0;
+  int err = 0;
+  for (int base = 0; base < ncand; base += CAP) {  // the same chunks for the home's lanes
+    const int nc = min(CAP, ncand - base);
+    {
+      int p0 = pre;
+#pragma unroll
+      for (int m = 0; m < MPL; ++m) {
+        const int lo = max(p0, base), hi = min(p0 + cnt[m], base + CAP);
+        const unsigned code0 = (unsigned)(st0[m] - p0) + ((unsigned)(k * MPL + m) << kNbSlotBits);  // + t
+        for (int t = lo; t < hi; ++t) list[t - base] = code0 + t;
+        p0 += cnt[m];
       }
-      float4 pv[U];
+    }
+    wave_sync();
+    int nq = 0;  // codes of in-cut pairs at the front of the list
+    for (int r0 = 0; r0 < nc; r0 += LPH * KU) {
+      unsigned code[KU];
+      float4 sh[KU], pv[KU];
 #pragma unroll
-      for (int u = 0; u < U; ++u) pv[u] = a.crel[jv[u]];
+      for (int u = 0; u < KU; ++u) code[u] = list[min(r0 + u * LPH + k, nc - 1)];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (!hv[u] || jv[u] == s) continue;
-        float dx = hsv[u].x - pv[u].x, dy = hsv[u].y - pv[u].y, dz = hsv[u].z - pv[u].z;
-        if (col) {
-          if (fabsf(dx) > Hf[0]) dx -= copysignf(Lf[0], dx);
-          if (fabsf(dy) > Hf[1]) dy -= copysignf(Lf[1], dy);
-          if (fabsf(dz) > Hf[2]) dz -= copysignf(Lf[2], dz);
+      for (int u = 0; u < KU; ++u) {
+        sh[u] = s_sh[code[u] >> kNbSlotBits];
+        pv[u] = a.crel[code[u] & kMask];
+      }
+      // in-cut candidates of this round (the home itself and uncharged partners are dropped
+      // when the queue is evaluated)
+      unsigned hm = 0;
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        float dx, dy, dz;
+        const float r2 = rel_disp(hr, sh[u], pv[u], col, Lf, Hf, dx, dy, dz);
+        if (r0 + u * LPH + k < nc && r2 < rc2) hm |= 1u << u;
+      }
+      const int nhit = __popc(hm);
+      int ng, hi;
+      if constexpr (LPH <= 16) {
+        hi = seg_scan<LPH>(nhit, k, ng);
+      } else {
+        hi = nhit;
+#pragma unroll
+        for (int off = 1; off < LPH; off <<= 1) {
+          const int t = __shfl_up(hi, off, LPH);
+          if (k >= off) hi += t;
         }
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (!(r2 < rc2)) continue;
-        const int tj = (int)pv[u].w;
-        if (tj >= kUncharged) continue;  // weak charges: uncharged partner
+        ng = __shfl(hi, LPH - 1, LPH);
+      }
+      // every entry of this round was read above: the compacted codes go below its end
+      int pos = nq + hi - nhit;
+#pragma unroll
+      for (int u = 0; u < KU; ++u)
+        if (hm & (1u << u)) list[pos++] = code[u];
+      nq = min(nq + ng, nc);
+    }
+    wave_sync();
+    for (int e0 = 0; e0 < nq; e0 += LPH * PB) {
+      unsigned cu[PB];
+      float4 pj[PB], shv[PB];
+#pragma unroll
+      for (int w = 0; w < PB; ++w) cu[w] = list[min(e0 + w * LPH + k, nq - 1)];
+#pragma unroll
+      for (int w = 0; w < PB; ++w) {
+        pj[w] = a.crel[cu[w] & kMask];
+        shv[w] = s_sh[cu[w] >> kNbSlotBits];
+      }
+      // the batch's sums in FP32 (at most PB terms), added to the FP64 sums once per batch
+      float ef[NT], vf = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+      for (int q = 0; q < NT; ++q) ef[q] = 0.f;
+#pragma unroll
+      for (int w = 0; w < PB; ++w) {
+        const int tj = (int)pj[w].w;
+        // tj >= kUncharged: weak charges, no pair energy with an uncharged partner
+        if (e0 + w * LPH + k >= nq || (int)(cu[w] & kMask) == s || tj >= kUncharged) continue;
+        float dx, dy, dz;
+        const float r2 = rel_disp(hr, shv[w], pj[w], col, Lf, Hf, dx, dy, dz);
         const int mi = ht * kMaxT + tj;
         const PairMeta<float> pm = s_pm[mi];
-        const int2 pi = s_pi[mi];
         float us, fs;
         if (r2 < pm.r2at) {
           us = 1e10f;
           fs = 1e10f;
         } else if (r2 < pm.r2um) {
-          acc.err |= FLAG_R2UMIN;
+          err |= FLAG_R2UMIN;
           continue;
         } else {
+          const int2 pi = s_pi[mi];
           int b = (int)((r2 - pm.zlo) * pm.invw);
           b = min(max(b, 0), kTabBins - 1);
           int ti = s_bins[pi.x * kTabBins + b];
@@ -730,34 +839,44 @@ __device__ __forceinline__ void pair_nb_rel_body(const PairArgs& a, int bid) {
           fs = t[7] + dd * (t[8] + dd * (t[9] + dd * (t[10] + dd * t[11])));
         }
 #pragma unroll
-        for (int q = 0; q < PairAcc<float, NTP>::NT; ++q) acc.et[q] += (q == tj) ? (double)us : 0.0;
-        acc.fx += (double)(fs * dx);
-        acc.fy += (double)(fs * dy);
-        acc.fz += (double)(fs * dz);
-        acc.vir -= (double)(fs * r2);
+        for (int q = 0; q < NT; ++q) ef[q] += q == tj ? us : 0.f;
+        gx += fs * dx;
+        gy += fs * dy;
+        gz += fs * dz;
+        vf -= fs * r2;
       }
+#pragma unroll
+      for (int q = 0; q < NT; ++q) et[q] += (double)ef[q];
+      fx += (double)gx;
+      fy += (double)gy;
+      fz += (double)gz;
+      vir += (double)vf;
     }
+    wave_sync();  // the next chunk overwrites the list
   }
 #pragma unroll
   for (int off = LPH / 2; off > 0; off >>= 1) {
-    acc.fx += __shfl_xor(acc.fx, off, 64);
-    acc.fy += __shfl_xor(acc.fy, off, 64);
-    acc.fz += __shfl_xor(acc.fz, off, 64);
+    fx += __shfl_xor(fx, off, 64);
+    fy += __shfl_xor(fy, off, 64);
+    fz += __shfl_xor(fz, off, 64);
   }
   if (home && k == 0) {
     const int ip = a.cell_ip[s];
-    a.force[3 * ip + 0] = acc.fx * a.scale;
-    a.force[3 * ip + 1] = acc.fy * a.scale;
-    a.force[3 * ip + 2] = acc.fz * a.scale;
+    a.force[3 * ip + 0] = fx * a.scale;
+    a.force[3 * ip + 1] = fy * a.scale;
+    a.force[3 * ip + 2] = fz * a.scale;
   }
-  if (acc.err) atomicOr(a.flags, acc.err);
+  if (err) atomicOr(a.flags, err);
 #pragma unroll
   for (int q = 0; q < NTP; ++q) {
-    const double v = msgd::wave_sum(acc.to_tp(q, ht, s_tp));
+    double v = 0.0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) v += (s_tp[ht * kMaxT + t] == q) ? et[t] : 0.0;
+    v = msgd::wave_sum(v);
     if (lane == 0) s_red[wv][q] = v;
   }
   {
-    const double v = msgd::wave_sum(acc.vir);
+    const double v = msgd::wave_sum(vir);
     if (lane == 0) s_red[wv][NTP] = v;
   }
   __syncthreads();
@@ -1434,7 +1553,9 @@ __global__ __launch_bounds__(256) void k_vlist_export(VlExportArgs a) {
 // The neighbour-parallel kernel serves every cell-list and all-pairs system (any stencil:
 // 27 cells, or fewer where dimensions collapse to one cell) unless tile 0 is chosen.
 bool pair_nbpar(const Ctx* c) {
-  return c->P.list_mode != MSG_LIST_VERLET && c->pair_lph > 0 && !pair_tiled(c);
+  // the mixed-path kernel codes a slot in 27 bits: larger systems take the work-item kernel
+  return c->P.list_mode != MSG_LIST_VERLET && c->pair_lph > 0 && !pair_tiled(c) &&
+         (c->fp64 || c->np < (1 << kNbSlotBits));
 }
 
 // Cell-tile kernel (tile codes 100 + lanes per home): mixed path, cell lists.
@@ -1514,6 +1635,9 @@ static PairArgs pair_args(Ctx* c, bool last) {
   a.scale = last ? c->P.force_scale : 1.0;
   a.nblk = c->npart_pair;
   a.tile_tx = pair_tiled(c) ? pair_tile_tx(c) : 0;
+  for (int d = 0; d < 3; ++d) a.csz[d] = (float)(1.0 / c->grid.csi[d]);
+  a.ncinv[0] = 1.0f / (float)c->grid.nc[0];
+  a.ncinv[1] = 1.0f / (float)(c->grid.nc[0] * c->grid.nc[1]);
   return a;
 }
 
