@@ -33,11 +33,13 @@ template <typename real> struct C2T;
 template <> struct C2T<float> { typedef float2 t; };
 template <> struct C2T<double> { typedef double2 t; };
 
+// Complex product with its fused multiply-adds written out, so every kernel that inlines it
+// (separate and fused FFT passes, single and batched launches) rounds it the same way.
 template <typename C2>
 __device__ inline C2 cmul(C2 a, C2 b) {
   C2 r;
-  r.x = a.x * b.x - a.y * b.y;
-  r.y = a.x * b.y + a.y * b.x;
+  r.x = fma(a.x, b.x, -(a.y * b.y));
+  r.y = fma(a.x, b.y, a.y * b.x);
   return r;
 }
 template <typename C2>
@@ -54,6 +56,44 @@ template <typename real> struct FftCfg;
 template <> struct FftCfg<float> { static constexpr int E = 2048, NT = 256; };
 template <> struct FftCfg<double> { static constexpr int E = 1024, NT = 256; };
 
+// One radix-4 butterfly of a Stockham stage (Ns = product of the earlier radices) of a line of
+// length K: reads s[j + m K/4], writes d[(j / Ns) 4 Ns + j mod Ns + m Ns], m = 0..3.
+template <typename C2, int K>
+__device__ __forceinline__ void r4_butterfly(const C2* s, C2* dd, int j, int Ns, const C2* __restrict__ tw,
+                                             bool inverse) {
+  const int step = K / (Ns * 4);
+  const int jr = j % Ns;
+  C2 v0 = s[j], v1 = s[j + K / 4], v2 = s[j + K / 2], v3 = s[j + 3 * K / 4];
+  C2 w1 = tw[jr * step], w2 = tw[2 * jr * step], w3 = tw[3 * jr * step];
+  if (inverse) { w1 = cconj(w1); w2 = cconj(w2); w3 = cconj(w3); }
+  v1 = cmul(v1, w1);
+  v2 = cmul(v2, w2);
+  v3 = cmul(v3, w3);
+  const C2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3);
+  const C2 d13 = csub(v1, v3);
+  // forward: a3 = -i (v1 - v3); inverse: +i (v1 - v3)
+  const C2 a3 = inverse ? cmk<C2>(-d13.y, d13.x) : cmk<C2>(d13.y, -d13.x);
+  C2* d = dd + (j / Ns) * Ns * 4 + jr;
+  d[0] = cadd(a0, a2);
+  d[Ns] = cadd(a1, a3);
+  d[2 * Ns] = csub(a0, a2);
+  d[3 * Ns] = csub(a1, a3);
+}
+
+// The closing radix-2 stage when log2 K is odd.
+template <typename C2, int K>
+__device__ __forceinline__ void r2_butterfly(const C2* s, C2* dd, int j, int Ns, const C2* __restrict__ tw,
+                                             bool inverse) {
+  const int step = K / (Ns * 2);
+  const int jr = j % Ns;
+  C2 w = tw[jr * step];
+  if (inverse) w = cconj(w);
+  const C2 v0 = s[j], v1 = cmul(s[j + K / 2], w);
+  C2* d = dd + (j / Ns) * Ns * 2 + jr;
+  d[0] = cadd(v0, v1);
+  d[Ns] = csub(v0, v1);
+}
+
 // Stockham autosort FFT of T lines of length K held line-major in LDS buffer A (B is
 // scratch), line i starting at element i * LS.  Radix-4 stages, then one radix-2 stage when
 // log2 K is odd.  Returns the buffer holding the result.  All NT threads of the workgroup
@@ -67,40 +107,37 @@ __device__ C2* lds_fft(C2* A, C2* B, const C2* __restrict__ tw, bool inverse) {
   C2* dst = B;
   int Ns = 1;
   for (; Ns * 4 <= K; Ns *= 4) {
-    const int step = K / (Ns * 4);
-    for (int j = tl; j < K / 4; j += TPL) {
-      const int jr = j % Ns;
-      const C2* s = src + line * LS;
-      C2 v0 = s[j], v1 = s[j + K / 4], v2 = s[j + K / 2], v3 = s[j + 3 * K / 4];
-      C2 w1 = tw[jr * step], w2 = tw[2 * jr * step], w3 = tw[3 * jr * step];
-      if (inverse) { w1 = cconj(w1); w2 = cconj(w2); w3 = cconj(w3); }
-      v1 = cmul(v1, w1);
-      v2 = cmul(v2, w2);
-      v3 = cmul(v3, w3);
-      const C2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3);
-      const C2 d13 = csub(v1, v3);
-      // forward: a3 = -i (v1 - v3); inverse: +i (v1 - v3)
-      const C2 a3 = inverse ? cmk<C2>(-d13.y, d13.x) : cmk<C2>(d13.y, -d13.x);
-      C2* d = dst + line * LS + (j / Ns) * Ns * 4 + jr;
-      d[0] = cadd(a0, a2);
-      d[Ns] = cadd(a1, a3);
-      d[2 * Ns] = csub(a0, a2);
-      d[3 * Ns] = csub(a1, a3);
+    for (int j = tl; j < K / 4; j += TPL) r4_butterfly<C2, K>(src + line * LS, dst + line * LS, j, Ns, tw, inverse);
+    __syncthreads();
+    C2* t = src; src = dst; dst = t;
+  }
+  if (Ns < K) {
+    for (int j = tl; j < K / 2; j += TPL) r2_butterfly<C2, K>(src + line * LS, dst + line * LS, j, Ns, tw, inverse);
+    __syncthreads();
+    C2* t = src; src = dst; dst = t;
+  }
+  return src;
+}
+
+// The same transform for nl lines (any count) and NT threads: the butterflies of a stage are
+// dealt to the threads as (line, butterfly) tasks.  Per line the arithmetic is lds_fft's.
+template <typename C2, int K, int NT>
+__device__ C2* lds_fft_lines(C2* A, C2* B, const C2* __restrict__ tw, bool inverse, int nl, int LS) {
+  C2* src = A;
+  C2* dst = B;
+  int Ns = 1;
+  for (; Ns * 4 <= K; Ns *= 4) {
+    for (int t = threadIdx.x; t < nl * (K / 4); t += NT) {
+      const int line = t / (K / 4), j = t % (K / 4);
+      r4_butterfly<C2, K>(src + line * LS, dst + line * LS, j, Ns, tw, inverse);
     }
     __syncthreads();
     C2* t = src; src = dst; dst = t;
   }
   if (Ns < K) {
-    const int step = K / (Ns * 2);
-    for (int j = tl; j < K / 2; j += TPL) {
-      const int jr = j % Ns;
-      const C2* s = src + line * LS;
-      C2 w = tw[jr * step];
-      if (inverse) w = cconj(w);
-      const C2 v0 = s[j], v1 = cmul(s[j + K / 2], w);
-      C2* d = dst + line * LS + (j / Ns) * Ns * 2 + jr;
-      d[0] = cadd(v0, v1);
-      d[Ns] = csub(v0, v1);
+    for (int t = threadIdx.x; t < nl * (K / 2); t += NT) {
+      const int line = t / (K / 2), j = t % (K / 2);
+      r2_butterfly<C2, K>(src + line * LS, dst + line * LS, j, Ns, tw, inverse);
     }
     __syncthreads();
     C2* t = src; src = dst; dst = t;
@@ -309,6 +346,109 @@ MSG_FFT_ENTRY(k_fft_y, fft_y_body, , 0)
 MSG_FFT_ENTRY(k_fft_yi, fft_y_body, , 1)
 MSG_FFT_ENTRY(k_fft_zconv, fft_zconv_body)
 #undef MSG_FFT_ENTRY
+
+// Fused plane passes (FP32, K <= 128, one system): a workgroup of 1024 threads per z-plane
+// holds the plane in LDS and does the x r2c and then the y FFT (k_fft_xy), or the inverse y
+// FFT and then the x c2r (k_fft_yxi), so the mesh crosses memory once per pair of passes and
+// two launches of the chain are saved.  Per line the arithmetic is that of k_fft_x / k_fft_y
+// / k_fft_yi / k_fft_xi (the same butterflies, twiddles, split and merge), so the results are
+// the same bits as the separate passes (which the batched passes keep).
+constexpr int kPlaneNT = 1024;
+template <int K>
+struct PlaneCfg {
+  static constexpr int H = K / 2 + 1, LS = K + 1, NH = K / 2;
+  static constexpr int NB = NH * K > H * LS ? NH * K : H * LS;  // complex elements per buffer
+  static constexpr size_t lds = (2 * (size_t)NB + K) * sizeof(float2);
+};
+
+template <int K>
+__global__ __launch_bounds__(kPlaneNT) void k_fft_xy(FftArgs fa) {
+  typedef float2 C2;
+  constexpr int H = PlaneCfg<K>::H, LS = PlaneCfg<K>::LS, NB = PlaneCfg<K>::NB, NH = PlaneCfg<K>::NH;
+  extern __shared__ __align__(16) double s_dyn[];
+  C2* A = (C2*)s_dyn;
+  C2* B = A + NB;
+  C2* s_tw = B + NB;
+  const float* __restrict__ Q = (const float*)fa.Q;
+  C2* __restrict__ Cm = (C2*)fa.Cm;
+  const C2* __restrict__ tw = (const C2*)fa.tw;
+  for (int k = threadIdx.x; k < K; k += kPlaneNT) s_tw[k] = tw[k];
+  const size_t z = blockIdx.x;
+  const float* __restrict__ Qz = Q + z * K * K;
+  // rows y = 2p and 2p + 1 packed as one complex line p
+  for (int e = threadIdx.x; e < NH * K; e += kPlaneNT) {
+    const int p = e / K, x = e % K;
+    A[p * K + x] = cmk<C2>(Qz[(2 * p) * K + x], Qz[(2 * p + 1) * K + x]);
+  }
+  __syncthreads();
+  C2* Z = lds_fft_lines<C2, K, kPlaneNT>(A, B, s_tw, false, NH, K);
+  C2* W = Z == A ? B : A;
+  const float h = 0.5f;
+  // split into the half spectra of the two rows, stored as y lines of column kx
+  for (int e = threadIdx.x; e < NH * H; e += kPlaneNT) {
+    const int p = e / H, k = e % H;
+    const C2 zz = Z[p * K + k], zc = cconj(Z[p * K + ((K - k) & (K - 1))]);
+    const C2 sm = cadd(zz, zc), d = csub(zz, zc);
+    W[k * LS + 2 * p] = cmk<C2>(h * sm.x, h * sm.y);
+    W[k * LS + 2 * p + 1] = cmk<C2>(h * d.y, -h * d.x);  // (z - zc) / (2i)
+  }
+  __syncthreads();
+  const C2* R = lds_fft_lines<C2, K, kPlaneNT>(W, Z, s_tw, false, H, LS);
+  C2* __restrict__ Nq = Cm + (size_t)NH * K * K;
+  for (int e = threadIdx.x; e < NH * K; e += kPlaneNT) {
+    const int y = e / NH, kx = e % NH;
+    Cm[kx + (size_t)NH * (y + K * z)] = R[kx * LS + y];
+  }
+  for (int y = threadIdx.x; y < K; y += kPlaneNT) Nq[y + K * z] = R[NH * LS + y];
+}
+
+template <int K>
+__global__ __launch_bounds__(kPlaneNT) void k_fft_yxi(FftArgs fa) {
+  typedef float2 C2;
+  constexpr int H = PlaneCfg<K>::H, LS = PlaneCfg<K>::LS, NB = PlaneCfg<K>::NB, NH = PlaneCfg<K>::NH;
+  extern __shared__ __align__(16) double s_dyn[];
+  C2* A = (C2*)s_dyn;
+  C2* B = A + NB;
+  C2* s_tw = B + NB;
+  const C2* __restrict__ Cm = (const C2*)fa.Cm;
+  float* __restrict__ Q = (float*)fa.Q;
+  const C2* __restrict__ tw = (const C2*)fa.tw;
+  for (int k = threadIdx.x; k < K; k += kPlaneNT) s_tw[k] = tw[k];
+  const size_t z = blockIdx.x;
+  const C2* __restrict__ Nq = Cm + (size_t)NH * K * K;
+  for (int e = threadIdx.x; e < NH * K; e += kPlaneNT) {
+    const int y = e / NH, kx = e % NH;
+    A[kx * LS + y] = Cm[kx + (size_t)NH * (y + K * z)];
+  }
+  for (int y = threadIdx.x; y < K; y += kPlaneNT) A[NH * LS + y] = Nq[y + K * z];
+  __syncthreads();
+  C2* R = lds_fft_lines<C2, K, kPlaneNT>(A, B, s_tw, true, H, LS);
+  C2* X = R == A ? B : A;
+  // rows 2p and 2p + 1 merged into one complex line (imaginary parts of kx = 0, K/2 ignored)
+  for (int e = threadIdx.x; e < NH * H; e += kPlaneNT) {
+    const int p = e / H, k = e % H;
+    C2 a = R[k * LS + 2 * p], b = R[k * LS + 2 * p + 1];
+    if (k == 0 || k == NH) { a.y = 0; b.y = 0; }
+    X[p * K + k] = cmk<C2>(a.x - b.y, a.y + b.x);                         // a + i b
+    if (k > 0 && k < NH) X[p * K + K - k] = cmk<C2>(a.x + b.y, b.x - a.y);  // conj(a) + i conj(b)
+  }
+  __syncthreads();
+  const C2* Zl = lds_fft_lines<C2, K, kPlaneNT>(X, R, s_tw, true, NH, K);
+  float* __restrict__ Qz = Q + z * K * K;
+  for (int e = threadIdx.x; e < NH * K; e += kPlaneNT) {
+    const int p = e / K, x = e % K;
+    const C2 v = Zl[p * K + x];
+    Qz[(2 * p) * K + x] = v.x;
+    Qz[(2 * p + 1) * K + x] = v.y;
+  }
+}
+
+// The fused plane passes serve the single-system mixed path at K <= 128 (the plane and its
+// scratch copy fit the LDS of one workgroup).
+template <typename real, int K>
+constexpr bool plane_fft() {
+  return sizeof(real) == 4 && K <= 128;
+}
 
 // ------------------------------------------------------------------------------------------
 // General path: any even K (the reference's EwaldSetup takes any even nmesh, e.g. its default
@@ -1074,6 +1214,18 @@ void launch_fft_all(Ctx* c) {
   int nb[5];
   fft_blocks<real, K>(nb);
   const FftArgs a = fft_args(c);
+  if constexpr (plane_fft<real, K>()) {
+    prof_begin(c, MSG_K_FFT_X);
+    hipLaunchKernelGGL((k_fft_xy<K>), dim3(K), dim3(kPlaneNT), PlaneCfg<K>::lds, c->stream, a);
+    prof_end(c, MSG_K_FFT_X);
+    prof_begin(c, MSG_K_FFT_ZCONV);
+    hipLaunchKernelGGL((k_fft_zconv<real, K>), dim3(nb[2]), dim3(256), 0, c->stream, a);
+    prof_end(c, MSG_K_FFT_ZCONV);
+    prof_begin(c, MSG_K_FFT_XI);
+    hipLaunchKernelGGL((k_fft_yxi<K>), dim3(K), dim3(kPlaneNT), PlaneCfg<K>::lds, c->stream, a);
+    prof_end(c, MSG_K_FFT_XI);
+    return;
+  }
   prof_begin(c, MSG_K_FFT_X);
   hipLaunchKernelGGL((k_fft_x<real, K>), dim3(nb[0]), dim3(256), 0, c->stream, a);
   prof_end(c, MSG_K_FFT_X);
@@ -1212,6 +1364,26 @@ int spme_setup(Ctx* c) {
     m.nb[0] = K / kBX;
     m.nb[1] = K / kBY;
     m.nb[2] = K / kBZ;
+    if (!c->fp64 && K <= 128) {
+      // the fused plane passes take more dynamic LDS than the default limit
+      hipError_t e = hipSuccess;
+      switch (K) {
+#define MSG_PLANE_ATTR(KK)                                                                          \
+  case KK:                                                                                        \
+    e = hipFuncSetAttribute((const void*)k_fft_xy<KK>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                            (int)PlaneCfg<KK>::lds);                                                \
+    if (e == hipSuccess)                                                                          \
+      e = hipFuncSetAttribute((const void*)k_fft_yxi<KK>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                              (int)PlaneCfg<KK>::lds);                                              \
+    break;
+        MSG_PLANE_ATTR(16)
+        MSG_PLANE_ATTR(32)
+        MSG_PLANE_ATTR(64)
+        MSG_PLANE_ATTR(128)
+#undef MSG_PLANE_ATTR
+      }
+      if (e != hipSuccess) return set_error(MSG_ERR_HIP, "hipFuncSetAttribute (plane FFT LDS)");
+    }
   } else {
     // bricks of at least max(8, p - 1) points, so a stencil reaches only the next brick
     const int bmin = std::max(8, p - 1);
