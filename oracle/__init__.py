@@ -172,6 +172,13 @@ def _declare(L):
     L.orc_celllist_set.restype = C.c_int
     L.orc_celllist_update.argtypes = [C.c_int, _dp, _dp, _ip, _dp, _ip, _ip, _ip, _ip, _ip]
     L.orc_celllist_update.restype = C.c_int
+    L.orc_trials_open.argtypes = [C.POINTER(OrcSys)]
+    L.orc_trials_open.restype = C.c_void_p
+    L.orc_trials_close.argtypes = [C.c_void_p]
+    L.orc_trials_close.restype = None
+    L.orc_trials_dutotal.argtypes = [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_int), _dp,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.orc_trials_dutotal.restype = C.c_int
 
 
 def _check(rc, what):
@@ -446,6 +453,13 @@ class System:
                                  dt, C.byref(drec), C.byref(dtot)), "dutotal")
         return dict(overlap=bool(ov.value), du_twob=dt, du_rec=drec.value, du_tot=dtot.value)
 
+    def trials(self):
+        """Trial energies with the configuration's cell list, structure factors (or charge
+        mesh) and dipole resident, as MOLSIM's MC loop keeps them: Trials.dutotal(ip, rnew)
+        costs O(partners + nkvec) with std Ewald.  The positions must not change while it is
+        open."""
+        return Trials(self)
+
     def dutotal_charge(self, ips):
         """ChargeChange trial: the charge states of particles ips (one or two, 0-based) flip."""
         nptpt = self.npt * (self.npt + 1) // 2
@@ -456,3 +470,30 @@ class System:
         _check(lib().orc_dutotal_charge(C.byref(self.s), int(ia.size), ia, C.byref(ov), dt,
                                         C.byref(drec), C.byref(dtot)), "dutotal_charge")
         return dict(overlap=bool(ov.value), du_twob=dt, du_rec=drec.value, du_tot=dtot.value)
+
+
+class Trials:
+    """Incremental DUTotal (orc_trials_*) over one System's configuration."""
+
+    def __init__(self, system):
+        self.sys = system
+        self.h = lib().orc_trials_open(C.byref(system.s))
+        if not self.h:
+            raise RuntimeError("oracle trials_open failed")
+
+    def dutotal(self, ip, rnew):
+        nptpt = self.sys.npt * (self.sys.npt + 1) // 2
+        dt = np.zeros(nptpt + 1)
+        ov = C.c_int()
+        drec, dtot = C.c_double(), C.c_double()
+        _check(lib().orc_trials_dutotal(self.h, int(ip), np.asarray(rnew, dtype=np.float64), C.byref(ov),
+                                        dt, C.byref(drec), C.byref(dtot)), "trials_dutotal")
+        return dict(overlap=bool(ov.value), du_twob=dt, du_rec=drec.value, du_tot=dtot.value)
+
+    def close(self):
+        if self.h:
+            lib().orc_trials_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -28,6 +28,8 @@
 //   orc_nlist_skin     Src/nlist.F90:597-621      NList skin test (inlist = 0), r2max1/r2max2 order
 //   orc_celllist_set   Src/celllist.F90:307-327,384-410  SetCellList with ipnext/ipprev
 //   orc_celllist_update Src/celllist.F90:329-382  UpdateCellIp = RmIpFromCell + AddIpToCell
+//   orc_trials_*       Src/denergy.F90:1275-1335,1413-1539,1708-1731  DUTotal with resident
+//                      structure factors / charge mesh / dipole (incremental trial energies)
 //
 // Conventions: positions ro[3*i+d] (Fortran ro(3,np)), particle types 0-based, type pair
 // index tp(i,j) = MOLSIM iptpt(i+1,j+1)-1, u_twob[0] = sum, u_twob[1+tp] per pair.
@@ -1089,48 +1091,59 @@ int orc_utotal(const OrcSys* S, double* u_twob, double* u_rec, double* u_tot, do
 // :728-928 / :289-495) and DUTwoBodyEwald :1240-1543 (std: sumeikr update; spm: the
 // reference's full r2c of the modified mesh).  *overlap = 1 on a hard-core overlap
 // (then only the overlap flag is meaningful, as in the reference).
-int orc_dutotal(const OrcSys* S, int ip, const double* rnew, int* overlap, double* du_twob,
-                double* du_rec, double* du_tot) {
-  int np = S->np, npt = S->npt, nptpt = npt * (npt + 1) / 2;
-  double rcut2 = S->rcut * S->rcut;
-  std::vector<double> unew(nptpt + 1, 0.0), uold(nptpt + 1, 0.0);
-  *overlap = 1;
-  *du_rec = 0;
-  *du_tot = 0;
-  for (int k = 0; k <= nptpt; ++k) du_twob[k] = 0;
-  std::vector<int> cand;
-  if (S->list == 1) {
-    int nc[3];
-    double cs[3];
-    std::vector<int> st(3 * 1000);
-    int nst = orc_cell_init(S->box, S->rcut + S->drnlist, S->rcut, nc, cs, st.data(), 1000);
-    if (nst < 0) return nst;
-    int ncell = nc[0] * nc[1] * nc[2];
-    std::vector<int> head(ncell), next(np), npart(ncell), cellof(np), nb(nst);
-    int rc = orc_cell_build(np, S->ro, S->box, nc, cs, head.data(), next.data(), npart.data(),
-                            cellof.data());
-    if (rc) return rc;
+// Cell list of a configuration for the real-space part of DUTotal (InitCellList +
+// SetCellList with the list's cut-off rcut + drnlist).
+struct CellIndex {
+  int nc[3];
+  double cs[3];
+  std::vector<int> st;
+  int nst = 0;
+  std::vector<int> head, next, npart, cellof;
+};
+
+static int cell_index_build(const OrcSys* S, CellIndex& C) {
+  C.st.assign(3 * 1000, 0);
+  C.nst = orc_cell_init(S->box, S->rcut + S->drnlist, S->rcut, C.nc, C.cs, C.st.data(), 1000);
+  if (C.nst < 0) return C.nst;
+  const int ncell = C.nc[0] * C.nc[1] * C.nc[2];
+  C.head.assign(ncell, 0);
+  C.next.assign(S->np, 0);
+  C.npart.assign(ncell, 0);
+  C.cellof.assign(S->np, 0);
+  return orc_cell_build(S->np, S->ro, S->box, C.nc, C.cs, C.head.data(), C.next.data(), C.npart.data(),
+                        C.cellof.data());
+}
+
+// Real-space part of DUTotal (UTwoBodyANew / UTwoBodyAOld, Src/denergy.F90:728-928, or their
+// all-pairs forms): per type pair, the new position's and the old position's pair energies
+// with every other particle.  Returns 0, 1 when the new position overlaps, or an error.
+static int dut_real(const OrcSys* S, const CellIndex* C, int ip, const double* rnew, double* unew,
+                    double* uold) {
+  const int np = S->np, npt = S->npt;
+  const double rcut2 = S->rcut * S->rcut;
+  if (C) {
     // new: cell of the trial position; old: cell of ip
     int ci[3];
     for (int k = 0; k < 3; ++k) {
-      ci[k] = cell_coord(rnew[k], S->box[k], 1.0 / cs[k]);
-      if (ci[k] == nc[k]) ci[k] = nc[k] - 1;
-      if (ci[k] < 0 || ci[k] >= nc[k]) return ORC_ERR_ARG;
+      ci[k] = cell_coord(rnew[k], S->box[k], 1.0 / C->cs[k]);
+      if (ci[k] == C->nc[k]) ci[k] = C->nc[k] - 1;
+      if (ci[k] < 0 || ci[k] >= C->nc[k]) return ORC_ERR_ARG;
     }
-    int cnew = ci[0] + nc[0] * (ci[1] + nc[1] * ci[2]);
+    const int cnew = ci[0] + C->nc[0] * (ci[1] + C->nc[1] * ci[2]);
+    std::vector<int> nb(C->nst);
     for (int pass = 0; pass < 2; ++pass) {
-      int c = pass == 0 ? cnew : cellof[ip];
+      const int c = pass == 0 ? cnew : C->cellof[ip];
       const double* ri = pass == 0 ? rnew : S->ro + 3 * ip;
-      int nn = cell_neigh(c, nc, st.data(), nst, nb.data());
-      for (int s = 0; s < nn; ++s)
-        for (int j = head[nb[s]]; j >= 0; j = next[j]) {
+      const int nn = cell_neigh(c, C->nc, C->st.data(), C->nst, nb.data());
+      for (int sidx = 0; sidx < nn; ++sidx)
+        for (int j = C->head[nb[sidx]]; j >= 0; j = C->next[j]) {
           if (j == ip) continue;
           int tp = tp_index(npt, S->ipt[ip], S->ipt[j]);
           double d[3] = {ri[0] - S->ro[3 * j], ri[1] - S->ro[3 * j + 1], ri[2] - S->ro[3 * j + 2]};
           double r2;
           pbc_r2(S->box, d, r2);
           if (!(r2 < rcut2)) continue;
-          if (pass == 0 && (r2 < S->r2atat[tp] || r2 < S->r2umin[tp])) return 0;  // overlap
+          if (pass == 0 && (r2 < S->r2atat[tp] || r2 < S->r2umin[tp])) return 1;  // overlap
           // weak charges: UWeakChargeANew/AOld skip uncharged pairs after the overlap checks
           // (Src/denergy.F90:3955-3959, 4066-4068)
           if (S->laz && (!S->laz[ip] || !S->laz[j])) continue;
@@ -1150,7 +1163,7 @@ int orc_dutotal(const OrcSys* S, int ip, const double* rnew, int* overlap, doubl
         double r2;
         pbc_r2(S->box, d, r2);
         if (r2 > rcut2) continue;
-        if (pass == 0 && (r2 < S->r2atat[tp] || r2 < S->r2umin[tp])) return 0;
+        if (pass == 0 && (r2 < S->r2atat[tp] || r2 < S->r2umin[tp])) return 1;
         if (S->laz && (!S->laz[ip] || !S->laz[j])) continue;
         double us, fs;
         int e = table_uf(S->ubuf, S->nbuf, S->iubuflow[tp], r2, 0.0, S->r2umin[tp], us, fs);
@@ -1158,6 +1171,27 @@ int orc_dutotal(const OrcSys* S, int ip, const double* rnew, int* overlap, doubl
         (pass == 0 ? unew : uold)[1 + tp] += us;
       }
     }
+  }
+  return 0;
+}
+
+int orc_dutotal(const OrcSys* S, int ip, const double* rnew, int* overlap, double* du_twob,
+                double* du_rec, double* du_tot) {
+  int np = S->np, npt = S->npt, nptpt = npt * (npt + 1) / 2;
+  std::vector<double> unew(nptpt + 1, 0.0), uold(nptpt + 1, 0.0);
+  *overlap = 1;
+  *du_rec = 0;
+  *du_tot = 0;
+  for (int k = 0; k <= nptpt; ++k) du_twob[k] = 0;
+  {
+    CellIndex C;
+    if (S->list == 1) {
+      int rc = cell_index_build(S, C);
+      if (rc) return rc;
+    }
+    int r = dut_real(S, S->list == 1 ? &C : nullptr, ip, rnew, unew.data(), uold.data());
+    if (r == 1) return 0;  // overlap
+    if (r) return r;
   }
   *overlap = 0;
   double s = 0;
@@ -1512,6 +1546,160 @@ int orc_celllist_update(int ip, const double* ro, const double* box, const int* 
   prev[ip] = -1;
   cellof[ip] = cnew;
   return 1;
+}
+
+// ----------------------------------------------------------------------------------
+// Trial energies the way MOLSIM's MC loop gets them (the CPU baseline of batched trials):
+// the configuration's cell list, structure factors S(k) (EwaldSetArray, Src/energy.F90:
+// 1401-1426), charge mesh and dipole stay resident, and each trial costs
+//   real space   the old and new positions' partners (UTwoBodyANew/AOld, as dut_real);
+//   std Ewald    O(nkvec): S_tm(k) = S(k) + q (e^{ik.rnew} - e^{ik.rold}) per octant sign
+//                combination and dE = sum kfac (|S_tm|^2 - |S|^2) (DUTwoBodyEwaldRecStd with
+//                EwaldSetArrayTM, Src/denergy.F90:1275-1335,1708-1731);
+//   SPME         QMeshTM = QMesh - old stencil + new stencil and one FFT energy
+//                (DUTwoBodyEwaldRecSPM, Src/denergy.F90:1413-1495);
+//   surface      the resident dipole (DUTwoBodyEwaldSurf, Src/denergy.F90:1504-1539).
+// Trials are independent (evaluated against the same configuration).
+struct OrcTrials {
+  OrcSys S;
+  CellIndex C;
+  std::vector<double> kfac;
+  int nk = 0;
+  std::vector<cplx> sk;  // 4 per k vector, in kfac order
+  std::vector<double> ef, Q;
+  double m[3] = {0, 0, 0};
+};
+
+// e^{i 2pi n x / L} for n = 0..ncut, by the recurrence orc_ewald_std_rec uses.
+static void eikr_line(double x, double tpb, int ncut, cplx* e) {
+  e[0] = cplx(1, 0);
+  if (ncut >= 1) e[1] = cplx(std::cos(tpb * x), std::sin(tpb * x));
+  for (int c = 2; c <= ncut; ++c) e[c] = e[c - 1] * e[1];
+}
+
+void* orc_trials_open(const OrcSys* S) {
+  OrcTrials* T = new OrcTrials;
+  T->S = *S;
+  if (S->list == 1 && cell_index_build(S, T->C)) {
+    delete T;
+    return nullptr;
+  }
+  if (S->ewald == 1) {
+    const int H = S->ncut + 1;
+    T->kfac.assign(4 * H * H * H, 0.0);
+    T->nk = orc_ewald_std_kfac(S->box, S->alpha, S->ncut, S->region, T->kfac.data(), (int)T->kfac.size());
+    if (T->nk < 0) {
+      delete T;
+      return nullptr;
+    }
+    std::vector<double> f(3 * S->np, 0.0), sr(8 * T->nk);
+    double u, v = 0;
+    orc_ewald_std_rec(S->np, S->ro, S->az, S->box, S->alpha, S->ncut, S->region, S->epsi4pi,
+                      T->kfac.data(), &u, f.data(), &v, sr.data());
+    T->sk.resize(4 * T->nk);
+    for (int q = 0; q < 4 * T->nk; ++q) T->sk[q] = cplx(sr[2 * q], sr[2 * q + 1]);
+  } else if (S->ewald == 2) {
+    const int K = S->K, H = K / 2 + 1;
+    const size_t g = (size_t)H * H * H;
+    std::vector<double> mf(g), vf(g);
+    T->ef.assign(g, 0.0);
+    T->Q.assign((size_t)K * K * K, 0.0);
+    if (orc_spme_setup(S->box, S->alpha, K, S->order, mf.data(), T->ef.data(), vf.data())) {
+      delete T;
+      return nullptr;
+    }
+    orc_spme_spread(S->np, S->ro, S->az, S->box, K, S->order, T->Q.data());
+  }
+  for (int a = 0; a < S->np; ++a)
+    for (int d = 0; d < 3; ++d) T->m[d] += S->az[a] * S->ro[3 * a + d];
+  return T;
+}
+
+void orc_trials_close(void* h) { delete (OrcTrials*)h; }
+
+int orc_trials_dutotal(void* h, int ip, const double* rnew, int* overlap, double* du_twob, double* du_rec,
+                       double* du_tot) {
+  OrcTrials* T = (OrcTrials*)h;
+  const OrcSys* S = &T->S;
+  const int np = S->np, npt = S->npt, nptpt = npt * (npt + 1) / 2;
+  if (ip < 0 || ip >= np) return ORC_ERR_ARG;
+  std::vector<double> unew(nptpt + 1, 0.0), uold(nptpt + 1, 0.0);
+  *overlap = 1;
+  *du_rec = 0;
+  *du_tot = 0;
+  for (int k = 0; k <= nptpt; ++k) du_twob[k] = 0;
+  const int r = dut_real(S, S->list == 1 ? &T->C : nullptr, ip, rnew, unew.data(), uold.data());
+  if (r == 1) return 0;
+  if (r) return r;
+  *overlap = 0;
+  double s = 0;
+  for (int k = 1; k <= nptpt; ++k) {
+    du_twob[k] = unew[k] - uold[k];
+    s += du_twob[k];
+  }
+  du_twob[0] = s;
+  double drec = 0;
+  const double qi = S->az[ip];
+  const double* ro = S->ro + 3 * ip;
+  if (S->ewald == 1) {
+    const int nc = S->ncut, ncut2 = S->region == 0 ? nc * nc : 3 * nc * nc;
+    std::vector<cplx> eo(3 * (nc + 1)), en(3 * (nc + 1));
+    for (int d = 0; d < 3; ++d) {
+      const double tpb = 2 * kPi / S->box[d];
+      eikr_line(ro[d], tpb, nc, eo.data() + d * (nc + 1));
+      eikr_line(rnew[d], tpb, nc, en.data() + d * (nc + 1));
+    }
+    const cplx* ox = eo.data();
+    const cplx* oy = ox + nc + 1;
+    const cplx* oz = oy + nc + 1;
+    const cplx* nx_ = en.data();
+    const cplx* ny_ = nx_ + nc + 1;
+    const cplx* nz_ = ny_ + nc + 1;
+    int kn = 0;
+    for (int nz = 0; nz <= nc; ++nz)
+      for (int ny = 0; ny <= nc; ++ny) {
+        if (ny * ny + nz * nz > ncut2) continue;
+        const cplx om = std::conj(oy[ny]) * oz[nz], op = oy[ny] * oz[nz];
+        const cplx nm = std::conj(ny_[ny]) * nz_[nz], npp = ny_[ny] * nz_[nz];
+        for (int nx = 0; nx <= nc; ++nx) {
+          if (nx * nx + ny * ny + nz * nz > ncut2) continue;
+          if (nx == 0 && ny == 0 && nz == 0) continue;
+          const cplx e_old[4] = {std::conj(ox[nx]) * om, std::conj(ox[nx]) * op, ox[nx] * om, ox[nx] * op};
+          const cplx e_new[4] = {std::conj(nx_[nx]) * nm, std::conj(nx_[nx]) * npp, nx_[nx] * nm, nx_[nx] * npp};
+          double term = 0;
+          for (int q = 0; q < 4; ++q) {
+            const cplx s0 = T->sk[4 * kn + q];
+            const cplx s1 = s0 + qi * (e_new[q] - e_old[q]);
+            term += std::norm(s1) - std::norm(s0);
+          }
+          drec += T->kfac[kn] * term;
+          ++kn;
+        }
+      }
+  } else if (S->ewald == 2) {
+    const int K = S->K;
+    const size_t n3 = (size_t)K * K * K;
+    std::vector<double> Qtm(T->Q), dq(n3);
+    const double uold_rec = orc_spme_energy(K, T->ef.data(), T->Q.data());
+    const double one_r[3] = {ro[0], ro[1], ro[2]};
+    const double mq = -qi;
+    orc_spme_spread(1, one_r, &mq, S->box, K, S->order, dq.data());
+    for (size_t i = 0; i < n3; ++i) Qtm[i] += dq[i];
+    orc_spme_spread(1, rnew, &qi, S->box, K, S->order, dq.data());
+    for (size_t i = 0; i < n3; ++i) Qtm[i] += dq[i];
+    drec = orc_spme_energy(K, T->ef.data(), Qtm.data()) - uold_rec;
+  }
+  if (S->ewald && S->surf) {
+    const double vol = S->box[0] * S->box[1] * S->box[2];
+    const double fac = 2 * kPi / (3 * vol);
+    double mt[3];
+    for (int d = 0; d < 3; ++d) mt[d] = T->m[d] + qi * (rnew[d] - ro[d]);
+    drec += fac * ((mt[0] * mt[0] + mt[1] * mt[1] + mt[2] * mt[2]) -
+                   (T->m[0] * T->m[0] + T->m[1] * T->m[1] + T->m[2] * T->m[2]));
+  }
+  *du_rec = S->epsi4pi * drec;
+  *du_tot = du_twob[0] + *du_rec;
+  return 0;
 }
 
 }  // extern "C"
