@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define MSG_MAX_TYPES 4
+#define MSG_MAX_TYPES 8
 
 enum msg_status {
   MSG_OK = 0,

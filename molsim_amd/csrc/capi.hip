@@ -366,7 +366,7 @@ int msg_create(const msg_params* p, msg_ctx** out) {
   if (!p || !out) return set_error(MSG_ERR_ARG, "msg_create: null argument");
   *out = nullptr;
   if (p->np < 1 || p->npt < 1 || p->npt > MSG_MAX_TYPES)
-    return set_error(MSG_ERR_ARG, "msg_create: np >= 1 and 1 <= npt <= 4 required");
+    return set_error(MSG_ERR_ARG, "msg_create: np >= 1 and 1 <= npt <= 8 required");
   for (int d = 0; d < 3; ++d)
     if (!(p->boxlen[d] > 0)) return set_error(MSG_ERR_ARG, "msg_create: box lengths must be > 0");
   if (!(p->rcut > 0)) return set_error(MSG_ERR_CELL, "msg_create: rcut must be > 0");

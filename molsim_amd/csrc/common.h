@@ -39,6 +39,8 @@ constexpr int kTabBins = 256;        // r^2 bins of the interval index per type 
 // this offset, so the pair kernels see the charge state in the record they load anyway.
 constexpr int kUncharged = 8;
 constexpr int kMaxTabIntervals = 512;   // records below rcut staged in LDS (<= 48 KB in FP64)
+constexpr int kLdsTypes = 4;            // particle types whose table the LDS-staging kernels take
+constexpr int kMaxTabIntervalsMany = 4096;  // records of a table of more types (read from memory)
 constexpr int kMaxStencil = 27;
 constexpr int kPairHB = 4;           // home particles per pair-kernel work item
 // SPME bricks: each mesh point belongs to one brick; spread and gather work per brick.  Meshes
@@ -272,6 +274,7 @@ int pair_blocks(Ctx* c);                           // workgroups of the pair ker
 bool pair_nbpar(const Ctx* c);                     // neighbour-parallel kernel in use
 bool pair_tiled(const Ctx* c);                     // cell-tile kernel in use
 int pair_tile_tx(const Ctx* c);                    // its cells per tile along x
+bool pair_many(const Ctx* c);                      // more types than the LDS-table kernels take
 int default_pair_tile(const Ctx* c);               // lanes per home for a new context
 int spme_setup(Ctx* c);                            // K-space arrays
 int launch_spme_mesh(Ctx* c);                      // spread, FFTs + influence function

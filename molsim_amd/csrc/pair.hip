@@ -1552,15 +1552,133 @@ __global__ __launch_bounds__(256) void k_vlist_export(VlExportArgs a) {
 // up to 8 workgroups per CU walks them with a stride.
 // The neighbour-parallel kernel serves every cell-list and all-pairs system (any stencil:
 // 27 cells, or fewer where dimensions collapse to one cell) unless tile 0 is chosen.
+namespace {
+
+// Pair kernel of systems with more than kLdsTypes particle types (up to MSG_MAX_TYPES, so up
+// to 36 type pairs): the table's records and bins are read from global memory (they no longer
+// fit the LDS of a workgroup) and every pair is evaluated in FP64, one thread per home, over
+// the cell list's stencil or the home's Verlet-list row.  The pair rules are UTwoBodyACellList
+// / UTwoBodyA's (PBCr2 single shift, r2 < rcut2 or, for all pairs and Verlet lists,
+// r2 <= rcut2, hard-core emulation below r2atat, an error below r2umin).  Per-type-pair
+// energies and the virial go to the block's partial row as in the other pair kernels.
+struct ManyArgs {
+  PairArgs p;
+  const double* coef;   // FP64 records (12 per interval), first/last per type pair in p.tm
+  int verlet;
+  const int* vl_home;
+  const int* vl_n;
+  const int* vl_nbr;
+  const double4* vl_pos;
+};
+
+__device__ __forceinline__ void many_eval(const ManyArgs& g, const double4& hp, int ht, const double4& pj,
+                                          double* et, double& vir, double* f, int& err) {
+  const PairArgs& a = g.p;
+  const int tj = (int)pj.w;
+  if (tj >= kUncharged) return;  // weak charges: uncharged partner
+  double d[3] = {hp.x - pj.x, hp.y - pj.y, hp.z - pj.z};
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    if (fabs(d[q]) > a.box.half[q]) d[q] -= copysign(a.box.L[q], d[q]);
+  const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  if (a.cut_inclusive ? (r2 > a.rcut2) : !(r2 < a.rcut2)) return;
+  const int tp = a.tm.tpidx[ht * kMaxT + tj];
+  double us, fs;
+  if (r2 < a.tm.r2atat[tp]) {
+    us = 1e10;
+    fs = 1e10;
+  } else if (r2 < a.tm.r2umin[tp]) {
+    err |= FLAG_R2UMIN;
+    return;
+  } else {
+    int b = (int)((r2 - a.tm.zlo[tp]) * a.tm.invw[tp]);
+    b = min(max(b, 0), kTabBins - 1);
+    int ti = a.tbins[tp * kTabBins + b];
+    const int last = a.tm.last[tp];
+    while (ti < last && r2 < g.coef[12 * ti]) ++ti;
+    const double* t = g.coef + 12 * ti;
+    const double dd = r2 - t[0];
+    us = t[1] + dd * (t[2] + dd * (t[3] + dd * (t[4] + dd * (t[5] + dd * t[6]))));
+    fs = t[7] + dd * (t[8] + dd * (t[9] + dd * (t[10] + dd * t[11])));
+  }
+#pragma unroll
+  for (int q = 0; q < kMaxT; ++q) et[q] += q == tj ? us : 0.0;
+  f[0] += fs * d[0];
+  f[1] += fs * d[1];
+  f[2] += fs * d[2];
+  vir -= fs * r2;
+}
+
+__global__ __launch_bounds__(256) void k_pair_many(ManyArgs g) {
+  const PairArgs& a = g.p;
+  __shared__ double s_red[4][kMaxTP + 1];
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double et[kMaxT], vir = 0.0, f[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < kMaxT; ++q) et[q] = 0.0;
+  int err = 0, ht = 0;
+  if (s < a.np) {
+    const double4 hp = g.verlet ? g.vl_pos[s] : a.cpos[s];
+    const int hw = (int)hp.w;
+    ht = hw & (kUncharged - 1);
+    if (hw < kUncharged) {  // an uncharged home has no pair energy
+      if (g.verlet) {
+        const int n = g.vl_n[s];
+        for (int k = 0; k < n; ++k) many_eval(g, hp, ht, g.vl_pos[g.vl_nbr[(size_t)k * a.np + s]], et, vir, f, err);
+      } else {
+        const int c = a.slot_cell[s];
+        const int cx = c % a.g.nc[0], cy = (c / a.g.nc[0]) % a.g.nc[1], cz = c / (a.g.nc[0] * a.g.nc[1]);
+        for (int k = 0; k < a.g.nst; ++k) {
+          int x = cx + a.g.st[3 * k], y = cy + a.g.st[3 * k + 1], z = cz + a.g.st[3 * k + 2];
+          x += x < 0 ? a.g.nc[0] : (x >= a.g.nc[0] ? -a.g.nc[0] : 0);
+          y += y < 0 ? a.g.nc[1] : (y >= a.g.nc[1] ? -a.g.nc[1] : 0);
+          z += z < 0 ? a.g.nc[2] : (z >= a.g.nc[2] ? -a.g.nc[2] : 0);
+          const int nb = x + a.g.nc[0] * (y + a.g.nc[1] * z);
+          const int j1 = a.cell_start[nb + 1];
+          for (int j = a.cell_start[nb]; j < j1; ++j)
+            if (j != s) many_eval(g, hp, ht, a.cpos[j], et, vir, f, err);
+        }
+      }
+    }
+    const int ip = g.verlet ? g.vl_home[s] : a.cell_ip[s];
+    a.force[3 * ip + 0] = f[0] * a.scale;
+    a.force[3 * ip + 1] = f[1] * a.scale;
+    a.force[3 * ip + 2] = f[2] * a.scale;
+  }
+  if (err) atomicOr(a.flags, err);
+  const int ntp = a.tm.ntp;
+  for (int q = 0; q < ntp; ++q) {
+    double v = 0.0;
+#pragma unroll
+    for (int t = 0; t < kMaxT; ++t) v += a.tm.tpidx[ht * kMaxT + t] == q ? et[t] : 0.0;
+    v = msgd::wave_sum(v);
+    if (lane == 0) s_red[wv][q] = v;
+  }
+  vir = msgd::wave_sum(vir);
+  if (lane == 0) s_red[wv][ntp] = vir;
+  __syncthreads();
+  if (threadIdx.x <= ntp) {
+    double t = 0.0;
+    for (int w = 0; w < 4; ++w) t += s_red[w][threadIdx.x];
+    a.part[blockIdx.x * (ntp + 1) + threadIdx.x] = t;
+  }
+}
+
+}  // namespace
+
+// Systems with more particle types than the LDS-table kernels take.
+bool pair_many(const Ctx* c) { return c->npt > kLdsTypes; }
+
 bool pair_nbpar(const Ctx* c) {
   // the mixed-path kernel codes a slot in 27 bits: larger systems take the work-item kernel
-  return c->P.list_mode != MSG_LIST_VERLET && c->pair_lph > 0 && !pair_tiled(c) &&
+  return c->P.list_mode != MSG_LIST_VERLET && c->pair_lph > 0 && !pair_tiled(c) && !pair_many(c) &&
          (c->fp64 || c->np < (1 << kNbSlotBits));
 }
 
 // Cell-tile kernel (tile codes 100 + lanes per home): mixed path, cell lists.
 bool pair_tiled(const Ctx* c) {
-  return c->P.list_mode == MSG_LIST_CELL && !c->fp64 && c->pair_lph >= 100;
+  return c->P.list_mode == MSG_LIST_CELL && !c->fp64 && c->pair_lph >= 100 && !pair_many(c);
 }
 
 // Lanes per home of the neighbour-parallel kernel (a cell-tile code on the FP64 path runs it
@@ -1593,7 +1711,7 @@ int default_pair_tile(const Ctx* c) {
 }
 
 int pair_blocks(Ctx* c) {
-  if (c->P.list_mode == MSG_LIST_VERLET) return (c->np + 255) / 256;
+  if (c->P.list_mode == MSG_LIST_VERLET || pair_many(c)) return (c->np + 255) / 256;
   if (pair_tiled(c)) {
     const int tx = pair_tile_tx(c);
     return c->grid.nc[1] * c->grid.nc[2] * ((c->grid.nc[0] + tx - 1) / tx);
@@ -1650,6 +1768,21 @@ int launch_pair(Ctx* c, bool last) {
   const int nb = c->npart_pair;
   const size_t lds = pair_lds(c);
   prof_begin(c, MSG_K_PAIR);
+  if (pair_many(c)) {
+    ManyArgs g;
+    g.p = a;
+    g.coef = c->d_tcoef64;
+    g.verlet = c->P.list_mode == MSG_LIST_VERLET;
+    if (g.verlet) g.p.cut_inclusive = 1;  // UTwoBodyA's r2 > rcut2 test
+    g.vl_home = c->d_vl_home;
+    g.vl_n = c->d_vl_n;
+    g.vl_nbr = c->d_vl_nbr;
+    g.vl_pos = (const double4*)c->d_vl_pos;
+    hipLaunchKernelGGL(k_pair_many, dim3(nb), dim3(256), 0, c->stream, g);
+    prof_end(c, MSG_K_PAIR);
+    MSG_HIP(hipGetLastError());
+    return MSG_OK;
+  }
   const bool nbpar = pair_nbpar(c);
   if (c->P.list_mode == MSG_LIST_VERLET) {
     VlPairArgs v;
@@ -1817,6 +1950,7 @@ int plan_pair(const std::vector<Ctx*>& cs, SegOwner& own, SegStage& out) {
   SegBuilder<PairArgs> b;
   for (Ctx* c : cs) {
     if (c->P.list_mode == MSG_LIST_VERLET) return set_error(MSG_ERR_ARG, "batch: Verlet-list systems run unbatched");
+    if (pair_many(c)) return set_error(MSG_ERR_ARG, "batch: systems of more than 4 particle types run unbatched");
     const int ntp = c->nptpt <= 1 ? 1 : c->nptpt <= 3 ? 3 : c->nptpt <= 6 ? 6 : 10;
     if (pair_tiled(c)) {
       const int key = 200000 + ntp * 1000 + c->pair_lph;

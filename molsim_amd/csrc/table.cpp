@@ -183,8 +183,10 @@ int pack_table(Ctx* c) {
   }
   m.nint = (int)coef.size() / 12;
   m.ntp = ntp;
-  if (m.nint > kMaxTabIntervals)
-    return set_error(MSG_ERR_TABLE_MEM, "msg_set_table: more than 512 intervals below rcut");
+  const int cap = c->npt > kLdsTypes ? kMaxTabIntervalsMany : kMaxTabIntervals;
+  if (m.nint > cap)
+    return set_error(MSG_ERR_TABLE_MEM, c->npt > kLdsTypes ? "msg_set_table: more than 4096 intervals below rcut"
+                                                           : "msg_set_table: more than 512 intervals below rcut");
   for (int i = 0; i < kMaxT; ++i)
     for (int j = 0; j < kMaxT; ++j)
       m.tpidx[i * kMaxT + j] = (i < c->npt && j < c->npt) ? tp_index(c->npt, i, j) : 0;
@@ -201,8 +203,9 @@ int pack_table(Ctx* c) {
   }
   const size_t rs = c->fp64 ? sizeof(double) : sizeof(float);
   if (!c->d_tcoef) {
-    if (int e = dalloc(c, &c->d_tcoef, kMaxTabIntervals * 12 * sizeof(double))) return e;
-    if (int e = dalloc(c, (void**)&c->d_tcoef64, kMaxTabIntervals * 12 * sizeof(double))) return e;
+    const size_t cap_b = (size_t)cap * 12 * sizeof(double);
+    if (int e = dalloc(c, &c->d_tcoef, cap_b)) return e;
+    if (int e = dalloc(c, (void**)&c->d_tcoef64, cap_b)) return e;
     if (int e = dalloc(c, (void**)&c->d_tbins, kMaxTP * kTabBins * sizeof(int))) return e;
   }
   if (c->fp64) {
