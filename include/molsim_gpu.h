@@ -294,9 +294,11 @@ int msg_unit_results(msg_unit* unit, double* u_tot, double* u_rec, double* viria
 int msg_query(msg_ctx* ctx);
 
 /* Pair-kernel tile of a cell-list system (no counterpart in the reference): pair_lanes lanes
- * per home particle, 256 / pair_lanes homes per workgroup (8, 16, 32 or 64; 32 by default), or
- * 0 for the wave-per-cell-chunk kernel.  Results do not depend on the tile beyond FP32
- * summation order.  Set by the autotuner per size bucket. */
+ * per home particle, 256 / pair_lanes homes per workgroup (4, 8, 16, 32 or 64; the default
+ * follows the expected candidates per home), 0 for the wave-per-cell-chunk kernel, or 108 /
+ * 116 for the cell-tile kernel (mixed precision; 8 or 16 lanes per home, a tile's neighbour
+ * cells staged in LDS).  Results do not depend on the tile beyond FP32 summation order.  Set
+ * by the autotuner per size bucket; systems of more than 4 particle types ignore it. */
 int msg_set_tile(msg_ctx* ctx, int pair_lanes);
 int msg_get_tile(msg_ctx* ctx);
 

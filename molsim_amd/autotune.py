@@ -6,8 +6,9 @@ ensemble).
 
 The pair kernel of a cell-list system takes its tile as a run-time choice
 (``msg_set_tile``): ``pair_lanes`` lanes of a wavefront share one home particle's
-27-cell candidate list (256 / pair_lanes homes per workgroup), or 0 for the
-wave-per-cell-chunk kernel.  Few lanes per home suit small, sparse systems (more homes
+27-cell candidate list (256 / pair_lanes homes per workgroup; 4 to 64), 0 for the
+wave-per-cell-chunk kernel, or 108 / 116 for the cell-tile kernel (8 / 16 lanes per home,
+the tile's neighbour cells staged in LDS; mixed precision only).  Few lanes per home suit small, sparse systems (more homes
 per workgroup, fewer idle lanes); many lanes suit dense candidate lists.  The best choice
 depends on the particle count and density, so the tuner times every tile on one
 representative system per size bucket (the median-sized member) with per-kernel event
@@ -21,7 +22,7 @@ import bisect
 import json
 import os
 
-TILES = (8, 16, 32, 64, 0)
+TILES = (4, 8, 16, 32, 64, 0, 108, 116)
 # bucket b holds systems with BUCKET_EDGES[b-1] <= n < BUCKET_EDGES[b]
 BUCKET_EDGES = (300, 600, 1200, 2500, 5000, 10000, 20000, 50000, 100000)
 

@@ -785,9 +785,9 @@ void* msg_stream(msg_ctx* h) { return h ? (void*)h->c.stream : nullptr; }
 
 int msg_set_tile(msg_ctx* h, int pair_lanes) {
   if (!h) return set_error(MSG_ERR_ARG, "msg_set_tile: null context");
-  if (pair_lanes != 0 && pair_lanes != 8 && pair_lanes != 16 && pair_lanes != 32 && pair_lanes != 64 &&
-      pair_lanes != 108 && pair_lanes != 116)
-    return set_error(MSG_ERR_ARG, "msg_set_tile: pair_lanes must be 0, 8, 16, 32, 64, 108 or 116");
+  if (pair_lanes != 0 && pair_lanes != 4 && pair_lanes != 8 && pair_lanes != 16 && pair_lanes != 32 &&
+      pair_lanes != 64 && pair_lanes != 108 && pair_lanes != 116)
+    return set_error(MSG_ERR_ARG, "msg_set_tile: pair_lanes must be 0, 4, 8, 16, 32, 64, 108 or 116");
   Ctx* c = &h->c;
   MSG_HIP(hipStreamSynchronize(c->stream));
   c->pair_lph = pair_lanes;

@@ -261,19 +261,18 @@ __device__ __forceinline__ void de_body(const DeArgs& a, int ip, const double* t
     }
   }
   drec = msgd::block_sum(drec, sh);
-  double out_tp[kMaxTP];
+  double tw = 0.0;  // thread 0: the real-space sum over type pairs
   for (int tp = 0; tp < a.ntp; ++tp) {
     double v = 0.0;
 #pragma unroll
     for (int k = 0; k < kMaxT; ++k) v += (a.tm.tpidx[ti * kMaxT + k] == tp) ? e[k] : 0.0;
-    out_tp[tp] = msgd::block_sum(v, sh);
+    v = msgd::block_sum(v, sh);
+    if (threadIdx.x == 0) {
+      o[1 + tp] = v;
+      tw += v;
+    }
   }
   if (threadIdx.x == 0) {
-    double tw = 0.0;
-    for (int tp = 0; tp < a.ntp; ++tp) {
-      o[1 + tp] = out_tp[tp];
-      tw += out_tp[tp];
-    }
     o[0] = tw;
     if (a.ewald && a.lsurf) {
       const double M[3] = {a.Mdip[0], a.Mdip[1], a.Mdip[2]};
@@ -423,15 +422,18 @@ __global__ __launch_bounds__(256) void k_dutotal_charge(DeChargeArgs d) {
     }
   }
   drec = msgd::block_sum(drec, sh);
-  double out_tp[kMaxTP];
-  for (int tp = 0; tp < a.ntp; ++tp) out_tp[tp] = msgd::block_sum(e[tp], sh);
+  double tw = 0.0;  // thread 0: the real-space sum over type pairs
+#pragma unroll
+  for (int tp = 0; tp < kMaxTP; ++tp) {
+    if (tp >= a.ntp) break;
+    const double v = msgd::block_sum(e[tp], sh);
+    if (threadIdx.x == 0) {
+      a.out[1 + tp] = v;
+      tw += v;
+    }
+  }
   if (threadIdx.x == 0) {
     double* o = a.out;
-    double tw = 0.0;
-    for (int tp = 0; tp < a.ntp; ++tp) {
-      o[1 + tp] = out_tp[tp];
-      tw += out_tp[tp];
-    }
     o[0] = tw;
     if (a.ewald) {
       for (int m = 0; m < d.nptm; ++m) {

@@ -629,7 +629,7 @@ __device__ __forceinline__ void pair_nb_rel_body(const PairArgs& a, int bid) {
   constexpr int HPB = 256 / LPH;
   constexpr int MPL = (kMaxStencil + LPH - 1) / LPH;
   constexpr int KU = kNbRound;
-  constexpr int CAP = LPH * KU > 128 ? LPH * KU : 128;  // list entries per home
+  constexpr int CAP = LPH * KU > 128 ? LPH * KU : 128;  // list entries per home (a chunk)
   constexpr int PB = kNbEval;
   constexpr unsigned kMask = (1u << kNbSlotBits) - 1;
   constexpr int NT = PairAcc<float, NTP>::NT;
@@ -711,7 +711,7 @@ __device__ __forceinline__ void pair_nb_rel_body(const PairArgs& a, int bid) {
     tot += cnt[m];
   }
   int ncand, incl;
-  if constexpr (LPH <= 16) {
+  if constexpr (LPH == 8 || LPH == 16) {
     incl = seg_scan<LPH>(tot, k, ncand);
   } else {
     incl = tot;
@@ -768,7 +768,7 @@ __device__ __forceinline__ void pair_nb_rel_body(const PairArgs& a, int bid) {
       }
       const int nhit = __popc(hm);
       int ng, hi;
-      if constexpr (LPH <= 16) {
+      if constexpr (LPH == 8 || LPH == 16) {
         hi = seg_scan<LPH>(nhit, k, ng);
       } else {
         hi = nhit;
@@ -1826,7 +1826,9 @@ int launch_pair(Ctx* c, bool last) {
   const bool nbp = nbpar || c->pair_lph >= 100;
 #define MSG_PAIR_LAUNCH(R, NTP)                                                              \
   do {                                                                                       \
-    if (nbp && lph == 8)                                                                     \
+    if (nbp && lph == 4)                                                                     \
+      hipLaunchKernelGGL((k_pair_nb<R, NTP, 4>), dim3(nb), dim3(256), lds, c->stream, a);   \
+    else if (nbp && lph == 8)                                                                \
       hipLaunchKernelGGL((k_pair_nb<R, NTP, 8>), dim3(nb), dim3(256), lds, c->stream, a);   \
     else if (nbp && lph == 16)                                                               \
       hipLaunchKernelGGL((k_pair_nb<R, NTP, 16>), dim3(nb), dim3(256), lds, c->stream, a);  \
@@ -1909,6 +1911,7 @@ void seg_pair_items(const SegGroup& g, hipStream_t s) {
 template <typename real, int NTP>
 SegLaunchFn pair_fn(int lph) {
   switch (lph) {
+    case 4: return seg_pair_nb<real, NTP, 4>;
     case 8: return seg_pair_nb<real, NTP, 8>;
     case 16: return seg_pair_nb<real, NTP, 16>;
     case 32: return seg_pair_nb<real, NTP, 32>;

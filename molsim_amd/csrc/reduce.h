@@ -72,18 +72,12 @@ __device__ inline void reduce_body(const ReduceArgs& a) {
   const int nev = a.ewald == 2 ? a.nspme : (a.ewald == 1 ? a.newstd : 0);
   const int np1 = a.ntp + 1;
   double m[3] = {0.0, 0.0, 0.0};
-  double pt[kMaxTP + 1];
-#pragma unroll
-  for (int k = 0; k <= kMaxTP; ++k) pt[k] = 0.0;
   int fl = 0;
   if (tid == 0) {
     m[0] = a.M[0];
     m[1] = a.M[1];
     m[2] = a.M[2];
     fl = a.flags[0];
-    if (a.npair == 1)
-#pragma unroll
-      for (int k = 0; k <= kMaxTP; ++k) pt[k] = k < np1 ? a.pair[k] : 0.0;
   }
 #pragma unroll
   for (int k = 0; k < R; ++k) {
@@ -104,12 +98,12 @@ __device__ inline void reduce_body(const ReduceArgs& a) {
       wred[(tid >> 6) + NW * k][1] = e1;
     }
   }
+  // the pair row totals: one row already, or summed here (into LDS)
+  const double* pt = a.pair;
   if constexpr (NT == 1024) {
     if (a.npair != 1) {
       pair_rows_total(a.pair, a.npair, np1, red);
-      if (tid == 0)
-#pragma unroll
-        for (int k = 0; k <= kMaxTP; ++k) pt[k] = k < np1 ? red[k] : 0.0;
+      pt = red;
     }
   }
   __syncthreads();

@@ -80,7 +80,10 @@ def test_tiles_agree(members, table):
             base = r
         else:
             ferr = np.linalg.norm(r["force"] - base["force"]) / np.linalg.norm(base["force"])
-            assert ferr <= 1e-6, (t, ferr)
+            # the tiles differ in FP32 summation order; the cell-tile kernel (>= 100) also in
+            # its coordinates (relative to the tile's first cell, up to 14 cells away), which
+            # rounds the displacements differently
+            assert ferr <= (5e-6 if t >= 100 else 1e-6), (t, ferr)
             assert abs(r["u_tot"] - base["u_tot"]) <= 1e-7 * abs(base["u_tot"]) + 1e-6
     e.close()
 

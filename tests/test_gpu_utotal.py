@@ -130,7 +130,7 @@ def test_r2umin_error_is_reported():
     eng.close()
 
 
-@pytest.mark.parametrize("tile", [116, 108, 8, 0])
+@pytest.mark.parametrize("tile", [116, 108, 4, 8, 64, 0])
 @pytest.mark.parametrize("case", ["poly", "collapsed", "dense"])
 def test_pair_tiles_match_oracle(case, tile):
     """Every pair-kernel tile (the cell-tile kernel with 16 or 8 lanes per home, the
