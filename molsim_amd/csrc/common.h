@@ -391,6 +391,15 @@ struct SegOwner {
 // ~100 times).  Slots an XCD has no work for are map entries with system -1.
 constexpr int kXcd = 8;
 
+// Workgroup b of a single-system launch of n workgroups runs on XCD b mod 8 (round-robin
+// dispatch); renumbered so that XCD x takes the x-th contiguous run of the logical workgroups,
+// consecutive work items (spatially close in cell or brick order) share an L2 and the data
+// they read around them is fetched into one L2 instead of eight.
+__device__ inline int xcd_logical(int b, int n) {
+  const int x = b % kXcd, k = b / kXcd, q = n / kXcd, r = n % kXcd;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 // Collects the argument blocks of one stage per template variant, then uploads them.
 template <class A>
 struct SegBuilder {

@@ -547,14 +547,6 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
   }
 }
 
-// Workgroup b of a single-system launch of n workgroups runs on XCD b mod 8 (round-robin
-// dispatch); renumbered so that XCD x takes the x-th contiguous run of the logical workgroups,
-// consecutive homes (spatially close in cell order) share an L2 and their neighbour cells'
-// records are fetched into one L2 instead of eight.
-__device__ inline int xcd_logical(int b, int n) {
-  const int x = b % kXcd, k = b / kXcd, q = n / kXcd, r = n % kXcd;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
-}
 
 constexpr int kNbRound = 6;    // candidates per lane and test round (loads in flight)
 constexpr int kNbEval = 4;     // in-cut pairs per lane evaluated together (loads in flight)

@@ -1109,14 +1109,16 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int bid) {
 }
 
 template <typename real, int P>
-__global__ __launch_bounds__(256) void k_gather(GatherArgs a) { gather_body<real, P>(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_gather(GatherArgs a) {
+  gather_body<real, P>(a, xcd_logical(blockIdx.x, gridDim.x));
+}
 
 template <typename real, int P>
 constexpr int gather_lanes() { return P <= 8 ? 8 : 16; }
 
 template <typename real, int P>
 __global__ __launch_bounds__(256) void k_gather_g(GatherArgs a) {
-  gather_body<real, P, gather_lanes<real, P>()>(a, blockIdx.x);
+  gather_body<real, P, gather_lanes<real, P>()>(a, xcd_logical(blockIdx.x, gridDim.x));
 }
 
 template <typename real, int P>
@@ -1132,7 +1134,7 @@ __global__ __launch_bounds__(256) void k_gather_reduce_g(GatherArgs a, ReduceArg
   if (blockIdx.x == gridDim.x - 1)
     reduce_body<256>(r);
   else
-    gather_body<real, P, gather_lanes<real, P>()>(a, blockIdx.x);
+    gather_body<real, P, gather_lanes<real, P>()>(a, xcd_logical(blockIdx.x, gridDim.x - 1));
 }
 
 // The gather with the pass's final reduction as one extra (last) workgroup.  The reduction
@@ -1144,7 +1146,7 @@ __global__ __launch_bounds__(256) void k_gather_reduce(GatherArgs a, ReduceArgs 
   if (blockIdx.x == gridDim.x - 1)
     reduce_body<256>(r);
   else
-    gather_body<real, P>(a, blockIdx.x);
+    gather_body<real, P>(a, xcd_logical(blockIdx.x, gridDim.x - 1));
 }
 
 template <typename real, int P>
