@@ -1,7 +1,8 @@
 !> Fortran caller of the C-ABI through the MolsimGpu module, in the way a MOLSIM build calls
 !! it: reads a system (positions, charges, types, box, Ewald parameters) from a text file,
-!! builds the table (PotTwoBodyTab semantics), runs UTotal, a few DUTotal trials, one
-!! MCUpdate and a short MCPass, and prints the results for the caller to compare.
+!! builds the table (PotTwoBodyTab semantics), runs UTotal, a few DUTotal trials (one at a time
+!! and batched), reads the cell list, runs one MCUpdate and a short MCPass, and prints the
+!! results for the caller to compare.
 !!
 !! Input file (free format): np npt / boxlen(3) / rcut alpha epsi4pi / nmesh order lsurf /
 !! form lewald / z(npt) / sig(npt) / eps(npt) / rad(npt) / (ro(1:3,ip), az(ip), iptpn(ip), ip=1,np)
@@ -22,7 +23,9 @@ program drop_in_driver
    real(c_double), allocatable :: u_twob(:), du_twob(:)
    real(c_double) :: boxlen(3), rcut, alpha, epsi4pi, u_rec, u_tot, virial, du_rec, du_tot
    integer(c_int) :: iseed, rngstate(2), naccept
-   integer(c_int), allocatable :: events(:)
+   integer(c_int), allocatable :: events(:), ovlb(:), cell_start(:), cell_ip(:)
+   integer(c_int) :: ncl(3)
+   real(c_double), allocatable :: dutb(:), durb(:)
    real(c_double) :: am, dusum, dtran(8)
 
    call get_command_argument(1, fin)
@@ -103,6 +106,18 @@ program drop_in_driver
                     'msg_dutotal')
       write(*,'(a,i6,i3,es24.16)') 'dutotal ', it, ovl, du_tot
    end do
+
+   ! the same trials as one batched call
+   allocate(ovlb(ntrial), dutb(ntrial), durb(ntrial))
+   call GpuCheck(msg_dutotal_batch(ctx, int(ntrial, c_int), iptrial, rtrial, ovlb, dutb, durb), 'msg_dutotal_batch')
+   do it = 1, ntrial
+      write(*,'(a,i6,i3,es24.16)') 'dutotal_batch ', it, ovlb(it), dutb(it)
+   end do
+
+   ! the cell list MOLSIM's linked lists would hold
+   allocate(cell_start(prm%np + 100000), cell_ip(prm%np))
+   call GpuCheck(msg_get_cell_list(ctx, ncl, cell_start, cell_ip), 'msg_get_cell_list')
+   write(*,'(a,3i6,i10)') 'cells ', ncl, sum(int(cell_ip, 8) * [(int(i, 8), i = 1, prm%np)])
 
    ! MCUpdate of the first trial, then UTotal again
    call GpuCheck(msg_commit(ctx, 1_c_int, iptrial(1:1), rtrial(:,1:1)), 'msg_commit')

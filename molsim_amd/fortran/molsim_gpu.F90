@@ -99,6 +99,47 @@ module MolsimGpu
          type(c_ptr), value :: ctx
       end function
 
+      integer(c_int) function msg_get_cell_list(ctx, nc, cell_start, cell_ip) bind(C, name='msg_get_cell_list')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), intent(out) :: nc(3), cell_start(*), cell_ip(*)
+      end function
+
+      ! nmlIntList's inlist (nlist.F90:88-95,615-618): 0 skin test, > 0 fixed interval
+      integer(c_int) function msg_nlist_interval(ctx, inlist) bind(C, name='msg_nlist_interval')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: inlist
+      end function
+
+      ! maxnneigh (nlist.F90:240-246); 0 sizes the list from the density
+      integer(c_int) function msg_set_nlist_width(ctx, maxnneigh) bind(C, name='msg_set_nlist_width')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: maxnneigh
+      end function
+
+      ! NList(iBeforeMacrostep) (nlist.F90:593-595)
+      integer(c_int) function msg_nlist_macrostep(ctx) bind(C, name='msg_nlist_macrostep')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
+      integer(c_long_long) function msg_nlist_builds(ctx) bind(C, name='msg_nlist_builds')
+         import :: c_long_long, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
+      ! the Verlet list in MOLSIM's layout: kind 0 SetVListMD, 1 SetVListMC (nlist.F90:724-884)
+      integer(c_int) function msg_get_verlet_list(ctx, kind, maxnneigh, nneighpn, jpnlist) &
+            bind(C, name='msg_get_verlet_list')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: kind, maxnneigh
+         integer(c_int), intent(out) :: nneighpn(*)
+         integer(c_int), intent(out) :: jpnlist(maxnneigh,*)
+      end function
+
       integer(c_int) function msg_utotal(ctx, u_twob, u_rec, u_tot, force, virial) &
             bind(C, name='msg_utotal')
          import :: c_int, c_double, c_ptr
@@ -119,6 +160,18 @@ module MolsimGpu
          integer(c_int), intent(out) :: lhsoverlap
          real(c_double), intent(out) :: du_twob(0:*)
          real(c_double), intent(out) :: du_rec, du_tot
+      end function
+
+      ! ntrial independent trials against the same state; outputs per trial
+      integer(c_int) function msg_dutotal_batch(ctx, ntrial, ip, rotm, lhsoverlap, du_tot, du_rec) &
+            bind(C, name='msg_dutotal_batch')
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: ntrial
+         integer(c_int), intent(in) :: ip(*)
+         real(c_double), intent(in) :: rotm(3,*)
+         integer(c_int), intent(out) :: lhsoverlap(*)
+         real(c_double), intent(out) :: du_tot(*), du_rec(*)
       end function
 
       integer(c_int) function msg_commit(ctx, nptm, ipnptm, rotm) bind(C, name='msg_commit')

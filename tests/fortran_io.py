@@ -31,15 +31,17 @@ def run(path):
 
 
 def parse(out):
-    res = {"force": {}, "dutotal": []}
+    res = {"force": {}, "dutotal": [], "dutotal_batch": []}
     for line in out.splitlines():
         t = line.split()
         if not t:
             continue
         if t[0] == "force":
             res["force"][int(t[1]) - 1] = [float(x) for x in t[2:5]]
-        elif t[0] == "dutotal":
-            res["dutotal"].append((int(t[2]), float(t[3])))
+        elif t[0] in ("dutotal", "dutotal_batch"):
+            res[t[0]].append((int(t[2]), float(t[3])))
+        elif t[0] == "cells":
+            res["cells"] = ([int(x) for x in t[1:4]], int(t[4]))
         elif t[0] in ("u_tot", "u_rec", "u_twob0", "virial", "u_tot_after"):
             res[t[0]] = float(t[1])
         elif t[0] == "mc_pass":

@@ -37,6 +37,14 @@ def test_fortran_driver_matches_oracle(tmp_path):
         assert bool(ovl) == d["overlap"]
         if not ovl:
             assert abs(du - d["du_tot"]) < 2e-5 + 1e-6 * abs(d["du_tot"])
+    # the batched trials equal the single ones, and the cell list is MOLSIM's (grid and the
+    # 1-based members in linked-list order, checksummed as sum(k * cell_ip[k]))
+    assert got["dutotal_batch"] == got["dutotal"]
+    onc, ocs, _ = oracle.cell_init(sp.box, sp.rcut + sp.drnlist, sp.rcut)
+    head, nxt, npart, _ = oracle.cell_build(sp.ro, sp.box, onc, ocs)
+    order = [ip + 1 for m in oracle.cell_members(head, nxt) for ip in m]
+    assert got["cells"][0] == list(onc)
+    assert got["cells"][1] == sum((k + 1) * v for k, v in enumerate(order))
     ro2 = sp.ro.copy()
     ro2[trials[0][0]] = trials[0][1]
     ref2 = oracle_system(systems.SystemSpec(**{**sp.__dict__, "ro": ro2}), tab).utotal()
