@@ -166,7 +166,7 @@ int msg_dutotal_batch(msg_ctx* ctx, int ntrial, const int* ip, const double* rot
 
 /* msg_dutotal_batch without allocation: no counterpart in the reference; for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving loops).  The trial
- * inputs and per-trial records go through a fixed device scratch array of the library (2.6 MB
+ * inputs and per-trial records go through a fixed device scratch array of the library (6.0 MB
  * per GPU, held until the process ends) instead of context buffers, so ntrial is at most
  * 16384 (MSG_ERR_ARG beyond).  Same results as msg_dutotal_batch bit for bit.  It needs the
  * state the allocating entry points set up: with SPME, one msg_dutotal / msg_dutotal_batch

@@ -884,7 +884,7 @@ constexpr size_t kOffR = kOffIp + align256(kNoAllocTrials * sizeof(int));
 constexpr size_t kOffOvl = kOffR + align256(3 * (size_t)kNoAllocTrials * sizeof(double));
 constexpr size_t kOffTot = kOffOvl + align256(kNoAllocTrials * sizeof(int));
 constexpr size_t kOffDrec = kOffTot + align256(kNoAllocTrials * sizeof(double));
-constexpr size_t kScratchBytes = kOffDrec + align256(kNoAllocTrials * sizeof(double));  // 2.6 MB
+constexpr size_t kScratchBytes = kOffDrec + align256(kNoAllocTrials * sizeof(double));  // 6.0 MB
 
 __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
