@@ -243,9 +243,12 @@ double msg_random(int* iseed, int* state, double* am);
  * beta * dE (expmax = 87); accepted moves are committed.  The generator state (iseed,
  * rng_state, am) is read and updated.  Outputs: *du_sum (sum of accepted dE), *naccept,
  * events[t] (1 accepted, 0 rejected, -1 hard-core overlap; may be NULL).
- * Standard-Ewald and Ewald-free systems run the whole pass on the device (one workgroup,
- * the generator on the device; a move accepted into another cell ends a launch and the pass
- * resumes after the cell list is rebuilt); SPME systems run the trials from the host. */
+ * Cell-list and all-pairs systems without weak charges run the pass on the device: one
+ * workgroup, the generator on the device, an accepted move committed inside the launch
+ * (UpdateCellIp on the cell list, S(k) for standard Ewald).  With SPME the launch corrects
+ * phi for the moves it has accepted and ends after 16 of them; a UTotal pass recomputes phi
+ * and the pass goes on (one host synchronisation per 16 accepted moves).  Verlet-list and
+ * weak-charge systems run the trials from the host. */
 int msg_mc_pass(msg_ctx* ctx, int ntrial, int isamp, const double* dtran, double beta,
                 int* iseed, int* rng_state, double* am, double* du_sum, int* naccept,
                 int* events);

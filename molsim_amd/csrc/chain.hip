@@ -16,7 +16,16 @@
 //              and UpdateCellIp (Src/celllist.F90:366-382) on the cell list in place: a move
 //              into another cell makes the particle its new cell's head and shifts the slots in
 //              between, so the chain runs on without leaving the launch.
-// SPME systems keep the host-driven chain (an accepted move changes the whole potential mesh).
+// SPME (DUTwoBodyEwaldRecSPM, Src/denergy.F90:1413-1495, evaluated locally): with phi = g * Q
+// the convolved mesh of the last UTotal pass and g the real-space influence kernel,
+//   dE = sum_m dQ(m) phi'(m) + 1/2 sum_{m,m'} dQ(m) g(m - m') dQ(m'),
+//   phi' = phi + sum_k g * dQ_k over the moves k accepted since phi was computed.
+// Each double sum over two stencils u, v (products of per-axis B-spline weights) collapses to a
+// sum over the (2p - 1)^3 mesh offsets d of g(mm_u - mm_v + d) times the product of the three
+// per-axis cross-correlations sum_a u(a) v(a + d), so the self term costs 3 (2p - 1)^3 and
+// each pending move 4 (2p - 1)^3 multiply-adds.  After kChainLog accepted moves the launch
+// ends; the host runs a UTotal pass (phi of the current positions, the patched cell order
+// kept) and launches the chain again where it stopped.
 #include <cmath>
 
 #include "cellmove.h"
@@ -29,6 +38,7 @@ namespace {
 constexpr int kChainThreads = 1024;
 constexpr int kChainKvec = 4;   // k-vectors per thread at most: nk <= 4096
 constexpr int kChainNcut = 31;  // largest k-vector index per axis
+constexpr int kChainLog = 16;   // SPME: accepted moves corrected for before phi is recomputed
 
 struct ChainArgs {
   int np, ntrial, isamp, ewald, lsurf;
@@ -54,6 +64,11 @@ struct ChainArgs {
   const double* kfac;
   double* sk;
   double tpb[3];
+  // SPME
+  MeshGrid m;
+  const void* phi;      // K^3 reals (float or double): g * Q of the last UTotal pass
+  int phi_fp64;
+  const double* gk;     // H^3 folded real-space influence kernel
   double* Mout;         // dipole sum kept with the energies
   double* Mtot;         // dipole sum read by the force kernels
   double surf_fac;
@@ -111,6 +126,26 @@ __device__ inline int chain_cell(const ChainArgs& a, const double* r) {
   return ci[0] + a.g.nc[0] * (ci[1] + a.g.nc[1] * ci[2]);
 }
 
+// |d| folded into one period of K (g is even along every axis)
+__device__ inline int chain_fold(int d, int K) {
+  d %= K;
+  if (d < 0) d += K;
+  return d <= K / 2 ? d : K - d;
+}
+
+// sum_a u(a) v(a + d) of two stencils' weights along one axis, d = dd - (P - 1)
+template <int P>
+__device__ inline double chain_xcorr(const double* u, const double* v, int dd) {
+  const int d = dd - (P - 1);
+  double s = 0.0;
+#pragma unroll
+  for (int a = 0; a < P; ++a) {
+    const int b = a + d;
+    if (b >= 0 && b < P) s += u[a] * v[b];
+  }
+  return s;
+}
+
 // Sum over the workgroup; the result is valid in every thread.
 __device__ inline double chain_sum(double v, double* sh) {
   v = msgd::wave_sum(v);
@@ -124,12 +159,27 @@ __device__ inline double chain_sum(double v, double* sh) {
 }
 
 // KV: k-vectors per thread (the smallest of 1, 2, 4 that covers nk, so the structure-factor
-// changes kept for the commit fit in registers).
-template <int KV>
+// changes kept for the commit fit in registers).  P: SPME order, 0 for standard or no Ewald.
+template <int KV, int P>
 __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
+  constexpr int PP = P > 0 ? P : 1;
+  constexpr int W = 2 * PP - 1;
+  constexpr int NL = P > 0 ? kChainLog : 1;
+  constexpr int NT = 3 + 4 * NL;  // terms: the trial's self terms, 4 per pending move
   __shared__ double sh[kChainThreads / 64];
   // e^{i n (2 pi / L_d) r_d} for n = 0..ncut, per axis d, at the trial (0) and old (1) position
-  __shared__ double2 s_eik[2][3][kChainNcut + 1];
+  __shared__ double2 s_eik[P > 0 ? 1 : 2][3][P > 0 ? 1 : kChainNcut + 1];
+  // SPME: the trial's per-axis weights and base points ([0] new, [1] old position), those of
+  // the moves accepted in this launch, and per term its cross-correlations, offset and factor
+  __shared__ double s_th[2][3][PP];
+  __shared__ int s_mm[2][3];
+  __shared__ double s_lth[NL][2][3][PP];
+  __shared__ int s_lmm[NL][2][3];
+  __shared__ double s_lq[NL];
+  __shared__ double s_cv[NT][3][W];
+  __shared__ int s_cd[NT][3];
+  __shared__ double s_cf[NT];
+  int npend = 0;
   __shared__ int s_ip, s_acc, s_ovl;
   __shared__ double s_rn[3];
   __shared__ int s_rng[3];
@@ -179,7 +229,19 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
     const double ro[3] = {a.ro[3 * ip], a.ro[3 * ip + 1], a.ro[3 * ip + 2]};
     const double q = a.az[ip];
     const int ti = a.ipt[ip];
-    if (a.ewald == 1 && tid < 6 * (a.ncut + 1)) {
+    if constexpr (P > 0) {
+      if (tid < 6) {  // B-spline weights and base point per (position, axis), in FP64
+        const int w = tid / 3, d = tid % 3;
+        const double x = a.m.dkdr[d] * (w == 0 ? rn[d] : ro[d]);
+        const double v = floor(x);
+        double th[PP];
+        msgd::bspline<double, PP>(x - v, th, (double*)nullptr);
+#pragma unroll
+        for (int k = 0; k < PP; ++k) s_th[w][d][k] = th[k];
+        const int k = (int)v % a.m.K;
+        s_mm[w][d] = k < 0 ? k + a.m.K : k;
+      }
+    } else if (a.ewald == 1 && tid < 6 * (a.ncut + 1)) {
       // one sincos per (position, axis, n); the k-vector loop below reads them
       const int n = tid % (a.ncut + 1), d = (tid / (a.ncut + 1)) % 3, w = tid / (3 * (a.ncut + 1));
       const double k = n * a.tpb[d];
@@ -222,10 +284,62 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
     }
     if (ovl) atomicOr(&s_ovl, 1);
     // standard Ewald: dS per k-vector kept for the commit
-    __syncthreads();  // s_eik
+    __syncthreads();  // s_eik, s_th
     double dS[KV][8];
     double drec = 0.0;
-    if (a.ewald == 1) {
+    if constexpr (P > 0) {
+      const int nt = 3 + 4 * npend;
+      for (int w = tid; w < nt * 3 * W; w += kChainThreads) {
+        const int dd = w % W, d = (w / W) % 3, tm = w / (3 * W);
+        const double* u;
+        const double* v;
+        int off;
+        double cf;
+        if (tm < 3) {  // new-new, old-old, new-old
+          u = s_th[tm == 1][d];
+          v = s_th[tm >= 1][d];
+          off = s_mm[tm == 1][d] - s_mm[tm >= 1][d];
+          cf = tm < 2 ? 0.5 * q * q : -q * q;
+        } else {       // trial new/old against pending move k's new/old
+          const int k = (tm - 3) >> 2, j = (tm - 3) & 3;
+          u = s_th[j >> 1][d];
+          v = s_lth[k][j & 1][d];
+          off = s_mm[j >> 1][d] - s_lmm[k][j & 1][d];
+          cf = ((j == 0 || j == 3) ? q : -q) * s_lq[k];
+        }
+        s_cv[tm][d][dd] = chain_xcorr<PP>(u, v, dd);
+        if (dd == 0) s_cd[tm][d] = off;
+        if (dd == 0 && d == 0) s_cf[tm] = cf;
+      }
+      __syncthreads();
+      const int K = a.m.K, H = a.m.H;
+      for (int r = tid; r < nt * W * W; r += kChainThreads) {
+        const int dy = r % W, dz = (r / W) % W, tm = r / (W * W);
+        const int oy = chain_fold(s_cd[tm][1] + dy - (PP - 1), K), oz = chain_fold(s_cd[tm][2] + dz - (PP - 1), K);
+        const double* grow = a.gk + (size_t)H * (oy + (size_t)H * oz);
+        double sx = 0.0;
+#pragma unroll
+        for (int dx = 0; dx < W; ++dx) sx += s_cv[tm][0][dx] * grow[chain_fold(s_cd[tm][0] + dx - (PP - 1), K)];
+        drec += s_cf[tm] * s_cv[tm][1][dy] * s_cv[tm][2][dz] * sx;
+      }
+      // sum dQ phi over the trial's two stencils
+      for (int i3 = tid; i3 < PP * PP * PP; i3 += kChainThreads) {
+        const int ix = i3 % PP, iy = (i3 / PP) % PP, iz = i3 / (PP * PP);
+        double sp = 0.0;
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+          int x = s_mm[w][0] - ix, y = s_mm[w][1] - iy, z = s_mm[w][2] - iz;
+          x += x < 0 ? K : 0;
+          y += y < 0 ? K : 0;
+          z += z < 0 ? K : 0;
+          const size_t mi = x + (size_t)K * (y + (size_t)K * z);
+          const double ph = a.phi_fp64 ? ((const double*)a.phi)[mi] : (double)((const float*)a.phi)[mi];
+          const double wt = s_th[w][0][ix] * s_th[w][1][iy] * s_th[w][2][iz];
+          sp += w == 0 ? wt * ph : -wt * ph;
+        }
+        drec += q * sp;
+      }
+    } else if (a.ewald == 1) {
 #pragma unroll
       for (int u = 0; u < KV; ++u) {
         const int kn = tid + u * kChainThreads;
@@ -294,7 +408,15 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
     __syncthreads();
     ++done;
     if (s_acc) {  // MCUpdate in place
-      if (a.ewald == 1) {
+      if constexpr (P > 0) {  // the move joins the pending corrections of phi
+        if (tid < 6 * PP) {
+          const int w = tid / (3 * PP), d = (tid / PP) % 3, k = tid % PP;
+          s_lth[npend][w][d][k] = s_th[w][d][k];
+        }
+        if (tid < 6) s_lmm[npend][tid / 3][tid % 3] = s_mm[tid / 3][tid % 3];
+        if (tid == 0) s_lq[npend] = q;
+        ++npend;
+      } else if (a.ewald == 1) {
 #pragma unroll
         for (int u = 0; u < KV; ++u) {
           const int kn = tid + u * kChainThreads;
@@ -314,6 +436,7 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
       __threadfence_block();
     }
     __syncthreads();
+    if (P > 0 && npend == NL) break;  // phi is recomputed before the chain goes on
   }
   if (tid == 0) {
     for (int k = 0; k < 3; ++k) a.rng[k] = s_rng[k];
@@ -330,11 +453,22 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
 
 bool chain_eligible(const Ctx* c) {
   // systems with weak charges run their trials from the host (msg_dutotal honours the states)
-  return c->P.ewald_mode != MSG_EWALD_SPM && c->P.list_mode != MSG_LIST_VERLET && c->h_laz.empty() &&
-         (c->P.ewald_mode != MSG_EWALD_STD || (c->nkvec <= kChainThreads * kChainKvec && c->P.ncut <= kChainNcut));
+  if (c->P.list_mode == MSG_LIST_VERLET || !c->h_laz.empty()) return false;
+  if (c->P.ewald_mode == MSG_EWALD_SPM) return c->mesh.order >= 2 && c->mesh.order <= 11;
+  return c->P.ewald_mode != MSG_EWALD_STD || (c->nkvec <= kChainThreads * kChainKvec && c->P.ncut <= kChainNcut);
 }
 
-// Runs ntrial trials of the chain in one launch.
+namespace {
+
+template <int P>
+void launch_chain_spm(hipStream_t s, const ChainArgs& a) {
+  hipLaunchKernelGGL((k_mc_chain<1, P>), dim3(1), dim3(kChainThreads), 0, s, a);
+}
+
+}  // namespace
+
+// Runs ntrial trials of the chain: one launch, or with SPME one launch per kChainLog accepted
+// moves with a UTotal pass (phi of the current positions) between them.
 int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, int* iseed, int* rng_state,
              double* am, double* du_sum, int* naccept, int* events) {
   // device scratch: rng (3 ints) + next (1 int) + events (ntrial ints); am + stat (4) + dtran (npt)
@@ -359,8 +493,8 @@ int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, in
   for (int t = 0; t < c->npt; ++t) hd[8 + t] = dtran[t];
   MSG_HIP(hipMemcpyAsync(d_i, hi, sizeof(hi), hipMemcpyHostToDevice, c->stream));
   MSG_HIP(hipMemcpyAsync(d_d, hd, sizeof(hd), hipMemcpyHostToDevice, c->stream));
-  if (!c->fresh)
-    if (int e = utotal_refresh(c)) return e;
+  const bool spm = c->P.ewald_mode == MSG_EWALD_SPM;
+  if (int e = spme_trial_setup(c)) return e;
   ChainArgs a;
   a.np = c->np;
   a.ntrial = ntrial;
@@ -394,6 +528,10 @@ int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, in
   a.kvec = c->d_kvec;
   a.kfac = c->d_kfac;
   a.sk = c->d_sk;
+  a.m = c->mesh;
+  a.phi = c->d_Q;
+  a.phi_fp64 = c->fp64;
+  a.gk = c->d_gk;
   a.Mout = c->d_out + c->nptpt + 4;
   a.Mtot = c->d_M;
   const double vol = c->P.boxlen[0] * c->P.boxlen[1] * c->P.boxlen[2];
@@ -406,17 +544,46 @@ int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, in
   a.next = d_i + 3;
   a.events = d_i + 4;
   a.stat = d_d + 1;
-  prof_begin(c, MSG_K_DE);
-  if (a.nk <= kChainThreads)
-    hipLaunchKernelGGL(k_mc_chain<1>, dim3(1), dim3(kChainThreads), 0, c->stream, a);
-  else if (a.nk <= 2 * kChainThreads)
-    hipLaunchKernelGGL(k_mc_chain<2>, dim3(1), dim3(kChainThreads), 0, c->stream, a);
-  else
-    hipLaunchKernelGGL(k_mc_chain<4>, dim3(1), dim3(kChainThreads), 0, c->stream, a);
-  prof_end(c, MSG_K_DE);
-  MSG_HIP(hipGetLastError());
-  double st[4];
-  MSG_HIP(hipMemcpyAsync(st, d_d + 1, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+  double dsum = 0.0;
+  int nacc = 0, done = 0;
+  do {
+    if (!c->fresh)
+      if (int e = utotal_refresh(c)) return e;
+    a.ntrial = ntrial - done;
+    a.events = d_i + 4 + done;
+    prof_begin(c, MSG_K_DE);
+    if (spm) {
+      switch (c->mesh.order) {
+        case 2: launch_chain_spm<2>(c->stream, a); break;
+        case 3: launch_chain_spm<3>(c->stream, a); break;
+        case 4: launch_chain_spm<4>(c->stream, a); break;
+        case 5: launch_chain_spm<5>(c->stream, a); break;
+        case 6: launch_chain_spm<6>(c->stream, a); break;
+        case 7: launch_chain_spm<7>(c->stream, a); break;
+        case 8: launch_chain_spm<8>(c->stream, a); break;
+        case 9: launch_chain_spm<9>(c->stream, a); break;
+        case 10: launch_chain_spm<10>(c->stream, a); break;
+        default: launch_chain_spm<11>(c->stream, a); break;
+      }
+    } else if (a.nk <= kChainThreads) {
+      hipLaunchKernelGGL((k_mc_chain<1, 0>), dim3(1), dim3(kChainThreads), 0, c->stream, a);
+    } else if (a.nk <= 2 * kChainThreads) {
+      hipLaunchKernelGGL((k_mc_chain<2, 0>), dim3(1), dim3(kChainThreads), 0, c->stream, a);
+    } else {
+      hipLaunchKernelGGL((k_mc_chain<4, 0>), dim3(1), dim3(kChainThreads), 0, c->stream, a);
+    }
+    prof_end(c, MSG_K_DE);
+    MSG_HIP(hipGetLastError());
+    double st[4];
+    MSG_HIP(hipMemcpyAsync(st, d_d + 1, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    MSG_HIP(hipStreamSynchronize(c->stream));
+    dsum += st[0];
+    nacc += (int)st[1];
+    done += (int)st[2];
+    if (st[3] > 0.0) c->cells_maintained = true;  // the cell order carries the chain's moves
+    if (spm && st[1] > 0.0) c->fresh = false;     // phi is that of the positions before the moves
+    if (st[2] < 1.0 && done < ntrial) return set_error(MSG_ERR_HIP, "mc_chain: a launch made no trial");
+  } while (done < ntrial);
   MSG_HIP(hipMemcpyAsync(hi, d_i, sizeof(hi), hipMemcpyDeviceToHost, c->stream));
   MSG_HIP(hipMemcpyAsync(hd, d_d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (events) MSG_HIP(hipMemcpyAsync(events, d_i + 4, ntrial * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -426,9 +593,8 @@ int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, in
   rng_state[1] = hi[2];
   *am = hd[0];
   c->mc_next = hi[3];
-  if (st[3] > 0.0) c->cells_maintained = true;  // the cell order carries the chain's moves
-  if (du_sum) *du_sum = st[0];
-  if (naccept) *naccept = (int)st[1];
+  if (du_sum) *du_sum = dsum;
+  if (naccept) *naccept = nacc;
   return MSG_OK;
 }
 

@@ -540,6 +540,11 @@ int make_gk(Ctx* c) {
 
 }  // namespace
 
+int spme_trial_setup(Ctx* c) {
+  if (c->P.ewald_mode != MSG_EWALD_SPM || c->have_gk) return MSG_OK;
+  return make_gk(c);
+}
+
 static DeArgs de_args(Ctx* c);
 
 int dutotal_launch(Ctx* c, int ntrial, const int* ip, const double* rotm) {

@@ -673,7 +673,42 @@ int orc_ewald_std_rec(int na, const double* r, const double* az, const double* b
 }
 
 // ----------------------------------------------------------------------------------
-// FFT (own code).  Radix-2 for powers of two, plain DFT otherwise.  sign -1 = forward.
+// FFT (own code).  Radix-2 for powers of two; otherwise a recursive mixed-radix
+// decimation in time (Cooley-Tukey over the prime factors of n, a plain DFT for a prime
+// length).  sign -1 = forward.
+//
+// out[k] = sum_j in[j * is] W^{jk}, W = e^{sign 2 pi i / n}; tw holds e^{sign 2 pi i e / N}
+// for e < N and W^e = tw[(e mod n) * (N / n)].
+static void fft_mixed(const cplx* in, int is, cplx* out, int n, int N, const std::vector<cplx>& tw) {
+  if (n == 1) {
+    out[0] = in[0];
+    return;
+  }
+  const int step = N / n;
+  int p = 2;
+  while (n % p) ++p;
+  if (p == n) {
+    for (int k = 0; k < n; ++k) {
+      cplx s = 0;
+      for (int j = 0; j < n; ++j) s += in[(size_t)j * is] * tw[(size_t)((long)j * k % n) * step];
+      out[k] = s;
+    }
+    return;
+  }
+  const int m = n / p;
+  // sub-sequence r (in[j p + r], j < m) transformed into out[r m .. r m + m)
+  for (int r = 0; r < p; ++r) fft_mixed(in + (size_t)r * is, is * p, out + (size_t)r * m, m, N, tw);
+  std::vector<cplx> comb(n);
+  for (int k = 0; k < m; ++k)
+    for (int s = 0; s < p; ++s) {
+      const int kk = k + m * s;
+      cplx v = 0;
+      for (int r = 0; r < p; ++r) v += tw[(size_t)((long)r * kk % n) * step] * out[(size_t)r * m + k];
+      comb[kk] = v;
+    }
+  for (int k = 0; k < n; ++k) out[k] = comb[k];
+}
+
 static void fft1(cplx* a, int n, int stride, int sign, std::vector<cplx>& buf) {
   buf.resize(n);
   for (int i = 0; i < n; ++i) buf[i] = a[(size_t)i * stride];
@@ -698,14 +733,13 @@ static void fft1(cplx* a, int n, int stride, int sign, std::vector<cplx>& buf) {
     }
     for (int i = 0; i < n; ++i) a[(size_t)i * stride] = buf[i];
   } else {
-    for (int k = 0; k < n; ++k) {
-      cplx s = 0;
-      for (int j = 0; j < n; ++j) {
-        double ang = sign * 2 * kPi * (double)((long)k * j % n) / n;
-        s += buf[j] * cplx(std::cos(ang), std::sin(ang));
-      }
-      a[(size_t)k * stride] = s;
+    std::vector<cplx> tw(n), out(n);
+    for (int e = 0; e < n; ++e) {
+      const double ang = sign * 2 * kPi * (double)e / n;
+      tw[e] = cplx(std::cos(ang), std::sin(ang));
     }
+    fft_mixed(buf.data(), 1, out.data(), n, n, tw);
+    for (int k = 0; k < n; ++k) a[(size_t)k * stride] = out[k];
   }
 }
 

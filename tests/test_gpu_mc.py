@@ -53,11 +53,15 @@ def _oracle_mc(sp, tab, ntrial, dtran, beta, seed):
 
 
 @pytest.mark.parametrize("ewald,precision", [("std", "fp64"), ("std_cell", "fp64"), ("spm", "fp64"),
-                                             ("spm", "mixed")])
+                                             ("spm", "mixed"), ("spm_k40_p5", "fp64"),
+                                             ("spm_all_p11", "fp64")])
 def test_mc_pass_matches_oracle_loop(ewald, precision):
     """std: the device-resident chain (all pairs); std_cell: the device chain on a 3x3x3 cell
     grid (accepted moves into another cell patch the cell list inside the launch, as
-    UpdateCellIp does); spm: the host-driven chain.  With a cell list, the cell order after the
+    UpdateCellIp does); spm: the device chain with SPME, phi corrected inside the launch for
+    the moves accepted since the last UTotal pass and recomputed after every 16 (15 launches
+    for the 228 accepted moves of this pass); spm_k40_p5: the reference decks' mesh (40^3, order 5, the general
+    mesh path); spm_all_p11: all pairs, order 11.  With a cell list, the cell order after the
     pass equals MOLSIM's linked lists after the same UpdateCellIp calls."""
     sp = systems.small_spme_system(n=200, L=36.0, nmesh=32, alpha=0.3, rcut=11.0)
     if ewald == "std":
@@ -66,6 +70,11 @@ def test_mc_pass_matches_oracle_loop(ewald, precision):
         sp.pot["rcut"] = 17.9
     elif ewald == "std_cell":
         sp = systems.SystemSpec(**{**sp.__dict__, "ewald": "std", "ncut": 6, "list_mode": "cell"})
+    elif ewald == "spm_k40_p5":
+        sp = systems.small_spme_system(n=200, L=36.0, nmesh=40, order=5, alpha=0.3, rcut=11.0)
+    elif ewald == "spm_all_p11":
+        sp = systems.small_spme_system(n=200, L=36.0, nmesh=32, order=11, alpha=0.3, rcut=17.9,
+                                       list_mode="all")
     tab = oracle.build_table(sp.pot)
     ntrial, dtran, beta = 300, 2.0, systems.beta_kjmol()
     print(f"mc ewald={ewald} precision={precision} n={sp.n}", flush=True)

@@ -162,3 +162,37 @@ def test_pair_tiles_match_oracle(case, tile):
     compare(eng.utotal(), ref, MIXED_FTOL, MIXED_ETOL)
     compare(eng.utotal(), ref, MIXED_FTOL, MIXED_ETOL)   # graph replay
     eng.close()
+
+
+@pytest.mark.parametrize("precision,ftol,etol", [("mixed", MIXED_FTOL, MIXED_ETOL), ("fp64", FP64_TOL, FP64_TOL)])
+def test_crowded_brick(precision, ftol, etol):
+    """700 of 800 ions in one of the four 16 x 8 x 8 mesh bricks (K = 16, L = 60 A): one brick
+    holds nearly all the charge the spread deposits and a few cells hold most of the pair
+    work; against the oracle, twice (graph replay)."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(17)
+    L = 60.0
+    lo = np.array([0.0, 0.0, 0.0])
+    hi = np.array([60.0, 30.0, 30.0])
+    pts = np.concatenate([rng.uniform(lo, hi, (700, 3)), rng.uniform(0.0, L, (100, 3))])
+    pts[700:, 1] = 30.0 + (pts[700:, 1] % 30.0)   # the other 100 outside that brick
+    for _ in range(300):
+        pairs = cKDTree(pts % L, boxsize=L).query_pairs(3.0, output_type="ndarray")
+        if not len(pairs):
+            break
+        bad = np.unique(pairs[:, 1])
+        inside = bad < 700
+        pts[bad[inside]] = rng.uniform(lo, hi, (int(inside.sum()), 3))
+        out = bad[~inside]
+        pts[out] = rng.uniform(0.0, L, (len(out), 3))
+        pts[out, 1] = 30.0 + (pts[out, 1] % 30.0)
+    assert not len(cKDTree(pts % L, boxsize=L).query_pairs(3.0, output_type="ndarray"))
+    sp = _small(n=800, L=L, nmesh=16, rcut=12.0)
+    sp = systems.SystemSpec(**{**sp.__dict__, "ro": (pts % L) - 0.5 * L})
+    print(f"crowded brick {precision}", flush=True)
+    tab = oracle.build_table(sp.pot)
+    ref = oracle_system(sp, tab).utotal()
+    eng = engine(sp, tab, precision)
+    compare(eng.utotal(), ref, ftol, etol)
+    compare(eng.utotal(), ref, ftol, etol)
+    eng.close()

@@ -39,7 +39,7 @@ def test_bspline_cubic_closed_form(orc):
         assert np.allclose(v, [m4(x + i) for i in range(4)], atol=1e-15)
 
 
-@pytest.mark.parametrize("K", [8, 16, 32, 10, 40, 48])
+@pytest.mark.parametrize("K", [8, 16, 32, 10, 18, 26, 40, 42, 48, 50])
 def test_fft3_matches_numpy(orc, K):
     rng = np.random.default_rng(K)
     q = rng.normal(size=(K, K, K))          # q[z, y, x], x fastest
