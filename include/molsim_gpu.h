@@ -272,6 +272,19 @@ void msg_batch_destroy(msg_batch* batch);
 int msg_batch_utotal(msg_batch* batch);
 int msg_batch_sync(msg_batch* batch);
 int msg_batch_query(msg_batch* batch);
+/* One MCPass per system of a batch, the chains side by side on the device (one 1024-thread
+ * workgroup each, one launch per SPME order present); with SPME, one batched UTotal pass
+ * after every launch in which a chain filled its log of 16 accepted moves (no counterpart
+ * in the reference, which runs one chain per process).  Every system must be SPME or
+ * Ewald-free without weak charges.  Per system s: ntrial trials as msg_mc_pass;
+ * dtran[s * MSG_MAX_TYPES + t] its displacement per type, iseed[s], rng_state[2 s .. 2 s + 1]
+ * and am[s] its generator state (read and updated), du_sum[s] and naccept[s] its results,
+ * events[s * ntrial + t] its decisions (each may be NULL).  After a batched refresh the cells
+ * are sorted afresh (SetCellList order) rather than kept in UpdateCellIp order, so
+ * real-space sums may round differently from a chain run alone. */
+int msg_batch_mc_pass(msg_batch* batch, int ntrial, int isamp, const double* dtran, double beta,
+                      int* iseed, int* rng_state, double* am, double* du_sum, int* naccept,
+                      int* events);
 int msg_batch_results(msg_batch* batch, double* u_tot, double* u_rec, double* virial);
 int msg_batch_dutotal(msg_batch* batch, const int* ntrial, const int* ip, const double* rotm,
                       int* lhsoverlap, double* du_tot);

@@ -388,6 +388,19 @@ int msg_batch_dutotal(msg_batch* b, const int* ntrial, const int* ip, const doub
   return MSG_OK;
 }
 
+int msg_batch_mc_pass(msg_batch* b, int ntrial, int isamp, const double* dtran, double beta, int* iseed,
+                      int* rng_state, double* am, double* du_sum, int* naccept, int* events) {
+  if (!b || ntrial < 0 || !dtran || !iseed || !rng_state || !am)
+    return set_error(MSG_ERR_ARG, "msg_batch_mc_pass: bad arguments");
+  if (int e = batch_current(b, "msg_batch_mc_pass")) return e;
+  bool stale = false;
+  for (Ctx* c : b->cs) stale |= !c->fresh;
+  if (stale)
+    if (int e = msg_batch_utotal(b)) return e;
+  return mc_chain_batch(b->cs, b->origin, [b]() { return msg_batch_utotal(b); }, ntrial, isamp, dtran, beta,
+                        iseed, rng_state, am, du_sum, naccept, events);
+}
+
 // ---- execution units (continuous batching) --------------------------------------------------
 // A unit executes one batch of requests at a time on its own streams: a UTotal pass over
 // the systems that asked for energy + forces (and over trial systems whose state is stale),

@@ -27,6 +27,7 @@
 // ends; the host runs a UTotal pass (phi of the current positions, the patched cell order
 // kept) and launches the chain again where it stopped.
 #include <cmath>
+#include <functional>
 
 #include "cellmove.h"
 #include "common.h"
@@ -161,7 +162,7 @@ __device__ inline double chain_sum(double v, double* sh) {
 // KV: k-vectors per thread (the smallest of 1, 2, 4 that covers nk, so the structure-factor
 // changes kept for the commit fit in registers).  P: SPME order, 0 for standard or no Ewald.
 template <int KV, int P>
-__global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
+__device__ __forceinline__ void chain_body(const ChainArgs& a) {
   constexpr int PP = P > 0 ? P : 1;
   constexpr int W = 2 * PP - 1;
   constexpr int NL = P > 0 ? kChainLog : 1;
@@ -449,6 +450,18 @@ __global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
   }
 }
 
+template <int KV, int P>
+__global__ __launch_bounds__(kChainThreads) void k_mc_chain(ChainArgs a) {
+  chain_body<KV, P>(a);
+}
+
+// One chain per system: workgroup b runs the chain of args[b] (an ensemble's passes side by
+// side, one workgroup each).
+template <int KV, int P>
+__global__ __launch_bounds__(kChainThreads) void k_mc_chain_seg(const ChainArgs* __restrict__ args) {
+  chain_body<KV, P>(args[blockIdx.x]);
+}
+
 }  // namespace
 
 bool chain_eligible(const Ctx* c) {
@@ -460,44 +473,17 @@ bool chain_eligible(const Ctx* c) {
 
 namespace {
 
-template <int P>
-void launch_chain_spm(hipStream_t s, const ChainArgs& a) {
-  hipLaunchKernelGGL((k_mc_chain<1, P>), dim3(1), dim3(kChainThreads), 0, s, a);
-}
+// Per-chain device state: ints {iseed, xorshift word, Park-Miller word, sequential cursor};
+// doubles {am, stat[4] (sum of accepted dE, accepted, trials done, moves into another cell),
+// 3 unused, dtran per type}.
+constexpr int kChainInts = 4;
+constexpr int kChainDoubles = 8 + kMaxT;
 
-}  // namespace
-
-// Runs ntrial trials of the chain: one launch, or with SPME one launch per kChainLog accepted
-// moves with a UTotal pass (phi of the current positions) between them.
-int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, int* iseed, int* rng_state,
-             double* am, double* du_sum, int* naccept, int* events) {
-  // device scratch: rng (3 ints) + next (1 int) + events (ntrial ints); am + stat (4) + dtran (npt)
-  int* d_i = nullptr;
-  double* d_d = nullptr;
-  MSG_HIP(hipMalloc((void**)&d_i, (4 + (size_t)std::max(ntrial, 1)) * sizeof(int)));
-  if (hipMalloc((void**)&d_d, (8 + kMaxT) * sizeof(double)) != hipSuccess) {
-    hipFree(d_i);
-    return set_error(MSG_ERR_HIP, "mc_chain: allocation failed");
-  }
-  struct Free {
-    void* a;
-    void* b;
-    ~Free() {
-      hipFree(a);
-      hipFree(b);
-    }
-  } guard{d_i, d_d};
-  int hi[4] = {*iseed, rng_state[0], rng_state[1], c->mc_next};
-  double hd[8 + kMaxT] = {};
-  hd[0] = *am;
-  for (int t = 0; t < c->npt; ++t) hd[8 + t] = dtran[t];
-  MSG_HIP(hipMemcpyAsync(d_i, hi, sizeof(hi), hipMemcpyHostToDevice, c->stream));
-  MSG_HIP(hipMemcpyAsync(d_d, hd, sizeof(hd), hipMemcpyHostToDevice, c->stream));
-  const bool spm = c->P.ewald_mode == MSG_EWALD_SPM;
-  if (int e = spme_trial_setup(c)) return e;
+// The chain's view of a context; the state pointers and the trial count are set per launch.
+ChainArgs chain_args(Ctx* c, int isamp, double beta) {
   ChainArgs a;
   a.np = c->np;
-  a.ntrial = ntrial;
+  a.ntrial = 0;
   a.isamp = isamp;
   a.ewald = c->P.ewald_mode;
   a.lsurf = c->P.lsurf;
@@ -537,41 +523,96 @@ int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, in
   const double vol = c->P.boxlen[0] * c->P.boxlen[1] * c->P.boxlen[2];
   a.surf_fac = 2.0 * M_PI / (3.0 * vol);
   a.epsi4pi = c->P.epsi4pi;
-  a.dtran = d_d + 8;
   a.beta = beta;
+  return a;
+}
+
+void chain_state(ChainArgs& a, int* d_i, double* d_d, int* d_events) {
   a.rng = d_i;
-  a.am = d_d;
   a.next = d_i + 3;
-  a.events = d_i + 4;
+  a.am = d_d;
   a.stat = d_d + 1;
+  a.dtran = d_d + 8;
+  a.events = d_events;
+}
+
+// order: the SPME order, 0 without SPME (kv then picks the k-vectors per thread)
+template <int KV, int P>
+void launch_one(hipStream_t s, const ChainArgs* a, const ChainArgs* d_args, int n) {
+  if (d_args)
+    hipLaunchKernelGGL((k_mc_chain_seg<KV, P>), dim3(n), dim3(kChainThreads), 0, s, d_args);
+  else
+    hipLaunchKernelGGL((k_mc_chain<KV, P>), dim3(1), dim3(kChainThreads), 0, s, *a);
+}
+
+void launch_chain(hipStream_t s, int order, int nk, const ChainArgs* a, const ChainArgs* d_args, int n) {
+  switch (order) {
+    case 0:
+      if (nk <= kChainThreads) launch_one<1, 0>(s, a, d_args, n);
+      else if (nk <= 2 * kChainThreads) launch_one<2, 0>(s, a, d_args, n);
+      else launch_one<4, 0>(s, a, d_args, n);
+      break;
+    case 2: launch_one<1, 2>(s, a, d_args, n); break;
+    case 3: launch_one<1, 3>(s, a, d_args, n); break;
+    case 4: launch_one<1, 4>(s, a, d_args, n); break;
+    case 5: launch_one<1, 5>(s, a, d_args, n); break;
+    case 6: launch_one<1, 6>(s, a, d_args, n); break;
+    case 7: launch_one<1, 7>(s, a, d_args, n); break;
+    case 8: launch_one<1, 8>(s, a, d_args, n); break;
+    case 9: launch_one<1, 9>(s, a, d_args, n); break;
+    case 10: launch_one<1, 10>(s, a, d_args, n); break;
+    default: launch_one<1, 11>(s, a, d_args, n); break;
+  }
+}
+
+int chain_order(const Ctx* c) { return c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.order : 0; }
+
+struct DeviceBuffers {
+  std::vector<void*> p;
+  ~DeviceBuffers() {
+    for (void* q : p) hipFree(q);
+  }
+  int alloc(void** out, size_t bytes) {
+    *out = nullptr;
+    if (hipMalloc(out, std::max(bytes, (size_t)16)) != hipSuccess)
+      return set_error(MSG_ERR_NOMEM, "mc_chain: device allocation failed");
+    p.push_back(*out);
+    return MSG_OK;
+  }
+};
+
+}  // namespace
+
+// Runs ntrial trials of the chain: one launch, or with SPME one launch per kChainLog accepted
+// moves with a UTotal pass (phi of the current positions) between them.
+int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, int* iseed, int* rng_state,
+             double* am, double* du_sum, int* naccept, int* events) {
+  DeviceBuffers buf;
+  int* d_i = nullptr;
+  double* d_d = nullptr;
+  int* d_ev = nullptr;
+  if (int e = buf.alloc((void**)&d_i, kChainInts * sizeof(int))) return e;
+  if (int e = buf.alloc((void**)&d_d, kChainDoubles * sizeof(double))) return e;
+  if (int e = buf.alloc((void**)&d_ev, (size_t)ntrial * sizeof(int))) return e;
+  int hi[kChainInts] = {*iseed, rng_state[0], rng_state[1], c->mc_next};
+  double hd[kChainDoubles] = {};
+  hd[0] = *am;
+  for (int t = 0; t < c->npt; ++t) hd[8 + t] = dtran[t];
+  MSG_HIP(hipMemcpyAsync(d_i, hi, sizeof(hi), hipMemcpyHostToDevice, c->stream));
+  MSG_HIP(hipMemcpyAsync(d_d, hd, sizeof(hd), hipMemcpyHostToDevice, c->stream));
+  const bool spm = c->P.ewald_mode == MSG_EWALD_SPM;
+  if (int e = spme_trial_setup(c)) return e;
+  ChainArgs a = chain_args(c, isamp, beta);
+  chain_state(a, d_i, d_d, d_ev);
   double dsum = 0.0;
   int nacc = 0, done = 0;
   do {
     if (!c->fresh)
       if (int e = utotal_refresh(c)) return e;
     a.ntrial = ntrial - done;
-    a.events = d_i + 4 + done;
+    a.events = d_ev + done;
     prof_begin(c, MSG_K_DE);
-    if (spm) {
-      switch (c->mesh.order) {
-        case 2: launch_chain_spm<2>(c->stream, a); break;
-        case 3: launch_chain_spm<3>(c->stream, a); break;
-        case 4: launch_chain_spm<4>(c->stream, a); break;
-        case 5: launch_chain_spm<5>(c->stream, a); break;
-        case 6: launch_chain_spm<6>(c->stream, a); break;
-        case 7: launch_chain_spm<7>(c->stream, a); break;
-        case 8: launch_chain_spm<8>(c->stream, a); break;
-        case 9: launch_chain_spm<9>(c->stream, a); break;
-        case 10: launch_chain_spm<10>(c->stream, a); break;
-        default: launch_chain_spm<11>(c->stream, a); break;
-      }
-    } else if (a.nk <= kChainThreads) {
-      hipLaunchKernelGGL((k_mc_chain<1, 0>), dim3(1), dim3(kChainThreads), 0, c->stream, a);
-    } else if (a.nk <= 2 * kChainThreads) {
-      hipLaunchKernelGGL((k_mc_chain<2, 0>), dim3(1), dim3(kChainThreads), 0, c->stream, a);
-    } else {
-      hipLaunchKernelGGL((k_mc_chain<4, 0>), dim3(1), dim3(kChainThreads), 0, c->stream, a);
-    }
+    launch_chain(c->stream, chain_order(c), a.nk, &a, nullptr, 1);
     prof_end(c, MSG_K_DE);
     MSG_HIP(hipGetLastError());
     double st[4];
@@ -586,7 +627,8 @@ int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, in
   } while (done < ntrial);
   MSG_HIP(hipMemcpyAsync(hi, d_i, sizeof(hi), hipMemcpyDeviceToHost, c->stream));
   MSG_HIP(hipMemcpyAsync(hd, d_d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  if (events) MSG_HIP(hipMemcpyAsync(events, d_i + 4, ntrial * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (events && ntrial > 0)
+    MSG_HIP(hipMemcpyAsync(events, d_ev, ntrial * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   MSG_HIP(hipStreamSynchronize(c->stream));
   *iseed = hi[0];
   rng_state[0] = hi[1];
@@ -595,6 +637,106 @@ int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, in
   c->mc_next = hi[3];
   if (du_sum) *du_sum = dsum;
   if (naccept) *naccept = nacc;
+  return MSG_OK;
+}
+
+// The chains of many systems side by side: one workgroup per system and one launch per SPME
+// order present; refresh() runs a UTotal pass of every system (phi of the current positions)
+// after a launch in which an SPME chain filled its log of accepted moves.
+int mc_chain_batch(const std::vector<Ctx*>& cs, hipStream_t s, const std::function<int()>& refresh, int ntrial,
+                   int isamp, const double* dtran, double beta, int* iseed, int* rng_state, double* am,
+                   double* du_sum, int* naccept, int* events) {
+  const int n = (int)cs.size();
+  for (Ctx* c : cs)
+    if (!chain_eligible(c) || c->P.ewald_mode == MSG_EWALD_STD)
+      return set_error(MSG_ERR_ARG, "msg_batch_mc_pass: every system must be SPME or Ewald-free, without weak charges");
+  DeviceBuffers buf;
+  int* d_i = nullptr;
+  double* d_d = nullptr;
+  int* d_ev = nullptr;
+  ChainArgs* d_args = nullptr;
+  if (int e = buf.alloc((void**)&d_i, (size_t)n * kChainInts * sizeof(int))) return e;
+  if (int e = buf.alloc((void**)&d_d, (size_t)n * kChainDoubles * sizeof(double))) return e;
+  if (int e = buf.alloc((void**)&d_ev, (size_t)n * ntrial * sizeof(int))) return e;
+  if (int e = buf.alloc((void**)&d_args, (size_t)n * sizeof(ChainArgs))) return e;
+  std::vector<int> hi((size_t)n * kChainInts);
+  std::vector<double> hd((size_t)n * kChainDoubles, 0.0);
+  for (int k = 0; k < n; ++k) {
+    hi[kChainInts * k] = iseed[k];
+    hi[kChainInts * k + 1] = rng_state[2 * k];
+    hi[kChainInts * k + 2] = rng_state[2 * k + 1];
+    hi[kChainInts * k + 3] = cs[k]->mc_next;
+    hd[(size_t)kChainDoubles * k] = am[k];
+    for (int t = 0; t < cs[k]->npt; ++t) hd[(size_t)kChainDoubles * k + 8 + t] = dtran[(size_t)MSG_MAX_TYPES * k + t];
+    if (int e = spme_trial_setup(cs[k])) return e;
+  }
+  MSG_HIP(hipMemcpyAsync(d_i, hi.data(), hi.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  MSG_HIP(hipMemcpyAsync(d_d, hd.data(), hd.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  // systems grouped by SPME order (one kernel instance each), contiguous in the argument array
+  std::vector<int> idx(n);
+  for (int k = 0; k < n; ++k) idx[k] = k;
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return chain_order(cs[x]) < chain_order(cs[y]); });
+  std::vector<ChainArgs> args(n);
+  for (int r = 0; r < n; ++r) {
+    const int k = idx[r];
+    args[r] = chain_args(cs[k], isamp, beta);
+    chain_state(args[r], d_i + kChainInts * k, d_d + (size_t)kChainDoubles * k, d_ev + (size_t)ntrial * k);
+  }
+  std::vector<int> done(n, 0);
+  std::vector<double> dsum(n, 0.0), st((size_t)4 * n);
+  std::vector<long long> nacc(n, 0);
+  bool any_spm_accept = false;
+  while (true) {
+    bool left = false;
+    for (int k = 0; k < n; ++k) left |= done[k] < ntrial;
+    if (!left) break;
+    for (int r = 0; r < n; ++r) {
+      const int k = idx[r];
+      args[r].ntrial = ntrial - done[k];
+      args[r].events = d_ev + (size_t)ntrial * k + done[k];
+    }
+    MSG_HIP(hipMemcpyAsync(d_args, args.data(), n * sizeof(ChainArgs), hipMemcpyHostToDevice, s));
+    for (int r0 = 0; r0 < n;) {
+      int r1 = r0;
+      while (r1 < n && chain_order(cs[idx[r1]]) == chain_order(cs[idx[r0]])) ++r1;
+      launch_chain(s, chain_order(cs[idx[r0]]), 0, nullptr, d_args + r0, r1 - r0);
+      MSG_HIP(hipGetLastError());
+      r0 = r1;
+    }
+    for (int k = 0; k < n; ++k)
+      MSG_HIP(hipMemcpyAsync(st.data() + 4 * k, d_d + (size_t)kChainDoubles * k + 1, 4 * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+    MSG_HIP(hipStreamSynchronize(s));
+    any_spm_accept = false;
+    for (int k = 0; k < n; ++k) {
+      dsum[k] += st[4 * k];
+      nacc[k] += (long long)st[4 * k + 1];
+      done[k] += (int)st[4 * k + 2];
+      if (st[4 * k + 3] > 0.0) cs[k]->cells_maintained = true;
+      if (cs[k]->P.ewald_mode == MSG_EWALD_SPM && st[4 * k + 1] > 0.0) {
+        cs[k]->fresh = false;
+        any_spm_accept = true;
+      }
+    }
+    left = false;
+    for (int k = 0; k < n; ++k) left |= done[k] < ntrial;
+    if (left && any_spm_accept)
+      if (int e = refresh()) return e;
+  }
+  MSG_HIP(hipMemcpyAsync(hi.data(), d_i, hi.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  MSG_HIP(hipMemcpyAsync(hd.data(), d_d, hd.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (events && ntrial > 0)
+    MSG_HIP(hipMemcpyAsync(events, d_ev, (size_t)n * ntrial * sizeof(int), hipMemcpyDeviceToHost, s));
+  MSG_HIP(hipStreamSynchronize(s));
+  for (int k = 0; k < n; ++k) {
+    iseed[k] = hi[kChainInts * k];
+    rng_state[2 * k] = hi[kChainInts * k + 1];
+    rng_state[2 * k + 1] = hi[kChainInts * k + 2];
+    cs[k]->mc_next = hi[kChainInts * k + 3];
+    am[k] = hd[(size_t)kChainDoubles * k];
+    if (du_sum) du_sum[k] = dsum[k];
+    if (naccept) naccept[k] = (int)nacc[k];
+  }
   return MSG_OK;
 }
 

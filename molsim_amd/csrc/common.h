@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -291,6 +292,9 @@ int commit_move(Ctx* c, int ip, const double* rnew);   // MCUpdate: S(k), dipole
 int set_charge_tag(Ctx* c, int ip);                  // cell record tag after a charge-state flip
 bool chain_eligible(const Ctx* c);                 // device-resident MC chain applies
 int spme_trial_setup(Ctx* c);                      // SPME trials' real-space influence kernel g
+int mc_chain_batch(const std::vector<Ctx*>& cs, hipStream_t s, const std::function<int()>& refresh, int ntrial,
+                   int isamp, const double* dtran, double beta, int* iseed, int* rng_state, double* am,
+                   double* du_sum, int* naccept, int* events);  // one chain per system, side by side
 int mc_chain(Ctx* c, int ntrial, int isamp, const double* dtran, double beta, int* iseed,
              int* rng_state, double* am, double* du_sum, int* naccept, int* events);
 int dutotal_launch(Ctx* c, int ntrial, const int* ip, const double* rotm);

@@ -142,7 +142,10 @@ __global__ __launch_bounds__(256) void k_skin(int np, const double* __restrict__
   __syncthreads();
   if (threadIdx.x == 0) {
     const double m = fmax(fmax(cand[0], cand[1]), fmax(cand[2], cand[3]));
-    if (m > 0.0) atomicMax(skin + 1, (unsigned long long)__double_as_longlong(m));
+    // the atomic only when the value may raise the maximum so far (a stale read only lets
+    // an unneeded atomic through): thousands of workgroups on one word otherwise queue up
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+    if (m > 0.0 && bits > __atomic_load_n(skin + 1, __ATOMIC_RELAXED)) atomicMax(skin + 1, bits);
   }
 }
 

@@ -29,6 +29,7 @@ _lib = None
 LIST_MODES = {"all": 0, "cell": 1, "verlet": 2}
 EWALD_MODES = {"none": 0, "std": 1, "spm": 2}
 PRECISIONS = {"mixed": 0, "fp64": 1}
+MAX_TYPES = 8          # MSG_MAX_TYPES of the C-ABI
 KERNELS = ["bin", "scan", "pair", "spread", "fft_x", "fft_y", "fft_zconv", "fft_yi",
            "fft_xi", "gather", "reduce", "ewald_std", "dutotal", "vlist", "scatter", "finalize"]
 
@@ -128,6 +129,7 @@ def lib():
         L.msg_batch_utotal.argtypes = [_vp]
         L.msg_batch_sync.argtypes = [_vp]
         L.msg_batch_dutotal.argtypes = [_vp, _ip, _ip, _dp, _ip, _dp]
+        L.msg_batch_mc_pass.argtypes = [_vp, C.c_int, C.c_int, _dp, C.c_double, _ip, _ip, _dp, _dp, _ip, _vp]
         L.msg_batch_results.argtypes = [_vp, _vp, _vp, _vp]
         L.msg_unit_create.argtypes = [C.POINTER(_vp)]
         L.msg_unit_destroy.argtypes = [_vp]
@@ -534,6 +536,28 @@ class Batch:
             out.append(dict(overlap=ovl[off:off + n].astype(bool), du_tot=du[off:off + n]))
             off += n
         return out
+
+    def mc_pass(self, ntrial: int, dtran, beta: float, rngs, isamp: int = 0, events: bool = True):
+        """One MCPass per system, the chains side by side; rngs[s] is system s's Random
+        (updated).  dtran: one value, or per system an array over its particle types.
+        Returns (du_sum per system, naccept per system, events (nsys, ntrial) or None)."""
+        n = len(self.engines)
+        dt = np.zeros((n, MAX_TYPES))
+        for s, e in enumerate(self.engines):
+            d = np.asarray(dtran[s] if np.ndim(dtran) > 0 else dtran, dtype=np.float64)
+            dt[s, :e.spec.npt] = np.broadcast_to(d, (e.spec.npt,))
+        iseed = np.array([g.iseed.value for g in rngs], dtype=np.int32)
+        st = np.ascontiguousarray(np.concatenate([g.state for g in rngs]), dtype=np.int32)
+        am = np.array([g.am.value for g in rngs])
+        dsum, nacc = np.zeros(n), np.zeros(n, dtype=np.int32)
+        ev = np.zeros((n, ntrial), dtype=np.int32) if events else None
+        _check(lib().msg_batch_mc_pass(self.h, int(ntrial), int(isamp), np.ascontiguousarray(dt), float(beta),
+                                       iseed, st, am, dsum, nacc, _ptr(ev) if ev is not None else None))
+        for s, g in enumerate(rngs):
+            g.iseed.value = int(iseed[s])
+            g.state[:] = st[2 * s:2 * s + 2]
+            g.am.value = float(am[s])
+        return dsum, nacc, ev
 
 
 class Unit:

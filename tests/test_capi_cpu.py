@@ -29,6 +29,15 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, name), name
 
 
+def test_binding_constants_match_header():
+    """The Python mirror's type limit and mode codes are the header's."""
+    text = open(os.path.join(ROOT, "include", "molsim_gpu.h")).read()
+    assert int(re.search(r"#define MSG_MAX_TYPES (\d+)", text).group(1)) == engine.MAX_TYPES
+    for name, code in (("MSG_LIST_CELL", engine.LIST_MODES["cell"]), ("MSG_EWALD_SPM", engine.EWALD_MODES["spm"])):
+        m = re.search(name + r"\s*=\s*(\d+)", text) or re.search(r"#define " + name + r" (\d+)", text)
+        assert m and int(m.group(1)) == code, name
+
+
 @pytest.mark.parametrize("make", [systems.electrolyte_primitive, systems.polyelectrolyte])
 def test_build_table_matches_oracle_bitwise(orc, make):
     # the potentials of the two workloads, built without generating positions

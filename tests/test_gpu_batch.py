@@ -163,3 +163,52 @@ def test_batch_dutotal_and_commits(members, table):
     b.close()
     for e in engs:
         e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_mc_pass_matches_single_chains(table, precision):
+    """One MCPass per system with the chains side by side (one workgroup each, one launch per
+    SPME order; batched UTotal passes recompute phi between launches): every system makes the
+    decisions its chain makes alone, with the same generator sequence.  FP64: the same
+    decisions, final positions and generator state; mixed: at least 99 % of the decisions.
+    One member runs at order 5, so two kernel instances share each launch round."""
+    from molsim_amd.engine import Random
+    sub = [systems.ensemble_member(11, 204, "electrolyte", 0.3),
+           systems.ensemble_member(12, 260, "polyelectrolyte", 0.5),
+           systems.ensemble_member(13, 700, "electrolyte", 0.0, order=5),
+           systems.ensemble_member(14, 1500, "polyelectrolyte", 0.2),
+           systems.ensemble_member(15, 2600, "electrolyte", 0.45)]
+    ntrial, dtran, beta = 200, 1.5, systems.beta_kjmol()
+    seeds = [-101 - k for k in range(len(sub))]
+    alone = []
+    for sp, seed in zip(sub, seeds):
+        e = engine(sp, table, precision)
+        g = Random(seed)
+        _, na, ev = e.mc_pass(ntrial, dtran, beta, g)
+        alone.append((na, ev, e.positions(), g.get_state()))
+        e.close()
+    print(f"batched MC pass {precision}: {len(sub)} systems x {ntrial} trials", flush=True)
+    engs = [engine(sp, table, precision) for sp in sub]
+    b = Batch(engs)
+    rngs = [Random(s) for s in seeds]
+    _, nacc, ev = b.mc_pass(ntrial, dtran, beta, rngs)
+    for k, (na, ev1, ro1, st1) in enumerate(alone):
+        agree = float(np.mean(ev[k] == ev1))
+        print(f"  system {k} (n={sub[k].n}): accepted {nacc[k]} (alone {na}), agreement {agree:.3f}", flush=True)
+        assert 0 < nacc[k] < ntrial
+        if precision == "fp64":
+            assert np.array_equal(ev[k], ev1)
+            assert nacc[k] == na
+            assert np.abs(engs[k].positions() - ro1).max() < 1e-12
+            assert rngs[k].get_state() == st1
+        else:
+            assert agree >= 0.99
+    # the systems' state after the pass is that of their final positions
+    got = b.utotal()
+    for k in (0, 2):
+        ref = oracle_system(systems.SystemSpec(**{**sub[k].__dict__, "ro": engs[k].positions()}), table).utotal()
+        tol = 1e-9 if precision == "fp64" else 1e-5
+        compare(got[k], ref, tol, tol)
+    b.close()
+    for e in engs:
+        e.close()
