@@ -145,9 +145,21 @@ def run_ensemble(args, specs=None):
     dt_de = time.perf_counter() - t0
     print(f"# batched dE: {len(specs) * args.ntrial / dt_de:.3e} trials/s", file=sys.stderr, flush=True)
 
+    # an MCPass per system, the chains side by side (after the trials above: it moves particles)
+    from molsim_amd.engine import Random
+    beta = systems.beta_kjmol()
+    batch.mc_pass(8, 1.0, beta, [Random(-7 - s) for s in range(len(specs))], events=False)
+    rngs = [Random(-1000 - s) for s in range(len(specs))]
+    t0 = time.perf_counter()
+    _, mc_acc, _ = batch.mc_pass(args.mc_pass_trials, 1.0, beta, rngs, events=False)
+    dt_mc = time.perf_counter() - t0
+    mc_rate = len(specs) * args.mc_pass_trials / dt_mc
+    print(f"# batched MCPass: {mc_rate:.3e} trials/s ({int(mc_acc.sum())} accepted)", file=sys.stderr, flush=True)
+
     # plain FP64 GPU path, one system at a time; its results are the reference for the
     # mixed batch's errors
     t_sum, t_de, n_de = 0.0, 0.0, 0
+    t_mc1, n_mc1 = 0.0, 0
     ferr, eerr, deerr = [], [], []
     for s, sp in enumerate(specs):
         e64 = Engine(sp, precision="fp64", table=table)
@@ -174,6 +186,11 @@ def run_ensemble(args, specs=None):
                 deerr.append(abs(d["du_tot"] - de[s]["du_tot"][t]))
         t_de += time.perf_counter() - t0
         n_de += k
+        # the same system's chain alone (FP64 device chain, one system at a time)
+        t0 = time.perf_counter()
+        e64.mc_pass(args.fp64_mc_trials, 1.0, beta, Random(-1000 - s))
+        t_mc1 += time.perf_counter() - t0
+        n_mc1 += args.fp64_mc_trials
         e64.close()
     value64 = ntot / t_sum
     print(f"# plain FP64 one at a time: {1e3 * t_sum:.2f} ms per ensemble pass, {value64:.3e} "
@@ -205,6 +222,12 @@ def run_ensemble(args, specs=None):
                "fp64_one_at_a_time_trials_per_s": n_de / t_de,
                "speedup": (len(specs) * args.ntrial / dt_de) / (n_de / t_de),
                "max_abs_err_vs_fp64_kJmol": float(max(deerr)) if deerr else None},
+        "mc_pass": {"trials_per_s": mc_rate, "trials_per_system": args.mc_pass_trials,
+                    "accepted": int(mc_acc.sum()), "dtran": 1.0,
+                    "fp64_one_system_at_a_time_trials_per_s": n_mc1 / t_mc1,
+                    "speedup": mc_rate / (n_mc1 / t_mc1),
+                    "what": "one device chain per system side by side (msg_batch_mc_pass), against "
+                            "each system's FP64 device chain run alone, one system after another"},
         "err_vs_fp64_path": {"force_rel_l2_max": float(max(ferr)), "u_tot_err_max": float(max(eerr)),
                              "u_tot_err_scale": "|u_tot| + |u_rec| + max |u_twob| of the FP64 result"},
         "setup_s": {"generate": gen_s, "autotune": tune_s, "capture": cap_s},
@@ -600,6 +623,10 @@ def main():
     ap.add_argument("--ntrial", type=int, default=1000, help="ensemble: MC trials per system")
     ap.add_argument("--fp64-trials", type=int, default=20,
                     help="ensemble: trials per system timed on the plain FP64 path")
+    ap.add_argument("--mc-pass-trials", type=int, default=200,
+                    help="ensemble: MCPass trials per system (chains side by side)")
+    ap.add_argument("--fp64-mc-trials", type=int, default=40,
+                    help="ensemble: MCPass trials per system of the FP64 chain run alone")
     args = ap.parse_args()
     if args.profile_only:
         args.quick = True
@@ -754,7 +781,7 @@ def main():
         # the batched ensemble and the served request stream, summarised in the same line
         ens = run_ensemble(args_for(args, steps=args.ens_steps, warmup=3), ens_specs)
         out["ensemble"] = {k: ens[k] for k in ("value", "unit", "ms_per_step", "fp64_gpu_path", "dE",
-                                              "err_vs_fp64_path", "config")}
+                                              "mc_pass", "err_vs_fp64_path", "config")}
         # the batched ensemble against the FP64 path's single-system rate on the headline
         # workload (the FP64 path at its best, not one small system at a time)
         out["ensemble"]["speedup_vs_fp64_single_system_rate"] = ens["value"] / value64

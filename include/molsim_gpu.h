@@ -274,7 +274,8 @@ int msg_batch_sync(msg_batch* batch);
 int msg_batch_query(msg_batch* batch);
 /* One MCPass per system of a batch, the chains side by side on the device (one 1024-thread
  * workgroup each, one launch per SPME order present); with SPME, one batched UTotal pass
- * after every launch in which a chain filled its log of 16 accepted moves (no counterpart
+ * after every launch in which a chain filled its log of accepted moves (32 at orders up to
+ * 6, else 16) (no counterpart
  * in the reference, which runs one chain per process).  Every system must be SPME or
  * Ewald-free without weak charges.  Per system s: ntrial trials as msg_mc_pass;
  * dtran[s * MSG_MAX_TYPES + t] its displacement per type, iseed[s], rng_state[2 s .. 2 s + 1]

@@ -23,9 +23,10 @@
 // Each double sum over two stencils u, v (products of per-axis B-spline weights) collapses to a
 // sum over the (2p - 1)^3 mesh offsets d of g(mm_u - mm_v + d) times the product of the three
 // per-axis cross-correlations sum_a u(a) v(a + d), so the self term costs 3 (2p - 1)^3 and
-// each pending move 4 (2p - 1)^3 multiply-adds.  After kChainLog accepted moves the launch
-// ends; the host runs a UTotal pass (phi of the current positions, the patched cell order
-// kept) and launches the chain again where it stopped.
+// each pending move 4 (2p - 1)^3 multiply-adds.  After kChainLog accepted moves (batched
+// chains: up to kChainLogMax at orders up to 6) the launch ends; the host runs a UTotal pass
+// (phi of the current positions, the patched cell order kept) and launches the chain again
+// where it stopped.
 #include <cmath>
 #include <functional>
 
@@ -40,6 +41,9 @@ constexpr int kChainThreads = 1024;
 constexpr int kChainKvec = 4;   // k-vectors per thread at most: nk <= 4096
 constexpr int kChainNcut = 31;  // largest k-vector index per axis
 constexpr int kChainLog = 16;   // SPME: accepted moves corrected for before phi is recomputed
+constexpr int kChainLogMax = 32; // the log's capacity at orders up to 6 (batched chains use it)
+
+__host__ __device__ constexpr int chain_log_cap(int order) { return order <= 6 ? kChainLogMax : kChainLog; }
 
 struct ChainArgs {
   int np, ntrial, isamp, ewald, lsurf;
@@ -81,6 +85,7 @@ struct ChainArgs {
   int* next;            // sequential sampling cursor
   int* events;          // per trial: -1 overlap, 0 rejected, 1 accepted
   double* stat;         // [0] sum of accepted dE, [1] accepted, [2] trials done, [3] moves into another cell
+  int maxlog;           // SPME: accepted moves per launch (at most the log's capacity)
 };
 
 // MOLSIM's Random: Marsaglia xorshift combined with Park-Miller by Schrage's method.
@@ -165,7 +170,7 @@ template <int KV, int P>
 __device__ __forceinline__ void chain_body(const ChainArgs& a) {
   constexpr int PP = P > 0 ? P : 1;
   constexpr int W = 2 * PP - 1;
-  constexpr int NL = P > 0 ? kChainLog : 1;
+  constexpr int NL = P > 0 ? chain_log_cap(P) : 1;
   constexpr int NT = 3 + 4 * NL;  // terms: the trial's self terms, 4 per pending move
   __shared__ double sh[kChainThreads / 64];
   // e^{i n (2 pi / L_d) r_d} for n = 0..ncut, per axis d, at the trial (0) and old (1) position
@@ -437,7 +442,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs& a) {
       __threadfence_block();
     }
     __syncthreads();
-    if (P > 0 && npend == NL) break;  // phi is recomputed before the chain goes on
+    if (P > 0 && npend >= a.maxlog) break;  // phi is recomputed before the chain goes on
   }
   if (tid == 0) {
     for (int k = 0; k < 3; ++k) a.rng[k] = s_rng[k];
@@ -524,6 +529,7 @@ ChainArgs chain_args(Ctx* c, int isamp, double beta) {
   a.surf_fac = 2.0 * M_PI / (3.0 * vol);
   a.epsi4pi = c->P.epsi4pi;
   a.beta = beta;
+  a.maxlog = kChainLog;
   return a;
 }
 
@@ -680,6 +686,7 @@ int mc_chain_batch(const std::vector<Ctx*>& cs, hipStream_t s, const std::functi
   for (int r = 0; r < n; ++r) {
     const int k = idx[r];
     args[r] = chain_args(cs[k], isamp, beta);
+    args[r].maxlog = chain_log_cap(chain_order(cs[k]));  // fewer batched refreshes
     chain_state(args[r], d_i + kChainInts * k, d_d + (size_t)kChainDoubles * k, d_ev + (size_t)ntrial * k);
   }
   std::vector<int> done(n, 0);
