@@ -239,8 +239,9 @@ def run_serve(args, pool=None):
     rescaled to N log-uniform in 100..100000), Poisson arrivals of requests for energy +
     forces or for 256 single-move trial energies, continuous batching over two execution
     units with HBM-footprint admission.  Beside it: the plain FP64 one-at-a-time path (each
-    request alone, trials one DUTotal call each), whose per-request service times are
-    measured and turned into latencies by the FIFO recursion over the same arrivals."""
+    request alone, trials one DUTotal call each), served end to end in real time at the loads
+    below its capacity, and as the FIFO recursion over its measured service times at every
+    load (above capacity its queue grows without bound)."""
     from molsim_amd import serve, systems
     t0 = time.perf_counter()
     if pool is None:
@@ -252,11 +253,14 @@ def run_serve(args, pool=None):
     print(f"# serve: {len(pool)} systems, N {ns.min()}..{ns.max()} (generated in {gen_s:.1f} s)",
           file=sys.stderr, flush=True)
 
-    # plain FP64 path: service time of each (system, kind), median of 3
+    # plain FP64 path: service time of each (system, kind), median of 3; the contexts stay
+    # resident for the end-to-end run below
     rng = np.random.default_rng(11)
     svc = {}
+    e64s = {}
     for sid, sp in pool.items():
         e = Engine(sp, precision="fp64", table=table)
+        e64s[sid] = e
         e.utotal()
         ts = []
         for _ in range(3):
@@ -274,7 +278,6 @@ def run_serve(args, pool=None):
                 e.dutotal(int(ips[k]), rn[k])
             ts.append(time.perf_counter() - t1)
         svc[(sid, "de")] = float(np.median(ts))
-        e.close()
     mean_svc = 0.5 * np.mean([v for (s_, k), v in svc.items() if k == "ef"]) + \
         0.5 * np.mean([v for (s_, k), v in svc.items() if k == "de"])
     cap64 = 1.0 / mean_svc
@@ -290,6 +293,22 @@ def run_serve(args, pool=None):
         lat = np.array(lat) * 1e3
         return {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                 "throughput_rps": len(reqs) / max(end, reqs[-1].t_arrival)}
+
+    # the same path served end to end in real time at the loads it can sustain (below its
+    # capacity; above it the queue grows without bound and only the recursion is reported)
+    e2e = {}
+    for mult in args.qps_mult:
+        if mult >= 1.0:
+            continue
+        reqs = serve.request_stream(pool, qps=mult * cap64, duration=args.duration, seed=int(100 * mult),
+                                    frac_ef=0.5, ntrial=args.ntrial_serve)
+        st = serve.serve_serial(e64s, reqs)
+        e2e[mult] = st
+        print(f"# plain FP64 path served end to end at {mult:.2f}x capacity: p50 {st['latency_ms']['p50']:.2f} "
+              f"p99 {st['latency_ms']['p99']:.2f} ms ({st['requests']} requests)", file=sys.stderr, flush=True)
+    for e in e64s.values():
+        e.close()
+    e64s.clear()
 
     srv = serve.Server(pool, table, hbm_budget=int(args.hbm_gb * 2 ** 30), units=args.units,
                        max_per_unit=64)
@@ -318,7 +337,14 @@ def run_serve(args, pool=None):
                          "p99_ms_de": st.get("latency_ms_de", {}).get("p99")},
               "fp64_one_at_a_time": {"p50_ms": base["p50"], "p99_ms": base["p99"],
                                      "throughput_rps": base["throughput_rps"],
-                                     "stable": mult < 1.0}}
+                                     "stable": mult < 1.0,
+                                     "what": "FIFO recursion over the measured service times"}}
+        if mult in e2e:
+            st64 = e2e[mult]
+            lv["fp64_one_at_a_time_end_to_end"] = {
+                "p50_ms": st64["latency_ms"]["p50"], "p99_ms": st64["latency_ms"]["p99"],
+                "throughput_rps": st64["throughput_rps"], "requests": st64["requests"],
+                "what": "served in real time, one request at a time on resident FP64 contexts"}
         levels.append(lv)
         print(f"# qps {qps:9.1f}: served p50 {st['latency_ms']['p50']:.3f} p99 {st['latency_ms']['p99']:.3f} ms "
               f"({st['throughput_rps']:.0f} rps) | fp64 serial p50 {base['p50']:.2f} p99 {base['p99']:.2f} ms",

@@ -246,6 +246,34 @@ class Server:
         self.resident.clear()
 
 
+def serve_serial(engines: dict, requests: list[Request], spin: float = 20e-6) -> dict:
+    """The one-at-a-time baseline served end to end: requests replayed in real time and
+    executed in arrival order, each alone on its system's resident context (energy + forces:
+    one UTotal with the forces copied to the host; trials: one DUTotal call per move, as
+    MOLSIM's caller makes them).  Latency runs from arrival to the result on the host."""
+    reqs = sorted(requests, key=lambda r: r.t_arrival)
+    t0 = time.perf_counter()
+    for r in reqs:
+        r.t_arrival += t0
+    for r in reqs:
+        wait = r.t_arrival - time.perf_counter()
+        if wait > spin:
+            time.sleep(wait - spin)
+        while time.perf_counter() < r.t_arrival:
+            pass
+        e = engines[r.sys_id]
+        r.t_launch = time.perf_counter()
+        if r.kind == "ef":
+            r.result = e.utotal()
+        else:
+            du = np.zeros(len(r.ips))
+            for k in range(len(r.ips)):
+                du[k] = e.dutotal(int(r.ips[k]), r.rnew[k])["du_tot"]
+            r.result = {"du_tot": du}
+        r.t_done = time.perf_counter()
+    return summarize(reqs, time.perf_counter() - t0)
+
+
 def summarize(reqs, wall: float) -> dict:
     lat = np.array([r.latency for r in reqs]) * 1e3
     out = {"requests": len(reqs), "wall_s": wall, "throughput_rps": len(reqs) / wall,

@@ -133,3 +133,40 @@ def test_footprint_bounds_device_bytes(pool, table):
         print(f"{sp.name}: held {held} estimate {est}", flush=True)
         assert held <= est < 2 * held + (4 << 20)
         e.close()
+
+
+def test_server_large_systems():
+    """The upper end of the served range: 100 000-, 30 000- and 3 000-particle systems (SPME
+    256^3, 128^3, 64^3) served in a short Poisson stream of both request kinds over two units.
+    Every result equals the same request served alone, bit for bit; the 100 000-particle
+    energies and forces match the FP64 oracle; the admission estimate bounds what each context
+    holds."""
+    big = {0: systems.served_system(71, 100000), 1: systems.served_system(72, 30000),
+           2: systems.served_system(73, 3000)}
+    table = oracle.build_table(big[0].pot)
+    print("large pool: " + ", ".join(f"{sp.n} (K={sp.nmesh})" for sp in big.values()), flush=True)
+    srv = serve.Server(big, cabi_table(table), hbm_budget=1 << 36, units=2, max_per_unit=8)
+    reqs = serve.request_stream(big, qps=3000.0, duration=0.008, seed=4, frac_ef=0.5, ntrial=64)
+    kinds = {(r.sys_id, r.kind) for r in reqs}
+    stats = srv.run(reqs)
+    print(f"served {stats['requests']} requests, p99 {stats['latency_ms']['p99']:.2f} ms", flush=True)
+    assert stats["requests"] == len(reqs) and len(srv.done) == len(reqs)
+    assert (0, "ef") in kinds or (0, "de") in kinds
+    for sid, e in srv.resident.items():
+        held, est = e.device_bytes(), serve.footprint(big[sid])
+        print(f"system {sid}: held {held} estimate {est}", flush=True)
+        assert held <= est < 2 * held + (4 << 20)
+    cache = {}
+    for r in srv.done:
+        sp = big[r.sys_id]
+        if r.kind == "ef":
+            if r.sys_id not in cache:
+                cache[r.sys_id] = _alone_ef(sp, table)
+            assert _same_ef(r.result, cache[r.sys_id]), (r.rid, sp.name)
+        else:
+            alone = _alone_de(sp, table, r.ips, r.rnew)
+            assert np.array_equal(r.result["du_tot"], alone["du_tot"]), (r.rid, sp.name)
+    srv.close()
+    ef0 = cache[0] if 0 in cache else _alone_ef(big[0], table)
+    ref = oracle_system(big[0], table).utotal()
+    compare(dict(ef0, u_twob=ref["u_twob"]), ref, 1e-5, 1e-5)
