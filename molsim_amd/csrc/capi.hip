@@ -496,6 +496,7 @@ int msg_set_particles(msg_ctx* h, const double* az, const int* iptpn) {
   MSG_HIP(hipMemcpy(c->d_ipt, t.data(), c->np * sizeof(int), hipMemcpyHostToDevice));
   c->h_ipt = t;
   c->h_az.assign(az, az + c->np);
+  spme_charge_scale(c);  // the spread's fixed-point unit (captured with the pass)
   c->have_particles = true;
   c->graph_ok = false;   // the captured reduction holds the self term
   c->fresh = false;

@@ -89,6 +89,7 @@ struct MeshGrid {
   int nbrick;
   double dkdr[3]; // K / L
   int gen;        // general path: any even K, orders 2..11 (runtime-K FFT, bricks of any extent)
+  int qshift;     // FP32 spread: contributions are integers in units of 2^-qshift (30 - e, 2^e >= max |q|)
 };
 
 // Brick of mesh point m (0 <= m < K) along a dimension split into nb bricks whose first points
@@ -278,6 +279,7 @@ int pair_tile_tx(const Ctx* c);                    // its cells per tile along x
 bool pair_many(const Ctx* c);                      // more types than the LDS-table kernels take
 int default_pair_tile(const Ctx* c);               // lanes per home for a new context
 int spme_setup(Ctx* c);                            // K-space arrays
+void spme_charge_scale(Ctx* c);                    // MeshGrid::qshift from the charges
 int launch_spme_mesh(Ctx* c);                      // spread, FFTs + influence function
 int launch_spme_gather(Ctx* c, bool with_reduce = false);                    // force gather (after the pair kernel)
 int ewstd_setup(Ctx* c);
@@ -502,6 +504,13 @@ struct msg_ctx {
 // ------------------------------------------------------------------------------------------
 // Device helpers shared by the kernel files.
 namespace msgd {
+
+// Wave-local LDS hand-off (all lanes of one wavefront, no workgroup barrier).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Cardinal B-spline values (and derivatives) of order n at fractional offset x in [0,1):
 // vals[i] = M_n(x+i), ders[i] = M_n'(x+i) (Schoenberg recursion, Src/mollib.F90:1629-1683).

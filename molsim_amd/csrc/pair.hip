@@ -66,12 +66,7 @@ __device__ inline void lds_copy16(void* dst, const void* src, int nbytes) {
   }
 }
 
-// Wave-local LDS hand-off (all lanes of one wavefront, no workgroup barrier).
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+using msgd::wave_sync;
 
 constexpr int kWaves = 4;  // wavefronts per workgroup; each works through its own items
 

@@ -679,29 +679,42 @@ template <typename real> struct R4T;
 template <> struct R4T<float> { typedef float4 t; };
 template <> struct R4T<double> { typedef double4 t; };
 
-// Fixed-point scale of the brick accumulator: contributions are added as 64-bit integers,
-// so the sum is exact and independent of the order the threads add in (deterministic
-// without ordering the threads); 2^48 leaves |Q| < 32768 per mesh point with a resolution
-// of 3.6e-15, below FP64 rounding of the mesh values.
+// Fixed-point accumulation of the brick: contributions are added to 64-bit integer LDS
+// accumulators, so the sum is exact and independent of the order the threads add in
+// (deterministic without ordering the threads).
+//   FP64 path: scale 2^48 (|Q| < 32768 per mesh point, resolution 3.6e-15, below FP64
+//   rounding of the mesh values), rounded to nearest.
+//   FP32 path: the charge weights carry the scale 2^qshift, qshift = 30 - e with 2^e >= the
+//   largest |charge| of the system, so every contribution (a product of three B-spline weights
+//   <= 1 and a charge) is below 2^30 in magnitude and converts to a 32-bit integer rounded
+//   half up; the resolution 2^(e-30) (1.9e-9 for unit charges) is far below the FP32 rounding
+//   of the contribution itself.  The scaling by a power of two is exact.
 constexpr double kQScale = 281474976710656.0;  // 2^48
 
-// floor(c * 2^48) as a 64-bit two's-complement integer from FP32 arithmetic only: the high
-// word is h = floor(c * 2^16) (|c| < 2^15), the low word the fraction (c * 2^16 - h) * 2^32,
-// both exact in FP32 (the fraction of an FP32 value is representable, and scaling by a power
-// of two is exact); below |c| = 2^-25 the low word is truncated.
-__device__ __forceinline__ unsigned long long fixed48(float c) {
-  const float sc = c * 65536.0f;
-  const float h = floorf(sc);
-  const float f = (sc - h) * 4294967296.0f;
-  const unsigned lo = (unsigned)f;
-  const unsigned hi = (unsigned)(int)h;
-  return ((unsigned long long)hi << 32) | lo;
+// floor(x) as a 32-bit integer in one instruction (V_CVT_FLR_I32_F32), |x| < 2^31.
+__device__ __forceinline__ int cvt_flr_i32(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
 }
-
+// The contribution a * b (scaled) as a 64-bit accumulator increment: FP32, a * b + 0.5 in one
+// rounding, then floor (round half up); FP64, a * b * 2^48 rounded to nearest.
+__device__ __forceinline__ unsigned long long to_fixed(float a, float b) {
+  return (unsigned long long)(long long)cvt_flr_i32(__builtin_fmaf(a, b, 0.5f));
+}
+__device__ __forceinline__ unsigned long long to_fixed(double a, double b) {
+  return (unsigned long long)__double2ll_rn(a * b * kQScale);
+}
+// Weight scale folded into the charge, and the factor back to mesh values.
 template <typename real>
-__device__ __forceinline__ unsigned long long to_fixed(real c) {
-  if constexpr (sizeof(real) == 4) return fixed48(c);
-  else return (unsigned long long)__double2ll_rn(c * kQScale);
+__device__ __forceinline__ real charge_scale(const MeshGrid& m) {
+  if constexpr (sizeof(real) == 4) return ldexpf(1.0f, m.qshift);
+  else return (real)1.0;
+}
+template <typename real>
+__device__ __forceinline__ double fixed_unit(const MeshGrid& m) {
+  if constexpr (sizeof(real) == 4) return ldexp(1.0, -m.qshift);
+  else return 1.0 / kQScale;
 }
 
 // Stencil range of one dimension inside a brick of extent B starting at the particle's
@@ -718,27 +731,52 @@ __device__ inline void brick_range(int l, int B, bool whole, int& i0, int& n) {
   }
 }
 
+// Exclusive prefix over the wavefront's lanes of v (0 <= v < 16) and the wavefront total,
+// from four ballots of its bits.
+__device__ __forceinline__ int wave_prefix4(int v, int lane, int& total) {
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int bit = 0; bit < 4; ++bit) {
+    const unsigned long long m = __ballot((v >> bit) & 1);
+    pre += __popcll(m & lt) << bit;
+    tot += __popcll(m) << bit;
+  }
+  total = tot;
+  return pre;
+}
+
 // One workgroup per brick.  The candidates are the particles of the brick and of the next
-// brick in each dimension (their stencils reach back into this one), taken 256 at a time:
-// (1) every thread clips its candidate's stencil to the brick from the packed base point,
-// and the candidates with points inside are compacted; (2) the thread that found a
-// contributor puts its B-spline weights (charge folded into z) in LDS at the contributor's
-// place; (3) the threads take (contributor, z offset) tasks and add the clipped y rows of
-// clipped x points into the brick with 64-bit integer LDS atomics.  Finally every mesh point
-// of the brick is written once.
+// brick in each dimension (their stencils reach back into this one).  The four wavefronts
+// work independently (no workgroup barrier between the zeroed accumulator and the final
+// write): each takes 64 candidates at a time, the next 64 loaded while it works on these.
+// (1) Every lane clips its candidate's stencil to the brick from the packed base point; the
+// candidates with points inside are ranked by ballot.  (2) A contributor writes its record to
+// the wavefront's LDS area (x weights with those of points outside the brick set to zero, y
+// weights, z weights times the charge, local base point, clipped y extent) and its tasks,
+// one per clipped z plane, at offsets from a ballot prefix sum; a task code carries the
+// contributor, the z index and the first clipped y offset, so a task's reads (record, y
+// weights, z weight) are issued together.  (3) The lanes take the tasks and add the clipped
+// y rows of P x points into the brick with 64-bit integer LDS atomics.  The x points outside
+// the brick add zero at addresses that stay inside the padded array.  Finally every mesh
+// point of the brick is written once.
 template <typename real, int P>
 __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   typedef typename R4T<real>::t R4;
-  constexpr int CAP = 256;
+  constexpr int NW = 4, WC = 64;  // wavefronts, candidates per wavefront round
   constexpr int NBP = kBX * kBY * kBZ;
   // rows padded to kBX + 1 accumulators: the lanes adding one particle's rows (same x,
-  // consecutive y/z) then fall into different LDS banks instead of one
+  // consecutive y/z) then fall into different LDS banks instead of one.  A row's x points
+  // lie at lx0 - ix, lx0 < 2 kBX (own or next brick), ix < P: the array is padded by P - 1
+  // in front and 2 kBX behind, so the zero-weight points of any row stay inside it.
   constexpr int ROW = kBX + 1, NACC = ROW * kBY * kBZ;
-  __shared__ unsigned long long acc[NACC];
-  __shared__ real th[CAP][3 * P];
-  __shared__ int loc[CAP];      // local base point lx | ly << 10 | lz << 20
-  __shared__ int rng[CAP];      // clipped ranges x0 | nx << 4 | y0 << 8 | ny << 12 | z0 << 16 | nz << 20
-  __shared__ int wcnt[4];
+  constexpr int PADL = P - 1;
+  // contributor record: wx[P] | wy[P] | wz[P] | base point | ny, padded to 16 bytes
+  constexpr int RW = sizeof(real) == 4 ? ((3 * P + 2 + 3) & ~3) : ((3 * P + 2 + 1) & ~1);
+  __shared__ unsigned long long acc_s[PADL + NACC + 2 * kBX];
+  unsigned long long* const acc = acc_s + PADL;
+  __shared__ __align__(16) real rec_s[NW][WC][RW];
+  __shared__ unsigned short tsk_s[NW][WC * P];  // tasks: contributor | z index << 8 | y0 << 11
   __shared__ int r_beg[8], r_pref[9], r_len[8];
   const int K = a.m.K;
   const int b = bid;
@@ -759,7 +797,7 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
     r_beg[t] = beg;
     r_len[t] = len;
   }
-  for (int k = threadIdx.x; k < NACC; k += 256) acc[k] = 0ull;
+  for (int k = threadIdx.x; k < PADL + NACC + 2 * kBX; k += 256) acc_s[k] = 0ull;
   __syncthreads();
   if (threadIdx.x == 0) {
     int pref = 0;
@@ -770,23 +808,46 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
     r_pref[8] = pref;
   }
   __syncthreads();
-  const int total = r_pref[8];
+  // the source ranges in registers: a candidate's place is found without dependent LDS reads
+  int pref[9], dlt[8];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) pref[t] = r_pref[t];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) dlt[t] = r_beg[t] - pref[t];
+  const int total = pref[8];
   const R4* bq = (const R4*)a.bq;
+  const real qs = charge_scale<real>(a.m);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int base = 0; base < total; base += CAP) {
-    const int t = base + threadIdx.x;
+  real (*rec)[RW] = rec_s[w];
+  unsigned short* tsk = tsk_s[w];
+  int t = w * WC + lane;
+  int packed = 0;
+  R4 v{};
+  if (t < total) {
+    int d = dlt[0];
+#pragma unroll
+    for (int r = 1; r < 8; ++r) d = t >= pref[r] ? dlt[r] : d;
+    packed = a.bmm[t + d];
+    v = bq[t + d];
+  }
+  for (; t - lane < total; t += NW * WC) {
+    // the next round's candidate, loaded while this one is worked on
+    const int tn = t + NW * WC;
+    int packed_n = 0;
+    R4 v_n{};
+    if (tn < total) {
+      int d = dlt[0];
+#pragma unroll
+      for (int r = 1; r < 8; ++r) d = tn >= pref[r] ? dlt[r] : d;
+      packed_n = a.bmm[tn + d];
+      v_n = bq[tn + d];
+    }
     bool hit = false;
     int lp = 0, rr = 0;
-    R4 v;
     if (t < total) {
-      int r = 0;
-      while (r < 7 && t >= r_pref[r + 1]) ++r;
-      const int idx = r_beg[r] + (t - r_pref[r]);
-      const int packed = a.bmm[idx];
-      v = bq[idx];
       int l[3] = {(packed & 1023) - lo[0], ((packed >> 10) & 1023) - lo[1], (packed >> 20) - lo[2]};
 #pragma unroll
-      for (int d = 0; d < 3; ++d) l[d] += l[d] < 0 ? K : 0;
+      for (int e = 0; e < 3; ++e) l[e] += l[e] < 0 ? K : 0;
       int x0, nx, y0, ny, z0, nz;
       brick_range<P>(l[0], kBX, whole[0], x0, nx);
       brick_range<P>(l[1], kBY, whole[1], y0, ny);
@@ -796,61 +857,80 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
       rr = x0 | (nx << 4) | (y0 << 8) | (ny << 12) | (z0 << 16) | (nz << 20);
     }
     const unsigned long long bal = __ballot(hit);
-    if (lane == 0) wcnt[w] = __popcll(bal);
-    __syncthreads();
-    int off = 0;
-    for (int k = 0; k < w; ++k) off += wcnt[k];
-    const int ncon = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
-    if (hit) {
-      const int k = off + __popcll(bal & ((1ull << lane) - 1ull));
-      loc[k] = lp;
-      rng[k] = rr;
-      real* T = th[k];
-      msgd::bspline<real, P>(v.x, T, (real*)nullptr);
-      msgd::bspline<real, P>(v.y, T + P, (real*)nullptr);
-      msgd::bspline<real, P>(v.z, T + 2 * P, (real*)nullptr);
+    if (bal) {  // wavefront-uniform
+      int ntask;
+      const int t0 = wave_prefix4(hit ? (rr >> 20) & 15 : 0, lane, ntask);
+      if (hit) {
+        const int k = __popcll(bal & ((1ull << lane) - 1ull));
+        const int x0 = rr & 15, nx = (rr >> 4) & 15, y0 = (rr >> 8) & 15, ny = (rr >> 12) & 15;
+        const int z0 = (rr >> 16) & 15, nz = (rr >> 20) & 15;
+        real wx[P], wy[P], wz[P];
+        msgd::bspline<real, P>(v.x, wx, (real*)nullptr);
+        msgd::bspline<real, P>(v.y, wy, (real*)nullptr);
+        msgd::bspline<real, P>(v.z, wz, (real*)nullptr);
+        const real qw = v.w * qs;
+        real* R = rec[k];
 #pragma unroll
-      for (int i = 0; i < P; ++i) T[2 * P + i] *= v.w;
-    }
-    __syncthreads();
-    for (int task = threadIdx.x; task < ncon * P; task += 256) {
-      const int k = task / P, zs = task - k * P;
-      const int cr = rng[k];
-      if (zs >= ((cr >> 20) & 15)) continue;
-      const int x0 = cr & 15, nx = (cr >> 4) & 15, y0 = (cr >> 8) & 15, ny = (cr >> 12) & 15;
-      const int iz = ((cr >> 16) & 15) + zs;
-      const int cl = loc[k];
-      const int lx0 = cl & 1023, ly0 = (cl >> 10) & 1023;
-      int lz = (cl >> 20) - iz;
-      lz += lz < 0 ? K : 0;  // wraps only where one brick spans the mesh
-      const real* T = th[k];
-      const real wz = T[2 * P + iz];
-      real tx[P];
+        for (int i = 0; i < P; ++i) {
+          R[i] = (i >= x0 && i < x0 + nx) ? wx[i] : (real)0;
+          R[P + i] = wy[i];
+          R[2 * P + i] = wz[i] * qw;
+        }
+        ((int*)(R + 3 * P))[0] = lp;
+        ((int*)(R + 3 * P))[sizeof(real) / sizeof(int)] = ny;
+        for (int j = 0; j < nz; ++j) tsk[t0 + j] = (unsigned short)(k | ((z0 + j) << 8) | (y0 << 11));
+      }
+      msgd::wave_sync();
+      for (int task = lane; task < ntask; task += 64) {
+        const int code = tsk[task];
+        const int k = code & 255, iz = (code >> 8) & 7, y0 = code >> 11;
+        const real* R = rec[k];
+        // every read of the task depends only on its code
+        real tx[P], ty[P];
 #pragma unroll
-      for (int ix = 0; ix < P; ++ix) tx[ix] = T[ix];
+        for (int i = 0; i < P; ++i) {
+          tx[i] = R[i];
+          ty[i] = R[P + y0 + i];  // P + y0 + i < 3 P: inside the record; beyond ny unused
+        }
+        const real wz = R[2 * P + iz];
+        const int cl = ((const int*)(R + 3 * P))[0];
+        const int ny = ((const int*)(R + 3 * P))[sizeof(real) / sizeof(int)];
+        const int lx0 = cl & 1023, ly0 = ((cl >> 10) & 1023) - y0;
+        int lz = (cl >> 20) - iz;
+        lz += lz < 0 ? K : 0;  // wraps only where one brick spans the mesh
 #pragma unroll
-      for (int ys = 0; ys < P; ++ys) {
-        if (ys >= ny) break;
-        const int iy = y0 + ys;
-        int ly = ly0 - iy;
-        ly += ly < 0 ? K : 0;
-        const real wyz = T[P + iy] * wz;
-        unsigned long long* rowp = acc + ROW * (ly + kBY * lz);
+        for (int ys = 0; ys < P; ++ys) {
+          if (ys >= ny) break;
+          int ly = ly0 - ys;
+          ly += ly < 0 ? K : 0;
+          const real wyz = ty[ys] * wz;
+          unsigned long long* rowp = acc + ROW * (ly + kBY * lz);
+          if (whole[0]) {
 #pragma unroll
-        for (int ix = 0; ix < P; ++ix) {
-          if (ix < x0 || ix >= x0 + nx) continue;
-          int lx = lx0 - ix;
-          lx += lx < 0 ? K : 0;
-          atomicAdd(rowp + lx, to_fixed<real>(tx[ix] * wyz));
+            for (int ix = 0; ix < P; ++ix) {
+              int lx = lx0 - ix;
+              lx += lx < 0 ? K : 0;
+              atomicAdd(rowp + lx, to_fixed(tx[ix], wyz));
+            }
+          } else {
+            // x points lx0 - ix as constant offsets from one base
+            unsigned long long* rp = rowp + lx0 - (P - 1);
+#pragma unroll
+            for (int ix = 0; ix < P; ++ix) atomicAdd(rp + (P - 1 - ix), to_fixed(tx[ix], wyz));
+          }
         }
       }
+      msgd::wave_sync();  // the records and tasks are rewritten next round
     }
-    __syncthreads();
+    packed = packed_n;
+    v = v_n;
   }
+  __syncthreads();
   real* Q = (real*)a.Q;
+  const double unit = fixed_unit<real>(a.m);
   for (int k = threadIdx.x; k < NBP; k += 256) {
     const int lx = k % kBX, ly = (k / kBX) % kBY, lz = k / (kBX * kBY);
-    const double v = (double)(long long)acc[ROW * (ly + kBY * lz) + lx] * (1.0 / kQScale);
+    const double v = (double)(long long)acc[ROW * (ly + kBY * lz) + lx] * unit;
     Q[(lo[0] + lx) + (size_t)K * ((lo[1] + ly) + (size_t)K * (lo[2] + lz))] = (real)v;
   }
 }
@@ -879,7 +959,9 @@ __device__ __forceinline__ void spread_body_g(const SpreadArgs& a, int bid) {
   __shared__ real th[CAP][3 * P];
   __shared__ int loc[CAP];
   __shared__ int rng[CAP];
+  __shared__ unsigned short tsk[CAP * P];  // tasks: contributor | z plane << 8
   __shared__ int wcnt[4];
+  __shared__ int ntsk;
   __shared__ int r_beg[8], r_pref[9], r_len[8];
   const int K = a.m.K;
   const int nbx = a.m.nb[0], nby = a.m.nb[1], nbz = a.m.nb[2];
@@ -906,6 +988,7 @@ __device__ __forceinline__ void spread_body_g(const SpreadArgs& a, int bid) {
     }
     r_beg[t] = beg;
     r_len[t] = len;
+    if (t == 0) ntsk = 0;
   }
   for (int k = threadIdx.x; k < NACC; k += 256) acc_g[k] = 0ull;
   __syncthreads();
@@ -920,6 +1003,7 @@ __device__ __forceinline__ void spread_body_g(const SpreadArgs& a, int bid) {
   __syncthreads();
   const int total = r_pref[8];
   const R4* bq = (const R4*)a.bq;
+  const real qs = charge_scale<real>(a.m);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int base = 0; base < total; base += CAP) {
     const int t = base + threadIdx.x;
@@ -948,7 +1032,6 @@ __device__ __forceinline__ void spread_body_g(const SpreadArgs& a, int bid) {
     __syncthreads();
     int off = 0;
     for (int k = 0; k < w; ++k) off += wcnt[k];
-    const int ncon = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
     if (hit) {
       const int k = off + __popcll(bal & ((1ull << lane) - 1ull));
       loc[k] = lp;
@@ -957,14 +1040,19 @@ __device__ __forceinline__ void spread_body_g(const SpreadArgs& a, int bid) {
       msgd::bspline<real, P>(v.x, T, (real*)nullptr);
       msgd::bspline<real, P>(v.y, T + P, (real*)nullptr);
       msgd::bspline<real, P>(v.z, T + 2 * P, (real*)nullptr);
+      const real qw = v.w * qs;
 #pragma unroll
-      for (int i = 0; i < P; ++i) T[2 * P + i] *= v.w;
+      for (int i = 0; i < P; ++i) T[2 * P + i] *= qw;
+      const int nz = (rr >> 20) & 15;
+      const int t0 = atomicAdd(&ntsk, nz);  // task order varies; integer sums do not
+      for (int j = 0; j < nz; ++j) tsk[t0 + j] = (unsigned short)(k | (j << 8));
     }
     __syncthreads();
-    for (int task = threadIdx.x; task < ncon * P; task += 256) {
-      const int k = task / P, zs = task - k * P;
+    const int ntask = ntsk;
+    for (int task = threadIdx.x; task < ntask; task += 256) {
+      const int code = tsk[task];
+      const int k = code & 255, zs = code >> 8;
       const int cr = rng[k];
-      if (zs >= ((cr >> 20) & 15)) continue;
       const int x0 = cr & 15, nx = (cr >> 4) & 15, y0 = (cr >> 8) & 15, ny = (cr >> 12) & 15;
       const int iz = ((cr >> 16) & 15) + zs;
       const int cl = loc[k];
@@ -982,17 +1070,19 @@ __device__ __forceinline__ void spread_body_g(const SpreadArgs& a, int bid) {
         for (int ix = x0; ix < x0 + nx; ++ix) {
           int lx = lx0 - ix;
           lx += lx < 0 ? K : 0;
-          atomicAdd(rowp + lx, to_fixed<real>(T[ix] * wyz));
+          atomicAdd(rowp + lx, to_fixed(T[ix], wyz));
         }
       }
     }
     __syncthreads();
+    if (threadIdx.x == 0) ntsk = 0;  // read by every thread before the barrier above
   }
   real* Q = (real*)a.Q;
   const int npts = ext[0] * ext[1] * ext[2];
+  const double unit = fixed_unit<real>(a.m);
   for (int k = threadIdx.x; k < npts; k += 256) {
     const int lx = k % ext[0], ly = (k / ext[0]) % ext[1], lz = k / (ext[0] * ext[1]);
-    const double v = (double)(long long)acc_g[ROW * (ly + ext[1] * lz) + lx] * (1.0 / kQScale);
+    const double v = (double)(long long)acc_g[ROW * (ly + ext[1] * lz) + lx] * unit;
     Q[(lo[0] + lx) + (size_t)K * ((lo[1] + ly) + (size_t)K * (lo[2] + lz))] = (real)v;
   }
 }
@@ -1014,11 +1104,6 @@ inline size_t spread_g_lds(const MeshGrid& m) {
   for (int d = 0; d < 3; ++d) e[d] = (m.K + m.nb[d] - 1) / m.nb[d];
   return (size_t)(e[0] + 1) * e[1] * e[2] * sizeof(unsigned long long);
 }
-template <typename real, int P>
-constexpr size_t spread_g_static_lds() {
-  return (size_t)(P <= 8 ? 128 : 64) * (3 * P * sizeof(real) + 2 * sizeof(int)) + 128;
-}
-
 struct GatherArgs {
   MeshGrid m;
   int np;
@@ -1349,6 +1434,16 @@ int spread_gather_dispatch(Ctx* c, bool spread, bool with_reduce = false) {
 
 // EwaldSetup, SPME branch (Src/energy.F90:6969-7018): B-spline moduli b^2(m) and the
 // influence function meshfac over [0..K/2]^3, computed in FP64 on the host.
+// 30 - e for the smallest e >= 0 with 2^e >= max |az|: a charge times three B-spline weights
+// (each <= 1) scaled by 2^qshift is below 2^30 in magnitude.
+void spme_charge_scale(Ctx* c) {
+  double qmax = 0.0;
+  for (double q : c->h_az) qmax = std::max(qmax, std::fabs(q));
+  int e = 0;
+  while (e < 60 && std::ldexp(1.0, e) < qmax) ++e;
+  c->mesh.qshift = 30 - e;
+}
+
 int spme_setup(Ctx* c) {
   const msg_params& P = c->P;
   const int K = P.nmesh, p = P.order;
@@ -1390,7 +1485,7 @@ int spme_setup(Ctx* c) {
     // bricks of at least max(8, p - 1) points, so a stencil reaches only the next brick
     const int bmin = std::max(8, p - 1);
     for (int d = 0; d < 3; ++d) m.nb[d] = std::max(1, K / bmin);
-    const size_t lds = spread_g_lds(m) + (c->fp64 ? (p <= 8 ? 128 : 64) * (3 * p * 8 + 8) : (p <= 8 ? 128 : 64) * (3 * p * 4 + 8)) + 256;
+    const size_t lds = spread_g_lds(m) + (p <= 8 ? 128 : 64) * (3 * p * (c->fp64 ? 8 : 4) + 8 + 2 * p) + 256;
     if (lds > 65536) return set_error(MSG_ERR_MESH, "nmesh too small for this SPME order (one brick spans the mesh)");
     if (K > (c->fp64 ? FftCfg<double>::E : FftCfg<float>::E) - 1)
       return set_error(MSG_ERR_MESH, "nmesh too large for the general FFT");
@@ -1405,6 +1500,7 @@ int spme_setup(Ctx* c) {
   }
   m.nbrick = m.nb[0] * m.nb[1] * m.nb[2];
   for (int d = 0; d < 3; ++d) m.dkdr[d] = K / P.boxlen[d];
+  spme_charge_scale(c);
   // spline at pos = 0: M_p(1..p-1) are the only non-zero integer-point values
   std::vector<double> spl(p, 0.0);
   {
