@@ -679,16 +679,16 @@ template <typename real> struct R4T;
 template <> struct R4T<float> { typedef float4 t; };
 template <> struct R4T<double> { typedef double4 t; };
 
-// Fixed-point accumulation of the brick: contributions are added to 64-bit integer LDS
-// accumulators, so the sum is exact and independent of the order the threads add in
-// (deterministic without ordering the threads).
-//   FP64 path: scale 2^48 (|Q| < 32768 per mesh point, resolution 3.6e-15, below FP64
-//   rounding of the mesh values), rounded to nearest.
-//   FP32 path: the charge weights carry the scale 2^qshift, qshift = 30 - e with 2^e >= the
-//   largest |charge| of the system, so every contribution (a product of three B-spline weights
-//   <= 1 and a charge) is below 2^30 in magnitude and converts to a 32-bit integer rounded
-//   half up; the resolution 2^(e-30) (1.9e-9 for unit charges) is far below the FP32 rounding
-//   of the contribution itself.  The scaling by a power of two is exact.
+// Fixed-point accumulation of the brick: contributions are added to integer LDS accumulators,
+// so the sum is exact and independent of the order the threads add in (deterministic without
+// ordering the threads).
+//   FP64 path: 64-bit accumulators, scale 2^48 (|Q| < 32768 per mesh point, resolution
+//   3.6e-15, below FP64 rounding of the mesh values), rounded to nearest.
+//   FP32 path: contributions are rounded half up to integers in units of 2^-s.  The charge
+//   weights carry the scale 2^s (exact: a power of two).  The general-mesh spread uses 64-bit
+//   accumulators with s = qshift = 30 - e, 2^e >= the largest |charge|: every contribution (a
+//   product of three B-spline weights <= 1 and a charge) is below 2^30 in magnitude.  The
+//   power-of-two spread uses 32-bit accumulators with a scale per brick (brick_shift).
 constexpr double kQScale = 281474976710656.0;  // 2^48
 
 // floor(x) as a 32-bit integer in one instruction (V_CVT_FLR_I32_F32), |x| < 2^31.
@@ -705,6 +705,34 @@ __device__ __forceinline__ unsigned long long to_fixed(float a, float b) {
 __device__ __forceinline__ unsigned long long to_fixed(double a, double b) {
   return (unsigned long long)__double2ll_rn(a * b * kQScale);
 }
+// Largest product of three order-P B-spline weights, (max_x M_P(x))^3, rounded up.
+__host__ __device__ constexpr float wmax3(int P) {
+  return P <= 2 ? 1.0f : P == 3 ? 0.4220f : P == 4 ? 0.2964f : P == 5 ? 0.2150f : P == 6 ? 0.1665f
+       : P == 7 ? 0.1335f : 0.1102f;
+}
+// ceil(log2(x)) for x > 0 (the power of two above or at x); 0 for x = 0.
+__device__ __forceinline__ int ceil_log2f(float x) {
+  int e;
+  const float m = frexpf(x, &e);
+  return m == 0.5f ? e - 1 : e;
+}
+// Scale exponent s of a brick's 32-bit accumulators.  A mesh point receives at most one
+// contribution from each of the brick's ncand candidates, each at most 2^e * wmax3(P) in
+// magnitude, so 2^s * ncand * 2^e * wmax3(P) <= 2^30 bounds every contribution and every sum
+// (with the half units of rounding) below 2^31: the integer sums are exact.
+template <int P>
+__device__ __forceinline__ int brick_shift(const MeshGrid& m, int ncand) {
+  return m.qshift - ceil_log2f((float)ncand * wmax3(P));
+}
+__device__ __forceinline__ void spread_add(unsigned* p, float a, float b) {
+  atomicAdd(p, (unsigned)cvt_flr_i32(__builtin_fmaf(a, b, 0.5f)));
+}
+__device__ __forceinline__ void spread_add(unsigned long long* p, double a, double b) {
+  atomicAdd(p, to_fixed(a, b));
+}
+__device__ __forceinline__ double acc_value(unsigned v) { return (double)(int)v; }
+__device__ __forceinline__ double acc_value(unsigned long long v) { return (double)(long long)v; }
+
 // Weight scale folded into the charge, and the factor back to mesh values.
 template <typename real>
 __device__ __forceinline__ real charge_scale(const MeshGrid& m) {
@@ -757,13 +785,15 @@ __device__ __forceinline__ int wave_prefix4(int v, int lane, int& total) {
 // one per clipped z plane, at offsets from a ballot prefix sum; a task code carries the
 // contributor, the z index and the first clipped y offset, so a task's reads (record, y
 // weights, z weight) are issued together.  (3) The lanes take the tasks and add the clipped
-// y rows of P x points into the brick with 64-bit integer LDS atomics.  The x points outside
+// y rows of P x points into the brick with integer LDS atomics (32-bit in FP32).  The x points outside
 // the brick add zero at addresses that stay inside the padded array.  Finally every mesh
 // point of the brick is written once.
 template <typename real, int P>
 __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   typedef typename R4T<real>::t R4;
-  constexpr int NW = 4, WC = 64;  // wavefronts, candidates per wavefront round
+  // wavefronts, candidates per wavefront round, contributors per record pass (a round with
+  // more contributors takes two passes: half the record area, more workgroups per CU)
+  constexpr int NW = 4, WC = 64, HC = 32;
   constexpr int NBP = kBX * kBY * kBZ;
   // rows padded to kBX + 1 accumulators: the lanes adding one particle's rows (same x,
   // consecutive y/z) then fall into different LDS banks instead of one.  A row's x points
@@ -773,10 +803,11 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   constexpr int PADL = P - 1;
   // contributor record: wx[P] | wy[P] | wz[P] | base point | ny, padded to 16 bytes
   constexpr int RW = sizeof(real) == 4 ? ((3 * P + 2 + 3) & ~3) : ((3 * P + 2 + 1) & ~1);
-  __shared__ unsigned long long acc_s[PADL + NACC + 2 * kBX];
-  unsigned long long* const acc = acc_s + PADL;
-  __shared__ __align__(16) real rec_s[NW][WC][RW];
-  __shared__ unsigned short tsk_s[NW][WC * P];  // tasks: contributor | z index << 8 | y0 << 11
+  typedef typename std::conditional<sizeof(real) == 4, unsigned, unsigned long long>::type acc_t;
+  __shared__ acc_t acc_s[PADL + NACC + 2 * kBX];
+  acc_t* const acc = acc_s + PADL;
+  __shared__ __align__(16) real rec_s[NW][HC][RW];
+  __shared__ unsigned short tsk_s[NW][HC * P];  // tasks: contributor | z index << 8 | y0 << 11
   __shared__ int r_beg[8], r_pref[9], r_len[8];
   const int K = a.m.K;
   const int b = bid;
@@ -797,7 +828,7 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
     r_beg[t] = beg;
     r_len[t] = len;
   }
-  for (int k = threadIdx.x; k < PADL + NACC + 2 * kBX; k += 256) acc_s[k] = 0ull;
+  for (int k = threadIdx.x; k < PADL + NACC + 2 * kBX; k += 256) acc_s[k] = 0;
   __syncthreads();
   if (threadIdx.x == 0) {
     int pref = 0;
@@ -808,15 +839,17 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
     r_pref[8] = pref;
   }
   __syncthreads();
-  // the source ranges in registers: a candidate's place is found without dependent LDS reads
+  // the source ranges in scalar registers: a candidate's place is found without LDS reads
   int pref[9], dlt[8];
 #pragma unroll
-  for (int t = 0; t < 9; ++t) pref[t] = r_pref[t];
+  for (int t = 0; t < 9; ++t) pref[t] = __builtin_amdgcn_readfirstlane(r_pref[t]);
 #pragma unroll
-  for (int t = 0; t < 8; ++t) dlt[t] = r_beg[t] - pref[t];
+  for (int t = 0; t < 8; ++t) dlt[t] = __builtin_amdgcn_readfirstlane(r_beg[t]) - pref[t];
   const int total = pref[8];
   const R4* bq = (const R4*)a.bq;
-  const real qs = charge_scale<real>(a.m);
+  // FP32: the brick's scale from its candidate count; FP64: 2^48 in the conversion
+  const int sh = sizeof(real) == 4 ? brick_shift<P>(a.m, total) : 0;
+  const real qs = sizeof(real) == 4 ? (real)ldexpf(1.0f, sh) : (real)1;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   real (*rec)[RW] = rec_s[w];
   unsigned short* tsk = tsk_s[w];
@@ -857,14 +890,17 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
       rr = x0 | (nx << 4) | (y0 << 8) | (ny << 12) | (z0 << 16) | (nz << 20);
     }
     const unsigned long long bal = __ballot(hit);
-    if (bal) {  // wavefront-uniform
+    const int nc = __popcll(bal);  // wavefront-uniform
+    const int kk = __popcll(bal & ((1ull << lane) - 1ull));
+    for (int kb = 0; kb < nc; kb += HC) {
+      const bool mine = hit && kk >= kb && kk < kb + HC;
       int ntask;
-      const int t0 = wave_prefix4(hit ? (rr >> 20) & 15 : 0, lane, ntask);
-      if (hit) {
-        const int k = __popcll(bal & ((1ull << lane) - 1ull));
+      const int t0 = wave_prefix4(mine ? (rr >> 20) & 15 : 0, lane, ntask);
+      if (mine) {
+        const int k = kk - kb;
         const int x0 = rr & 15, nx = (rr >> 4) & 15, y0 = (rr >> 8) & 15, ny = (rr >> 12) & 15;
         const int z0 = (rr >> 16) & 15, nz = (rr >> 20) & 15;
-        real wx[P], wy[P], wz[P];
+        real wx[P], wy[P], wz[P];  // each contributor is in exactly one pass
         msgd::bspline<real, P>(v.x, wx, (real*)nullptr);
         msgd::bspline<real, P>(v.y, wy, (real*)nullptr);
         msgd::bspline<real, P>(v.z, wz, (real*)nullptr);
@@ -904,42 +940,45 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
           int ly = ly0 - ys;
           ly += ly < 0 ? K : 0;
           const real wyz = ty[ys] * wz;
-          unsigned long long* rowp = acc + ROW * (ly + kBY * lz);
+          acc_t* rowp = acc + ROW * (ly + kBY * lz);
           if (whole[0]) {
 #pragma unroll
             for (int ix = 0; ix < P; ++ix) {
               int lx = lx0 - ix;
               lx += lx < 0 ? K : 0;
-              atomicAdd(rowp + lx, to_fixed(tx[ix], wyz));
+              spread_add(rowp + lx, tx[ix], wyz);
             }
           } else {
             // x points lx0 - ix as constant offsets from one base
-            unsigned long long* rp = rowp + lx0 - (P - 1);
+            acc_t* rp = rowp + lx0 - (P - 1);
 #pragma unroll
-            for (int ix = 0; ix < P; ++ix) atomicAdd(rp + (P - 1 - ix), to_fixed(tx[ix], wyz));
+            for (int ix = 0; ix < P; ++ix) spread_add(rp + (P - 1 - ix), tx[ix], wyz);
           }
         }
       }
-      msgd::wave_sync();  // the records and tasks are rewritten next round
+      msgd::wave_sync();  // the records and tasks are rewritten by the next pass
     }
     packed = packed_n;
     v = v_n;
   }
   __syncthreads();
   real* Q = (real*)a.Q;
-  const double unit = fixed_unit<real>(a.m);
+  const double unit = sizeof(real) == 4 ? ldexp(1.0, -sh) : 1.0 / kQScale;
   for (int k = threadIdx.x; k < NBP; k += 256) {
     const int lx = k % kBX, ly = (k / kBX) % kBY, lz = k / (kBX * kBY);
-    const double v = (double)(long long)acc[ROW * (ly + kBY * lz) + lx] * unit;
+    const double v = acc_value(acc[ROW * (ly + kBY * lz) + lx]) * unit;
     Q[(lo[0] + lx) + (size_t)K * ((lo[1] + ly) + (size_t)K * (lo[2] + lz))] = (real)v;
   }
 }
 
+// Registers capped for six workgroups per CU (the LDS use allows seven).
 template <typename real, int P>
-__global__ __launch_bounds__(256) void k_spread(SpreadArgs a) { spread_body<real, P>(a, blockIdx.x); }
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_spread(SpreadArgs a) {
+  spread_body<real, P>(a, blockIdx.x);
+}
 
 template <typename real, int P>
-__global__ __launch_bounds__(256) void k_spread_seg(const SpreadArgs* __restrict__ args,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_spread_seg(const SpreadArgs* __restrict__ args,
                                                     const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
   if (s.x < 0) return;  // no work for this XCD slot
