@@ -56,93 +56,127 @@ template <typename real> struct FftCfg;
 template <> struct FftCfg<float> { static constexpr int E = 2048, NT = 256; };
 template <> struct FftCfg<double> { static constexpr int E = 1024, NT = 256; };
 
-// One radix-4 butterfly of a Stockham stage (Ns = product of the earlier radices) of a line of
-// length K: reads s[j + m K/4], writes d[(j / Ns) 4 Ns + j mod Ns + m Ns], m = 0..3.
-template <typename C2, int K>
-__device__ __forceinline__ void r4_butterfly(const C2* s, C2* dd, int j, int Ns, const C2* __restrict__ tw,
-                                             bool inverse) {
-  const int step = K / (Ns * 4);
-  const int jr = j % Ns;
-  C2 v0 = s[j], v1 = s[j + K / 4], v2 = s[j + K / 2], v3 = s[j + 3 * K / 4];
-  C2 w1 = tw[jr * step], w2 = tw[2 * jr * step], w3 = tw[3 * jr * step];
-  if (inverse) { w1 = cconj(w1); w2 = cconj(w2); w3 = cconj(w3); }
-  v1 = cmul(v1, w1);
-  v2 = cmul(v2, w2);
-  v3 = cmul(v3, w3);
-  const C2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3);
-  const C2 d13 = csub(v1, v3);
-  // forward: a3 = -i (v1 - v3); inverse: +i (v1 - v3)
-  const C2 a3 = inverse ? cmk<C2>(-d13.y, d13.x) : cmk<C2>(d13.y, -d13.x);
-  C2* d = dd + (j / Ns) * Ns * 4 + jr;
-  d[0] = cadd(a0, a2);
-  d[Ns] = cadd(a1, a3);
-  d[2 * Ns] = csub(a0, a2);
-  d[3 * Ns] = csub(a1, a3);
+// ------------------------------------------------------------------------------------------
+// Power-of-two meshes: register-resident line transforms.  A line of K = 8 M points is held by
+// M consecutive lanes, 8 points per lane: lane b holds x[M n1 + b], n1 = 0..7.  The transform
+// is the K = 8 x M Cooley-Tukey split: a radix-8 DFT over n1 in registers (three radix-2
+// decimation-in-frequency stages), the twiddles W_K^(b k1), then the M-point DFT over the lanes
+// by log2 M radix-2 decimation-in-frequency stages between partner lanes (shuffles, no LDS).
+// Afterwards register r of lane b holds X[br3(r) + 8 brM(b)] (br: bit reversal).
+// W = e^(-2 pi i / K) forward, its conjugate for the inverse; unnormalised, as FFTW's.
+constexpr int kLineElems = 2048;  // points per 256-thread workgroup of a separate pass
+template <int K>
+struct LineCfg {
+  static constexpr int M = K / 8;  // lanes per line
+  static constexpr int LB = M == 2 ? 1 : M == 4 ? 2 : M == 8 ? 3 : M == 16 ? 4 : 5;
+  static constexpr int T = kLineElems / K;  // lines per 256-thread workgroup
+  // LDS line stride: a point k of a line sits at lpos(k) = k + k / 8, so the eight points a lane
+  // holds after the transform (stride 8) and the lanes' points before it fall into distinct
+  // banks; the odd extra point staggers the lines
+  static constexpr int LSX = K + K / 8 + 1;
+  static_assert(K >= 16 && K <= 256 && (K & (K - 1)) == 0, "power-of-two mesh 16..256");
+};
+
+__device__ __forceinline__ int lpos(int k) { return k + (k >> 3); }
+
+__device__ __forceinline__ int br3(int r) { return ((r & 1) << 2) | (r & 2) | ((r >> 2) & 1); }
+
+// The point of the line held by register r of lane b after line_fft.
+template <int K>
+__device__ __forceinline__ int line_pos(int r, int b) {
+  return br3(r) + 8 * (int)(__brev((unsigned)b) >> (32 - LineCfg<K>::LB));
 }
 
-// The closing radix-2 stage when log2 K is odd.
-template <typename C2, int K>
-__device__ __forceinline__ void r2_butterfly(const C2* s, C2* dd, int j, int Ns, const C2* __restrict__ tw,
-                                             bool inverse) {
-  const int step = K / (Ns * 2);
-  const int jr = j % Ns;
-  C2 w = tw[jr * step];
-  if (inverse) w = cconj(w);
-  const C2 v0 = s[j], v1 = cmul(s[j + K / 2], w);
-  C2* d = dd + (j / Ns) * Ns * 2 + jr;
-  d[0] = cadd(v0, v1);
-  d[Ns] = csub(v0, v1);
+// a * W8^j (j = 1, 2, 3), W8 = e^(-2 pi i / 8); conjugated for the inverse.
+template <bool INV, typename C2>
+__device__ __forceinline__ C2 mul_w8(C2 a, int j) {
+  typedef decltype(a.x) R;
+  const R s = R(0.70710678118654752440);
+  if (j == 1) return INV ? cmk<C2>(s * (a.x - a.y), s * (a.x + a.y)) : cmk<C2>(s * (a.x + a.y), s * (a.y - a.x));
+  if (j == 2) return INV ? cmk<C2>(-a.y, a.x) : cmk<C2>(a.y, -a.x);
+  return INV ? cmk<C2>(-(s * (a.x + a.y)), s * (a.x - a.y)) : cmk<C2>(s * (a.y - a.x), -(s * (a.x + a.y)));
 }
 
-// Stockham autosort FFT of T lines of length K held line-major in LDS buffer A (B is
-// scratch), line i starting at element i * LS.  Radix-4 stages, then one radix-2 stage when
-// log2 K is odd.  Returns the buffer holding the result.  All NT threads of the workgroup
-// must call it.
-template <typename C2, int K, int T, int LS = K>
-__device__ C2* lds_fft(C2* A, C2* B, const C2* __restrict__ tw, bool inverse) {
-  constexpr int NT = 256;
-  constexpr int TPL = NT / T;  // threads per line
-  const int line = threadIdx.x / TPL, tl = threadIdx.x % TPL;
-  C2* src = A;
-  C2* dst = B;
-  int Ns = 1;
-  for (; Ns * 4 <= K; Ns *= 4) {
-    for (int j = tl; j < K / 4; j += TPL) r4_butterfly<C2, K>(src + line * LS, dst + line * LS, j, Ns, tw, inverse);
-    __syncthreads();
-    C2* t = src; src = dst; dst = t;
+template <bool INV, typename C2>
+__device__ __forceinline__ C2 twiddle(const C2* s_tw, int j) {
+  const C2 w = s_tw[j];
+  return INV ? cconj(w) : w;
+}
+
+// The value of lane b ^ D (D < 32), b the lane's index within its line of M >= 2D lanes: DPP
+// within a row of 16 lanes (quad permutes for 1 and 2, row shifts for 4, a row rotation for 8),
+// a swizzle for 16.  row_shl:n reads lane + n, row_shr:n lane - n.
+template <int D>
+__device__ __forceinline__ int xor_lanes(int v, int b) {
+  if constexpr (D == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  if constexpr (D == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  if constexpr (D == 4) {
+    const int up = __builtin_amdgcn_mov_dpp(v, 0x104, 0xF, 0xF, false);    // row_shl:4
+    const int down = __builtin_amdgcn_mov_dpp(v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    return (b & 4) ? down : up;
   }
-  if (Ns < K) {
-    for (int j = tl; j < K / 2; j += TPL) r2_butterfly<C2, K>(src + line * LS, dst + line * LS, j, Ns, tw, inverse);
-    __syncthreads();
-    C2* t = src; src = dst; dst = t;
-  }
-  return src;
+  if constexpr (D == 8) return __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, false);  // row_ror:8
+  return __builtin_amdgcn_ds_swizzle(v, 0x1F | (D << 10));  // bit mask mode: lane ^ D
+}
+template <int D>
+__device__ __forceinline__ float xor_lanes(float v, int b) { return __int_as_float(xor_lanes<D>(__float_as_int(v), b)); }
+template <int D>
+__device__ __forceinline__ double xor_lanes(double v, int b) {
+  const long long u = __double_as_longlong(v);
+  const int lo = xor_lanes<D>((int)(unsigned)u, b), hi = xor_lanes<D>((int)(u >> 32), b);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// The same transform for nl lines (any count) and NT threads: the butterflies of a stage are
-// dealt to the threads as (line, butterfly) tasks.  Per line the arithmetic is lds_fft's.
-template <typename C2, int K, int NT>
-__device__ C2* lds_fft_lines(C2* A, C2* B, const C2* __restrict__ tw, bool inverse, int nl, int LS) {
-  C2* src = A;
-  C2* dst = B;
-  int Ns = 1;
-  for (; Ns * 4 <= K; Ns *= 4) {
-    for (int t = threadIdx.x; t < nl * (K / 4); t += NT) {
-      const int line = t / (K / 4), j = t % (K / 4);
-      r4_butterfly<C2, K>(src + line * LS, dst + line * LS, j, Ns, tw, inverse);
+// One radix-2 stage between partner lanes at distance D (see line_fft).
+template <int K, int D, bool INV, typename C2>
+__device__ __forceinline__ void lane_stage(C2 (&v)[8], int b, const C2* s_tw) {
+  const bool up = (b & D) != 0;
+  const C2 w = twiddle<INV>(s_tw, (b & (D - 1)) * (K / (2 * D)));
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    C2 p;
+    p.x = xor_lanes<D>(v[r].x, b);
+    p.y = xor_lanes<D>(v[r].y, b);
+    v[r] = up ? cmul(csub(p, v[r]), w) : cadd(v[r], p);
+  }
+}
+
+// The transform of the line whose points lane b of the line's M lanes holds (see above); s_tw
+// holds W_K^j, j < K.  All M lanes of the line take part.
+template <int K, bool INV, typename C2>
+__device__ __forceinline__ void line_fft(C2 (&v)[8], int b, const C2* s_tw) {
+  constexpr int M = LineCfg<K>::M;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {  // span 4
+    const C2 x = v[j], y = v[j + 4];
+    v[j] = cadd(x, y);
+    v[j + 4] = j == 0 ? csub(x, y) : mul_w8<INV>(csub(x, y), j);
+  }
+#pragma unroll
+  for (int g = 0; g < 8; g += 4) {  // span 2
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const C2 x = v[g + j], y = v[g + j + 2];
+      v[g + j] = cadd(x, y);
+      v[g + j + 2] = j == 0 ? csub(x, y) : mul_w8<INV>(csub(x, y), 2);
     }
-    __syncthreads();
-    C2* t = src; src = dst; dst = t;
   }
-  if (Ns < K) {
-    for (int t = threadIdx.x; t < nl * (K / 2); t += NT) {
-      const int line = t / (K / 2), j = t % (K / 2);
-      r2_butterfly<C2, K>(src + line * LS, dst + line * LS, j, Ns, tw, inverse);
-    }
-    __syncthreads();
-    C2* t = src; src = dst; dst = t;
+#pragma unroll
+  for (int g = 0; g < 8; g += 2) {  // span 1
+    const C2 x = v[g], y = v[g + 1];
+    v[g] = cadd(x, y);
+    v[g + 1] = csub(x, y);
   }
-  return src;
+  // register r holds k1 = br3(r): twiddle W_K^(b k1)
+#pragma unroll
+  for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], twiddle<INV>(s_tw, b * br3(r)));
+  // the M-point DFT over the lanes: partners at distance d, twiddle W_2d^(b mod d) on the
+  // difference
+  if constexpr (M >= 32) lane_stage<K, 16, INV>(v, b, s_tw);
+  if constexpr (M >= 16) lane_stage<K, 8, INV>(v, b, s_tw);
+  if constexpr (M >= 8) lane_stage<K, 4, INV>(v, b, s_tw);
+  if constexpr (M >= 4) lane_stage<K, 2, INV>(v, b, s_tw);
+  lane_stage<K, 1, INV>(v, b, s_tw);
 }
 
 struct ConvArgs {
@@ -163,28 +197,71 @@ struct FftArgs {
   int rad[kMaxRadices];
 };
 
-// r2c along x.  Workgroup: T line pairs (2T real rows); row = y + K*z.
+// Twiddles W_K^j into LDS (the caller's barrier publishes them).
+template <typename C2, int K, int NT>
+__device__ __forceinline__ void stage_twiddles(C2* s_tw, const void* tw) {
+  const C2* __restrict__ t = (const C2*)tw;
+  for (int k = threadIdx.x; k < K; k += NT) s_tw[k] = t[k];
+}
+
+// The r2c split of a packed line Z (rows y0 = 2p real part, y0 + 1 imaginary part; points at
+// lpos) at kx:
+// the spectrum of row y0 (odd = false) or y0 + 1 (odd = true).
+template <typename C2, int K>
+__device__ __forceinline__ C2 r2c_split(const C2* Z, int kx, bool odd) {
+  typedef decltype(C2().x) R;
+  const R h = R(0.5);
+  const C2 zz = Z[lpos(kx)], zc = cconj(Z[lpos((K - kx) & (K - 1))]);
+  if (!odd) {
+    const C2 sm = cadd(zz, zc);
+    return cmk<C2>(h * sm.x, h * sm.y);
+  }
+  const C2 d = csub(zz, zc);
+  return cmk<C2>(h * d.y, -h * d.x);  // (z - zc) / (2i)
+}
+
+// The c2r merge of the half spectra a (row 2p) and b (row 2p + 1) into point k of the packed
+// line: a + i b for k <= K/2 (imaginary parts of the kx = 0 and K/2 terms ignored), conj(a) +
+// i conj(b) taken from kx = K - k above.  spec(kx, odd) returns the half spectra.
+template <typename C2, int K, typename F>
+__device__ __forceinline__ C2 c2r_merge(int k, F spec) {
+  if (k <= K / 2) {
+    C2 a = spec(k, false), b = spec(k, true);
+    if (k == 0 || k == K / 2) {
+      a.y = 0;
+      b.y = 0;
+    }
+    return cmk<C2>(a.x - b.y, a.y + b.x);
+  }
+  const C2 a = spec(K - k, false), b = spec(K - k, true);
+  return cmk<C2>(a.x + b.y, b.x - a.y);
+}
+
+// r2c along x.  Workgroup: T packed lines (rows 2p, 2p + 1 as one complex line; row = y + K z),
+// M lanes each.
 template <typename real, int K>
 __device__ __forceinline__ void fft_x_body(const FftArgs& fa, int bid) {
   typedef typename C2T<real>::t C2;
+  constexpr int M = LineCfg<K>::M, T = LineCfg<K>::T, H = K / 2 + 1, LSX = LineCfg<K>::LSX;
   const real* __restrict__ Q = (const real*)fa.Q;
   C2* __restrict__ Cm = (C2*)fa.Cm;
-  const C2* __restrict__ tw = (const C2*)fa.tw;
-  constexpr int E = FftCfg<real>::E, T = E / K, TPL = 256 / T, H = K / 2 + 1;
-  __shared__ C2 A[E], B[E];
-  __shared__ C2 s_tw[K];  // twiddles staged in LDS
-  for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
-  const int line = threadIdx.x / TPL, tl = threadIdx.x % TPL;
+  __shared__ C2 A[T * LSX];
+  __shared__ C2 s_tw[K];
+  stage_twiddles<C2, K, 256>(s_tw, fa.tw);
+  const int line = threadIdx.x / M, b = threadIdx.x % M;
   const size_t r0 = 2 * ((size_t)bid * T + line), r1 = r0 + 1;
-  for (int k = tl; k < K; k += TPL) A[line * K + k] = cmk<C2>(Q[r0 * K + k], Q[r1 * K + k]);
+  C2 v[8];
+#pragma unroll
+  for (int n = 0; n < 8; ++n) v[n] = cmk<C2>(Q[r0 * K + M * n + b], Q[r1 * K + M * n + b]);
   __syncthreads();
-  const C2* Z = lds_fft<C2, K, T>(A, B, s_tw, false) + line * K;
-  const real h = real(0.5);
+  line_fft<K, false>(v, b, s_tw);
+  C2* Z = A + line * LSX;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) Z[lpos(line_pos<K>(r, b))] = v[r];
+  __syncthreads();
   C2* __restrict__ Nq = Cm + (size_t)(K / 2) * K * K;
-  for (int k = tl; k < H; k += TPL) {
-    const C2 z = Z[k], zc = cconj(Z[(K - k) & (K - 1)]);
-    const C2 s = cadd(z, zc), d = csub(z, zc);
-    const C2 c0 = cmk<C2>(h * s.x, h * s.y), c1 = cmk<C2>(h * d.y, -h * d.x);  // (z - zc) / (2i)
+  for (int k = b; k < H; k += M) {
+    const C2 c0 = r2c_split<C2, K>(Z, k, false), c1 = r2c_split<C2, K>(Z, k, true);
     if (k < K / 2) {
       Cm[r0 * (K / 2) + k] = c0;
       Cm[r1 * (K / 2) + k] = c1;
@@ -195,32 +272,43 @@ __device__ __forceinline__ void fft_x_body(const FftArgs& fa, int bid) {
   }
 }
 
-// c2r along x (the imaginary parts of the kx = 0 and K/2 terms are ignored, as in c2r).
+// c2r along x.
 template <typename real, int K>
 __device__ __forceinline__ void fft_xi_body(const FftArgs& fa, int bid) {
   typedef typename C2T<real>::t C2;
+  constexpr int M = LineCfg<K>::M, T = LineCfg<K>::T, H = K / 2 + 1, LSX = LineCfg<K>::LSX;
+  static_assert(LSX >= 2 * H, "the half spectra fit a line");
   const C2* __restrict__ Cm = (const C2*)fa.Cm;
   real* __restrict__ Q = (real*)fa.Q;
-  const C2* __restrict__ tw = (const C2*)fa.tw;
-  constexpr int E = FftCfg<real>::E, T = E / K, TPL = 256 / T, H = K / 2 + 1;
-  __shared__ C2 A[E], B[E];
-  __shared__ C2 s_tw[K];  // twiddles staged in LDS
-  for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
-  const int line = threadIdx.x / TPL, tl = threadIdx.x % TPL;
-  const size_t r0 = 2 * ((size_t)bid * T + line), r1 = r0 + 1;
+  // first the two rows' half spectra of each line (2H points), then the transformed lines
+  __shared__ C2 A[T * LSX];
+  C2* S = A;
+  __shared__ C2 s_tw[K];
+  stage_twiddles<C2, K, 256>(s_tw, fa.tw);
+  const int line = threadIdx.x / M, b = threadIdx.x % M;
+  const size_t row0 = 2 * (size_t)bid * T;  // the workgroup's 2T rows are consecutive
   const C2* __restrict__ Nq = Cm + (size_t)(K / 2) * K * K;
-  for (int k = tl; k < H; k += TPL) {
-    C2 a = k < K / 2 ? Cm[r0 * (K / 2) + k] : Nq[r0], b = k < K / 2 ? Cm[r1 * (K / 2) + k] : Nq[r1];
-    if (k == 0 || k == K / 2) { a.y = 0; b.y = 0; }
-    A[line * K + k] = cmk<C2>(a.x - b.y, a.y + b.x);                 // a + i b
-    if (k > 0 && k < K / 2) A[line * K + K - k] = cmk<C2>(a.x + b.y, b.x - a.y);  // conj(a) + i conj(b)
+  for (int e = threadIdx.x; e < 2 * T * (K / 2); e += 256) {
+    const int rr = e / (K / 2), k = e % (K / 2);
+    S[(rr >> 1) * LSX + (rr & 1) * H + k] = Cm[(row0 + rr) * (K / 2) + k];
   }
+  for (int rr = threadIdx.x; rr < 2 * T; rr += 256) S[(rr >> 1) * LSX + (rr & 1) * H + K / 2] = Nq[row0 + rr];
   __syncthreads();
-  const C2* Z = lds_fft<C2, K, T>(A, B, s_tw, true) + line * K;
-  for (int k = tl; k < K; k += TPL) {
-    const C2 z = Z[k];
-    Q[r0 * K + k] = z.x;
-    Q[r1 * K + k] = z.y;
+  const C2* Sl = S + line * LSX;
+  C2 v[8];
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+    v[n] = c2r_merge<C2, K>(M * n + b, [&](int kx, bool odd) { return Sl[(odd ? H : 0) + kx]; });
+  line_fft<K, true>(v, b, s_tw);
+  __syncthreads();  // every half spectrum read before the lines overwrite them
+  C2* Z = A + line * LSX;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) Z[lpos(line_pos<K>(r, b))] = v[r];
+  __syncthreads();
+  for (int e = threadIdx.x; e < 2 * T * K; e += 256) {  // the 2T rows, coalesced
+    const int rr = e / K, k = e % K;
+    const C2 z = A[(rr >> 1) * LSX + lpos(k)];
+    Q[(row0 + rr) * K + k] = (rr & 1) ? z.y : z.x;
   }
 }
 
@@ -245,82 +333,115 @@ struct HcLines {
   static constexpr int NB = (N + T - 1) / T;
 };
 
+// T lines from the half-complex mesh into LDS (line-major, padded rows of LS), and back.
+template <typename C2, int K, bool YPASS>
+__device__ __forceinline__ void hc_load(C2* A, const C2* __restrict__ Cm, int L0) {
+  constexpr int T = LineCfg<K>::T, LSX = LineCfg<K>::LSX, NL = HcLines<K, T>::N;
+  for (int e = threadIdx.x; e < T * K; e += 256) {
+    const int kl = e % T, w = e / T, L = L0 + kl;
+    A[kl * LSX + lpos(w)] = L < NL ? Cm[hc_index<K, YPASS>(L, w)] : cmk<C2>(0, 0);
+  }
+}
+template <typename C2, int K, bool YPASS>
+__device__ __forceinline__ void hc_store(C2* __restrict__ Cm, const C2* A, int L0) {
+  constexpr int T = LineCfg<K>::T, LSX = LineCfg<K>::LSX, NL = HcLines<K, T>::N;
+  for (int e = threadIdx.x; e < T * K; e += 256) {
+    const int kl = e % T, w = e / T, L = L0 + kl;
+    if (L < NL) Cm[hc_index<K, YPASS>(L, w)] = A[kl * LSX + lpos(w)];
+  }
+}
+
 // Complex FFT along y for T lines (consecutive kx at one z, or Nyquist lines).
 template <typename real, int K>
 __device__ __forceinline__ void fft_y_body(const FftArgs& fa, int bid, int inverse) {
   typedef typename C2T<real>::t C2;
+  constexpr int M = LineCfg<K>::M, T = LineCfg<K>::T, LSX = LineCfg<K>::LSX;
   C2* __restrict__ Cm = (C2*)fa.Cm;
-  const C2* __restrict__ tw = (const C2*)fa.tw;
-  constexpr int E = FftCfg<real>::E, T = E / K, NL = HcLines<K, T>::N;
-  constexpr int LS = K + 1;  // padded lines: the transposing loads and stores avoid LDS bank conflicts
-  __shared__ C2 A[T * LS], B[T * LS];
-  __shared__ C2 s_tw[K];  // twiddles staged in LDS
-  for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
+  __shared__ C2 A[T * LSX];
+  __shared__ C2 s_tw[K];
+  stage_twiddles<C2, K, 256>(s_tw, fa.tw);
   const int L0 = bid * T;
-  for (int e = threadIdx.x; e < E; e += 256) {
-    const int kl = e % T, y = e / T, L = L0 + kl;
-    A[kl * LS + y] = L < NL ? Cm[hc_index<K, true>(L, y)] : cmk<C2>(0, 0);
-  }
+  hc_load<C2, K, true>(A, Cm, L0);
   __syncthreads();
-  const C2* R = lds_fft<C2, K, T, LS>(A, B, s_tw, inverse != 0);
-  for (int e = threadIdx.x; e < E; e += 256) {
-    const int kl = e % T, y = e / T, L = L0 + kl;
-    if (L < NL) Cm[hc_index<K, true>(L, y)] = R[kl * LS + y];
-  }
+  const int line = threadIdx.x / M, b = threadIdx.x % M;
+  C2* Al = A + line * LSX;
+  C2 v[8];
+#pragma unroll
+  for (int n = 0; n < 8; ++n) v[n] = Al[lpos(M * n + b)];
+  if (inverse) line_fft<K, true>(v, b, s_tw);
+  else line_fft<K, false>(v, b, s_tw);
+  __syncthreads();  // every line read before any is overwritten
+#pragma unroll
+  for (int r = 0; r < 8; ++r) Al[lpos(line_pos<K>(r, b))] = v[r];
+  __syncthreads();
+  hc_store<C2, K, true>(Cm, A, L0);
 }
 
 // Forward FFT along z, influence-function multiply + FP64 energy/virial, inverse FFT along z.
 template <typename real, int K>
 __device__ __forceinline__ void fft_zconv_body(const FftArgs& fa, int bid) {
   typedef typename C2T<real>::t C2;
+  constexpr int M = LineCfg<K>::M, T = LineCfg<K>::T, H = K / 2 + 1, NL = HcLines<K, T>::N;
+  constexpr int NH = K / 2, NREG = NH * K, LSX = LineCfg<K>::LSX;
   C2* __restrict__ Cm = (C2*)fa.Cm;
-  const C2* __restrict__ tw = (const C2*)fa.tw;
   const real* __restrict__ G = (const real*)fa.G;
   const ConvArgs& ca = fa.ca;
   double* __restrict__ part = fa.part;
-  constexpr int E = FftCfg<real>::E, T = E / K, H = K / 2 + 1, NL = HcLines<K, T>::N;
-  constexpr int NH = K / 2, NREG = NH * K;
-  constexpr int LS = K + 1;  // padded lines (see fft_y_body)
-  __shared__ C2 A[T * LS], B[T * LS];
-  __shared__ C2 s_tw[K];  // twiddles staged in LDS
-  for (int k = threadIdx.x; k < K; k += 256) s_tw[k] = tw[k];
+  __shared__ C2 A[T * LSX];
+  __shared__ C2 s_tw[K];
   __shared__ double red[8];
+  stage_twiddles<C2, K, 256>(s_tw, fa.tw);
   const int L0 = bid * T;
-  for (int e = threadIdx.x; e < E; e += 256) {
-    const int kl = e % T, z = e / T, L = L0 + kl;
-    A[kl * LS + z] = L < NL ? Cm[hc_index<K, false>(L, z)] : cmk<C2>(0, 0);
+  hc_load<C2, K, false>(A, Cm, L0);
+  const int line = threadIdx.x / M, b = threadIdx.x % M, L = L0 + line;
+  const bool valid = L < NL;
+  const int kx = L < NREG ? L % NH : NH, y = L < NREG ? L / NH : L - NREG;
+  const int ky = y <= K / 2 ? y : K - y;
+  // the influence function at the points this lane will hold, loaded beside the lines
+  real g[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int kz = line_pos<K>(r, b), kzz = kz <= K / 2 ? kz : K - kz;
+    g[r] = valid ? G[kx + H * (ky + H * kzz)] : real(0);
   }
   __syncthreads();
-  C2* R = lds_fft<C2, K, T, LS>(A, B, s_tw, false);
+  C2* Al = A + line * LSX;
+  C2 v[8];
+#pragma unroll
+  for (int n = 0; n < 8; ++n) v[n] = Al[lpos(M * n + b)];
+  line_fft<K, false>(v, b, s_tw);
   double esum = 0.0, vsum = 0.0;
-  for (int e = threadIdx.x; e < E; e += 256) {
+  if (valid) {
     // no contraction of the FP64 energy/virial terms: the value must not depend on how
     // the compiler fuses them in the single-system and the batched instantiation
 #pragma clang fp contract(off)
-    const int kl = e / K, kz = e % K, L = L0 + kl;
-    if (L >= NL) continue;
-    const int kx = L < NREG ? L % NH : NH, y = L < NREG ? L / NH : L - NREG;
-    const int ky = y <= K / 2 ? y : K - y;
-    const int kzz = kz <= K / 2 ? kz : K - kz;
-    const real g = G[kx + H * (ky + H * kzz)];
-    C2 f = R[kl * LS + kz];
-    const double f2 = (double)f.x * (double)f.x + (double)f.y * (double)f.y;
     const double w = (kx == 0 || kx == K / 2) ? 0.5 : 1.0;
-    const double m2 = ca.tpl2[0] * kx * kx + ca.tpl2[1] * ky * ky + ca.tpl2[2] * kzz * kzz;
-    const double eg = w * (double)g * f2;
-    esum += eg;
-    vsum += eg * (1.0 + 2.0 * m2 * ca.expf);
-    f.x *= g;
-    f.y *= g;
-    R[kl * LS + kz] = f;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int kz = line_pos<K>(r, b), kzz = kz <= K / 2 ? kz : K - kz;
+      C2 f = v[r];
+      const double f2 = (double)f.x * (double)f.x + (double)f.y * (double)f.y;
+      const double m2 = ca.tpl2[0] * kx * kx + ca.tpl2[1] * ky * ky + ca.tpl2[2] * kzz * kzz;
+      const double eg = w * (double)g[r] * f2;
+      esum += eg;
+      vsum += eg * (1.0 + 2.0 * m2 * ca.expf);
+      f.x *= g[r];
+      f.y *= g[r];
+      v[r] = f;
+    }
   }
+  __syncthreads();  // every line read before any is overwritten
+#pragma unroll
+  for (int r = 0; r < 8; ++r) Al[lpos(line_pos<K>(r, b))] = v[r];
   __syncthreads();
-  C2* other = (R == A) ? B : A;
-  const C2* S = lds_fft<C2, K, T, LS>(R, other, s_tw, true);
-  for (int e = threadIdx.x; e < E; e += 256) {
-    const int kl = e % T, z = e / T, L = L0 + kl;
-    if (L < NL) Cm[hc_index<K, false>(L, z)] = S[kl * LS + z];
-  }
+#pragma unroll
+  for (int n = 0; n < 8; ++n) v[n] = Al[lpos(M * n + b)];
+  line_fft<K, true>(v, b, s_tw);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 8; ++r) Al[lpos(line_pos<K>(r, b))] = v[r];
+  __syncthreads();
+  hc_store<C2, K, false>(Cm, A, L0);
   esum = msgd::block_sum(esum, red);
   vsum = msgd::block_sum(vsum, red);
   if (threadIdx.x == 0) {
@@ -351,13 +472,13 @@ MSG_FFT_ENTRY(k_fft_zconv, fft_zconv_body)
 // holds the plane in LDS and does the x r2c and then the y FFT (k_fft_xy), or the inverse y
 // FFT and then the x c2r (k_fft_yxi), so the mesh crosses memory once per pair of passes and
 // two launches of the chain are saved.  Per line the arithmetic is that of k_fft_x / k_fft_y
-// / k_fft_yi / k_fft_xi (the same butterflies, twiddles, split and merge), so the results are
-// the same bits as the separate passes (which the batched passes keep).
+// / k_fft_yi / k_fft_xi (the same line transform, split and merge), so the results are the
+// same bits as the separate passes (which the batched passes keep).
 constexpr int kPlaneNT = 1024;
 template <int K>
 struct PlaneCfg {
-  static constexpr int H = K / 2 + 1, LS = K + 1, NH = K / 2;
-  static constexpr int NB = NH * K > H * LS ? NH * K : H * LS;  // complex elements per buffer
+  static constexpr int H = K / 2 + 1, LS = LineCfg<K>::LSX, NH = K / 2;
+  static constexpr int NB = H * LS;  // complex elements per buffer (NH x lines or H y lines)
   static constexpr size_t lds = (2 * (size_t)NB + K) * sizeof(float2);
 };
 
@@ -365,81 +486,104 @@ template <int K>
 __global__ __launch_bounds__(kPlaneNT) void k_fft_xy(FftArgs fa) {
   typedef float2 C2;
   constexpr int H = PlaneCfg<K>::H, LS = PlaneCfg<K>::LS, NB = PlaneCfg<K>::NB, NH = PlaneCfg<K>::NH;
+  constexpr int M = LineCfg<K>::M, LPR = kPlaneNT / M;  // lines per round of the workgroup
   extern __shared__ __align__(16) double s_dyn[];
   C2* A = (C2*)s_dyn;
   C2* B = A + NB;
   C2* s_tw = B + NB;
   const float* __restrict__ Q = (const float*)fa.Q;
   C2* __restrict__ Cm = (C2*)fa.Cm;
-  const C2* __restrict__ tw = (const C2*)fa.tw;
-  for (int k = threadIdx.x; k < K; k += kPlaneNT) s_tw[k] = tw[k];
+  stage_twiddles<C2, K, kPlaneNT>(s_tw, fa.tw);
   const size_t z = blockIdx.x;
   const float* __restrict__ Qz = Q + z * K * K;
-  // rows y = 2p and 2p + 1 packed as one complex line p
-  for (int e = threadIdx.x; e < NH * K; e += kPlaneNT) {
-    const int p = e / K, x = e % K;
-    A[p * K + x] = cmk<C2>(Qz[(2 * p) * K + x], Qz[(2 * p + 1) * K + x]);
+  const int b = threadIdx.x % M;
+  // x: packed line p = rows y = 2p and 2p + 1
+  {
+    const int p = threadIdx.x / M;
+    C2 v[8];
+    if (p < NH) {
+#pragma unroll
+      for (int n = 0; n < 8; ++n) v[n] = cmk<C2>(Qz[(2 * p) * K + M * n + b], Qz[(2 * p + 1) * K + M * n + b]);
+    }
+    __syncthreads();  // twiddles
+    if (p < NH) {
+      line_fft<K, false>(v, b, s_tw);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) A[p * LS + lpos(line_pos<K>(r, b))] = v[r];
+    }
   }
   __syncthreads();
-  C2* Z = lds_fft_lines<C2, K, kPlaneNT>(A, B, s_tw, false, NH, K);
-  C2* W = Z == A ? B : A;
-  const float h = 0.5f;
-  // split into the half spectra of the two rows, stored as y lines of column kx
-  for (int e = threadIdx.x; e < NH * H; e += kPlaneNT) {
-    const int p = e / H, k = e % H;
-    const C2 zz = Z[p * K + k], zc = cconj(Z[p * K + ((K - k) & (K - 1))]);
-    const C2 sm = cadd(zz, zc), d = csub(zz, zc);
-    W[k * LS + 2 * p] = cmk<C2>(h * sm.x, h * sm.y);
-    W[k * LS + 2 * p + 1] = cmk<C2>(h * d.y, -h * d.x);  // (z - zc) / (2i)
+  // y: line kx = 0 .. K/2 over y, its points split from the packed x lines
+  for (int kx = threadIdx.x / M; kx < H; kx += LPR) {
+    C2 v[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+      const int y = M * n + b;
+      v[n] = r2c_split<C2, K>(A + (y >> 1) * LS, kx, (y & 1) != 0);
+    }
+    line_fft<K, false>(v, b, s_tw);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) B[kx * LS + lpos(line_pos<K>(r, b))] = v[r];
   }
   __syncthreads();
-  const C2* R = lds_fft_lines<C2, K, kPlaneNT>(W, Z, s_tw, false, H, LS);
   C2* __restrict__ Nq = Cm + (size_t)NH * K * K;
   for (int e = threadIdx.x; e < NH * K; e += kPlaneNT) {
     const int y = e / NH, kx = e % NH;
-    Cm[kx + (size_t)NH * (y + K * z)] = R[kx * LS + y];
+    Cm[kx + (size_t)NH * (y + K * z)] = B[kx * LS + lpos(y)];
   }
-  for (int y = threadIdx.x; y < K; y += kPlaneNT) Nq[y + K * z] = R[NH * LS + y];
+  for (int y = threadIdx.x; y < K; y += kPlaneNT) Nq[y + K * z] = B[NH * LS + lpos(y)];
 }
 
 template <int K>
 __global__ __launch_bounds__(kPlaneNT) void k_fft_yxi(FftArgs fa) {
   typedef float2 C2;
   constexpr int H = PlaneCfg<K>::H, LS = PlaneCfg<K>::LS, NB = PlaneCfg<K>::NB, NH = PlaneCfg<K>::NH;
+  constexpr int M = LineCfg<K>::M, LPR = kPlaneNT / M;
   extern __shared__ __align__(16) double s_dyn[];
   C2* A = (C2*)s_dyn;
   C2* B = A + NB;
   C2* s_tw = B + NB;
   const C2* __restrict__ Cm = (const C2*)fa.Cm;
   float* __restrict__ Q = (float*)fa.Q;
-  const C2* __restrict__ tw = (const C2*)fa.tw;
-  for (int k = threadIdx.x; k < K; k += kPlaneNT) s_tw[k] = tw[k];
+  stage_twiddles<C2, K, kPlaneNT>(s_tw, fa.tw);
   const size_t z = blockIdx.x;
   const C2* __restrict__ Nq = Cm + (size_t)NH * K * K;
   for (int e = threadIdx.x; e < NH * K; e += kPlaneNT) {
     const int y = e / NH, kx = e % NH;
-    A[kx * LS + y] = Cm[kx + (size_t)NH * (y + K * z)];
+    A[kx * LS + lpos(y)] = Cm[kx + (size_t)NH * (y + K * z)];
   }
-  for (int y = threadIdx.x; y < K; y += kPlaneNT) A[NH * LS + y] = Nq[y + K * z];
+  for (int y = threadIdx.x; y < K; y += kPlaneNT) A[NH * LS + lpos(y)] = Nq[y + K * z];
   __syncthreads();
-  C2* R = lds_fft_lines<C2, K, kPlaneNT>(A, B, s_tw, true, H, LS);
-  C2* X = R == A ? B : A;
-  // rows 2p and 2p + 1 merged into one complex line (imaginary parts of kx = 0, K/2 ignored)
-  for (int e = threadIdx.x; e < NH * H; e += kPlaneNT) {
-    const int p = e / H, k = e % H;
-    C2 a = R[k * LS + 2 * p], b = R[k * LS + 2 * p + 1];
-    if (k == 0 || k == NH) { a.y = 0; b.y = 0; }
-    X[p * K + k] = cmk<C2>(a.x - b.y, a.y + b.x);                         // a + i b
-    if (k > 0 && k < NH) X[p * K + K - k] = cmk<C2>(a.x + b.y, b.x - a.y);  // conj(a) + i conj(b)
+  const int b = threadIdx.x % M;
+  // inverse y: lines kx = 0 .. K/2
+  for (int kx = threadIdx.x / M; kx < H; kx += LPR) {
+    C2 v[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) v[n] = A[kx * LS + lpos(M * n + b)];
+    line_fft<K, true>(v, b, s_tw);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) B[kx * LS + lpos(line_pos<K>(r, b))] = v[r];
   }
   __syncthreads();
-  const C2* Zl = lds_fft_lines<C2, K, kPlaneNT>(X, R, s_tw, true, NH, K);
+  // inverse x: packed line p merged from the half spectra of rows 2p and 2p + 1
+  {
+    const int p = threadIdx.x / M;
+    if (p < NH) {
+      C2 v[8];
+#pragma unroll
+      for (int n = 0; n < 8; ++n)
+        v[n] = c2r_merge<C2, K>(M * n + b, [&](int kx, bool odd) { return B[kx * LS + lpos(2 * p + (odd ? 1 : 0))]; });
+      line_fft<K, true>(v, b, s_tw);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) A[p * LS + lpos(line_pos<K>(r, b))] = v[r];
+    }
+  }
+  __syncthreads();
   float* __restrict__ Qz = Q + z * K * K;
-  for (int e = threadIdx.x; e < NH * K; e += kPlaneNT) {
-    const int p = e / K, x = e % K;
-    const C2 v = Zl[p * K + x];
-    Qz[(2 * p) * K + x] = v.x;
-    Qz[(2 * p + 1) * K + x] = v.y;
+  for (int e = threadIdx.x; e < K * K; e += kPlaneNT) {  // rows y, coalesced
+    const int y = e / K, x = e % K;
+    const C2 v = A[(y >> 1) * LS + lpos(x)];
+    Qz[y * K + x] = (y & 1) ? v.y : v.x;
   }
 }
 
@@ -1330,7 +1474,7 @@ void launch_gfft_all(Ctx* c) {
 // Workgroups of the five FFT stages for mesh K: x, y, zconv, yi, xi.
 template <typename real, int K>
 void fft_blocks(int* nb) {
-  constexpr int E = FftCfg<real>::E, T = E / K;
+  constexpr int T = LineCfg<K>::T;
   nb[0] = nb[4] = K * K / 2 / T;
   nb[1] = nb[2] = nb[3] = HcLines<K, T>::NB;
 }
@@ -1613,7 +1757,7 @@ int spme_setup(Ctx* c) {
     const int T = c->fp64 ? gen_lines_yz<double>(K) : gen_lines_yz<float>(K);
     c->nzblocks = (H * K + T - 1) / T;  // workgroups of the general z pass
   } else {
-    const int T = (c->fp64 ? FftCfg<double>::E : FftCfg<float>::E) / K;
+    const int T = kLineElems / K;
     c->nzblocks = (H * K + T - 1) / T;  // workgroups of the z pass (HcLines<K, T>::NB)
   }
   return MSG_OK;
