@@ -444,7 +444,7 @@ void msg_destroy(msg_ctx* h) {
                   c->d_frac, c->d_Q, c->d_C, c->d_G, c->d_tw, c->d_kfac, c->d_kvec, c->d_sk,
                   c->d_force, c->d_part, c->d_out, c->d_flags, c->d_de, c->d_de_ip, c->d_de_r,
                   c->d_tcoef64, c->d_gk, c->d_item_cell, c->d_bq, c->d_bmm, c->d_M,
-                  c->d_vl_home, c->d_vl_n, c->d_vl_nbr, c->d_vl_pos, c->d_vl_slot, c->d_ro_ref, c->d_vel,
+                  c->d_vl_home, c->d_vl_n, c->d_vl_nbr, c->d_vl_pos, c->d_vl_rel, c->d_vl_slot, c->d_ro_ref, c->d_vel,
                   c->d_acc, c->d_minv, c->d_ke_part, c->d_rank, c->d_laz, c->d_drosum, c->d_r2, c->d_bm,
                   c->d_pm, c->d_skin};
   for (void* p : ptrs)

@@ -215,6 +215,9 @@ struct Ctx {
   int* d_vl_n = nullptr;
   int* d_vl_nbr = nullptr;
   double* d_vl_pos = nullptr;    // double4 per slot: position, type (+ kUncharged when uncharged)
+  int* d_vl_rel = nullptr;       // mixed path, int4 per slot: position relative to the corner of its
+                                 // cell at the build in units of cellsize / 2^28; w = that cell
+                                 // (9 bits per dimension) | type tag << 27
   int* d_vl_slot = nullptr;      // slot of each particle
   double* d_ro_ref = nullptr;    // positions at the last build
   // NList skin test (Src/nlist.F90:597-621): per-particle squared displacement since the build
@@ -279,6 +282,8 @@ int pair_tile_tx(const Ctx* c);                    // its cells per tile along x
 bool pair_many(const Ctx* c);                      // more types than the LDS-table kernels take
 int default_pair_tile(const Ctx* c);               // lanes per home for a new context
 int spme_setup(Ctx* c);                            // K-space arrays
+bool vl_rel_ok(const Ctx* c);                      // the Verlet pair loop takes the fixed-point records
+constexpr double kVlUnits = 268435456.0;           // 2^28 fixed-point units per cell of the Verlet grid
 void spme_charge_scale(Ctx* c);                    // MeshGrid::qshift from the charges
 int launch_spme_mesh(Ctx* c);                      // spread, FFTs + influence function
 int launch_spme_gather(Ctx* c, bool with_reduce = false);                    // force gather (after the pair kernel)

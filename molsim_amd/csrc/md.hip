@@ -45,15 +45,40 @@ __device__ inline void block_max_out(double v, double* bm) {
 // The Verlet list's slot-ordered records (position, type + kUncharged when uncharged) of the
 // current positions: one thread per slot gathers its particle's (a gather with coalesced
 // stores, where a scatter from the drift would read each record's line before writing it).
-__global__ __launch_bounds__(256) void k_vl_records(int np, const int* __restrict__ vl_home,
-                                                    const double* __restrict__ ro,
-                                                    const int* __restrict__ ipt,
-                                                    const int* __restrict__ laz, double4* vl_pos) {
+// With vl_rel (mixed path) also the fixed-point record relative to the corner of the slot's
+// cell at the build (the cell is kept in the record): (ro + L/2) - cell * cellsize in FP64,
+// taken back by L when the particle crossed the periodic boundary since the build, in units of
+// cellsize / 2^28 (rounded to nearest).
+struct VlRecArgs {
+  int np;
+  const int* vl_home;
+  const double* ro;
+  const int* ipt;
+  const int* laz;
+  double4* vl_pos;
+  int4* vl_rel;
+  Box box;
+  double csz[3];
+};
+
+__global__ __launch_bounds__(256) void k_vl_records(VlRecArgs a) {
   const int s = blockIdx.x * 256 + threadIdx.x;
-  if (s >= np) return;
-  const int i = vl_home[s];
-  vl_pos[s] = make_double4(ro[3 * i], ro[3 * i + 1], ro[3 * i + 2],
-                           (double)(ipt[i] + ((laz && !laz[i]) ? kUncharged : 0)));
+  if (s >= a.np) return;
+  const int i = a.vl_home[s];
+  const int tag = a.ipt[i] + ((a.laz && !a.laz[i]) ? kUncharged : 0);
+  const double r[3] = {a.ro[3 * i], a.ro[3 * i + 1], a.ro[3 * i + 2]};
+  a.vl_pos[s] = make_double4(r[0], r[1], r[2], (double)tag);
+  if (a.vl_rel) {
+    const int cw = a.vl_rel[s].w & ((1 << 27) - 1);
+    int q[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double x = (r[d] + a.box.half[d]) - ((cw >> (9 * d)) & 511) * a.csz[d];
+      if (fabs(x) > a.box.half[d]) x -= copysign(a.box.L[d], x);
+      q[d] = __double2int_rn(x * (kVlUnits / a.csz[d]));
+    }
+    a.vl_rel[s] = make_int4(q[0], q[1], q[2], cw | (tag << 27));
+  }
 }
 
 __global__ __launch_bounds__(256) void k_disp(int np, const double* __restrict__ ro,
@@ -312,6 +337,8 @@ int verlet_setup(Ctx* c) {
   if (int e = dalloc(c, (void**)&c->d_vl_home, np * sizeof(int))) return e;
   if (int e = dalloc(c, (void**)&c->d_vl_n, np * sizeof(int))) return e;
   if (int e = dalloc(c, (void**)&c->d_vl_pos, 4 * (size_t)np * sizeof(double))) return e;
+  if (!c->fp64)
+    if (int e = dalloc(c, (void**)&c->d_vl_rel, 4 * (size_t)np * sizeof(int))) return e;
   if (int e = dalloc(c, (void**)&c->d_vl_slot, np * sizeof(int))) return e;
   if (int e = dalloc(c, (void**)&c->d_ro_ref, 3 * (size_t)np * sizeof(double))) return e;
   if (int e = dalloc(c, (void**)&c->d_drosum, 3 * (size_t)np * sizeof(double))) return e;
@@ -326,8 +353,17 @@ int verlet_setup(Ctx* c) {
 }
 
 int launch_vl_records(Ctx* c) {
-  hipLaunchKernelGGL(k_vl_records, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, c->np, c->d_vl_home,
-                     c->d_ro, c->d_ipt, c->d_laz, (double4*)c->d_vl_pos);
+  VlRecArgs a;
+  a.np = c->np;
+  a.vl_home = c->d_vl_home;
+  a.ro = c->d_ro;
+  a.ipt = c->d_ipt;
+  a.laz = c->d_laz;
+  a.vl_pos = (double4*)c->d_vl_pos;
+  a.vl_rel = vl_rel_ok(c) ? (int4*)c->d_vl_rel : nullptr;
+  a.box = make_box(c->P);
+  for (int d = 0; d < 3; ++d) a.csz[d] = c->P.boxlen[d] / c->grid.nc[d];
+  hipLaunchKernelGGL(k_vl_records, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, a);
   MSG_HIP(hipGetLastError());
   return MSG_OK;
 }
@@ -493,6 +529,9 @@ int msg_md_run(msg_ctx* h, int nsteps, double dt, double* log) {
 
 int msg_nlist_interval(msg_ctx* h, int inlist) {
   if (!h || inlist < 0) return set_error(MSG_ERR_ARG, "msg_nlist_interval: inlist must be >= 0");  // Src/nlist.F90:617-618
+  // the mixed pair loop's fixed-point records assume the skin rule's bounded drift since the
+  // build: back from a fixed interval, the next pass builds the list afresh
+  if (h->c.inlist > 0 && inlist == 0) h->c.vl_valid = false;
   h->c.inlist = inlist;
   return MSG_OK;
 }

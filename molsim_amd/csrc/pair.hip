@@ -1244,6 +1244,7 @@ struct VlistArgs {
   int* vl_n;
   int* vl_nbr;
   double4* vl_pos;
+  int4* vl_rel;        // mixed path: fixed-point records relative to the cell (null otherwise)
   int* vl_slot;
   int* flags;
   int stage;           // lists staged in LDS (rows of maxn + 1) and written out row by row
@@ -1284,6 +1285,9 @@ __global__ __launch_bounds__(256) void k_vlist_build(VlistArgs a) {
     csz[d] = a.box.L[d] / a.g.nc[d];
     hrel[d] = (hh[d] + a.box.half[d]) - cc3[d] * csz[d];
   }
+  if (home && k == 0 && a.vl_rel)  // the cell of the build and the type tag in w
+    a.vl_rel[s] = make_int4(__double2int_rn(hrel[0] * (kVlUnits / csz[0])), __double2int_rn(hrel[1] * (kVlUnits / csz[1])),
+                            __double2int_rn(hrel[2] * (kVlUnits / csz[2])), cx | (cy << 9) | (cz << 18) | ((int)hp.w << 27));
   const double cull2 = a.rl2 * (1.0 + 1e-5);
   int st0[MPL], cnt[MPL], tot = 0;
 #pragma unroll
@@ -1472,6 +1476,153 @@ __global__ __launch_bounds__(256) void k_pair_vl(VlPairArgs v) {
   {
     const double t = msgd::wave_sum(acc.vir);
     if (lane == 0) s_red[wv][NTP] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x <= NTP) {
+    double t = 0.0;
+    for (int w = 0; w < 4; ++w) t += s_red[w][threadIdx.x];
+    a.part[blockIdx.x * (a.tm.ntp + 1) + (threadIdx.x == NTP ? a.tm.ntp : threadIdx.x)] = t;
+  }
+}
+
+// Mixed-path pair loop over the Verlet list with fixed-point records relative to the corner of
+// each slot's cell at the build (16 bytes per partner instead of 32; units of cellsize / 2^28,
+// 7.5e-8 A for 20 A cells): the displacement of home i and partner j is
+// (rel_i - rel_j) + (c_i - c_j) 2^28 units, exact in 32-bit integers, with the cell difference
+// taken to -1..1 per dimension (the partners of a home sat in its cell or a neighbouring one at
+// the build, and every dimension has at least three cells), which is the minimum image; it is
+// converted to FP32 with one rounding, as the FP64 difference is in the other kernels.
+// Positions are refreshed each pass relative to the same cells.  Four partners per round with
+// their loads issued first; the round's sums in FP32 (at most four terms), added to the FP64
+// sums once per round.
+struct VlRelArgs {
+  PairArgs p;
+  const int* vl_home;
+  const int* vl_n;
+  const int* vl_nbr;
+  const int4* vl_rel;
+  float unit[3];  // cellsize / 2^28
+  int nc[3];
+};
+
+template <int NTP>
+__global__ __launch_bounds__(256) void k_pair_vl_rel(VlRelArgs v) {
+  constexpr int NT = PairAcc<float, NTP>::NT, PB = 4;
+  const PairArgs& a = v.p;
+  extern __shared__ __align__(16) double s_dyn[];
+  __shared__ PairMeta<float> s_pm[kMaxT * kMaxT];
+  __shared__ int2 s_pi[kMaxT * kMaxT];
+  __shared__ int s_tp[kMaxT * kMaxT];
+  __shared__ double s_red[4][kMaxTP + 1];
+  const int nint = a.tm.nint;
+  float* s_coef = (float*)s_dyn;
+  int* s_bins = (int*)(s_coef + nint * 12);
+  lds_copy16(s_coef, (const float*)a.tcoef, nint * 12 * (int)sizeof(float));
+  lds_copy16(s_bins, a.tbins, NTP * kTabBins * (int)sizeof(int));
+  fill_pair_meta<float>(a.tm, s_pm, s_pi, s_tp);
+  __syncthreads();
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const float rc2 = (float)a.rcut2;
+  double et[NT], vir = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+#pragma unroll
+  for (int q = 0; q < NT; ++q) et[q] = 0.0;
+  int err = 0, ht = 0;
+  if (s < a.np) {
+    const int ip = v.vl_home[s];
+    const int4 hr = v.vl_rel[s];
+    const int hw = hr.w;
+    const int htag = hw >> 27;
+    ht = htag & (kUncharged - 1);
+    const int n = htag >= kUncharged ? 0 : v.vl_n[s];  // an uncharged home has no pair energy
+    const int hx[3] = {hr.x, hr.y, hr.z};
+    for (int k0 = 0; k0 < n; k0 += PB) {
+      int j[PB];
+      int4 p[PB];
+#pragma unroll
+      for (int u = 0; u < PB; ++u) j[u] = k0 + u < n ? v.vl_nbr[(size_t)(k0 + u) * a.np + s] : s;
+#pragma unroll
+      for (int u = 0; u < PB; ++u) p[u] = v.vl_rel[j[u]];
+      float ef[NT], vf = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+      for (int q = 0; q < NT; ++q) ef[q] = 0.f;
+#pragma unroll
+      for (int u = 0; u < PB; ++u) {
+        const int pw = p[u].w;
+        const int tj = pw >> 27;
+        // tj >= kUncharged: weak charges, no pair energy with an uncharged partner
+        if (k0 + u >= n || tj >= kUncharged) continue;
+        const int pr[3] = {p[u].x, p[u].y, p[u].z};
+        float d[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          int dc = ((hw >> (9 * q)) & 511) - ((pw >> (9 * q)) & 511);
+          dc -= dc > 1 ? v.nc[q] : 0;
+          dc += dc < -1 ? v.nc[q] : 0;
+          d[q] = (float)((hx[q] - pr[q]) + (dc << 28)) * v.unit[q];
+        }
+        const float r2 = fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]));
+        if (r2 > rc2) continue;  // UTwoBodyA's r2 > rcut2 test
+        const int mi = ht * kMaxT + tj;
+        const PairMeta<float> pm = s_pm[mi];
+        float us, fs;
+        if (r2 < pm.r2at) {
+          us = 1e10f;
+          fs = 1e10f;
+        } else if (r2 < pm.r2um) {
+          err |= FLAG_R2UMIN;
+          continue;
+        } else {
+          const int2 pi = s_pi[mi];
+          int b = (int)((r2 - pm.zlo) * pm.invw);
+          b = min(max(b, 0), kTabBins - 1);
+          int ti = s_bins[pi.x * kTabBins + b];
+          alignas(16) float t[12];
+          {
+            const float4* rp = (const float4*)(s_coef + 12 * ti);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) *(float4*)(t + 4 * q) = rp[q];
+          }
+          while (ti < pi.y && r2 < t[0]) {
+            ++ti;
+            const float4* rp = (const float4*)(s_coef + 12 * ti);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) *(float4*)(t + 4 * q) = rp[q];
+          }
+          const float dd = r2 - t[0];
+          us = t[1] + dd * (t[2] + dd * (t[3] + dd * (t[4] + dd * (t[5] + dd * t[6]))));
+          fs = t[7] + dd * (t[8] + dd * (t[9] + dd * (t[10] + dd * t[11])));
+        }
+#pragma unroll
+        for (int q = 0; q < NT; ++q) ef[q] += q == tj ? us : 0.f;
+        gx += fs * d[0];
+        gy += fs * d[1];
+        gz += fs * d[2];
+        vf -= fs * r2;
+      }
+#pragma unroll
+      for (int q = 0; q < NT; ++q) et[q] += (double)ef[q];
+      fx += (double)gx;
+      fy += (double)gy;
+      fz += (double)gz;
+      vir += (double)vf;
+    }
+    a.force[3 * ip + 0] = fx * a.scale;
+    a.force[3 * ip + 1] = fy * a.scale;
+    a.force[3 * ip + 2] = fz * a.scale;
+  }
+  if (err) atomicOr(a.flags, err);
+#pragma unroll
+  for (int q = 0; q < NTP; ++q) {
+    double val = 0.0;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) val += (s_tp[ht * kMaxT + t] == q) ? et[t] : 0.0;
+    const double tot = msgd::wave_sum(val);
+    if (lane == 0) s_red[wv][q] = tot;
+  }
+  {
+    const double tot = msgd::wave_sum(vir);
+    if (lane == 0) s_red[wv][NTP] = tot;
   }
   __syncthreads();
   if (threadIdx.x <= NTP) {
@@ -1750,6 +1901,18 @@ static size_t pair_lds(const Ctx* c) {
   return (size_t)c->tmeta.nint * 12 * (c->fp64 ? 8 : 4) + 8 + (size_t)c->tmeta.ntp * kTabBins * sizeof(int);
 }
 
+// The mixed path keeps the fixed-point Verlet records when every dimension of the list's cell
+// grid has 3 to 512 cells (the cell difference of two listed particles is then -1..1 after
+// wrapping, and a cell coordinate fits 9 bits) and the table is staged in LDS; its pair loop
+// takes them under the skin rule (inlist = 0), which bounds every particle's drift since the
+// build to the skin, so the records stay far inside 32 bits.
+bool vl_rel_ok(const Ctx* c) {
+  if (c->fp64 || !c->d_vl_rel || pair_many(c)) return false;
+  for (int d = 0; d < 3; ++d)
+    if (c->grid.nc[d] < 3 || c->grid.nc[d] > 512) return false;
+  return true;
+}
+
 int launch_pair(Ctx* c, bool last) {
   const PairArgs a = pair_args(c, last);
   const int nb = c->npart_pair;
@@ -1771,6 +1934,28 @@ int launch_pair(Ctx* c, bool last) {
     return MSG_OK;
   }
   const bool nbpar = pair_nbpar(c);
+  if (c->P.list_mode == MSG_LIST_VERLET && vl_rel_ok(c) && c->inlist == 0) {
+    VlRelArgs v;
+    v.p = a;
+    v.p.cut_inclusive = 1;
+    v.vl_home = c->d_vl_home;
+    v.vl_n = c->d_vl_n;
+    v.vl_nbr = c->d_vl_nbr;
+    v.vl_rel = (const int4*)c->d_vl_rel;
+    for (int d = 0; d < 3; ++d) {
+      v.unit[d] = (float)(c->P.boxlen[d] / c->grid.nc[d] / kVlUnits);
+      v.nc[d] = c->grid.nc[d];
+    }
+    switch (c->nptpt) {
+      case 1: hipLaunchKernelGGL((k_pair_vl_rel<1>), dim3(nb), dim3(256), lds, c->stream, v); break;
+      case 3: hipLaunchKernelGGL((k_pair_vl_rel<3>), dim3(nb), dim3(256), lds, c->stream, v); break;
+      case 6: hipLaunchKernelGGL((k_pair_vl_rel<6>), dim3(nb), dim3(256), lds, c->stream, v); break;
+      default: hipLaunchKernelGGL((k_pair_vl_rel<10>), dim3(nb), dim3(256), lds, c->stream, v); break;
+    }
+    prof_end(c, MSG_K_PAIR);
+    MSG_HIP(hipGetLastError());
+    return MSG_OK;
+  }
   if (c->P.list_mode == MSG_LIST_VERLET) {
     VlPairArgs v;
     v.p = a;
@@ -1863,6 +2048,7 @@ int launch_vlist_build(Ctx* c) {
   v.vl_n = c->d_vl_n;
   v.vl_nbr = c->d_vl_nbr;
   v.vl_pos = (double4*)c->d_vl_pos;
+  v.vl_rel = vl_rel_ok(c) ? (int4*)c->d_vl_rel : nullptr;
   v.vl_slot = c->d_vl_slot;
   v.flags = c->d_flags;
   prof_begin(c, MSG_K_VLIST);
