@@ -55,7 +55,7 @@ struct VlRecArgs {
   const double* ro;
   const int* ipt;
   const int* laz;
-  double4* vl_pos;
+  double4* vl_pos;   // null when the pass's pair loop reads only the fixed-point records
   int4* vl_rel;
   Box box;
   double csz[3];
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void k_vl_records(VlRecArgs a) {
   const int i = a.vl_home[s];
   const int tag = a.ipt[i] + ((a.laz && !a.laz[i]) ? kUncharged : 0);
   const double r[3] = {a.ro[3 * i], a.ro[3 * i + 1], a.ro[3 * i + 2]};
-  a.vl_pos[s] = make_double4(r[0], r[1], r[2], (double)tag);
+  if (a.vl_pos) a.vl_pos[s] = make_double4(r[0], r[1], r[2], (double)tag);
   if (a.vl_rel) {
     const int cw = a.vl_rel[s].w & ((1 << 27) - 1);
     int q[3];
@@ -111,8 +111,16 @@ __global__ __launch_bounds__(1024) void k_skin_scan(int nb, const double* __rest
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int per = (nb + 1023) / 1024;
   const int b0 = tid * per, b1 = min(nb, b0 + per);
+  // up to eight of the thread's block maxima kept in registers (their loads in flight
+  // together), the rest read in the loops
+  constexpr int R = 8;
+  double keep[R];
+#pragma unroll
+  for (int u = 0; u < R; ++u) keep[u] = b0 + u < b1 ? bm[b0 + u] : 0.0;
   double own = 0.0;
-  for (int b = b0; b < b1; ++b) own = fmax(own, bm[b]);
+#pragma unroll
+  for (int u = 0; u < R; ++u) own = fmax(own, keep[u]);
+  for (int b = b0 + R; b < b1; ++b) own = fmax(own, bm[b]);
   // inclusive scan of the threads' maxima: wavefront, then the wavefront totals
   double inc = own;
 #pragma unroll
@@ -126,7 +134,13 @@ __global__ __launch_bounds__(1024) void k_skin_scan(int nb, const double* __rest
   for (int k = 0; k < w; ++k) before = fmax(before, wtot[k]);
   double run = fmax(before, __shfl_up(inc, 1, 64));
   if (lane == 0) run = before;
-  for (int b = b0; b < b1; ++b) {
+#pragma unroll
+  for (int u = 0; u < R; ++u)
+    if (b0 + u < b1) {
+      pm[b0 + u] = run;
+      run = fmax(run, keep[u]);
+    }
+  for (int b = b0 + R; b < b1; ++b) {
     pm[b] = run;
     run = fmax(run, bm[b]);
   }
@@ -359,8 +373,8 @@ int launch_vl_records(Ctx* c) {
   a.ro = c->d_ro;
   a.ipt = c->d_ipt;
   a.laz = c->d_laz;
-  a.vl_pos = (double4*)c->d_vl_pos;
   a.vl_rel = vl_rel_ok(c) ? (int4*)c->d_vl_rel : nullptr;
+  a.vl_pos = (a.vl_rel && c->inlist == 0) ? nullptr : (double4*)c->d_vl_pos;  // as launch_pair chooses
   a.box = make_box(c->P);
   for (int d = 0; d < 3; ++d) a.csz[d] = c->P.boxlen[d] / c->grid.nc[d];
   hipLaunchKernelGGL(k_vl_records, dim3((c->np + 255) / 256), dim3(256), 0, c->stream, a);
