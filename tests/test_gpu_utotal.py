@@ -28,10 +28,14 @@ def _small(**kw):
 @pytest.mark.parametrize("precision,ftol,etol", [("mixed", MIXED_FTOL, MIXED_ETOL),
                                                  ("fp64", FP64_TOL, FP64_TOL)])
 @pytest.mark.parametrize("case", ["cell_spme", "cell_spme_nosurf", "all_spme", "cell_k16",
-                                  "cell_order4", "three_types", "dense_bricks"])
+                                  "cell_order4", "three_types", "dense_bricks", "big_charges",
+                                  "clustered"])
 def test_utotal_small(case, precision, ftol, etol):
-    """dense_bricks: 78 particles per 16x8x8 mesh brick, so the spread takes its row search
-    variant (the others have at most 48 per brick and take the row table)."""
+    """dense_bricks: 78 particles per 16x8x8 mesh brick (about 600 spread candidates per brick:
+    several candidate rounds per wavefront, passes of more than 32 contributors, and a coarser
+    32-bit fixed-point scale of the FP32 spread).  big_charges: charges +-20 (the scale moves
+    down by 2^5).  clustered: 250 ions in a 25 A cube inside a 60 A box, a few bricks holding
+    all of them."""
     kw = {}
     if case == "cell_spme_nosurf":
         kw = dict(lsurf=False)
@@ -45,7 +49,13 @@ def test_utotal_small(case, precision, ftol, etol):
         kw = dict(charges=(1.0, -2.0, 1.0), n=399)
     elif case == "dense_bricks":
         kw = dict(n=2500, L=60.0)
+    elif case == "big_charges":
+        kw = dict(charges=(20.0, -20.0))
+    elif case == "clustered":
+        kw = dict(n=250, L=25.0)
     sp = _small(**kw)
+    if case == "clustered":  # the ions as inserted into 25 A, now in a 60 A box (mesh 32^3)
+        sp = systems.SystemSpec(**{**sp.__dict__, "box": np.array([60.0, 60.0, 60.0])})
     if case == "dense_bricks":
         assert sp.n > 48 * (sp.nmesh // 16) * (sp.nmesh // 8) ** 2
     print(f"case={case} precision={precision} n={sp.n} K={sp.nmesh} order={sp.order}", flush=True)

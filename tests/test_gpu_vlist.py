@@ -133,6 +133,44 @@ def test_md_rebuilds_match_reference_loop():
     eng.close()
 
 
+def test_mixed_md_boundary_crossing_and_interval_switch():
+    """Mixed path (fixed-point Verlet records relative to each particle's cell at the build):
+    particles crossing the periodic boundary between builds follow the oracle-driven
+    velocity-Verlet loop to the mixed tolerance; after a run with a fixed rebuild interval,
+    returning to the skin rule builds the list afresh and the forces match the oracle."""
+    sp = _verlet(n=1500, L=80.0, skin=0.6, seed=11)
+    tab = oracle.build_table(sp.pot)
+    mass = [20.0, 35.0]
+    rng = np.random.default_rng(5)
+    v0 = rng.normal(size=sp.ro.shape) * 3.0
+    near = sp.ro[:, 0] > 0.5 * sp.box[0] - 1.0  # move out through the +x face
+    v0[near, 0] = np.abs(v0[near, 0]) + 20.0
+    dt, nsteps, scale = 0.004, 12, 100.0
+    eng = Engine(sp, precision="mixed", table=cabi_table(tab), force_scale=scale)
+    eng.md_init(mass, v0)
+    eng.md_run(nsteps, dt)
+    x = eng.positions()
+    xr, _, rebuilds, _ = vv_reference(sp, tab, mass, v0, dt, nsteps, scale)
+    crossed = int(np.count_nonzero(near & (x[:, 0] < 0.0)))
+    dx = x - xr
+    dx -= sp.box[0] * np.rint(dx / sp.box[0])
+    print(f"{crossed} particles crossed +x; rebuild steps {rebuilds}; max |dx| {np.abs(dx).max():.2e}", flush=True)
+    assert crossed > 0
+    assert np.abs(dx).max() < 1e-5
+    eng.nlist_interval(4)
+    eng.md_run(6, dt)
+    eng.nlist_interval(0)
+    b = eng.nlist_builds()
+    got = eng.utotal()
+    assert eng.nlist_builds() == b + 1
+    osp = systems.SystemSpec(**{**sp.__dict__, "ro": eng.positions(), "list_mode": "cell"})
+    ref = oracle_system(osp, tab).utotal()
+    ref["force"] = ref["force"] * scale
+    from gpu_helpers import compare
+    compare(got, ref, 1e-5, 1e-5)
+    eng.close()
+
+
 def test_md_fixed_interval_and_macrostep():
     sp = _verlet(n=1500, L=80.0, skin=0.6, seed=10)
     tab = oracle.build_table(sp.pot)
