@@ -556,6 +556,34 @@ __device__ inline T wave_sum(T v) {
   return v;
 }
 
+// One DPP move of a double (both halves, the same lane pattern).
+template <int CTL>
+__device__ inline double dpp_f64(double x) {
+  const long long u = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, CTL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over a wavefront, valid in every lane, in a fixed order: within each row of 16 lanes by
+// DPP rotations (8, 4, 2, 1), then the four row totals of lanes 0, 16, 32, 48 in that order.
+// No LDS round trip and no wait between the steps.
+__device__ inline double wave_sum_rows(double v) {
+  v += dpp_f64<0x128>(v);  // row_ror:8
+  v += dpp_f64<0x124>(v);  // row_ror:4
+  v += dpp_f64<0x122>(v);  // row_ror:2
+  v += dpp_f64<0x121>(v);  // row_ror:1
+  const long long u = __double_as_longlong(v);
+  double r[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)u, 16 * k);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), 16 * k);
+    r[k] = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  }
+  return ((r[0] + r[1]) + r[2]) + r[3];
+}
+
 // Sum over a workgroup of blockDim.x (multiple of 64) threads; result valid in thread 0.
 template <typename T>
 __device__ inline T block_sum(T v, T* sh) {

@@ -44,7 +44,7 @@ struct BinArgs {
 
 template <typename real>
 __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
-  __shared__ double sh[8];
+  __shared__ double sh[4][3];
   const int i = bid * 256 + threadIdx.x;
   double dx = 0.0, dy = 0.0, dz = 0.0;
   if (i < a.np) {
@@ -90,14 +90,19 @@ __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
     dz = q * r[2];
   }
   if (!a.dip_part) return;  // the same for the whole grid
-  dx = msgd::block_sum(dx, sh);
-  dy = msgd::block_sum(dy, sh);
-  dz = msgd::block_sum(dz, sh);
-  if (threadIdx.x == 0) {
-    a.dip_part[3 * bid + 0] = dx;
-    a.dip_part[3 * bid + 1] = dy;
-    a.dip_part[3 * bid + 2] = dz;
+  // the workgroup's partial dipole: wavefront sums (DPP), then the four wavefronts in order
+  dx = msgd::wave_sum_rows(dx);
+  dy = msgd::wave_sum_rows(dy);
+  dz = msgd::wave_sum_rows(dz);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[w][0] = dx;
+    sh[w][1] = dy;
+    sh[w][2] = dz;
   }
+  __syncthreads();
+  if (threadIdx.x < 3)
+    a.dip_part[3 * bid + threadIdx.x] = ((sh[0][threadIdx.x] + sh[1][threadIdx.x]) + sh[2][threadIdx.x]) + sh[3][threadIdx.x];
 }
 
 template <typename real>
