@@ -549,11 +549,59 @@ __device__ inline void bspline(real x, real* vals, real* ders) {
   }
 }
 
-template <typename T>
-__device__ inline T wave_sum(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+// The value of lane b ^ D, b = the lane's index (only its bit D matters): DPP within a row of
+// 16 lanes (quad permutes for 1 and 2, row shifts for 4, a row rotation for 8), a swizzle for
+// 16, a lane permute for 32.  row_shl:n reads lane + n, row_shr:n lane - n.  Every lane of
+// the wavefront must be active.
+template <int D>
+__device__ __forceinline__ int xor_lanes(int v, int b) {
+  if constexpr (D == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  if constexpr (D == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  if constexpr (D == 4) {
+    const int up = __builtin_amdgcn_mov_dpp(v, 0x104, 0xF, 0xF, false);    // row_shl:4
+    const int down = __builtin_amdgcn_mov_dpp(v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    return (b & 4) ? down : up;
+  }
+  if constexpr (D == 8) return __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, false);  // row_ror:8
+  if constexpr (D == 16) return __builtin_amdgcn_ds_swizzle(v, 0x1F | (16 << 10));  // bit mask mode: lane ^ 16
+  static_assert(D > 0 && D < 64 && (D & (D - 1)) == 0, "xor_lanes: D is a power of two below 64");
+  return __shfl_xor(v, 32, 64);
+}
+template <int D>
+__device__ __forceinline__ float xor_lanes(float v, int b) { return __int_as_float(xor_lanes<D>(__float_as_int(v), b)); }
+template <int D>
+__device__ __forceinline__ double xor_lanes(double v, int b) {
+  const long long u = __double_as_longlong(v);
+  const int lo = xor_lanes<D>((int)(unsigned)u, b), hi = xor_lanes<D>((int)(u >> 32), b);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over aligned groups of L lanes (a power of two, at most 64), valid in every lane of the
+// group: the butterfly over partners at distance L/2, ..., 2, 1 (the order of a shuffle
+// butterfly), through xor_lanes.  b = the lane's index.
+template <int L, typename T>
+__device__ __forceinline__ T xor_sum(T v, int b) {
+  if constexpr (L >= 64) v += xor_lanes<32>(v, b);
+  if constexpr (L >= 32) v += xor_lanes<16>(v, b);
+  if constexpr (L >= 16) v += xor_lanes<8>(v, b);
+  if constexpr (L >= 8) v += xor_lanes<4>(v, b);
+  if constexpr (L >= 4) v += xor_lanes<2>(v, b);
+  if constexpr (L >= 2) v += xor_lanes<1>(v, b);
   return v;
+}
+
+// Inclusive prefix sum over a wavefront's 64 lanes by DPP (no LDS round trip): row shifts by
+// 1, 2, 4, 8 scan each row of 16 lanes (a lane whose source lies in another row adds 0), then
+// lane 15 of rows 0 and 2 is added to rows 1 and 3 (row_bcast:15) and lane 31 to rows 2 and 3
+// (row_bcast:31).  Every lane of the wavefront must be active.
+__device__ __forceinline__ int wave_incl_scan(int x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);  // row_bcast:15, rows 1 and 3
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);  // row_bcast:31, rows 2 and 3
+  return x;
 }
 
 // One DPP move of a double (both halves, the same lane pattern).
@@ -583,6 +631,15 @@ __device__ inline double wave_sum_rows(double v) {
   }
   return ((r[0] + r[1]) + r[2]) + r[3];
 }
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+// FP64 sums take the DPP form.
+__device__ inline double wave_sum(double v) { return wave_sum_rows(v); }
 
 // Sum over a workgroup of blockDim.x (multiple of 64) threads; result valid in thread 0.
 template <typename T>

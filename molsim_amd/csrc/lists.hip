@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256) void k_bin_seg(const BinArgs* __restrict__ arg
 
 // Exclusive scans of cnt[0..n) and of ceil(cnt / DIV) (out2, may be null) by one workgroup of
 // NT threads, in tiles of NT * V values.  Thread t takes the values t, t + NT, ... of a tile
-// (coalesced loads and stores), the wavefronts scan the V rows with shuffles, one barrier
+// (coalesced loads and stores), the wavefronts scan the V rows by DPP, one barrier
 // exchanges the wavefront totals, and the running totals carry into the next tile.
 // out1[n] / out2[n] get the totals.  out1 / out2 may point to LDS or global memory.
 template <int NT, int V, int DIV>
@@ -137,15 +137,10 @@ __device__ void block_scan_tiled(int n, const int* __restrict__ cnt, int* out1, 
       i2[u] = (v[u] + DIV - 1) / DIV;
     }
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1)
-#pragma unroll
-      for (int u = 0; u < V; ++u) {
-        const int t1 = __shfl_up(i1[u], off, 64), t2 = __shfl_up(i2[u], off, 64);
-        if (lane >= off) {
-          i1[u] += t1;
-          i2[u] += t2;
-        }
-      }
+    for (int u = 0; u < V; ++u) {
+      i1[u] = msgd::wave_incl_scan(i1[u]);
+      i2[u] = msgd::wave_incl_scan(i2[u]);
+    }
     if (lane == 63)
 #pragma unroll
       for (int u = 0; u < V; ++u) {
@@ -398,7 +393,8 @@ struct FinalizeArgs {
 
 // One wavefront per cell: rank members by particle index (descending, the reference's
 // linked-list order) and write the cell-ordered arrays; cells also list their pair-kernel
-// work items.  The members are compared 64 at a time (one coalesced load, then shuffles);
+// work items.  The members are compared 64 at a time (one coalesced load, then each member
+// read from its lane into a scalar register: no LDS round trip);
 // a cell that holds every particle (one cell for the whole box) has the closed-form rank
 // np - 1 - ip.
 __device__ __forceinline__ void finalize_body(const FinalizeArgs& a, int bid) {
@@ -415,7 +411,7 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& a, int bid) {
         for (int j0 = 0; j0 < n; j0 += 64) {
           const int jv = j0 + lane < n ? a.cell_tmp[s0 + j0 + lane] : -1;
           const int m = min(64, n - j0);
-          for (int j = 0; j < m; ++j) rank += __shfl(jv, j, 64) > ip;
+          for (int j = 0; j < m; ++j) rank += __builtin_amdgcn_readlane(jv, j) > ip;
         }
       }
       if (k >= n) continue;

@@ -512,12 +512,9 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
                                  s_pm, s_pi, acc);
     }
   }
-#pragma unroll
-  for (int off = LPH / 2; off > 0; off >>= 1) {
-    acc.fx += __shfl_xor(acc.fx, off, 64);
-    acc.fy += __shfl_xor(acc.fy, off, 64);
-    acc.fz += __shfl_xor(acc.fz, off, 64);
-  }
+  acc.fx = msgd::xor_sum<LPH>(acc.fx, threadIdx.x);
+  acc.fy = msgd::xor_sum<LPH>(acc.fy, threadIdx.x);
+  acc.fz = msgd::xor_sum<LPH>(acc.fz, threadIdx.x);
   if (home && k == 0) {
     const int ip = a.cell_ip[s];
     a.force[3 * ip + 0] = acc.fx * a.scale;
@@ -841,12 +838,9 @@ __device__ __forceinline__ void pair_nb_rel_body(const PairArgs& a, int bid) {
     }
     wave_sync();  // the next chunk overwrites the list
   }
-#pragma unroll
-  for (int off = LPH / 2; off > 0; off >>= 1) {
-    fx += __shfl_xor(fx, off, 64);
-    fy += __shfl_xor(fy, off, 64);
-    fz += __shfl_xor(fz, off, 64);
-  }
+  fx = msgd::xor_sum<LPH>(fx, threadIdx.x);
+  fy = msgd::xor_sum<LPH>(fy, threadIdx.x);
+  fz = msgd::xor_sum<LPH>(fz, threadIdx.x);
   if (home && k == 0) {
     const int ip = a.cell_ip[s];
     a.force[3 * ip + 0] = fx * a.scale;

@@ -103,29 +103,7 @@ __device__ __forceinline__ C2 twiddle(const C2* s_tw, int j) {
   return INV ? cconj(w) : w;
 }
 
-// The value of lane b ^ D (D < 32), b the lane's index within its line of M >= 2D lanes: DPP
-// within a row of 16 lanes (quad permutes for 1 and 2, row shifts for 4, a row rotation for 8),
-// a swizzle for 16.  row_shl:n reads lane + n, row_shr:n lane - n.
-template <int D>
-__device__ __forceinline__ int xor_lanes(int v, int b) {
-  if constexpr (D == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
-  if constexpr (D == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
-  if constexpr (D == 4) {
-    const int up = __builtin_amdgcn_mov_dpp(v, 0x104, 0xF, 0xF, false);    // row_shl:4
-    const int down = __builtin_amdgcn_mov_dpp(v, 0x114, 0xF, 0xF, false);  // row_shr:4
-    return (b & 4) ? down : up;
-  }
-  if constexpr (D == 8) return __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, false);  // row_ror:8
-  return __builtin_amdgcn_ds_swizzle(v, 0x1F | (D << 10));  // bit mask mode: lane ^ D
-}
-template <int D>
-__device__ __forceinline__ float xor_lanes(float v, int b) { return __int_as_float(xor_lanes<D>(__float_as_int(v), b)); }
-template <int D>
-__device__ __forceinline__ double xor_lanes(double v, int b) {
-  const long long u = __double_as_longlong(v);
-  const int lo = xor_lanes<D>((int)(unsigned)u, b), hi = xor_lanes<D>((int)(u >> 32), b);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
+using msgd::xor_lanes;
 
 // One radix-2 stage between partner lanes at distance D (see line_fft).
 template <int K, int D, bool INV, typename C2>
@@ -1358,12 +1336,9 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int bid) {
       }
     }
   }
-#pragma unroll
-  for (int off = LPP / 2; off > 0; off >>= 1) {
-    e0 += __shfl_xor(e0, off, 64);
-    e1 += __shfl_xor(e1, off, 64);
-    e2 += __shfl_xor(e2, off, 64);
-  }
+  e0 = msgd::xor_sum<LPP>(e0, threadIdx.x);
+  e1 = msgd::xor_sum<LPP>(e1, threadIdx.x);
+  e2 = msgd::xor_sum<LPP>(e2, threadIdx.x);
   if (valid && ix == 0) {
     const int ip = a.brick_ip[i];
     const double q = (double)v.w;
