@@ -45,11 +45,14 @@ constexpr int kMaxTabIntervalsMany = 4096;  // records of a table of more types 
 constexpr int kMaxStencil = 27;
 constexpr int kPairHB = 4;           // home particles per pair-kernel work item
 // SPME bricks: each mesh point belongs to one brick; spread and gather work per brick.  Meshes
-// of a power-of-two K in 16..256 with orders 3..8 take bricks of exactly 16 x 8 x 8 points;
-// other meshes and orders (the general path) split each dimension into nb bricks of
-// floor(K / nb) or ceil(K / nb) points, nb = max(1, floor(K / kBG)), kBG >= order - 1, so a
-// particle's stencil reaches only its own brick and the next one in every dimension.
-constexpr int kBX = 16, kBY = 8, kBZ = 8;
+// of a power-of-two K in 16..256 with orders 3..8 take bricks of exactly bx x 8 x 8 points,
+// bx = 32 for K >= 32 (fewer candidates per brick point: the stencil halo is shared by more
+// points) and 16 for K = 16 (brick_x); other meshes and orders (the general path) split each
+// dimension into nb bricks of floor(K / nb) or ceil(K / nb) points, nb = max(1, floor(K / kBG)),
+// kBG >= order - 1, so a particle's stencil reaches only its own brick and the next one in
+// every dimension.
+constexpr int kBY = 8, kBZ = 8;
+constexpr int brick_x(int K) { return K >= 32 ? 32 : 16; }
 constexpr int kMinOrder = 2, kMaxOrder = 11;  // SPME orders (Src/potential.F90:359-360,3268)
 constexpr int kMaxRadices = 16;               // FFT stages of the general path
 

@@ -540,9 +540,12 @@ static Box make_box(const msg_params& P) {
 }
 
 // Fused scatter (offsets scanned per workgroup, slots from k_bin's ranks) for a role whose
-// counts fit in LDS: 1 cells, 2 bricks.
+// counts fit in LDS and whose per-workgroup scans (every workgroup reads all counts) stay
+// below kLsWork count reads: 1 cells, 2 bricks.
+constexpr long long kLsWork = 1ll << 22;
 static bool sort_fused(const Ctx* c, int role) {
-  return role == 1 ? c->grid.ncell <= kLsMax : c->mesh.nbrick <= kLsMax;
+  const long long n = role == 1 ? c->grid.ncell : c->mesh.nbrick;
+  return n <= kLsMax && n * ((c->np + 255) / 256) <= kLsWork;
 }
 
 static BinArgs bin_args(Ctx* c) {

@@ -1,13 +1,15 @@
 // K5-K8: smooth particle-mesh Ewald reciprocal space (UEwaldRecSPM + SPMFFTRec,
 // Src/energy.F90:1207-1297,6525-6587; influence function of EwaldSetup, :6969-7018).
 //
-//   k_spread    one workgroup per mesh brick (16x8x8 points).  Particles were binned by the
-//               brick of their B-spline base point (meshmax); a brick's contributors are the
-//               particles of its own and the next brick in each dimension.  Contributions are
-//               accumulated in LDS as 64-bit fixed-point integers (exact, order-independent,
-//               hence deterministic; no global atomics) and every mesh point is written once.
-//   k_fft_x     r2c along x: two real rows packed as one complex line, Stockham radix-4/2
-//               in LDS, split into the two half spectra.
+//   k_spread    one workgroup per mesh brick (32x8x8 points, 16x8x8 for K = 16).  Particles
+//               were binned by the brick of their B-spline base point (meshmax); a brick's
+//               contributors are the particles of its own and the next brick in each
+//               dimension.  Contributions are accumulated in LDS as fixed-point integers
+//               (32-bit with a per-brick scale in FP32, 64-bit in FP64: exact,
+//               order-independent, hence deterministic; no global atomics) and every mesh
+//               point is written once.
+//   k_fft_x     r2c along x: two real rows packed as one complex line, a line transform held
+//               in registers (below), split into the two half spectra.
 //   k_fft_y     complex FFT along y on tiles of T lines (forward or inverse).
 // Half-complex layout: kx = 0..K/2-1 as Cm[kx + (K/2)(y + K z)] (rows of K/2 values, so a
 // tile of consecutive kx is an aligned segment), and the Nyquist column kx = K/2 as its own
@@ -910,23 +912,23 @@ __device__ __forceinline__ int wave_prefix4(int v, int lane, int& total) {
 // y rows of P x points into the brick with integer LDS atomics (32-bit in FP32).  The x points outside
 // the brick add zero at addresses that stay inside the padded array.  Finally every mesh
 // point of the brick is written once.
-template <typename real, int P>
+template <typename real, int P, int BX>
 __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   typedef typename R4T<real>::t R4;
   // wavefronts, candidates per wavefront round, contributors per record pass (a round with
   // more contributors takes two passes: half the record area, more workgroups per CU)
   constexpr int NW = 4, WC = 64, HC = 32;
-  constexpr int NBP = kBX * kBY * kBZ;
-  // rows padded to kBX + 1 accumulators: the lanes adding one particle's rows (same x,
+  constexpr int NBP = BX * kBY * kBZ;
+  // rows padded to BX + 1 accumulators: the lanes adding one particle's rows (same x,
   // consecutive y/z) then fall into different LDS banks instead of one.  A row's x points
-  // lie at lx0 - ix, lx0 < 2 kBX (own or next brick), ix < P: the array is padded by P - 1
-  // in front and 2 kBX behind, so the zero-weight points of any row stay inside it.
-  constexpr int ROW = kBX + 1, NACC = ROW * kBY * kBZ;
+  // lie at lx0 - ix, lx0 < 2 BX (own or next brick), ix < P: the array is padded by P - 1
+  // in front and 2 BX behind, so the zero-weight points of any row stay inside it.
+  constexpr int ROW = BX + 1, NACC = ROW * kBY * kBZ;
   constexpr int PADL = P - 1;
   // contributor record: wx[P] | wy[P] | wz[P] | base point | ny, padded to 16 bytes
   constexpr int RW = sizeof(real) == 4 ? ((3 * P + 2 + 3) & ~3) : ((3 * P + 2 + 1) & ~1);
   typedef typename std::conditional<sizeof(real) == 4, unsigned, unsigned long long>::type acc_t;
-  __shared__ acc_t acc_s[PADL + NACC + 2 * kBX];
+  __shared__ acc_t acc_s[PADL + NACC + 2 * BX];
   acc_t* const acc = acc_s + PADL;
   __shared__ __align__(16) real rec_s[NW][HC][RW];
   __shared__ unsigned short tsk_s[NW][HC * P];  // tasks: contributor | z index << 8 | y0 << 11
@@ -934,7 +936,7 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   const int K = a.m.K;
   const int b = bid;
   const int bx = b % a.m.nb[0], by = (b / a.m.nb[0]) % a.m.nb[1], bz = b / (a.m.nb[0] * a.m.nb[1]);
-  const int lo[3] = {bx * kBX, by * kBY, bz * kBZ};
+  const int lo[3] = {bx * BX, by * kBY, bz * kBZ};
   const bool whole[3] = {a.m.nb[0] == 1, a.m.nb[1] == 1, a.m.nb[2] == 1};
   if (threadIdx.x < 8) {  // the (up to) 8 source bricks: own and next in each dimension
     const int ox = whole[0] ? 1 : 2, oy = whole[1] ? 1 : 2, oz = whole[2] ? 1 : 2;
@@ -950,7 +952,7 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
     r_beg[t] = beg;
     r_len[t] = len;
   }
-  for (int k = threadIdx.x; k < PADL + NACC + 2 * kBX; k += 256) acc_s[k] = 0;
+  for (int k = threadIdx.x; k < PADL + NACC + 2 * BX; k += 256) acc_s[k] = 0;
   __syncthreads();
   if (threadIdx.x == 0) {
     int pref = 0;
@@ -1004,7 +1006,7 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
 #pragma unroll
       for (int e = 0; e < 3; ++e) l[e] += l[e] < 0 ? K : 0;
       int x0, nx, y0, ny, z0, nz;
-      brick_range<P>(l[0], kBX, whole[0], x0, nx);
+      brick_range<P>(l[0], BX, whole[0], x0, nx);
       brick_range<P>(l[1], kBY, whole[1], y0, ny);
       brick_range<P>(l[2], kBZ, whole[2], z0, nz);
       hit = nx > 0 && ny > 0 && nz > 0;
@@ -1087,24 +1089,24 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   real* Q = (real*)a.Q;
   const double unit = sizeof(real) == 4 ? ldexp(1.0, -sh) : 1.0 / kQScale;
   for (int k = threadIdx.x; k < NBP; k += 256) {
-    const int lx = k % kBX, ly = (k / kBX) % kBY, lz = k / (kBX * kBY);
+    const int lx = k % BX, ly = (k / BX) % kBY, lz = k / (BX * kBY);
     const double v = acc_value(acc[ROW * (ly + kBY * lz) + lx]) * unit;
     Q[(lo[0] + lx) + (size_t)K * ((lo[1] + ly) + (size_t)K * (lo[2] + lz))] = (real)v;
   }
 }
 
 // Registers capped for six workgroups per CU (the LDS use allows seven).
-template <typename real, int P>
+template <typename real, int P, int BX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_spread(SpreadArgs a) {
-  spread_body<real, P>(a, blockIdx.x);
+  spread_body<real, P, BX>(a, blockIdx.x);
 }
 
-template <typename real, int P>
+template <typename real, int P, int BX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_spread_seg(const SpreadArgs* __restrict__ args,
                                                     const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
   if (s.x < 0) return;  // no work for this XCD slot
-  spread_body<real, P>(args[s.x], s.y);
+  spread_body<real, P, BX>(args[s.x], s.y);
 }
 
 // General-path spread: the same algorithm over bricks of any extent.  Brick b of a dimension
@@ -1529,7 +1531,10 @@ void launch_spread_gather(Ctx* c, bool spread, bool with_reduce) {
   if (spread) {
     const SpreadArgs a = spread_args(c);
     prof_begin(c, MSG_K_SPREAD);
-    hipLaunchKernelGGL((k_spread<real, P>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
+    if (brick_x(c->mesh.K) == 32)
+      hipLaunchKernelGGL((k_spread<real, P, 32>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL((k_spread<real, P, 16>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
     prof_end(c, MSG_K_SPREAD);
   } else {
     const GatherArgs g = gather_args(c);
@@ -1616,7 +1621,7 @@ int spme_setup(Ctx* c) {
   const bool pow2 = K >= 16 && K <= 256 && (K & (K - 1)) == 0;
   m.gen = !(pow2 && p >= 3 && p <= 8);
   if (!m.gen) {
-    m.nb[0] = K / kBX;
+    m.nb[0] = K / brick_x(K);
     m.nb[1] = K / kBY;
     m.nb[2] = K / kBZ;
     if (!c->fp64 && K <= 128) {
@@ -1829,9 +1834,9 @@ int add_fft_k(Ctx* c, SegBuilder<FftArgs>* b) {
   return MSG_OK;
 }
 
-template <typename real, int P>
+template <typename real, int P, int BX>
 void seg_spread(const SegGroup& g, hipStream_t s) {
-  hipLaunchKernelGGL((k_spread_seg<real, P>), dim3(g.nblk), dim3(256), 0, s, (const SpreadArgs*)g.d_args,
+  hipLaunchKernelGGL((k_spread_seg<real, P, BX>), dim3(g.nblk), dim3(256), 0, s, (const SpreadArgs*)g.d_args,
                      g.d_map);
 }
 template <typename real, int P>
@@ -1873,10 +1878,11 @@ int add_spread_gather_g(Ctx* c, SegBuilder<SpreadArgs>& bs, SegBuilder<GatherArg
 template <typename real>
 int add_spread_gather(Ctx* c, SegBuilder<SpreadArgs>& bs, SegBuilder<GatherArgs>& bg) {
   if (c->mesh.gen) return add_spread_gather_g<real>(c, bs, bg);
-  const int key = (sizeof(real) == 8 ? 100 : 0) + c->mesh.order;
+  const bool wide = brick_x(c->mesh.K) == 32;
+  const int key = (sizeof(real) == 8 ? 100 : 0) + (wide ? 1000 : 0) + c->mesh.order;
   SegLaunchFn fs, fg;
-#define MSG_SG(P)              \
-  fs = seg_spread<real, P>; \
+#define MSG_SG(P)                                                 \
+  fs = wide ? seg_spread<real, P, 32> : seg_spread<real, P, 16>; \
   fg = seg_gather<real, P>;
   switch (c->mesh.order) {
     case 3: MSG_SG(3) break;
