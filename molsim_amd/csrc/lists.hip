@@ -42,11 +42,31 @@ struct BinArgs {
                     // scatter keeps its own counters: large grids, batched passes)
 };
 
+// Counter increments of a wavefront aggregated over runs of consecutive lanes with the same
+// key (particles of a chain are consecutive in index and space, so they fill the same cell or
+// brick in runs): the first lane of a run adds the run's length, and every lane's place is
+// the value its run's first lane got plus its position in the run.  A lane with key < 0 adds
+// nothing.  Every lane of the wavefront must take part.  Returns the lane's place.
+__device__ __forceinline__ int run_add(int* cnt, int key, int lane) {
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long H = __ballot(head);
+  const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0 .. lane
+  const int hl = 63 - __clzll(H & upto);                  // this run's first lane
+  const unsigned long long above = H & ~upto;
+  const int next = above ? __ffsll(above) - 1 : 64;       // the next run's first lane
+  int base = 0;
+  if (head && key >= 0) base = atomicAdd(cnt + key, next - lane);
+  return __shfl(base, hl, 64) + (lane - hl);
+}
+
 template <typename real>
 __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
   __shared__ double sh[4][3];
   const int i = bid * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
   double dx = 0.0, dy = 0.0, dz = 0.0;
+  int ckey = -1, bkey = -1;  // the particle's cell and brick (-1: no particle)
   if (i < a.np) {
     const double r[3] = {a.ro[3 * i], a.ro[3 * i + 1], a.ro[3 * i + 2]};
     const double q = a.az[i];
@@ -60,12 +80,8 @@ __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
     }
     if (bad) atomicOr(a.flags, FLAG_CELL);
     if (a.use_cells) {
-      const int c = ci[0] + a.g.nc[0] * (ci[1] + a.g.nc[1] * ci[2]);
-      a.cellid[i] = c;
-      if (a.rank_cell)
-        a.rank_cell[i] = atomicAdd(&a.cell_count[c], 1);
-      else
-        atomicAdd(&a.cell_count[c], 1);
+      ckey = ci[0] + a.g.nc[0] * (ci[1] + a.g.nc[1] * ci[2]);
+      a.cellid[i] = ckey;
     }
     if (a.use_mesh) {
       real* frac = (real*)a.frac;
@@ -79,15 +95,20 @@ __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
         mi[d] = k < 0 ? k + a.m.K : k;
       }
       a.mm[i] = mi[0] | (mi[1] << 10) | (mi[2] << 20);
-      const int b = brick_index(a.m, mi[0], mi[1], mi[2]);
-      if (a.rank_brick)
-        a.rank_brick[i] = atomicAdd(&a.brick_count[b], 1);
-      else
-        atomicAdd(&a.brick_count[b], 1);
+      bkey = brick_index(a.m, mi[0], mi[1], mi[2]);
     }
     dx = q * r[0];
     dy = q * r[1];
     dz = q * r[2];
+  }
+  // the cell and brick counters (uniform conditions: the whole wavefront takes part)
+  if (a.use_cells) {
+    const int rk = run_add(a.cell_count, ckey, lane);
+    if (a.rank_cell && i < a.np) a.rank_cell[i] = rk;
+  }
+  if (a.use_mesh) {
+    const int rk = run_add(a.brick_count, bkey, lane);
+    if (a.rank_brick && i < a.np) a.rank_brick[i] = rk;
   }
   if (!a.dip_part) return;  // the same for the whole grid
   // the workgroup's partial dipole: wavefront sums (DPP), then the four wavefronts in order
