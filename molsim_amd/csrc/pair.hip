@@ -646,6 +646,7 @@ __device__ __forceinline__ void pair_nb_rel_body(const PairArgs& a, int bid) {
   const int hs = home ? s : a.np - 1;
   const float4 hp = a.crel[hs];
   const int c = a.slot_cell[hs];
+  const int ip = a.cell_ip[hs];  // the home's particle index, for the force store at the end
   __syncthreads();  // table, metadata and stencil in LDS
   const int hw = (int)hp.w;
   const int ht = hw & (kUncharged - 1);
@@ -842,7 +843,6 @@ __device__ __forceinline__ void pair_nb_rel_body(const PairArgs& a, int bid) {
   fy = msgd::xor_sum<LPH>(fy, threadIdx.x);
   fz = msgd::xor_sum<LPH>(fz, threadIdx.x);
   if (home && k == 0) {
-    const int ip = a.cell_ip[s];
     a.force[3 * ip + 0] = fx * a.scale;
     a.force[3 * ip + 1] = fy * a.scale;
     a.force[3 * ip + 2] = fz * a.scale;

@@ -932,49 +932,44 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   acc_t* const acc = acc_s + PADL;
   __shared__ __align__(16) real rec_s[NW][HC][RW];
   __shared__ unsigned short tsk_s[NW][HC * P];  // tasks: contributor | z index << 8 | y0 << 11
-  __shared__ int r_beg[8], r_pref[9], r_len[8];
   const int K = a.m.K;
   const int b = bid;
   const int bx = b % a.m.nb[0], by = (b / a.m.nb[0]) % a.m.nb[1], bz = b / (a.m.nb[0] * a.m.nb[1]);
   const int lo[3] = {bx * BX, by * kBY, bz * kBZ};
   const bool whole[3] = {a.m.nb[0] == 1, a.m.nb[1] == 1, a.m.nb[2] == 1};
-  if (threadIdx.x < 8) {  // the (up to) 8 source bricks: own and next in each dimension
+  // the (up to) 8 source bricks, own and next in each dimension: lanes 0..7 of every
+  // wavefront load their ranges, which every wavefront then holds in scalar registers (no
+  // LDS round trip, no barrier), so a candidate's place is found without LDS reads
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int beg = 0, len = 0;
+  if (lane < 8) {
     const int ox = whole[0] ? 1 : 2, oy = whole[1] ? 1 : 2, oz = whole[2] ? 1 : 2;
-    const int t = threadIdx.x;
-    int beg = 0, len = 0;
-    if (t < ox * oy * oz) {
-      const int ix = t % ox, iy = (t / ox) % oy, iz = t / (ox * oy);
+    if (lane < ox * oy * oz) {
+      const int ix = lane % ox, iy = (lane / ox) % oy, iz = lane / (ox * oy);
       const int mb = (bx + ix) % a.m.nb[0] +
                      a.m.nb[0] * ((by + iy) % a.m.nb[1] + a.m.nb[1] * ((bz + iz) % a.m.nb[2]));
       beg = a.brick_start[mb];
       len = a.brick_start[mb + 1] - beg;
     }
-    r_beg[t] = beg;
-    r_len[t] = len;
   }
   for (int k = threadIdx.x; k < PADL + NACC + 2 * BX; k += 256) acc_s[k] = 0;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int pref = 0;
-    for (int t = 0; t < 8; ++t) {
-      r_pref[t] = pref;
-      pref += r_len[t];
-    }
-    r_pref[8] = pref;
-  }
-  __syncthreads();
-  // the source ranges in scalar registers: a candidate's place is found without LDS reads
   int pref[9], dlt[8];
+  {
+    int run = 0;
 #pragma unroll
-  for (int t = 0; t < 9; ++t) pref[t] = __builtin_amdgcn_readfirstlane(r_pref[t]);
-#pragma unroll
-  for (int t = 0; t < 8; ++t) dlt[t] = __builtin_amdgcn_readfirstlane(r_beg[t]) - pref[t];
+    for (int t = 0; t < 8; ++t) {
+      pref[t] = run;
+      dlt[t] = __builtin_amdgcn_readlane(beg, t) - run;
+      run += __builtin_amdgcn_readlane(len, t);
+    }
+    pref[8] = run;
+  }
   const int total = pref[8];
+  __syncthreads();  // the accumulator is zeroed
   const R4* bq = (const R4*)a.bq;
   // FP32: the brick's scale from its candidate count; FP64: 2^48 in the conversion
   const int sh = sizeof(real) == 4 ? brick_shift<P>(a.m, total) : 0;
   const real qs = sizeof(real) == 4 ? (real)ldexpf(1.0f, sh) : (real)1;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   real (*rec)[RW] = rec_s[w];
   unsigned short* tsk = tsk_s[w];
   int t = w * WC + lane;
@@ -1296,6 +1291,14 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int bid) {
   real e0 = 0, e1 = 0, e2 = 0;
   R4 v;
   v.x = v.y = v.z = v.w = real(0);
+  // the particle's index and pair force, loaded first so the loads overlap the field sums
+  int ip = 0;
+  double pf[3] = {0.0, 0.0, 0.0};
+  if (valid && ix == 0) {
+    ip = a.brick_ip[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pf[d] = a.force[3 * ip + d];
+  }
   if (valid) {
     v = ((const R4*)a.bq)[i];
     const int packed = a.bmm[i];
@@ -1342,13 +1345,12 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, int bid) {
   e1 = msgd::xor_sum<LPP>(e1, threadIdx.x);
   e2 = msgd::xor_sum<LPP>(e2, threadIdx.x);
   if (valid && ix == 0) {
-    const int ip = a.brick_ip[i];
     const double q = (double)v.w;
     const double es[3] = {(double)e0, (double)e1, (double)e2};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const double f = -q * es[d] * a.m.dkdr[d] * a.epsi4pi - a.epsi4pi * a.surf2fac * q * M[d];
-      a.force[3 * ip + d] = (a.force[3 * ip + d] + f) * a.scale;
+      a.force[3 * ip + d] = (pf[d] + f) * a.scale;
     }
   }
 }
