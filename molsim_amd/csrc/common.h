@@ -644,6 +644,30 @@ __device__ inline T wave_sum(T v) {
 // FP64 sums take the DPP form.
 __device__ inline double wave_sum(double v) { return wave_sum_rows(v); }
 
+// Sums of two FP64 values over a workgroup of blockDim.x (a multiple of 64, at most 1024)
+// threads, valid in thread 0: the same order as block_sum (wavefront sums, then the
+// wavefronts in order) with one barrier for both.  sh holds 2 * 16 values; it must not be
+// reused before a later barrier.
+__device__ inline void block_sum2(double& a, double& b, double* sh) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[w] = a;
+    sh[16 + w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0.0, sb = 0.0;
+    for (int i = 0; i < nw; ++i) {
+      sa += sh[i];
+      sb += sh[16 + i];
+    }
+    a = sa;
+    b = sb;
+  }
+}
+
 // Sum over a workgroup of blockDim.x (multiple of 64) threads; result valid in thread 0.
 template <typename T>
 __device__ inline T block_sum(T v, T* sh) {

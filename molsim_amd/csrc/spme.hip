@@ -369,7 +369,7 @@ __device__ __forceinline__ void fft_zconv_body(const FftArgs& fa, int bid) {
   double* __restrict__ part = fa.part;
   __shared__ C2 A[T * LSX];
   __shared__ C2 s_tw[K];
-  __shared__ double red[8];
+  __shared__ double red[32];
   stage_twiddles<C2, K, 256>(s_tw, fa.tw);
   const int L0 = bid * T;
   hc_load<C2, K, false>(A, Cm, L0);
@@ -422,8 +422,7 @@ __device__ __forceinline__ void fft_zconv_body(const FftArgs& fa, int bid) {
   for (int r = 0; r < 8; ++r) Al[lpos(line_pos<K>(r, b))] = v[r];
   __syncthreads();
   hc_store<C2, K, false>(Cm, A, L0);
-  esum = msgd::block_sum(esum, red);
-  vsum = msgd::block_sum(vsum, red);
+  msgd::block_sum2(esum, vsum, red);
   if (threadIdx.x == 0) {
     part[2 * bid] = esum;
     part[2 * bid + 1] = vsum;
@@ -725,7 +724,7 @@ __device__ __forceinline__ void gfft_zconv_body(const FftArgs& fa, int bid) {
   typedef typename C2T<real>::t C2;
   constexpr int E = FftCfg<real>::E;
   __shared__ C2 A[E], B[E], s_tw[E];
-  __shared__ double red[8];
+  __shared__ double red[32];
   const int K = fa.K, H = K / 2 + 1, T = gen_lines_yz<real>(K), LS = K + 1, NL = H * K;
   const int NH = K / 2, NREG = NH * K;
   C2* __restrict__ Cm = (C2*)fa.Cm;
@@ -766,8 +765,7 @@ __device__ __forceinline__ void gfft_zconv_body(const FftArgs& fa, int bid) {
     const int kl = e % T, z = e / T, L = L0 + kl;
     if (L < NL) Cm[hc_index_g<real>(K, L, z, false)] = S[kl * LS + z];
   }
-  esum = msgd::block_sum(esum, red);
-  vsum = msgd::block_sum(vsum, red);
+  msgd::block_sum2(esum, vsum, red);
   if (threadIdx.x == 0) {
     fa.part[2 * bid] = esum;
     fa.part[2 * bid + 1] = vsum;
