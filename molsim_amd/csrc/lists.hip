@@ -45,18 +45,31 @@ struct BinArgs {
 // Counter increments of a wavefront aggregated over runs of consecutive lanes with the same
 // key (particles of a chain are consecutive in index and space, so they fill the same cell or
 // brick in runs): the first lane of a run adds the run's length, and every lane's place is
-// the value its run's first lane got plus its position in the run.  A lane with key < 0 adds
-// nothing.  Every lane of the wavefront must take part.  Returns the lane's place.
-__device__ __forceinline__ int run_add(int* cnt, int key, int lane) {
+// the value its run's first lane got plus its position in the run.  A wavefront without runs
+// takes one increment per lane.  A lane with key < 0 adds nothing.  Every lane of the
+// wavefront must take part.  Returns the lane's place when want is set (uniform), else 0.
+__device__ __forceinline__ int run_add(int* cnt, int key, int lane, bool want) {
   const int prev = __shfl_up(key, 1, 64);
   const bool head = lane == 0 || prev != key;
   const unsigned long long H = __ballot(head);
+  if (H == ~0ull) {  // no runs in this wavefront
+    if (key < 0) return 0;
+    if (want) return atomicAdd(cnt + key, 1);
+    atomicAdd(cnt + key, 1);
+    return 0;
+  }
   const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0 .. lane
   const int hl = 63 - __clzll(H & upto);                  // this run's first lane
   const unsigned long long above = H & ~upto;
   const int next = above ? __ffsll(above) - 1 : 64;       // the next run's first lane
   int base = 0;
-  if (head && key >= 0) base = atomicAdd(cnt + key, next - lane);
+  if (head && key >= 0) {
+    if (want)
+      base = atomicAdd(cnt + key, next - lane);
+    else
+      atomicAdd(cnt + key, next - lane);
+  }
+  if (!want) return 0;
   return __shfl(base, hl, 64) + (lane - hl);
 }
 
@@ -103,11 +116,11 @@ __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
   }
   // the cell and brick counters (uniform conditions: the whole wavefront takes part)
   if (a.use_cells) {
-    const int rk = run_add(a.cell_count, ckey, lane);
+    const int rk = run_add(a.cell_count, ckey, lane, a.rank_cell != nullptr);
     if (a.rank_cell && i < a.np) a.rank_cell[i] = rk;
   }
   if (a.use_mesh) {
-    const int rk = run_add(a.brick_count, bkey, lane);
+    const int rk = run_add(a.brick_count, bkey, lane, a.rank_brick != nullptr);
     if (a.rank_brick && i < a.np) a.rank_brick[i] = rk;
   }
   if (!a.dip_part) return;  // the same for the whole grid
@@ -304,16 +317,21 @@ struct ScatterArgs {
 // and the mesh branch does not wait for a ranking pass.
 template <typename real>
 __device__ __forceinline__ void scatter_body(const ScatterArgs& a, int bid) {
-  const int i = bid * 256 + threadIdx.x;
-  if (i >= a.np) return;
+  // slots from the fill counters, increments aggregated over runs (the whole wavefront takes
+  // part; lanes past the end add nothing)
+  const int i = bid * 256 + threadIdx.x, lane = threadIdx.x & 63;
+  const bool ok = i < a.np;
   if (a.parts & 1) {
-    const int c = a.cellid[i];
-    a.cell_tmp[a.cell_start[c] + atomicAdd(&a.cell_fill[c], 1)] = i;
+    const int c = ok ? a.cellid[i] : -1;
+    const int rk = run_add(a.cell_fill, c, lane, true);
+    if (ok) a.cell_tmp[a.cell_start[c] + rk] = i;
   }
   if ((a.parts & 2) && a.use_mesh) {
-    const int p = a.mm[i];
-    const int b = brick_index(a.m, p & 1023, (p >> 10) & 1023, p >> 20);
-    const int dst = a.brick_start[b] + atomicAdd(&a.brick_fill[b], 1);
+    const int p = ok ? a.mm[i] : 0;
+    const int b = ok ? brick_index(a.m, p & 1023, (p >> 10) & 1023, p >> 20) : -1;
+    const int rk = run_add(a.brick_fill, b, lane, true);
+    if (!ok) return;
+    const int dst = a.brick_start[b] + rk;
     typedef typename std::conditional<sizeof(real) == 4, float4, double4>::type R4;
     const real* fr = (const real*)a.frac;
     R4 v;

@@ -1879,7 +1879,8 @@ template <typename real>
 int add_spread_gather(Ctx* c, SegBuilder<SpreadArgs>& bs, SegBuilder<GatherArgs>& bg) {
   if (c->mesh.gen) return add_spread_gather_g<real>(c, bs, bg);
   const bool wide = brick_x(c->mesh.K) == 32;
-  const int key = (sizeof(real) == 8 ? 100 : 0) + (wide ? 1000 : 0) + c->mesh.order;
+  // the spread depends on the brick shape, the gather does not
+  const int key = (sizeof(real) == 8 ? 100 : 0) + c->mesh.order, skey = key + (wide ? 1000 : 0);
   SegLaunchFn fs, fg;
 #define MSG_SG(P)                                                 \
   fs = wide ? seg_spread<real, P, 32> : seg_spread<real, P, 16>; \
@@ -1894,7 +1895,7 @@ int add_spread_gather(Ctx* c, SegBuilder<SpreadArgs>& bs, SegBuilder<GatherArgs>
     default: return set_error(MSG_ERR_ARG, "SPME order must be 3..8");
   }
 #undef MSG_SG
-  bs.add(key, fs, spread_args(c), c->mesh.nbrick);
+  bs.add(skey, fs, spread_args(c), c->mesh.nbrick);
   bg.add(key, fg, gather_args(c), (8 * c->np + 255) / 256);
   return MSG_OK;
 }
