@@ -31,9 +31,9 @@ def _small(**kw):
                                   "cell_order4", "three_types", "dense_bricks", "big_charges",
                                   "clustered"])
 def test_utotal_small(case, precision, ftol, etol):
-    """dense_bricks: 78 particles per 16x8x8 mesh brick (about 600 spread candidates per brick:
-    several candidate rounds per wavefront, passes of more than 32 contributors, and a coarser
-    32-bit fixed-point scale of the FP32 spread).  big_charges: charges +-20 (the scale moves
+    """dense_bricks: about 156 particles per 32x8x8 mesh brick (about 1250 spread candidates per
+    brick: several candidate rounds per wavefront, passes of more than 32 contributors, and a
+    coarser 32-bit fixed-point scale of the FP32 spread).  big_charges: charges +-20 (the scale moves
     down by 2^5).  clustered: 250 ions in a 25 A cube inside a 60 A box, a few bricks holding
     all of them."""
     kw = {}
