@@ -120,7 +120,7 @@ void seg_reduce(const SegGroup& g, hipStream_t s) {
 
 int launch_zero(Ctx* c) {
   const int na = c->grid.ncell;
-  const int nb = c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.nbrick : 0;
+  const int nb = c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.nkey : 0;
   const int n = std::max(std::max(na, nb), 1);
   hipLaunchKernelGGL(k_zero, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_cell_count, na,
                      c->d_brick_count, nb, c->d_flags);
@@ -159,7 +159,7 @@ ReduceArgs reduce_args(Ctx* c, bool presum) {
   a.cell_count = c->d_cell_count;
   a.ncell = c->grid.ncell;
   a.brick_count = c->d_brick_count;
-  a.nbrick = c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.nbrick : 0;
+  a.nbrick = c->P.ewald_mode == MSG_EWALD_SPM ? c->mesh.nkey : 0;
   return a;
 }
 

@@ -53,6 +53,16 @@ constexpr int kPairHB = 4;           // home particles per pair-kernel work item
 // every dimension.
 constexpr int kBY = 8, kBZ = 8;
 constexpr int brick_x(int K) { return K >= 32 ? 32 : 16; }
+// The fused sort (every scatter workgroup scans all counts itself) is used while the counts
+// number at most kLsMax and all workgroups' scans read at most kLsWork counts together.
+constexpr int kLsMax = 8192;
+constexpr long long kLsWork = 1ll << 22;
+// Brick classes (brick_key) multiply the brick sort's counters by 8.  They pay where the counters
+// stay few (at most 2048: the small meshes of batched ensembles, whose pass got 7 % shorter);
+// on the 32 768-particle 128^3 pass they lengthened the fused sort's per-workgroup scans more
+// than they shortened the spread, and at 1 M particles the longer scan and binning ate the
+// spread's and the gather's gain.
+inline bool brick_classes(int nbrick) { return 8ll * nbrick <= 2048; }
 constexpr int kMinOrder = 2, kMaxOrder = 11;  // SPME orders (Src/potential.F90:359-360,3268)
 constexpr int kMaxRadices = 16;               // FFT stages of the general path
 
@@ -90,6 +100,9 @@ struct MeshGrid {
   int K, H, order;
   int nb[3];      // bricks per dimension
   int nbrick;
+  int bx;         // brick extent in x on the power-of-two path
+  int ncls;       // classes per brick (8 on the power-of-two path, else 1), see brick_key
+  int nkey;       // nbrick * ncls: counters of the brick sort
   double dkdr[3]; // K / L
   int gen;        // general path: any even K, orders 2..11 (runtime-K FFT, bricks of any extent)
   int qshift;     // FP32 spread: contributions are integers in units of 2^-qshift (30 - e, 2^e >= max |q|)
@@ -101,6 +114,18 @@ struct MeshGrid {
 __host__ __device__ inline int brick_of(int m, int nb, int K) { return ((m + 1) * nb - 1) / K; }
 __host__ __device__ inline int brick_index(const MeshGrid& g, int mx, int my, int mz) {
   return brick_of(mx, g.nb[0], g.K) + g.nb[0] * (brick_of(my, g.nb[1], g.K) + g.nb[1] * brick_of(mz, g.nb[2], g.K));
+}
+// Sort key of a particle with base point (mx, my, mz): its brick, and on the power-of-two path
+// (bricks of bx x kBY x kBZ points) its class within the brick, bit d set when the base point
+// lies within order - 1 points of the brick's low side in dimension d.  Only such particles
+// have stencil points in the previous brick along d, so the spread of a brick reads, from each
+// of its next bricks, just the classes that reach back into it.
+__host__ __device__ inline int brick_key(const MeshGrid& g, int mx, int my, int mz) {
+  const int b = brick_index(g, mx, my, mz);
+  if (g.ncls == 1) return b;
+  const int r = g.order - 1;
+  const int cls = ((mx & (g.bx - 1)) < r ? 1 : 0) | ((my & (kBY - 1)) < r ? 2 : 0) | ((mz & (kBZ - 1)) < r ? 4 : 0);
+  return b * 8 + cls;
 }
 
 enum KernelId : int;

@@ -108,7 +108,7 @@ __device__ __forceinline__ void bin_body(const BinArgs& a, int bid) {
         mi[d] = k < 0 ? k + a.m.K : k;
       }
       a.mm[i] = mi[0] | (mi[1] << 10) | (mi[2] << 20);
-      bkey = brick_index(a.m, mi[0], mi[1], mi[2]);
+      bkey = brick_key(a.m, mi[0], mi[1], mi[2]);
     }
     dx = q * r[0];
     dy = q * r[1];
@@ -253,7 +253,7 @@ struct ScanArgs {
   int* start0;
   int* fill0;
   int* chunk0;
-  int n1;                 // bricks (0 without SPME)
+  int n1;                 // brick keys (0 without SPME)
   const int* cnt1;
   int* start1;
   int* fill1;
@@ -328,7 +328,7 @@ __device__ __forceinline__ void scatter_body(const ScatterArgs& a, int bid) {
   }
   if ((a.parts & 2) && a.use_mesh) {
     const int p = ok ? a.mm[i] : 0;
-    const int b = ok ? brick_index(a.m, p & 1023, (p >> 10) & 1023, p >> 20) : -1;
+    const int b = ok ? brick_key(a.m, p & 1023, (p >> 10) & 1023, p >> 20) : -1;
     const int rk = run_add(a.brick_fill, b, lane, true);
     if (!ok) return;
     const int dst = a.brick_start[b] + rk;
@@ -361,13 +361,12 @@ __global__ __launch_bounds__(256) void k_scatter_seg(const ScatterArgs* __restri
 // also publishes them (with the pair kernel's home-chunk offsets for cells, the total dipole
 // for bricks); a particle's slot is its group's offset plus the rank k_bin's counter gave it.
 // This replaces a separate scan launch and the scatter's own counters.
-constexpr int kLsMax = 8192;  // counts one workgroup scans in LDS
 constexpr int kLsV = 8;
 
 struct ScatterLsArgs {
   ScatterArgs s;
   int role;            // 1: cells, 2: bricks
-  int n;               // ncell or nbrick (<= kLsMax)
+  int n;               // ncell or nkey (<= kLsMax)
   const int* cnt;
   int* start;          // offsets, n + 1 (written by workgroup 0)
   int* chunk;          // cells: home-chunk offsets, n + 1 (written by workgroup 0)
@@ -396,7 +395,7 @@ __global__ __launch_bounds__(256) void k_scatter_ls(ScatterLsArgs a) {
     a.s.cell_tmp[s_start[a.s.cellid[i]] + rk] = i;
   } else {
     const int p = a.s.mm[i];
-    const int b = brick_index(a.s.m, p & 1023, (p >> 10) & 1023, p >> 20);
+    const int b = brick_key(a.s.m, p & 1023, (p >> 10) & 1023, p >> 20);
     const int dst = s_start[b] + rk;
     typedef typename std::conditional<sizeof(real) == 4, float4, double4>::type R4;
     const real* fr = (const real*)a.s.frac;
@@ -579,11 +578,10 @@ static Box make_box(const msg_params& P) {
 }
 
 // Fused scatter (offsets scanned per workgroup, slots from k_bin's ranks) for a role whose
-// counts fit in LDS and whose per-workgroup scans (every workgroup reads all counts) stay
-// below kLsWork count reads: 1 cells, 2 bricks.
-constexpr long long kLsWork = 1ll << 22;
+// counts fit in LDS (kLsMax) and whose per-workgroup scans (every workgroup reads all counts)
+// stay below kLsWork count reads: 1 cells, 2 bricks.
 static bool sort_fused(const Ctx* c, int role) {
-  const long long n = role == 1 ? c->grid.ncell : c->mesh.nbrick;
+  const long long n = role == 1 ? c->grid.ncell : c->mesh.nkey;
   return n <= kLsMax && n * ((c->np + 255) / 256) <= kLsWork;
 }
 
@@ -618,7 +616,7 @@ static ScanArgs scan_args(Ctx* c, int bid0) {
   a.start0 = c->d_cell_start;
   a.fill0 = c->d_cell_fill;
   a.chunk0 = c->d_cell_chunk;
-  a.n1 = mesh ? c->mesh.nbrick : 0;
+  a.n1 = mesh ? c->mesh.nkey : 0;
   a.cnt1 = c->d_brick_count;
   a.start1 = c->d_brick_start;
   a.fill1 = c->d_brick_fill;
@@ -702,7 +700,7 @@ static void launch_scatter_ls(Ctx* c, int role, bool dipole) {
   ScatterLsArgs a;
   a.s = scatter_args(c, role);
   a.role = role;
-  a.n = role == 1 ? c->grid.ncell : c->mesh.nbrick;
+  a.n = role == 1 ? c->grid.ncell : c->mesh.nkey;
   a.cnt = role == 1 ? c->d_cell_count : c->d_brick_count;
   a.start = role == 1 ? c->d_cell_start : c->d_brick_start;
   a.chunk = c->d_cell_chunk;

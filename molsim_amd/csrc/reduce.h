@@ -21,7 +21,7 @@ struct ReduceArgs {
   int* cell_count;
   int ncell;
   int* brick_count;
-  int nbrick;
+  int nbrick;          // brick sort counters to clear (MeshGrid::nkey)
 };
 
 // Fixed-order column totals of the pair kernel's rows (np1 columns: energies by type pair +

@@ -910,7 +910,7 @@ __device__ __forceinline__ int wave_prefix4(int v, int lane, int& total) {
 // y rows of P x points into the brick with integer LDS atomics (32-bit in FP32).  The x points outside
 // the brick add zero at addresses that stay inside the padded array.  Finally every mesh
 // point of the brick is written once.
-template <typename real, int P, int BX>
+template <typename real, int P, int BX, bool CLS>
 __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   typedef typename R4T<real>::t R4;
   // wavefronts, candidates per wavefront round, contributors per record pass (a round with
@@ -935,12 +935,38 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   const int bx = b % a.m.nb[0], by = (b / a.m.nb[0]) % a.m.nb[1], bz = b / (a.m.nb[0] * a.m.nb[1]);
   const int lo[3] = {bx * BX, by * kBY, bz * kBZ};
   const bool whole[3] = {a.m.nb[0] == 1, a.m.nb[1] == 1, a.m.nb[2] == 1};
-  // the (up to) 8 source bricks, own and next in each dimension: lanes 0..7 of every
-  // wavefront load their ranges, which every wavefront then holds in scalar registers (no
-  // LDS round trip, no barrier), so a candidate's place is found without LDS reads
+  // source ranges: the brick's own particles and its next bricks' in each dimension.  With
+  // classes (CLS, brick_key) a next brick contributes only the classes whose stencils reach back
+  // into this brick: 14 ranges of consecutive classes, range t = (next-brick offset bits
+  // x | y << 1 | z << 2, first class, last class), packed three bits per range; without, the 8
+  // bricks whole.  Lanes 0..NR-1 of every wavefront load them; every wavefront then holds them
+  // in scalar registers (no LDS round trip, no barrier), so a candidate's place is found
+  // without LDS reads.
+  constexpr int NR = CLS ? 14 : 8;
+  constexpr unsigned long long kOff = 0ull | 1ull << 3 | 1ull << 6 | 1ull << 9 | 1ull << 12 | 2ull << 15 |
+                                      2ull << 18 | 4ull << 21 | 3ull << 24 | 3ull << 27 | 5ull << 30 |
+                                      5ull << 33 | 6ull << 36 | 7ull << 39;
+  constexpr unsigned long long kC0 = 0ull | 1ull << 3 | 3ull << 6 | 5ull << 9 | 7ull << 12 | 2ull << 15 |
+                                     6ull << 18 | 4ull << 21 | 3ull << 24 | 7ull << 27 | 5ull << 30 |
+                                     7ull << 33 | 6ull << 36 | 7ull << 39;
+  constexpr unsigned long long kC1 = 7ull | 1ull << 3 | 3ull << 6 | 5ull << 9 | 7ull << 12 | 3ull << 15 |
+                                     7ull << 18 | 7ull << 21 | 3ull << 24 | 7ull << 27 | 5ull << 30 |
+                                     7ull << 33 | 7ull << 36 | 7ull << 39;  // last class of the range
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int beg = 0, len = 0;
-  if (lane < 8) {
+  if constexpr (CLS) {
+    if (lane < NR) {
+      const int sh3 = 3 * lane;
+      const int off = (int)((kOff >> sh3) & 7), c0 = (int)((kC0 >> sh3) & 7), c1 = (int)((kC1 >> sh3) & 7) + 1;
+      const int ix = off & 1, iy = (off >> 1) & 1, iz = off >> 2;
+      if (!((ix && whole[0]) || (iy && whole[1]) || (iz && whole[2]))) {
+        const int mb = (bx + ix) % a.m.nb[0] +
+                       a.m.nb[0] * ((by + iy) % a.m.nb[1] + a.m.nb[1] * ((bz + iz) % a.m.nb[2]));
+        beg = a.brick_start[8 * mb + c0];
+        len = a.brick_start[8 * mb + c1] - beg;
+      }
+    }
+  } else if (lane < 8) {  // without classes: the (up to) 8 bricks whole
     const int ox = whole[0] ? 1 : 2, oy = whole[1] ? 1 : 2, oz = whole[2] ? 1 : 2;
     if (lane < ox * oy * oz) {
       const int ix = lane % ox, iy = (lane / ox) % oy, iz = lane / (ox * oy);
@@ -951,18 +977,18 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
     }
   }
   for (int k = threadIdx.x; k < PADL + NACC + 2 * BX; k += 256) acc_s[k] = 0;
-  int pref[9], dlt[8];
+  int pref[NR + 1], dlt[NR];
   {
     int run = 0;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
+    for (int t = 0; t < NR; ++t) {
       pref[t] = run;
       dlt[t] = __builtin_amdgcn_readlane(beg, t) - run;
       run += __builtin_amdgcn_readlane(len, t);
     }
-    pref[8] = run;
+    pref[NR] = run;
   }
-  const int total = pref[8];
+  const int total = pref[NR];
   __syncthreads();  // the accumulator is zeroed
   const R4* bq = (const R4*)a.bq;
   // FP32: the brick's scale from its candidate count; FP64: 2^48 in the conversion
@@ -976,7 +1002,7 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
   if (t < total) {
     int d = dlt[0];
 #pragma unroll
-    for (int r = 1; r < 8; ++r) d = t >= pref[r] ? dlt[r] : d;
+    for (int r = 1; r < NR; ++r) d = t >= pref[r] ? dlt[r] : d;
     packed = a.bmm[t + d];
     v = bq[t + d];
   }
@@ -988,7 +1014,7 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
     if (tn < total) {
       int d = dlt[0];
 #pragma unroll
-      for (int r = 1; r < 8; ++r) d = tn >= pref[r] ? dlt[r] : d;
+      for (int r = 1; r < NR; ++r) d = tn >= pref[r] ? dlt[r] : d;
       packed_n = a.bmm[tn + d];
       v_n = bq[tn + d];
     }
@@ -1089,17 +1115,17 @@ __device__ __forceinline__ void spread_body(const SpreadArgs& a, int bid) {
 }
 
 // Registers capped for six workgroups per CU (the LDS use allows seven).
-template <typename real, int P, int BX>
+template <typename real, int P, int BX, bool CLS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_spread(SpreadArgs a) {
-  spread_body<real, P, BX>(a, blockIdx.x);
+  spread_body<real, P, BX, CLS>(a, blockIdx.x);
 }
 
-template <typename real, int P, int BX>
+template <typename real, int P, int BX, bool CLS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_spread_seg(const SpreadArgs* __restrict__ args,
                                                     const int2* __restrict__ map) {
   const int2 s = map[blockIdx.x];
   if (s.x < 0) return;  // no work for this XCD slot
-  spread_body<real, P, BX>(args[s.x], s.y);
+  spread_body<real, P, BX, CLS>(args[s.x], s.y);
 }
 
 // General-path spread: the same algorithm over bricks of any extent.  Brick b of a dimension
@@ -1531,10 +1557,13 @@ void launch_spread_gather(Ctx* c, bool spread, bool with_reduce) {
   if (spread) {
     const SpreadArgs a = spread_args(c);
     prof_begin(c, MSG_K_SPREAD);
-    if (brick_x(c->mesh.K) == 32)
-      hipLaunchKernelGGL((k_spread<real, P, 32>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
+    // K = 16 (16-point bricks) always has classes (at most 4 bricks)
+    if (c->mesh.bx == 16)
+      hipLaunchKernelGGL((k_spread<real, P, 16, true>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
+    else if (c->mesh.ncls == 8)
+      hipLaunchKernelGGL((k_spread<real, P, 32, true>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
     else
-      hipLaunchKernelGGL((k_spread<real, P, 16>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
+      hipLaunchKernelGGL((k_spread<real, P, 32, false>), dim3(c->mesh.nbrick), dim3(256), 0, c->stream, a);
     prof_end(c, MSG_K_SPREAD);
   } else {
     const GatherArgs g = gather_args(c);
@@ -1662,6 +1691,9 @@ int spme_setup(Ctx* c) {
     c->nrad = n;
   }
   m.nbrick = m.nb[0] * m.nb[1] * m.nb[2];
+  m.bx = m.gen ? 0 : brick_x(K);
+  m.ncls = !m.gen && brick_classes(m.nbrick) ? 8 : 1;
+  m.nkey = m.nbrick * m.ncls;
   for (int d = 0; d < 3; ++d) m.dkdr[d] = K / P.boxlen[d];
   spme_charge_scale(c);
   // spline at pos = 0: M_p(1..p-1) are the only non-zero integer-point values
@@ -1710,16 +1742,16 @@ int spme_setup(Ctx* c) {
     if (int e = A(&c->d_C, (size_t)H * K * K * 2 * rs)) return e;
     if (int e = A(&c->d_G, (size_t)H * H * H * rs)) return e;
     if (int e = A(&c->d_tw, (size_t)K * 2 * rs)) return e;
-    if (int e = A((void**)&c->d_brick_count, m.nbrick * sizeof(int))) return e;
-    if (int e = A((void**)&c->d_brick_start, (m.nbrick + 1) * sizeof(int))) return e;
-    if (int e = A((void**)&c->d_brick_fill, m.nbrick * sizeof(int))) return e;
+    if (int e = A((void**)&c->d_brick_count, m.nkey * sizeof(int))) return e;
+    if (int e = A((void**)&c->d_brick_start, (m.nkey + 1) * sizeof(int))) return e;
+    if (int e = A((void**)&c->d_brick_fill, m.nkey * sizeof(int))) return e;
     if (int e = A((void**)&c->d_brick_ip, np * sizeof(int))) return e;
     if (int e = A((void**)&c->d_mm, np * sizeof(int))) return e;
     if (int e = A(&c->d_bq, (size_t)np * 4 * rs)) return e;
     if (int e = A((void**)&c->d_bmm, np * sizeof(int))) return e;
     if (int e = A(&c->d_frac, (size_t)np * 3 * rs)) return e;
   }
-  MSG_HIP(hipMemset(c->d_brick_count, 0, m.nbrick * sizeof(int)));
+  MSG_HIP(hipMemset(c->d_brick_count, 0, m.nkey * sizeof(int)));
   std::vector<double> tw(2 * K);
   for (int i = 0; i < K; ++i) {
     tw[2 * i] = std::cos(2.0 * M_PI * i / K);
@@ -1834,9 +1866,9 @@ int add_fft_k(Ctx* c, SegBuilder<FftArgs>* b) {
   return MSG_OK;
 }
 
-template <typename real, int P, int BX>
+template <typename real, int P, int BX, bool CLS>
 void seg_spread(const SegGroup& g, hipStream_t s) {
-  hipLaunchKernelGGL((k_spread_seg<real, P, BX>), dim3(g.nblk), dim3(256), 0, s, (const SpreadArgs*)g.d_args,
+  hipLaunchKernelGGL((k_spread_seg<real, P, BX, CLS>), dim3(g.nblk), dim3(256), 0, s, (const SpreadArgs*)g.d_args,
                      g.d_map);
 }
 template <typename real, int P>
@@ -1878,12 +1910,14 @@ int add_spread_gather_g(Ctx* c, SegBuilder<SpreadArgs>& bs, SegBuilder<GatherArg
 template <typename real>
 int add_spread_gather(Ctx* c, SegBuilder<SpreadArgs>& bs, SegBuilder<GatherArgs>& bg) {
   if (c->mesh.gen) return add_spread_gather_g<real>(c, bs, bg);
-  const bool wide = brick_x(c->mesh.K) == 32;
-  // the spread depends on the brick shape, the gather does not
-  const int key = (sizeof(real) == 8 ? 100 : 0) + c->mesh.order, skey = key + (wide ? 1000 : 0);
+  const bool wide = c->mesh.bx == 32, cls = c->mesh.ncls == 8;
+  // the spread depends on the brick shape and classes, the gather does not
+  const int key = (sizeof(real) == 8 ? 100 : 0) + c->mesh.order;
+  const int skey = key + (wide ? 1000 : 0) + (cls ? 2000 : 0);
   SegLaunchFn fs, fg;
-#define MSG_SG(P)                                                 \
-  fs = wide ? seg_spread<real, P, 32> : seg_spread<real, P, 16>; \
+#define MSG_SG(P)                                                                             \
+  fs = !wide ? seg_spread<real, P, 16, true>                                                 \
+             : (cls ? seg_spread<real, P, 32, true> : seg_spread<real, P, 32, false>); \
   fg = seg_gather<real, P>;
   switch (c->mesh.order) {
     case 3: MSG_SG(3) break;
