@@ -540,7 +540,7 @@ __device__ __forceinline__ void pair_nb_body(const PairArgs& a, int bid) {
 }
 
 
-constexpr int kNbRound = 6;    // candidates per lane and test round (loads in flight)
+constexpr int kNbRound = 4;    // candidates per lane and test round (loads in flight)
 constexpr int kNbEval = 4;     // in-cut pairs per lane evaluated together (loads in flight)
 constexpr int kNbSlotBits = 27;  // a code (unsigned) holds a slot below 2^27 and a stencil index above
 
