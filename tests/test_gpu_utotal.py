@@ -206,3 +206,30 @@ def test_crowded_brick(precision, ftol, etol):
     compare(eng.utotal(), ref, ftol, etol)
     compare(eng.utotal(), ref, ftol, etol)
     eng.close()
+
+
+@pytest.mark.parametrize("precision,ftol,etol", [("mixed", MIXED_FTOL, MIXED_ETOL),
+                                                 ("fp64", FP64_TOL, FP64_TOL)])
+def test_brick_class_boundaries(precision, ftol, etol):
+    """A 13 x 13 x 14 ion lattice in a 64 A box with a 64^3 mesh (1 A spacing, order 6, bricks
+    of 32 x 8 x 8 points sorted by class): the B-spline base points take every offset 0..7 from
+    a brick's low side in y, and offsets on both sides of the order - 1 = 5 point limit that
+    decides whether a particle of the next brick reaches back into a brick in x (2 and 5) and z
+    (4 and 6).  A class boundary drawn one point too tight drops contributions; energy and
+    forces against the oracle."""
+    sp = _small(n=2366, L=64.0, nmesh=64)
+    ax = [np.arange(13) * (64.0 / 13), np.arange(13) * (64.0 / 13), np.arange(14) * (64.0 / 14)]
+    g = np.stack(np.meshgrid(*ax, indexing="ij"), -1).reshape(-1, 3)
+    ro = g + 0.37 - 32.0
+    par = np.stack(np.meshgrid(np.arange(13), np.arange(13), np.arange(14), indexing="ij"), -1).reshape(-1, 3).sum(1)
+    ipt = (par % 2).astype(np.int32)
+    az = np.where(ipt == 0, 1.0, -1.0)
+    sp = systems.SystemSpec(**{**sp.__dict__, "ro": ro, "ipt": ipt, "az": az})
+    base = np.floor(ro).astype(int) % 64  # dkdr = K / L = 1 per A
+    assert set(base[:, 1] % 8) == set(range(8)) and {4, 6} <= set(base[:, 2] % 8)
+    assert {2, 5} <= set(base[:, 0] % 32)
+    print(f"brick classes precision={precision} n={sp.n}", flush=True)
+    tab = oracle.build_table(sp.pot)
+    ref = oracle_system(sp, tab).utotal()
+    eng = engine(sp, tab, precision)
+    compare(eng.utotal(), ref, ftol, etol)
